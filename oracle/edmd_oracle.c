@@ -1,0 +1,789 @@
+/*
+ * CPU restatement of the reference's event-driven hard-disk engine.
+ * TEST INFRASTRUCTURE ONLY: see edmd_oracle.h.  Build strict IEEE
+ * (-O2 -fno-fast-math -ffp-contract=off, oracle/Makefile).
+ *
+ * Reference line numbers are those of src/EDMD.c unless a file is named.
+ */
+#include "edmd_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NEVER 100000000000000000000000000.0 /* EDMD.h:8 */
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+#define NIL (-1)
+
+typedef struct {
+    int lft, rgt, top; /* BST links, or bucket-list links (lft = prev, rgt = next) */
+    int i, j, q;       /* owner, partner / direction, bucket index */
+    double t;
+    int type;
+    uint64_t coll_actual;
+} ev_node;
+
+struct orc_ctx {
+    int n;
+    double Lx, Ly, halfLx, halfLy;
+    int ncx, ncy;
+    double csx, csy, cfx, cfy;
+    orc_params prm;
+    double t;
+    int status;
+    /* particles (structure of arrays) */
+    double *x, *y, *vx, *vy, *tl, *rad, *m;
+    uint64_t* coll;
+    int *cx, *cy, *crossX, *crossY;
+    /* cell lists: head per cell, doubly linked through particles (EDMD.c:2053-2095) */
+    int *head, *nxt, *prv;
+    /* event node pool: [0,N) crossings, [N,2N) collisions, 2N root,
+     * 2N+1 screenshot, 2N+2 noise, 2N+3 thermo (EDMD.c:1935-1948, 4421-4440, 4814-4822) */
+    ev_node* ev;
+    int root, tree_min;
+    double paul_time, dt_paul;
+    int cur_list, n_lists;
+    int* lists; /* n_lists + 1 heads, last = overflow */
+    /* counters and accumulators */
+    uint64_t ncol, ncross, nevents;
+    double vir_xx, vir_yy, vir_xy, vir_yx;
+    double E, px, py;
+    double last_thermo;
+    orc_thermo_row* rows;
+    int nrows, caprows;
+};
+
+/* ------------------------------------------------------------ kinematics */
+
+/* PBC, EDMD.c:5896-5913: minimum image with >= L/2 and < -L/2. */
+static void min_image(const orc_ctx* s, double* dx, double* dy) {
+    if (*dx >= s->halfLx) *dx -= s->Lx;
+    else if (*dx < -s->halfLx) *dx += s->Lx;
+    if (*dy >= s->halfLy) *dy -= s->Ly;
+    else if (*dy < -s->halfLy) *dy += s->Ly;
+}
+
+/* PBCinsideCellX/Y, EDMD.c:5922-5936. */
+static double wrap_inside(double d, double half, double L) {
+    if (d >= half) return d - L;
+    if (d < -half) return d + L;
+    return d;
+}
+
+/* PBCpostX/Y, EDMD.c:5948-5959. */
+static double wrap_post(double v, double L) {
+    if (v < 0) return v + L;
+    if (v >= L) return v - L;
+    return v;
+}
+
+/* freeFlyNormal, EDMD.c:4954-4990 (no field, no damping). */
+static void free_fly(orc_ctx* s, int p) {
+    double dt = s->t - s->tl[p];
+    s->tl[p] = s->t;
+    s->x[p] += dt * s->vx[p];
+    s->y[p] += dt * s->vy[p];
+    s->x[p] = wrap_post(s->x[p], s->Lx);
+    s->y[p] = wrap_post(s->y[p], s->Ly);
+}
+
+/* DIST2, EDMD.c:35-43 (non-additive geometric mean contact). */
+static double contact2(double r1, double r2) { return 4 * r1 * r2; }
+
+/* ------------------------------------------------------------ calendar */
+
+static void tree_insert(orc_ctx* s, int e) {
+    /* addEventToTree, EDMD.c:2175-2203: equal keys go right. */
+    ev_node* ev = s->ev;
+    int w = s->root;
+    for (;;) {
+        if (ev[e].t < ev[w].t) {
+            if (ev[w].lft == NIL) { ev[w].lft = e; break; }
+            w = ev[w].lft;
+        } else {
+            if (ev[w].rgt == NIL) { ev[w].rgt = e; break; }
+            w = ev[w].rgt;
+        }
+    }
+    ev[e].lft = NIL;
+    ev[e].rgt = NIL;
+    ev[e].top = w;
+    if (s->tree_min == NIL || ev[e].t < ev[s->tree_min].t) s->tree_min = e;
+}
+
+static void queue_add(orc_ctx* s, int e) {
+    /* addEventToQueue, EDMD.c:2144-2170. */
+    ev_node* ev = s->ev;
+    double dt = ev[e].t - s->paul_time;
+    if (dt < s->dt_paul) {
+        ev[e].q = s->cur_list;
+        tree_insert(s, e);
+        return;
+    }
+    int idx;
+    if (dt >= s->dt_paul * s->n_lists) {
+        idx = s->n_lists;
+    } else {
+        idx = s->cur_list + (int)(dt / s->dt_paul);
+        if (idx >= s->n_lists) idx -= s->n_lists;
+    }
+    ev[e].q = idx;
+    ev[e].lft = NIL;
+    ev[e].rgt = s->lists[idx];
+    if (ev[e].rgt != NIL) ev[ev[e].rgt].lft = e;
+    s->lists[idx] = e;
+}
+
+static void tree_remove(orc_ctx* s, int e) {
+    /* removeEventFromTree, EDMD.c:2228-2280 (successor splice). */
+    ev_node* ev = s->ev;
+    int top = ev[e].top;
+    int n;
+    if (e == s->tree_min) {
+        if (ev[e].rgt != NIL) {
+            int m = ev[e].rgt;
+            while (ev[m].lft != NIL) m = ev[m].lft;
+            s->tree_min = m;
+        } else {
+            s->tree_min = (top == s->root) ? NIL : top;
+        }
+    }
+    if (ev[e].lft == NIL && ev[e].rgt == NIL) {
+        n = NIL;
+    } else if (ev[e].lft == NIL) {
+        n = ev[e].rgt;
+        ev[n].top = top;
+    } else if (ev[e].rgt == NIL) {
+        n = ev[e].lft;
+        ev[n].top = top;
+    } else {
+        n = ev[e].rgt;
+        while (ev[n].lft != NIL) n = ev[n].lft;
+        if (ev[n].top != e) {
+            ev[ev[n].top].lft = ev[n].rgt;
+            if (ev[n].rgt != NIL) ev[ev[n].rgt].top = ev[n].top;
+            ev[ev[e].rgt].top = n;
+            ev[n].rgt = ev[e].rgt;
+        }
+        ev[ev[e].lft].top = n;
+        ev[n].lft = ev[e].lft;
+        ev[n].top = top;
+    }
+    if (ev[top].lft == e) ev[top].lft = n;
+    else ev[top].rgt = n;
+}
+
+static void queue_remove(orc_ctx* s, int e) {
+    /* removeEventFromQueue, EDMD.c:2209-2221. */
+    ev_node* ev = s->ev;
+    if (ev[e].q != s->cur_list) {
+        if (ev[e].rgt != NIL) ev[ev[e].rgt].lft = ev[e].lft;
+        if (ev[e].lft != NIL) ev[ev[e].lft].rgt = ev[e].rgt;
+        else s->lists[ev[e].q] = ev[e].rgt;
+    } else {
+        tree_remove(s, e);
+    }
+}
+
+static void next_bucket(orc_ctx* s) {
+    /* addNextPaulEvent, EDMD.c:2287-2307. */
+    s->cur_list++;
+    s->paul_time += s->dt_paul;
+    if (s->cur_list == s->n_lists) {
+        s->cur_list = 0;
+        int e = s->lists[s->n_lists];
+        s->lists[s->n_lists] = NIL;
+        while (e != NIL) {
+            int nx = s->ev[e].rgt;
+            queue_add(s, e);
+            e = nx;
+        }
+    }
+    int e = s->lists[s->cur_list];
+    while (e != NIL) {
+        int nx = s->ev[e].rgt;
+        tree_insert(s, e);
+        e = nx;
+    }
+    s->lists[s->cur_list] = NIL;
+}
+
+static int queue_pop_min(orc_ctx* s) {
+    /* findNextEvent, EDMD.c:2313-2327. */
+    while (s->tree_min == NIL) next_bucket(s);
+    return s->tree_min;
+}
+
+/* ------------------------------------------------------------ cell list */
+
+static int cell_of(const orc_ctx* s, double a, double fac, int nc) {
+    /* coordToCell, EDMD.c:2098-2107; clamped to the grid (a coordinate one
+     * ulp below L can round to the grid size). */
+    int c = (int)(a * fac);
+    if (c >= nc) c = nc - 1;
+    if (c < 0) c = 0;
+    (void)s;
+    return c;
+}
+
+static void cell_push_head(orc_ctx* s, int p) {
+    /* head insertion, EDMD.c:2071-2077 and 2576-2585 */
+    int c = s->cy[p] * s->ncx + s->cx[p];
+    s->prv[p] = NIL;
+    s->nxt[p] = s->head[c];
+    s->head[c] = p;
+    if (s->nxt[p] != NIL) s->prv[s->nxt[p]] = p;
+}
+
+static void cell_unlink(orc_ctx* s, int p) {
+    /* removeFromCell, EDMD.c:2083-2095 */
+    if (s->prv[p] == NIL) s->head[s->cy[p] * s->ncx + s->cx[p]] = s->nxt[p];
+    else s->nxt[s->prv[p]] = s->nxt[p];
+    if (s->nxt[p] != NIL) s->prv[s->nxt[p]] = s->prv[p];
+}
+
+static int wrap_cell(int a, int nc) {
+    /* PBCcellX/Y, EDMD.c:2110-2124 */
+    if (a < 0) return a + nc;
+    if (a >= nc) return a - nc;
+    return a;
+}
+
+/* ------------------------------------------------------------ predictions */
+
+static double coll_time(orc_ctx* s, int i, int j) {
+    /* collisionTimeNormal, EDMD.c:2661-2723 (no damping, field or shear). */
+    double lat2 = s->t - s->tl[j];
+    double dvx = s->vx[j] - s->vx[i];
+    double dvy = s->vy[j] - s->vy[i];
+    double dx = (s->x[j] + lat2 * s->vx[j]) - s->x[i];
+    double dy = (s->y[j] + lat2 * s->vy[j]) - s->y[i];
+    min_image(s, &dx, &dy);
+    double b = dx * dvx + dy * dvy;
+    if (b > 0) return NEVER;
+    double v2 = dvx * dvx + dvy * dvy;
+    double d = dx * dx + dy * dy - contact2(s->rad[i], s->rad[j]);
+    double det = b * b - v2 * d;
+    if (d < -0.01) {
+        /* overlap abort, EDMD.c:2708-2717 */
+        if (s->status == ORC_OK) s->status = ORC_ERR_OVERLAP;
+    }
+    if (det < 0) return NEVER;
+    return (-b - sqrt(det)) / v2;
+}
+
+static void predict_collision(orc_ctx* s, int i, double* t_abs, int* partner) {
+    /* collisionEventNormal, EDMD.c:2829-3003, 3091-3093: strict-less minimum
+     * over the 3x3 cells, y offset outer, x offset inner, list order. */
+    int best = 0;
+    double dt = NEVER;
+    int X = s->cx[i], Y = s->cy[i];
+    for (int j = -1; j <= 1; j++) {
+        for (int k = -1; k <= 1; k++) {
+            int c = wrap_cell(Y + j, s->ncy) * s->ncx + wrap_cell(X + k, s->ncx);
+            for (int p = s->head[c]; p != NIL; p = s->nxt[p]) {
+                if (p == i) continue;
+                double dtt = coll_time(s, i, p);
+                if (dt > dtt) {
+                    best = p;
+                    dt = dtt;
+                }
+            }
+        }
+    }
+    *t_abs = s->t + dt;
+    *partner = best;
+}
+
+static void predict_crossing(const orc_ctx* s, int i, double* t_abs, int* dir) {
+    /* crossingEventNormal, EDMD.c:2405-2482: ties go to y. */
+    double tx, ty;
+    int xx, yy;
+    if (s->vx[i] < 0) {
+        tx = wrap_inside(s->cx[i] * s->csx - s->x[i], s->halfLx, s->Lx) / s->vx[i];
+        xx = 1;
+    } else {
+        tx = wrap_inside((1 + s->cx[i]) * s->csx - s->x[i], s->halfLx, s->Lx) / s->vx[i];
+        xx = 2;
+    }
+    if (s->vy[i] < 0) {
+        ty = wrap_inside(s->cy[i] * s->csy - s->y[i], s->halfLy, s->Ly) / s->vy[i];
+        yy = 3;
+    } else {
+        ty = wrap_inside((1 + s->cy[i]) * s->csy - s->y[i], s->halfLy, s->Ly) / s->vy[i];
+        yy = 4;
+    }
+    if (tx < ty) {
+        *t_abs = s->t + tx;
+        *dir = xx;
+    } else {
+        *t_abs = s->t + ty;
+        *dir = yy;
+    }
+}
+
+static void schedule_crossing(orc_ctx* s, int i) {
+    /* crossingEventNormal + addCrossingEvent, EDMD.c:2405-2496 */
+    double tc;
+    int dir;
+    predict_crossing(s, i, &tc, &dir);
+    ev_node* e = &s->ev[i];
+    e->j = dir;
+    e->type = ORC_CELLCROSS;
+    e->t = tc;
+    queue_add(s, i);
+}
+
+static void schedule_collision(orc_ctx* s, int i) {
+    /* collisionEventNormal + addCollisionEvent, EDMD.c:3091-3093, 3286-3297 */
+    double tc;
+    int partner;
+    predict_collision(s, i, &tc, &partner);
+    ev_node* e = &s->ev[s->n + i];
+    e->j = partner;
+    e->type = ORC_COLLISION;
+    e->t = tc;
+    e->coll_actual = s->coll[partner];
+    queue_add(s, s->n + i);
+}
+
+static void schedule_special(orc_ctx* s, int slot, int type, double t) {
+    /* addEventScreenshot / addEventNoise / addEventThermo, EDMD.c:4421-4440, 4814-4822 */
+    ev_node* e = &s->ev[2 * s->n + slot];
+    e->type = type;
+    e->t = t;
+    e->j = 0;
+    queue_add(s, 2 * s->n + slot);
+}
+
+/* ------------------------------------------------------------ handlers */
+
+static void physical_q(orc_ctx* s) {
+    /* physicalQ, EDMD.c:5968-6013 (2D, no shear) */
+    s->px = 0;
+    s->py = 0;
+    s->E = 0;
+    for (int i = 0; i < s->n; i++) {
+        s->px += s->vx[i] * s->m[i];
+        s->py += s->vy[i] * s->m[i];
+        double vx = s->vx[i];
+        s->E += 0.5 * s->m[i] * (vx * vx + s->vy[i] * s->vy[i]);
+    }
+}
+
+static void do_collision(orc_ctx* s, const ev_node* ev) {
+    /* doTheCollisionNormal, EDMD.c:3542-3562, 3583-3606, 3802-3827, 3869-3879 */
+    int i = ev->i, j = ev->j;
+    free_fly(s, i);
+    if (ev->coll_actual != s->coll[j]) {
+        schedule_collision(s, i);
+        return;
+    }
+    s->ncol++;
+    free_fly(s, j);
+    s->coll[i]++;
+    s->coll[j]++;
+    double dx = s->x[j] - s->x[i];
+    double dy = s->y[j] - s->y[i];
+    double dvx = s->vx[j] - s->vx[i];
+    double dvy = s->vy[j] - s->vy[i];
+    double inv_mass = 1 / (s->m[i] + s->m[j]);
+    min_image(s, &dx, &dy);
+    double kernel = inv_mass * (1 + s->prm.res) * (dx * dvx + dy * dvy);
+    double ff = kernel / contact2(s->rad[i], s->rad[j]);
+    double w = s->m[i] * s->m[j] * ff;
+    s->vir_xx += w * dx * dx;
+    s->vir_yy += w * dy * dy;
+    s->vir_xy += w * dx * dy;
+    s->vir_yx += w * dy * dx;
+    s->vx[i] += ff * s->m[j] * dx;
+    s->vy[i] += ff * s->m[j] * dy;
+    s->vx[j] -= ff * s->m[i] * dx;
+    s->vy[j] -= ff * s->m[i] * dy;
+    queue_remove(s, i);
+    queue_remove(s, j);
+    schedule_crossing(s, i);
+    schedule_crossing(s, j);
+    queue_remove(s, s->n + j);
+    schedule_collision(s, i);
+    schedule_collision(s, j);
+}
+
+static void do_crossing(orc_ctx* s, const ev_node* ev) {
+    /* doTheCrossing, EDMD.c:2501-2590 (no shear) */
+    int i = ev->i;
+    free_fly(s, i);
+    cell_unlink(s, i);
+    switch (ev->j) {
+        case 1:
+            if (s->cx[i] - 1 == -1) { s->cx[i] = s->ncx - 1; s->crossX[i] -= 1; }
+            else s->cx[i] -= 1;
+            break;
+        case 2:
+            if (s->cx[i] + 1 == s->ncx) { s->cx[i] = 0; s->crossX[i] += 1; }
+            else s->cx[i] += 1;
+            break;
+        case 3:
+            if (s->cy[i] - 1 == -1) { s->cy[i] = s->ncy - 1; s->crossY[i] -= 1; }
+            else s->cy[i] -= 1;
+            break;
+        case 4:
+            if (s->cy[i] + 1 == s->ncy) { s->cy[i] = 0; s->crossY[i] += 1; }
+            else s->cy[i] += 1;
+            break;
+    }
+    cell_push_head(s, i);
+    schedule_crossing(s, i);
+    queue_remove(s, s->n + i);
+    schedule_collision(s, i);
+}
+
+static void do_screenshot(orc_ctx* s) {
+    /* takeAScreenshot, EDMD.c:4657-4677: free-fly everyone (a state update
+     * point), reschedule; the dump write has no effect on the state. */
+    for (int i = 0; i < s->n; i++) free_fly(s, i);
+    schedule_special(s, 1, ORC_SCREENSHOT, s->t + s->prm.dtime);
+}
+
+static void do_thermo(orc_ctx* s) {
+    /* takeAThermo + saveThermo, EDMD.c:4679-4713, 5232-5333 */
+    physical_q(s);
+    double area = s->Lx * s->Ly;
+    double dT = s->t - s->last_thermo;
+    s->last_thermo = s->t;
+    double ex = 0, ey = 0, exy = 0, eyx = 0;
+    for (int i = 0; i < s->n; i++) {
+        double vx = s->vx[i], vy = s->vy[i];
+        ex += s->m[i] * vx * vx;
+        ey += s->m[i] * vy * vy;
+        exy += s->m[i] * vx * vy;
+        eyx += s->m[i] * vy * vx;
+    }
+    orc_thermo_row row;
+    row.t = s->t;
+    row.ncol = s->ncol;
+    row.e_per_n = s->E / s->n;
+    row.pxx = -1 * s->vir_xx / (area * dT) + ex / area;
+    row.pyy = -1 * s->vir_yy / (area * dT) + ey / area;
+    row.pxy = -1 * s->vir_xy / (area * dT) + exy / area;
+    row.pyx = -1 * s->vir_yx / (area * dT) + eyx / area;
+    row.pressure = (-1 / (dT)) * (s->vir_xx + s->vir_yy) / (2 * area) + s->E / area;
+    s->vir_xx = s->vir_yy = s->vir_xy = s->vir_yx = 0;
+    if (s->nrows == s->caprows) {
+        int cap = s->caprows ? 2 * s->caprows : 16;
+        orc_thermo_row* r = realloc(s->rows, (size_t)cap * sizeof *r);
+        if (r) { s->rows = r; s->caprows = cap; }
+    }
+    if (s->nrows < s->caprows) s->rows[s->nrows++] = row;
+    schedule_special(s, 3, ORC_THERMO, s->t + s->prm.dtime_thermo);
+}
+
+static void do_rescale(orc_ctx* s) {
+    /* addNoise with the velocity-rescale branch (noise == 2),
+     * EDMD.c:4828-4832, 4880-4915, 4922. */
+    physical_q(s);
+    for (int i = 0; i < s->n; i++) {
+        s->coll[i]++;
+        free_fly(s, i);
+        s->vx[i] /= sqrt(s->E / s->n / s->prm.temperature);
+        s->vy[i] /= sqrt(s->E / s->n / s->prm.temperature);
+    }
+    for (int j = 0; j < s->n; j++) {
+        queue_remove(s, j);
+        schedule_crossing(s, j);
+        queue_remove(s, s->n + j);
+        schedule_collision(s, j);
+    }
+    schedule_special(s, 2, ORC_ADDINGNOISE, s->t + s->prm.dtnoise);
+}
+
+static int step(orc_ctx* s, orc_trace* rec) {
+    /* One pass of the dispatch loop, EDMD.c:534-610. */
+    int e = queue_pop_min(s);
+    ev_node* ev = &s->ev[e];
+    if (rec) {
+        rec->type = ev->type;
+        rec->i = ev->i;
+        rec->j = ev->j;
+        rec->t = ev->t;
+        rec->valid = 1;
+        if (ev->type == ORC_COLLISION) rec->valid = (ev->coll_actual == s->coll[ev->j]);
+    }
+    s->t = ev->t;
+    queue_remove(s, e);
+    switch (ev->type) {
+        case ORC_COLLISION: do_collision(s, ev); break;
+        case ORC_CELLCROSS: s->ncross++; do_crossing(s, ev); break;
+        case ORC_SCREENSHOT: do_screenshot(s); break;
+        case ORC_THERMO: do_thermo(s); break;
+        case ORC_ADDINGNOISE: do_rescale(s); break;
+        default: break;
+    }
+    s->nevents++;
+    return s->status;
+}
+
+/* ------------------------------------------------------------ public API */
+
+void orc_default_params(orc_params* p) {
+    /* EDMD.c:244-248, 372, 379, 386 */
+    p->res = 1;
+    p->first_screen = 1;
+    p->dtime = 100;
+    p->first_thermo = 1000;
+    p->dtime_thermo = 100;
+    p->noise = 0;
+    p->dtnoise = 0.5;
+    p->temperature = 1;
+    p->bucket_count = 0;
+}
+
+orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* y,
+                    const double* vx, const double* vy, const double* rad,
+                    const double* m, const orc_params* prm) {
+    if (n < 2 || !(Lx > 0) || !(Ly > 0)) return NULL;
+    orc_ctx* s = calloc(1, sizeof *s);
+    if (!s) return NULL;
+    s->n = n;
+    if (prm) s->prm = *prm;
+    else orc_default_params(&s->prm);
+    /* boxConstantHelper, EDMD.c:679-715 */
+    s->Lx = Lx;
+    s->Ly = Ly;
+    s->halfLx = Lx / 2;
+    s->halfLy = Ly / 2;
+    s->ncx = (int)(s->halfLx);
+    s->ncy = (int)(s->halfLy);
+    if (s->ncx < 1 || s->ncy < 1) { free(s); return NULL; }
+    s->csx = Lx / s->ncx;
+    s->csy = Ly / s->ncy;
+    s->cfx = 1 / s->csx;
+    s->cfy = 1 / s->csy;
+    s->n_lists = s->prm.bucket_count > 0 ? s->prm.bucket_count : n;
+    s->dt_paul = 5.0 / (double)n;
+
+    size_t nd = (size_t)n;
+    s->x = malloc(nd * sizeof(double));
+    s->y = malloc(nd * sizeof(double));
+    s->vx = malloc(nd * sizeof(double));
+    s->vy = malloc(nd * sizeof(double));
+    s->tl = calloc(nd, sizeof(double));
+    s->rad = malloc(nd * sizeof(double));
+    s->m = malloc(nd * sizeof(double));
+    s->coll = calloc(nd, sizeof(uint64_t));
+    s->cx = calloc(nd, sizeof(int));
+    s->cy = calloc(nd, sizeof(int));
+    s->crossX = calloc(nd, sizeof(int));
+    s->crossY = calloc(nd, sizeof(int));
+    s->head = malloc((size_t)s->ncx * s->ncy * sizeof(int));
+    s->nxt = malloc(nd * sizeof(int));
+    s->prv = malloc(nd * sizeof(int));
+    s->ev = calloc(2 * nd + 10, sizeof(ev_node));
+    s->lists = malloc(((size_t)s->n_lists + 1) * sizeof(int));
+    if (!s->x || !s->y || !s->vx || !s->vy || !s->tl || !s->rad || !s->m || !s->coll ||
+        !s->cx || !s->cy || !s->crossX || !s->crossY || !s->head || !s->nxt || !s->prv ||
+        !s->ev || !s->lists) {
+        orc_destroy(s);
+        return NULL;
+    }
+    /* load path, EDMD.c:1424-1442: wrap positions into the box, t = 0 */
+    for (int i = 0; i < n; i++) {
+        s->x[i] = wrap_post(x[i], Lx);
+        s->y[i] = wrap_post(y[i], Ly);
+        s->vx[i] = vx[i];
+        s->vy[i] = vy[i];
+        s->rad[i] = rad[i];
+        s->m[i] = m[i];
+    }
+    /* cellListInit, EDMD.c:1906-1920 */
+    for (int c = 0; c < s->ncx * s->ncy; c++) s->head[c] = NIL;
+    for (int i = 0; i < n; i++) {
+        s->cx[i] = cell_of(s, s->x[i], s->cfx, s->ncx);
+        s->cy[i] = cell_of(s, s->y[i], s->cfy, s->ncy);
+        cell_push_head(s, i);
+    }
+    for (int k = 0; k <= s->n_lists; k++) s->lists[k] = NIL;
+    return s;
+}
+
+void orc_destroy(orc_ctx* s) {
+    if (!s) return;
+    free(s->x); free(s->y); free(s->vx); free(s->vy); free(s->tl); free(s->rad);
+    free(s->m); free(s->coll); free(s->cx); free(s->cy); free(s->crossX);
+    free(s->crossY); free(s->head); free(s->nxt); free(s->prv); free(s->ev);
+    free(s->lists); free(s->rows);
+    free(s);
+}
+
+int orc_init_events(orc_ctx* s) {
+    /* eventListInit, EDMD.c:1935-2012 (load path: Normal functions) */
+    int n = s->n;
+    for (int e = 0; e < 2 * n + 10; e++) {
+        s->ev[e].lft = s->ev[e].rgt = s->ev[e].top = NIL;
+    }
+    s->root = 2 * n;
+    s->ev[s->root].t = NEVER + 1;
+    s->tree_min = NIL;
+    schedule_special(s, 3, ORC_THERMO, s->prm.first_thermo + 0.000000001);
+    schedule_special(s, 1, ORC_SCREENSHOT, s->prm.first_screen);
+    if (s->prm.noise) schedule_special(s, 2, ORC_ADDINGNOISE, s->prm.dtnoise);
+    for (int i = 0; i < n; i++) {
+        s->ev[i].i = i;
+        s->ev[n + i].i = i;
+        schedule_crossing(s, i);
+        schedule_collision(s, i);
+    }
+    return s->status;
+}
+
+int orc_run(orc_ctx* s, int64_t nevents, orc_trace* trace, int64_t* done) {
+    int64_t k = 0;
+    int st = s->status;
+    while (k < nevents && st == ORC_OK) {
+        st = step(s, trace ? trace + k : NULL);
+        k++;
+    }
+    if (done) *done = k;
+    return st;
+}
+
+int orc_run_until(orc_ctx* s, double tmax, int64_t max_events, int64_t* done) {
+    int64_t k = 0;
+    int st = s->status;
+    while (s->t <= tmax && k < max_events && st == ORC_OK) {
+        st = step(s, NULL);
+        k++;
+    }
+    if (done) *done = k;
+    return st;
+}
+
+int orc_run_collisions(orc_ctx* s, uint64_t target, int64_t max_events, int64_t* done) {
+    int64_t k = 0;
+    int st = s->status;
+    while (s->ncol < target && k < max_events && st == ORC_OK) {
+        st = step(s, NULL);
+        k++;
+    }
+    if (done) *done = k;
+    return st;
+}
+
+int orc_n(const orc_ctx* s) { return s->n; }
+double orc_time(const orc_ctx* s) { return s->t; }
+int orc_status(const orc_ctx* s) { return s->status; }
+
+void orc_grid(const orc_ctx* s, int* ncx, int* ncy, double* csx, double* csy) {
+    *ncx = s->ncx;
+    *ncy = s->ncy;
+    *csx = s->csx;
+    *csy = s->csy;
+}
+
+void orc_get_state(const orc_ctx* s, double* x, double* y, double* vx, double* vy,
+                   double* tl, uint64_t* coll, int32_t* cx, int32_t* cy) {
+    for (int i = 0; i < s->n; i++) {
+        if (x) x[i] = s->x[i];
+        if (y) y[i] = s->y[i];
+        if (vx) vx[i] = s->vx[i];
+        if (vy) vy[i] = s->vy[i];
+        if (tl) tl[i] = s->tl[i];
+        if (coll) coll[i] = s->coll[i];
+        if (cx) cx[i] = s->cx[i];
+        if (cy) cy[i] = s->cy[i];
+    }
+}
+
+void orc_get_events(const orc_ctx* s, double* t_cross, int32_t* dir, double* t_coll,
+                    int32_t* partner, uint64_t* coll_actual) {
+    int n = s->n;
+    for (int i = 0; i < n; i++) {
+        if (t_cross) t_cross[i] = s->ev[i].t;
+        if (dir) dir[i] = s->ev[i].j;
+        if (t_coll) t_coll[i] = s->ev[n + i].t;
+        if (partner) partner[i] = s->ev[n + i].j;
+        if (coll_actual) coll_actual[i] = s->ev[n + i].coll_actual;
+    }
+}
+
+void orc_observables(orc_ctx* s, uint64_t* counters, double* sums) {
+    physical_q(s);
+    if (counters) {
+        counters[0] = s->ncol;
+        counters[1] = s->ncross;
+        counters[2] = s->nevents;
+    }
+    if (sums) {
+        sums[0] = s->t;
+        sums[1] = s->vir_xx;
+        sums[2] = s->vir_yy;
+        sums[3] = s->vir_xy;
+        sums[4] = s->vir_yx;
+        sums[5] = s->E;
+        sums[6] = s->px;
+        sums[7] = s->py;
+    }
+}
+
+int orc_thermo_rows(const orc_ctx* s, orc_thermo_row* out, int cap) {
+    int k = s->nrows < cap ? s->nrows : cap;
+    if (out) memcpy(out, s->rows, (size_t)k * sizeof *out);
+    return s->nrows;
+}
+
+double orc_collision_time(orc_ctx* s, int i, int j) { return coll_time(s, i, j); }
+
+void orc_predict_collision(orc_ctx* s, int i, double* t_abs, int32_t* partner) {
+    int p;
+    predict_collision(s, i, t_abs, &p);
+    *partner = p;
+}
+
+void orc_predict_crossing(orc_ctx* s, int i, double* t_abs, int32_t* dir) {
+    int d;
+    predict_crossing(s, i, t_abs, &d);
+    *dir = d;
+}
+
+void orc_sync_all(orc_ctx* s, double t) {
+    s->t = t;
+    for (int i = 0; i < s->n; i++) free_fly(s, i);
+}
+
+int orc_pcf(int n, const double* x, const double* y, double Lx, double Ly,
+            double dr, double max_r, double* r, double* g) {
+    /* calculate_pcf, pcf.c:16-75: O(N^2) histogram, each pair counted twice,
+     * normalised by the 2D shell area 2 pi r dr times density times N. */
+    int nb = (int)(max_r / dr);
+    double rmax = dr * nb;
+    double halfLx = Lx / 2, halfLy = Ly / 2;
+    for (int b = 0; b < nb; b++) {
+        r[b] = (b + 0.5) * dr;
+        g[b] = 0;
+    }
+    for (int i = 0; i < n; i++) {
+        for (int j = i + 1; j < n; j++) {
+            double dx = x[j] - x[i];
+            double dy = y[j] - y[i];
+            if (dx >= halfLx) dx -= Lx;
+            else if (dx < -halfLx) dx += Lx;
+            if (dy >= halfLy) dy -= Ly;
+            else if (dy < -halfLy) dy += Ly;
+            double rr = sqrt(dx * dx + dy * dy);
+            if (rr < rmax) {
+                int b = (int)(rr / dr);
+                if (b < nb) g[b] += 2.0;
+            }
+        }
+    }
+    double density = n / (Lx * Ly);
+    for (int b = 0; b < nb; b++) {
+        double shell = 2 * M_PI * r[b] * dr;
+        double norm = shell * density * n;
+        if (norm > 0) g[b] /= norm;
+        else g[b] = 0.0;
+    }
+    return nb;
+}

@@ -1,0 +1,217 @@
+/*
+ * Trace harness around the reference engine (test infrastructure only).
+ *
+ * This file is NOT part of the product.  It includes the reference's own
+ * translation unit (src/EDMD.c, compiled where it lies under the reference
+ * tree, see oracle/Makefile) and drives the same start-up sequence and event
+ * dispatch as the reference's main() (EDMD.c:498-618, G == 0 branch), with
+ * every file the reference would write sent to /dev/null.  It is used to
+ *   - write golden vectors (initial prediction sweep, event traces, final
+ *     states, virial sums, g(r) histograms) for the tests, and
+ *   - time the reference's own CPU path for the CPU baseline.
+ *
+ * Modes (all start from a LAMMPS dump loaded with "-l", EDMD.c:799-807,
+ * 987-995, 1387-1451):
+ *   sweep <dump> <out>                 initial sweep after eventListInit
+ *   trace <dump> <out> <nevents>       event trace + final state
+ *   pcf   <dump> <out> <dr> <max_r>    calculate_pcf on the loaded positions
+ *   bench <dump> <ncoll> [maxsec]      valid collisions / wall second
+ * Extra arguments after the mode arguments are passed to constantInit.
+ */
+#define main edmd_reference_main
+#include "EDMD.c"
+#undef main
+
+#include <stdint.h>
+#include <time.h>
+
+typedef struct {
+    int32_t type;
+    int32_t i;
+    int32_t j;
+    int32_t valid;
+    double t;
+} trace_rec;
+
+static void fail(const char* msg) {
+    fprintf(stderr, "ref_harness: %s\n", msg);
+    exit(2);
+}
+
+static void open_sinks(void) {
+    fichier = fopen("/dev/null", "w");
+    thermo = fopen("/dev/null", "w");
+    if (!fichier || !thermo) fail("cannot open /dev/null");
+}
+
+/* CS1: constantInit -> particlesInit -> cellListInit -> eventListInit ->
+ * physicalQ, as EDMD.c:508-518 (customName and initThermo only name and open
+ * output files and are replaced by open_sinks). */
+static void start(const char* dump, int nextra, char** extra) {
+    char* args[64];
+    int n = 0;
+    args[n++] = "edmd";
+    args[n++] = "-l";
+    args[n++] = (char*)dump;
+    args[n++] = "-t";
+    args[n++] = "1e300";
+    for (int k = 0; k < nextra && n < 63; k++) args[n++] = extra[k];
+    args[n] = NULL;
+    init_genrand(1);
+    constantInit(n, args);
+    configureTransverseVelocityCorrelationSampling(&dtimeThermo, &logSpacing);
+    particlesInit();
+    cellListInit();
+    eventListInit();
+    physicalQ();
+    open_sinks();
+}
+
+/* One pass of the dispatch switch of EDMD.c:534-610. */
+static int step(trace_rec* rec) {
+    nextEvent = findNextEvent();
+    rec->type = (int32_t)nextEvent->type;
+    rec->i = nextEvent->i;
+    rec->j = nextEvent->j;
+    rec->t = nextEvent->t;
+    rec->valid = 1;
+    if (nextEvent->type == COLLISION)
+        rec->valid = (nextEvent->collActual == particles[nextEvent->j].coll);
+    t = nextEvent->t;
+    removeEventFromQueue(nextEvent);
+    switch (nextEvent->type) {
+        case COLLISION: doTheCollision(); break;
+        case WALL: doTheWall(); break;
+        case CELLCROSS: ncross++; doTheCrossing(); break;
+        case ADDINGNOISE: addNoise(); break;
+        case SCREENSHOT: takeAScreenshot(); break;
+        case THERMO: takeAThermo(); break;
+        default: fail("unexpected event type");
+    }
+    return 0;
+}
+
+static void write_all(FILE* f, const void* p, size_t sz, size_t n) {
+    if (fwrite(p, sz, n, f) != n) fail("short write");
+}
+
+/* Final-state block: N, then per particle x y vx vy t (double), coll (u64),
+ * cell[0] cell[1] (int32), then ncol ncross (u64), t, virial sums, E px py. */
+static void write_state(FILE* f) {
+    int32_t n32 = N;
+    write_all(f, &n32, 4, 1);
+    for (int i = 0; i < N; i++) {
+        particle* p = particles + i;
+        double d[5] = {p->x, p->y, p->vx, p->vy, p->t};
+        uint64_t c = p->coll;
+        int32_t cell[2] = {p->cell[0], p->cell[1]};
+        write_all(f, d, 8, 5);
+        write_all(f, &c, 8, 1);
+        write_all(f, cell, 4, 2);
+    }
+    uint64_t counters[2] = {ncol, ncross};
+    write_all(f, counters, 8, 2);
+    physicalQ();
+    double g[8] = {t, collTermX, collTermY, collTermXY, collTermYX, E, px, py};
+    write_all(f, g, 8, 8);
+}
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 3) fail("usage: mode dump ...");
+    /* The reference prints a progress bar at every screenshot/thermo event. */
+    if (!freopen("/dev/null", "w", stdout)) fail("cannot silence stdout");
+    const char* mode = argv[1];
+    const char* dump = argv[2];
+
+    if (!strcmp(mode, "sweep")) {
+        if (argc < 4) fail("sweep dump out");
+        start(dump, argc - 4, argv + 4);
+        FILE* f = fopen(argv[3], "wb");
+        if (!f) fail("cannot open output");
+        int32_t n32 = N;
+        double box[2] = {Lx, Ly};
+        int32_t cells[2] = {Nxcells, Nycells};
+        write_all(f, &n32, 4, 1);
+        write_all(f, box, 8, 2);
+        write_all(f, cells, 4, 2);
+        for (int i = 0; i < N; i++) {
+            node* cr = eventList[i];
+            node* co = eventList[N + i];
+            double tt[2] = {cr->t, co->t};
+            int32_t jj[2] = {cr->j, co->j};
+            uint64_t ca = co->collActual;
+            int32_t cell[2] = {particles[i].cell[0], particles[i].cell[1]};
+            write_all(f, tt, 8, 2);
+            write_all(f, jj, 4, 2);
+            write_all(f, &ca, 8, 1);
+            write_all(f, cell, 4, 2);
+        }
+        fclose(f);
+        return 0;
+    }
+    if (!strcmp(mode, "trace")) {
+        if (argc < 5) fail("trace dump out nevents");
+        long nev = atol(argv[4]);
+        start(dump, argc - 5, argv + 5);
+        FILE* f = fopen(argv[3], "wb");
+        if (!f) fail("cannot open output");
+        int64_t n64 = nev;
+        write_all(f, &n64, 8, 1);
+        for (long k = 0; k < nev; k++) {
+            trace_rec rec;
+            step(&rec);
+            write_all(f, &rec, sizeof rec, 1);
+        }
+        write_state(f);
+        fclose(f);
+        return 0;
+    }
+    if (!strcmp(mode, "pcf")) {
+        if (argc < 6) fail("pcf dump out dr max_r");
+        double dr = atof(argv[4]);
+        double max_r = atof(argv[5]);
+        start(dump, argc - 6, argv + 6);
+        pcf_data* d = calculate_pcf(particles, N, dr, max_r, Lx, Ly);
+        if (!d) fail("calculate_pcf failed");
+        FILE* f = fopen(argv[3], "wb");
+        if (!f) fail("cannot open output");
+        int32_t nb = d->num_bins;
+        write_all(f, &nb, 4, 1);
+        write_all(f, d->r, 8, (size_t)nb);
+        write_all(f, d->g_r, 8, (size_t)nb);
+        fclose(f);
+        return 0;
+    }
+    if (!strcmp(mode, "bench")) {
+        if (argc < 4) fail("bench dump ncoll [maxsec]");
+        unsigned long target = strtoul(argv[3], NULL, 10);
+        double maxsec = argc > 4 ? atof(argv[4]) : 1e30;
+        start(dump, argc - 5 > 0 ? argc - 5 : 0, argv + 5);
+        unsigned long events = 0;
+        double t0 = now_s();
+        double el = 0;
+        trace_rec rec;
+        while (ncol < target) {
+            step(&rec);
+            events++;
+            if ((events & 0xffff) == 0) {
+                el = now_s() - t0;
+                if (el > maxsec) break;
+            }
+        }
+        el = now_s() - t0;
+        fprintf(stderr,
+                "{\"ncol\": %lu, \"events\": %lu, \"seconds\": %.6f, "
+                "\"coll_per_s\": %.6e, \"events_per_s\": %.6e, \"t\": %.17g}\n",
+                ncol, events, el, ncol / el, events / el, t);
+        return 0;
+    }
+    fail("unknown mode");
+    return 1;
+}

@@ -1,0 +1,202 @@
+"""Fixture helpers shared by the golden-vector generator and the tests.
+
+Inputs are plain numpy arrays (full FP64 precision).  The LAMMPS dump text
+written here is the input format the reference's load path reads
+(EDMD.c:799-807, 987-995, 1387-1451; parser.c:360-395, 444-479, 900-917):
+the box length is row 0 column 1 of the BOX section and atom columns are
+found by name prefix, so ``id type x y vx vy radius m coll`` is read back
+exactly when the values are printed with 17 significant digits.
+"""
+from __future__ import annotations
+
+import math
+import os
+import struct
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF_DIR = os.path.join(REPO, "oracle", "_ref")
+
+CELLCROSS, COLLISION, SCREENSHOT, THERMO, ADDINGNOISE = 0, 1, 2, 3, 4
+
+
+def box_length(n, phi, rad=0.5, aspect=1.0):
+    """Square-box side for n monodisperse disks of radius ``rad`` at area
+    fraction ``phi`` (the reference's sizing, EDMD.c:1150, written for a
+    general radius)."""
+    return math.sqrt(math.pi * n * rad * rad / (phi * aspect))
+
+
+def lattice_state(n, phi, seed, rad=0.5, lattice="square", temperature=1.0):
+    """Seeded initial state: lattice positions, Gaussian velocities with zero
+    total momentum scaled to E/N = temperature (the reference's
+    normalizePhysicalQ convention, EDMD.c:5723-5764, where E = sum m v^2 / 2).
+    Returns a dict of numpy arrays plus the box length."""
+    rng = np.random.default_rng(seed)
+    if lattice == "square":
+        L = box_length(n, phi, rad)
+        side = int(math.ceil(math.sqrt(n)))
+        a = L / side
+        k = np.arange(n)
+        x = (k % side + 0.5) * a
+        y = (k // side + 0.5) * a
+        Lx = Ly = L
+    elif lattice == "tri":
+        # Triangular lattice as in the reference's Hex start (EDMD.c:1024-1049,
+        # 1582-1637): Ny rows (even), Nx columns, alternate rows shifted.
+        ny = int(math.sqrt(n))
+        if ny % 2 == 1:
+            ny += 1
+        nx = n // ny
+        if nx * ny != n:
+            raise ValueError("triangular lattice needs n = nx*ny")
+        Ly = math.sqrt(math.sqrt(3) / 2 * ny / nx * math.pi * n * rad * rad / phi)
+        Lx = 2 / math.sqrt(3) * nx / ny * Ly
+        dx = Lx / nx
+        dy = Ly / ny
+        i = np.repeat(np.arange(nx), ny)
+        j = np.tile(np.arange(ny), nx)
+        x = i * dx + (j % 2) * (dx / 2) + dx / 4
+        y = j * dy + dy / 2
+        L = None
+    else:
+        raise ValueError(lattice)
+    v = rng.standard_normal((n, 2))
+    v -= v.mean(axis=0)
+    e_per = 0.5 * (v * v).sum() / n
+    v *= math.sqrt(temperature / e_per)
+    st = {
+        "x": np.ascontiguousarray(x, dtype=np.float64),
+        "y": np.ascontiguousarray(y, dtype=np.float64),
+        "vx": np.ascontiguousarray(v[:, 0]),
+        "vy": np.ascontiguousarray(v[:, 1]),
+        "rad": np.full(n, rad, dtype=np.float64),
+        "m": np.ones(n, dtype=np.float64),
+        "Lx": float(Lx if L is None else L),
+        "Ly": float(Ly if L is None else L),
+    }
+    return st
+
+
+def scale_state(st, f):
+    """Same system with all lengths multiplied by f (velocities unchanged):
+    collision times scale by f, so event order is unchanged."""
+    out = dict(st)
+    for k in ("x", "y", "rad"):
+        out[k] = st[k] * f
+    out["Lx"] = st["Lx"] * f
+    out["Ly"] = st["Ly"] * f
+    return out
+
+
+def write_dump(path, st):
+    n = len(st["x"])
+    lines = [
+        "ITEM: TIMESTEP", "0", "ITEM: NUMBER OF ATOMS", str(n),
+        "ITEM: BOX BOUNDS pp pp pp",
+        "0 %.17g" % st["Lx"], "0 %.17g" % st["Ly"], "0 0",
+        "ITEM: ATOMS id type x y vx vy radius m coll",
+    ]
+    for i in range(n):
+        lines.append("%d 1 %.17g %.17g %.17g %.17g %.17g %.17g 0" % (
+            i, st["x"][i], st["y"][i], st["vx"][i], st["vy"][i],
+            st["rad"][i], st["m"][i]))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def read_dump(path):
+    """Reads the last frame of a dump written by write_dump or by the
+    library's writer (columns located by name, as parser.c:900-917)."""
+    with open(path) as f:
+        text = f.read().split("\n")
+    starts = [k for k, l in enumerate(text) if l.startswith("ITEM: TIMESTEP")]
+    k = starts[-1]
+    n = int(text[k + 3])
+    Lx = float(text[k + 5].split()[1])
+    Ly = float(text[k + 6].split()[1])
+    cols = text[k + 8].split()[2:]
+    rows = [text[k + 9 + i].split() for i in range(n)]
+
+    def col(name):
+        idx = next(c for c, nm in enumerate(cols) if nm.startswith(name))
+        return np.array([float(r[idx]) for r in rows])
+    return {"x": col("x"), "y": col("y"), "vx": col("vx"), "vy": col("vy"),
+            "rad": col("radius"), "m": col("m"), "Lx": Lx, "Ly": Ly}
+
+
+# ---------------------------------------------------------------- harness I/O
+
+def ref_binary(kind="strict"):
+    return os.path.join(REF_DIR, "ref_" + kind)
+
+
+def run_ref(args, kind="strict", timeout=600):
+    return subprocess.run([ref_binary(kind)] + [str(a) for a in args],
+                          check=True, capture_output=True, text=True,
+                          timeout=timeout)
+
+
+SWEEP_REC = np.dtype([("t_cross", "<f8"), ("t_coll", "<f8"), ("dir", "<i4"),
+                      ("partner", "<i4"), ("coll_actual", "<u8"),
+                      ("cx", "<i4"), ("cy", "<i4")])
+TRACE_REC = np.dtype([("type", "<i4"), ("i", "<i4"), ("j", "<i4"),
+                      ("valid", "<i4"), ("t", "<f8")])
+STATE_REC = np.dtype([("x", "<f8"), ("y", "<f8"), ("vx", "<f8"), ("vy", "<f8"),
+                      ("t", "<f8"), ("coll", "<u8"), ("cx", "<i4"),
+                      ("cy", "<i4")])
+
+
+def parse_sweep(path):
+    with open(path, "rb") as f:
+        raw = f.read()
+    n, = struct.unpack_from("<i", raw, 0)
+    Lx, Ly = struct.unpack_from("<dd", raw, 4)
+    ncx, ncy = struct.unpack_from("<ii", raw, 20)
+    rec = np.frombuffer(raw, dtype=SWEEP_REC, count=n, offset=28)
+    return {"N": n, "Lx": Lx, "Ly": Ly, "ncx": ncx, "ncy": ncy, "rec": rec.copy()}
+
+
+def parse_trace(path):
+    with open(path, "rb") as f:
+        raw = f.read()
+    nev, = struct.unpack_from("<q", raw, 0)
+    off = 8
+    tr = np.frombuffer(raw, dtype=TRACE_REC, count=nev, offset=off).copy()
+    off += nev * TRACE_REC.itemsize
+    n, = struct.unpack_from("<i", raw, off)
+    off += 4
+    state = np.frombuffer(raw, dtype=STATE_REC, count=n, offset=off).copy()
+    off += n * STATE_REC.itemsize
+    ncol, ncross = struct.unpack_from("<QQ", raw, off)
+    off += 16
+    g = struct.unpack_from("<8d", raw, off)
+    glob = dict(zip(["t", "virXX", "virYY", "virXY", "virYX", "E", "px", "py"], g))
+    glob["ncol"] = ncol
+    glob["ncross"] = ncross
+    return {"trace": tr, "state": state, "glob": glob}
+
+
+def parse_pcf(path):
+    with open(path, "rb") as f:
+        raw = f.read()
+    nb, = struct.unpack_from("<i", raw, 0)
+    r = np.frombuffer(raw, dtype="<f8", count=nb, offset=4).copy()
+    g = np.frombuffer(raw, dtype="<f8", count=nb, offset=4 + 8 * nb).copy()
+    return r, g
+
+
+def canonical_pairs(tr):
+    """Collision pairs as unordered (min, max): the mirrored events (i, j) and
+    (j, i) of a pair predicted from a common synchronised state carry equal
+    times and either may pop first (EDMD.c:2179, 2164-2168)."""
+    out = tr.copy()
+    coll = tr["type"] == COLLISION
+    lo = np.minimum(tr["i"], tr["j"])
+    hi = np.maximum(tr["i"], tr["j"])
+    out["i"] = np.where(coll, lo, tr["i"])
+    out["j"] = np.where(coll, hi, tr["j"])
+    return out

@@ -1,0 +1,196 @@
+"""ctypes binding of the oracle (oracle/build/liboracle.so).
+
+Test infrastructure: the oracle is the checker the GPU library is compared
+against, never a code path of the library itself.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(REPO, "oracle", "build", "liboracle.so")
+
+NEVER = 1e26
+
+
+class Params(C.Structure):
+    _fields_ = [("res", C.c_double), ("first_screen", C.c_double),
+                ("dtime", C.c_double), ("first_thermo", C.c_double),
+                ("dtime_thermo", C.c_double), ("noise", C.c_int),
+                ("dtnoise", C.c_double), ("temperature", C.c_double),
+                ("bucket_count", C.c_int)]
+
+
+class ThermoRow(C.Structure):
+    _fields_ = [("t", C.c_double), ("ncol", C.c_uint64), ("e_per_n", C.c_double),
+                ("pressure", C.c_double), ("pxx", C.c_double), ("pyy", C.c_double),
+                ("pxy", C.c_double), ("pyx", C.c_double)]
+
+
+TRACE_DT = np.dtype([("type", "<i4"), ("i", "<i4"), ("j", "<i4"),
+                     ("valid", "<i4"), ("t", "<f8")])
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        L.orc_default_params.argtypes = [C.POINTER(Params)]
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_int, C.c_double, C.c_double] + [dp] * 6 + [C.POINTER(Params)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_init_events.argtypes = [C.c_void_p]
+        L.orc_run.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_int64)]
+        L.orc_run_until.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.POINTER(C.c_int64)]
+        L.orc_run_collisions.argtypes = [C.c_void_p, C.c_uint64, C.c_int64, C.POINTER(C.c_int64)]
+        L.orc_n.argtypes = [C.c_void_p]
+        L.orc_time.restype = C.c_double
+        L.orc_time.argtypes = [C.c_void_p]
+        L.orc_status.argtypes = [C.c_void_p]
+        L.orc_grid.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), dp, dp]
+        L.orc_get_state.argtypes = [C.c_void_p] + [C.c_void_p] * 8
+        L.orc_get_events.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+        L.orc_observables.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_thermo_rows.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_collision_time.restype = C.c_double
+        L.orc_collision_time.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_predict_collision.argtypes = [C.c_void_p, C.c_int, dp, C.POINTER(C.c_int32)]
+        L.orc_predict_crossing.argtypes = [C.c_void_p, C.c_int, dp, C.POINTER(C.c_int32)]
+        L.orc_sync_all.argtypes = [C.c_void_p, C.c_double]
+        L.orc_pcf.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double,
+                              C.c_double, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def default_params(**kw):
+    p = Params()
+    lib().orc_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Oracle:
+    """One reference-semantics system on the CPU."""
+
+    def __init__(self, st, init_events=True, **params):
+        self._keep = {k: np.ascontiguousarray(st[k], dtype=np.float64)
+                      for k in ("x", "y", "vx", "vy", "rad", "m")}
+        self.n = len(self._keep["x"])
+        self.params = default_params(**params)
+        k = self._keep
+        h = lib().orc_create(self.n, st["Lx"], st["Ly"], _dp(k["x"]), _dp(k["y"]),
+                             _dp(k["vx"]), _dp(k["vy"]), _dp(k["rad"]), _dp(k["m"]),
+                             C.byref(self.params))
+        if not h:
+            raise ValueError("orc_create failed")
+        self.h = h
+        if init_events:
+            self.status = lib().orc_init_events(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def run(self, nevents, trace=True):
+        tr = np.zeros(nevents, dtype=TRACE_DT) if trace else None
+        done = C.c_int64()
+        st = lib().orc_run(self.h, nevents, tr.ctypes.data if trace else None, C.byref(done))
+        if trace:
+            tr = tr[:done.value]
+        return st, tr
+
+    def run_until(self, tmax, max_events=1 << 62):
+        done = C.c_int64()
+        st = lib().orc_run_until(self.h, tmax, max_events, C.byref(done))
+        return st, done.value
+
+    def run_collisions(self, target, max_events=1 << 62):
+        done = C.c_int64()
+        st = lib().orc_run_collisions(self.h, target, max_events, C.byref(done))
+        return st, done.value
+
+    @property
+    def t(self):
+        return lib().orc_time(self.h)
+
+    def grid(self):
+        ncx, ncy = C.c_int(), C.c_int()
+        csx, csy = C.c_double(), C.c_double()
+        lib().orc_grid(self.h, C.byref(ncx), C.byref(ncy), C.byref(csx), C.byref(csy))
+        return ncx.value, ncy.value, csx.value, csy.value
+
+    def state(self):
+        n = self.n
+        out = {k: np.zeros(n) for k in ("x", "y", "vx", "vy", "t")}
+        out["coll"] = np.zeros(n, dtype=np.uint64)
+        out["cx"] = np.zeros(n, dtype=np.int32)
+        out["cy"] = np.zeros(n, dtype=np.int32)
+        lib().orc_get_state(self.h, *[out[k].ctypes.data for k in
+                                      ("x", "y", "vx", "vy", "t", "coll", "cx", "cy")])
+        return out
+
+    def events(self):
+        n = self.n
+        out = {"t_cross": np.zeros(n), "dir": np.zeros(n, dtype=np.int32),
+               "t_coll": np.zeros(n), "partner": np.zeros(n, dtype=np.int32),
+               "coll_actual": np.zeros(n, dtype=np.uint64)}
+        lib().orc_get_events(self.h, *[out[k].ctypes.data for k in
+                                       ("t_cross", "dir", "t_coll", "partner", "coll_actual")])
+        return out
+
+    def observables(self):
+        cnt = np.zeros(3, dtype=np.uint64)
+        sums = np.zeros(8)
+        lib().orc_observables(self.h, cnt.ctypes.data, sums.ctypes.data)
+        names = ["t", "virXX", "virYY", "virXY", "virYX", "E", "px", "py"]
+        out = dict(zip(names, sums.tolist()))
+        out.update(ncol=int(cnt[0]), ncross=int(cnt[1]), nevents=int(cnt[2]))
+        return out
+
+    def thermo_rows(self):
+        n = lib().orc_thermo_rows(self.h, None, 0)
+        arr = (ThermoRow * max(n, 1))()
+        lib().orc_thermo_rows(self.h, arr, n)
+        return [dict(t=r.t, ncol=r.ncol, e_per_n=r.e_per_n, pressure=r.pressure,
+                     pxx=r.pxx, pyy=r.pyy, pxy=r.pxy, pyx=r.pyx) for r in arr[:n]]
+
+    def collision_time(self, i, j):
+        return lib().orc_collision_time(self.h, i, j)
+
+    def predict_collision(self, i):
+        t = C.c_double()
+        p = C.c_int32()
+        lib().orc_predict_collision(self.h, i, C.byref(t), C.byref(p))
+        return t.value, p.value
+
+    def predict_crossing(self, i):
+        t = C.c_double()
+        d = C.c_int32()
+        lib().orc_predict_crossing(self.h, i, C.byref(t), C.byref(d))
+        return t.value, d.value
+
+    def sync_all(self, t):
+        lib().orc_sync_all(self.h, t)
+
+
+def pcf(x, y, Lx, Ly, dr, max_r):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    nb = int(max_r / dr)
+    r = np.zeros(max(nb, 1))
+    g = np.zeros(max(nb, 1))
+    nb = lib().orc_pcf(len(x), _dp(x), _dp(y), Lx, Ly, dr, max_r, _dp(r), _dp(g))
+    return r[:nb], g[:nb]
