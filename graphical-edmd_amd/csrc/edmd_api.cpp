@@ -1,0 +1,817 @@
+/*
+ * Host side of libedmd_hip.so: handle management, device layout, launches,
+ * lattice initial states and LAMMPS dump I/O.  See include/edmd.h.
+ */
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/edmd.h"
+#include "edmd_internal.h"
+#include "edmd_types.h"
+
+/* ------------------------------------------------------------ errors */
+
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(EDMD_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+    } while (0)
+
+extern "C" const char* edmd_last_error(void) { return g_err; }
+
+extern "C" int edmd_device_count(int* n) {
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) c = 0;
+    if (n) *n = c;
+    return c > 0 ? EDMD_OK : fail(EDMD_ERR_NODEVICE, "no HIP device visible");
+}
+
+extern "C" void edmd_default_desc(edmd_system_desc* d) {
+    /* reference defaults, EDMD.c:244-248, 372, 379, 386 */
+    memset(d, 0, sizeof *d);
+    d->res = 1;
+    d->first_screen = 1;
+    d->dtime = 100;
+    d->first_thermo = 1000;
+    d->dtime_thermo = 100;
+    d->noise = 0;
+    d->dtnoise = 0.5;
+    d->temperature = 1;
+}
+
+/* ------------------------------------------------------------ handles */
+
+struct HostSys {
+    EdmdConst c;
+    size_t p0;     /* first particle in the pooled particle arrays */
+    size_t cell0;  /* first cell */
+    size_t slot0;  /* first cell slot */
+    size_t node0[EDMD_MAX_LEVELS];
+    size_t trace0; /* first trace record */
+    size_t thermo0;
+};
+
+struct edmd_batch {
+    int nsys = 0;
+    std::vector<HostSys> hs;
+    std::vector<EdmdSys> sys;
+    EdmdSys* d_sys = nullptr;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int max_n = 0, max_nlev = 0;
+    int max_lsize[EDMD_MAX_LEVELS + 1] = {0};
+    size_t np = 0, ncell = 0, nslot = 0, nnode = 0, ntrace = 0, nthermo = 0;
+    std::vector<void*> allocs;
+    /* pooled device arrays */
+    double *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr, *tl = nullptr,
+           *rad = nullptr, *m = nullptr, *tcoll = nullptr, *tcross = nullptr, *key = nullptr;
+    uint32_t *coll = nullptr, *cact = nullptr;
+    int32_t *cx = nullptr, *cy = nullptr, *crx = nullptr, *cry = nullptr, *partner = nullptr,
+            *dir = nullptr;
+    double* lkey = nullptr;
+    int32_t* lidx = nullptr;
+    int32_t *ccnt = nullptr, *cslot = nullptr;
+    EdmdState* st = nullptr;
+    EdmdTraceRec* trace = nullptr;
+    EdmdThermoRow* thermo = nullptr;
+};
+
+struct edmd_ctx {
+    edmd_batch* b;
+};
+
+template <class T>
+static int dalloc(edmd_batch* b, T** p, size_t n) {
+    if (n == 0) n = 1;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess) return fail(EDMD_ERR_NOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+    b->allocs.push_back(q);
+    *p = (T*)q;
+    return EDMD_OK;
+}
+
+static void batch_free(edmd_batch* b) {
+    if (!b) return;
+    for (void* p : b->allocs) (void)hipFree(p);
+    if (b->d_sys) (void)hipFree(b->d_sys);
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+static double wrap_post(double v, double L) {
+    /* PBCpostX/Y, EDMD.c:5948-5959 */
+    if (v < 0) return v + L;
+    if (v >= L) return v - L;
+    return v;
+}
+
+static int coord_to_cell(double a, double fac, int nc) {
+    /* coordToCell, EDMD.c:2098-2107, clamped to the grid */
+    int c = (int)(a * fac);
+    if (c >= nc) c = nc - 1;
+    if (c < 0) c = 0;
+    return c;
+}
+
+/* Largest number of disk centres a cell can hold: centres are at least the
+ * smallest contact distance apart (less 1% for the overlap tolerance of
+ * EDMD.c:2708), so at most (s/d + 1)^2 per square cell of side s, with 20%
+ * slack for hexagonal packing. */
+static int derived_capacity(double csx, double csy, double rmin) {
+    double d = 2 * rmin * 0.99;
+    double b = ceil((csx / d + 1) * (csy / d + 1) * 1.2);
+    int cap = (int)b;
+    if (cap < 4) cap = 4;
+    cap = (cap + 3) & ~3;
+    return cap;
+}
+
+static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
+    if (!d) return fail(EDMD_ERR_ARG, "system %d: null descriptor", idx);
+    if (d->n < 2) return fail(EDMD_ERR_ARG, "system %d: need at least 2 disks", idx);
+    if (!(d->Lx > 0) || !(d->Ly > 0)) return fail(EDMD_ERR_ARG, "system %d: box must be positive", idx);
+    if (!d->x || !d->y || !d->vx || !d->vy || !d->rad || !d->m)
+        return fail(EDMD_ERR_ARG, "system %d: missing particle arrays", idx);
+    EdmdConst& c = h->c;
+    memset(&c, 0, sizeof c);
+    c.n = d->n;
+    /* boxConstantHelper, EDMD.c:679-715 */
+    c.Lx = d->Lx;
+    c.Ly = d->Ly;
+    c.halfLx = d->Lx / 2;
+    c.halfLy = d->Ly / 2;
+    c.ncx = (int)(c.halfLx);
+    c.ncy = (int)(c.halfLy);
+    if (c.ncx < 1 || c.ncy < 1) return fail(EDMD_ERR_ARG, "system %d: box side below 2 (no cell grid)", idx);
+    if ((double)c.ncx * c.ncy > 2.0e9) return fail(EDMD_ERR_ARG, "system %d: cell grid too large", idx);
+    c.csx = d->Lx / c.ncx;
+    c.csy = d->Ly / c.ncy;
+    c.res = d->res;
+    c.dtime = d->dtime;
+    c.dtime_thermo = d->dtime_thermo;
+    c.noise = d->noise;
+    c.dtnoise = d->dtnoise;
+    c.temperature = d->temperature;
+    if (c.noise != 0 && c.noise != 2) return fail(EDMD_ERR_ARG, "system %d: noise must be 0 or 2", idx);
+    if (d->trace_capacity < 0 || d->trace_capacity > (int64_t)1 << 31)
+        return fail(EDMD_ERR_ARG, "system %d: bad trace capacity", idx);
+    c.trace_cap = (int32_t)d->trace_capacity;
+    c.thermo_cap = d->thermo_capacity > 0 ? d->thermo_capacity : 4096;
+    double rmin = d->rad[0];
+    for (int i = 0; i < d->n; i++) {
+        if (!(d->rad[i] > 0) || !(d->m[i] > 0))
+            return fail(EDMD_ERR_ARG, "system %d: particle %d has non-positive radius or mass", idx, i);
+        rmin = std::min(rmin, d->rad[i]);
+    }
+    int cap = d->cell_capacity > 0 ? d->cell_capacity
+                                   : derived_capacity(c.csx, c.csy, rmin);
+    if (cap > 64) cap = 64;
+    c.cap = cap;
+    /* calendar levels */
+    c.lsize[0] = c.n;
+    int l = 0;
+    while (c.lsize[l] > 64) {
+        if (l + 1 > EDMD_MAX_LEVELS) return fail(EDMD_ERR_ARG, "system %d: too many disks", idx);
+        c.lsize[l + 1] = (c.lsize[l] + 63) / 64;
+        l++;
+    }
+    c.nlev = l;
+    return EDMD_OK;
+}
+
+static int upload_system(edmd_batch* b, int s, const edmd_system_desc* d) {
+    HostSys& h = b->hs[s];
+    const EdmdConst& c = h.c;
+    const int n = c.n;
+    const double cfx = 1 / c.csx, cfy = 1 / c.csy;
+    std::vector<double> x(n), y(n);
+    std::vector<int32_t> cx(n), cy(n);
+    const size_t ncell = (size_t)c.ncx * c.ncy;
+    std::vector<int32_t> cnt(ncell, 0), slot(ncell * c.cap, -1);
+    for (int i = 0; i < n; i++) {
+        /* load path wraps positions into the box, EDMD.c:1429-1430 */
+        x[i] = wrap_post(d->x[i], c.Lx);
+        y[i] = wrap_post(d->y[i], c.Ly);
+        cx[i] = coord_to_cell(x[i], cfx, c.ncx);
+        cy[i] = coord_to_cell(y[i], cfy, c.ncy);
+        size_t cell = (size_t)cy[i] * c.ncx + cx[i];
+        if (cnt[cell] >= c.cap)
+            return fail(EDMD_ERR_CELL_OVERFLOW,
+                        "system %d: cell (%d,%d) holds more than %d disks; raise cell_capacity",
+                        s, cx[i], cy[i], c.cap);
+        /* cellListInit adds particles in index order, EDMD.c:1916-1919 */
+        slot[cell * c.cap + cnt[cell]++] = i;
+    }
+    const size_t p0 = h.p0;
+    const size_t nb = (size_t)n * sizeof(double);
+    HIPCHK(hipMemcpy(b->x + p0, x.data(), nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->y + p0, y.data(), nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->vx + p0, d->vx, nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->vy + p0, d->vy, nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->rad + p0, d->rad, nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->m + p0, d->m, nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(b->tl + p0, 0, nb));
+    HIPCHK(hipMemset(b->coll + p0, 0, (size_t)n * sizeof(uint32_t)));
+    HIPCHK(hipMemset(b->crx + p0, 0, (size_t)n * sizeof(int32_t)));
+    HIPCHK(hipMemset(b->cry + p0, 0, (size_t)n * sizeof(int32_t)));
+    HIPCHK(hipMemcpy(b->cx + p0, cx.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->cy + p0, cy.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->ccnt + h.cell0, cnt.data(), ncell * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->cslot + h.slot0, slot.data(), ncell * c.cap * sizeof(int32_t),
+                     hipMemcpyHostToDevice));
+    EdmdState st;
+    memset(&st, 0, sizeof st);
+    st.t = 0;
+    st.next_screen = d->first_screen;
+    st.next_thermo = d->first_thermo + 0.000000001; /* EDMD.c:1970 */
+    st.next_noise = d->noise ? d->dtnoise : __builtin_huge_val();
+    st.last_thermo = 0;
+    HIPCHK(hipMemcpy(b->st + s, &st, sizeof st, hipMemcpyHostToDevice));
+    return EDMD_OK;
+}
+
+static int batch_predict_all(edmd_batch* b) {
+    HIPCHK(edmd_launch_sweep(b->d_sys, b->nsys, b->max_n, b->stream));
+    HIPCHK(edmd_launch_tree(b->d_sys, b->nsys, b->max_lsize, b->max_nlev, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return EDMD_OK;
+}
+
+static int check_status(edmd_batch* b) {
+    std::vector<EdmdState> st(b->nsys);
+    HIPCHK(hipMemcpy(st.data(), b->st, b->nsys * sizeof(EdmdState), hipMemcpyDeviceToHost));
+    for (int s = 0; s < b->nsys; s++) {
+        if (st[s].status == EDMD_DEV_OVERLAP)
+            return fail(EDMD_ERR_OVERLAP, "system %d: overlapping disks near particle %d at t=%.17g",
+                        s, st[s].err_i, st[s].t);
+        if (st[s].status == EDMD_DEV_CELL_OVERFLOW)
+            return fail(EDMD_ERR_CELL_OVERFLOW, "system %d: cell %d overflow (particle %d) at t=%.17g",
+                        s, st[s].err_j, st[s].err_i, st[s].t);
+        if (st[s].status != 0) return fail(EDMD_ERR_STATE, "system %d: status %d", s, st[s].status);
+    }
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_batch** out) {
+    if (!out) return fail(EDMD_ERR_ARG, "null output handle");
+    *out = nullptr;
+    if (!descs || nsys <= 0) return fail(EDMD_ERR_ARG, "need at least one system");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(EDMD_ERR_NODEVICE, "no HIP device visible");
+    edmd_batch* b = new edmd_batch();
+    b->nsys = nsys;
+    b->hs.resize(nsys);
+    for (int s = 0; s < nsys; s++) {
+        int rc = setup_system(descs + s, &b->hs[s], s);
+        if (rc) { batch_free(b); return rc; }
+        HostSys& h = b->hs[s];
+        h.p0 = b->np;
+        b->np += h.c.n;
+        h.cell0 = b->ncell;
+        b->ncell += (size_t)h.c.ncx * h.c.ncy;
+        h.slot0 = b->nslot;
+        b->nslot += (size_t)h.c.ncx * h.c.ncy * h.c.cap;
+        for (int l = 1; l <= h.c.nlev; l++) {
+            h.node0[l - 1] = b->nnode;
+            b->nnode += h.c.lsize[l];
+        }
+        h.trace0 = b->ntrace;
+        b->ntrace += h.c.trace_cap;
+        h.thermo0 = b->nthermo;
+        b->nthermo += h.c.thermo_cap;
+        b->max_n = std::max(b->max_n, h.c.n);
+        b->max_nlev = std::max(b->max_nlev, h.c.nlev);
+        for (int l = 0; l <= h.c.nlev; l++) b->max_lsize[l] = std::max(b->max_lsize[l], h.c.lsize[l]);
+    }
+    int rc = EDMD_OK;
+    const size_t np = b->np;
+    if (!rc) rc = dalloc(b, &b->x, np);
+    if (!rc) rc = dalloc(b, &b->y, np);
+    if (!rc) rc = dalloc(b, &b->vx, np);
+    if (!rc) rc = dalloc(b, &b->vy, np);
+    if (!rc) rc = dalloc(b, &b->tl, np);
+    if (!rc) rc = dalloc(b, &b->rad, np);
+    if (!rc) rc = dalloc(b, &b->m, np);
+    if (!rc) rc = dalloc(b, &b->tcoll, np);
+    if (!rc) rc = dalloc(b, &b->tcross, np);
+    if (!rc) rc = dalloc(b, &b->key, np);
+    if (!rc) rc = dalloc(b, &b->coll, np);
+    if (!rc) rc = dalloc(b, &b->cact, np);
+    if (!rc) rc = dalloc(b, &b->cx, np);
+    if (!rc) rc = dalloc(b, &b->cy, np);
+    if (!rc) rc = dalloc(b, &b->crx, np);
+    if (!rc) rc = dalloc(b, &b->cry, np);
+    if (!rc) rc = dalloc(b, &b->partner, np);
+    if (!rc) rc = dalloc(b, &b->dir, np);
+    if (!rc) rc = dalloc(b, &b->lkey, b->nnode);
+    if (!rc) rc = dalloc(b, &b->lidx, b->nnode);
+    if (!rc) rc = dalloc(b, &b->ccnt, b->ncell);
+    if (!rc) rc = dalloc(b, &b->cslot, b->nslot);
+    if (!rc) rc = dalloc(b, &b->st, (size_t)nsys);
+    if (!rc) rc = dalloc(b, &b->trace, b->ntrace);
+    if (!rc) rc = dalloc(b, &b->thermo, b->nthermo);
+    if (rc) { batch_free(b); return rc; }
+    if (hipStreamCreate(&b->stream) != hipSuccess) {
+        batch_free(b);
+        return fail(EDMD_ERR_HIP, "hipStreamCreate failed");
+    }
+    b->own_stream = true;
+    b->sys.resize(nsys);
+    for (int s = 0; s < nsys; s++) {
+        HostSys& h = b->hs[s];
+        EdmdSys& S = b->sys[s];
+        memset(&S, 0, sizeof S);
+        S.c = h.c;
+        EdmdArrays& A = S.a;
+        size_t p0 = h.p0;
+        A.x = b->x + p0; A.y = b->y + p0; A.vx = b->vx + p0; A.vy = b->vy + p0;
+        A.tl = b->tl + p0; A.rad = b->rad + p0; A.m = b->m + p0;
+        A.coll = b->coll + p0; A.cx = b->cx + p0; A.cy = b->cy + p0;
+        A.crx = b->crx + p0; A.cry = b->cry + p0;
+        A.tcoll = b->tcoll + p0; A.partner = b->partner + p0; A.cact = b->cact + p0;
+        A.tcross = b->tcross + p0; A.dir = b->dir + p0; A.key = b->key + p0;
+        for (int l = 1; l <= h.c.nlev; l++) {
+            A.lkey[l - 1] = b->lkey + h.node0[l - 1];
+            A.lidx[l - 1] = b->lidx + h.node0[l - 1];
+        }
+        A.ccnt = b->ccnt + h.cell0;
+        A.cslot = b->cslot + h.slot0;
+        A.st = b->st + s;
+        A.trace = b->trace + h.trace0;
+        A.thermo = b->thermo + h.thermo0;
+        rc = upload_system(b, s, descs + s);
+        if (rc) { batch_free(b); return rc; }
+    }
+    if (hipMalloc(&b->d_sys, nsys * sizeof(EdmdSys)) != hipSuccess) {
+        b->d_sys = nullptr;
+        batch_free(b);
+        return fail(EDMD_ERR_NOMEM, "hipMalloc of system table failed");
+    }
+    if (hipMemcpy(b->d_sys, b->sys.data(), nsys * sizeof(EdmdSys), hipMemcpyHostToDevice) != hipSuccess) {
+        batch_free(b);
+        return fail(EDMD_ERR_HIP, "upload of system table failed");
+    }
+    rc = batch_predict_all(b);
+    if (!rc) rc = check_status(b);
+    if (rc) { batch_free(b); return rc; }
+    *out = b;
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_destroy(edmd_batch* b) {
+    batch_free(b);
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_size(edmd_batch* b, int* nsys) {
+    if (!b || !nsys) return fail(EDMD_ERR_ARG, "null argument");
+    *nsys = b->nsys;
+    return EDMD_OK;
+}
+
+extern "C" void* edmd_batch_stream(edmd_batch* b) { return b ? (void*)b->stream : nullptr; }
+
+extern "C" int edmd_batch_set_stream(edmd_batch* b, void* stream) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (b->own_stream) (void)hipStreamDestroy(b->stream);
+    b->stream = (hipStream_t)stream;
+    b->own_stream = false;
+    return EDMD_OK;
+}
+
+static int batch_run_mask(edmd_batch* b, int64_t max_events, uint64_t max_collisions, double tmax,
+                          int stop_mask) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    EdmdRunArgs ra;
+    memset(&ra, 0, sizeof ra);
+    ra.max_events = max_events;
+    ra.max_coll = max_collisions;
+    ra.tmax = tmax;
+    ra.stop_mask = stop_mask;
+    HIPCHK(edmd_launch_run(b->d_sys, b->nsys, ra, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return check_status(b);
+}
+
+extern "C" int edmd_batch_run(edmd_batch* b, int64_t max_events, uint64_t max_collisions,
+                              double tmax) {
+    return batch_run_mask(b, max_events, max_collisions, tmax, 0);
+}
+
+extern "C" int edmd_batch_sweep_predict(edmd_batch* b) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    HIPCHK(edmd_launch_sync_all(b->d_sys, b->nsys, b->max_n, b->stream));
+    int rc = batch_predict_all(b);
+    if (rc) return rc;
+    return check_status(b);
+}
+
+extern "C" int edmd_batch_time_sweep(edmd_batch* b, int reps, double* avg_ms) {
+    if (!b || reps <= 0 || !avg_ms) return fail(EDMD_ERR_ARG, "bad arguments");
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, b->stream));
+    for (int r = 0; r < reps; r++) HIPCHK(edmd_launch_sweep(b->d_sys, b->nsys, b->max_n, b->stream));
+    HIPCHK(hipEventRecord(e1, b->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
+    return check_status(b);
+}
+
+static int sys_check(edmd_batch* b, int s) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    if (s < 0 || s >= b->nsys) return fail(EDMD_ERR_ARG, "system index %d out of range", s);
+    return EDMD_OK;
+}
+
+template <class T>
+static int d2h(T* dst, const T* src, size_t n) {
+    if (!dst) return EDMD_OK;
+    HIPCHK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyDeviceToHost));
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_get_state(edmd_batch* b, int s, double* x, double* y, double* vx,
+                                    double* vy, double* t_local, uint32_t* coll, int32_t* cx,
+                                    int32_t* cy) {
+    int rc = sys_check(b, s);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const HostSys& h = b->hs[s];
+    size_t n = h.c.n, p0 = h.p0;
+    if ((rc = d2h(x, b->x + p0, n))) return rc;
+    if ((rc = d2h(y, b->y + p0, n))) return rc;
+    if ((rc = d2h(vx, b->vx + p0, n))) return rc;
+    if ((rc = d2h(vy, b->vy + p0, n))) return rc;
+    if ((rc = d2h(t_local, b->tl + p0, n))) return rc;
+    if ((rc = d2h(coll, b->coll + p0, n))) return rc;
+    if ((rc = d2h(cx, b->cx + p0, n))) return rc;
+    return d2h(cy, b->cy + p0, n);
+}
+
+extern "C" int edmd_batch_get_events(edmd_batch* b, int s, double* t_coll, int32_t* partner,
+                                     double* t_cross, int32_t* dir) {
+    int rc = sys_check(b, s);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const HostSys& h = b->hs[s];
+    size_t n = h.c.n, p0 = h.p0;
+    if ((rc = d2h(t_coll, b->tcoll + p0, n))) return rc;
+    if ((rc = d2h(partner, b->partner + p0, n))) return rc;
+    if ((rc = d2h(t_cross, b->tcross + p0, n))) return rc;
+    return d2h(dir, b->dir + p0, n);
+}
+
+extern "C" int edmd_batch_get_observables(edmd_batch* b, int s, edmd_observables* o) {
+    int rc = sys_check(b, s);
+    if (rc) return rc;
+    if (!o) return fail(EDMD_ERR_ARG, "null output");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const HostSys& h = b->hs[s];
+    EdmdState st;
+    HIPCHK(hipMemcpy(&st, b->st + s, sizeof st, hipMemcpyDeviceToHost));
+    size_t n = h.c.n;
+    std::vector<double> vx(n), vy(n), m(n);
+    if ((rc = d2h(vx.data(), b->vx + h.p0, n))) return rc;
+    if ((rc = d2h(vy.data(), b->vy + h.p0, n))) return rc;
+    if ((rc = d2h(m.data(), b->m + h.p0, n))) return rc;
+    /* physicalQ, EDMD.c:5968-6013, summed in particle order */
+    double E = 0, px = 0, py = 0;
+    for (size_t i = 0; i < n; i++) {
+        px += vx[i] * m[i];
+        py += vy[i] * m[i];
+        double a = vx[i];
+        E += 0.5 * m[i] * (a * a + vy[i] * vy[i]);
+    }
+    o->t = st.t;
+    o->ncol = st.ncol;
+    o->ncross = st.ncross;
+    o->nevents = st.nevents;
+    o->vir_xx = st.vir[0];
+    o->vir_yy = st.vir[1];
+    o->vir_xy = st.vir[2];
+    o->vir_yx = st.vir[3];
+    o->E = E;
+    o->px = px;
+    o->py = py;
+    o->status = st.status;
+    o->stop_reason = st.stop_reason;
+    o->err_i = st.err_i;
+    o->err_j = st.err_j;
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_get_trace(edmd_batch* b, int s, edmd_trace_rec* out, int64_t cap,
+                                    int64_t* n) {
+    int rc = sys_check(b, s);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const HostSys& h = b->hs[s];
+    EdmdState st;
+    HIPCHK(hipMemcpy(&st, b->st + s, sizeof st, hipMemcpyDeviceToHost));
+    int64_t have = std::min<int64_t>(st.ntrace, h.c.trace_cap);
+    int64_t k = std::min<int64_t>(have, cap);
+    static_assert(sizeof(edmd_trace_rec) == sizeof(EdmdTraceRec), "trace layout");
+    if (k > 0 && out)
+        HIPCHK(hipMemcpy(out, b->trace + h.trace0, (size_t)k * sizeof(EdmdTraceRec),
+                         hipMemcpyDeviceToHost));
+    if (n) *n = k;
+    int64_t zero = 0;
+    HIPCHK(hipMemcpy((char*)(b->st + s) + offsetof(EdmdState, ntrace), &zero, sizeof zero,
+                     hipMemcpyHostToDevice));
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_get_thermo(edmd_batch* b, int s, edmd_thermo_row* out, int cap, int* n) {
+    int rc = sys_check(b, s);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const HostSys& h = b->hs[s];
+    EdmdState st;
+    HIPCHK(hipMemcpy(&st, b->st + s, sizeof st, hipMemcpyDeviceToHost));
+    int have = std::min<int>(st.nthermo, h.c.thermo_cap);
+    int k = std::min(have, cap);
+    static_assert(sizeof(edmd_thermo_row) == sizeof(EdmdThermoRow), "thermo layout");
+    if (k > 0 && out)
+        HIPCHK(hipMemcpy(out, b->thermo + h.thermo0, (size_t)k * sizeof(EdmdThermoRow),
+                         hipMemcpyDeviceToHost));
+    if (n) *n = have;
+    return EDMD_OK;
+}
+
+/* ------------------------------------------------------------ single system */
+
+extern "C" int edmd_ctx_create(const edmd_system_desc* desc, edmd_ctx** out) {
+    if (!out) return fail(EDMD_ERR_ARG, "null output handle");
+    *out = nullptr;
+    edmd_batch* b = nullptr;
+    int rc = edmd_batch_create(desc, 1, &b);
+    if (rc) return rc;
+    edmd_ctx* c = new edmd_ctx;
+    c->b = b;
+    *out = c;
+    return EDMD_OK;
+}
+
+extern "C" int edmd_ctx_destroy(edmd_ctx* c) {
+    if (!c) return EDMD_OK;
+    batch_free(c->b);
+    delete c;
+    return EDMD_OK;
+}
+
+extern "C" int edmd_sweep_predict(edmd_ctx* c) {
+    return c ? edmd_batch_sweep_predict(c->b) : fail(EDMD_ERR_ARG, "null context");
+}
+
+extern "C" int edmd_run(edmd_ctx* c, int64_t max_events, uint64_t max_collisions, double tmax) {
+    return c ? edmd_batch_run(c->b, max_events, max_collisions, tmax)
+             : fail(EDMD_ERR_ARG, "null context");
+}
+
+extern "C" int edmd_get_state(edmd_ctx* c, double* x, double* y, double* vx, double* vy,
+                              double* t_local, uint32_t* coll, int32_t* cx, int32_t* cy) {
+    return c ? edmd_batch_get_state(c->b, 0, x, y, vx, vy, t_local, coll, cx, cy)
+             : fail(EDMD_ERR_ARG, "null context");
+}
+
+extern "C" int edmd_get_events(edmd_ctx* c, double* t_cross, int32_t* dir, double* t_coll,
+                               int32_t* partner, uint32_t* coll_actual) {
+    if (!c) return fail(EDMD_ERR_ARG, "null context");
+    int rc = edmd_batch_get_events(c->b, 0, t_coll, partner, t_cross, dir);
+    if (rc) return rc;
+    return d2h(coll_actual, c->b->cact + c->b->hs[0].p0, (size_t)c->b->hs[0].c.n);
+}
+
+extern "C" int edmd_get_observables(edmd_ctx* c, edmd_observables* o) {
+    return c ? edmd_batch_get_observables(c->b, 0, o) : fail(EDMD_ERR_ARG, "null context");
+}
+
+extern "C" int edmd_get_trace(edmd_ctx* c, edmd_trace_rec* out, int64_t cap, int64_t* n) {
+    return c ? edmd_batch_get_trace(c->b, 0, out, cap, n) : fail(EDMD_ERR_ARG, "null context");
+}
+
+extern "C" int edmd_get_thermo(edmd_ctx* c, edmd_thermo_row* out, int cap, int* n) {
+    return c ? edmd_batch_get_thermo(c->b, 0, out, cap, n) : fail(EDMD_ERR_ARG, "null context");
+}
+
+extern "C" int edmd_get_grid(edmd_ctx* c, int* ncx, int* ncy, int* cap, double* csx, double* csy) {
+    if (!c) return fail(EDMD_ERR_ARG, "null context");
+    const EdmdConst& k = c->b->hs[0].c;
+    if (ncx) *ncx = k.ncx;
+    if (ncy) *ncy = k.ncy;
+    if (cap) *cap = k.cap;
+    if (csx) *csx = k.csx;
+    if (csy) *csy = k.csy;
+    return EDMD_OK;
+}
+
+extern "C" void* edmd_stream(edmd_ctx* c) { return c ? (void*)c->b->stream : nullptr; }
+
+extern "C" int edmd_run_until_kind(edmd_ctx* c, int64_t max_events, uint64_t max_collisions,
+                                   double tmax, int stop_mask, int* stop_reason, int* last_kind) {
+    if (!c) return fail(EDMD_ERR_ARG, "null context");
+    int rc = batch_run_mask(c->b, max_events, max_collisions, tmax, stop_mask);
+    EdmdState st;
+    HIPCHK(hipMemcpy(&st, c->b->st, sizeof st, hipMemcpyDeviceToHost));
+    if (stop_reason) *stop_reason = st.stop_reason;
+    if (last_kind) *last_kind = st.last_type;
+    return rc;
+}
+
+extern "C" int edmd_peek_event(edmd_ctx* c, double* t, int* kind, int* i, int* j,
+                               uint32_t* coll_actual) {
+    if (!c) return fail(EDMD_ERR_ARG, "null context");
+    EdmdPeek* d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(EdmdPeek)));
+    EdmdPeek h;
+    hipError_t e = edmd_launch_peek(c->b->d_sys, 1, d, c->b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->b->stream);
+    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(EDMD_ERR_HIP, "peek failed: %s", hipGetErrorString(e));
+    if (t) *t = h.t;
+    if (kind) *kind = h.type;
+    if (i) *i = h.i;
+    if (j) *j = h.j;
+    if (coll_actual) *coll_actual = h.coll_actual;
+    return EDMD_OK;
+}
+
+static int particle_check(edmd_ctx* c, int i) {
+    if (!c) return fail(EDMD_ERR_ARG, "null context");
+    if (i < 0 || i >= c->b->hs[0].c.n) return fail(EDMD_ERR_ARG, "particle %d out of range", i);
+    return EDMD_OK;
+}
+
+extern "C" int edmd_collision_time(edmd_ctx* c, int i, int j, double* dt) {
+    int rc = particle_check(c, i);
+    if (!rc) rc = particle_check(c, j);
+    if (rc) return rc;
+    double* d = nullptr;
+    HIPCHK(hipMalloc(&d, 2 * sizeof(double)));
+    double h[2] = {0, 0};
+    hipError_t e = edmd_launch_pair_time(c->b->d_sys, i, j, d, c->b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->b->stream);
+    if (e == hipSuccess) e = hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(EDMD_ERR_HIP, "pair time failed: %s", hipGetErrorString(e));
+    if (dt) *dt = h[0];
+    if (h[1] != 0) return fail(EDMD_ERR_OVERLAP, "particles %d and %d overlap", i, j);
+    return EDMD_OK;
+}
+
+extern "C" int edmd_repredict(edmd_ctx* c, int i, int which) {
+    int rc = particle_check(c, i);
+    if (rc) return rc;
+    if (which < 1 || which > 3) return fail(EDMD_ERR_ARG, "which must be 1, 2 or 3");
+    HIPCHK(edmd_launch_repredict(c->b->d_sys, i, which, c->b->stream));
+    HIPCHK(hipStreamSynchronize(c->b->stream));
+    return check_status(c->b);
+}
+
+extern "C" int edmd_free_fly(edmd_ctx* c, int i) {
+    int rc = particle_check(c, i);
+    if (rc) return rc;
+    HIPCHK(edmd_launch_fly_one(c->b->d_sys, i, c->b->stream));
+    HIPCHK(hipStreamSynchronize(c->b->stream));
+    return EDMD_OK;
+}
+
+extern "C" int edmd_get_box(edmd_ctx* c, int* n, double* Lx, double* Ly) {
+    if (!c) return fail(EDMD_ERR_ARG, "null context");
+    const EdmdConst& k = c->b->hs[0].c;
+    if (n) *n = k.n;
+    if (Lx) *Lx = k.Lx;
+    if (Ly) *Ly = k.Ly;
+    return EDMD_OK;
+}
+
+/* ------------------------------------------------------------ diagnostics */
+
+extern "C" int edmd_fp64_check(int n, const double* a, const double* b, double* q, double* s) {
+    if (n <= 0 || !a || !b || !q || !s) return fail(EDMD_ERR_ARG, "bad arguments");
+    double *da = nullptr, *db = nullptr, *dq = nullptr, *ds = nullptr;
+    size_t nb = (size_t)n * sizeof(double);
+    HIPCHK(hipMalloc(&da, nb));
+    HIPCHK(hipMalloc(&db, nb));
+    HIPCHK(hipMalloc(&dq, nb));
+    HIPCHK(hipMalloc(&ds, nb));
+    HIPCHK(hipMemcpy(da, a, nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(db, b, nb, hipMemcpyHostToDevice));
+    HIPCHK(edmd_launch_fp64_check(n, da, db, dq, ds, nullptr));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(q, dq, nb, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(s, ds, nb, hipMemcpyDeviceToHost));
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dq);
+    (void)hipFree(ds);
+    return EDMD_OK;
+}
+
+/* ------------------------------------------------------------ initial states */
+
+static uint64_t splitmix64(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* uniform in (0, 1] with 53 random bits */
+static double unit_open0(uint64_t* s) { return ((splitmix64(s) >> 11) + 1) * (1.0 / 9007199254740992.0); }
+
+extern "C" int edmd_init_lattice(int n, double phi, double rad, int lattice, uint64_t seed,
+                                 double temperature, double* x, double* y, double* vx, double* vy,
+                                 double* Lx, double* Ly) {
+    if (n < 2 || !(phi > 0) || !(phi < 1) || !(rad > 0) || !x || !y || !vx || !vy || !Lx || !Ly)
+        return fail(EDMD_ERR_ARG, "bad lattice arguments");
+    const double pi = 3.14159265358979323846;
+    if (lattice == 0) {
+        double L = sqrt(pi * n * rad * rad / phi);
+        int side = (int)ceil(sqrt((double)n));
+        double a = L / side;
+        if (a < 2 * rad) return fail(EDMD_ERR_ARG, "square lattice overlaps at phi=%g", phi);
+        for (int k = 0; k < n; k++) {
+            x[k] = (k % side + 0.5) * a;
+            y[k] = (k / side + 0.5) * a;
+        }
+        *Lx = *Ly = L;
+    } else if (lattice == 1) {
+        int ny = (int)sqrt((double)n);
+        if (ny % 2 == 1) ny += 1;
+        int nx = n / ny;
+        if (nx * ny != n) return fail(EDMD_ERR_ARG, "triangular lattice needs n = nx*ny with ny even");
+        double ly = sqrt(sqrt(3.0) / 2 * ny / nx * pi * n * rad * rad / phi);
+        double lx = 2 / sqrt(3.0) * nx / ny * ly;
+        double dx = lx / nx, dy = ly / ny;
+        if (dx < 2 * rad) return fail(EDMD_ERR_ARG, "triangular lattice overlaps at phi=%g", phi);
+        int k = 0;
+        for (int i = 0; i < nx; i++)
+            for (int j = 0; j < ny; j++, k++) {
+                x[k] = i * dx + (j % 2) * (dx / 2) + dx / 4;
+                y[k] = j * dy + dy / 2;
+            }
+        *Lx = lx;
+        *Ly = ly;
+    } else {
+        return fail(EDMD_ERR_ARG, "unknown lattice %d", lattice);
+    }
+    uint64_t s = seed;
+    double mx = 0, my = 0;
+    for (int k = 0; k < n; k++) {
+        double u1 = unit_open0(&s), u2 = unit_open0(&s);
+        double r = sqrt(-2 * log(u1));
+        vx[k] = r * cos(2 * pi * u2);
+        vy[k] = r * sin(2 * pi * u2);
+        mx += vx[k];
+        my += vy[k];
+    }
+    mx /= n;
+    my /= n;
+    double e = 0;
+    for (int k = 0; k < n; k++) {
+        vx[k] -= mx;
+        vy[k] -= my;
+        e += 0.5 * (vx[k] * vx[k] + vy[k] * vy[k]);
+    }
+    double f = sqrt(temperature / (e / n));
+    for (int k = 0; k < n; k++) {
+        vx[k] *= f;
+        vy[k] *= f;
+    }
+    return EDMD_OK;
+}
+
+/* error hook for the other translation units of the library */
+int edmd_internal_fail(int code, const char* msg) { return fail(code, "%s", msg); }

@@ -1,0 +1,386 @@
+/*
+ * Reference-compatible entry points (include/edmd_compat.h) on a default
+ * single-system context.  Everything here is host bookkeeping: the particle
+ * state, the calendar and every event handler run on the GPU through the
+ * handle API of edmd_api.cpp.
+ */
+#include <getopt.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/edmd.h"
+#include "../../include/edmd_compat.h"
+
+/* ---- globals of the reference (EDMD.c:80-495) ------------------------- */
+extern "C" {
+int N = 500;
+double phi = 0.38;
+double Lx = 0, Ly = 0;
+double t = 0;
+double tmax = 4000;
+double E = 0, px = 0, py = 0, pressure = 0;
+double res = 1;
+unsigned long int ncol = 0, ncross = 0;
+int Nxcells = 0, Nycells = 0;
+node* nextEvent = nullptr;
+void (*collisionEvent)(int) = collisionEventNormal;
+void (*crossingEvent)(int) = crossingEventNormal;
+void (*doTheCollision)(void) = doTheCollisionNormal;
+}
+
+namespace {
+
+struct Config {
+    int load = 0;
+    std::string file = "data.txt";
+    double sizeratio = 0.4;      /* EDMD.c:83 */
+    double fraction_small = 0.3; /* EDMD.c:84 */
+    double aspect = 1.0;
+    int control_length = 0;
+    double dtime = 100, dtime_thermo = 100, first_screen = 1, first_thermo = 1000;
+    double einit = 1;
+    double temperature = 0.05; /* EDMD.c:379 */
+    int version = 1;
+    int lattice = 0;
+    double noise_period = 0;
+    uint64_t max_coll = ~(uint64_t)0;
+    int full_precision = 0;
+    int no_output = 0;
+};
+
+Config cfg;
+edmd_ctx* g_ctx = nullptr;
+int g_status = EDMD_OK;
+node g_node;
+std::vector<double> hx, hy, hvx, hvy, hrad, hm;
+std::string dump_path, thermo_path;
+
+void report(int rc, const char* where) {
+    if (rc == EDMD_OK) return;
+    g_status = rc;
+    if (rc == EDMD_ERR_OVERLAP)
+        fprintf(stderr, "\nERROR: Overlaps detected during SIMULATION (%s): %s\n", where,
+                edmd_last_error());
+    else
+        fprintf(stderr, "\nERROR: %s: %s\n", where, edmd_last_error());
+}
+
+void refresh() {
+    if (!g_ctx) return;
+    edmd_observables o;
+    int rc = edmd_get_observables(g_ctx, &o);
+    if (rc) { report(rc, "observables"); return; }
+    t = o.t;
+    ncol = o.ncol;
+    ncross = o.ncross;
+    E = o.E;
+    px = o.px;
+    py = o.py;
+    edmd_thermo_row rows[1];
+    int nrow = 0;
+    if (edmd_get_thermo(g_ctx, nullptr, 0, &nrow) == EDMD_OK && nrow > 0) {
+        std::vector<edmd_thermo_row> all(nrow);
+        if (edmd_get_thermo(g_ctx, all.data(), nrow, &nrow) == EDMD_OK) {
+            rows[0] = all[nrow - 1];
+            pressure = rows[0].pressure;
+        }
+    }
+}
+
+/* one event of the expected kind, handled on the GPU */
+void step_one(int kind, const char* where) {
+    if (!g_ctx) { report(EDMD_ERR_STATE, where); return; }
+    double tt;
+    int k, i, j;
+    uint32_t ca;
+    int rc = edmd_peek_event(g_ctx, &tt, &k, &i, &j, &ca);
+    if (rc) { report(rc, where); return; }
+    if (k != kind) {
+        fprintf(stderr, "%s: the next event is of kind %d, not %d\n", where, k, kind);
+        g_status = EDMD_ERR_STATE;
+        return;
+    }
+    edmd_observables o;
+    rc = edmd_get_observables(g_ctx, &o);
+    if (!rc) rc = edmd_run(g_ctx, (int64_t)o.nevents + 1, ~(uint64_t)0, 1e300);
+    report(rc, where);
+    refresh();
+}
+
+std::string versioned(const std::string& base, const char* ext, int* version) {
+    for (;;) {
+        std::string f = base + std::to_string(*version) + ext;
+        if (access(f.c_str(), F_OK) != 0) return f;
+        (*version)++;
+    }
+}
+
+}  // namespace
+
+extern "C" int edmd_compat_status(void) { return g_status; }
+extern "C" edmd_ctx* edmd_compat_context(void) { return g_ctx; }
+
+extern "C" void constantInit(int argc, char* argv[]) {
+    /* option table of EDMD.c:750-797 plus this engine's extra options */
+    static struct option longopt[] = {
+        {"load", required_argument, NULL, 'l'},
+        {"number", required_argument, NULL, 'N'},
+        {"phi", required_argument, NULL, 'p'},
+        {"res", required_argument, NULL, 'r'},
+        {"delta", required_argument, NULL, 'd'},
+        {"gamma", required_argument, NULL, 'g'},
+        {"time", required_argument, NULL, 't'},
+        {"dt", required_argument, NULL, 'D'},
+        {"initial-energy", required_argument, NULL, 'E'},
+        {"potential", required_argument, NULL, 'U'},
+        {"well-radius", required_argument, NULL, 'R'},
+        {"field", required_argument, NULL, 'f'},
+        {"ts", required_argument, NULL, 's'},
+        {"xs", required_argument, NULL, 'x'},
+        {"sizeratio", required_argument, NULL, 'q'},
+        {"temperature", required_argument, NULL, 'T'},
+        {"aspect", required_argument, NULL, 'a'},
+        {"Ly", required_argument, NULL, 'L'},
+        {"Lx", required_argument, NULL, 'X'},
+        {"deltaE", required_argument, NULL, 'e'},
+        {"additionalEnergy", required_argument, NULL, 'A'},
+        {"dtimeThermo", required_argument, NULL, 'o'},
+        {"dr_pressure", required_argument, NULL, 1001},
+        {"phil", required_argument, NULL, 'P'},
+        {"phig", required_argument, NULL, 'G'},
+        {"version", required_argument, NULL, 'v'},
+        {"strainRate", required_argument, NULL, 'S'},
+        {"cluster-mode", required_argument, NULL, 'C'},
+        {"cluster-neighbor-threshold", required_argument, NULL, 'H'},
+        {"muller-plathe", no_argument, NULL, 1002},
+        {"mp-interval", required_argument, NULL, 1003},
+        {"mp-slabs", required_argument, NULL, 1004},
+        {"mp-hot", required_argument, NULL, 1005},
+        {"mp-cold", required_argument, NULL, 1006},
+        {"temp-profile", no_argument, NULL, 1007},
+        {"temp-slabs", required_argument, NULL, 1008},
+        {"intruderMass", required_argument, NULL, 'M'},
+        {"intruderDelta", required_argument, NULL, 'z'},
+        {"intruderRes", required_argument, NULL, 'Y'},
+        {"intruderKMBurn", required_argument, NULL, 'B'},
+        {"lattice", required_argument, NULL, 3001},
+        {"noise-period", required_argument, NULL, 3002},
+        {"first-screen", required_argument, NULL, 3003},
+        {"first-thermo", required_argument, NULL, 3004},
+        {"max-collisions", required_argument, NULL, 3005},
+        {"full-precision", no_argument, NULL, 3006},
+        {"no-output", no_argument, NULL, 3007},
+        {NULL, 0, NULL, 0}};
+    optind = 1;
+    int c;
+    while ((c = getopt_long(argc, argv, "l:N:p:r:d:g:t:D:E:U:R:f:s:x:q:T:a:L:X:e:A:o:P:G:v:S:C:H:M:B:z:Y:n:",
+                            longopt, NULL)) != -1) {
+        switch (c) {
+            case 'l': {
+                cfg.file = optarg;
+                size_t len = cfg.file.size();
+                cfg.load = (len > 0 && cfg.file[len - 1] == 'p') ? 2 : 1;
+                break;
+            }
+            case 'N': cfg.load = 0; sscanf(optarg, "%d", &N); break;
+            case 'p': cfg.load = 0; sscanf(optarg, "%lf", &phi); break;
+            case 'x': cfg.load = 0; sscanf(optarg, "%lf", &cfg.fraction_small); break;
+            case 'a': cfg.load = 0; sscanf(optarg, "%lf", &cfg.aspect); break;
+            case 'q': cfg.load = 0; sscanf(optarg, "%lf", &cfg.sizeratio); break;
+            case 'r': sscanf(optarg, "%lf", &res); break;
+            case 't': sscanf(optarg, "%lf", &tmax); break;
+            case 'D': sscanf(optarg, "%lf", &cfg.dtime); break;
+            case 'E': sscanf(optarg, "%lf", &cfg.einit); break;
+            case 'T': sscanf(optarg, "%lf", &cfg.temperature); break;
+            case 'L': cfg.control_length = 1; cfg.load = 0; sscanf(optarg, "%lf", &Ly); break;
+            case 'X': cfg.control_length = 2; cfg.load = 0; sscanf(optarg, "%lf", &Lx); break;
+            case 'o': sscanf(optarg, "%lf", &cfg.dtime_thermo); break;
+            case 'v': sscanf(optarg, "%d", &cfg.version); break;
+            case 3001: cfg.lattice = (strcmp(optarg, "tri") == 0) ? 1 : 0; break;
+            case 3002:
+                sscanf(optarg, "%lf", &cfg.noise_period);
+                break;
+            case 3003: sscanf(optarg, "%lf", &cfg.first_screen); break;
+            case 3004: sscanf(optarg, "%lf", &cfg.first_thermo); break;
+            case 3005: cfg.max_coll = strtoull(optarg, NULL, 10); break;
+            case 3006: cfg.full_precision = 1; break;
+            case 3007: cfg.no_output = 1; break;
+            case '?': break;
+            default:
+                fprintf(stderr, "WARNING: option -%c (long code %d) belongs to a model variant this "
+                                "engine does not implement; ignored\n", c < 256 ? c : '-', c);
+                break;
+        }
+    }
+    g_status = EDMD_OK;
+}
+
+extern "C" void particlesInit(void) {
+    /* particlesInit, EDMD.c:1356-1451 (load path) and the lattice starts */
+    if (cfg.load == 2) {
+        double lx, ly;
+        int n;
+        int rc = edmd_dump_info(cfg.file.c_str(), &n, &lx, &ly);
+        if (rc) { report(rc, "particlesInit"); return; }
+        N = n;
+        Lx = lx;
+        Ly = ly;
+        hx.resize(n); hy.resize(n); hvx.resize(n); hvy.resize(n); hrad.resize(n); hm.resize(n);
+        rc = edmd_dump_read(cfg.file.c_str(), n, hx.data(), hy.data(), hvx.data(), hvy.data(),
+                            hrad.data(), hm.data());
+        if (rc) { report(rc, "particlesInit"); return; }
+        double area = 0;
+        for (int i = 0; i < n; i++) area += M_PI * hrad[i] * hrad[i];
+        phi = area / (Lx * Ly);
+    } else if (cfg.load == 1) {
+        fprintf(stderr, "ERROR: the text load format (load == 1) draws velocities from the "
+                        "reference's MT19937 stream; load a LAMMPS dump (*.dump) instead\n");
+        g_status = EDMD_ERR_ARG;
+        return;
+    } else {
+        if (cfg.fraction_small != 0 || cfg.aspect != 1 || cfg.control_length != 0) {
+            fprintf(stderr, "ERROR: only the monodisperse lattice start is available without -l; "
+                            "pass -x 0 (and no -a/-L/-X)\n");
+            g_status = EDMD_ERR_ARG;
+            return;
+        }
+        hx.resize(N); hy.resize(N); hvx.resize(N); hvy.resize(N);
+        hrad.assign(N, 1.0); /* the reference's large disks have radius 1 */
+        hm.assign(N, 1.0);
+        int rc = edmd_init_lattice(N, phi, 1.0, cfg.lattice, (uint64_t)cfg.version, cfg.einit,
+                                   hx.data(), hy.data(), hvx.data(), hvy.data(), &Lx, &Ly);
+        if (rc) { report(rc, "particlesInit"); return; }
+    }
+    /* output names in the spirit of customName, EDMD.c:6015-6065 */
+    char base[400];
+    snprintf(base, sizeof base, "dump/N_%dres_%.3lfphi_%.6lfLx_%.3lfLy_%.3lfv_", N, res, phi, Lx, Ly);
+    int version = cfg.version;
+    dump_path = versioned(base, ".dump", &version);
+    thermo_path = std::string(base) + std::to_string(version) + ".thermo";
+}
+
+extern "C" void cellListInit(void) {
+    /* the cell grid and lists are built on the GPU side by eventListInit */
+}
+
+extern "C" void eventListInit(void) {
+    if (g_status) return;
+    if (g_ctx) { edmd_ctx_destroy(g_ctx); g_ctx = nullptr; }
+    edmd_system_desc d;
+    edmd_default_desc(&d);
+    d.n = N;
+    d.Lx = Lx;
+    d.Ly = Ly;
+    d.x = hx.data(); d.y = hy.data(); d.vx = hvx.data(); d.vy = hvy.data();
+    d.rad = hrad.data(); d.m = hm.data();
+    d.res = res;
+    d.first_screen = cfg.first_screen;
+    d.dtime = cfg.dtime;
+    d.first_thermo = cfg.first_thermo;
+    d.dtime_thermo = cfg.dtime_thermo;
+    if (cfg.noise_period > 0) {
+        d.noise = 2;
+        d.dtnoise = cfg.noise_period;
+        d.temperature = cfg.temperature;
+    }
+    int rc = edmd_ctx_create(&d, &g_ctx);
+    if (rc) { report(rc, "eventListInit"); g_ctx = nullptr; return; }
+    int ncx, ncy;
+    edmd_get_grid(g_ctx, &ncx, &ncy, nullptr, nullptr, nullptr);
+    Nxcells = ncx;
+    Nycells = ncy;
+    refresh();
+}
+
+extern "C" void freeArrays(void) {
+    if (g_ctx) edmd_ctx_destroy(g_ctx);
+    g_ctx = nullptr;
+}
+
+extern "C" void physicalQ(void) { refresh(); }
+
+extern "C" node* findNextEvent(void) {
+    memset(&g_node, 0, sizeof g_node);
+    if (!g_ctx) { g_status = EDMD_ERR_STATE; return &g_node; }
+    double tt;
+    int k, i, j;
+    uint32_t ca;
+    int rc = edmd_peek_event(g_ctx, &tt, &k, &i, &j, &ca);
+    if (rc) { report(rc, "findNextEvent"); return &g_node; }
+    g_node.t = tt;
+    g_node.type = k;
+    g_node.i = i;
+    g_node.j = j;
+    g_node.collActual = ca;
+    nextEvent = &g_node;
+    return &g_node;
+}
+
+extern "C" void removeEventFromQueue(node* toRemove) {
+    /* the GPU calendar replaces a slot when its event is handled */
+    (void)toRemove;
+}
+
+extern "C" void doTheCollisionNormal(void) { step_one(EDMD_EV_COLLISION, "doTheCollisionNormal"); }
+extern "C" void doTheCrossing(void) { step_one(EDMD_EV_CELLCROSS, "doTheCrossing"); }
+extern "C" void takeAScreenshot(void) { step_one(EDMD_EV_SCREENSHOT, "takeAScreenshot"); }
+extern "C" void takeAThermo(void) { step_one(EDMD_EV_THERMO, "takeAThermo"); }
+extern "C" void addNoise(void) { step_one(EDMD_EV_ADDINGNOISE, "addNoise"); }
+
+extern "C" double collisionTimeNormal(int i, int j) {
+    double dt = 1e26;
+    if (!g_ctx) { g_status = EDMD_ERR_STATE; return dt; }
+    report(edmd_collision_time(g_ctx, i, j, &dt), "collisionTimeNormal");
+    return dt;
+}
+
+extern "C" void collisionEventNormal(int i) {
+    if (!g_ctx) { g_status = EDMD_ERR_STATE; return; }
+    report(edmd_repredict(g_ctx, i, 2), "collisionEventNormal");
+}
+
+extern "C" void crossingEventNormal(int i) {
+    if (!g_ctx) { g_status = EDMD_ERR_STATE; return; }
+    report(edmd_repredict(g_ctx, i, 1), "crossingEventNormal");
+}
+
+extern "C" void freeFlyNormal(int i) {
+    if (!g_ctx) { g_status = EDMD_ERR_STATE; return; }
+    report(edmd_free_fly(g_ctx, i), "freeFlyNormal");
+}
+
+extern "C" void saveTXT(void) {
+    /* saveTXT, EDMD.c:5012-5153: appends one frame of the current state */
+    if (!g_ctx || cfg.no_output) return;
+    std::vector<double> x(N), y(N), vx(N), vy(N);
+    int rc = edmd_get_state(g_ctx, x.data(), y.data(), vx.data(), vy.data(), nullptr, nullptr,
+                            nullptr, nullptr);
+    if (!rc) rc = edmd_dump_write(dump_path.c_str(), 1, t, N, Lx, Ly, x.data(), y.data(), vx.data(),
+                                  vy.data(), hrad.data(), hm.data(), cfg.full_precision);
+    report(rc, "saveTXT");
+}
+
+extern "C" int edmd_compat_run(int stop_mask) {
+    if (!g_ctx) { g_status = EDMD_ERR_STATE; return -1; }
+    int reason = 0, kind = -1;
+    int rc = edmd_run_until_kind(g_ctx, INT64_MAX, cfg.max_coll, tmax, stop_mask, &reason, &kind);
+    report(rc, "run");
+    refresh();
+    if (rc) return -1;
+    return reason == EDMD_STOP_SPECIAL ? kind : -2;
+}
+
+extern "C" int edmd_compat_outputs(const char** dp, const char** tp, int* full) {
+    if (dp) *dp = cfg.no_output ? nullptr : dump_path.c_str();
+    if (tp) *tp = cfg.no_output ? nullptr : thermo_path.c_str();
+    if (full) *full = cfg.full_precision;
+    return EDMD_OK;
+}

@@ -1,0 +1,120 @@
+/*
+ * Device-resident layout of one hard-disk system, shared by host and device
+ * code of libedmd_hip.so.
+ *
+ * A system is stored as structure-of-arrays in HBM:
+ *   particles    x y vx vy t_local rad m (f64), coll (u32), cell x/y and
+ *                periodic-image counters (i32)
+ *   event slots  per particle: collision (t, partner, partner's collision
+ *                counter at prediction) and cell crossing (t, direction),
+ *                the two slots the reference keeps per particle in
+ *                eventList[N+i] and eventList[i] (EDMD.c:1935-1948)
+ *   calendar     leaf key per particle = earliest of its two slots, plus a
+ *                64-ary tournament tree above the leaves (one level per
+ *                factor of 64); every node keeps the minimum key of its 64
+ *                children and the tagged leaf that holds it
+ *   cell lists   per cell a count and `cap` slots holding particle indices in
+ *                insertion order; the newest entry is visited first, which
+ *                is the order of the reference's head-inserted linked lists
+ *                (EDMD.c:2053-2095, 2576-2585)
+ */
+#ifndef EDMD_TYPES_H
+#define EDMD_TYPES_H
+
+#include <stdint.h>
+
+#define EDMD_MAX_LEVELS 6
+#define EDMD_NEVER 100000000000000000000000000.0 /* EDMD.h:8 */
+
+/* Event kinds as numbered by the reference's enum event (EDMD.h:21-34). */
+#define EDMD_EV_CELLCROSS 0
+#define EDMD_EV_COLLISION 1
+#define EDMD_EV_SCREENSHOT 2
+#define EDMD_EV_THERMO 3
+#define EDMD_EV_ADDINGNOISE 4
+
+/* Why a run launch returned. */
+#define EDMD_STOP_NONE 0
+#define EDMD_STOP_EVENTS 1
+#define EDMD_STOP_COLLISIONS 2
+#define EDMD_STOP_TIME 3
+#define EDMD_STOP_ERROR 4
+#define EDMD_STOP_EMPTY 5
+#define EDMD_STOP_SPECIAL 6
+
+/* Device status codes (mirrored in include/edmd.h). */
+#define EDMD_DEV_OK 0
+#define EDMD_DEV_OVERLAP 3
+#define EDMD_DEV_CELL_OVERFLOW 7
+
+typedef struct EdmdConst {
+    int32_t n, ncx, ncy, cap;
+    int32_t nlev;                          /* tree levels above the leaves */
+    int32_t lsize[EDMD_MAX_LEVELS + 1];    /* lsize[0] = n, lsize[nlev] <= 64 */
+    int32_t noise;                         /* 0 or 2 (velocity rescale) */
+    int32_t trace_cap, thermo_cap;
+    double Lx, Ly, halfLx, halfLy, csx, csy;
+    double res, dtime, dtime_thermo, dtnoise, temperature;
+} EdmdConst;
+
+typedef struct EdmdState {
+    double t;
+    double next_screen, next_thermo, next_noise, last_thermo;
+    double vir[4]; /* xx yy xy yx, reset at every thermo event */
+    uint64_t ncol, ncross, nevents;
+    int32_t status, err_i, err_j, stop_reason;
+    int64_t ntrace;
+    int32_t nthermo, last_type; /* last_type: kind of the last event handled */
+} EdmdState;
+
+typedef struct EdmdTraceRec {
+    int32_t type, i, j, valid;
+    double t;
+} EdmdTraceRec;
+
+typedef struct EdmdThermoRow {
+    double t;
+    uint64_t ncol;
+    double e_per_n, pressure, pxx, pyy, pxy, pyx;
+} EdmdThermoRow;
+
+typedef struct EdmdArrays {
+    double *x, *y, *vx, *vy, *tl, *rad, *m;
+    uint32_t* coll;
+    int32_t *cx, *cy, *crx, *cry;
+    double* tcoll;
+    int32_t* partner;
+    uint32_t* cact;
+    double* tcross;
+    int32_t* dir;
+    double* key;                         /* leaf keys, sanitised (NaN -> +inf) */
+    double* lkey[EDMD_MAX_LEVELS];       /* level l+1 node keys */
+    int32_t* lidx[EDMD_MAX_LEVELS];      /* level l+1 tagged leaves: 2*p + is_collision */
+    int32_t* ccnt;
+    int32_t* cslot;
+    EdmdState* st;
+    EdmdTraceRec* trace;
+    EdmdThermoRow* thermo;
+} EdmdArrays;
+
+typedef struct EdmdSys {
+    EdmdConst c;
+    EdmdArrays a;
+} EdmdSys;
+
+typedef struct EdmdRunArgs {
+    int64_t max_events; /* absolute bound on the system's event counter */
+    uint64_t max_coll;  /* absolute bound on the valid-collision counter */
+    double tmax;        /* the reference's loop bound: run while t <= tmax */
+    int32_t stop_mask;  /* stop after handling an event of kind k when bit k is set */
+    int32_t pad;
+} EdmdRunArgs;
+
+/* Next pending event as the reference's findNextEvent would return it. */
+typedef struct EdmdPeek {
+    double t;
+    int32_t type, i, j;
+    uint32_t coll_actual;
+} EdmdPeek;
+
+#endif

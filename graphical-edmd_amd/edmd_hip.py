@@ -1,0 +1,341 @@
+"""ctypes binding of libedmd_hip.so (include/edmd.h).
+
+Used by the tests and by bench.py.  The library is the product; this module
+only marshals numpy arrays across the C ABI.  Loading fails loudly when the
+built library is missing: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libedmd_hip.so")
+HEADER_PATHS = [os.path.join(os.path.dirname(HERE), "include", "edmd.h"),
+                os.path.join(os.path.dirname(HERE), "include", "edmd_compat.h")]
+
+EDMD_OK = 0
+EV_CELLCROSS, EV_COLLISION, EV_SCREENSHOT, EV_THERMO, EV_ADDINGNOISE = 0, 1, 2, 3, 4
+STOP_NAMES = {0: "none", 1: "events", 2: "collisions", 3: "time", 4: "error", 5: "empty",
+              6: "special"}
+NEVER = 1e26
+INT64_MAX = (1 << 63) - 1
+UINT64_MAX = (1 << 64) - 1
+
+
+class EdmdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("edmd error %d: %s" % (code, msg))
+        self.code = code
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [("n", C.c_int), ("Lx", C.c_double), ("Ly", C.c_double),
+                ("x", C.c_void_p), ("y", C.c_void_p), ("vx", C.c_void_p),
+                ("vy", C.c_void_p), ("rad", C.c_void_p), ("m", C.c_void_p),
+                ("res", C.c_double), ("first_screen", C.c_double),
+                ("dtime", C.c_double), ("first_thermo", C.c_double),
+                ("dtime_thermo", C.c_double), ("noise", C.c_int),
+                ("dtnoise", C.c_double), ("temperature", C.c_double),
+                ("cell_capacity", C.c_int), ("trace_capacity", C.c_int64),
+                ("thermo_capacity", C.c_int)]
+
+
+class Observables(C.Structure):
+    _fields_ = [("t", C.c_double), ("ncol", C.c_uint64), ("ncross", C.c_uint64),
+                ("nevents", C.c_uint64), ("vir_xx", C.c_double), ("vir_yy", C.c_double),
+                ("vir_xy", C.c_double), ("vir_yx", C.c_double), ("E", C.c_double),
+                ("px", C.c_double), ("py", C.c_double), ("status", C.c_int32),
+                ("stop_reason", C.c_int32), ("err_i", C.c_int32), ("err_j", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+TRACE_DT = np.dtype([("type", "<i4"), ("i", "<i4"), ("j", "<i4"), ("valid", "<i4"),
+                     ("t", "<f8")])
+THERMO_DT = np.dtype([("t", "<f8"), ("ncol", "<u8"), ("e_per_n", "<f8"), ("pressure", "<f8"),
+                      ("pxx", "<f8"), ("pyy", "<f8"), ("pxy", "<f8"), ("pyx", "<f8")])
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libedmd_hip.so is not built (%s); run __graft_entry__.build()"
+                               % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        vp, i64, u64, dbl, i32 = C.c_void_p, C.c_int64, C.c_uint64, C.c_double, C.c_int
+        sig = {
+            "edmd_last_error": (C.c_char_p, []),
+            "edmd_device_count": (i32, [vp]),
+            "edmd_default_desc": (None, [vp]),
+            "edmd_ctx_create": (i32, [vp, vp]),
+            "edmd_ctx_destroy": (i32, [vp]),
+            "edmd_sweep_predict": (i32, [vp]),
+            "edmd_run": (i32, [vp, i64, u64, dbl]),
+            "edmd_run_until_kind": (i32, [vp, i64, u64, dbl, i32, vp, vp]),
+            "edmd_get_state": (i32, [vp] * 9),
+            "edmd_get_events": (i32, [vp] * 6),
+            "edmd_get_observables": (i32, [vp, vp]),
+            "edmd_get_trace": (i32, [vp, vp, i64, vp]),
+            "edmd_get_thermo": (i32, [vp, vp, i32, vp]),
+            "edmd_get_grid": (i32, [vp] * 6),
+            "edmd_get_box": (i32, [vp] * 4),
+            "edmd_stream": (vp, [vp]),
+            "edmd_peek_event": (i32, [vp] * 6),
+            "edmd_collision_time": (i32, [vp, i32, i32, vp]),
+            "edmd_repredict": (i32, [vp, i32, i32]),
+            "edmd_free_fly": (i32, [vp, i32]),
+            "edmd_batch_create": (i32, [vp, i32, vp]),
+            "edmd_batch_destroy": (i32, [vp]),
+            "edmd_batch_run": (i32, [vp, i64, u64, dbl]),
+            "edmd_batch_sweep_predict": (i32, [vp]),
+            "edmd_batch_get_observables": (i32, [vp, i32, vp]),
+            "edmd_batch_get_state": (i32, [vp, i32] + [vp] * 8),
+            "edmd_batch_get_events": (i32, [vp, i32] + [vp] * 4),
+            "edmd_batch_get_trace": (i32, [vp, i32, vp, i64, vp]),
+            "edmd_batch_get_thermo": (i32, [vp, i32, vp, i32, vp]),
+            "edmd_batch_size": (i32, [vp, vp]),
+            "edmd_batch_stream": (vp, [vp]),
+            "edmd_batch_set_stream": (i32, [vp, vp]),
+            "edmd_batch_time_sweep": (i32, [vp, i32, vp]),
+            "edmd_init_lattice": (i32, [i32, dbl, dbl, i32, u64, dbl] + [vp] * 6),
+            "edmd_dump_info": (i32, [C.c_char_p, vp, vp, vp]),
+            "edmd_dump_read": (i32, [C.c_char_p, i32] + [vp] * 6),
+            "edmd_dump_write": (i32, [C.c_char_p, i32, dbl, i32, dbl, dbl] + [vp] * 6 + [i32]),
+            "edmd_fp64_check": (i32, [i32, vp, vp, vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != EDMD_OK:
+        raise EdmdError(rc, lib().edmd_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def make_desc(st, keep, **kw):
+    """Fills a SystemDesc from a state dict (x y vx vy rad m Lx Ly); the numpy
+    arrays are appended to `keep` so they outlive the call."""
+    d = SystemDesc()
+    lib().edmd_default_desc(C.byref(d))
+    arrs = {k: _f64(st[k]) for k in ("x", "y", "vx", "vy", "rad", "m")}
+    keep.append(arrs)
+    d.n = len(arrs["x"])
+    d.Lx = float(st["Lx"])
+    d.Ly = float(st["Ly"])
+    for k, a in arrs.items():
+        setattr(d, k, a.ctypes.data)
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d
+
+
+class System:
+    """One system on the GPU (the plain single-system path)."""
+
+    def __init__(self, st, **kw):
+        keep = []
+        d = make_desc(st, keep, **kw)
+        h = C.c_void_p()
+        check(lib().edmd_ctx_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.n = d.n
+        self.trace_capacity = d.trace_capacity
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().edmd_ctx_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def run(self, max_events=INT64_MAX, max_collisions=UINT64_MAX, tmax=1e300):
+        check(lib().edmd_run(self.h, int(max_events), int(max_collisions), float(tmax)))
+
+    def run_until_kind(self, stop_mask, max_events=INT64_MAX, max_collisions=UINT64_MAX,
+                       tmax=1e300):
+        reason, kind = C.c_int(), C.c_int()
+        check(lib().edmd_run_until_kind(self.h, int(max_events), int(max_collisions),
+                                        float(tmax), int(stop_mask), C.byref(reason),
+                                        C.byref(kind)))
+        return reason.value, kind.value
+
+    def sweep_predict(self):
+        check(lib().edmd_sweep_predict(self.h))
+
+    def state(self):
+        n = self.n
+        out = {k: np.zeros(n) for k in ("x", "y", "vx", "vy", "t")}
+        out["coll"] = np.zeros(n, dtype=np.uint32)
+        out["cx"] = np.zeros(n, dtype=np.int32)
+        out["cy"] = np.zeros(n, dtype=np.int32)
+        check(lib().edmd_get_state(self.h, *[_ptr(out[k]) for k in
+                                             ("x", "y", "vx", "vy", "t", "coll", "cx", "cy")]))
+        return out
+
+    def events(self):
+        n = self.n
+        out = {"t_cross": np.zeros(n), "dir": np.zeros(n, dtype=np.int32),
+               "t_coll": np.zeros(n), "partner": np.zeros(n, dtype=np.int32),
+               "coll_actual": np.zeros(n, dtype=np.uint32)}
+        check(lib().edmd_get_events(self.h, *[_ptr(out[k]) for k in
+                                              ("t_cross", "dir", "t_coll", "partner",
+                                               "coll_actual")]))
+        return out
+
+    def observables(self):
+        o = Observables()
+        check(lib().edmd_get_observables(self.h, C.byref(o)))
+        return o.as_dict()
+
+    def trace(self, cap=None):
+        cap = self.trace_capacity if cap is None else cap
+        out = np.zeros(max(int(cap), 1), dtype=TRACE_DT)
+        n = C.c_int64()
+        check(lib().edmd_get_trace(self.h, out.ctypes.data, int(cap), C.byref(n)))
+        return out[:n.value]
+
+    def thermo(self):
+        n = C.c_int()
+        check(lib().edmd_get_thermo(self.h, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=THERMO_DT)
+        check(lib().edmd_get_thermo(self.h, out.ctypes.data, n.value, C.byref(n)))
+        return out[:n.value]
+
+    def grid(self):
+        ncx, ncy, cap = C.c_int(), C.c_int(), C.c_int()
+        csx, csy = C.c_double(), C.c_double()
+        check(lib().edmd_get_grid(self.h, C.byref(ncx), C.byref(ncy), C.byref(cap),
+                                  C.byref(csx), C.byref(csy)))
+        return ncx.value, ncy.value, cap.value, csx.value, csy.value
+
+    def peek(self):
+        t, k, i, j, ca = C.c_double(), C.c_int(), C.c_int(), C.c_int(), C.c_uint32()
+        check(lib().edmd_peek_event(self.h, C.byref(t), C.byref(k), C.byref(i), C.byref(j),
+                                    C.byref(ca)))
+        return t.value, k.value, i.value, j.value, ca.value
+
+    def collision_time(self, i, j):
+        dt = C.c_double()
+        check(lib().edmd_collision_time(self.h, int(i), int(j), C.byref(dt)))
+        return dt.value
+
+    def repredict(self, i, which=3):
+        check(lib().edmd_repredict(self.h, int(i), int(which)))
+
+    def free_fly(self, i):
+        check(lib().edmd_free_fly(self.h, int(i)))
+
+
+class Batch:
+    """Many independent systems, one wavefront each."""
+
+    def __init__(self, states, **kw):
+        keep = []
+        descs = (SystemDesc * len(states))()
+        for k, st in enumerate(states):
+            descs[k] = make_desc(st, keep, **kw)
+        h = C.c_void_p()
+        check(lib().edmd_batch_create(descs, len(states), C.byref(h)))
+        self.h = h
+        self.ns = [len(st["x"]) for st in states]
+        self.trace_capacity = descs[0].trace_capacity
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().edmd_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def __len__(self):
+        return len(self.ns)
+
+    def run(self, max_events=INT64_MAX, max_collisions=UINT64_MAX, tmax=1e300):
+        check(lib().edmd_batch_run(self.h, int(max_events), int(max_collisions), float(tmax)))
+
+    def sweep_predict(self):
+        check(lib().edmd_batch_sweep_predict(self.h))
+
+    def time_sweep(self, reps):
+        ms = C.c_double()
+        check(lib().edmd_batch_time_sweep(self.h, int(reps), C.byref(ms)))
+        return ms.value
+
+    def observables(self, s):
+        o = Observables()
+        check(lib().edmd_batch_get_observables(self.h, int(s), C.byref(o)))
+        return o.as_dict()
+
+    def state(self, s):
+        n = self.ns[s]
+        out = {k: np.zeros(n) for k in ("x", "y", "vx", "vy", "t")}
+        out["coll"] = np.zeros(n, dtype=np.uint32)
+        out["cx"] = np.zeros(n, dtype=np.int32)
+        out["cy"] = np.zeros(n, dtype=np.int32)
+        check(lib().edmd_batch_get_state(self.h, int(s), *[_ptr(out[k]) for k in
+                                                           ("x", "y", "vx", "vy", "t", "coll",
+                                                            "cx", "cy")]))
+        return out
+
+    def events(self, s):
+        n = self.ns[s]
+        out = {"t_coll": np.zeros(n), "partner": np.zeros(n, dtype=np.int32),
+               "t_cross": np.zeros(n), "dir": np.zeros(n, dtype=np.int32)}
+        check(lib().edmd_batch_get_events(self.h, int(s), *[_ptr(out[k]) for k in
+                                                            ("t_coll", "partner", "t_cross",
+                                                             "dir")]))
+        return out
+
+    def trace(self, s, cap=None):
+        cap = self.trace_capacity if cap is None else cap
+        out = np.zeros(max(int(cap), 1), dtype=TRACE_DT)
+        n = C.c_int64()
+        check(lib().edmd_batch_get_trace(self.h, int(s), out.ctypes.data, int(cap),
+                                         C.byref(n)))
+        return out[:n.value]
+
+    def stream(self):
+        return lib().edmd_batch_stream(self.h)
+
+
+def init_lattice(n, phi, rad=0.5, lattice=0, seed=1, temperature=1.0):
+    x, y, vx, vy = (np.zeros(n) for _ in range(4))
+    Lx, Ly = C.c_double(), C.c_double()
+    check(lib().edmd_init_lattice(int(n), float(phi), float(rad), int(lattice), int(seed),
+                                  float(temperature), x.ctypes.data, y.ctypes.data,
+                                  vx.ctypes.data, vy.ctypes.data, C.byref(Lx), C.byref(Ly)))
+    return {"x": x, "y": y, "vx": vx, "vy": vy, "rad": np.full(n, rad), "m": np.ones(n),
+            "Lx": Lx.value, "Ly": Ly.value}
+
+
+def fp64_check(a, b):
+    a, b = _f64(a), _f64(b)
+    q, s = np.zeros_like(a), np.zeros_like(a)
+    check(lib().edmd_fp64_check(len(a), a.ctypes.data, b.ctypes.data, q.ctypes.data,
+                                s.ctypes.data))
+    return q, s
+
+
+def device_count():
+    n = C.c_int()
+    lib().edmd_device_count(C.byref(n))
+    return n.value

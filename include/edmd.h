@@ -1,0 +1,201 @@
+/*
+ * libedmd_hip.so — C ABI of the MI355X event-driven hard-disk engine.
+ *
+ * Plain pointers and sizes only.  Every function returns an int status
+ * (EDMD_OK = 0) and never exits the process; edmd_last_error() describes the
+ * last failure of the calling thread.  Host arrays passed in are copied;
+ * particle and event state lives on the GPU.  One host thread per handle;
+ * handles are independent; each handle owns one HIP stream.
+ *
+ * The reference keeps its engine in global state behind void functions
+ * (EDMD.h:78-193, EDMD.c:80-495).  This header replaces that path twice:
+ *   - a handle API (edmd_ctx for one system, edmd_batch for many
+ *     independent replicas), and
+ *   - same-named compatibility entry points on a default context
+ *     (include/edmd_compat.h).
+ */
+#ifndef EDMD_H
+#define EDMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define EDMD_OK 0
+#define EDMD_ERR_OVERLAP 3        /* reference: saveTXT + exit(3), EDMD.c:2708-2717 */
+#define EDMD_ERR_IO 5             /* reference parser: exit(1), parser.c:32-35 */
+#define EDMD_ERR_CELL_OVERFLOW 7  /* a cell list exceeded its capacity */
+#define EDMD_ERR_NOMEM 12
+#define EDMD_ERR_NODEVICE 19
+#define EDMD_ERR_ARG 22
+#define EDMD_ERR_HIP 30
+#define EDMD_ERR_STATE 31
+
+/* ---- event kinds: the reference's enum event, EDMD.h:21-34 -------------- */
+#define EDMD_EV_CELLCROSS 0
+#define EDMD_EV_COLLISION 1
+#define EDMD_EV_SCREENSHOT 2
+#define EDMD_EV_THERMO 3
+#define EDMD_EV_ADDINGNOISE 4
+
+/* ---- why a run returned ------------------------------------------------- */
+#define EDMD_STOP_NONE 0
+#define EDMD_STOP_EVENTS 1
+#define EDMD_STOP_COLLISIONS 2
+#define EDMD_STOP_TIME 3
+#define EDMD_STOP_ERROR 4
+#define EDMD_STOP_EMPTY 5
+#define EDMD_STOP_SPECIAL 6 /* stopped after an event kind listed in a stop mask */
+
+/* One system: N disks in a periodic Lx x Ly box.  Replaces the state that
+ * constantInit/particlesInit build (EDMD.c:745-1219, 1356-1451).  rad is a
+ * radius (contact distance 2 sqrt(r_i r_j), EDMD.c:35-43). */
+typedef struct edmd_system_desc {
+    int n;
+    double Lx, Ly;
+    const double *x, *y, *vx, *vy, *rad, *m; /* host arrays of n values */
+    double res;             /* restitution coefficient, EDMD.c:372 (-r) */
+    double first_screen;    /* first screenshot time, EDMD.c:244 */
+    double dtime;           /* screenshot period, EDMD.c:245 (-D) */
+    double first_thermo;    /* first thermo time (an extra 1e-9 is added), EDMD.c:248, 1970 */
+    double dtime_thermo;    /* thermo period, EDMD.c:247 (-o) */
+    int noise;              /* 0 none, 2 velocity-rescale thermostat, EDMD.c:4880-4915 */
+    double dtnoise;         /* thermostat period, EDMD.c:386 */
+    double temperature;     /* thermostat target E/N, EDMD.c:379 */
+    int cell_capacity;      /* max disks per cell, 0 = derived from the radii */
+    int64_t trace_capacity; /* device event-trace records kept, 0 = none */
+    int thermo_capacity;    /* thermo rows kept, 0 = 4096 */
+} edmd_system_desc;
+
+typedef struct edmd_trace_rec {
+    int32_t type, i, j, valid; /* crossing: j = direction 1..4; collision: j = partner */
+    double t;
+} edmd_trace_rec;
+
+typedef struct edmd_thermo_row {
+    double t;
+    uint64_t ncol;
+    double e_per_n, pressure, pxx, pyy, pxy, pyx; /* saveThermo, EDMD.c:5232-5333 */
+} edmd_thermo_row;
+
+typedef struct edmd_observables {
+    double t;
+    uint64_t ncol, ncross, nevents;
+    double vir_xx, vir_yy, vir_xy, vir_yx; /* collTermX/Y/XY/YX since the last thermo */
+    double E, px, py;                      /* physicalQ, EDMD.c:5968-6013 */
+    int32_t status, stop_reason, err_i, err_j;
+} edmd_observables;
+
+void edmd_default_desc(edmd_system_desc* d);
+const char* edmd_last_error(void);
+int edmd_device_count(int* n);
+
+/* ---- single system (plain path) ----------------------------------------- */
+typedef struct edmd_ctx edmd_ctx;
+
+/* Builds the cell grid and cell lists (boxConstantHelper + cellListInit,
+ * EDMD.c:679-715, 1906-1920), uploads the state and runs the initial
+ * prediction sweep and calendar build (eventListInit, EDMD.c:1935-2012). */
+int edmd_ctx_create(const edmd_system_desc* desc, edmd_ctx** out);
+int edmd_ctx_destroy(edmd_ctx* ctx);
+/* Full prediction sweep at the current time (kernel k_sweep): free-flies
+ * every particle to the current time first, then re-predicts every
+ * crossing and collision slot and rebuilds the calendar (the pattern of
+ * EDMD.c:2007-2012 and 4909-4915). */
+int edmd_sweep_predict(edmd_ctx* ctx);
+/* Event loop (main, EDMD.c:523-610): handles events while t <= tmax, until
+ * the system's event counter reaches max_events or its valid-collision
+ * counter reaches max_collisions (both absolute counts). */
+int edmd_run(edmd_ctx* ctx, int64_t max_events, uint64_t max_collisions, double tmax);
+int edmd_get_state(edmd_ctx* ctx, double* x, double* y, double* vx, double* vy,
+                   double* t_local, uint32_t* coll, int32_t* cx, int32_t* cy);
+int edmd_get_events(edmd_ctx* ctx, double* t_cross, int32_t* dir, double* t_coll,
+                    int32_t* partner, uint32_t* coll_actual);
+int edmd_get_observables(edmd_ctx* ctx, edmd_observables* out);
+/* Copies up to cap trace records kept since the last call and clears them. */
+int edmd_get_trace(edmd_ctx* ctx, edmd_trace_rec* out, int64_t cap, int64_t* n);
+int edmd_get_thermo(edmd_ctx* ctx, edmd_thermo_row* out, int cap, int* n);
+int edmd_get_grid(edmd_ctx* ctx, int* ncx, int* ncy, int* cap, double* csx, double* csy);
+int edmd_get_box(edmd_ctx* ctx, int* n, double* Lx, double* Ly);
+void* edmd_stream(edmd_ctx* ctx);
+
+/* Step-level entry points of the single-system path, for drivers that
+ * interleave their own work between events (the reference's GUI loop,
+ * graphics.c:980-1016) and for tests. */
+/* As edmd_run, but also stops right after handling an event whose kind k
+ * has bit (1 << k) set in stop_mask; reports why it stopped and the kind of
+ * the last event handled. */
+int edmd_run_until_kind(edmd_ctx* ctx, int64_t max_events, uint64_t max_collisions, double tmax,
+                        int stop_mask, int* stop_reason, int* last_kind);
+/* findNextEvent (EDMD.c:2313-2327): the pending event that would be handled
+ * next, without handling it.  For a crossing j is the direction code. */
+int edmd_peek_event(edmd_ctx* ctx, double* t, int* kind, int* i, int* j, uint32_t* coll_actual);
+/* collisionTimeNormal(p_i, p_j) (EDMD.c:2661-2723) at the current time:
+ * relative time, EDMD_NEVER-valued 1e26 when the pair does not meet;
+ * EDMD_ERR_OVERLAP when the pair overlaps by more than the tolerance. */
+int edmd_collision_time(edmd_ctx* ctx, int i, int j, double* dt);
+/* crossingEvent(i) (which & 1) and/or collisionEvent(i) (which & 2) at the
+ * current time, EDMD.c:2405-2496, 2829-3093; i must be synchronised. */
+int edmd_repredict(edmd_ctx* ctx, int i, int which);
+/* freeFlyNormal(p_i) to the current time, EDMD.c:4954-4990. */
+int edmd_free_fly(edmd_ctx* ctx, int i);
+
+/* ---- batched replicas ---------------------------------------------------- */
+typedef struct edmd_batch edmd_batch;
+
+/* nsys independent systems of any sizes, one wavefront each in one
+ * persistent launch. */
+int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_batch** out);
+int edmd_batch_destroy(edmd_batch* b);
+/* Every system runs until its own event counter reaches max_events or its
+ * collision counter reaches max_collisions, or t > tmax. */
+int edmd_batch_run(edmd_batch* b, int64_t max_events, uint64_t max_collisions, double tmax);
+int edmd_batch_sweep_predict(edmd_batch* b);
+int edmd_batch_get_observables(edmd_batch* b, int sys, edmd_observables* out);
+int edmd_batch_get_state(edmd_batch* b, int sys, double* x, double* y, double* vx, double* vy,
+                         double* t_local, uint32_t* coll, int32_t* cx, int32_t* cy);
+int edmd_batch_get_events(edmd_batch* b, int sys, double* t_coll, int32_t* partner,
+                          double* t_cross, int32_t* dir);
+int edmd_batch_get_trace(edmd_batch* b, int sys, edmd_trace_rec* out, int64_t cap, int64_t* n);
+int edmd_batch_get_thermo(edmd_batch* b, int sys, edmd_thermo_row* out, int cap, int* n);
+int edmd_batch_size(edmd_batch* b, int* nsys);
+void* edmd_batch_stream(edmd_batch* b);
+/* Uses the caller's HIP stream (hipStream_t) for all later work. */
+int edmd_batch_set_stream(edmd_batch* b, void* stream);
+
+/* ---- initial states and files -------------------------------------------- */
+/* Seeded lattice start: lattice 0 = square (first n sites of a
+ * ceil(sqrt n)^2 grid), 1 = triangular (the reference's Hex layout,
+ * EDMD.c:1024-1049, 1582-1637; n must factor as nx*ny with ny even);
+ * Gaussian velocities with zero total momentum scaled to E/N = temperature
+ * (normalizePhysicalQ, EDMD.c:5723-5764).  Writes n values to each array and
+ * the box lengths. */
+int edmd_init_lattice(int n, double phi, double rad, int lattice, uint64_t seed,
+                      double temperature, double* x, double* y, double* vx, double* vy,
+                      double* Lx, double* Ly);
+/* LAMMPS dump reader (the load path, EDMD.c:987-995, 1387-1451; parser.c):
+ * last frame, box from row 0 column 1 of BOX, columns by name prefix. */
+int edmd_dump_info(const char* path, int* n, double* Lx, double* Ly);
+int edmd_dump_read(const char* path, int n, double* x, double* y, double* vx, double* vy,
+                   double* rad, double* m);
+/* Dump writer (saveTXT layout, EDMD.c:5012-5153); full_precision != 0
+ * prints every value with 17 significant digits. */
+int edmd_dump_write(const char* path, int append, double t, int n, double Lx, double Ly,
+                    const double* x, const double* y, const double* vx, const double* vy,
+                    const double* rad, const double* m, int full_precision);
+
+/* ---- diagnostics ----------------------------------------------------------- */
+/* q[k] = a[k] / b[k] and s[k] = sqrt(a[k]) evaluated on the GPU, for the
+ * IEEE-rounding tests of the FP64 operations the engine relies on. */
+int edmd_fp64_check(int n, const double* a, const double* b, double* q, double* s);
+/* Average duration (ms) of `reps` prediction sweeps of every system of the
+ * batch, timed with HIP events on the batch's stream. */
+int edmd_batch_time_sweep(edmd_batch* b, int reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

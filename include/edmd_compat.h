@@ -1,0 +1,93 @@
+/*
+ * Reference-compatible entry points of libedmd_hip.so.
+ *
+ * Same names, argument meaning and call order as the reference's engine API
+ * (EDMD.h:78-193) and the globals its callers read (EDMD.c:80-495), acting
+ * on one default single-system context whose state lives on the GPU.
+ * Differences, all forced by the state being device-resident:
+ *   - particle* arguments are particle indices;
+ *   - findNextEvent returns a snapshot node whose tree links are NULL;
+ *   - instead of exit(3) on an overlap (EDMD.c:2715-2716) the call prints
+ *     the same diagnosis to stderr and records a status readable with
+ *     edmd_compat_status(); the `edmd` command exits with that code.
+ *
+ * Call order, as main() (EDMD.c:508-522):
+ *   constantInit(argc, argv); particlesInit(); cellListInit();
+ *   eventListInit(); physicalQ();
+ *   while (t <= tmax) { nextEvent = findNextEvent(); t = nextEvent->t;
+ *     removeEventFromQueue(nextEvent); switch (nextEvent->type) {...} }
+ */
+#ifndef EDMD_COMPAT_H
+#define EDMD_COMPAT_H
+
+#include "edmd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The reference's event node (EDMD.h:66-74); `type` holds an EDMD_EV_* kind. */
+typedef struct node node;
+struct node {
+    node *lft, *rgt, *top;
+    int i, j, q;
+    double t;
+    int type;
+    unsigned long int collActual;
+};
+
+/* Globals of the reference read by its callers (EDMD.c:80-495); refreshed
+ * after every call below. */
+extern int N;
+extern double phi, Lx, Ly, t, tmax, E, px, py, pressure, res;
+extern unsigned long int ncol, ncross;
+extern int Nxcells, Nycells;
+extern node* nextEvent;
+/* The reference's dispatch pointers (EDMD.c:317-321) for the Normal model. */
+extern void (*collisionEvent)(int);
+extern void (*crossingEvent)(int);
+extern void (*doTheCollision)(void);
+
+/* CLI options of constantInit (EDMD.c:750-977), same letters and long names.
+ * Options of model variants this engine does not implement are accepted and
+ * reported on stderr.  Extra long options (no reference counterpart):
+ *   --lattice square|tri   lattice start instead of the growth start
+ *   --noise-period P       velocity-rescale thermostat every P (noise = 2)
+ *   --first-screen T, --first-thermo T
+ *   --max-collisions C     also stop after C valid collisions
+ *   --full-precision       dump values with 17 significant digits
+ *   --no-output            write no dump or thermo files */
+void constantInit(int argc, char* argv[]);
+void particlesInit(void);
+void cellListInit(void);
+void eventListInit(void);
+void freeArrays(void);
+void physicalQ(void);
+node* findNextEvent(void);
+void removeEventFromQueue(node* toRemove);
+void doTheCollisionNormal(void);
+void doTheCrossing(void);
+void takeAScreenshot(void);
+void takeAThermo(void);
+void addNoise(void);
+double collisionTimeNormal(int i, int j);
+void collisionEventNormal(int i);
+void crossingEventNormal(int i);
+void freeFlyNormal(int i);
+void saveTXT(void);
+
+/* Status of the most recent compat call (EDMD_OK when none failed). */
+int edmd_compat_status(void);
+/* The default context (NULL before eventListInit). */
+edmd_ctx* edmd_compat_context(void);
+/* Fast path of main's loop: handles events on the GPU until t > tmax, the
+ * collision cap, or (when stop_mask != 0) right after an event of a kind in
+ * stop_mask; returns the kind of the last event handled or -1 on error. */
+int edmd_compat_run(int stop_mask);
+/* Output settings parsed by constantInit, for the command-line host. */
+int edmd_compat_outputs(const char** dump_path, const char** thermo_path, int* full_precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
