@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark of the event-driven hard-disk engine on one MI355X.
+
+Headline (`value`): aggregate valid collisions per second of batched
+replicas -- 4096 independent systems, N uniform in [256, 2048], phi uniform
+in [0.50, 0.75], diameter 1, E/N = 1, per-replica seed, FP64 -- all in one
+persistent launch.  Starts are triangular lattices fitted to the square box
+(a square lattice of N sites overlaps above phi ~ 0.70 for most N).  One step = every
+replica advances by --coll-per-step valid collisions (default 10^4), so the
+default 10 timed steps are the workload's 10^5 collisions per replica.
+
+Reported beside it, in the same run:
+  plain        single large system: N = 2^20 disks at phi = 0.80 on a
+               triangular lattice, the same event loop on one system
+               (collisions/s), and the batched/plain ratio;
+  sweep        the full prediction sweep of that system (kernel k_sweep):
+               GB/s of algorithmic traffic, 80 B per disk, against the
+               MI355X HBM peak (the `roofline` object);
+  cpu_baseline the reference's own CPU event loop (built with its shipped
+               flags) on a bounded sample of replicas, one core.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(HERE, "graphical-edmd_amd")
+HBM_PEAK_GBS = 8000.0
+SWEEP_BYTES_PER_DISK = 80  # x y vx vy t (40) + cell + coll + cell table (12) + slots (28)
+
+
+def load_binding():
+    spec = importlib.util.spec_from_file_location("edmd_hip", os.path.join(PKG, "edmd_hip.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["edmd_hip"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def replica_params(nrep, seed):
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(256, 2049, size=nrep)
+    phis = rng.uniform(0.50, 0.75, size=nrep)
+    return ns, phis
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(E, ns, phis, base_seed, coll, seconds):
+    """The reference's event loop (oracle/_ref/ref_fast: the reference's own
+    sources built with its Makefile flags) on the first replicas of the same
+    workload, one process on one core, until ~`seconds` of work."""
+    ref = os.path.join(HERE, "oracle", "_ref", "ref_fast")
+    kind = "reference"
+    if not os.path.exists(ref):
+        kind = "port"
+    tmp = tempfile.mkdtemp(prefix="edmd_cpu_")
+    total_coll = 0
+    total_s = 0.0
+    k = 0
+    while total_s < seconds and k < len(ns):
+        st = E.init_lattice(int(ns[k]), float(phis[k]), rad=0.5, lattice=2,
+                            seed=base_seed + k)
+        if kind == "reference":
+            path = os.path.join(tmp, "r%d.dump" % k)
+            args = [st[f].ctypes.data for f in ("x", "y", "vx", "vy", "rad", "m")]
+            E.check(E.lib().edmd_dump_write(path.encode(), 0, 0.0, len(st["x"]), st["Lx"],
+                                            st["Ly"], *args, 1))
+            r = subprocess.run([ref, "bench", path, str(coll)], capture_output=True, text=True,
+                               timeout=600, check=True)
+            res = json.loads(r.stderr.strip().splitlines()[-1])
+            total_coll += res["ncol"]
+            total_s += res["seconds"]
+        else:
+            sys.path.insert(0, os.path.join(HERE, "tests"))
+            import oracle_bind
+            o = oracle_bind.Oracle(st)
+            t0 = time.perf_counter()
+            o.run_collisions(coll)
+            total_s += time.perf_counter() - t0
+            total_coll += o.observables()["ncol"]
+        k += 1
+    return {"value": total_coll / total_s, "unit": "collisions/s", "cores": 1, "kind": kind,
+            "sample": "%d replicas of the batched workload (N~U[256,2048], phi~U[0.5,0.75]), "
+                      "%d collisions each, one process" % (k, coll)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--replicas", type=int, default=4096)
+    ap.add_argument("--coll-per-step", type=int, default=10000)
+    ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--plain-n", type=int, default=1 << 20)
+    ap.add_argument("--plain-collisions", type=int, default=300000)
+    ap.add_argument("--sweep-reps", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-collisions", type=int, default=100000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; replicas would shard across GPUs with no "
+              "exchange, but --gpus %d is not supported here" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    E = load_binding()
+    if E.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    # ---- batched replicas -------------------------------------------------
+    ns, phis = replica_params(args.replicas, args.seed)
+    t0 = time.perf_counter()
+    states = [E.init_lattice(int(n), float(p), rad=0.5, lattice=2, seed=args.seed + k)
+              for k, (n, p) in enumerate(zip(ns, phis))]
+    b = E.Batch(states)
+    log("batch of %d replicas (%d disks) built in %.1f s" % (len(states), int(ns.sum()),
+                                                            time.perf_counter() - t0))
+    cps = args.coll_per_step
+    for w in range(args.warmup):
+        b.run(max_collisions=cps * (w + 1), max_events=1 << 62)
+    ncol0 = sum(b.observables(k)["ncol"] for k in range(len(states)))
+    nev0 = sum(b.observables(k)["nevents"] for k in range(len(states)))
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        b.run(max_collisions=cps * (args.warmup + s + 1), max_events=1 << 62)
+    t1 = time.perf_counter()
+    obs = [b.observables(k) for k in range(len(states))]
+    ncol = sum(o["ncol"] for o in obs) - ncol0
+    nev = sum(o["nevents"] for o in obs) - nev0
+    elapsed = t1 - t0
+    value = ncol / elapsed
+    log("batched: %d collisions (%d events) in %.3f s = %.4e coll/s" % (ncol, nev, elapsed, value))
+    b.close()
+    del states
+
+    # ---- plain single large system: sweep roofline and event loop ---------
+    st2 = E.init_lattice(args.plain_n, 0.80, rad=0.5, lattice=1, seed=args.seed)
+    big = E.Batch([st2])
+    sweep_ms = big.time_sweep(args.sweep_reps)
+    sweep_bytes = SWEEP_BYTES_PER_DISK * args.plain_n
+    sweep_gbs = sweep_bytes / (sweep_ms * 1e-3) / 1e9
+    log("sweep: N=%d  %.4f ms  %.1f GB/s" % (args.plain_n, sweep_ms, sweep_gbs))
+    big.run(max_events=20000)  # first events off the lattice
+    o0 = big.observables(0)
+    t0 = time.perf_counter()
+    big.run(max_collisions=o0["ncol"] + args.plain_collisions, max_events=1 << 62)
+    t1 = time.perf_counter()
+    o1 = big.observables(0)
+    plain = (o1["ncol"] - o0["ncol"]) / (t1 - t0)
+    plain_ev = (o1["nevents"] - o0["nevents"]) / (t1 - t0)
+    log("plain: %d collisions in %.3f s = %.4e coll/s" % (o1["ncol"] - o0["ncol"], t1 - t0, plain))
+    big.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(E, ns, phis, args.seed, args.cpu_collisions, args.cpu_seconds)
+        log("cpu baseline: %.4e coll/s (%s)" % (cpu["value"], cpu["kind"]))
+
+    out = {
+        "metric": "collisions/sec, batched replicas vs plain single system; prediction GB/s",
+        "value": value,
+        "unit": "collisions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched replicas: %d systems, N~U[256,2048], phi~U[0.50,0.75], "
+                        "diameter 1, E/N=1, %d collisions per replica per step"
+                        % (args.replicas, cps),
+            "replicas": args.replicas,
+            "disks": int(ns.sum()),
+            "plain_system": "N=%d, phi=0.80, triangular lattice start" % args.plain_n,
+        },
+        "events_per_s": nev / elapsed,
+        "plain_collisions_per_s": plain,
+        "plain_events_per_s": plain_ev,
+        "batched_over_plain": value / plain,
+        "sweep": {"n": args.plain_n, "ms": sweep_ms, "GB_per_s": sweep_gbs},
+        "roofline": {"kernel": "k_sweep", "bound": "hbm", "achieved": sweep_gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS,
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

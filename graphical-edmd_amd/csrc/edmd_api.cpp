@@ -784,6 +784,30 @@ extern "C" int edmd_init_lattice(int n, double phi, double rad, int lattice, uin
             }
         *Lx = lx;
         *Ly = ly;
+    } else if (lattice == 2) {
+        /* triangular stacking fitted to a square box for any n: ny (even)
+         * rows of nx = ceil(n/ny) sites, the row count chosen to maximise
+         * the nearest-neighbour distance; sites filled row by row */
+        double L = sqrt(pi * n * rad * rad / phi);
+        int best_ny = 0;
+        double best_d = 0;
+        int lim = 2 * (int)ceil(sqrt((double)n)) + 4;
+        for (int ny = 2; ny <= lim; ny += 2) {
+            int nx = (n + ny - 1) / ny;
+            double dx = L / nx, dy = L / ny;
+            double d = std::min(dx, sqrt(dx * dx / 4 + dy * dy));
+            d = std::min(d, 2 * dy);
+            if (d > best_d) { best_d = d; best_ny = ny; }
+        }
+        if (best_d < 2 * rad) return fail(EDMD_ERR_ARG, "fitted lattice overlaps at phi=%g", phi);
+        int ny = best_ny, nx = (n + ny - 1) / ny;
+        double dx = L / nx, dy = L / ny;
+        for (int k = 0; k < n; k++) {
+            int i = k % nx, j = k / nx;
+            x[k] = (i + 0.25 + 0.5 * (j % 2)) * dx;
+            y[k] = (j + 0.5) * dy;
+        }
+        *Lx = *Ly = L;
     } else {
         return fail(EDMD_ERR_ARG, "unknown lattice %d", lattice);
     }

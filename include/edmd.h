@@ -169,7 +169,10 @@ int edmd_batch_set_stream(edmd_batch* b, void* stream);
 /* ---- initial states and files -------------------------------------------- */
 /* Seeded lattice start: lattice 0 = square (first n sites of a
  * ceil(sqrt n)^2 grid), 1 = triangular (the reference's Hex layout,
- * EDMD.c:1024-1049, 1582-1637; n must factor as nx*ny with ny even);
+ * EDMD.c:1024-1049, 1582-1637; n must factor as nx*ny with ny even),
+ * 2 = triangular stacking fitted to a square box for any n (ny even rows of
+ * ceil(n/ny) sites, ny chosen for the largest nearest-neighbour distance);
+ * a lattice whose spacing is below the diameter is refused;
  * Gaussian velocities with zero total momentum scaled to E/N = temperature
  * (normalizePhysicalQ, EDMD.c:5723-5764).  Writes n values to each array and
  * the box lengths. */

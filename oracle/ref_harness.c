@@ -15,6 +15,8 @@
  *   sweep <dump> <out>                 initial sweep after eventListInit
  *   trace <dump> <out> <nevents>       event trace + final state
  *   pcf   <dump> <out> <dr> <max_r>    calculate_pcf on the loaded positions
+ *   run   <dump> <out> <tmax> <thermo> main's loop to tmax, final state,
+ *                                      the reference's thermo rows
  *   bench <dump> <ncoll> [maxsec]      valid collisions / wall second
  * Extra arguments after the mode arguments are passed to constantInit.
  */
@@ -186,6 +188,32 @@ int main(int argc, char** argv) {
         write_all(f, d->r, 8, (size_t)nb);
         write_all(f, d->g_r, 8, (size_t)nb);
         fclose(f);
+        return 0;
+    }
+    if (!strcmp(mode, "run")) {
+        /* run <dump> <out> <tmax> <thermo_out>: main's loop bound (t <= tmax);
+         * the reference's own thermo rows are written to thermo_out. */
+        if (argc < 6) fail("run dump out tmax thermo_out");
+        double tm = atof(argv[4]);
+        start(dump, argc - 6, argv + 6);
+        fclose(thermo);
+        thermo = fopen(argv[5], "w");
+        if (!thermo) fail("cannot open thermo output");
+        trace_rec rec;
+        long nev = 0;
+        while (t <= tm) {
+            step(&rec);
+            nev++;
+        }
+        fclose(thermo);
+        thermo = fopen("/dev/null", "w");
+        FILE* f = fopen(argv[3], "wb");
+        if (!f) fail("cannot open output");
+        int64_t n64 = 0;
+        write_all(f, &n64, 8, 1);
+        write_state(f);
+        fclose(f);
+        fprintf(stderr, "{\"events\": %ld}\n", nev);
         return 0;
     }
     if (!strcmp(mode, "bench")) {

@@ -62,8 +62,30 @@ def lattice_state(n, phi, seed, rad=0.5, lattice="square", temperature=1.0):
         x = i * dx + (j % 2) * (dx / 2) + dx / 4
         y = j * dy + dy / 2
         L = None
+    elif lattice == "fit":
+        # triangular stacking fitted to a square box for any n: even row count
+        # chosen for the largest nearest-neighbour distance
+        L = box_length(n, phi, rad)
+        best = None
+        for ny in range(2, 2 * int(math.ceil(math.sqrt(n))) + 5, 2):
+            nx = -(-n // ny)
+            dx, dy = L / nx, L / ny
+            d = min(dx, math.sqrt(dx * dx / 4 + dy * dy), 2 * dy)
+            if best is None or d > best[0]:
+                best = (d, nx, ny)
+        d, nx, ny = best
+        if d < 2 * rad:
+            raise ValueError("fitted lattice overlaps")
+        dx, dy = L / nx, L / ny
+        k = np.arange(n)
+        i, j = k % nx, k // nx
+        x = (i + 0.25 + 0.5 * (j % 2)) * dx
+        y = (j + 0.5) * dy
+        Lx = Ly = L
     else:
         raise ValueError(lattice)
+    if lattice == "square" and L / int(math.ceil(math.sqrt(n))) < 2 * rad:
+        raise ValueError("square lattice overlaps at phi=%g" % phi)
     v = rng.standard_normal((n, 2))
     v -= v.mean(axis=0)
     e_per = 0.5 * (v * v).sum() / n
@@ -200,3 +222,28 @@ def canonical_pairs(tr):
     out["i"] = np.where(coll, lo, tr["i"])
     out["j"] = np.where(coll, hi, tr["j"])
     return out
+
+
+def canonical_trace(tr):
+    """canonical_pairs, and events of exactly equal time as a set: the
+    reference orders simultaneous events by the insertion history of its
+    queue (BST ties go right, bucket lists are LIFO, EDMD.c:2144-2203), not
+    by anything physical.  Runs of equal time are sorted by (kind, i, j)."""
+    out = canonical_pairs(tr)
+    t = out["t"]
+    k = 0
+    n = len(out)
+    while k < n:
+        e = k + 1
+        while e < n and t[e] == t[k]:
+            e += 1
+        if e - k > 1:
+            seg = out[k:e]
+            order = np.lexsort((seg["j"], seg["i"], seg["type"]))
+            out[k:e] = seg[order]
+        k = e
+    return out
+
+
+def bits_equal(a, b):
+    return np.float64(a).view(np.uint64) == np.float64(b).view(np.uint64)

@@ -1,0 +1,144 @@
+"""Writes the golden vectors of tests/golden/*.npz from the reference.
+
+Runs the trace harness built from the reference's own sources
+(oracle/_ref/ref_strict, strict IEEE build, see oracle/Makefile) on seeded
+initial states and stores inputs and outputs as compressed numpy archives:
+
+  inputs     x y vx vy rad m (float64), Lx Ly
+  sweep_*    the initial prediction sweep (eventListInit, EDMD.c:1935-2012):
+             crossing time/direction and collision time/partner/counter
+  trace_*    the first events popped by the dispatch loop (EDMD.c:534-610):
+             kind, i, j, valid flag, time
+  state_*    the state after the trace: x y vx vy t_local coll cell
+  glob_*     t, virial sums, E, px, py, ncol, ncross after the trace
+  thermo     the reference's own thermo rows (saveThermo, EDMD.c:5232-5586)
+  pcf_*      calculate_pcf (pcf.c:16-75) on the input positions
+
+Run from the repository root after `make -C oracle ref`:
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import math
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import edmd_fixtures as F  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def poly_state(n, phi, seed, mean_rad=0.5, rel_sd=0.1, trunc=2.0):
+    """Square-lattice start with Gaussian radii (mean_rad, rel_sd) truncated at
+    +-trunc sd; the lattice spacing exceeds the largest contact distance."""
+    rng = np.random.default_rng(seed)
+    rad = np.empty(0)
+    while len(rad) < n:
+        r = rng.normal(mean_rad, rel_sd * mean_rad, 2 * n)
+        r = r[np.abs(r - mean_rad) <= trunc * rel_sd * mean_rad]
+        rad = np.concatenate([rad, r])
+    rad = rad[:n]
+    L = math.sqrt(np.pi * np.sum(rad * rad) / phi)
+    side = int(math.ceil(math.sqrt(n)))
+    a = L / side
+    assert a > 2 * rad.max(), "lattice too dense for these radii"
+    k = np.arange(n)
+    st = F.lattice_state(n, phi, seed)  # for the velocities
+    st["x"] = (k % side + 0.5) * a
+    st["y"] = (k // side + 0.5) * a
+    st["rad"] = rad
+    st["Lx"] = st["Ly"] = L
+    return st
+
+
+CASES = [
+    # name, state builder, trace length
+    ("cfg1_d1", lambda: F.lattice_state(2000, 0.70, seed=1), 20000),
+    ("cfg1_r1", lambda: F.scale_state(F.lattice_state(2000, 0.70, seed=1), 2.0), 10000),
+    ("n400_phi060", lambda: F.lattice_state(400, 0.60, seed=2), 20000),
+    ("tri1024_phi080", lambda: F.lattice_state(1024, 0.80, seed=5, lattice="tri"), 10000),
+    ("poly500_phi050", lambda: poly_state(500, 0.50, seed=6), 10000),
+    ("tiny_ncx3", lambda: F.lattice_state(20, 0.30, seed=9), 4000),
+    # Boxes of one or two cells per side: the reference's crossing times wrap
+    # wrongly once a cell spans half the box (PBCinsideCellX, EDMD.c:5922-5936)
+    # and its event loop aborts on an overlap, so only the initial sweep (a
+    # 3x3 stencil visiting wrapped duplicates) is kept for these.
+    ("tiny_ncx2", lambda: F.lattice_state(12, 0.30, seed=7), 0),
+    ("tiny_ncx1", lambda: F.lattice_state(4, 0.35, seed=8), 0),
+]
+
+THERMO_CASE = ("thermo64", lambda: F.lattice_state(64, 0.50, seed=4), 1300.0, ["-o", "100"])
+
+
+def state_arrays(st):
+    return {"x": st["x"], "y": st["y"], "vx": st["vx"], "vy": st["vy"], "rad": st["rad"],
+            "m": st["m"], "Lx": np.float64(st["Lx"]), "Ly": np.float64(st["Ly"])}
+
+
+def pack_state(prefix, tr):
+    s = tr["state"]
+    out = {prefix + "_" + k: s[k] for k in s.dtype.names}
+    g = tr["glob"]
+    for k, v in g.items():
+        out["glob_" + k] = np.asarray(v)
+    return out
+
+
+def main():
+    tmp = tempfile.mkdtemp(prefix="edmd_golden_")
+    for name, build, nev in CASES:
+        st = build()
+        dump = os.path.join(tmp, name + ".dump")
+        F.write_dump(dump, st)
+        sw = os.path.join(tmp, name + ".sweep")
+        tr = os.path.join(tmp, name + ".trace")
+        F.run_ref(["sweep", dump, sw])
+        s = F.parse_sweep(sw)
+        arrs = state_arrays(st)
+        rec = s["rec"]
+        for k in rec.dtype.names:
+            arrs["sweep_" + k] = rec[k]
+        arrs["grid"] = np.array([s["ncx"], s["ncy"]], dtype=np.int32)
+        if nev == 0:
+            np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
+            print(name, "N=%d sweep only" % len(st["x"]))
+            continue
+        F.run_ref(["trace", dump, tr, nev])
+        t = F.parse_trace(tr)
+        trace = t["trace"]
+        for k in trace.dtype.names:
+            arrs["trace_" + k] = trace[k]
+        arrs.update(pack_state("state", t))
+        if name == "cfg1_d1":
+            pcf = os.path.join(tmp, name + ".pcf")
+            F.run_ref(["pcf", dump, pcf, 0.1, st["Lx"] / 2])
+            r, g = F.parse_pcf(pcf)
+            arrs["pcf_r"] = r
+            arrs["pcf_g"] = g
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
+        print(name, "N=%d events=%d ncol=%d" % (len(st["x"]), nev, t["glob"]["ncol"]))
+
+    name, build, tmax, extra = THERMO_CASE
+    st = build()
+    dump = os.path.join(tmp, name + ".dump")
+    F.write_dump(dump, st)
+    out = os.path.join(tmp, name + ".state")
+    th = os.path.join(tmp, name + ".thermo")
+    F.run_ref(["run", dump, out, tmax, th] + extra)
+    t = F.parse_trace(out)
+    rows = [list(map(float, line.split())) for line in open(th) if line.strip()]
+    arrs = state_arrays(st)
+    arrs.update(pack_state("state", t))
+    arrs["thermo_rows"] = np.array([r[:8] for r in rows])
+    arrs["tmax"] = np.float64(tmax)
+    arrs["dtime_thermo"] = np.float64(float(extra[1]))
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
+    print(name, "rows=%d ncol=%d" % (len(rows), t["glob"]["ncol"]))
+
+
+if __name__ == "__main__":
+    main()

@@ -97,7 +97,11 @@ class Oracle:
             raise ValueError("orc_create failed")
         self.h = h
         if init_events:
-            self.status = lib().orc_init_events(self.h)
+            lib().orc_init_events(self.h)
+
+    @property
+    def status(self):
+        return lib().orc_status(self.h)
 
     def __del__(self):
         if getattr(self, "h", None):

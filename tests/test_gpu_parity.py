@@ -1,0 +1,333 @@
+"""GPU parity: libedmd_hip.so (through its C ABI) against the reference's
+golden vectors and against the oracle, bit for bit.
+
+Mirrored collision events of a pair predicted from a common synchronised
+state tie exactly and either may pop first (EDMD.c:2179, 2164-2168); the
+state they produce is the same, so traces are compared with collision pairs
+unordered.  Events of exactly equal time (disks starting exactly on a cell
+edge cross at t = 0 together) are compared as sets: the reference orders
+them by its queue's insertion history.  Everything else (kinds, times,
+valid flags, states, virial sums) is compared bitwise, and final states are
+always compared bitwise."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import edmd_fixtures as F
+from conftest import SWEEP_ONLY, TRACED, bits, golden_state, load_golden
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def assert_trace_equal(got, ref):
+    a = F.canonical_trace(ref)
+    b = F.canonical_trace(got)
+    assert len(a) == len(b)
+    for k in ("type", "i", "j", "valid"):
+        bad = np.nonzero(a[k] != b[k])[0]
+        assert len(bad) == 0, "%s differs first at event %d" % (k, bad[0])
+    bad = np.nonzero(bits(a["t"]) != bits(b["t"]))[0]
+    assert len(bad) == 0, "time differs first at event %d" % bad[0]
+
+
+def assert_state_equal(got, ref, prefix=""):
+    for k in ("x", "y", "vx", "vy", "t"):
+        np.testing.assert_array_equal(bits(got[k]), bits(ref[prefix + k]), err_msg=k)
+    np.testing.assert_array_equal(got["coll"].astype(np.uint64),
+                                  np.asarray(ref[prefix + "coll"]).astype(np.uint64))
+    np.testing.assert_array_equal(got["cx"], ref[prefix + "cx"])
+    np.testing.assert_array_equal(got["cy"], ref[prefix + "cy"])
+
+
+def test_fp64_division_and_sqrt_are_ieee(gpu):
+    rng = np.random.default_rng(0)
+    n = 1 << 20
+    a = np.exp(rng.uniform(-40, 40, n))
+    b = np.exp(rng.uniform(-40, 40, n)) * rng.choice([-1.0, 1.0], n)
+    # near-contact cancellation: det = b*b - v2*d with d tiny
+    a[:1000] = 1.0 + rng.uniform(0, 1e-12, 1000)
+    q, s = gpu.fp64_check(a, b)
+    np.testing.assert_array_equal(bits(q), bits(a / b))
+    np.testing.assert_array_equal(bits(s), bits(np.sqrt(a)))
+
+
+@pytest.mark.parametrize("name", TRACED + SWEEP_ONLY)
+def test_sweep_matches_reference(gpu, name):
+    g = load_golden(name)
+    s = gpu.System(golden_state(g))
+    ncx, ncy, cap, _, _ = s.grid()
+    assert (ncx, ncy) == tuple(g["grid"])
+    ev = s.events()
+    np.testing.assert_array_equal(bits(ev["t_cross"]), bits(g["sweep_t_cross"]))
+    np.testing.assert_array_equal(bits(ev["t_coll"]), bits(g["sweep_t_coll"]))
+    np.testing.assert_array_equal(ev["dir"], g["sweep_dir"])
+    np.testing.assert_array_equal(ev["partner"], g["sweep_partner"])
+    np.testing.assert_array_equal(ev["coll_actual"].astype(np.uint64), g["sweep_coll_actual"])
+
+
+@pytest.mark.parametrize("name", TRACED)
+def test_trace_matches_reference(gpu, name):
+    g = load_golden(name)
+    nev = len(g["trace_t"])
+    s = gpu.System(golden_state(g), trace_capacity=nev)
+    s.run(max_events=nev)
+    tr = s.trace()
+    ref = np.zeros(nev, dtype=gpu.TRACE_DT)
+    for k in ("type", "i", "j", "valid", "t"):
+        ref[k] = g["trace_" + k]
+    assert_trace_equal(tr, ref)
+    assert_state_equal(s.state(), g, "state_")
+    ob = s.observables()
+    assert ob["ncol"] == int(g["glob_ncol"]) and ob["ncross"] == int(g["glob_ncross"])
+    for k, r in (("t", "t"), ("vir_xx", "virXX"), ("vir_yy", "virYY"), ("vir_xy", "virXY"),
+                 ("vir_yx", "virYX"), ("E", "E"), ("px", "px"), ("py", "py")):
+        assert bits(ob[k]) == bits(g["glob_" + r]), k
+
+
+def test_thermo_run_matches_reference(gpu, oracle):
+    """t <= tmax loop through 13 screenshots and 4 thermo events (291951
+    collisions): final state bit-exact with the reference; thermo rows equal
+    to the oracle's bitwise and to the reference's printed rows."""
+    g = load_golden("thermo64")
+    dt_th = float(g["dtime_thermo"])
+    s = gpu.System(golden_state(g), dtime_thermo=dt_th)
+    s.run(tmax=float(g["tmax"]), max_events=10_000_000)
+    assert_state_equal(s.state(), g, "state_")
+    assert s.observables()["ncol"] == int(g["glob_ncol"])
+    rows = s.thermo()
+    o = oracle.Oracle(golden_state(g), dtime_thermo=dt_th)
+    o.run_until(float(g["tmax"]))
+    orows = o.thermo_rows()
+    assert len(rows) == len(orows) == len(g["thermo_rows"])
+    for r, q, ref in zip(rows, orows, g["thermo_rows"]):
+        for k in ("t", "e_per_n", "pressure", "pxx", "pyy", "pxy", "pyx"):
+            assert bits(r[k]) == bits(q[k]), k
+        assert int(r["ncol"]) == q["ncol"] == int(ref[1])
+        np.testing.assert_allclose([r["t"], r["e_per_n"], r["pressure"]],
+                                   [ref[0], ref[2], ref[3]], rtol=0, atol=6e-7)
+
+
+CASES = [(300, 0.55, 11), (1000, 0.70, 12), (1500, 0.75, 13), (777, 0.62, 14), (256, 0.50, 15)]
+
+
+@pytest.mark.parametrize("n,phi,seed", CASES)
+def test_random_states_match_oracle(gpu, oracle, n, phi, seed):
+    st = F.lattice_state(n, phi, seed=seed)
+    nev = 30000
+    s = gpu.System(st, trace_capacity=nev)
+    s.run(max_events=nev)
+    o = oracle.Oracle(st)
+    code, tr = o.run(nev)
+    assert code == 0
+    assert_trace_equal(s.trace(), tr)
+    so = o.state()
+    sg = s.state()
+    for k in ("x", "y", "vx", "vy", "t"):
+        np.testing.assert_array_equal(bits(sg[k]), bits(so[k]), err_msg=k)
+
+
+def test_determinism(gpu):
+    st = F.lattice_state(1200, 0.72, seed=3)
+    out = []
+    for _ in range(2):
+        s = gpu.System(st, trace_capacity=20000)
+        s.run(max_events=20000)
+        out.append((s.trace(), s.state()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for k in ("x", "y", "vx", "vy"):
+        np.testing.assert_array_equal(bits(out[0][1][k]), bits(out[1][1][k]))
+
+
+def test_batch_heterogeneous_replicas_match_oracle(gpu, oracle):
+    """Replicas of uneven N and phi in one launch: each evolves exactly as
+    the oracle evolves it alone."""
+    rng = np.random.default_rng(7)
+    states = []
+    for k in range(48):
+        n = int(rng.integers(256, 2049))
+        phi = float(rng.uniform(0.5, 0.75))
+        states.append(F.lattice_state(n, phi, seed=100 + k, lattice="fit"))
+    nev = 15000
+    b = gpu.Batch(states, trace_capacity=nev)
+    b.run(max_events=nev)
+    for k, st in enumerate(states):
+        o = oracle.Oracle(st)
+        code, tr = o.run(nev)
+        assert code == 0
+        so, sb = o.state(), b.state(k)
+        for f in ("x", "y", "vx", "vy", "t"):
+            np.testing.assert_array_equal(bits(sb[f]), bits(so[f]), err_msg="replica %d %s" % (k, f))
+        assert b.observables(k)["ncol"] == o.observables()["ncol"]
+        if k < 3:
+            assert_trace_equal(b.trace(k), tr)
+
+
+def test_batch_collision_cap(gpu):
+    states = [F.lattice_state(400 + 100 * k, 0.6, seed=k) for k in range(6)]
+    b = gpu.Batch(states)
+    b.run(max_collisions=5000, max_events=10**7)
+    for k in range(6):
+        ob = b.observables(k)
+        assert ob["ncol"] == 5000 and ob["stop_reason"] == 2
+
+
+def test_full_size_sweep_and_run(gpu, oracle):
+    """N = 2^20 disks at phi = 0.80 on a triangular lattice: the initial
+    sweep and the first 40000 events match the oracle bit for bit; energy
+    and momentum are conserved."""
+    st = F.lattice_state(1 << 20, 0.80, seed=2024, lattice="tri")
+    s = gpu.System(st, trace_capacity=40000)
+    o = oracle.Oracle(st)
+    eg, eo = s.events(), o.events()
+    for k in ("t_cross", "t_coll"):
+        np.testing.assert_array_equal(bits(eg[k]), bits(eo[k]), err_msg=k)
+    np.testing.assert_array_equal(eg["partner"], eo["partner"])
+    np.testing.assert_array_equal(eg["dir"], eo["dir"])
+    s.run(max_events=40000)
+    code, tr = o.run(40000)
+    assert code == 0
+    assert_trace_equal(s.trace(), tr)
+    og, oo = s.observables(), o.observables()
+    assert bits(og["E"]) == bits(oo["E"])
+    e0 = 0.5 * np.sum(st["vx"] ** 2 + st["vy"] ** 2)
+    assert abs(og["E"] - e0) < 1e-12 * e0
+    assert abs(og["px"]) < 1e-9 and abs(og["py"]) < 1e-9
+
+
+def test_thermostat_rescale_matches_oracle(gpu, oracle):
+    """Velocity-rescale thermostat events (EDMD.c:4828-4923) every 0.05: the
+    trace through ~30 rescales and full re-predictions matches the oracle,
+    and E/N equals the target right after each rescale."""
+    st = F.lattice_state(400, 0.6, seed=77, temperature=2.0)
+    kw = dict(noise=2, dtnoise=0.05, temperature=1.5)
+    nev = 25000
+    s = gpu.System(st, trace_capacity=nev, **kw)
+    s.run(max_events=nev)
+    o = oracle.Oracle(st, **kw)
+    code, tr = o.run(nev)
+    assert code == 0
+    gt = s.trace()
+    assert_trace_equal(gt, tr)
+    assert np.sum(gt["type"] == 4) >= 20
+    so, sg = o.state(), s.state()
+    for k in ("x", "y", "vx", "vy", "t"):
+        np.testing.assert_array_equal(bits(sg[k]), bits(so[k]), err_msg=k)
+    s2 = gpu.System(st, **kw)
+    reason, kind = s2.run_until_kind(1 << 4, max_events=nev)
+    assert kind == 4
+    assert abs(s2.observables()["E"] / 400 - 1.5) < 1e-13
+
+
+def test_overlap_is_reported(gpu):
+    st = F.lattice_state(100, 0.5, seed=1)
+    st["x"][1] = st["x"][0] + 0.5  # centres half a diameter apart
+    st["y"][1] = st["y"][0]
+    with pytest.raises(gpu.EdmdError) as ei:
+        gpu.System(st)
+    assert ei.value.code == 3
+
+
+def test_cell_overflow_is_reported(gpu):
+    st = F.lattice_state(900, 0.7, seed=1)
+    s0 = gpu.System(st)
+    cx, cy = s0.state()["cx"], s0.state()["cy"]
+    occ = np.bincount(cy * s0.grid()[0] + cx).max()
+    s0.close()
+    s = gpu.System(st, cell_capacity=int(occ))
+    with pytest.raises(gpu.EdmdError) as ei:
+        s.run(max_events=200000)
+    assert ei.value.code == 7
+
+
+def test_step_level_api_matches_oracle(gpu, oracle):
+    st = F.lattice_state(500, 0.65, seed=5)
+    s = gpu.System(st)
+    o = oracle.Oracle(st)
+    ev = o.events()
+    t, kind, i, j, ca = s.peek()
+    k = np.argmin(np.minimum(ev["t_coll"], ev["t_cross"]))
+    assert t == min(ev["t_coll"][k], ev["t_cross"][k]) and i == k
+    for a, b_ in [(0, 1), (3, 25), (100, 101), (499, 0)]:
+        assert bits(s.collision_time(a, b_)) == bits(o.collision_time(a, b_))
+    # re-predicting a synchronised particle reproduces its slots
+    before = s.events()
+    s.repredict(17, 3)
+    after = s.events()
+    assert bits(after["t_coll"][17]) == bits(before["t_coll"][17])
+    assert after["partner"][17] == before["partner"][17]
+
+
+def test_compat_entry_points_follow_reference_loop(gpu, oracle, tmp_path):
+    """constantInit / particlesInit / eventListInit and the step-wise loop of
+    main() (EDMD.c:508-610) driven through the compatibility layer."""
+    L = gpu.lib()
+    st = F.lattice_state(300, 0.6, seed=9)
+    dump = str(tmp_path / "init.dump")
+    F.write_dump(dump, st)
+    node_p = C.c_void_p
+    L.constantInit.argtypes = [C.c_int, C.POINTER(C.c_char_p)]
+    L.findNextEvent.restype = node_p
+    L.edmd_compat_status.restype = C.c_int
+    args = [b"edmd", b"-l", dump.encode(), b"-t", b"1e9", b"--no-output"]
+    argv = (C.c_char_p * len(args))(*args)
+    L.constantInit(len(args), argv)
+    L.particlesInit()
+    L.cellListInit()
+    L.eventListInit()
+    L.physicalQ()
+    assert L.edmd_compat_status() == 0
+    assert C.c_int.in_dll(L, "N").value == 300
+
+    class Node(C.Structure):
+        _fields_ = [("lft", C.c_void_p), ("rgt", C.c_void_p), ("top", C.c_void_p),
+                    ("i", C.c_int), ("j", C.c_int), ("q", C.c_int), ("t", C.c_double),
+                    ("type", C.c_int), ("collActual", C.c_ulong)]
+
+    o = oracle.Oracle(st)
+    code, tr = o.run(300)
+    handlers = {0: L.doTheCrossing, 1: L.doTheCollisionNormal, 2: L.takeAScreenshot,
+                3: L.takeAThermo}
+    got = []
+    for _ in range(300):
+        nd = Node.from_address(L.findNextEvent())
+        got.append((nd.type, nd.i, nd.j, nd.t))
+        L.removeEventFromQueue(C.c_void_p(C.addressof(nd)))
+        handlers[nd.type]()
+        assert L.edmd_compat_status() == 0
+    for (ty, i, j, t), r in zip(got, tr):
+        assert ty == r["type"] and bits(t) == bits(r["t"])
+        if ty == 1:
+            assert sorted((i, j)) == sorted((r["i"], r["j"]))
+        else:
+            assert (i, j) == (r["i"], r["j"])
+    assert C.c_ulong.in_dll(L, "ncol").value == o.observables()["ncol"]
+    L.freeArrays()
+
+
+def test_cli_writes_reference_outputs(gpu, oracle, tmp_path):
+    """`edmd -l init.dump -t 1.5 --full-precision`: the screenshot frame at
+    t = 1 holds the state of the reference after its free flight of every
+    disk (EDMD.c:4659-4661)."""
+    st = F.lattice_state(400, 0.6, seed=13)
+    dump = str(tmp_path / "init.dump")
+    F.write_dump(dump, st)
+    cli = os.path.join(REPO, "graphical-edmd_amd", "bin", "edmd")
+    r = subprocess.run([cli, "-l", dump, "-t", "1.5", "--full-precision"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    dumps = [f for f in os.listdir(tmp_path / "dump") if f.endswith(".dump")]
+    assert len(dumps) == 1
+    frame = F.read_dump(str(tmp_path / "dump" / dumps[0]))
+    o = oracle.Oracle(st)
+    o.run_until(1.0 - 1e-12)
+    # handle events up to and including the screenshot at t = 1
+    while o.t < 1.0:
+        o.run(1, trace=False)
+    so = o.state()
+    np.testing.assert_array_equal(bits(frame["x"]), bits(so["x"]))
+    np.testing.assert_array_equal(bits(frame["vy"]), bits(so["vy"]))

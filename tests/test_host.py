@@ -1,0 +1,156 @@
+"""CPU-side checks of libedmd_hip.so: the C ABI exports every entry point its
+headers declare, failures are reported as status codes (never a fallback),
+and the host-only pieces (lattice starts, LAMMPS dump I/O, option parsing)
+behave as specified.  No kernel runs here."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import edmd_hip as E
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(REPO, "graphical-edmd_amd", "bin", "edmd")
+
+
+def declared_functions(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set()
+    for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text, re.M):
+        line = m.group(0)
+        if line.startswith(("typedef", "#", "extern void (*")):
+            continue
+        names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_entry_point():
+    L = E.lib()
+    missing = []
+    for h in E.HEADER_PATHS:
+        names = declared_functions(h)
+        assert len(names) > 10
+        for n in sorted(names):
+            if not hasattr(L, n):
+                missing.append(n)
+    assert not missing, missing
+
+
+def test_compat_globals_exported():
+    L = E.lib()
+    assert C.c_int.in_dll(L, "N").value == 500  # the reference's default, EDMD.c:80
+    assert C.c_double.in_dll(L, "phi").value == 0.38
+    assert C.c_double.in_dll(L, "tmax").value == 4000
+    for name in ("Lx", "Ly", "t", "E", "px", "py", "pressure", "res"):
+        C.c_double.in_dll(L, name)
+    for name in ("ncol", "ncross"):
+        C.c_ulong.in_dll(L, name)
+
+
+@pytest.mark.skipif(E.device_count() > 0, reason="checks the no-device path")
+def test_no_device_is_an_error_not_a_fallback():
+    st = E.init_lattice(64, 0.5)
+    with pytest.raises(E.EdmdError) as ei:
+        E.System(st)
+    assert ei.value.code == 19
+    assert "no HIP device" in str(ei.value)
+
+
+@pytest.mark.parametrize("lattice,n", [(0, 1000), (0, 4096), (1, 1024), (1, 4096), (2, 257),
+                                       (2, 2048), (2, 1111)])
+def test_lattice_start(lattice, n):
+    phi, rad, temp = (0.75 if lattice == 2 else 0.7), 0.5, 1.3
+    st = E.init_lattice(n, phi, rad=rad, lattice=lattice, seed=9, temperature=temp)
+    vx, vy = st["vx"], st["vy"]
+    assert abs(vx.sum()) < 1e-12 * n and abs(vy.sum()) < 1e-12 * n
+    assert abs(0.5 * np.sum(vx * vx + vy * vy) / n - temp) < 1e-12
+    area = st["Lx"] * st["Ly"]
+    assert abs(n * np.pi * rad * rad / area - phi) < 1e-12
+    x, y = st["x"], st["y"]
+    assert x.min() >= 0 and x.max() < st["Lx"] and y.min() >= 0 and y.max() < st["Ly"]
+    # nearest-neighbour distance through the periodic box
+    from scipy.spatial import cKDTree
+    tree = cKDTree(np.c_[x, y], boxsize=[st["Lx"], st["Ly"]])
+    d, _ = tree.query(np.c_[x, y], k=2)
+    assert d[:, 1].min() > 2 * rad
+    st2 = E.init_lattice(n, phi, rad=rad, lattice=lattice, seed=9, temperature=temp)
+    assert np.array_equal(st2["vx"], vx)  # seeded
+    st3 = E.init_lattice(n, phi, rad=rad, lattice=lattice, seed=10, temperature=temp)
+    assert not np.array_equal(st3["vx"], vx)
+
+
+def test_lattice_rejects_impossible_packing():
+    with pytest.raises(E.EdmdError) as ei:
+        E.init_lattice(100, 0.85, rad=0.5, lattice=0)  # square lattice overlaps above pi/4
+    assert ei.value.code == 22
+
+
+def test_dump_round_trip(tmp_path):
+    L = E.lib()
+    st = E.init_lattice(300, 0.6, rad=0.5, seed=4)
+    st["rad"] = np.linspace(0.4, 0.6, 300)
+    p = str(tmp_path / "a.dump").encode()
+    args = [st[k].ctypes.data for k in ("x", "y", "vx", "vy", "rad", "m")]
+    E.check(L.edmd_dump_write(p, 0, 1.5, 300, st["Lx"], st["Ly"], *args, 1))
+    n, lx, ly = C.c_int(), C.c_double(), C.c_double()
+    E.check(L.edmd_dump_info(p, C.byref(n), C.byref(lx), C.byref(ly)))
+    assert (n.value, lx.value, ly.value) == (300, st["Lx"], st["Ly"])
+    back = {k: np.zeros(300) for k in ("x", "y", "vx", "vy", "rad", "m")}
+    E.check(L.edmd_dump_read(p, 300, *[back[k].ctypes.data for k in
+                                       ("x", "y", "vx", "vy", "rad", "m")]))
+    for k in back:
+        assert np.array_equal(back[k], st[k]), k
+    # the reference's own layout (x printed with 3 decimals, EDMD.c:5124),
+    # appended as a second frame: the reader takes the last frame
+    E.check(L.edmd_dump_write(p, 1, 2.5, 300, st["Lx"], st["Ly"], *args, 0))
+    E.check(L.edmd_dump_read(p, 300, *[back[k].ctypes.data for k in
+                                       ("x", "y", "vx", "vy", "rad", "m")]))
+    assert np.max(np.abs(back["x"] - st["x"])) <= 5e-4
+    assert np.max(np.abs(back["vx"] - st["vx"])) <= 5e-7
+
+
+def test_dump_reader_matches_fixture_writer(tmp_path):
+    """Dumps in the layout the reference loads (tests/golden/edmd_fixtures)."""
+    import edmd_fixtures as F
+    st = F.lattice_state(500, 0.65, seed=3)
+    path = str(tmp_path / "init.dump")
+    F.write_dump(path, st)
+    n, lx, ly = C.c_int(), C.c_double(), C.c_double()
+    E.check(E.lib().edmd_dump_info(path.encode(), C.byref(n), C.byref(lx), C.byref(ly)))
+    assert n.value == 500 and lx.value == st["Lx"]
+    back = {k: np.zeros(500) for k in ("x", "y", "vx", "vy", "rad", "m")}
+    E.check(E.lib().edmd_dump_read(path.encode(), 500, *[back[k].ctypes.data for k in
+                                                          ("x", "y", "vx", "vy", "rad", "m")]))
+    for k in back:
+        assert np.array_equal(back[k], st[k])
+
+
+def test_dump_errors(tmp_path):
+    L = E.lib()
+    n = C.c_int()
+    assert L.edmd_dump_info(str(tmp_path / "missing.dump").encode(), C.byref(n), None, None) == 5
+    bad = tmp_path / "bad.dump"
+    bad.write_text("not a dump\n")
+    assert L.edmd_dump_info(str(bad).encode(), C.byref(n), None, None) == 5
+
+
+@pytest.mark.skipif(E.device_count() > 0, reason="checks the no-device path")
+def test_cli_reports_missing_device():
+    r = subprocess.run([CLI, "-N", "100", "-p", "0.5", "-x", "0", "-t", "10", "--no-output"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 19
+    assert "no HIP device" in r.stderr
+
+
+def test_cli_rejects_growth_start():
+    """Without -l and without -x 0 the reference starts a binary mixture by
+    random insertion and growth; this engine reports that it has no such
+    start instead of substituting another state."""
+    r = subprocess.run([CLI, "-N", "100", "-p", "0.5", "-t", "10", "--no-output"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 22
+    assert "-x 0" in r.stderr
