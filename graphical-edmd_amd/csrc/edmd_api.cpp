@@ -84,13 +84,11 @@ struct edmd_batch {
     size_t np = 0, ncell = 0, nslot = 0, nnode = 0, ntrace = 0, nthermo = 0;
     std::vector<void*> allocs;
     /* pooled device arrays */
-    double *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr, *tl = nullptr,
-           *rad = nullptr, *m = nullptr, *tcoll = nullptr, *tcross = nullptr, *key = nullptr;
-    uint32_t *coll = nullptr, *cact = nullptr;
-    int32_t *cx = nullptr, *cy = nullptr, *crx = nullptr, *cry = nullptr, *partner = nullptr,
-            *dir = nullptr;
-    double* lkey = nullptr;
-    int32_t* lidx = nullptr;
+    EdmdPart* part = nullptr;
+    EdmdEv* ev = nullptr;
+    EdmdNode* leaf = nullptr;
+    EdmdNode* nodes = nullptr;
+    int32_t* img = nullptr;
     int32_t *ccnt = nullptr, *cslot = nullptr;
     EdmdState* st = nullptr;
     EdmdTraceRec* trace = nullptr;
@@ -165,7 +163,8 @@ static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
     c.ncx = (int)(c.halfLx);
     c.ncy = (int)(c.halfLy);
     if (c.ncx < 1 || c.ncy < 1) return fail(EDMD_ERR_ARG, "system %d: box side below 2 (no cell grid)", idx);
-    if ((double)c.ncx * c.ncy > 2.0e9) return fail(EDMD_ERR_ARG, "system %d: cell grid too large", idx);
+    if (c.ncx > 32767 || c.ncy > 32767 || (double)c.ncx * c.ncy > 2.0e9)
+        return fail(EDMD_ERR_ARG, "system %d: cell grid too large", idx);
     c.csx = d->Lx / c.ncx;
     c.csy = d->Ly / c.ncy;
     c.res = d->res;
@@ -225,19 +224,23 @@ static int upload_system(edmd_batch* b, int s, const edmd_system_desc* d) {
         slot[cell * c.cap + cnt[cell]++] = i;
     }
     const size_t p0 = h.p0;
-    const size_t nb = (size_t)n * sizeof(double);
-    HIPCHK(hipMemcpy(b->x + p0, x.data(), nb, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->y + p0, y.data(), nb, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->vx + p0, d->vx, nb, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->vy + p0, d->vy, nb, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->rad + p0, d->rad, nb, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->m + p0, d->m, nb, hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(b->tl + p0, 0, nb));
-    HIPCHK(hipMemset(b->coll + p0, 0, (size_t)n * sizeof(uint32_t)));
-    HIPCHK(hipMemset(b->crx + p0, 0, (size_t)n * sizeof(int32_t)));
-    HIPCHK(hipMemset(b->cry + p0, 0, (size_t)n * sizeof(int32_t)));
-    HIPCHK(hipMemcpy(b->cx + p0, cx.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->cy + p0, cy.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+    std::vector<EdmdPart> part(n);
+    for (int i = 0; i < n; i++) {
+        EdmdPart& r = part[i];
+        memset(&r, 0, sizeof r);
+        r.x = x[i];
+        r.y = y[i];
+        r.vx = d->vx[i];
+        r.vy = d->vy[i];
+        r.tl = 0;
+        r.rad = d->rad[i];
+        r.m = d->m[i];
+        r.coll = 0;
+        r.cx = (int16_t)cx[i];
+        r.cy = (int16_t)cy[i];
+    }
+    HIPCHK(hipMemcpy(b->part + p0, part.data(), (size_t)n * sizeof(EdmdPart), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(b->img + 2 * p0, 0, (size_t)n * 2 * sizeof(int32_t)));
     HIPCHK(hipMemcpy(b->ccnt + h.cell0, cnt.data(), ncell * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->cslot + h.slot0, slot.data(), ncell * c.cap * sizeof(int32_t),
                      hipMemcpyHostToDevice));
@@ -308,26 +311,11 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
     }
     int rc = EDMD_OK;
     const size_t np = b->np;
-    if (!rc) rc = dalloc(b, &b->x, np);
-    if (!rc) rc = dalloc(b, &b->y, np);
-    if (!rc) rc = dalloc(b, &b->vx, np);
-    if (!rc) rc = dalloc(b, &b->vy, np);
-    if (!rc) rc = dalloc(b, &b->tl, np);
-    if (!rc) rc = dalloc(b, &b->rad, np);
-    if (!rc) rc = dalloc(b, &b->m, np);
-    if (!rc) rc = dalloc(b, &b->tcoll, np);
-    if (!rc) rc = dalloc(b, &b->tcross, np);
-    if (!rc) rc = dalloc(b, &b->key, np);
-    if (!rc) rc = dalloc(b, &b->coll, np);
-    if (!rc) rc = dalloc(b, &b->cact, np);
-    if (!rc) rc = dalloc(b, &b->cx, np);
-    if (!rc) rc = dalloc(b, &b->cy, np);
-    if (!rc) rc = dalloc(b, &b->crx, np);
-    if (!rc) rc = dalloc(b, &b->cry, np);
-    if (!rc) rc = dalloc(b, &b->partner, np);
-    if (!rc) rc = dalloc(b, &b->dir, np);
-    if (!rc) rc = dalloc(b, &b->lkey, b->nnode);
-    if (!rc) rc = dalloc(b, &b->lidx, b->nnode);
+    if (!rc) rc = dalloc(b, &b->part, np);
+    if (!rc) rc = dalloc(b, &b->ev, np);
+    if (!rc) rc = dalloc(b, &b->leaf, np);
+    if (!rc) rc = dalloc(b, &b->nodes, b->nnode);
+    if (!rc) rc = dalloc(b, &b->img, 2 * np);
     if (!rc) rc = dalloc(b, &b->ccnt, b->ncell);
     if (!rc) rc = dalloc(b, &b->cslot, b->nslot);
     if (!rc) rc = dalloc(b, &b->st, (size_t)nsys);
@@ -347,16 +335,11 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
         S.c = h.c;
         EdmdArrays& A = S.a;
         size_t p0 = h.p0;
-        A.x = b->x + p0; A.y = b->y + p0; A.vx = b->vx + p0; A.vy = b->vy + p0;
-        A.tl = b->tl + p0; A.rad = b->rad + p0; A.m = b->m + p0;
-        A.coll = b->coll + p0; A.cx = b->cx + p0; A.cy = b->cy + p0;
-        A.crx = b->crx + p0; A.cry = b->cry + p0;
-        A.tcoll = b->tcoll + p0; A.partner = b->partner + p0; A.cact = b->cact + p0;
-        A.tcross = b->tcross + p0; A.dir = b->dir + p0; A.key = b->key + p0;
-        for (int l = 1; l <= h.c.nlev; l++) {
-            A.lkey[l - 1] = b->lkey + h.node0[l - 1];
-            A.lidx[l - 1] = b->lidx + h.node0[l - 1];
-        }
+        A.p = b->part + p0;
+        A.e = b->ev + p0;
+        A.leaf = b->leaf + p0;
+        A.img = b->img + 2 * p0;
+        for (int l = 1; l <= h.c.nlev; l++) A.lev[l - 1] = b->nodes + h.node0[l - 1];
         A.ccnt = b->ccnt + h.cell0;
         A.cslot = b->cslot + h.slot0;
         A.st = b->st + s;
@@ -412,7 +395,7 @@ static int batch_run_mask(edmd_batch* b, int64_t max_events, uint64_t max_collis
     ra.max_coll = max_collisions;
     ra.tmax = tmax;
     ra.stop_mask = stop_mask;
-    HIPCHK(edmd_launch_run(b->d_sys, b->nsys, ra, b->stream));
+    HIPCHK(edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, ra, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     return check_status(b);
 }
@@ -467,15 +450,21 @@ extern "C" int edmd_batch_get_state(edmd_batch* b, int s, double* x, double* y, 
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(b->stream));
     const HostSys& h = b->hs[s];
-    size_t n = h.c.n, p0 = h.p0;
-    if ((rc = d2h(x, b->x + p0, n))) return rc;
-    if ((rc = d2h(y, b->y + p0, n))) return rc;
-    if ((rc = d2h(vx, b->vx + p0, n))) return rc;
-    if ((rc = d2h(vy, b->vy + p0, n))) return rc;
-    if ((rc = d2h(t_local, b->tl + p0, n))) return rc;
-    if ((rc = d2h(coll, b->coll + p0, n))) return rc;
-    if ((rc = d2h(cx, b->cx + p0, n))) return rc;
-    return d2h(cy, b->cy + p0, n);
+    size_t n = h.c.n;
+    std::vector<EdmdPart> part(n);
+    if ((rc = d2h(part.data(), b->part + h.p0, n))) return rc;
+    for (size_t i = 0; i < n; i++) {
+        const EdmdPart& r = part[i];
+        if (x) x[i] = r.x;
+        if (y) y[i] = r.y;
+        if (vx) vx[i] = r.vx;
+        if (vy) vy[i] = r.vy;
+        if (t_local) t_local[i] = r.tl;
+        if (coll) coll[i] = r.coll;
+        if (cx) cx[i] = r.cx;
+        if (cy) cy[i] = r.cy;
+    }
+    return EDMD_OK;
 }
 
 extern "C" int edmd_batch_get_events(edmd_batch* b, int s, double* t_coll, int32_t* partner,
@@ -484,11 +473,16 @@ extern "C" int edmd_batch_get_events(edmd_batch* b, int s, double* t_coll, int32
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(b->stream));
     const HostSys& h = b->hs[s];
-    size_t n = h.c.n, p0 = h.p0;
-    if ((rc = d2h(t_coll, b->tcoll + p0, n))) return rc;
-    if ((rc = d2h(partner, b->partner + p0, n))) return rc;
-    if ((rc = d2h(t_cross, b->tcross + p0, n))) return rc;
-    return d2h(dir, b->dir + p0, n);
+    size_t n = h.c.n;
+    std::vector<EdmdEv> ev(n);
+    if ((rc = d2h(ev.data(), b->ev + h.p0, n))) return rc;
+    for (size_t i = 0; i < n; i++) {
+        if (t_coll) t_coll[i] = ev[i].tcoll;
+        if (partner) partner[i] = ev[i].partner;
+        if (t_cross) t_cross[i] = ev[i].tcross;
+        if (dir) dir[i] = ev[i].dir;
+    }
+    return EDMD_OK;
 }
 
 extern "C" int edmd_batch_get_observables(edmd_batch* b, int s, edmd_observables* o) {
@@ -500,17 +494,16 @@ extern "C" int edmd_batch_get_observables(edmd_batch* b, int s, edmd_observables
     EdmdState st;
     HIPCHK(hipMemcpy(&st, b->st + s, sizeof st, hipMemcpyDeviceToHost));
     size_t n = h.c.n;
-    std::vector<double> vx(n), vy(n), m(n);
-    if ((rc = d2h(vx.data(), b->vx + h.p0, n))) return rc;
-    if ((rc = d2h(vy.data(), b->vy + h.p0, n))) return rc;
-    if ((rc = d2h(m.data(), b->m + h.p0, n))) return rc;
+    std::vector<EdmdPart> part(n);
+    if ((rc = d2h(part.data(), b->part + h.p0, n))) return rc;
     /* physicalQ, EDMD.c:5968-6013, summed in particle order */
     double E = 0, px = 0, py = 0;
     for (size_t i = 0; i < n; i++) {
-        px += vx[i] * m[i];
-        py += vy[i] * m[i];
-        double a = vx[i];
-        E += 0.5 * m[i] * (a * a + vy[i] * vy[i]);
+        const EdmdPart& r = part[i];
+        px += r.vx * r.m;
+        py += r.vy * r.m;
+        double a = r.vx;
+        E += 0.5 * r.m * (a * a + r.vy * r.vy);
     }
     o->t = st.t;
     o->ncol = st.ncol;
@@ -608,8 +601,12 @@ extern "C" int edmd_get_events(edmd_ctx* c, double* t_cross, int32_t* dir, doubl
                                int32_t* partner, uint32_t* coll_actual) {
     if (!c) return fail(EDMD_ERR_ARG, "null context");
     int rc = edmd_batch_get_events(c->b, 0, t_coll, partner, t_cross, dir);
-    if (rc) return rc;
-    return d2h(coll_actual, c->b->cact + c->b->hs[0].p0, (size_t)c->b->hs[0].c.n);
+    if (rc || !coll_actual) return rc;
+    size_t n = c->b->hs[0].c.n;
+    std::vector<EdmdEv> ev(n);
+    if ((rc = d2h(ev.data(), c->b->ev + c->b->hs[0].p0, n))) return rc;
+    for (size_t i = 0; i < n; i++) coll_actual[i] = ev[i].cact;
+    return EDMD_OK;
 }
 
 extern "C" int edmd_get_observables(edmd_ctx* c, edmd_observables* o) {
@@ -694,7 +691,7 @@ extern "C" int edmd_repredict(edmd_ctx* c, int i, int which) {
     int rc = particle_check(c, i);
     if (rc) return rc;
     if (which < 1 || which > 3) return fail(EDMD_ERR_ARG, "which must be 1, 2 or 3");
-    HIPCHK(edmd_launch_repredict(c->b->d_sys, i, which, c->b->stream));
+    HIPCHK(edmd_launch_repredict(c->b->d_sys, c->b->max_nlev, i, which, c->b->stream));
     HIPCHK(hipStreamSynchronize(c->b->stream));
     return check_status(c->b);
 }

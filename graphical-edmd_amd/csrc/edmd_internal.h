@@ -11,10 +11,12 @@ hipError_t edmd_launch_sweep(const EdmdSys* d_sys, int nsys, int max_n, hipStrea
 hipError_t edmd_launch_tree(const EdmdSys* d_sys, int nsys, const int* max_lsize, int max_nlev,
                             hipStream_t s);
 hipError_t edmd_launch_sync_all(const EdmdSys* d_sys, int nsys, int max_n, hipStream_t s);
-hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, EdmdRunArgs ra, hipStream_t s);
+hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, EdmdRunArgs ra,
+                           hipStream_t s);
 hipError_t edmd_launch_peek(const EdmdSys* d_sys, int nsys, EdmdPeek* out, hipStream_t s);
 hipError_t edmd_launch_pair_time(const EdmdSys* d_sys, int i, int j, double* out, hipStream_t s);
-hipError_t edmd_launch_repredict(const EdmdSys* d_sys, int i, int which, hipStream_t s);
+hipError_t edmd_launch_repredict(const EdmdSys* d_sys, int max_nlev, int i, int which,
+                                 hipStream_t s);
 hipError_t edmd_launch_fly_one(const EdmdSys* d_sys, int i, hipStream_t s);
 hipError_t edmd_launch_fp64_check(int n, const double* a, const double* b, double* q, double* sq,
                                   hipStream_t s);

@@ -11,6 +11,8 @@
  *                  earliest event, applies the reference's handlers
  *                  (EDMD.c:534-610, 2501-2590, 3542-3915, 4657-4713,
  *                  4828-4923) and re-predicts with all 64 lanes
+ *   k_peek, k_pair_time, k_repredict, k_fly_one
+ *                  single steps for the reference-compatible entry points
  *   k_sync_all     free flight of every particle to the system time
  *   k_fp64_check   IEEE division / square root probe for the tests
  *
@@ -32,6 +34,15 @@
  * or LDS stores in program order; this only keeps the compiler from moving
  * memory operations across the point. */
 #define WAVE_SYNC() __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront")
+
+/* Values that are equal on every lane, pinned to scalar registers. */
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uni(double v) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readfirstlane((int)b);
+    int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 /* ------------------------------------------------------------ kinematics */
 
@@ -76,7 +87,7 @@ __device__ __forceinline__ void fly(const EdmdConst& C, double t, double& x, dou
 }
 
 /* collisionTimeNormal, EDMD.c:2661-2723: relative time until contact of the
- * synchronised owner (x1..) with candidate j last updated at tlj. */
+ * synchronised owner (x1..) with candidate 2 last updated at tl2. */
 __device__ __forceinline__ double coll_time(const EdmdConst& C, double t, double x1, double y1,
                                             double vx1, double vy1, double r1, double x2,
                                             double y2, double vx2, double vy2, double tl2,
@@ -127,11 +138,15 @@ __device__ __forceinline__ void cross_time(const EdmdConst& C, double t, double 
     }
 }
 
-/* Leaf key of a particle: its earlier slot; the crossing wins ties and a NaN
- * key is parked at +inf so that it is never popped. */
+/* Calendar leaf of a particle: its earlier slot (the crossing wins ties, a
+ * NaN key is parked at +inf so that it is never popped) and the event code:
+ * the partner for a collision, minus the direction for a crossing. */
 __device__ __forceinline__ double leaf_key(double tcoll, double tcross) {
     double k = (tcoll < tcross) ? tcoll : tcross;
     return (k == k) ? k : INF_D;
+}
+__device__ __forceinline__ int leaf_aux(double tcoll, double tcross, int partner, int dir) {
+    return (tcoll < tcross) ? partner : -dir;
 }
 
 /* ------------------------------------------------------------ predictions */
@@ -141,22 +156,22 @@ __device__ __forceinline__ double leaf_key(double tcoll, double tcross) {
  * entry first (EDMD.c:2959-3003).  Returns the relative time; NEVER and
  * partner 0 when no candidate qualifies (EDMD.c:2830-2831). */
 __device__ __forceinline__ double predict_seq(const EdmdConst& C, const EdmdArrays& A, double t,
-                                              int p, double x, double y, double vx, double vy,
-                                              double r, int cx, int cy, int& partner,
+                                              int p, const EdmdPart& o, int& partner,
                                               bool& overlap) {
     double dt = EDMD_NEVER;
     int best = 0;
     for (int j = -1; j <= 1; ++j) {
-        int row = wrap_cell(cy + j, C.ncy) * C.ncx;
+        int row = wrap_cell(o.cy + j, C.ncy) * C.ncx;
         for (int k = -1; k <= 1; ++k) {
-            int c = row + wrap_cell(cx + k, C.ncx);
+            int c = row + wrap_cell(o.cx + k, C.ncx);
             int cnt = A.ccnt[c];
             const int32_t* sl = A.cslot + (size_t)c * C.cap;
             for (int s = cnt - 1; s >= 0; --s) {
                 int q = sl[s];
                 if (q == p) continue;
-                double tt = coll_time(C, t, x, y, vx, vy, r, A.x[q], A.y[q], A.vx[q], A.vy[q],
-                                      A.tl[q], A.rad[q], overlap);
+                const EdmdPart c2 = A.p[q];
+                double tt = coll_time(C, t, o.x, o.y, o.vx, o.vy, o.rad, c2.x, c2.y, c2.vx, c2.vy,
+                                      c2.tl, c2.rad, overlap);
                 if (dt > tt) {
                     dt = tt;
                     best = q;
@@ -168,148 +183,253 @@ __device__ __forceinline__ double predict_seq(const EdmdConst& C, const EdmdArra
     return dt;
 }
 
-/* Wave-cooperative neighbour scan for one owner: the candidates of the nine
- * cells are laid out in traversal order and spread over the 64 lanes; the
- * lexicographic minimum of (time, traversal rank) over candidates with time
- * < NEVER is the reference's strict-less first minimum.  All lanes return
- * the same relative time, partner and the partner's collision counter. */
-__device__ __forceinline__ double predict_wave(const EdmdConst& C, const EdmdArrays& A, int lane,
-                                               double t, int p, double x, double y, double vx,
-                                               double vy, double r, int cx, int cy,
-                                               int& partner, uint32_t& pcoll, bool& overlap) {
+struct Owner {
+    int p;
+    double x, y, vx, vy, r;
+    int cx, cy;
+};
+
+/* Wave-cooperative neighbour scan for one or two owners at once.
+ *
+ * Lanes 0-8 read the nine cell counts of owner a, lanes 16-24 those of
+ * owner b; a segmented scan lays each owner's candidates out in traversal
+ * order (y offset outer, x offset inner, newest list entry first, EDMD.c:
+ * 2959-3003).  Lane f evaluates candidate f of a and candidate f of b, so
+ * both owners share the slot and record round trips.  The minimum over
+ * (time, traversal rank) among candidates with time < NEVER is the
+ * reference's strict-less first minimum; no candidate gives NEVER and
+ * partner 0 (EDMD.c:2830-2831, 3091-3093).  Results are uniform. */
+__device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArrays& A, int lane,
+                                               double t, int nown, const Owner& a, const Owner& b,
+                                               double& dta, int& pa, uint32_t& ca, double& dtb,
+                                               int& pb, uint32_t& cb, bool& overlap) {
+    const int sub = lane & 15;
+    const int grp = lane >> 4;
     int cell = 0, cnt = 0;
-    if (lane < 9) {
-        int dyo = lane / 3 - 1, dxo = lane % 3 - 1;
-        cell = wrap_cell(cy + dyo, C.ncy) * C.ncx + wrap_cell(cx + dxo, C.ncx);
+    if (sub < 9 && (grp == 0 || (grp == 1 && nown == 2))) {
+        const int ocx = grp ? b.cx : a.cx;
+        const int ocy = grp ? b.cy : a.cy;
+        cell = wrap_cell(ocy + sub / 3 - 1, C.ncy) * C.ncx + wrap_cell(ocx + sub % 3 - 1, C.ncx);
         cnt = A.ccnt[cell];
     }
     int incl = cnt;
 #pragma unroll
     for (int d = 1; d < 16; d <<= 1) {
-        int v = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += v;
+        int v = __shfl_up(incl, d, 16);
+        if (sub >= d) incl += v;
     }
-    int excl = incl - cnt;
-    int total = __shfl(incl, 8, 64);
+    const int excl = incl - cnt;
+    const int totA = __builtin_amdgcn_readlane(incl, 8);
+    const int totB = (nown == 2) ? __builtin_amdgcn_readlane(incl, 24) : 0;
+    const int tot = totA > totB ? totA : totB;
 
-    double bt = INF_D;
-    int br = 0x7fffffff;
-    int bq = 0;
-    uint32_t bc = 0;
-    for (int base = 0; base < total; base += 64) {
-        int f = base + lane;
-        int seg = 0;
+    double btA = INF_D, btB = INF_D;
+    int brA = 0x7fffffff, brB = 0x7fffffff, bqA = 0, bqB = 0;
+    uint32_t bcA = 0, bcB = 0;
+    for (int base = 0; base < tot; base += 64) {
+        const int f = base + lane;
+        int sA = 0, sB = 16;
 #pragma unroll
-        for (int s = 1; s < 9; ++s) seg += (f >= __builtin_amdgcn_readlane(excl, s)) ? 1 : 0;
-        int scell = __shfl(cell, seg, 64);
-        int scnt = __shfl(cnt, seg, 64);
-        int sexcl = __shfl(excl, seg, 64);
-        if (f < total) {
-            int local = f - sexcl;
-            int q = A.cslot[(size_t)scell * C.cap + (scnt - 1 - local)];
-            if (q != p) {
-                double tt = coll_time(C, t, x, y, vx, vy, r, A.x[q], A.y[q], A.vx[q], A.vy[q],
-                                      A.tl[q], A.rad[q], overlap);
-                if (tt < EDMD_NEVER && tt < bt) {
-                    bt = tt;
-                    br = seg * C.cap + local;
-                    bq = q;
-                    bc = A.coll[q];
-                }
+        for (int s = 1; s < 9; ++s) {
+            sA += (f >= __builtin_amdgcn_readlane(excl, s)) ? 1 : 0;
+            sB += (f >= __builtin_amdgcn_readlane(excl, 16 + s)) ? 1 : 0;
+        }
+        const int cellA = __shfl(cell, sA, 64), cntA = __shfl(cnt, sA, 64), exA = __shfl(excl, sA, 64);
+        const int cellB = __shfl(cell, sB, 64), cntB = __shfl(cnt, sB, 64), exB = __shfl(excl, sB, 64);
+        int qA = -1, qB = -1;
+        if (f < totA) qA = A.cslot[(size_t)cellA * C.cap + (cntA - 1 - (f - exA))];
+        if (f < totB) qB = A.cslot[(size_t)cellB * C.cap + (cntB - 1 - (f - exB))];
+        const bool vA = qA >= 0 && qA != a.p;
+        const bool vB = qB >= 0 && qB != b.p;
+        EdmdPart rA, rB;
+        if (vA) rA = A.p[qA];
+        if (vB) rB = A.p[qB];
+        if (vA) {
+            double tt = coll_time(C, t, a.x, a.y, a.vx, a.vy, a.r, rA.x, rA.y, rA.vx, rA.vy, rA.tl,
+                                  rA.rad, overlap);
+            if (tt < EDMD_NEVER && tt < btA) {
+                btA = tt;
+                brA = sA * C.cap + (f - exA);
+                bqA = qA;
+                bcA = rA.coll;
+            }
+        }
+        if (vB) {
+            double tt = coll_time(C, t, b.x, b.y, b.vx, b.vy, b.r, rB.x, rB.y, rB.vx, rB.vy, rB.tl,
+                                  rB.rad, overlap);
+            if (tt < EDMD_NEVER && tt < btB) {
+                btB = tt;
+                brB = (sB - 16) * C.cap + (f - exB);
+                bqB = qB;
+                bcB = rB.coll;
             }
         }
     }
 #pragma unroll
     for (int m = 1; m < 64; m <<= 1) {
-        double ot = __shfl_xor(bt, m, 64);
-        int orr = __shfl_xor(br, m, 64);
-        int oq = __shfl_xor(bq, m, 64);
-        uint32_t oc = (uint32_t)__shfl_xor((int)bc, m, 64);
-        if (ot < bt || (ot == bt && orr < br)) {
-            bt = ot;
-            br = orr;
-            bq = oq;
-            bc = oc;
-        }
+        double otA = __shfl_xor(btA, m, 64);
+        int orA = __shfl_xor(brA, m, 64), oqA = __shfl_xor(bqA, m, 64);
+        uint32_t ocA = (uint32_t)__shfl_xor((int)bcA, m, 64);
+        double otB = __shfl_xor(btB, m, 64);
+        int orB = __shfl_xor(brB, m, 64), oqB = __shfl_xor(bqB, m, 64);
+        uint32_t ocB = (uint32_t)__shfl_xor((int)bcB, m, 64);
+        if (otA < btA || (otA == btA && orA < brA)) { btA = otA; brA = orA; bqA = oqA; bcA = ocA; }
+        if (otB < btB || (otB == btB && orB < brB)) { btB = otB; brB = orB; bqB = oqB; bcB = ocB; }
     }
-    if (bt < EDMD_NEVER) {
-        partner = bq;
-        pcoll = bc;
-        return bt;
+    if (btA < EDMD_NEVER) {
+        dta = uni(btA); pa = uni(bqA); ca = (uint32_t)uni((int)bcA);
+    } else {
+        dta = EDMD_NEVER; pa = 0; ca = A.p[0].coll;
     }
-    partner = 0;
-    pcoll = A.coll[0];
-    return EDMD_NEVER;
+    if (btB < EDMD_NEVER) {
+        dtb = uni(btB); pb = uni(bqB); cb = (uint32_t)uni((int)bcB);
+    } else {
+        dtb = EDMD_NEVER; pb = 0; cb = A.p[0].coll;
+    }
 }
 
 /* ------------------------------------------------------------ calendar */
 
-__device__ __forceinline__ void wave_argmin(double& k, int& i) {
+/* Lexicographic minimum over the wave of (key, leaf), carrying the event
+ * code: ties go to the smaller leaf index. */
+__device__ __forceinline__ void wave_argmin3(double& k, int& i, int& x) {
 #pragma unroll
     for (int m = 1; m < 64; m <<= 1) {
         double ok = __shfl_xor(k, m, 64);
         int oi = __shfl_xor(i, m, 64);
+        int ox = __shfl_xor(x, m, 64);
         if (ok < k || (ok == k && oi < i)) {
             k = ok;
             i = oi;
+            x = ox;
         }
     }
 }
 
-/* Earliest pending particle event of the system: minimum of the top level. */
+/* Level l of the calendar: the leaves for l = 0, else nodes of level l. */
+__device__ __forceinline__ EdmdNode* level(const EdmdArrays& A, int l) {
+    return l == 0 ? A.leaf : A.lev[l - 1];
+}
+
+/* Earliest pending particle event: minimum of the top level. */
 __device__ __forceinline__ void top_min(const EdmdConst& C, const EdmdArrays& A, int lane,
-                                        double& key, int& leaf) {
+                                        double& key, int& leaf, int& aux) {
     const int top = C.nlev;
     double k = INF_D;
-    int i = 0x7fffffff;
+    int i = 0x7fffffff, x = 0;
     if (lane < C.lsize[top]) {
-        if (top == 0) {
-            k = A.key[lane];
-            i = lane;
-        } else {
-            k = A.lkey[top - 1][lane];
-            i = A.lidx[top - 1][lane];
-        }
+        const EdmdNode nd = level(A, top)[lane];
+        k = nd.key;
+        i = nd.idx;
+        x = nd.aux;
     }
-    wave_argmin(k, i);
-    key = k;
-    leaf = i;
+    wave_argmin3(k, i, x);
+    key = uni(k);
+    leaf = uni(i);
+    aux = uni(x);
 }
 
-/* Stores the new key of leaf p and recomputes the minima on its path to the
- * top.  The freshly computed child value is taken from registers, so the
- * path needs no read-after-write through memory. */
-__device__ __forceinline__ void ascend(const EdmdConst& C, const EdmdArrays& A, int lane, int p,
-                                       double kp) {
-    if (lane == 0) A.key[p] = kp;
-    double v = kp;
-    int vi = p;
-    int child = p;
-    for (int l = 1; l <= C.nlev; ++l) {
-        int g = child >> 6;
-        int c = (g << 6) + lane;
-        double k = INF_D;
-        int i = 0x7fffffff;
-        if (c < C.lsize[l - 1]) {
-            if (c == child) {
-                k = v;
-                i = vi;
-            } else if (l == 1) {
-                k = A.key[c];
-                i = c;
-            } else {
-                k = A.lkey[l - 2][c];
-                i = A.lidx[l - 2][c];
+/* The 64 children of every node on the calendar paths of up to two leaves,
+ * read in one round trip before the event changes anything.  ML bounds the
+ * number of levels above the leaves. */
+template <int ML>
+struct PathCache {
+    double k[2][ML];
+    int i[2][ML];
+    int x[2][ML];
+};
+
+template <int ML>
+__device__ __forceinline__ void path_prefetch(const EdmdConst& C, const EdmdArrays& A, int lane,
+                                              int pA, int pB, PathCache<ML>& P) {
+#pragma unroll
+    for (int l = 0; l < ML; ++l) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int p = h ? pB : pA;
+            double k = INF_D;
+            int i = 0x7fffffff, x = 0;
+            if (l < C.nlev && p >= 0) {
+                const int c = ((p >> (6 * (l + 1))) << 6) + lane;
+                if (c < C.lsize[l]) {
+                    const EdmdNode nd = level(A, l)[c];
+                    k = nd.key;
+                    i = nd.idx;
+                    x = nd.aux;
+                }
+            }
+            P.k[h][l] = k;
+            P.i[h][l] = i;
+            P.x[h][l] = x;
+        }
+    }
+}
+
+/* Stores the new leaves of pA (and pB >= 0) and recomputes every node on
+ * their paths from the prefetched children, substituting the freshly
+ * computed path nodes; where the paths meet one node takes both. */
+template <int ML>
+__device__ __forceinline__ void path_update(const EdmdConst& C, const EdmdArrays& A, int lane,
+                                            const PathCache<ML>& P, int pA, double kA, int xA,
+                                            int pB, double kB, int xB) {
+    if (lane == 0) {
+        EdmdNode n;
+        n.key = kA;
+        n.idx = pA;
+        n.aux = xA;
+        A.leaf[pA] = n;
+        if (pB >= 0) {
+            n.key = kB;
+            n.idx = pB;
+            n.aux = xB;
+            A.leaf[pB] = n;
+        }
+    }
+    int iA = pA, iB = pB;
+    int cA = pA, cB = pB;
+#pragma unroll
+    for (int l = 0; l < ML; ++l) {
+        if (l >= C.nlev) break;
+        const int gA = cA >> 6;
+        const int gB = pB >= 0 ? (cB >> 6) : -1;
+        int c = (gA << 6) + lane;
+        double k = P.k[0][l];
+        int i = P.i[0][l], x = P.x[0][l];
+        if (c == cA) { k = kA; i = iA; x = xA; }
+        else if (pB >= 0 && c == cB) { k = kB; i = iB; x = xB; }
+        wave_argmin3(k, i, x);
+        const double nkA = uni(k);
+        const int niA = uni(i), nxA = uni(x);
+        double nkB = nkA;
+        int niB = niA, nxB = nxA;
+        if (pB >= 0 && gB != gA) {
+            c = (gB << 6) + lane;
+            k = P.k[1][l];
+            i = P.i[1][l];
+            x = P.x[1][l];
+            if (c == cB) { k = kB; i = iB; x = xB; }
+            wave_argmin3(k, i, x);
+            nkB = uni(k);
+            niB = uni(i);
+            nxB = uni(x);
+        }
+        if (lane == 0) {
+            EdmdNode n;
+            n.key = nkA;
+            n.idx = niA;
+            n.aux = nxA;
+            A.lev[l][gA] = n;
+            if (pB >= 0 && gB != gA) {
+                n.key = nkB;
+                n.idx = niB;
+                n.aux = nxB;
+                A.lev[l][gB] = n;
             }
         }
-        wave_argmin(k, i);
-        v = k;
-        vi = i;
-        if (lane == 0) {
-            A.lkey[l - 1][g] = v;
-            A.lidx[l - 1][g] = vi;
-        }
-        child = g;
+        kA = nkA; iA = niA; xA = nxA;
+        kB = nkB; iB = niB; xB = nxB;
+        cA = gA;
+        cB = gB;
     }
     WAVE_SYNC();
 }
@@ -320,47 +440,41 @@ __device__ __forceinline__ void node_min(const EdmdConst& C, const EdmdArrays& A
     int lo = g << 6;
     int hi = lo + 64;
     if (hi > C.lsize[l - 1]) hi = C.lsize[l - 1];
-    double k = INF_D;
-    int idx = 0x7fffffff;
-    for (int c = lo; c < hi; ++c) {
-        double kc;
-        int ic;
-        if (l == 1) {
-            kc = A.key[c];
-            ic = c;
-        } else {
-            kc = A.lkey[l - 2][c];
-            ic = A.lidx[l - 2][c];
-        }
-        if (kc < k || idx == 0x7fffffff) {
-            k = kc;
-            idx = ic;
-        }
+    const EdmdNode* ch = level(A, l - 1);
+    EdmdNode best = ch[lo];
+    for (int c = lo + 1; c < hi; ++c) {
+        const EdmdNode nd = ch[c];
+        if (nd.key < best.key) best = nd;
     }
-    A.lkey[l - 1][g] = k;
-    A.lidx[l - 1][g] = idx;
+    A.lev[l - 1][g] = best;
 }
 
 /* ------------------------------------------------------------ sweep */
 
 /* One particle's full prediction (crossing + collision) written to its two
- * slots and its leaf key; the particle is assumed synchronised at t. */
+ * slots and its leaf; the particle is assumed synchronised at t. */
 __device__ __forceinline__ bool sweep_one(const EdmdConst& C, const EdmdArrays& A, double t, int p) {
-    double x = A.x[p], y = A.y[p], vx = A.vx[p], vy = A.vy[p], r = A.rad[p];
-    int cx = A.cx[p], cy = A.cy[p];
+    const EdmdPart o = A.p[p];
     double tc;
     int dir;
-    cross_time(C, t, x, y, vx, vy, cx, cy, tc, dir);
+    cross_time(C, t, o.x, o.y, o.vx, o.vy, o.cx, o.cy, tc, dir);
     bool ov = false;
     int partner;
-    double dt = predict_seq(C, A, t, p, x, y, vx, vy, r, cx, cy, partner, ov);
+    double dt = predict_seq(C, A, t, p, o, partner, ov);
     double tk = t + dt;
-    A.tcross[p] = tc;
-    A.dir[p] = dir;
-    A.tcoll[p] = tk;
-    A.partner[p] = partner;
-    A.cact[p] = A.coll[partner];
-    A.key[p] = leaf_key(tk, tc);
+    EdmdEv e;
+    e.tcoll = tk;
+    e.tcross = tc;
+    e.partner = partner;
+    e.cact = A.p[partner].coll;
+    e.dir = dir;
+    e.pad = 0;
+    A.e[p] = e;
+    EdmdNode n;
+    n.key = leaf_key(tk, tc);
+    n.idx = p;
+    n.aux = leaf_aux(tk, tc, partner, dir);
+    A.leaf[p] = n;
     return ov;
 }
 
@@ -387,38 +501,50 @@ __global__ __launch_bounds__(256) void k_sync_all(const EdmdSys* __restrict__ sy
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= S->c.n) return;
     const double t = S->a.st->t;
-    double x = S->a.x[p], y = S->a.y[p], tl = S->a.tl[p];
-    fly(S->c, t, x, y, S->a.vx[p], S->a.vy[p], tl);
-    S->a.x[p] = x;
-    S->a.y[p] = y;
-    S->a.tl[p] = tl;
+    EdmdPart* r = S->a.p + p;
+    double x = r->x, y = r->y, tl = r->tl;
+    fly(S->c, t, x, y, r->vx, r->vy, tl);
+    r->x = x;
+    r->y = y;
+    r->tl = tl;
 }
 
 /* ------------------------------------------------------------ event loop */
 
+/* Loop state kept in registers; the rest of EdmdState stays in memory and
+ * is touched only by the events that use it. */
 struct LoopRegs {
     double t;
-    double next_screen, next_thermo, next_noise, last_thermo;
-    double vir[4];
-    uint64_t ncol, ncross, nevents;
-    int64_t ntrace;
-    int32_t nthermo;
+    double next_special; /* earliest of the screenshot / thermo / noise times */
+    int special;         /* its kind */
+    uint64_t ncol, nevents;
     int32_t status, err_i, err_j;
 };
 
-__device__ __forceinline__ void trace_push(const EdmdConst& C, const EdmdArrays& A, LoopRegs& R,
-                                           int lane, int type, int i, int j, int valid) {
-    if (R.ntrace < C.trace_cap) {
-        if (lane == 0) {
+__device__ __forceinline__ void next_special(const EdmdState* st, LoopRegs& R) {
+    double tsp = st->next_screen;
+    int sp = EDMD_EV_SCREENSHOT;
+    if (st->next_thermo < tsp) { tsp = st->next_thermo; sp = EDMD_EV_THERMO; }
+    if (st->next_noise < tsp) { tsp = st->next_noise; sp = EDMD_EV_ADDINGNOISE; }
+    R.next_special = uni(tsp);
+    R.special = uni(sp);
+}
+
+__device__ __forceinline__ void trace_push(const EdmdConst& C, const EdmdArrays& A, int lane,
+                                           double t, int type, int i, int j, int valid) {
+    if (C.trace_cap <= 0) return;
+    if (lane == 0) {
+        const int64_t k = A.st->ntrace;
+        if (k < C.trace_cap) {
             EdmdTraceRec rec;
             rec.type = type;
             rec.i = i;
             rec.j = j;
             rec.valid = valid;
-            rec.t = R.t;
-            A.trace[R.ntrace] = rec;
+            rec.t = t;
+            A.trace[k] = rec;
         }
-        R.ntrace++;
+        A.st->ntrace = k + 1;
     }
 }
 
@@ -439,52 +565,57 @@ __device__ __forceinline__ double ordered_sum(int n, int lane, F f) {
     return acc;
 }
 
-__device__ void ev_screenshot(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R) {
+__device__ void ev_screenshot(const EdmdConst& C, const EdmdArrays& A, int lane, double t) {
     /* takeAScreenshot, EDMD.c:4657-4677: free-fly every particle */
     for (int q = lane; q < C.n; q += 64) {
-        double x = A.x[q], y = A.y[q], tl = A.tl[q];
-        fly(C, R.t, x, y, A.vx[q], A.vy[q], tl);
-        A.x[q] = x;
-        A.y[q] = y;
-        A.tl[q] = tl;
+        EdmdPart* r = A.p + q;
+        double x = r->x, y = r->y, tl = r->tl;
+        fly(C, t, x, y, r->vx, r->vy, tl);
+        r->x = x;
+        r->y = y;
+        r->tl = tl;
     }
     WAVE_SYNC();
-    R.next_screen = R.t + C.dtime;
+    if (lane == 0) A.st->next_screen = t + C.dtime;
+    WAVE_SYNC();
 }
 
-__device__ void ev_thermo(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R) {
+__device__ void ev_thermo(const EdmdConst& C, const EdmdArrays& A, int lane, double t,
+                          uint64_t ncol) {
     /* takeAThermo + saveThermo, EDMD.c:4679-4713, 5232-5333 */
-    const double* vx = A.vx;
-    const double* vy = A.vy;
-    const double* m = A.m;
+    const EdmdPart* P = A.p;
     double E = ordered_sum(C.n, lane, [&](int q) {
-        double a = vx[q];
-        return 0.5 * m[q] * (a * a + vy[q] * vy[q]);
+        double a = P[q].vx;
+        return 0.5 * P[q].m * (a * a + P[q].vy * P[q].vy);
     });
-    double ex = ordered_sum(C.n, lane, [&](int q) { return m[q] * vx[q] * vx[q]; });
-    double ey = ordered_sum(C.n, lane, [&](int q) { return m[q] * vy[q] * vy[q]; });
-    double exy = ordered_sum(C.n, lane, [&](int q) { return m[q] * vx[q] * vy[q]; });
-    double eyx = ordered_sum(C.n, lane, [&](int q) { return m[q] * vy[q] * vx[q]; });
-    double area = C.Lx * C.Ly;
-    double dT = R.t - R.last_thermo;
-    R.last_thermo = R.t;
-    if (R.nthermo < C.thermo_cap) {
-        if (lane == 0) {
+    double ex = ordered_sum(C.n, lane, [&](int q) { return P[q].m * P[q].vx * P[q].vx; });
+    double ey = ordered_sum(C.n, lane, [&](int q) { return P[q].m * P[q].vy * P[q].vy; });
+    double exy = ordered_sum(C.n, lane, [&](int q) { return P[q].m * P[q].vx * P[q].vy; });
+    double eyx = ordered_sum(C.n, lane, [&](int q) { return P[q].m * P[q].vy * P[q].vx; });
+    EdmdState* st = A.st;
+    const double area = C.Lx * C.Ly;
+    const double dT = t - st->last_thermo;
+    const double v0 = st->vir[0], v1 = st->vir[1], v2 = st->vir[2], v3 = st->vir[3];
+    const int k = st->nthermo;
+    if (lane == 0) {
+        if (k < C.thermo_cap) {
             EdmdThermoRow row;
-            row.t = R.t;
-            row.ncol = R.ncol;
+            row.t = t;
+            row.ncol = ncol;
             row.e_per_n = E / C.n;
-            row.pxx = -1 * R.vir[0] / (area * dT) + ex / area;
-            row.pyy = -1 * R.vir[1] / (area * dT) + ey / area;
-            row.pxy = -1 * R.vir[2] / (area * dT) + exy / area;
-            row.pyx = -1 * R.vir[3] / (area * dT) + eyx / area;
-            row.pressure = (-1 / (dT)) * (R.vir[0] + R.vir[1]) / (2 * area) + E / area;
-            A.thermo[R.nthermo] = row;
+            row.pxx = -1 * v0 / (area * dT) + ex / area;
+            row.pyy = -1 * v1 / (area * dT) + ey / area;
+            row.pxy = -1 * v2 / (area * dT) + exy / area;
+            row.pyx = -1 * v3 / (area * dT) + eyx / area;
+            row.pressure = (-1 / (dT)) * (v0 + v1) / (2 * area) + E / area;
+            A.thermo[k] = row;
         }
-        R.nthermo++;
+        st->nthermo = k + 1;
+        st->last_thermo = t;
+        st->vir[0] = st->vir[1] = st->vir[2] = st->vir[3] = 0;
+        st->next_thermo = t + C.dtime_thermo;
     }
-    R.vir[0] = R.vir[1] = R.vir[2] = R.vir[3] = 0;
-    R.next_thermo = R.t + C.dtime_thermo;
+    WAVE_SYNC();
 }
 
 /* Rebuilds every level of the calendar after a full re-prediction. */
@@ -498,89 +629,90 @@ __device__ void rebuild_tree(const EdmdConst& C, const EdmdArrays& A, int lane) 
 __device__ void ev_rescale(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R) {
     /* addNoise, velocity-rescale branch (noise == 2): EDMD.c:4828-4832,
      * 4880-4915, 4922 */
-    const double* vx = A.vx;
-    const double* vy = A.vy;
-    const double* m = A.m;
+    const EdmdPart* P = A.p;
+    const double t = R.t;
     double E = ordered_sum(C.n, lane, [&](int q) {
-        double a = vx[q];
-        return 0.5 * m[q] * (a * a + vy[q] * vy[q]);
+        double a = P[q].vx;
+        return 0.5 * P[q].m * (a * a + P[q].vy * P[q].vy);
     });
     for (int q = lane; q < C.n; q += 64) {
-        A.coll[q] = A.coll[q] + 1;
-        double x = A.x[q], y = A.y[q], tl = A.tl[q];
-        double a = A.vx[q], b = A.vy[q];
-        fly(C, R.t, x, y, a, b, tl);
-        a /= sqrt(E / C.n / C.temperature);
-        b /= sqrt(E / C.n / C.temperature);
-        A.x[q] = x;
-        A.y[q] = y;
-        A.tl[q] = tl;
-        A.vx[q] = a;
-        A.vy[q] = b;
+        EdmdPart r = A.p[q];
+        r.coll = r.coll + 1;
+        fly(C, t, r.x, r.y, r.vx, r.vy, r.tl);
+        r.vx /= sqrt(E / C.n / C.temperature);
+        r.vy /= sqrt(E / C.n / C.temperature);
+        A.p[q] = r;
     }
     WAVE_SYNC();
     bool ov = false;
-    for (int q = lane; q < C.n; q += 64) ov |= sweep_one(C, A, R.t, q);
+    for (int q = lane; q < C.n; q += 64) ov |= sweep_one(C, A, t, q);
     if (__ballot(ov) != 0ull) R.status = EDMD_DEV_OVERLAP;
     WAVE_SYNC();
     rebuild_tree(C, A, lane);
-    R.next_noise = R.t + C.dtnoise;
+    if (lane == 0) A.st->next_noise = t + C.dtnoise;
+    WAVE_SYNC();
 }
 
-/* Collision slot of owner p re-predicted by the whole wave and written. */
-__device__ __forceinline__ double repredict_coll(const EdmdConst& C, const EdmdArrays& A,
-                                                 int lane, LoopRegs& R, int p, double x, double y,
-                                                 double vx, double vy, double r, int cx, int cy) {
-    bool ov = false;
-    int partner;
-    uint32_t pc;
-    double dt = predict_wave(C, A, lane, R.t, p, x, y, vx, vy, r, cx, cy, partner, pc, ov);
+__device__ __forceinline__ void flag_overlap(LoopRegs& R, bool ov, int p) {
     if (__ballot(ov) != 0ull && R.status == EDMD_DEV_OK) {
         R.status = EDMD_DEV_OVERLAP;
         R.err_i = p;
     }
-    double tk = R.t + dt;
-    if (lane == 0) {
-        A.tcoll[p] = tk;
-        A.partner[p] = partner;
-        A.cact[p] = pc;
-    }
-    return tk;
 }
 
-__device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R, int i) {
+template <int ML>
+__device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R, int i,
+                             int j) {
     /* doTheCollisionNormal, EDMD.c:3542-3562, 3583-3606, 3802-3827, 3869-3879 */
-    const int j = A.partner[i];
-    const uint32_t ca = A.cact[i];
-    double xi = A.x[i], yi = A.y[i], vxi = A.vx[i], vyi = A.vy[i], tli = A.tl[i];
-    double ri = A.rad[i], mi = A.m[i];
-    int cxi = A.cx[i], cyi = A.cy[i];
-    uint32_t colli = A.coll[i];
-    double tci = A.tcross[i];
-    double xj = A.x[j], yj = A.y[j], vxj = A.vx[j], vyj = A.vy[j], tlj = A.tl[j];
-    double rj = A.rad[j], mj = A.m[j];
-    int cxj = A.cx[j], cyj = A.cy[j];
-    uint32_t collj = A.coll[j];
+    const EdmdEv ei = A.e[i];
+    const EdmdPart pi = A.p[i];
+    const EdmdPart pj = A.p[j];
+    PathCache<ML> P;
+    path_prefetch<ML>(C, A, lane, i, j, P);
+    const double t = R.t;
+    double xi = uni(pi.x), yi = uni(pi.y), vxi = uni(pi.vx), vyi = uni(pi.vy), tli = uni(pi.tl);
+    const double ri = uni(pi.rad), mi = uni(pi.m);
+    const int cxi = uni((int)pi.cx), cyi = uni((int)pi.cy);
+    const uint32_t colli = (uint32_t)uni((int)pi.coll);
+    const uint32_t collj = (uint32_t)uni((int)pj.coll);
+    const uint32_t ca = (uint32_t)uni((int)ei.cact);
 
-    fly(C, R.t, xi, yi, vxi, vyi, tli);
+    fly(C, t, xi, yi, vxi, vyi, tli);
+    xi = uni(xi);
+    yi = uni(yi);
     const bool valid = (ca == collj);
-    trace_push(C, A, R, lane, EDMD_EV_COLLISION, i, j, valid ? 1 : 0);
+    trace_push(C, A, lane, t, EDMD_EV_COLLISION, i, j, valid ? 1 : 0);
     if (!valid) {
         /* stale prediction: free flight of i and a new collision slot */
         if (lane == 0) {
-            A.x[i] = xi;
-            A.y[i] = yi;
-            A.tl[i] = tli;
+            A.p[i].x = xi;
+            A.p[i].y = yi;
+            A.p[i].tl = t;
         }
         WAVE_SYNC();
-        double tk = repredict_coll(C, A, lane, R, i, xi, yi, vxi, vyi, ri, cxi, cyi);
-        ascend(C, A, lane, i, leaf_key(tk, tci));
+        const Owner oi{i, xi, yi, vxi, vyi, ri, cxi, cyi};
+        bool ov = false;
+        double dt, dtb;
+        int pa, pb;
+        uint32_t cpa, cpb;
+        predict_owners(C, A, lane, t, 1, oi, oi, dt, pa, cpa, dtb, pb, cpb, ov);
+        flag_overlap(R, ov, i);
+        const double tk = t + dt;
+        const double tci = uni(ei.tcross);
+        const int diri = uni(ei.dir);
+        if (lane == 0) {
+            A.e[i].tcoll = tk;
+            A.e[i].partner = pa;
+            A.e[i].cact = cpa;
+        }
+        path_update<ML>(C, A, lane, P, i, leaf_key(tk, tci), leaf_aux(tk, tci, pa, diri), -1, 0, 0);
         return;
     }
     R.ncol++;
-    fly(C, R.t, xj, yj, vxj, vyj, tlj);
-    colli++;
-    collj++;
+    double xj = uni(pj.x), yj = uni(pj.y), vxj = uni(pj.vx), vyj = uni(pj.vy), tlj = uni(pj.tl);
+    const double rj = uni(pj.rad), mj = uni(pj.m);
+    const int cxj = uni((int)pj.cx), cyj = uni((int)pj.cy);
+    fly(C, t, xj, yj, vxj, vyj, tlj);
     double dx = xj - xi;
     double dy = yj - yi;
     double dvx = vxj - vxi;
@@ -590,45 +722,63 @@ __device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, 
     double kernel = inv_mass * (1 + C.res) * (dx * dvx + dy * dvy);
     double ff = kernel / (4 * ri * rj);
     double w = mi * mj * ff;
-    R.vir[0] += w * dx * dx;
-    R.vir[1] += w * dy * dy;
-    R.vir[2] += w * dx * dy;
-    R.vir[3] += w * dy * dx;
-    vxi += ff * mj * dx;
-    vyi += ff * mj * dy;
-    vxj -= ff * mi * dx;
-    vyj -= ff * mi * dy;
+    vxi = uni(vxi + ff * mj * dx);
+    vyi = uni(vyi + ff * mj * dy);
+    vxj = uni(vxj - ff * mi * dx);
+    vyj = uni(vyj - ff * mi * dy);
+    xj = uni(xj);
+    yj = uni(yj);
     if (lane == 0) {
-        A.x[i] = xi; A.y[i] = yi; A.vx[i] = vxi; A.vy[i] = vyi; A.tl[i] = tli; A.coll[i] = colli;
-        A.x[j] = xj; A.y[j] = yj; A.vx[j] = vxj; A.vy[j] = vyj; A.tl[j] = tlj; A.coll[j] = collj;
+        EdmdState* st = A.st;
+        st->vir[0] += w * dx * dx;
+        st->vir[1] += w * dy * dy;
+        st->vir[2] += w * dx * dy;
+        st->vir[3] += w * dy * dx;
+        EdmdPart* q = A.p + i;
+        q->x = xi; q->y = yi; q->vx = vxi; q->vy = vyi; q->tl = t; q->coll = colli + 1;
+        q = A.p + j;
+        q->x = xj; q->y = yj; q->vx = vxj; q->vy = vyj; q->tl = t; q->coll = collj + 1;
     }
     WAVE_SYNC();
+    const Owner oi{i, xi, yi, vxi, vyi, ri, cxi, cyi};
+    const Owner oj{j, xj, yj, vxj, vyj, rj, cxj, cyj};
+    bool ov = false;
+    double dti, dtj;
+    int pi_, pj_;
+    uint32_t cpi, cpj;
+    predict_owners(C, A, lane, t, 2, oi, oj, dti, pi_, cpi, dtj, pj_, cpj, ov);
+    flag_overlap(R, ov, i);
     double tci2, tcj2;
     int di, dj;
-    cross_time(C, R.t, xi, yi, vxi, vyi, cxi, cyi, tci2, di);
-    cross_time(C, R.t, xj, yj, vxj, vyj, cxj, cyj, tcj2, dj);
+    cross_time(C, t, xi, yi, vxi, vyi, cxi, cyi, tci2, di);
+    cross_time(C, t, xj, yj, vxj, vyj, cxj, cyj, tcj2, dj);
+    const double tki = t + dti, tkj = t + dtj;
     if (lane == 0) {
-        A.tcross[i] = tci2;
-        A.dir[i] = di;
-        A.tcross[j] = tcj2;
-        A.dir[j] = dj;
+        EdmdEv e;
+        e.tcoll = tki; e.tcross = tci2; e.partner = pi_; e.cact = cpi; e.dir = di; e.pad = 0;
+        A.e[i] = e;
+        e.tcoll = tkj; e.tcross = tcj2; e.partner = pj_; e.cact = cpj; e.dir = dj;
+        A.e[j] = e;
     }
-    double tki = repredict_coll(C, A, lane, R, i, xi, yi, vxi, vyi, ri, cxi, cyi);
-    double tkj = repredict_coll(C, A, lane, R, j, xj, yj, vxj, vyj, rj, cxj, cyj);
-    WAVE_SYNC();
-    ascend(C, A, lane, i, leaf_key(tki, tci2));
-    ascend(C, A, lane, j, leaf_key(tkj, tcj2));
+    path_update<ML>(C, A, lane, P, i, leaf_key(tki, tci2), leaf_aux(tki, tci2, pi_, di), j,
+                    leaf_key(tkj, tcj2), leaf_aux(tkj, tcj2, pj_, dj));
 }
 
-__device__ void ev_crossing(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R, int p) {
+template <int ML>
+__device__ void ev_crossing(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R, int p,
+                            int d) {
     /* doTheCrossing, EDMD.c:2501-2590 */
-    const int d = A.dir[p];
-    double x = A.x[p], y = A.y[p], vx = A.vx[p], vy = A.vy[p], tl = A.tl[p], r = A.rad[p];
-    int cx = A.cx[p], cy = A.cy[p];
-    int crx = A.crx[p], cry = A.cry[p];
-    trace_push(C, A, R, lane, EDMD_EV_CELLCROSS, p, d, 1);
-    R.ncross++;
-    fly(C, R.t, x, y, vx, vy, tl);
+    const EdmdPart pp = A.p[p];
+    PathCache<ML> P;
+    path_prefetch<ML>(C, A, lane, p, -1, P);
+    const double t = R.t;
+    double x = uni(pp.x), y = uni(pp.y), tl = uni(pp.tl);
+    const double vx = uni(pp.vx), vy = uni(pp.vy), r = uni(pp.rad);
+    int cx = uni((int)pp.cx), cy = uni((int)pp.cy);
+    trace_push(C, A, lane, t, EDMD_EV_CELLCROSS, p, d, 1);
+    fly(C, t, x, y, vx, vy, tl);
+    x = uni(x);
+    y = uni(y);
 
     /* removeFromCell: drop p, keeping the order of the others */
     const int c0 = cy * C.ncx + cx;
@@ -639,18 +789,19 @@ __device__ void ev_crossing(const EdmdConst& C, const EdmdArrays& A, int lane, L
     if (lane > pos && lane < n0) A.cslot[(size_t)c0 * C.cap + lane - 1] = v;
     if (lane == 0) A.ccnt[c0] = n0 - 1;
 
+    int wx = 0, wy = 0; /* periodic image change */
     switch (d) {
         case 1:
-            if (cx - 1 == -1) { cx = C.ncx - 1; crx -= 1; } else cx -= 1;
+            if (cx - 1 == -1) { cx = C.ncx - 1; wx = -1; } else cx -= 1;
             break;
         case 2:
-            if (cx + 1 == C.ncx) { cx = 0; crx += 1; } else cx += 1;
+            if (cx + 1 == C.ncx) { cx = 0; wx = 1; } else cx += 1;
             break;
         case 3:
-            if (cy - 1 == -1) { cy = C.ncy - 1; cry -= 1; } else cy -= 1;
+            if (cy - 1 == -1) { cy = C.ncy - 1; wy = -1; } else cy -= 1;
             break;
         default:
-            if (cy + 1 == C.ncy) { cy = 0; cry += 1; } else cy += 1;
+            if (cy + 1 == C.ncy) { cy = 0; wy = 1; } else cy += 1;
             break;
     }
     WAVE_SYNC();
@@ -667,43 +818,52 @@ __device__ void ev_crossing(const EdmdConst& C, const EdmdArrays& A, int lane, L
     if (lane == 0) {
         A.cslot[(size_t)c1 * C.cap + n1] = p; /* newest entry: visited first */
         A.ccnt[c1] = n1 + 1;
-        A.x[p] = x; A.y[p] = y; A.tl[p] = tl;
-        A.cx[p] = cx; A.cy[p] = cy; A.crx[p] = crx; A.cry[p] = cry;
+        EdmdPart* q = A.p + p;
+        q->x = x;
+        q->y = y;
+        q->tl = tl;
+        q->cx = (int16_t)cx;
+        q->cy = (int16_t)cy;
+        if (wx) A.img[2 * p] += wx;
+        if (wy) A.img[2 * p + 1] += wy;
     }
     WAVE_SYNC();
+    const Owner o{p, x, y, vx, vy, r, cx, cy};
+    bool ov = false;
+    double dt, dtb;
+    int pa, pb;
+    uint32_t cpa, cpb;
+    predict_owners(C, A, lane, t, 1, o, o, dt, pa, cpa, dtb, pb, cpb, ov);
+    flag_overlap(R, ov, p);
     double tc;
     int dir;
-    cross_time(C, R.t, x, y, vx, vy, cx, cy, tc, dir);
+    cross_time(C, t, x, y, vx, vy, cx, cy, tc, dir);
+    const double tk = t + dt;
     if (lane == 0) {
-        A.tcross[p] = tc;
-        A.dir[p] = dir;
+        EdmdEv e;
+        e.tcoll = tk; e.tcross = tc; e.partner = pa; e.cact = cpa; e.dir = dir; e.pad = 0;
+        A.e[p] = e;
     }
-    double tk = repredict_coll(C, A, lane, R, p, x, y, vx, vy, r, cx, cy);
-    WAVE_SYNC();
-    ascend(C, A, lane, p, leaf_key(tk, tc));
+    path_update<ML>(C, A, lane, P, p, leaf_key(tk, tc), leaf_aux(tk, tc, pa, dir), -1, 0, 0);
 }
 
-/* Persistent event loop, one wavefront per system (blockIdx.x). */
+/* Persistent event loop, one wavefront per system (blockIdx.x); ML bounds
+ * the calendar depth of every system in the launch. */
+template <int ML>
 __global__ __launch_bounds__(64) void k_run(const EdmdSys* __restrict__ systems, EdmdRunArgs ra) {
     const EdmdConst& C = systems[blockIdx.x].c;
     const EdmdArrays& A = systems[blockIdx.x].a;
     const int lane = threadIdx.x;
     EdmdState* st = A.st;
     LoopRegs R;
-    R.t = st->t;
-    R.next_screen = st->next_screen;
-    R.next_thermo = st->next_thermo;
-    R.next_noise = st->next_noise;
-    R.last_thermo = st->last_thermo;
-    for (int k = 0; k < 4; ++k) R.vir[k] = st->vir[k];
+    R.t = uni(st->t);
     R.ncol = st->ncol;
-    R.ncross = st->ncross;
     R.nevents = st->nevents;
-    R.ntrace = st->ntrace;
-    R.nthermo = st->nthermo;
     R.status = st->status;
     R.err_i = st->err_i;
     R.err_j = st->err_j;
+    next_special(st, R);
+    uint64_t ncross = st->ncross;
     int reason = EDMD_STOP_NONE;
     int last_type = st->last_type;
 
@@ -713,32 +873,29 @@ __global__ __launch_bounds__(64) void k_run(const EdmdSys* __restrict__ systems,
         if (R.ncol >= ra.max_coll) { reason = EDMD_STOP_COLLISIONS; break; }
         if (!(R.t <= ra.tmax)) { reason = EDMD_STOP_TIME; break; }
         double key;
-        int p;
-        top_min(C, A, lane, key, p);
-        double tsp = R.next_screen;
-        int sp = EDMD_EV_SCREENSHOT;
-        if (R.next_thermo < tsp) { tsp = R.next_thermo; sp = EDMD_EV_THERMO; }
-        if (R.next_noise < tsp) { tsp = R.next_noise; sp = EDMD_EV_ADDINGNOISE; }
-        if (!(key < tsp)) {
-            if (!(tsp < INF_D)) { reason = EDMD_STOP_EMPTY; break; }
-            R.t = tsp;
-            trace_push(C, A, R, lane, sp, 0, 0, 1);
-            if (sp == EDMD_EV_SCREENSHOT) ev_screenshot(C, A, lane, R);
-            else if (sp == EDMD_EV_THERMO) ev_thermo(C, A, lane, R);
+        int p, aux;
+        top_min(C, A, lane, key, p, aux);
+        if (!(key < R.next_special)) {
+            if (!(R.next_special < INF_D)) { reason = EDMD_STOP_EMPTY; break; }
+            R.t = R.next_special;
+            const int sp = R.special;
+            trace_push(C, A, lane, R.t, sp, 0, 0, 1);
+            if (sp == EDMD_EV_SCREENSHOT) ev_screenshot(C, A, lane, R.t);
+            else if (sp == EDMD_EV_THERMO) ev_thermo(C, A, lane, R.t, R.ncol);
             else ev_rescale(C, A, lane, R);
+            next_special(st, R);
             R.nevents++;
             last_type = sp;
             if (ra.stop_mask & (1 << sp)) { reason = EDMD_STOP_SPECIAL; break; }
             continue;
         }
         R.t = key;
-        const double tk = A.tcoll[p];
-        const double tc = A.tcross[p];
-        if (tk < tc) {
-            ev_collision(C, A, lane, R, p);
+        if (aux >= 0) {
+            ev_collision<ML>(C, A, lane, R, p, aux);
             last_type = EDMD_EV_COLLISION;
         } else {
-            ev_crossing(C, A, lane, R, p);
+            ev_crossing<ML>(C, A, lane, R, p, -aux);
+            ncross++;
             last_type = EDMD_EV_CELLCROSS;
         }
         R.nevents++;
@@ -747,16 +904,9 @@ __global__ __launch_bounds__(64) void k_run(const EdmdSys* __restrict__ systems,
 
     if (lane == 0) {
         st->t = R.t;
-        st->next_screen = R.next_screen;
-        st->next_thermo = R.next_thermo;
-        st->next_noise = R.next_noise;
-        st->last_thermo = R.last_thermo;
-        for (int k = 0; k < 4; ++k) st->vir[k] = R.vir[k];
         st->ncol = R.ncol;
-        st->ncross = R.ncross;
+        st->ncross = ncross;
         st->nevents = R.nevents;
-        st->ntrace = R.ntrace;
-        st->nthermo = R.nthermo;
         st->status = R.status;
         st->err_i = R.err_i;
         st->err_j = R.err_j;
@@ -773,22 +923,19 @@ __global__ __launch_bounds__(64) void k_peek(const EdmdSys* __restrict__ systems
     const EdmdConst& C = systems[blockIdx.x].c;
     const EdmdArrays& A = systems[blockIdx.x].a;
     const int lane = threadIdx.x;
-    const EdmdState* st = A.st;
     double key;
-    int p;
-    top_min(C, A, lane, key, p);
-    double tsp = st->next_screen;
-    int sp = EDMD_EV_SCREENSHOT;
-    if (st->next_thermo < tsp) { tsp = st->next_thermo; sp = EDMD_EV_THERMO; }
-    if (st->next_noise < tsp) { tsp = st->next_noise; sp = EDMD_EV_ADDINGNOISE; }
+    int p, aux;
+    top_min(C, A, lane, key, p, aux);
+    LoopRegs R;
+    next_special(A.st, R);
     if (lane != 0) return;
     EdmdPeek r;
-    if (!(key < tsp)) {
-        r.t = tsp; r.type = sp; r.i = 0; r.j = 0; r.coll_actual = 0;
-    } else if (A.tcoll[p] < A.tcross[p]) {
-        r.t = key; r.type = EDMD_EV_COLLISION; r.i = p; r.j = A.partner[p]; r.coll_actual = A.cact[p];
+    if (!(key < R.next_special)) {
+        r.t = R.next_special; r.type = R.special; r.i = 0; r.j = 0; r.coll_actual = 0;
+    } else if (aux >= 0) {
+        r.t = key; r.type = EDMD_EV_COLLISION; r.i = p; r.j = aux; r.coll_actual = A.e[p].cact;
     } else {
-        r.t = key; r.type = EDMD_EV_CELLCROSS; r.i = p; r.j = A.dir[p]; r.coll_actual = 0;
+        r.t = key; r.type = EDMD_EV_CELLCROSS; r.i = p; r.j = -aux; r.coll_actual = 0;
     }
     out[blockIdx.x] = r;
 }
@@ -799,38 +946,41 @@ __global__ void k_pair_time(const EdmdSys* __restrict__ systems, int i, int j, d
     const EdmdConst& C = systems[0].c;
     const EdmdArrays& A = systems[0].a;
     bool ov = false;
-    double t = A.st->t;
-    out[0] = coll_time(C, t, A.x[i], A.y[i], A.vx[i], A.vy[i], A.rad[i], A.x[j], A.y[j], A.vx[j],
-                       A.vy[j], A.tl[j], A.rad[j], ov);
+    const double t = A.st->t;
+    const EdmdPart a = A.p[i], b = A.p[j];
+    out[0] = coll_time(C, t, a.x, a.y, a.vx, a.vy, a.rad, b.x, b.y, b.vx, b.vy, b.tl, b.rad, ov);
     out[1] = ov ? 1.0 : 0.0;
 }
 
 /* collisionEvent(i) / crossingEvent(i) for one particle (which: bit 0 =
  * crossing, bit 1 = collision) at the system time, then the calendar path
  * of i is updated. */
-__global__ __launch_bounds__(64) void k_repredict(const EdmdSys* __restrict__ systems, int i, int which) {
+template <int ML>
+__global__ __launch_bounds__(64) void k_repredict(const EdmdSys* __restrict__ systems, int i,
+                                                 int which) {
     const EdmdConst& C = systems[0].c;
     const EdmdArrays& A = systems[0].a;
     const int lane = threadIdx.x;
-    LoopRegs R;
-    R.t = A.st->t;
-    R.status = EDMD_DEV_OK;
-    double x = A.x[i], y = A.y[i], vx = A.vx[i], vy = A.vy[i], r = A.rad[i];
-    int cx = A.cx[i], cy = A.cy[i];
-    double tc = A.tcross[i], tk = A.tcoll[i];
-    if (which & 1) {
-        int dir;
-        cross_time(C, R.t, x, y, vx, vy, cx, cy, tc, dir);
-        if (lane == 0) {
-            A.tcross[i] = tc;
-            A.dir[i] = dir;
-        }
+    const double t = A.st->t;
+    const EdmdPart pi = A.p[i];
+    EdmdEv e = A.e[i];
+    PathCache<ML> P;
+    path_prefetch<ML>(C, A, lane, i, -1, P);
+    if (which & 1) cross_time(C, t, pi.x, pi.y, pi.vx, pi.vy, pi.cx, pi.cy, e.tcross, e.dir);
+    bool ov = false;
+    if (which & 2) {
+        const Owner o{i, pi.x, pi.y, pi.vx, pi.vy, pi.rad, pi.cx, pi.cy};
+        double dt, dtb;
+        int pb;
+        uint32_t cpb;
+        predict_owners(C, A, lane, t, 1, o, o, dt, e.partner, e.cact, dtb, pb, cpb, ov);
+        e.tcoll = t + dt;
     }
-    if (which & 2) tk = repredict_coll(C, A, lane, R, i, x, y, vx, vy, r, cx, cy);
-    WAVE_SYNC();
-    ascend(C, A, lane, i, leaf_key(tk, tc));
-    if (lane == 0 && R.status != EDMD_DEV_OK) {
-        A.st->status = R.status;
+    if (lane == 0) A.e[i] = e;
+    path_update<ML>(C, A, lane, P, i, leaf_key(e.tcoll, e.tcross),
+                    leaf_aux(e.tcoll, e.tcross, e.partner, e.dir), -1, 0, 0);
+    if (lane == 0 && __ballot(ov) != 0ull) {
+        A.st->status = EDMD_DEV_OVERLAP;
         A.st->err_i = i;
     }
 }
@@ -839,11 +989,12 @@ __global__ __launch_bounds__(64) void k_repredict(const EdmdSys* __restrict__ sy
 __global__ void k_fly_one(const EdmdSys* __restrict__ systems, int i) {
     const EdmdConst& C = systems[0].c;
     const EdmdArrays& A = systems[0].a;
-    double x = A.x[i], y = A.y[i], tl = A.tl[i];
-    fly(C, A.st->t, x, y, A.vx[i], A.vy[i], tl);
-    A.x[i] = x;
-    A.y[i] = y;
-    A.tl[i] = tl;
+    EdmdPart* r = A.p + i;
+    double x = r->x, y = r->y, tl = r->tl;
+    fly(C, A.st->t, x, y, r->vx, r->vy, tl);
+    r->x = x;
+    r->y = y;
+    r->tl = tl;
 }
 
 __global__ void k_fp64_check(int n, const double* __restrict__ a, const double* __restrict__ b,
@@ -880,8 +1031,12 @@ hipError_t edmd_launch_sync_all(const EdmdSys* d_sys, int nsys, int max_n, hipSt
     return hipGetLastError();
 }
 
-hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, EdmdRunArgs ra, hipStream_t s) {
-    hipLaunchKernelGGL(k_run, dim3(nsys), dim3(64), 0, s, d_sys, ra);
+hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, EdmdRunArgs ra,
+                           hipStream_t s) {
+    if (max_nlev <= 1) hipLaunchKernelGGL(k_run<1>, dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    else if (max_nlev == 2) hipLaunchKernelGGL(k_run<2>, dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    else if (max_nlev <= 4) hipLaunchKernelGGL(k_run<4>, dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    else hipLaunchKernelGGL(k_run<EDMD_MAX_LEVELS>, dim3(nsys), dim3(64), 0, s, d_sys, ra);
     return hipGetLastError();
 }
 
@@ -895,8 +1050,12 @@ hipError_t edmd_launch_pair_time(const EdmdSys* d_sys, int i, int j, double* out
     return hipGetLastError();
 }
 
-hipError_t edmd_launch_repredict(const EdmdSys* d_sys, int i, int which, hipStream_t s) {
-    hipLaunchKernelGGL(k_repredict, dim3(1), dim3(64), 0, s, d_sys, i, which);
+hipError_t edmd_launch_repredict(const EdmdSys* d_sys, int max_nlev, int i, int which,
+                                 hipStream_t s) {
+    if (max_nlev <= 1) hipLaunchKernelGGL(k_repredict<1>, dim3(1), dim3(64), 0, s, d_sys, i, which);
+    else if (max_nlev <= 4)
+        hipLaunchKernelGGL(k_repredict<4>, dim3(1), dim3(64), 0, s, d_sys, i, which);
+    else hipLaunchKernelGGL(k_repredict<EDMD_MAX_LEVELS>, dim3(1), dim3(64), 0, s, d_sys, i, which);
     return hipGetLastError();
 }
 

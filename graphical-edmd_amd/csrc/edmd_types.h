@@ -2,21 +2,25 @@
  * Device-resident layout of one hard-disk system, shared by host and device
  * code of libedmd_hip.so.
  *
- * A system is stored as structure-of-arrays in HBM:
- *   particles    x y vx vy t_local rad m (f64), coll (u32), cell x/y and
- *                periodic-image counters (i32)
- *   event slots  per particle: collision (t, partner, partner's collision
- *                counter at prediction) and cell crossing (t, direction),
+ * A system is stored in HBM as three record arrays indexed by particle:
+ *   particle     64 B: x y vx vy t_local rad m (f64), collision counter
+ *                (u32), cell x / y (i16) -- one cache-line half per disk,
+ *                gathered whole when the disk is a collision candidate
+ *   event        32 B: collision slot (time, partner, partner's collision
+ *                counter at prediction) and crossing slot (time, direction),
  *                the two slots the reference keeps per particle in
  *                eventList[N+i] and eventList[i] (EDMD.c:1935-1948)
- *   calendar     leaf key per particle = earliest of its two slots, plus a
- *                64-ary tournament tree above the leaves (one level per
- *                factor of 64); every node keeps the minimum key of its 64
- *                children and the tagged leaf that holds it
- *   cell lists   per cell a count and `cap` slots holding particle indices in
- *                insertion order; the newest entry is visited first, which
- *                is the order of the reference's head-inserted linked lists
- *                (EDMD.c:2053-2095, 2576-2585)
+ *   calendar     16 B nodes {key, leaf, event code}: one leaf per particle
+ *                (key = earliest of its two slots) and a 64-ary tournament
+ *                tree above the leaves; every node keeps the minimum key of
+ *                its 64 children, the leaf holding it and that leaf's event
+ *                code (partner index for a collision, minus the direction
+ *                for a crossing), so one read of the top level names both
+ *                disks of the next event
+ * plus cell lists: per cell a count and `cap` slots holding particle indices
+ * in insertion order; the newest entry is visited first, which is the order
+ * of the reference's head-inserted linked lists (EDMD.c:2053-2095,
+ * 2576-2585), and periodic image counters per particle.
  */
 #ifndef EDMD_TYPES_H
 #define EDMD_TYPES_H
@@ -78,18 +82,31 @@ typedef struct EdmdThermoRow {
     double e_per_n, pressure, pxx, pyy, pxy, pyx;
 } EdmdThermoRow;
 
+typedef struct __attribute__((aligned(64))) EdmdPart {
+    double x, y, vx, vy, tl, rad, m;
+    uint32_t coll;
+    int16_t cx, cy;
+} EdmdPart;
+
+typedef struct __attribute__((aligned(32))) EdmdEv {
+    double tcoll, tcross;
+    int32_t partner;
+    uint32_t cact;
+    int32_t dir, pad;
+} EdmdEv;
+
+typedef struct __attribute__((aligned(16))) EdmdNode {
+    double key;  /* sanitised: NaN -> +inf */
+    int32_t idx; /* leaf holding the key */
+    int32_t aux; /* that leaf's event: partner (>= 0) or -direction */
+} EdmdNode;
+
 typedef struct EdmdArrays {
-    double *x, *y, *vx, *vy, *tl, *rad, *m;
-    uint32_t* coll;
-    int32_t *cx, *cy, *crx, *cry;
-    double* tcoll;
-    int32_t* partner;
-    uint32_t* cact;
-    double* tcross;
-    int32_t* dir;
-    double* key;                         /* leaf keys, sanitised (NaN -> +inf) */
-    double* lkey[EDMD_MAX_LEVELS];       /* level l+1 node keys */
-    int32_t* lidx[EDMD_MAX_LEVELS];      /* level l+1 tagged leaves: 2*p + is_collision */
+    EdmdPart* p;
+    EdmdEv* e;
+    EdmdNode* leaf;
+    EdmdNode* lev[EDMD_MAX_LEVELS]; /* lev[l] = level l+1 nodes */
+    int32_t* img;                   /* periodic image counters, 2 per particle */
     int32_t* ccnt;
     int32_t* cslot;
     EdmdState* st;
