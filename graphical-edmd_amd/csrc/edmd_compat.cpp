@@ -125,6 +125,8 @@ std::string versioned(const std::string& base, const char* ext, int* version) {
 
 }  // namespace
 
+extern "C" void saveTXT(void);
+
 extern "C" int edmd_compat_status(void) { return g_status; }
 extern "C" edmd_ctx* edmd_compat_context(void) { return g_ctx; }
 
@@ -331,7 +333,12 @@ extern "C" void removeEventFromQueue(node* toRemove) {
 
 extern "C" void doTheCollisionNormal(void) { step_one(EDMD_EV_COLLISION, "doTheCollisionNormal"); }
 extern "C" void doTheCrossing(void) { step_one(EDMD_EV_CELLCROSS, "doTheCrossing"); }
-extern "C" void takeAScreenshot(void) { step_one(EDMD_EV_SCREENSHOT, "takeAScreenshot"); }
+extern "C" void takeAScreenshot(void) {
+    /* free flight of every disk, next screenshot, then the dump frame
+     * (takeAScreenshot, EDMD.c:4657-4677) */
+    step_one(EDMD_EV_SCREENSHOT, "takeAScreenshot");
+    if (g_status == EDMD_OK) saveTXT();
+}
 extern "C" void takeAThermo(void) { step_one(EDMD_EV_THERMO, "takeAThermo"); }
 extern "C" void addNoise(void) { step_one(EDMD_EV_ADDINGNOISE, "addNoise"); }
 
