@@ -836,3 +836,73 @@ extern "C" int edmd_init_lattice(int n, double phi, double rad, int lattice, uin
 
 /* error hook for the other translation units of the library */
 int edmd_internal_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+
+/* ------------------------------------------------------------ g(r) */
+
+/* Histogram of one system or of host arrays, normalised as calculate_pcf
+ * (pcf.c:16-75): g = 2 count / (2 pi r dr * (N / (Lx Ly)) * N). */
+static int pcf_common(const double* hx, const double* hy, const EdmdPart* dpart, int n, double t,
+                      int extrap, double Lx, double Ly, double dr, double max_r, double* r,
+                      double* g, int* nbins, hipStream_t stream) {
+    if (n < 1 || !(dr > 0) || !(max_r > 0) || !r || !g || !nbins)
+        return fail(EDMD_ERR_ARG, "bad g(r) arguments");
+    const int nb = (int)(max_r / dr);
+    if (nb < 1 || nb > 32768) return fail(EDMD_ERR_ARG, "g(r) needs 1..32768 bins, got %d", nb);
+    if (*nbins < nb) {
+        *nbins = nb;
+        return fail(EDMD_ERR_ARG, "g(r) output arrays need %d entries", nb);
+    }
+    double *dx = nullptr, *dy = nullptr;
+    unsigned long long* dh = nullptr;
+    HIPCHK(hipMalloc(&dh, (size_t)nb * sizeof(unsigned long long)));
+    hipError_t e = hipMemsetAsync(dh, 0, (size_t)nb * sizeof(unsigned long long), stream);
+    if (e == hipSuccess && hx) {
+        e = hipMalloc(&dx, (size_t)n * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&dy, (size_t)n * sizeof(double));
+        if (e == hipSuccess) e = hipMemcpy(dx, hx, (size_t)n * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dy, hy, (size_t)n * sizeof(double), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess)
+        e = edmd_launch_pcf(dx, dy, hx ? nullptr : dpart, n, t, extrap, Lx, Ly, dr, nb, dh, stream);
+    std::vector<unsigned long long> cnt(nb);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess)
+        e = hipMemcpy(cnt.data(), dh, (size_t)nb * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+    (void)hipFree(dh);
+    if (dx) (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    if (e != hipSuccess) return fail(EDMD_ERR_HIP, "g(r) failed: %s", hipGetErrorString(e));
+    const double pi = 3.14159265358979323846;
+    const double volume = Lx * Ly;
+    const double density = n / volume;
+    for (int b = 0; b < nb; b++) {
+        r[b] = (b + 0.5) * dr;
+        const double shell = 2 * pi * r[b] * dr;
+        const double norm = shell * density * n;
+        const double c = 2.0 * (double)cnt[b];
+        g[b] = norm > 0 ? c / norm : 0.0;
+    }
+    *nbins = nb;
+    return EDMD_OK;
+}
+
+extern "C" int edmd_pcf(int n, const double* x, const double* y, double Lx, double Ly, double dr,
+                        double max_r, double* r, double* g, int* nbins) {
+    if (!x || !y) return fail(EDMD_ERR_ARG, "null positions");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(EDMD_ERR_NODEVICE, "no HIP device visible");
+    return pcf_common(x, y, nullptr, n, 0, 0, Lx, Ly, dr, max_r, r, g, nbins, nullptr);
+}
+
+extern "C" int edmd_batch_pcf(edmd_batch* b, int s, int extrapolate, double dr, double max_r,
+                              double* r, double* g, int* nbins) {
+    int rc = sys_check(b, s);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const HostSys& h = b->hs[s];
+    EdmdState st;
+    HIPCHK(hipMemcpy(&st, b->st + s, sizeof st, hipMemcpyDeviceToHost));
+    return pcf_common(nullptr, nullptr, b->part + h.p0, h.c.n, st.t, extrapolate, h.c.Lx, h.c.Ly,
+                      dr, max_r, r, g, nbins, b->stream);
+}

@@ -18,6 +18,9 @@ hipError_t edmd_launch_pair_time(const EdmdSys* d_sys, int i, int j, double* out
 hipError_t edmd_launch_repredict(const EdmdSys* d_sys, int max_nlev, int i, int which,
                                  hipStream_t s);
 hipError_t edmd_launch_fly_one(const EdmdSys* d_sys, int i, hipStream_t s);
+hipError_t edmd_launch_pcf(const double* x, const double* y, const EdmdPart* part, int n, double t,
+                           int extrap, double Lx, double Ly, double dr, int nb,
+                           unsigned long long* hist, hipStream_t s);
 hipError_t edmd_launch_fp64_check(int n, const double* a, const double* b, double* q, double* sq,
                                   hipStream_t s);
 

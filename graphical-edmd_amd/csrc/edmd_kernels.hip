@@ -997,6 +997,69 @@ __global__ void k_fly_one(const EdmdSys* __restrict__ systems, int i) {
     r->tl = tl;
 }
 
+/* calculate_pcf, pcf.c:16-75: histogram of minimum-image pair distances
+ * below rmax = dr * nb, one count per unordered pair (the reference adds
+ * 2.0 per pair; the host doubles the counts).  Blocks take 256 x 256 tiles
+ * of the upper pair triangle and count into an LDS histogram.  extrap != 0
+ * uses positions free-flown to time t without storing them. */
+__global__ __launch_bounds__(256) void k_pcf(const double* __restrict__ px,
+                                            const double* __restrict__ py,
+                                            const EdmdPart* __restrict__ part, int n, double t,
+                                            int extrap, double Lx, double Ly, double dr, int nb,
+                                            unsigned long long* __restrict__ hist) {
+    extern __shared__ unsigned int h[];
+    __shared__ double sx[256], sy[256];
+    const int ti = blockIdx.x, tj = blockIdx.y;
+    if (tj < ti) return;
+    const double halfLx = Lx / 2, halfLy = Ly / 2;
+    const double rmax = dr * nb;
+    for (int b = threadIdx.x; b < nb; b += 256) h[b] = 0;
+    auto pos = [&](int q, double& xx, double& yy) {
+        if (part) {
+            const EdmdPart r = part[q];
+            xx = r.x;
+            yy = r.y;
+            if (extrap) {
+                double dt = t - r.tl;
+                xx += dt * r.vx;
+                yy += dt * r.vy;
+                xx = wrap_post(xx, Lx);
+                yy = wrap_post(yy, Ly);
+            }
+        } else {
+            xx = px[q];
+            yy = py[q];
+        }
+    };
+    const int j0 = tj * 256;
+    if (j0 + (int)threadIdx.x < n) pos(j0 + threadIdx.x, sx[threadIdx.x], sy[threadIdx.x]);
+    __syncthreads();
+    const int i = ti * 256 + threadIdx.x;
+    if (i < n) {
+        double xi, yi;
+        pos(i, xi, yi);
+        const int jn = min(256, n - j0);
+        for (int k = 0; k < jn; ++k) {
+            const int j = j0 + k;
+            if (j <= i) continue;
+            double dx = sx[k] - xi;
+            double dy = sy[k] - yi;
+            if (dx >= halfLx) dx -= Lx;
+            else if (dx < -halfLx) dx += Lx;
+            if (dy >= halfLy) dy -= Ly;
+            else if (dy < -halfLy) dy += Ly;
+            double r = sqrt(dx * dx + dy * dy);
+            if (r < rmax) {
+                int bin = (int)(r / dr);
+                if (bin < nb) atomicAdd(&h[bin], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += 256)
+        if (h[b]) atomicAdd(&hist[b], (unsigned long long)h[b]);
+}
+
 __global__ void k_fp64_check(int n, const double* __restrict__ a, const double* __restrict__ b,
                              double* __restrict__ q, double* __restrict__ s) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1061,6 +1124,15 @@ hipError_t edmd_launch_repredict(const EdmdSys* d_sys, int max_nlev, int i, int 
 
 hipError_t edmd_launch_fly_one(const EdmdSys* d_sys, int i, hipStream_t s) {
     hipLaunchKernelGGL(k_fly_one, dim3(1), dim3(1), 0, s, d_sys, i);
+    return hipGetLastError();
+}
+
+hipError_t edmd_launch_pcf(const double* x, const double* y, const EdmdPart* part, int n, double t,
+                           int extrap, double Lx, double Ly, double dr, int nb,
+                           unsigned long long* hist, hipStream_t s) {
+    const int tiles = (n + 255) / 256;
+    hipLaunchKernelGGL(k_pcf, dim3(tiles, tiles), dim3(256), (size_t)nb * sizeof(unsigned int), s,
+                       x, y, part, n, t, extrap, Lx, Ly, dr, nb, hist);
     return hipGetLastError();
 }
 

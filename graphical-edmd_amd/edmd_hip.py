@@ -109,6 +109,8 @@ def lib():
             "edmd_dump_read": (i32, [C.c_char_p, i32] + [vp] * 6),
             "edmd_dump_write": (i32, [C.c_char_p, i32, dbl, i32, dbl, dbl] + [vp] * 6 + [i32]),
             "edmd_fp64_check": (i32, [i32, vp, vp, vp, vp]),
+            "edmd_pcf": (i32, [i32, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]),
+            "edmd_batch_pcf": (i32, [vp, i32, i32, dbl, dbl, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -316,6 +318,14 @@ class Batch:
     def stream(self):
         return lib().edmd_batch_stream(self.h)
 
+    def pcf(self, s, dr, max_r, extrapolate=False):
+        nb = int(max_r / dr)
+        r, g = np.zeros(max(nb, 1)), np.zeros(max(nb, 1))
+        n = C.c_int(max(nb, 1))
+        check(lib().edmd_batch_pcf(self.h, int(s), int(bool(extrapolate)), float(dr),
+                                   float(max_r), r.ctypes.data, g.ctypes.data, C.byref(n)))
+        return r[:n.value], g[:n.value]
+
 
 def init_lattice(n, phi, rad=0.5, lattice=0, seed=1, temperature=1.0):
     x, y, vx, vy = (np.zeros(n) for _ in range(4))
@@ -333,6 +343,17 @@ def fp64_check(a, b):
     check(lib().edmd_fp64_check(len(a), a.ctypes.data, b.ctypes.data, q.ctypes.data,
                                 s.ctypes.data))
     return q, s
+
+
+def pcf(x, y, Lx, Ly, dr, max_r):
+    """calculate_pcf on the GPU for host positions."""
+    x, y = _f64(x), _f64(y)
+    nb = int(max_r / dr)
+    r, g = np.zeros(max(nb, 1)), np.zeros(max(nb, 1))
+    n = C.c_int(max(nb, 1))
+    check(lib().edmd_pcf(len(x), x.ctypes.data, y.ctypes.data, float(Lx), float(Ly), float(dr),
+                         float(max_r), r.ctypes.data, g.ctypes.data, C.byref(n)))
+    return r[:n.value], g[:n.value]
 
 
 def device_count():

@@ -190,6 +190,20 @@ int edmd_dump_write(const char* path, int append, double t, int n, double Lx, do
                     const double* x, const double* y, const double* vx, const double* vy,
                     const double* rad, const double* m, int full_precision);
 
+/* ---- observables ----------------------------------------------------------- */
+/* calculate_pcf (pcf.c:16-75) on the GPU: g(r) of n positions with the
+ * minimum image, bins of width dr up to (int)(max_r/dr)*dr, normalised by
+ * 2 pi r dr * (n / (Lx Ly)) * n.  On entry *nbins is the capacity of r and
+ * g; on return the bin count.  Pair counts are integers, so the result is
+ * the reference's bit for bit. */
+int edmd_pcf(int n, const double* x, const double* y, double Lx, double Ly, double dr,
+             double max_r, double* r, double* g, int* nbins);
+/* The same for system `sys` of a batch: its stored positions (as the
+ * reference's thermo-time g(r) uses them) or, with extrapolate != 0, all
+ * disks free-flown to the current time without changing the state. */
+int edmd_batch_pcf(edmd_batch* b, int sys, int extrapolate, double dr, double max_r, double* r,
+                   double* g, int* nbins);
+
 /* ---- diagnostics ----------------------------------------------------------- */
 /* q[k] = a[k] / b[k] and s[k] = sqrt(a[k]) evaluated on the GPU, for the
  * IEEE-rounding tests of the FP64 operations the engine relies on. */

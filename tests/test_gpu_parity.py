@@ -331,3 +331,33 @@ def test_cli_writes_reference_outputs(gpu, oracle, tmp_path):
     so = o.state()
     np.testing.assert_array_equal(bits(frame["x"]), bits(so["x"]))
     np.testing.assert_array_equal(bits(frame["vy"]), bits(so["vy"]))
+
+
+def test_pcf_matches_reference(gpu, oracle):
+    """calculate_pcf (pcf.c:16-75) on the GPU: bit-exact with the reference's
+    golden histogram and with the oracle on an evolved state."""
+    g = load_golden("cfg1_d1")
+    L = float(g["Lx"])
+    r, gr = gpu.pcf(g["x"], g["y"], L, L, 0.1, L / 2)
+    np.testing.assert_array_equal(bits(r), bits(g["pcf_r"]))
+    np.testing.assert_array_equal(bits(gr), bits(g["pcf_g"]))
+    st = F.lattice_state(3000, 0.65, seed=8, lattice="fit")
+    b = gpu.Batch([st])
+    b.run(max_events=30000)
+    o = oracle.Oracle(st)
+    o.run(30000, trace=False)
+    so = o.state()
+    L = st["Lx"]
+    r1, g1 = b.pcf(0, 0.05, L / 2)
+    r2, g2 = oracle.pcf(so["x"], so["y"], L, L, 0.05, L / 2)
+    np.testing.assert_array_equal(bits(g1), bits(g2))
+    # free-flown to the current time without touching the state
+    t = b.observables(0)["t"]
+    r3, g3 = b.pcf(0, 0.05, L / 2, extrapolate=True)
+    o.sync_all(t)
+    so2 = o.state()
+    r4, g4 = oracle.pcf(so2["x"], so2["y"], L, L, 0.05, L / 2)
+    np.testing.assert_array_equal(bits(g3), bits(g4))
+    sb = b.state(0)
+    np.testing.assert_array_equal(bits(sb["x"]), bits(so["x"]))  # state untouched
+    assert np.all(g3[r3 < 0.95] == 0)
