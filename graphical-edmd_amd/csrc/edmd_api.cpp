@@ -93,6 +93,20 @@ struct edmd_batch {
     EdmdState* st = nullptr;
     EdmdTraceRec* trace = nullptr;
     EdmdThermoRow* thermo = nullptr;
+    /* derivatives: tape pools, checkpoints (one snapshot of every pool per
+     * segment, with the run bounds the segment was run with) */
+    bool deriv_on = false;
+    int64_t seg = 0, tape_cap = 0;
+    std::vector<void*> dallocs;
+    EdmdTapeRec* tape = nullptr;
+    double *dv = nullptr, *dtl = nullptr, *otl = nullptr;
+    EdmdDeriv* dr = nullptr;
+    struct Ckpt {
+        void* mem;
+        EdmdRunArgs ra;
+    };
+    std::vector<Ckpt> ckpts;
+    void* ckpt_end = nullptr;
 };
 
 struct edmd_ctx {
@@ -110,8 +124,23 @@ static int dalloc(edmd_batch* b, T** p, size_t n) {
     return EDMD_OK;
 }
 
+static void deriv_release(edmd_batch* b) {
+    for (auto& c : b->ckpts) (void)hipFree(c.mem);
+    b->ckpts.clear();
+    if (b->ckpt_end) (void)hipFree(b->ckpt_end);
+    b->ckpt_end = nullptr;
+    for (void* p : b->dallocs) (void)hipFree(p);
+    b->dallocs.clear();
+    b->tape = nullptr;
+    b->dv = b->dtl = b->otl = nullptr;
+    b->dr = nullptr;
+    b->deriv_on = false;
+    b->seg = b->tape_cap = 0;
+}
+
 static void batch_free(edmd_batch* b) {
     if (!b) return;
+    deriv_release(b);
     for (void* p : b->allocs) (void)hipFree(p);
     if (b->d_sys) (void)hipFree(b->d_sys);
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
@@ -251,6 +280,7 @@ static int upload_system(edmd_batch* b, int s, const edmd_system_desc* d) {
     st.next_thermo = d->first_thermo + 0.000000001; /* EDMD.c:1970 */
     st.next_noise = d->noise ? d->dtnoise : __builtin_huge_val();
     st.last_thermo = 0;
+    st.ev_limit = INT64_MAX;
     HIPCHK(hipMemcpy(b->st + s, &st, sizeof st, hipMemcpyHostToDevice));
     return EDMD_OK;
 }
@@ -905,4 +935,299 @@ extern "C" int edmd_batch_pcf(edmd_batch* b, int s, int extrapolate, double dr, 
     HIPCHK(hipMemcpy(&st, b->st + s, sizeof st, hipMemcpyDeviceToHost));
     return pcf_common(nullptr, nullptr, b->part + h.p0, h.c.n, st.t, extrapolate, h.c.Lx, h.c.Ly,
                       dr, max_r, r, g, nbins, b->stream);
+}
+
+/* ------------------------------------------------------------ derivatives */
+
+static_assert(sizeof(edmd_tape_rec) == sizeof(EdmdTapeRec), "tape record layout");
+
+/* Byte size of one checkpoint: every pool that k_run reads or writes,
+ * except the trace and thermo rows, which a recomputed segment rewrites
+ * with the same values. */
+struct PoolSpan {
+    void* p;
+    size_t bytes;
+};
+
+static std::vector<PoolSpan> state_pools(edmd_batch* b) {
+    return {{b->part, b->np * sizeof(EdmdPart)},
+            {b->ev, b->np * sizeof(EdmdEv)},
+            {b->leaf, b->np * sizeof(EdmdNode)},
+            {b->nodes, b->nnode * sizeof(EdmdNode)},
+            {b->img, 2 * b->np * sizeof(int32_t)},
+            {b->ccnt, b->ncell * sizeof(int32_t)},
+            {b->cslot, b->nslot * sizeof(int32_t)},
+            {b->st, (size_t)b->nsys * sizeof(EdmdState)}};
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static size_t ckpt_bytes(edmd_batch* b) {
+    size_t t = 0;
+    for (auto& q : state_pools(b)) t += align256(q.bytes);
+    return t;
+}
+
+static int ckpt_copy(edmd_batch* b, void* mem, bool save) {
+    char* c = (char*)mem;
+    for (auto& q : state_pools(b)) {
+        if (q.bytes) {
+            if (save) HIPCHK(hipMemcpyAsync(c, q.p, q.bytes, hipMemcpyDeviceToDevice, b->stream));
+            else HIPCHK(hipMemcpyAsync(q.p, c, q.bytes, hipMemcpyDeviceToDevice, b->stream));
+        }
+        c += align256(q.bytes);
+    }
+    return EDMD_OK;
+}
+
+static int upload_table(edmd_batch* b) {
+    HIPCHK(hipMemcpy(b->d_sys, b->sys.data(), b->nsys * sizeof(EdmdSys), hipMemcpyHostToDevice));
+    return EDMD_OK;
+}
+
+template <class T>
+static int deriv_alloc(edmd_batch* b, T** p, size_t n) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(T));
+    if (e != hipSuccess)
+        return fail(EDMD_ERR_NOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+    b->dallocs.push_back(q);
+    *p = (T*)q;
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_deriv_end(edmd_batch* b) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    deriv_release(b);
+    for (int s = 0; s < b->nsys; s++) {
+        EdmdSys& S = b->sys[s];
+        S.c.tape_cap = 0;
+        S.a.tape = nullptr;
+        S.a.dv = S.a.dtl = S.a.otl = nullptr;
+        S.a.dr = nullptr;
+    }
+    int rc = upload_table(b);
+    if (rc) return rc;
+    HIPCHK(edmd_launch_seg_begin(b->d_sys, b->nsys, 0, 1, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_deriv_begin(edmd_batch* b, int64_t checkpoint_every, int64_t tape_capacity) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    if (checkpoint_every < 0) return fail(EDMD_ERR_ARG, "checkpoint_every must be >= 0");
+    const int64_t cap = checkpoint_every > 0 ? checkpoint_every : tape_capacity;
+    if (cap < 1 || cap > INT32_MAX)
+        return fail(EDMD_ERR_ARG, "tape capacity %lld out of range [1, 2^31)", (long long)cap);
+    for (int s = 0; s < b->nsys; s++)
+        if (b->hs[s].c.noise)
+            return fail(EDMD_ERR_ARG, "system %d: derivatives need a system without the "
+                                      "velocity-rescale thermostat", s);
+    int rc = edmd_batch_deriv_end(b);
+    if (rc) return rc;
+    if ((double)b->nsys * (double)cap * sizeof(EdmdTapeRec) > 1.0e12)
+        return fail(EDMD_ERR_NOMEM, "tape of %lld records for %d systems is too large",
+                    (long long)cap, b->nsys);
+    if (!rc) rc = deriv_alloc(b, &b->tape, (size_t)b->nsys * (size_t)cap);
+    if (!rc) rc = deriv_alloc(b, &b->dv, 4 * b->np);
+    if (!rc) rc = deriv_alloc(b, &b->dtl, b->np);
+    if (!rc) rc = deriv_alloc(b, &b->otl, b->np);
+    if (!rc) rc = deriv_alloc(b, &b->dr, (size_t)b->nsys);
+    if (rc) { deriv_release(b); return rc; }
+    for (int s = 0; s < b->nsys; s++) {
+        EdmdSys& S = b->sys[s];
+        const size_t p0 = b->hs[s].p0;
+        S.c.tape_cap = (int32_t)cap;
+        S.a.tape = b->tape + (size_t)s * (size_t)cap;
+        S.a.dv = b->dv + 4 * p0;
+        S.a.dtl = b->dtl + p0;
+        S.a.otl = b->otl + p0;
+        S.a.dr = b->dr + s;
+    }
+    rc = upload_table(b);
+    if (rc) { deriv_release(b); return rc; }
+    b->seg = checkpoint_every;
+    b->tape_cap = cap;
+    b->deriv_on = true;
+    HIPCHK(hipMemsetAsync(b->dv, 0, 4 * b->np * sizeof(double), b->stream));
+    HIPCHK(hipMemsetAsync(b->dr, 0, b->nsys * sizeof(EdmdDeriv), b->stream));
+    HIPCHK(edmd_launch_deriv_origin(b->d_sys, b->nsys, b->max_n, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return EDMD_OK;
+}
+
+static int tape_overflow_check(edmd_batch* b) {
+    std::vector<EdmdState> st(b->nsys);
+    HIPCHK(hipMemcpy(st.data(), b->st, b->nsys * sizeof(EdmdState), hipMemcpyDeviceToHost));
+    for (int s = 0; s < b->nsys; s++)
+        if (st[s].ntape > b->tape_cap)
+            return fail(EDMD_ERR_TAPE_FULL, "system %d: %lld collisions exceed the tape capacity %lld",
+                        s, (long long)st[s].ntape, (long long)b->tape_cap);
+    return EDMD_OK;
+}
+
+/* Runs one checkpoint segment: at most `seg` events per system, new tape. */
+static int run_segment(edmd_batch* b, const EdmdRunArgs& ra) {
+    HIPCHK(edmd_launch_seg_begin(b->d_sys, b->nsys, b->seg, 1, b->stream));
+    HIPCHK(edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, ra, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return check_status(b);
+}
+
+extern "C" int edmd_batch_deriv_run(edmd_batch* b, int64_t max_events, uint64_t max_collisions,
+                                    double tmax) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    if (!b->deriv_on) return fail(EDMD_ERR_STATE, "taping not started (edmd_batch_deriv_begin)");
+    if (b->seg == 0) {
+        int rc = batch_run_mask(b, max_events, max_collisions, tmax, 0);
+        if (rc) return rc;
+        return tape_overflow_check(b);
+    }
+    EdmdRunArgs ra;
+    memset(&ra, 0, sizeof ra);
+    ra.max_events = max_events;
+    ra.max_coll = max_collisions;
+    ra.tmax = tmax;
+    const size_t bytes = ckpt_bytes(b);
+    std::vector<EdmdState> st(b->nsys);
+    for (;;) {
+        void* mem = nullptr;
+        hipError_t e = hipMalloc(&mem, bytes);
+        if (e != hipSuccess)
+            return fail(EDMD_ERR_NOMEM, "checkpoint %zu (%zu bytes): %s", b->ckpts.size(), bytes,
+                        hipGetErrorString(e));
+        b->ckpts.push_back({mem, ra});
+        int rc = ckpt_copy(b, mem, true);
+        if (!rc) rc = run_segment(b, ra);
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(st.data(), b->st, b->nsys * sizeof(EdmdState), hipMemcpyDeviceToHost));
+        bool more = false;
+        for (int s = 0; s < b->nsys; s++)
+            if (st[s].stop_reason == EDMD_STOP_EVENTS && (int64_t)st[s].nevents < max_events)
+                more = true;
+        if (!more) break;
+    }
+    HIPCHK(edmd_launch_seg_begin(b->d_sys, b->nsys, 0, 0, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return EDMD_OK;
+}
+
+/* Recomputes segment k's tape from its checkpoint. */
+static int replay_segment(edmd_batch* b, size_t k) {
+    int rc = ckpt_copy(b, b->ckpts[k].mem, false);
+    if (!rc) rc = run_segment(b, b->ckpts[k].ra);
+    return rc;
+}
+
+static int save_end(edmd_batch* b) {
+    if (!b->ckpt_end) {
+        hipError_t e = hipMalloc(&b->ckpt_end, ckpt_bytes(b));
+        if (e != hipSuccess) {
+            b->ckpt_end = nullptr;
+            return fail(EDMD_ERR_NOMEM, "end-state snapshot: %s", hipGetErrorString(e));
+        }
+    }
+    return ckpt_copy(b, b->ckpt_end, true);
+}
+
+static int restore_end(edmd_batch* b) {
+    int rc = ckpt_copy(b, b->ckpt_end, false);
+    if (rc) return rc;
+    HIPCHK(edmd_launch_seg_begin(b->d_sys, b->nsys, 0, 0, b->stream));
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_tangent(edmd_batch* b, const double* tan_in, double* tan_out, double* dw,
+                                  double* dtlast) {
+    if (!b || !tan_in || !tan_out) return fail(EDMD_ERR_ARG, "null argument");
+    if (!b->deriv_on) return fail(EDMD_ERR_STATE, "taping not started (edmd_batch_deriv_begin)");
+    HIPCHK(hipMemcpy(b->dv, tan_in, 4 * b->np * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(b->dr, 0, b->nsys * sizeof(EdmdDeriv), b->stream));
+    HIPCHK(edmd_launch_deriv_init(b->d_sys, b->nsys, b->max_n, 0, b->stream));
+    int rc = EDMD_OK;
+    if (b->seg == 0) {
+        HIPCHK(edmd_launch_tangent(b->d_sys, b->nsys, b->max_n, b->stream));
+    } else {
+        rc = save_end(b);
+        for (size_t k = 0; !rc && k < b->ckpts.size(); k++) {
+            rc = replay_segment(b, k);
+            if (!rc) HIPCHK(edmd_launch_tangent(b->d_sys, b->nsys, b->max_n, b->stream));
+        }
+        int rc2 = restore_end(b);
+        if (!rc) rc = rc2;
+        if (rc) return rc;
+    }
+    HIPCHK(edmd_launch_deriv_flush(b->d_sys, b->nsys, b->max_n, 0, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(tan_out, b->dv, 4 * b->np * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<EdmdDeriv> d(b->nsys);
+    HIPCHK(hipMemcpy(d.data(), b->dr, b->nsys * sizeof(EdmdDeriv), hipMemcpyDeviceToHost));
+    for (int s = 0; s < b->nsys; s++) {
+        if (dw) dw[s] = d[s].dsum;
+        if (dtlast) dtlast[s] = d[s].dtlast;
+    }
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_adjoint(edmd_batch* b, const double* lw, const double* lt,
+                                  const double* lam_end, double* grad) {
+    if (!b || !grad) return fail(EDMD_ERR_ARG, "null argument");
+    if (!b->deriv_on) return fail(EDMD_ERR_STATE, "taping not started (edmd_batch_deriv_begin)");
+    std::vector<EdmdDeriv> d(b->nsys);
+    for (int s = 0; s < b->nsys; s++) {
+        memset(&d[s], 0, sizeof d[s]);
+        d[s].lw = lw ? lw[s] : 0.0;
+        d[s].lt = lt ? lt[s] : 0.0;
+        d[s].lt_pending = 1;
+    }
+    HIPCHK(hipMemcpy(b->dr, d.data(), b->nsys * sizeof(EdmdDeriv), hipMemcpyHostToDevice));
+    if (lam_end) HIPCHK(hipMemcpy(b->dv, lam_end, 4 * b->np * sizeof(double), hipMemcpyHostToDevice));
+    else HIPCHK(hipMemsetAsync(b->dv, 0, 4 * b->np * sizeof(double), b->stream));
+    HIPCHK(edmd_launch_deriv_init(b->d_sys, b->nsys, b->max_n, 1, b->stream));
+    int rc = EDMD_OK;
+    if (b->seg == 0) {
+        HIPCHK(edmd_launch_adjoint(b->d_sys, b->nsys, b->max_n, b->stream));
+    } else {
+        rc = save_end(b);
+        for (size_t k = b->ckpts.size(); !rc && k-- > 0;) {
+            rc = replay_segment(b, k);
+            if (!rc) HIPCHK(edmd_launch_adjoint(b->d_sys, b->nsys, b->max_n, b->stream));
+        }
+        int rc2 = restore_end(b);
+        if (!rc) rc = rc2;
+        if (rc) return rc;
+    }
+    HIPCHK(edmd_launch_deriv_flush(b->d_sys, b->nsys, b->max_n, 1, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(grad, b->dv, 4 * b->np * sizeof(double), hipMemcpyDeviceToHost));
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_deriv_values(edmd_batch* b, int sys, double* w, double* tlast,
+                                       int64_t* records) {
+    int rc = sys_check(b, sys);
+    if (rc) return rc;
+    if (!b->deriv_on) return fail(EDMD_ERR_STATE, "taping not started (edmd_batch_deriv_begin)");
+    EdmdState st;
+    HIPCHK(hipMemcpy(&st, b->st + sys, sizeof st, hipMemcpyDeviceToHost));
+    if (w) *w = st.wsum;
+    if (tlast) *tlast = st.tlast;
+    if (records) *records = st.ntape < b->tape_cap ? st.ntape : b->tape_cap;
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_get_tape(edmd_batch* b, int sys, edmd_tape_rec* out, int64_t cap,
+                                   int64_t* n) {
+    int64_t k = 0;
+    int rc = edmd_batch_deriv_values(b, sys, nullptr, nullptr, &k);
+    if (rc) return rc;
+    if (n) *n = k;
+    if (out && cap > 0) {
+        const int64_t c = std::min(k, cap);
+        if (c > 0)
+            HIPCHK(hipMemcpy(out, b->tape + (size_t)sys * (size_t)b->tape_cap, c * sizeof(EdmdTapeRec),
+                             hipMemcpyDeviceToHost));
+    }
+    return EDMD_OK;
 }

@@ -24,4 +24,15 @@ hipError_t edmd_launch_pcf(const double* x, const double* y, const EdmdPart* par
 hipError_t edmd_launch_fp64_check(int n, const double* a, const double* b, double* q, double* sq,
                                   hipStream_t s);
 
+/* derivatives of the event map (edmd_deriv.hip) */
+hipError_t edmd_launch_tangent(const EdmdSys* d_sys, int nsys, int max_n, hipStream_t s);
+hipError_t edmd_launch_adjoint(const EdmdSys* d_sys, int nsys, int max_n, hipStream_t s);
+hipError_t edmd_launch_deriv_init(const EdmdSys* d_sys, int nsys, int max_n, int mode,
+                                  hipStream_t s);
+hipError_t edmd_launch_deriv_flush(const EdmdSys* d_sys, int nsys, int max_n, int mode,
+                                   hipStream_t s);
+hipError_t edmd_launch_deriv_origin(const EdmdSys* d_sys, int nsys, int max_n, hipStream_t s);
+hipError_t edmd_launch_seg_begin(const EdmdSys* d_sys, int nsys, int64_t seg, int clear,
+                                 hipStream_t s);
+
 #endif

@@ -738,6 +738,18 @@ __device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, 
         q->x = xi; q->y = yi; q->vx = vxi; q->vy = vyi; q->tl = t; q->coll = colli + 1;
         q = A.p + j;
         q->x = xj; q->y = yj; q->vx = vxj; q->vy = vyj; q->tl = t; q->coll = collj + 1;
+        if (C.tape_cap > 0) {
+            const int64_t k = st->ntape;
+            if (k < C.tape_cap) {
+                EdmdTapeRec rec;
+                rec.t = t; rec.ax = dx; rec.ay = dy; rec.cx = dvx; rec.cy = dvy; rec.ff = ff;
+                rec.i = i; rec.j = j; rec.pad[0] = 0; rec.pad[1] = 0;
+                A.tape[k] = rec;
+            }
+            st->ntape = k + 1;
+            st->wsum += w * (dx * dx + dy * dy);
+            st->tlast = t;
+        }
     }
     WAVE_SYNC();
     const Owner oi{i, xi, yi, vxi, vyi, ri, cxi, cyi};
@@ -867,9 +879,11 @@ __global__ __launch_bounds__(64) void k_run(const EdmdSys* __restrict__ systems,
     int reason = EDMD_STOP_NONE;
     int last_type = st->last_type;
 
+    const int64_t ev_limit = st->ev_limit < ra.max_events ? st->ev_limit : ra.max_events;
+
     for (;;) {
         if (R.status != EDMD_DEV_OK) { reason = EDMD_STOP_ERROR; break; }
-        if ((int64_t)R.nevents >= ra.max_events) { reason = EDMD_STOP_EVENTS; break; }
+        if ((int64_t)R.nevents >= ev_limit) { reason = EDMD_STOP_EVENTS; break; }
         if (R.ncol >= ra.max_coll) { reason = EDMD_STOP_COLLISIONS; break; }
         if (!(R.t <= ra.tmax)) { reason = EDMD_STOP_TIME; break; }
         double key;

@@ -57,6 +57,7 @@ typedef struct EdmdConst {
     int32_t lsize[EDMD_MAX_LEVELS + 1];    /* lsize[0] = n, lsize[nlev] <= 64 */
     int32_t noise;                         /* 0 or 2 (velocity rescale) */
     int32_t trace_cap, thermo_cap;
+    int32_t tape_cap;                      /* collision tape records kept, 0 = no tape */
     double Lx, Ly, halfLx, halfLy, csx, csy;
     double res, dtime, dtime_thermo, dtnoise, temperature;
 } EdmdConst;
@@ -69,7 +70,29 @@ typedef struct EdmdState {
     int32_t status, err_i, err_j, stop_reason;
     int64_t ntrace;
     int32_t nthermo, last_type; /* last_type: kind of the last event handled */
+    int64_t ev_limit;           /* per-system bound on nevents (checkpoint segments) */
+    int64_t ntape;              /* collisions taped since the tape was cleared */
+    double wsum;                /* sum of m_i m_j ff |a|^2 over taped collisions */
+    double tlast;               /* time of the last taped collision */
 } EdmdState;
+
+/* One valid collision as the derivative kernels need it: time, contact
+ * vector a = r_j - r_i (minimum image), relative velocity c = v_j - v_i
+ * before the collision and the impulse factor ff of the collision rule. */
+typedef struct __attribute__((aligned(64))) EdmdTapeRec {
+    double t, ax, ay, cx, cy, ff;
+    int32_t i, j;
+    int32_t pad[2];
+} EdmdTapeRec;
+
+/* Per-system accumulators of the tangent / adjoint passes. */
+typedef struct EdmdDeriv {
+    double dsum;   /* tangent: derivative of wsum */
+    double dtlast; /* tangent: derivative of tlast */
+    double lw, lt; /* adjoint seeds on wsum and tlast */
+    int32_t lt_pending; /* adjoint: the tlast seed is still to be applied */
+    int32_t pad;
+} EdmdDeriv;
 
 typedef struct EdmdTraceRec {
     int32_t type, i, j, valid;
@@ -112,6 +135,11 @@ typedef struct EdmdArrays {
     EdmdState* st;
     EdmdTraceRec* trace;
     EdmdThermoRow* thermo;
+    EdmdTapeRec* tape;   /* tape_cap records */
+    double* dv;          /* tangent or adjoint vector, 4 per particle */
+    double* dtl;         /* the time each particle's dv refers to */
+    double* otl;         /* particle local times when taping began */
+    EdmdDeriv* dr;
 } EdmdArrays;
 
 typedef struct EdmdSys {

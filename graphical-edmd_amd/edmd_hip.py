@@ -59,6 +59,10 @@ TRACE_DT = np.dtype([("type", "<i4"), ("i", "<i4"), ("j", "<i4"), ("valid", "<i4
 THERMO_DT = np.dtype([("t", "<f8"), ("ncol", "<u8"), ("e_per_n", "<f8"), ("pressure", "<f8"),
                       ("pxx", "<f8"), ("pyy", "<f8"), ("pxy", "<f8"), ("pyx", "<f8")])
 
+TAPE_DT = np.dtype([("t", "<f8"), ("ax", "<f8"), ("ay", "<f8"), ("cx", "<f8"),
+                    ("cy", "<f8"), ("ff", "<f8"), ("i", "<i4"), ("j", "<i4"),
+                    ("pad", "<i4", (2,))])
+
 _lib = None
 
 
@@ -111,6 +115,13 @@ def lib():
             "edmd_fp64_check": (i32, [i32, vp, vp, vp, vp]),
             "edmd_pcf": (i32, [i32, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]),
             "edmd_batch_pcf": (i32, [vp, i32, i32, dbl, dbl, vp, vp, vp]),
+            "edmd_batch_deriv_begin": (i32, [vp, i64, i64]),
+            "edmd_batch_deriv_run": (i32, [vp, i64, u64, dbl]),
+            "edmd_batch_tangent": (i32, [vp] * 5),
+            "edmd_batch_adjoint": (i32, [vp] * 5),
+            "edmd_batch_deriv_values": (i32, [vp, i32, vp, vp, vp]),
+            "edmd_batch_get_tape": (i32, [vp, i32, vp, i64, vp]),
+            "edmd_batch_deriv_end": (i32, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -317,6 +328,66 @@ class Batch:
 
     def stream(self):
         return lib().edmd_batch_stream(self.h)
+
+    # ---- derivatives of the event map
+
+    def deriv_begin(self, checkpoint_every=256, tape_capacity=0):
+        check(lib().edmd_batch_deriv_begin(self.h, int(checkpoint_every), int(tape_capacity)))
+
+    def deriv_run(self, max_events=INT64_MAX, max_collisions=UINT64_MAX, tmax=1e300):
+        check(lib().edmd_batch_deriv_run(self.h, int(max_events), int(max_collisions),
+                                         float(tmax)))
+
+    def deriv_end(self):
+        check(lib().edmd_batch_deriv_end(self.h))
+
+    def _split(self, flat):
+        out, o = [], 0
+        for n in self.ns:
+            out.append(flat[o:o + n])
+            o += n
+        return out
+
+    def tangent(self, tan_in):
+        """tan_in: list of (n_s, 4) arrays (dx dy dvx dvy per disk).
+
+        Returns (list of final tangents, dW per system, d t_last per system)."""
+        flat = _f64(np.concatenate([np.asarray(t, dtype=np.float64).reshape(-1, 4)
+                                    for t in tan_in]))
+        assert flat.shape == (sum(self.ns), 4)
+        out = np.zeros_like(flat)
+        dw = np.zeros(len(self.ns))
+        dtl = np.zeros(len(self.ns))
+        check(lib().edmd_batch_tangent(self.h, _ptr(flat), _ptr(out), _ptr(dw), _ptr(dtl)))
+        return self._split(out), dw, dtl
+
+    def adjoint(self, lw=None, lt=None, lam_end=None):
+        """Gradient (list of (n_s, 4)) of sum_s lw_s W_s + lt_s t_last_s
+        + <lam_end, final state> with respect to the taping-origin state."""
+        ns = len(self.ns)
+        lw = None if lw is None else _f64(np.broadcast_to(lw, (ns,)))
+        lt = None if lt is None else _f64(np.broadcast_to(lt, (ns,)))
+        le = None
+        if lam_end is not None:
+            le = _f64(np.concatenate([np.asarray(a, dtype=np.float64).reshape(-1, 4)
+                                      for a in lam_end]))
+            assert le.shape == (sum(self.ns), 4)
+        g = np.zeros((sum(self.ns), 4))
+        check(lib().edmd_batch_adjoint(self.h, _ptr(lw), _ptr(lt), _ptr(le), _ptr(g)))
+        return self._split(g)
+
+    def deriv_values(self, s):
+        w, tl, k = C.c_double(), C.c_double(), C.c_int64()
+        check(lib().edmd_batch_deriv_values(self.h, int(s), C.byref(w), C.byref(tl),
+                                            C.byref(k)))
+        return w.value, tl.value, k.value
+
+    def tape(self, s):
+        k = C.c_int64()
+        check(lib().edmd_batch_get_tape(self.h, int(s), None, 0, C.byref(k)))
+        out = np.zeros(max(k.value, 1), dtype=TAPE_DT)
+        check(lib().edmd_batch_get_tape(self.h, int(s), out.ctypes.data, k.value, C.byref(k)))
+        return out[:k.value]
 
     def pcf(self, s, dr, max_r, extrapolate=False):
         nb = int(max_r / dr)

@@ -28,6 +28,7 @@ extern "C" {
 #define EDMD_ERR_OVERLAP 3        /* reference: saveTXT + exit(3), EDMD.c:2708-2717 */
 #define EDMD_ERR_IO 5             /* reference parser: exit(1), parser.c:32-35 */
 #define EDMD_ERR_CELL_OVERFLOW 7  /* a cell list exceeded its capacity */
+#define EDMD_ERR_TAPE_FULL 8      /* more collisions than the tape holds */
 #define EDMD_ERR_NOMEM 12
 #define EDMD_ERR_NODEVICE 19
 #define EDMD_ERR_ARG 22
@@ -203,6 +204,56 @@ int edmd_pcf(int n, const double* x, const double* y, double Lx, double Ly, doub
  * disks free-flown to the current time without changing the state. */
 int edmd_batch_pcf(edmd_batch* b, int sys, int extrapolate, double dr, double max_r, double* r,
                    double* g, int* nbins);
+
+/* ---- derivatives of the event map ------------------------------------------ */
+/* The reference computes no derivatives; these entry points are new.  For
+ * the sequence of valid collisions a run produced, they differentiate the
+ * map from the state when taping began (x, y, vx, vy of every disk at its
+ * local time) to
+ *   W      = sum over taped collisions of m_i m_j ff |r_j - r_i|^2 (the
+ *            collision virial XX + YY that the reference accumulates,
+ *            EDMD.c:3804-3813, so the time-averaged virial is
+ *            W / (t_last - t_origin)),
+ *   t_last = time of the last taped collision,
+ *   and the state of every disk free-flown to the final time,
+ * holding the collision sequence fixed (the map is smooth while it is).
+ * Systems must not use the velocity-rescale thermostat.
+ *
+ * One valid collision: time, contact vector a = r_j - r_i (minimum image),
+ * relative velocity c = v_j - v_i before the collision, impulse factor ff
+ * (v_i += ff m_j a, v_j -= ff m_i a). */
+typedef struct edmd_tape_rec {
+    double t, ax, ay, cx, cy, ff;
+    int32_t i, j;
+    int32_t pad[2];
+} edmd_tape_rec;
+
+/* Starts taping at the current state.  checkpoint_every = K > 0: the run
+ * keeps a full checkpoint of every system every K events and the tape of
+ * one K-event segment; the tangent and adjoint passes recompute each
+ * segment's tape from its checkpoint.  K = 0: the whole tape is kept,
+ * tape_capacity collisions per system (EDMD_ERR_TAPE_FULL beyond). */
+int edmd_batch_deriv_begin(edmd_batch* b, int64_t checkpoint_every, int64_t tape_capacity);
+/* edmd_batch_run with taping (and checkpoints); may be called repeatedly. */
+int edmd_batch_deriv_run(edmd_batch* b, int64_t max_events, uint64_t max_collisions, double tmax);
+/* Tangent (forward) pass.  tan_in / tan_out hold 4 values (dx dy dvx dvy)
+ * per disk, systems in batch order; tan_out is the tangent of the final
+ * state at the systems' current time.  dw[s], dtlast[s] (may be NULL):
+ * derivatives of W and t_last of system s. */
+int edmd_batch_tangent(edmd_batch* b, const double* tan_in, double* tan_out, double* dw,
+                       double* dtlast);
+/* Adjoint (reverse) pass: gradient with respect to the taping-origin state
+ * (4 per disk) of  sum_s lw[s] W_s + lt[s] t_last,s + <lam_end, final state>.
+ * lw, lt: one per system (NULL = 0); lam_end: 4 per disk or NULL. */
+int edmd_batch_adjoint(edmd_batch* b, const double* lw, const double* lt, const double* lam_end,
+                       double* grad);
+/* W, t_last and the number of tape records currently held for system sys. */
+int edmd_batch_deriv_values(edmd_batch* b, int sys, double* w, double* tlast, int64_t* records);
+/* Copies up to cap records of system sys's tape (the whole tape, or the
+ * last segment with checkpoints); *n = records held. */
+int edmd_batch_get_tape(edmd_batch* b, int sys, edmd_tape_rec* out, int64_t cap, int64_t* n);
+/* Stops taping and frees tapes and checkpoints. */
+int edmd_batch_deriv_end(edmd_batch* b);
 
 /* ---- diagnostics ----------------------------------------------------------- */
 /* q[k] = a[k] / b[k] and s[k] = sqrt(a[k]) evaluated on the GPU, for the

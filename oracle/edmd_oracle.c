@@ -53,6 +53,9 @@ struct orc_ctx {
     double last_thermo;
     orc_thermo_row* rows;
     int nrows, caprows;
+    orc_tape_rec* tape;
+    int64_t ntape, captape;
+    double wsum;
 };
 
 /* ------------------------------------------------------------ kinematics */
@@ -398,6 +401,21 @@ static void do_collision(orc_ctx* s, const ev_node* ev) {
     s->vir_yy += w * dy * dy;
     s->vir_xy += w * dx * dy;
     s->vir_yx += w * dy * dx;
+    if (s->tape && s->ntape < s->captape) {
+        orc_tape_rec* r = &s->tape[s->ntape];
+        r->i = i;
+        r->j = j;
+        r->t = s->t;
+        r->ax = dx;
+        r->ay = dy;
+        r->cx = dvx;
+        r->cy = dvy;
+        r->ff = ff;
+    }
+    if (s->tape) {
+        s->ntape++;
+        s->wsum += w * (dx * dx + dy * dy);
+    }
     s->vx[i] += ff * s->m[j] * dx;
     s->vy[i] += ff * s->m[j] * dy;
     s->vx[j] -= ff * s->m[i] * dx;
@@ -613,7 +631,7 @@ void orc_destroy(orc_ctx* s) {
     free(s->x); free(s->y); free(s->vx); free(s->vy); free(s->tl); free(s->rad);
     free(s->m); free(s->coll); free(s->cx); free(s->cy); free(s->crossX);
     free(s->crossY); free(s->head); free(s->nxt); free(s->prv); free(s->ev);
-    free(s->lists); free(s->rows);
+    free(s->lists); free(s->rows); free(s->tape);
     free(s);
 }
 
@@ -786,4 +804,239 @@ int orc_pcf(int n, const double* x, const double* y, double Lx, double Ly,
         else g[b] = 0.0;
     }
     return nb;
+}
+
+/* ------------------------------------------------------------ derivatives */
+
+int orc_tape_enable(orc_ctx* s, int64_t capacity) {
+    free(s->tape);
+    s->tape = malloc((size_t)(capacity > 0 ? capacity : 1) * sizeof(orc_tape_rec));
+    s->captape = capacity;
+    s->ntape = 0;
+    s->wsum = 0;
+    return s->tape ? ORC_OK : ORC_ERR_NOMEM;
+}
+
+int64_t orc_tape_count(const orc_ctx* s) { return s->ntape; }
+
+void orc_get_tape(const orc_ctx* s, orc_tape_rec* out) {
+    int64_t k = s->ntape < s->captape ? s->ntape : s->captape;
+    memcpy(out, s->tape, (size_t)k * sizeof *out);
+}
+
+void orc_tape_clear(orc_ctx* s) {
+    s->ntape = 0;
+    s->wsum = 0;
+}
+
+double orc_tape_wsum(const orc_ctx* s) { return s->wsum; }
+
+double orc_tangent(int n, const double* m, const double* tl0, const orc_tape_rec* tape,
+                   int64_t ntape, double t_end, double* tan, double* dt_last) {
+    double* tl = malloc((size_t)n * sizeof(double));
+    memcpy(tl, tl0, (size_t)n * sizeof(double));
+    double dsum = 0, dtl = 0;
+    for (int64_t k = 0; k < ntape; k++) {
+        const orc_tape_rec* r = &tape[k];
+        const int i = r->i, j = r->j;
+        double* ti = tan + 4 * i;
+        double* tj = tan + 4 * j;
+        double h = r->t - tl[i];
+        ti[0] += h * ti[2];
+        ti[1] += h * ti[3];
+        tl[i] = r->t;
+        h = r->t - tl[j];
+        tj[0] += h * tj[2];
+        tj[1] += h * tj[3];
+        tl[j] = r->t;
+        const double ax = r->ax, ay = r->ay, cx = r->cx, cy = r->cy, ff = r->ff;
+        const double mi = m[i], mj = m[j];
+        const double b = ax * cx + ay * cy;
+        const double px = tj[0] - ti[0], py = tj[1] - ti[1];
+        const double qx = tj[2] - ti[2], qy = tj[3] - ti[3];
+        const double dt = -(ax * px + ay * py) / b;
+        const double rcx = px + cx * dt, rcy = py + cy * dt;
+        const double kap = ff / b;
+        const double dff = kap * ((rcx * cx + rcy * cy) + (ax * qx + ay * qy));
+        const double ux = dff * ax + ff * rcx, uy = dff * ay + ff * rcy;
+        ti[2] += mj * ux;
+        ti[3] += mj * uy;
+        tj[2] -= mi * ux;
+        tj[3] -= mi * uy;
+        const double sh = ff * dt;
+        ti[0] -= mj * sh * ax;
+        ti[1] -= mj * sh * ay;
+        tj[0] += mi * sh * ax;
+        tj[1] += mi * sh * ay;
+        dsum += mi * mj * (ax * ax + ay * ay) * dff;
+        dtl = dt;
+    }
+    for (int i = 0; i < n; i++) {
+        double h = t_end - tl[i];
+        tan[4 * i] += h * tan[4 * i + 2];
+        tan[4 * i + 1] += h * tan[4 * i + 3];
+    }
+    free(tl);
+    if (dt_last) *dt_last = dtl;
+    return dsum;
+}
+
+void orc_adjoint(int n, const double* m, const double* tl0, const orc_tape_rec* tape,
+                 int64_t ntape, double t_end, double lw, double lt, const double* lam_end,
+                 double* grad) {
+    double* tl = malloc((size_t)n * sizeof(double));
+    for (int i = 0; i < n; i++) tl[i] = t_end;
+    if (lam_end) memcpy(grad, lam_end, (size_t)n * 4 * sizeof(double));
+    else memset(grad, 0, (size_t)n * 4 * sizeof(double));
+    for (int64_t k = ntape - 1; k >= 0; k--) {
+        const orc_tape_rec* r = &tape[k];
+        const int i = r->i, j = r->j;
+        double* li = grad + 4 * i;
+        double* lj = grad + 4 * j;
+        double h = tl[i] - r->t;
+        li[2] += h * li[0];
+        li[3] += h * li[1];
+        tl[i] = r->t;
+        h = tl[j] - r->t;
+        lj[2] += h * lj[0];
+        lj[3] += h * lj[1];
+        tl[j] = r->t;
+        const double ax = r->ax, ay = r->ay, cx = r->cx, cy = r->cy, ff = r->ff;
+        const double mi = m[i], mj = m[j];
+        const double b = ax * cx + ay * cy;
+        const double kap = ff / b;
+        const double lux = mj * li[2] - mi * lj[2], luy = mj * li[3] - mi * lj[3];
+        const double lsh = -mj * (ax * li[0] + ay * li[1]) + mi * (ax * lj[0] + ay * lj[1]);
+        const double ldff = (ax * lux + ay * luy) + lw * mi * mj * (ax * ax + ay * ay);
+        const double lrcx = ff * lux + kap * ldff * cx, lrcy = ff * luy + kap * ldff * cy;
+        const double lqx = kap * ldff * ax, lqy = kap * ldff * ay;
+        double ldt = ff * lsh + (cx * lrcx + cy * lrcy);
+        if (k == ntape - 1) ldt += lt;
+        const double lpx = lrcx - (ldt / b) * ax, lpy = lrcy - (ldt / b) * ay;
+        li[0] -= lpx;
+        li[1] -= lpy;
+        lj[0] += lpx;
+        lj[1] += lpy;
+        li[2] -= lqx;
+        li[3] -= lqy;
+        lj[2] += lqx;
+        lj[3] += lqy;
+    }
+    for (int i = 0; i < n; i++) {
+        double h = tl[i] - tl0[i];
+        grad[4 * i + 2] += h * grad[4 * i];
+        grad[4 * i + 3] += h * grad[4 * i + 1];
+    }
+    free(tl);
+}
+
+double orc_replay(int n, double Lx, double Ly, const double* rad, const double* m,
+                  double res, const orc_tape_rec* tape, int64_t ntape, double t_end,
+                  double* x, double* y, double* vx, double* vy, double* tl,
+                  double* t_last) {
+    double S = 0, tk = 0;
+    for (int64_t k = 0; k < ntape; k++) {
+        const orc_tape_rec* r = &tape[k];
+        const int i = r->i, j = r->j;
+        const double tr = r->t;
+        double xi = x[i] + (tr - tl[i]) * vx[i], yi = y[i] + (tr - tl[i]) * vy[i];
+        double xj = x[j] + (tr - tl[j]) * vx[j], yj = y[j] + (tr - tl[j]) * vy[j];
+        double ax = xj - xi, ay = yj - yi;
+        ax -= Lx * nearbyint(ax / Lx);
+        ay -= Ly * nearbyint(ay / Ly);
+        const double cx = vx[j] - vx[i], cy = vy[j] - vy[i];
+        const double sig2 = 4 * rad[i] * rad[j];
+        const double b = ax * cx + ay * cy, c2 = cx * cx + cy * cy;
+        const double d = ax * ax + ay * ay - sig2;
+        const double det = b * b - c2 * d;
+        if (!(det >= 0) || !(b < 0)) return NAN;
+        const double sh = d / (-b + sqrt(det)); /* smaller root, cancellation free */
+        tk = tr + sh;
+        x[i] = xi + sh * vx[i]; y[i] = yi + sh * vy[i];
+        x[j] = xj + sh * vx[j]; y[j] = yj + sh * vy[j];
+        ax += sh * cx;
+        ay += sh * cy;
+        tl[i] = tl[j] = tk;
+        const double ff = (1 + res) / (m[i] + m[j]) * (ax * cx + ay * cy) / sig2;
+        S += m[i] * m[j] * ff * (ax * ax + ay * ay);
+        vx[i] += ff * m[j] * ax; vy[i] += ff * m[j] * ay;
+        vx[j] -= ff * m[i] * ax; vy[j] -= ff * m[i] * ay;
+    }
+    for (int i = 0; i < n; i++) {
+        x[i] += (t_end - tl[i]) * vx[i];
+        y[i] += (t_end - tl[i]) * vy[i];
+        tl[i] = t_end;
+    }
+    if (t_last) *t_last = tk;
+    return S;
+}
+
+typedef long double ld;
+
+static ld replay_objective_ld(int n, ld Lx, ld Ly, const double* rad, const double* m, ld res,
+                              const orc_tape_rec* tape, int64_t ntape, ld t_end, ld* x, ld* y,
+                              ld* vx, ld* vy, ld* tl, ld lw, ld lt, const double* lam) {
+    ld S = 0, tk = 0;
+    for (int64_t k = 0; k < ntape; k++) {
+        const orc_tape_rec* r = &tape[k];
+        const int i = r->i, j = r->j;
+        const ld tr = r->t;
+        ld xi = x[i] + (tr - tl[i]) * vx[i], yi = y[i] + (tr - tl[i]) * vy[i];
+        ld xj = x[j] + (tr - tl[j]) * vx[j], yj = y[j] + (tr - tl[j]) * vy[j];
+        ld ax = xj - xi, ay = yj - yi;
+        ax -= Lx * nearbyintl(ax / Lx);
+        ay -= Ly * nearbyintl(ay / Ly);
+        const ld cx = vx[j] - vx[i], cy = vy[j] - vy[i];
+        const ld sig2 = 4 * (ld)rad[i] * (ld)rad[j];
+        const ld b = ax * cx + ay * cy, c2 = cx * cx + cy * cy;
+        const ld d = ax * ax + ay * ay - sig2;
+        const ld det = b * b - c2 * d;
+        if (!(det >= 0) || !(b < 0)) return NAN;
+        const ld sh = d / (-b + sqrtl(det));
+        tk = tr + sh;
+        x[i] = xi + sh * vx[i]; y[i] = yi + sh * vy[i];
+        x[j] = xj + sh * vx[j]; y[j] = yj + sh * vy[j];
+        ax += sh * cx;
+        ay += sh * cy;
+        tl[i] = tl[j] = tk;
+        const ld mi = m[i], mj = m[j];
+        const ld ff = (1 + res) / (mi + mj) * (ax * cx + ay * cy) / sig2;
+        S += mi * mj * ff * (ax * ax + ay * ay);
+        vx[i] += ff * mj * ax; vy[i] += ff * mj * ay;
+        vx[j] -= ff * mi * ax; vy[j] -= ff * mi * ay;
+    }
+    ld J = lw * S + lt * tk;
+    for (int i = 0; i < n; i++) {
+        x[i] += (t_end - tl[i]) * vx[i];
+        y[i] += (t_end - tl[i]) * vy[i];
+        if (lam)
+            J += lam[4 * i] * x[i] + lam[4 * i + 1] * y[i] + lam[4 * i + 2] * vx[i] +
+                 lam[4 * i + 3] * vy[i];
+    }
+    return J;
+}
+
+double orc_replay_fd(int n, double Lx, double Ly, const double* rad, const double* m,
+                     double res, const orc_tape_rec* tape, int64_t ntape, double t_end,
+                     const double* x, const double* y, const double* vx, const double* vy,
+                     const double* tl, const double* e, double h, double lw, double lt,
+                     const double* lam) {
+    ld* buf = malloc((size_t)n * 5 * sizeof(ld));
+    if (!buf) return NAN;
+    ld J[2];
+    for (int side = 0; side < 2; side++) {
+        const ld hh = side == 0 ? (ld)h : -(ld)h;
+        ld *X = buf, *Y = buf + n, *VX = buf + 2 * n, *VY = buf + 3 * n, *TL = buf + 4 * n;
+        for (int i = 0; i < n; i++) {
+            X[i] = (ld)x[i] + hh * e[4 * i];
+            Y[i] = (ld)y[i] + hh * e[4 * i + 1];
+            VX[i] = (ld)vx[i] + hh * e[4 * i + 2];
+            VY[i] = (ld)vy[i] + hh * e[4 * i + 3];
+            TL[i] = tl[i];
+        }
+        J[side] = replay_objective_ld(n, Lx, Ly, rad, m, res, tape, ntape, t_end, X, Y, VX, VY,
+                                      TL, lw, lt, lam);
+    }
+    free(buf);
+    return (double)((J[0] - J[1]) / (2 * (ld)h));
 }

@@ -100,6 +100,65 @@ void orc_predict_crossing(orc_ctx* s, int i, double* t_abs, int32_t* dir);
  * as the screenshot event does (EDMD.c:4659-4661). */
 void orc_sync_all(orc_ctx* s, double t);
 
+/* ---- derivatives of the event map (no reference counterpart) ----------
+ * A tape records every valid collision: the pair, the time, the contact
+ * vector a = r_j - r_i (minimum image), the relative velocity c = v_j - v_i
+ * before the collision and the impulse factor ff of the collision rule
+ * (EDMD.c:3583-3606).  For a fixed event sequence the tangent map of one
+ * collision (dt = -(a.p)/b, b = a.c, p = dr_j - dr_i, q = dv_j - dv_i) is
+ *   rc = p + c dt, dff = (ff/b)(rc.c + a.q), u = dff a + ff rc,
+ *   dv_i += m_j u, dv_j -= m_i u, dr_i -= m_j ff dt a, dr_j += m_i ff dt a,
+ *   dw = m_i m_j |a|^2 dff        (w = m_i m_j ff |a|^2, the virial term)
+ * and free flight is dr += (t - t_last) dv. */
+typedef struct {
+    int32_t i, j;
+    double t, ax, ay, cx, cy, ff;
+} orc_tape_rec;
+
+int orc_tape_enable(orc_ctx* s, int64_t capacity);
+int64_t orc_tape_count(const orc_ctx* s);
+void orc_get_tape(const orc_ctx* s, orc_tape_rec* out);
+void orc_tape_clear(orc_ctx* s);
+/* Tangent propagation of tan (4 per particle: dx dy dvx dvy at time tl0[i])
+ * through ntape collisions, then free flight of every particle to t_end.
+ * Returns the derivative of sum_k w_k; *dt_last = derivative of the time of
+ * the last taped collision. */
+double orc_tangent(int n, const double* m, const double* tl0, const orc_tape_rec* tape,
+                   int64_t ntape, double t_end, double* tan, double* dt_last);
+/* Adjoint of the same map: the objective is lw * sum_k w_k + lt * t_last
+ * + <lam_end, state at t_end> (lam_end may be NULL).  Writes its gradient
+ * with respect to (x, y, vx, vy) stored at times tl0 into grad (4 per
+ * particle). */
+void orc_adjoint(int n, const double* m, const double* tl0, const orc_tape_rec* tape,
+                 int64_t ntape, double t_end, double lw, double lt, const double* lam_end,
+                 double* grad);
+/* Sum of w_k = m_i m_j ff |a|^2 over the taped collisions, in tape order. */
+double orc_tape_wsum(const orc_ctx* s);
+
+/* Fixed-sequence replay: the nonlinear map whose derivative the tangent and
+ * adjoint compute.  Starting from (x, y, vx, vy) at times tl (updated in
+ * place, 4 arrays + tl), every taped collision (i, j) is redone in tape
+ * order: both disks fly to the taped time, the contact time is re-solved
+ * from the perturbed state (stable root of |a + c s|^2 = 4 r_i r_j), both
+ * fly to it and the collision rule is applied; finally every disk flies to
+ * t_end.  Returns sum of w_k; *t_last = time of the last replayed collision.
+ * Returns NaN when a pair no longer meets. */
+double orc_replay(int n, double Lx, double Ly, const double* rad, const double* m,
+                  double res, const orc_tape_rec* tape, int64_t ntape, double t_end,
+                  double* x, double* y, double* vx, double* vy, double* tl,
+                  double* t_last);
+
+/* Central difference (J(s0 + h e) - J(s0 - h e)) / 2h of
+ *   J = lw * sum_k w_k + lt * t_last + <lam, state at t_end>   (lam may be NULL)
+ * along direction e (4 per particle) over the fixed-sequence replay above,
+ * evaluated in extended precision (long double) so that rounding noise
+ * stays far below the derivative. */
+double orc_replay_fd(int n, double Lx, double Ly, const double* rad, const double* m,
+                     double res, const orc_tape_rec* tape, int64_t ntape, double t_end,
+                     const double* x, const double* y, const double* vx, const double* vy,
+                     const double* tl, const double* e, double h, double lw, double lt,
+                     const double* lam);
+
 /* g(r) histogram of calculate_pcf (pcf.c:16-75).  Returns the bin count;
  * r and g must hold at least (int)(max_r/dr) values. */
 int orc_pcf(int n, const double* x, const double* y, double Lx, double Ly,

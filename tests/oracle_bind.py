@@ -65,8 +65,89 @@ def lib():
         L.orc_sync_all.argtypes = [C.c_void_p, C.c_double]
         L.orc_pcf.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double,
                               C.c_double, dp, dp]
+        L.orc_tape_enable.argtypes = [C.c_void_p, C.c_int64]
+        L.orc_tape_count.restype = C.c_int64
+        L.orc_tape_count.argtypes = [C.c_void_p]
+        L.orc_get_tape.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_tape_clear.argtypes = [C.c_void_p]
+        L.orc_tangent.restype = C.c_double
+        L.orc_tangent.argtypes = [C.c_int, dp, dp, C.c_void_p, C.c_int64, C.c_double, dp, dp]
+        L.orc_adjoint.argtypes = [C.c_int, dp, dp, C.c_void_p, C.c_int64, C.c_double,
+                                  C.c_double, C.c_double, C.c_void_p, dp]
+        L.orc_replay_fd.restype = C.c_double
+        L.orc_replay_fd.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp, C.c_double,
+                                    C.c_void_p, C.c_int64, C.c_double] + [dp] * 6 + [
+                                    C.c_double, C.c_double, C.c_double, C.c_void_p]
+        L.orc_tape_wsum.restype = C.c_double
+        L.orc_tape_wsum.argtypes = [C.c_void_p]
+        L.orc_replay.restype = C.c_double
+        L.orc_replay.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp, C.c_double,
+                                 C.c_void_p, C.c_int64, C.c_double] + [dp] * 5 + [dp]
         _lib = L
     return _lib
+
+
+TAPE_DT = np.dtype([("i", "<i4"), ("j", "<i4"), ("t", "<f8"), ("ax", "<f8"),
+                    ("ay", "<f8"), ("cx", "<f8"), ("cy", "<f8"), ("ff", "<f8")])
+
+
+def tangent(m, tl0, tape, t_end, tan):
+    """Propagates tan (n x 4: dx dy dvx dvy) in place.
+
+    Returns (d sum w, d t_last)."""
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    tl0 = np.ascontiguousarray(tl0, dtype=np.float64)
+    assert tan.dtype == np.float64 and tan.flags.c_contiguous
+    tape = np.ascontiguousarray(tape, dtype=TAPE_DT)
+    dtl = C.c_double()
+    dS = lib().orc_tangent(len(m), _dp(m), _dp(tl0), tape.ctypes.data, len(tape), t_end,
+                           _dp(tan), C.byref(dtl))
+    return dS, dtl.value
+
+
+def adjoint(m, tl0, tape, t_end, lw, lt=0.0, lam_end=None):
+    """Gradient (n x 4) of lw * sum w + lt * t_last + <lam_end, final state>
+    with respect to (x, y, vx, vy) at tl0."""
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    tl0 = np.ascontiguousarray(tl0, dtype=np.float64)
+    tape = np.ascontiguousarray(tape, dtype=TAPE_DT)
+    g = np.zeros((len(m), 4))
+    le = None
+    if lam_end is not None:
+        le = np.ascontiguousarray(lam_end, dtype=np.float64).reshape(len(m), 4)
+    lib().orc_adjoint(len(m), _dp(m), _dp(tl0), tape.ctypes.data, len(tape), t_end, lw, lt,
+                      le.ctypes.data if le is not None else None, _dp(g))
+    return g
+
+
+def replay_fd(st, tape, t_end, e, h=1e-7, lw=1.0, lt=0.0, lam=None, res=1.0):
+    """Extended-precision central difference of lw * sum w + lt * t_last +
+    <lam, final state> along e (n x 4) over the fixed-sequence replay."""
+    k = {key: np.ascontiguousarray(st[key], dtype=np.float64)
+         for key in ("x", "y", "vx", "vy", "t", "rad", "m")}
+    e = np.ascontiguousarray(e, dtype=np.float64).reshape(-1, 4)
+    tape = np.ascontiguousarray(tape, dtype=TAPE_DT)
+    la = None if lam is None else np.ascontiguousarray(lam, dtype=np.float64).reshape(-1, 4)
+    return lib().orc_replay_fd(len(k["m"]), st["Lx"], st["Ly"], _dp(k["rad"]), _dp(k["m"]),
+                               res, tape.ctypes.data, len(tape), t_end, _dp(k["x"]),
+                               _dp(k["y"]), _dp(k["vx"]), _dp(k["vy"]), _dp(k["t"]), _dp(e),
+                               h, lw, lt, la.ctypes.data if la is not None else None)
+
+
+def replay(st, tape, t_end, res=1.0):
+    """Fixed-sequence replay of tape from st (x y vx vy and t per disk).
+
+    Returns (sum w, time of the last collision, final state dict)."""
+    out = {k: np.array(st[k], dtype=np.float64, copy=True) for k in ("x", "y", "vx", "vy")}
+    tl = np.array(st["t"], dtype=np.float64, copy=True)
+    rad = np.ascontiguousarray(st["rad"], dtype=np.float64)
+    m = np.ascontiguousarray(st["m"], dtype=np.float64)
+    tape = np.ascontiguousarray(tape, dtype=TAPE_DT)
+    tk = C.c_double()
+    S = lib().orc_replay(len(m), st["Lx"], st["Ly"], _dp(rad), _dp(m), res,
+                         tape.ctypes.data, len(tape), t_end, _dp(out["x"]), _dp(out["y"]),
+                         _dp(out["vx"]), _dp(out["vy"]), _dp(tl), C.byref(tk))
+    return S, tk.value, out
 
 
 def _dp(a):
@@ -188,6 +269,21 @@ class Oracle:
 
     def sync_all(self, t):
         lib().orc_sync_all(self.h, t)
+
+    def tape_enable(self, capacity):
+        assert lib().orc_tape_enable(self.h, capacity) == 0
+
+    def tape(self):
+        k = lib().orc_tape_count(self.h)
+        out = np.zeros(k, dtype=TAPE_DT)
+        lib().orc_get_tape(self.h, out.ctypes.data)
+        return out
+
+    def tape_clear(self):
+        lib().orc_tape_clear(self.h)
+
+    def tape_wsum(self):
+        return lib().orc_tape_wsum(self.h)
 
 
 def pcf(x, y, Lx, Ly, dr, max_r):

@@ -1,0 +1,191 @@
+"""GPU derivatives of the event map (tape, tangent, adjoint, checkpointed
+recomputation) against the oracle, and the adjoint against finite
+differences through a torch.autograd.Function that exists only here.
+
+Tapes, tangents and gradients are compared bit for bit with the oracle's:
+the kernels perform the oracle's operations in the oracle's order.
+"""
+import numpy as np
+import pytest
+
+import edmd_fixtures as F
+from conftest import bits, golden_state, load_golden
+
+pytestmark = pytest.mark.gpu
+
+TAPE_FIELDS = ("t", "ax", "ay", "cx", "cy", "ff", "i", "j")
+NCOLL = 10000
+
+
+def states_mixed():
+    out = [F.lattice_state(400, 0.60, seed=200 + k, lattice="fit") for k in range(6)]
+    out.append(golden_state(load_golden("poly500_phi050")))
+    out.append(F.lattice_state(1024, 0.80, seed=5, lattice="tri"))
+    return out
+
+
+def oracle_taped(oracle, st, ncoll):
+    o = oracle.Oracle(st)
+    s0 = o.state()
+    o.tape_enable(ncoll + 16)
+    o.run_collisions(ncoll)
+    return o, s0, o.tape()
+
+
+def canonical_tape(tape):
+    """A mirrored pair event (i, j) / (j, i) ties exactly and either may be
+    handled; the record of (j, i) is that of (i, j) with a and c negated.
+    Orders every record as i < j."""
+    t = tape.copy()
+    sw = t["i"] > t["j"]
+    t["i"][sw], t["j"][sw] = tape["j"][sw], tape["i"][sw]
+    for f in ("ax", "ay", "cx", "cy"):
+        t[f][sw] = -tape[f][sw]
+    return t
+
+
+def tangent_inputs(states, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.standard_normal((len(st["x"]), 4)) for st in states]
+
+
+@pytest.fixture(scope="module")
+def full_tape(gpu, oracle):
+    states = states_mixed()
+    b = gpu.Batch(states)
+    b.deriv_begin(checkpoint_every=0, tape_capacity=NCOLL + 16)
+    b.deriv_run(max_collisions=NCOLL)
+    refs = [oracle_taped(oracle, st, NCOLL) for st in states]
+    return states, b, refs
+
+
+def test_tape_matches_oracle(full_tape):
+    states, b, refs = full_tape
+    for k, (o, s0, tape) in enumerate(refs):
+        got = canonical_tape(b.tape(k))
+        tape = canonical_tape(tape)
+        assert len(got) == len(tape) == NCOLL, k
+        for f in TAPE_FIELDS:
+            np.testing.assert_array_equal(got[f], tape[f], err_msg="system %d %s" % (k, f))
+        w, tl, nrec = b.deriv_values(k)
+        assert nrec == NCOLL
+        assert bits(w) == bits(o.tape_wsum())
+        assert tl == tape["t"][-1]
+
+
+def test_tangent_and_adjoint_match_oracle(full_tape, oracle):
+    states, b, refs = full_tape
+    tin = tangent_inputs(states, 1)
+    tout, dw, dtl = b.tangent(tin)
+    rng = np.random.default_rng(2)
+    lw = rng.uniform(0.5, 2.0, len(states))
+    lt = rng.uniform(-3.0, 3.0, len(states))
+    lam = [rng.standard_normal((len(st["x"]), 4)) for st in states]
+    grad = b.adjoint(lw, lt, lam)
+    for k, (st, (o, s0, tape)) in enumerate(zip(states, refs)):
+        te = o.t
+        tan = tin[k].copy()
+        rdw, rdtl = oracle.tangent(st["m"], s0["t"], tape, te, tan)
+        np.testing.assert_array_equal(bits(tout[k]), bits(tan), err_msg="tangent %d" % k)
+        assert bits(dw[k]) == bits(rdw) and bits(dtl[k]) == bits(rdtl), k
+        g = oracle.adjoint(st["m"], s0["t"], tape, te, lw[k], lt[k], lam[k])
+        np.testing.assert_array_equal(bits(grad[k]), bits(g), err_msg="adjoint %d" % k)
+
+
+@pytest.mark.parametrize("every", [256, 97])
+def test_checkpointed_recomputation_equals_full_tape(gpu, full_tape, every):
+    """Checkpoint every K events, recompute each segment's tape in the
+    passes: identical results, and the state is left as the run left it."""
+    states, bfull, refs = full_tape
+    b = gpu.Batch(states, trace_capacity=4000)
+    b.deriv_begin(checkpoint_every=every)
+    b.deriv_run(max_collisions=NCOLL)
+    before = [b.state(k) for k in range(len(states))]
+    tin = tangent_inputs(states, 1)
+    t_seg, dw_seg, dtl_seg = b.tangent(tin)
+    t_full, dw_full, dtl_full = bfull.tangent(tin)
+    lam = [np.full((len(st["x"]), 4), 0.25) for st in states]
+    g_seg = b.adjoint(1.5, -0.5, lam)
+    g_full = bfull.adjoint(1.5, -0.5, lam)
+    for k in range(len(states)):
+        np.testing.assert_array_equal(bits(t_seg[k]), bits(t_full[k]))
+        np.testing.assert_array_equal(bits(g_seg[k]), bits(g_full[k]))
+        assert bits(dw_seg[k]) == bits(dw_full[k]) and bits(dtl_seg[k]) == bits(dtl_full[k])
+        w, tl, _ = b.deriv_values(k)
+        wf, tlf, _ = bfull.deriv_values(k)
+        assert bits(w) == bits(wf) and tl == tlf
+        after = b.state(k)
+        for f in ("x", "y", "vx", "vy", "t", "coll"):
+            np.testing.assert_array_equal(after[f], before[k][f], err_msg="%d %s" % (k, f))
+    # the system goes on exactly as one that never took derivatives
+    b.deriv_end()
+    ref = gpu.Batch(states, trace_capacity=4000)
+    ref.run(max_collisions=NCOLL)
+    b.run(max_collisions=NCOLL + 2000)
+    ref.run(max_collisions=NCOLL + 2000)
+    for k in range(len(states)):
+        sb, sr = b.state(k), ref.state(k)
+        for f in ("x", "y", "vx", "vy", "t"):
+            np.testing.assert_array_equal(bits(sb[f]), bits(sr[f]))
+
+
+def test_adjoint_matches_finite_differences_through_autograd(gpu, oracle):
+    """The time-averaged collision virial W / t_last over 600 collisions of
+    N=400 at phi=0.60, as a torch.autograd.Function whose backward is the
+    GPU adjoint (checkpoint every 256 events); checked against central
+    differences (step 1e-7, extended precision) of the oracle's replay."""
+    torch = pytest.importorskip("torch")
+    st = F.lattice_state(400, 0.60, seed=31, lattice="fit")
+    ncoll = 600
+
+    class VirialAverage(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, x0):
+            s = dict(st)
+            a = x0.detach().cpu().numpy()
+            for c, key in enumerate(("x", "y", "vx", "vy")):
+                s[key] = a[:, c].copy()
+            b = gpu.Batch([s])
+            b.deriv_begin(checkpoint_every=256)
+            b.deriv_run(max_collisions=ncoll)
+            W, T, _ = b.deriv_values(0)
+            ctx.b, ctx.W, ctx.T = b, W, T
+            return torch.tensor(W / T, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gJ):
+            g = float(gJ)
+            grad = ctx.b.adjoint(lw=g / ctx.T, lt=-g * ctx.W / ctx.T ** 2)[0]
+            return torch.from_numpy(grad)
+
+    x0 = torch.tensor(np.stack([st["x"], st["y"], st["vx"], st["vy"]], axis=1),
+                      dtype=torch.float64, requires_grad=True)
+    J = VirialAverage.apply(x0)
+    J.backward()
+    grad = x0.grad.numpy()
+
+    o, s0, tape = oracle_taped(oracle, st, ncoll)
+    W, T = o.tape_wsum(), tape["t"][-1]
+    assert float(J.detach()) == W / T
+    s0.update(rad=st["rad"], m=st["m"], Lx=st["Lx"], Ly=st["Ly"])
+    rng = np.random.default_rng(8)
+    for _ in range(3):
+        e = rng.standard_normal((400, 4))
+        fd = oracle.replay_fd(s0, tape, o.t, e, 1e-7, 1 / T, -W / T ** 2)
+        ad = (grad * e).sum()
+        assert abs(fd - ad) <= 1e-6 * abs(ad)
+
+
+def test_tape_capacity_is_enforced(gpu):
+    b = gpu.Batch([F.lattice_state(400, 0.6, seed=1, lattice="fit")])
+    b.deriv_begin(checkpoint_every=0, tape_capacity=100)
+    with pytest.raises(gpu.EdmdError) as e:
+        b.deriv_run(max_collisions=1000)
+    assert e.value.code == 8
+
+
+def test_thermostatted_systems_are_rejected(gpu):
+    b = gpu.Batch([F.lattice_state(400, 0.6, seed=1, lattice="fit")], noise=2)
+    with pytest.raises(gpu.EdmdError) as e:
+        b.deriv_begin()
+    assert e.value.code == 22
