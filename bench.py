@@ -16,6 +16,12 @@ Reported beside it, in the same run:
   sweep        the full prediction sweep of that system (kernel k_sweep):
                GB/s of algorithmic traffic, 80 B per disk, against the
                MI355X HBM peak (the `roofline` object);
+  adjoint      derivatives of the event map: 512 replicas of N = 400 at
+               phi = 0.60 (seeds base + id), 10^4 collisions each, checkpoint
+               every 256 events: forward run with checkpoints, adjoint of the
+               time-averaged collision virial with recomputation of every
+               segment, tangent pass (finite-time growth of a random unit
+               tangent), and the same with the whole tape kept in HBM;
   cpu_baseline the reference's own CPU event loop (built with its shipped
                flags) on a bounded sample of replicas, one core.
 """
@@ -94,6 +100,46 @@ def cpu_baseline(E, ns, phis, base_seed, coll, seconds):
                       "%d collisions each, one process" % (k, coll)}
 
 
+def adjoint_leg(E, args):
+    """Forward run with checkpoints, adjoint and tangent passes with
+    recomputation, then the whole-tape variant, on the same replicas."""
+    nrep, n, ncoll = args.adj_replicas, args.adj_n, args.adj_collisions
+    states = [E.init_lattice(n, args.adj_phi, rad=0.5, lattice=2, seed=args.seed + k)
+              for k in range(nrep)]
+    rng = np.random.default_rng(args.seed)
+    tin = []
+    for _ in range(nrep):
+        v = rng.standard_normal((n, 4))
+        tin.append(v / np.linalg.norm(v))
+    res = {"workload": "%d replicas, N=%d, phi=%.2f, %d collisions each"
+                       % (nrep, n, args.adj_phi, ncoll),
+           "checkpoint_every": args.adj_every}
+    for mode, every, cap in (("checkpointed", args.adj_every, 0), ("whole_tape", 0, ncoll)):
+        b = E.Batch(states)
+        b.deriv_begin(checkpoint_every=every, tape_capacity=cap)
+        t0 = time.perf_counter()
+        b.deriv_run(max_collisions=ncoll, max_events=1 << 62)
+        t1 = time.perf_counter()
+        W = np.array([b.deriv_values(k)[0] for k in range(nrep)])
+        T = np.array([b.deriv_values(k)[1] for k in range(nrep)])
+        t2 = time.perf_counter()
+        grad = b.adjoint(lw=1.0 / T, lt=-W / T ** 2)
+        t3 = time.perf_counter()
+        tout, dw, dtl = b.tangent(tin)
+        t4 = time.perf_counter()
+        growth = np.array([np.log(np.linalg.norm(v)) for v in tout]) / T
+        res[mode] = {"forward_s": t1 - t0, "adjoint_s": t3 - t2, "tangent_s": t4 - t3,
+                     "collisions_per_s_forward": nrep * ncoll / (t1 - t0),
+                     "collisions_per_s_forward_plus_adjoint": nrep * ncoll / (t1 - t0 + t3 - t2)}
+        log("adjoint/%s: forward %.3f s, adjoint %.3f s, tangent %.3f s"
+            % (mode, t1 - t0, t3 - t2, t4 - t3))
+        res["virial_time_average_mean"] = float(np.mean(W / T))
+        res["tangent_growth_rate_mean"] = float(np.mean(growth))
+        res["gradient_norm_mean"] = float(np.mean([np.linalg.norm(g) for g in grad]))
+        b.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +154,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-collisions", type=int, default=100000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--adj-replicas", type=int, default=512)
+    ap.add_argument("--adj-n", type=int, default=400)
+    ap.add_argument("--adj-phi", type=float, default=0.60)
+    ap.add_argument("--adj-collisions", type=int, default=10000)
+    ap.add_argument("--adj-every", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; replicas would shard across GPUs with no "
@@ -163,6 +214,10 @@ def main():
     log("plain: %d collisions in %.3f s = %.4e coll/s" % (o1["ncol"] - o0["ncol"], t1 - t0, plain))
     big.close()
 
+    adj = None
+    if args.adj_replicas > 0:
+        adj = adjoint_leg(E, args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(E, ns, phis, args.seed, args.cpu_collisions, args.cpu_seconds)
@@ -197,6 +252,7 @@ def main():
         "roofline": {"kernel": "k_sweep", "bound": "hbm", "achieved": sweep_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS,
                      "traffic": None},
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
