@@ -79,7 +79,7 @@ struct edmd_batch {
     EdmdSys* d_sys = nullptr;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    int max_n = 0, max_nlev = 0;
+    int max_n = 0, max_nlev = 0, max_tiles = 0;
     int max_lsize[EDMD_MAX_LEVELS + 1] = {0};
     size_t np = 0, ncell = 0, nslot = 0, nnode = 0, ntrace = 0, nthermo = 0;
     std::vector<void*> allocs;
@@ -286,7 +286,7 @@ static int upload_system(edmd_batch* b, int s, const edmd_system_desc* d) {
 }
 
 static int batch_predict_all(edmd_batch* b) {
-    HIPCHK(edmd_launch_sweep(b->d_sys, b->nsys, b->max_n, b->stream));
+    HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->stream));
     HIPCHK(edmd_launch_tree(b->d_sys, b->nsys, b->max_lsize, b->max_nlev, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     return EDMD_OK;
@@ -336,6 +336,7 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
         h.thermo0 = b->nthermo;
         b->nthermo += h.c.thermo_cap;
         b->max_n = std::max(b->max_n, h.c.n);
+        b->max_tiles = std::max(b->max_tiles, edmd_sweep_tiles(h.c.ncx, h.c.ncy));
         b->max_nlev = std::max(b->max_nlev, h.c.nlev);
         for (int l = 0; l <= h.c.nlev; l++) b->max_lsize[l] = std::max(b->max_lsize[l], h.c.lsize[l]);
     }
@@ -449,7 +450,7 @@ extern "C" int edmd_batch_time_sweep(edmd_batch* b, int reps, double* avg_ms) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, b->stream));
-    for (int r = 0; r < reps; r++) HIPCHK(edmd_launch_sweep(b->d_sys, b->nsys, b->max_n, b->stream));
+    for (int r = 0; r < reps; r++) HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->stream));
     HIPCHK(hipEventRecord(e1, b->stream));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;

@@ -7,7 +7,8 @@
 
 #include "edmd_types.h"
 
-hipError_t edmd_launch_sweep(const EdmdSys* d_sys, int nsys, int max_n, hipStream_t s);
+hipError_t edmd_launch_sweep_tiled(const EdmdSys* d_sys, int nsys, int max_tiles, hipStream_t s);
+int edmd_sweep_tiles(int ncx, int ncy);
 hipError_t edmd_launch_tree(const EdmdSys* d_sys, int nsys, const int* max_lsize, int max_nlev,
                             hipStream_t s);
 hipError_t edmd_launch_sync_all(const EdmdSys* d_sys, int nsys, int max_n, hipStream_t s);

@@ -1,11 +1,12 @@
 /*
  * HIP kernels of libedmd_hip.so for gfx950 (CDNA4, 64-lane wavefronts).
  *
- *   k_sweep        prediction sweep: one lane per particle computes the
- *                  next cell crossing and the earliest collision partner over
- *                  the 3x3 cell neighbourhood (EDMD.c:2405-2482, 2661-2723,
- *                  2829-3003; invoked for all particles at EDMD.c:2007-2012
- *                  and 4909-4915)
+ *   k_sweep_tiled  prediction sweep: one workgroup per 8x8-cell tile stages
+ *                  the tile and its halo in LDS; one lane per particle
+ *                  computes the next cell crossing and the earliest
+ *                  collision partner over the 3x3 cell neighbourhood
+ *                  (EDMD.c:2405-2482, 2661-2723, 2829-3003; invoked for all
+ *                  particles at EDMD.c:2007-2012 and 4909-4915)
  *   k_tree_level   builds one level of the 64-ary minimum calendar
  *   k_run          persistent event loop: one wavefront per system pops the
  *                  earliest event, applies the reference's handlers
@@ -88,15 +89,16 @@ __device__ __forceinline__ void fly(const EdmdConst& C, double t, double& x, dou
 
 /* collisionTimeNormal, EDMD.c:2661-2723: relative time until contact of the
  * synchronised owner (x1..) with candidate 2 last updated at tl2. */
-__device__ __forceinline__ double coll_time(const EdmdConst& C, double t, double x1, double y1,
-                                            double vx1, double vy1, double r1, double x2,
-                                            double y2, double vx2, double vy2, double tl2,
-                                            double r2, bool& overlap) {
-    double lat2 = t - tl2;
+/* The same with candidate 2's position already flown to t:
+ * xt2 = x2 + (t - tl2) * vx2, yt2 likewise (EDMD.c:2667-2671). */
+__device__ __forceinline__ double coll_time_at(const EdmdConst& C, double x1, double y1,
+                                               double vx1, double vy1, double r1, double xt2,
+                                               double yt2, double vx2, double vy2, double r2,
+                                               bool& overlap) {
     double dvx = vx2 - vx1;
     double dvy = vy2 - vy1;
-    double dx = (x2 + lat2 * vx2) - x1;
-    double dy = (y2 + lat2 * vy2) - y1;
+    double dx = xt2 - x1;
+    double dy = yt2 - y1;
     min_image(C, dx, dy);
     double b = dx * dvx + dy * dvy;
     if (b > 0) return EDMD_NEVER;
@@ -106,6 +108,15 @@ __device__ __forceinline__ double coll_time(const EdmdConst& C, double t, double
     if (d < -0.01) overlap = true; /* EDMD.c:2708-2717 */
     if (det < 0) return EDMD_NEVER;
     return (-b - sqrt(det)) / v2;
+}
+
+__device__ __forceinline__ double coll_time(const EdmdConst& C, double t, double x1, double y1,
+                                            double vx1, double vy1, double r1, double x2,
+                                            double y2, double vx2, double vy2, double tl2,
+                                            double r2, bool& overlap) {
+    double lat2 = t - tl2;
+    return coll_time_at(C, x1, y1, vx1, vy1, r1, x2 + lat2 * vx2, y2 + lat2 * vy2, vx2, vy2, r2,
+                        overlap);
 }
 
 /* crossingEventNormal, EDMD.c:2405-2482: absolute crossing time and
@@ -478,14 +489,210 @@ __device__ __forceinline__ bool sweep_one(const EdmdConst& C, const EdmdArrays& 
     return ov;
 }
 
-__global__ __launch_bounds__(256) void k_sweep(const EdmdSys* __restrict__ systems) {
+/* Cell-tiled sweep: one workgroup per SW_TX x SW_TY cells.  The tile and a
+ * one-cell halo (wrapped periodically) are staged into LDS as SoA, the
+ * tile's own disks first; each thread then predicts one disk of the tile
+ * from LDS, scanning its 3x3 cells in the reference's order (y offset
+ * outer, x offset inner, newest list entry first, EDMD.c:2959-3003) with
+ * the strict-less rule, so the result is that of predict_seq bit for bit.
+ * Every disk record is read from HBM by the one tile that owns it and the
+ * (at most eight) tiles whose halo holds it, instead of once per neighbour
+ * scan that touches it.  Workgroups are dealt to the eight XCDs round
+ * robin; tiles are renumbered so that each XCD sweeps a contiguous band of
+ * tile rows and halo re-reads hit its own L2.  A tile whose staged disks
+ * exceed SW_STAGE falls back to predict_seq from global memory. */
+#ifndef SW_TX
+#define SW_TX 8
+#endif
+#ifndef SW_TY
+#define SW_TY 8
+#endif
+#ifndef SW_STAGE
+#define SW_STAGE 512
+#endif
+#ifndef SW_THREADS
+#define SW_THREADS 256
+#endif
+#define SW_HW (SW_TX + 2)
+#define SW_HH (SW_TY + 2)
+#define SW_NH (SW_HW * SW_HH)
+static_assert(SW_NH <= 128 && SW_NH <= 255, "staged cells: two per lane of wave 0, u8 index");
+static_assert(SW_THREADS >= 64 && SW_THREADS % 64 == 0, "whole waves");
+
+struct SweepLds {
+    /* staged disks: position flown to the sweep time, velocity, radius */
+    double xt[SW_STAGE], yt[SW_STAGE], vx[SW_STAGE], vy[SW_STAGE], rad[SW_STAGE];
+    int32_t id[SW_STAGE];
+    uint8_t hcell[SW_STAGE]; /* staged cell of every staged disk */
+    int32_t cnt[SW_NH];
+    int32_t start[SW_NH];
+    int32_t gcell[SW_NH];
+    int32_t ninner, total;
+};
+
+__device__ __forceinline__ int mod_cell(int a, int nc) {
+    int r = a % nc;
+    return r < 0 ? r + nc : r;
+}
+
+/* Exclusive scan of v over the SW_NH staged cells by wave 0 (two cells per
+ * lane); returns the total.  Cells are ordered h = (ly + 1) * SW_HW + lx + 1. */
+__device__ __forceinline__ int sweep_scan(int lane, const int* v, int* out, int base) {
+    int a = (2 * lane < SW_NH) ? v[2 * lane] : 0;
+    int b = (2 * lane + 1 < SW_NH) ? v[2 * lane + 1] : 0;
+    int incl = a + b;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int u = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += u;
+    }
+    const int excl = incl - a - b;
+    if (2 * lane < SW_NH) out[2 * lane] = base + excl;
+    if (2 * lane + 1 < SW_NH) out[2 * lane + 1] = base + excl + a;
+    return __shfl(incl, 63, 64);
+}
+
+__global__ __launch_bounds__(SW_THREADS) void k_sweep_tiled(const EdmdSys* __restrict__ systems) {
+    __shared__ SweepLds L;
+    __shared__ int vtmp[SW_NH];
     const EdmdSys* S = systems + blockIdx.y;
-    const int n = S->c.n;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const double t = S->a.st->t;
-    bool ov = sweep_one(S->c, S->a, t, p);
-    if (ov) atomicExch(&S->a.st->status, EDMD_DEV_OVERLAP);
+    const EdmdConst& C = S->c;
+    const EdmdArrays& A = S->a;
+    const int tid = threadIdx.x;
+    const int ncx = C.ncx, ncy = C.ncy;
+    const int ntx = (ncx + SW_TX - 1) / SW_TX, nty = (ncy + SW_TY - 1) / SW_TY;
+    const int per = gridDim.x / 8; /* gridDim.x is a multiple of 8 */
+    const int tile = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (tile >= ntx * nty) return; /* uniform over the workgroup */
+    const int tx0 = (tile % ntx) * SW_TX, ty0 = (tile / ntx) * SW_TY;
+    const int vw = min(SW_TX, ncx - tx0), vh = min(SW_TY, ncy - ty0);
+    const double t = A.st->t;
+
+    for (int h = tid; h < SW_NH; h += SW_THREADS) {
+        const int lx = h % SW_HW - 1, ly = h / SW_HW - 1;
+        const int g = mod_cell(ty0 + ly, ncy) * ncx + mod_cell(tx0 + lx, ncx);
+        L.gcell[h] = g;
+        L.cnt[h] = A.ccnt[g];
+    }
+    __syncthreads();
+    /* tile cells first, then the halo ring and clipped cells */
+    if (tid < 64) {
+        for (int h = 2 * tid; h < 2 * tid + 2 && h < SW_NH; ++h) {
+            const int lx = h % SW_HW - 1, ly = h / SW_HW - 1;
+            const bool own = lx >= 0 && lx < vw && ly >= 0 && ly < vh;
+            vtmp[h] = own ? L.cnt[h] : 0;
+        }
+        const int nin = sweep_scan(tid, vtmp, vtmp, 0);
+        int other[2] = {0, 0};
+        int sown[2] = {0, 0};
+        for (int k = 0; k < 2; ++k) {
+            const int h = 2 * tid + k;
+            if (h < SW_NH) {
+                const int lx = h % SW_HW - 1, ly = h / SW_HW - 1;
+                const bool own = lx >= 0 && lx < vw && ly >= 0 && ly < vh;
+                sown[k] = own ? vtmp[h] : -1;
+                other[k] = own ? 0 : L.cnt[h];
+            }
+        }
+        for (int k = 0; k < 2; ++k)
+            if (2 * tid + k < SW_NH) vtmp[2 * tid + k] = other[k];
+        const int nout = sweep_scan(tid, vtmp, vtmp, nin);
+        for (int k = 0; k < 2; ++k) {
+            const int h = 2 * tid + k;
+            if (h < SW_NH) L.start[h] = sown[k] >= 0 ? sown[k] : vtmp[h];
+        }
+        if (tid == 0) {
+            L.ninner = nin;
+            L.total = nin + nout;
+        }
+    }
+    __syncthreads();
+    const int total = L.total, ninner = L.ninner;
+    bool ov = false;
+    if (total <= SW_STAGE) {
+        const int cap = C.cap;
+        for (int h = tid; h < SW_NH; h += SW_THREADS) {
+            const int st0 = L.start[h], c = L.cnt[h];
+            for (int sl = 0; sl < c; ++sl) L.hcell[st0 + sl] = (uint8_t)h;
+        }
+        __syncthreads();
+        /* two staged disks per thread and pass: both slot reads, then both
+         * record reads, are in flight together */
+        for (int u0 = tid; u0 < total; u0 += 2 * SW_THREADS) {
+            const int u1 = u0 + SW_THREADS;
+            const int h0 = L.hcell[u0];
+            const int q0 = A.cslot[(size_t)L.gcell[h0] * cap + (u0 - L.start[h0])];
+            int q1 = -1;
+            if (u1 < total) {
+                const int h1 = L.hcell[u1];
+                q1 = A.cslot[(size_t)L.gcell[h1] * cap + (u1 - L.start[h1])];
+            }
+            const EdmdPart r0 = A.p[q0];
+            EdmdPart r1;
+            if (q1 >= 0) r1 = A.p[q1];
+            const double lat0 = t - r0.tl;
+            L.xt[u0] = r0.x + lat0 * r0.vx; L.yt[u0] = r0.y + lat0 * r0.vy;
+            L.vx[u0] = r0.vx; L.vy[u0] = r0.vy; L.rad[u0] = r0.rad; L.id[u0] = q0;
+            if (q1 >= 0) {
+                const double lat1 = t - r1.tl;
+                L.xt[u1] = r1.x + lat1 * r1.vx; L.yt[u1] = r1.y + lat1 * r1.vy;
+                L.vx[u1] = r1.vx; L.vy[u1] = r1.vy; L.rad[u1] = r1.rad; L.id[u1] = q1;
+            }
+        }
+        __syncthreads();
+        for (int u = tid; u < ninner; u += SW_THREADS) {
+            const int p = L.id[u];
+            const int h = L.hcell[u];
+            const int lx = h % SW_HW - 1, ly = h / SW_HW - 1;
+            /* the owner is synchronised at t (its staged position is its own) */
+            const double ox = L.xt[u], oy = L.yt[u], ovx = L.vx[u], ovy = L.vy[u], orad = L.rad[u];
+            double tc;
+            int dir;
+            cross_time(C, t, ox, oy, ovx, ovy, tx0 + lx, ty0 + ly, tc, dir);
+            double dt = EDMD_NEVER;
+            int best = -1;
+            for (int j = -1; j <= 1; ++j) {
+                for (int k = -1; k <= 1; ++k) {
+                    const int hh = (ly + 1 + j) * SW_HW + (lx + 1 + k);
+                    const int st0 = L.start[hh];
+                    for (int v = st0 + L.cnt[hh] - 1; v >= st0; --v) {
+                        if (L.id[v] == p) continue;
+                        double tt = coll_time_at(C, ox, oy, ovx, ovy, orad, L.xt[v], L.yt[v],
+                                                 L.vx[v], L.vy[v], L.rad[v], ov);
+                        if (dt > tt) {
+                            dt = tt;
+                            best = v;
+                        }
+                    }
+                }
+            }
+            const double tk = t + dt;
+            const int partner = best >= 0 ? L.id[best] : 0;
+            EdmdEv e;
+            e.tcoll = tk;
+            e.tcross = tc;
+            e.partner = partner;
+            e.cact = A.p[partner].coll;
+            e.dir = dir;
+            e.pad = 0;
+            A.e[p] = e;
+            EdmdNode n;
+            n.key = leaf_key(tk, tc);
+            n.idx = p;
+            n.aux = leaf_aux(tk, tc, partner, dir);
+            A.leaf[p] = n;
+        }
+    } else {
+        const int cap = C.cap;
+        for (int w = tid; w < SW_TX * SW_TY * cap; w += SW_THREADS) {
+            const int c = w / cap, sl = w - c * cap;
+            const int lx = c % SW_TX, ly = c / SW_TX;
+            if (lx >= vw || ly >= vh) continue;
+            const int g = (ty0 + ly) * ncx + tx0 + lx;
+            if (sl < A.ccnt[g]) ov |= sweep_one(C, A, t, A.cslot[(size_t)g * cap + sl]);
+        }
+    }
+    if (ov) atomicExch(&A.st->status, EDMD_DEV_OVERLAP);
 }
 
 __global__ __launch_bounds__(256) void k_tree_level(const EdmdSys* __restrict__ systems, int l) {
@@ -862,7 +1069,7 @@ __device__ void ev_crossing(const EdmdConst& C, const EdmdArrays& A, int lane, L
 /* Persistent event loop, one wavefront per system (blockIdx.x); ML bounds
  * the calendar depth of every system in the launch. */
 template <int ML>
-__global__ __launch_bounds__(64) void k_run(const EdmdSys* __restrict__ systems, EdmdRunArgs ra) {
+__device__ __forceinline__ void run_loop(const EdmdSys* __restrict__ systems, EdmdRunArgs ra) {
     const EdmdConst& C = systems[blockIdx.x].c;
     const EdmdArrays& A = systems[blockIdx.x].a;
     const int lane = threadIdx.x;
@@ -927,6 +1134,22 @@ __global__ __launch_bounds__(64) void k_run(const EdmdSys* __restrict__ systems,
         st->stop_reason = reason;
         st->last_type = last_type;
     }
+}
+
+/* Systems of up to 64^2 disks (calendar depth <= 2): bounded to 128 VGPRs so
+ * that four waves share each SIMD and a batch of 4096 replicas is resident
+ * at once on 256 CUs. */
+template <int ML>
+__global__ __launch_bounds__(64, 4) void k_run_small(const EdmdSys* __restrict__ systems,
+                                                     EdmdRunArgs ra) {
+    run_loop<ML>(systems, ra);
+}
+
+/* Deeper calendars (large systems, typically one per launch): no register
+ * bound, so the path caches stay in registers. */
+template <int ML>
+__global__ __launch_bounds__(64) void k_run(const EdmdSys* __restrict__ systems, EdmdRunArgs ra) {
+    run_loop<ML>(systems, ra);
 }
 
 /* ------------------------------------------------------------ single steps */
@@ -1084,11 +1307,15 @@ __global__ void k_fp64_check(int n, const double* __restrict__ a, const double* 
 
 /* ------------------------------------------------------------ launchers */
 
-hipError_t edmd_launch_sweep(const EdmdSys* d_sys, int nsys, int max_n, hipStream_t s) {
-    if (nsys <= 0 || max_n <= 0) return hipSuccess;
-    dim3 grid((max_n + 255) / 256, nsys);
-    hipLaunchKernelGGL(k_sweep, grid, dim3(256), 0, s, d_sys);
+hipError_t edmd_launch_sweep_tiled(const EdmdSys* d_sys, int nsys, int max_tiles, hipStream_t s) {
+    if (nsys <= 0 || max_tiles <= 0) return hipSuccess;
+    dim3 grid((max_tiles + 7) / 8 * 8, nsys);
+    hipLaunchKernelGGL(k_sweep_tiled, grid, dim3(SW_THREADS), 0, s, d_sys);
     return hipGetLastError();
+}
+
+int edmd_sweep_tiles(int ncx, int ncy) {
+    return ((ncx + SW_TX - 1) / SW_TX) * ((ncy + SW_TY - 1) / SW_TY);
 }
 
 hipError_t edmd_launch_tree(const EdmdSys* d_sys, int nsys, const int* max_lsize, int max_nlev,
@@ -1110,8 +1337,8 @@ hipError_t edmd_launch_sync_all(const EdmdSys* d_sys, int nsys, int max_n, hipSt
 
 hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, EdmdRunArgs ra,
                            hipStream_t s) {
-    if (max_nlev <= 1) hipLaunchKernelGGL(k_run<1>, dim3(nsys), dim3(64), 0, s, d_sys, ra);
-    else if (max_nlev == 2) hipLaunchKernelGGL(k_run<2>, dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    if (max_nlev <= 1) hipLaunchKernelGGL(k_run_small<1>, dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    else if (max_nlev == 2) hipLaunchKernelGGL(k_run_small<2>, dim3(nsys), dim3(64), 0, s, d_sys, ra);
     else if (max_nlev <= 4) hipLaunchKernelGGL(k_run<4>, dim3(nsys), dim3(64), 0, s, d_sys, ra);
     else hipLaunchKernelGGL(k_run<EDMD_MAX_LEVELS>, dim3(nsys), dim3(64), 0, s, d_sys, ra);
     return hipGetLastError();

@@ -199,6 +199,37 @@ def test_full_size_sweep_and_run(gpu, oracle):
     assert abs(og["px"]) < 1e-9 and abs(og["py"]) < 1e-9
 
 
+@pytest.mark.parametrize("rad", [0.5, 0.2])
+def test_tiled_sweep_matches_oracle_after_events(gpu, oracle, rad):
+    """The cell-tiled sweep from a state whose disks carry different local
+    times and cell lists in event order: rad 0.5 stages every tile in LDS,
+    rad 0.2 (about 25 disks per cell) overflows the LDS stage and takes the
+    global-memory path."""
+    st = F.lattice_state(6400, 0.70, seed=77, rad=rad, lattice="fit")
+    s = gpu.System(st)
+    o = oracle.Oracle(st)
+    eg, eo = s.events(), o.events()
+    for k in ("t_cross", "t_coll"):
+        np.testing.assert_array_equal(bits(eg[k]), bits(eo[k]), err_msg=k)
+    np.testing.assert_array_equal(eg["partner"], eo["partner"])
+    s.run(max_events=30000)
+    code, _ = o.run(30000, trace=False)
+    assert code == 0
+    t = o.t
+    o.sync_all(t)
+    s.sweep_predict()
+    eg = s.events()
+    so, sg = o.state(), s.state()
+    for k in ("x", "y", "vx", "vy"):
+        np.testing.assert_array_equal(bits(sg[k]), bits(so[k]), err_msg=k)
+    # the oracle's re-prediction of every disk at t, in the sweep's form
+    for i in range(0, len(st["x"]), 97):
+        tc, d = o.predict_crossing(i)
+        tk, p = o.predict_collision(i)
+        assert bits(eg["t_cross"][i]) == bits(tc) and eg["dir"][i] == d, i
+        assert bits(eg["t_coll"][i]) == bits(tk) and eg["partner"][i] == p, i
+
+
 def test_thermostat_rescale_matches_oracle(gpu, oracle):
     """Velocity-rescale thermostat events (EDMD.c:4828-4923) every 0.05: the
     trace through ~30 rescales and full re-predictions matches the oracle,
