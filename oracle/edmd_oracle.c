@@ -40,7 +40,8 @@ struct orc_ctx {
     /* cell lists: head per cell, doubly linked through particles (EDMD.c:2053-2095) */
     int *head, *nxt, *prv;
     /* event node pool: [0,N) crossings, [N,2N) collisions, 2N root,
-     * 2N+1 screenshot, 2N+2 noise, 2N+3 thermo (EDMD.c:1935-1948, 4421-4440, 4814-4822) */
+     * 2N+1 screenshot, 2N+2 noise, 2N+3 thermo, 2N+5 growth stop
+     * (EDMD.c:1935-1948, 4421-4440, 4802-4808, 4814-4822) */
     ev_node* ev;
     int root, tree_min;
     double paul_time, dt_paul;
@@ -56,6 +57,9 @@ struct orc_ctx {
     orc_tape_rec* tape;
     int64_t ntape, captape;
     double wsum;
+    /* growth start: per-disk radius growth rate, set while growing */
+    double* vr;
+    int growing;
 };
 
 /* ------------------------------------------------------------ kinematics */
@@ -90,6 +94,23 @@ static void free_fly(orc_ctx* s, int p) {
     s->y[p] += dt * s->vy[p];
     s->x[p] = wrap_post(s->x[p], s->Lx);
     s->y[p] = wrap_post(s->y[p], s->Ly);
+}
+
+/* freeFlyGrow, EDMD.c:4992-5007: the radius grows with the flight. */
+static void free_fly_grow(orc_ctx* s, int p) {
+    double dt = s->t - s->tl[p];
+    s->tl[p] = s->t;
+    s->x[p] += dt * s->vx[p];
+    s->y[p] += dt * s->vy[p];
+    s->rad[p] += dt * s->vr[p];
+    s->x[p] = wrap_post(s->x[p], s->Lx);
+    s->y[p] = wrap_post(s->y[p], s->Ly);
+}
+
+/* The freeFly dispatch pointer (EDMD.c:1979, 4757). */
+static void fly(orc_ctx* s, int p) {
+    if (s->growing) free_fly_grow(s, p);
+    else free_fly(s, p);
 }
 
 /* DIST2, EDMD.c:35-43 (non-additive geometric mean contact). */
@@ -277,7 +298,65 @@ static double coll_time(orc_ctx* s, int i, int j) {
     return (-b - sqrt(det)) / v2;
 }
 
+static double coll_time_grow(orc_ctx* s, int i, int j) {
+    /* collisionTimeGrow, EDMD.c:2598-2659 (2D): contact distance
+     * DIST(r_i, r_j(t)) growing at vr_i + vr_j; the earlier positive root,
+     * the later one under the reference's tolerance rule, overlap abort. */
+    double lat2 = s->t - s->tl[j];
+    double dvx = s->vx[j] - s->vx[i];
+    double dvy = s->vy[j] - s->vy[i];
+    double dvr = s->vr[i] + s->vr[j];
+    double dx = (s->x[j] + lat2 * s->vx[j]) - s->x[i];
+    double dy = (s->y[j] + lat2 * s->vy[j]) - s->y[i];
+    double dr = sqrt(4 * s->rad[i] * (s->rad[j] + lat2 * s->vr[j]));
+    min_image(s, &dx, &dy);
+    double b = dx * dvx + dy * dvy - dvr * dr;
+    double v2 = dvx * dvx + dvy * dvy;
+    double dist2 = dx * dx + dy * dy;
+    double det = b * b - (v2 - dvr * dvr) * (dist2 - dr * dr);
+    if (det < 0) return NEVER;
+    double plus = (-b + sqrt(det)) / (v2 - dvr * dvr);
+    double minus = (-b - sqrt(det)) / (v2 - dvr * dvr);
+    if (((minus > 0) && (plus > 0) && (minus < plus)) || ((minus > 0) && (plus < 0)))
+        return minus;
+    else if (((minus > 0) && (plus > 0.000000001) && (plus < minus)) ||
+             ((minus < 0) && (plus > 0.000000001)))
+        return plus;
+    if (dist2 - dr * dr < -0.01) {
+        /* growth overlap abort, EDMD.c:2653-2656 */
+        if (s->status == ORC_OK) s->status = ORC_ERR_OVERLAP;
+    }
+    return NEVER;
+}
+
+static void predict_collision_grow(orc_ctx* s, int i, double* t_abs, int* partner) {
+    /* collisionEventGrow, EDMD.c:3104-3230 without walls: the minimum
+     * starts at 1e7, not never, and is scanned as in the Normal version. */
+    int best = 0;
+    double dt = 10000000;
+    int X = s->cx[i], Y = s->cy[i];
+    for (int j = -1; j <= 1; j++) {
+        for (int k = -1; k <= 1; k++) {
+            int c = wrap_cell(Y + j, s->ncy) * s->ncx + wrap_cell(X + k, s->ncx);
+            for (int p = s->head[c]; p != NIL; p = s->nxt[p]) {
+                if (p == i) continue;
+                double dtt = coll_time_grow(s, i, p);
+                if (dt > dtt) {
+                    best = p;
+                    dt = dtt;
+                }
+            }
+        }
+    }
+    *t_abs = s->t + dt;
+    *partner = best;
+}
+
 static void predict_collision(orc_ctx* s, int i, double* t_abs, int* partner) {
+    if (s->growing) {
+        predict_collision_grow(s, i, t_abs, partner);
+        return;
+    }
     /* collisionEventNormal, EDMD.c:2829-3003, 3091-3093: strict-less minimum
      * over the 3x3 cells, y offset outer, x offset inner, list order. */
     int best = 0;
@@ -376,8 +455,52 @@ static void physical_q(orc_ctx* s) {
     }
 }
 
+static void do_collision_grow(orc_ctx* s, const ev_node* ev) {
+    /* doTheCollisionGrow, EDMD.c:3465-3540 (2D): unit contact normal, the
+     * growth speed enters the impulse; res and delta are the compile-time
+     * 1 and 0 of EDMD.c:3467-3468; no virial. */
+    const double delta = 0.1 * 0;
+    const double res = 0.75 * 0 + 1;
+    int i = ev->i, j = ev->j;
+    free_fly_grow(s, i);
+    if (ev->coll_actual != s->coll[j]) {
+        schedule_collision(s, i);
+        return;
+    }
+    s->ncol++;
+    free_fly_grow(s, j);
+    s->coll[i]++;
+    s->coll[j]++;
+    double dx = s->x[j] - s->x[i];
+    double dy = s->y[j] - s->y[i];
+    double dvx = s->vx[j] - s->vx[i];
+    double dvy = s->vy[j] - s->vy[i];
+    double dvr = s->vr[i] + s->vr[j];
+    min_image(s, &dx, &dy);
+    double dist = sqrt(dx * dx + dy * dy);
+    double dxr = dx / dist;
+    double dyr = dy / dist;
+    double inv_mass = 1 / (s->m[i] + s->m[j]);
+    double kernel = inv_mass * (1 + res) * (dxr * dvx + dyr * dvy - dvr);
+    s->vx[i] += kernel * s->m[j] * dxr - 2 * s->m[j] * inv_mass * delta * dxr;
+    s->vy[i] += kernel * s->m[j] * dyr - 2 * s->m[j] * inv_mass * delta * dyr;
+    s->vx[j] -= kernel * s->m[i] * dxr - 2 * s->m[i] * inv_mass * delta * dxr;
+    s->vy[j] -= kernel * s->m[i] * dyr - 2 * s->m[i] * inv_mass * delta * dyr;
+    queue_remove(s, i);
+    queue_remove(s, j);
+    schedule_crossing(s, i);
+    schedule_crossing(s, j);
+    queue_remove(s, s->n + j);
+    schedule_collision(s, i);
+    schedule_collision(s, j);
+}
+
 static void do_collision(orc_ctx* s, const ev_node* ev) {
     /* doTheCollisionNormal, EDMD.c:3542-3562, 3583-3606, 3802-3827, 3869-3879 */
+    if (s->growing) {
+        do_collision_grow(s, ev);
+        return;
+    }
     int i = ev->i, j = ev->j;
     free_fly(s, i);
     if (ev->coll_actual != s->coll[j]) {
@@ -430,9 +553,10 @@ static void do_collision(orc_ctx* s, const ev_node* ev) {
 }
 
 static void do_crossing(orc_ctx* s, const ev_node* ev) {
-    /* doTheCrossing, EDMD.c:2501-2590 (no shear) */
+    /* doTheCrossing, EDMD.c:2501-2590 (no shear; freeFly is the dispatch
+     * pointer, so the radius grows here while growing) */
     int i = ev->i;
-    free_fly(s, i);
+    fly(s, i);
     cell_unlink(s, i);
     switch (ev->j) {
         case 1:
@@ -461,7 +585,7 @@ static void do_crossing(orc_ctx* s, const ev_node* ev) {
 static void do_screenshot(orc_ctx* s) {
     /* takeAScreenshot, EDMD.c:4657-4677: free-fly everyone (a state update
      * point), reschedule; the dump write has no effect on the state. */
-    for (int i = 0; i < s->n; i++) free_fly(s, i);
+    for (int i = 0; i < s->n; i++) fly(s, i);
     schedule_special(s, 1, ORC_SCREENSHOT, s->t + s->prm.dtime);
 }
 
@@ -504,7 +628,7 @@ static void do_rescale(orc_ctx* s) {
     physical_q(s);
     for (int i = 0; i < s->n; i++) {
         s->coll[i]++;
-        free_fly(s, i);
+        fly(s, i);
         s->vx[i] /= sqrt(s->E / s->n / s->prm.temperature);
         s->vy[i] /= sqrt(s->E / s->n / s->prm.temperature);
     }
@@ -515,6 +639,46 @@ static void do_rescale(orc_ctx* s) {
         schedule_collision(s, j);
     }
     schedule_special(s, 2, ORC_ADDINGNOISE, s->t + s->prm.dtnoise);
+}
+
+static void normalize_physical_q(orc_ctx* s) {
+    /* normalizePhysicalQ, EDMD.c:5723-5764 (no circular wall): remove the
+     * mean momentum, then scale to E/N = Einit. */
+    physical_q(s);
+    for (int i = 0; i < s->n; i++) {
+        s->vx[i] -= s->px / (s->n * s->m[i]);
+        s->vy[i] -= s->py / (s->n * s->m[i]);
+    }
+    physical_q(s);
+    for (int i = 0; i < s->n; i++) {
+        s->vx[i] /= sqrt(s->E / s->n / s->prm.einit);
+        s->vy[i] /= sqrt(s->E / s->n / s->prm.einit);
+    }
+}
+
+static void do_stop_grow(orc_ctx* s) {
+    /* stopGrow, EDMD.c:4740-4785 (2D, no wells, no intruder): final free
+     * flight with growth, counters and image counters reset, Normal
+     * dispatch, velocity normalisation, then per disk: counter++, new
+     * collision slot, new crossing slot (this interleaving decides which
+     * new predictions start stale). */
+    for (int i = 0; i < s->n; i++) {
+        free_fly_grow(s, i);
+        s->coll[i] = 0;
+        s->ncol = 0;
+        s->ncross = 0;
+        s->crossX[i] = 0;
+        s->crossY[i] = 0;
+    }
+    s->growing = 0;
+    normalize_physical_q(s);
+    for (int i = 0; i < s->n; i++) {
+        s->coll[i]++;
+        queue_remove(s, s->n + i);
+        schedule_collision(s, i);
+        queue_remove(s, i);
+        schedule_crossing(s, i);
+    }
 }
 
 static int step(orc_ctx* s, orc_trace* rec) {
@@ -537,6 +701,7 @@ static int step(orc_ctx* s, orc_trace* rec) {
         case ORC_SCREENSHOT: do_screenshot(s); break;
         case ORC_THERMO: do_thermo(s); break;
         case ORC_ADDINGNOISE: do_rescale(s); break;
+        case ORC_GROWSTOP: do_stop_grow(s); break;
         default: break;
     }
     s->nevents++;
@@ -556,6 +721,8 @@ void orc_default_params(orc_params* p) {
     p->dtnoise = 0.5;
     p->temperature = 1;
     p->bucket_count = 0;
+    p->vr = 0;
+    p->einit = 1;
 }
 
 orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* y,
@@ -600,9 +767,10 @@ orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* 
     s->prv = malloc(nd * sizeof(int));
     s->ev = calloc(2 * nd + 10, sizeof(ev_node));
     s->lists = malloc(((size_t)s->n_lists + 1) * sizeof(int));
+    s->vr = calloc(nd, sizeof(double));
     if (!s->x || !s->y || !s->vx || !s->vy || !s->tl || !s->rad || !s->m || !s->coll ||
         !s->cx || !s->cy || !s->crossX || !s->crossY || !s->head || !s->nxt || !s->prv ||
-        !s->ev || !s->lists) {
+        !s->ev || !s->lists || !s->vr) {
         orc_destroy(s);
         return NULL;
     }
@@ -614,6 +782,14 @@ orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* 
         s->vy[i] = vy[i];
         s->rad[i] = rad[i];
         s->m[i] = m[i];
+    }
+    if (s->prm.vr > 0) {
+        /* growth start (EDMD.c:1770-1773): radius 0, rate vr * target */
+        s->growing = 1;
+        for (int i = 0; i < n; i++) {
+            s->vr[i] = s->prm.vr * s->rad[i];
+            s->rad[i] = 0;
+        }
     }
     /* cellListInit, EDMD.c:1906-1920 */
     for (int c = 0; c < s->ncx * s->ncy; c++) s->head[c] = NIL;
@@ -631,7 +807,7 @@ void orc_destroy(orc_ctx* s) {
     free(s->x); free(s->y); free(s->vx); free(s->vy); free(s->tl); free(s->rad);
     free(s->m); free(s->coll); free(s->cx); free(s->cy); free(s->crossX);
     free(s->crossY); free(s->head); free(s->nxt); free(s->prv); free(s->ev);
-    free(s->lists); free(s->rows); free(s->tape);
+    free(s->lists); free(s->rows); free(s->tape); free(s->vr);
     free(s);
 }
 
@@ -644,8 +820,15 @@ int orc_init_events(orc_ctx* s) {
     s->root = 2 * n;
     s->ev[s->root].t = NEVER + 1;
     s->tree_min = NIL;
-    schedule_special(s, 3, ORC_THERMO, s->prm.first_thermo + 0.000000001);
-    schedule_special(s, 1, ORC_SCREENSHOT, s->prm.first_screen);
+    double first_screen = s->prm.first_screen, first_thermo = s->prm.first_thermo;
+    if (s->growing) {
+        /* EDMD.c:1957-1964: output starts after the growth */
+        if (first_screen < 1 / s->prm.vr) first_screen = 1 / s->prm.vr + first_screen;
+        if (first_thermo < 1 / s->prm.vr) first_thermo = 1 / s->prm.vr + first_thermo;
+    }
+    schedule_special(s, 3, ORC_THERMO, first_thermo + 0.000000001);
+    if (s->growing) schedule_special(s, 5, ORC_GROWSTOP, 1 / s->prm.vr);
+    schedule_special(s, 1, ORC_SCREENSHOT, first_screen);
     if (s->prm.noise) schedule_special(s, 2, ORC_ADDINGNOISE, s->prm.dtnoise);
     for (int i = 0; i < n; i++) {
         s->ev[i].i = i;
@@ -687,6 +870,10 @@ int orc_run_collisions(orc_ctx* s, uint64_t target, int64_t max_events, int64_t*
     }
     if (done) *done = k;
     return st;
+}
+
+void orc_get_radii(const orc_ctx* s, double* rad) {
+    memcpy(rad, s->rad, (size_t)s->n * sizeof(double));
 }
 
 int orc_n(const orc_ctx* s) { return s->n; }
@@ -767,7 +954,7 @@ void orc_predict_crossing(orc_ctx* s, int i, double* t_abs, int32_t* dir) {
 
 void orc_sync_all(orc_ctx* s, double t) {
     s->t = t;
-    for (int i = 0; i < s->n; i++) free_fly(s, i);
+    for (int i = 0; i < s->n; i++) fly(s, i);
 }
 
 int orc_pcf(int n, const double* x, const double* y, double Lx, double Ly,

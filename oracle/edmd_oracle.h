@@ -26,7 +26,7 @@ extern "C" {
 
 /* Event kinds, numbered as the reference's enum event (EDMD.h:21-34). */
 enum { ORC_CELLCROSS = 0, ORC_COLLISION = 1, ORC_SCREENSHOT = 2, ORC_THERMO = 3,
-       ORC_ADDINGNOISE = 4 };
+       ORC_ADDINGNOISE = 4, ORC_GROWSTOP = 9 };
 
 /* Status codes. */
 enum { ORC_OK = 0, ORC_ERR_OVERLAP = 3, ORC_ERR_ARG = 22, ORC_ERR_NOMEM = 12 };
@@ -41,6 +41,12 @@ typedef struct {
     double dtnoise;      /* thermostat period (EDMD.c:386) */
     double temperature;  /* thermostat target T (EDMD.c:379) */
     int bucket_count;    /* calendar bucket count, 0 = N (EDMD.c:712-714) */
+    /* Growth start (EDMD.c:1691-1810, 1957-1981): vr > 0 grows every disk
+     * from radius 0 at rate vr * rad[i] (rad[] holds the target radii) until
+     * the GROWSTOP event at t = 1/vr, then stopGrow (EDMD.c:4740-4800)
+     * rescales velocities to E/N = einit. */
+    double vr;
+    double einit;        /* Einit (EDMD.c:369, -E) */
 } orc_params;
 
 typedef struct {
@@ -82,6 +88,8 @@ double orc_time(const orc_ctx* s);
 void orc_grid(const orc_ctx* s, int* ncx, int* ncy, double* csx, double* csy);
 void orc_get_state(const orc_ctx* s, double* x, double* y, double* vx, double* vy,
                    double* tl, uint64_t* coll, int32_t* cx, int32_t* cy);
+/* Current radius of every disk (they change while growing). */
+void orc_get_radii(const orc_ctx* s, double* rad);
 /* Current calendar slots: crossing (time, direction) and collision (time,
  * partner, partner's counter at prediction) for every particle. */
 void orc_get_events(const orc_ctx* s, double* t_cross, int32_t* dir, double* t_coll,

@@ -18,6 +18,12 @@
  *   run   <dump> <out> <tmax> <thermo> main's loop to tmax, final state,
  *                                      the reference's thermo rows
  *   bench <dump> <ncoll> [maxsec]      valid collisions / wall second
+ *   grow  <dump> <out> <nevents> <vr>  growth start: the dump's radii are the
+ *                                      targets; every disk starts at radius
+ *                                      0 growing at vr * target (EDMD.c:
+ *                                      1770-1773) and eventListInit takes its
+ *                                      growth branch (EDMD.c:1957-1981);
+ *                                      event trace, final state and radii
  * Extra arguments after the mode arguments are passed to constantInit.
  */
 #define main edmd_reference_main
@@ -69,6 +75,36 @@ static void start(const char* dump, int nextra, char** extra) {
     open_sinks();
 }
 
+/* start() with the growth branch of particlesInit / eventListInit applied to
+ * the loaded disks: rad -> 0, vr_i = vr * target radius. */
+static void start_grow(const char* dump, double rate, int nextra, char** extra) {
+    char* args[64];
+    int n = 0;
+    args[n++] = "edmd";
+    args[n++] = "-l";
+    args[n++] = (char*)dump;
+    args[n++] = "-t";
+    args[n++] = "1e300";
+    for (int k = 0; k < nextra && n < 63; k++) args[n++] = extra[k];
+    args[n] = NULL;
+    init_genrand(1);
+    constantInit(n, args);
+    configureTransverseVelocityCorrelationSampling(&dtimeThermo, &logSpacing);
+    particlesInit();
+    vr = rate;
+    for (int i = 0; i < N; i++) {
+        particles[i].vr = vr * particles[i].rad;
+        particles[i].rad = 0;
+    }
+    cellListInit();
+    int saved = load;
+    load = 0;
+    eventListInit();
+    load = saved;
+    physicalQ();
+    open_sinks();
+}
+
 /* One pass of the dispatch switch of EDMD.c:534-610. */
 static int step(trace_rec* rec) {
     nextEvent = findNextEvent();
@@ -88,6 +124,7 @@ static int step(trace_rec* rec) {
         case ADDINGNOISE: addNoise(); break;
         case SCREENSHOT: takeAScreenshot(); break;
         case THERMO: takeAThermo(); break;
+        case GROWSTOP: stopGrow(); break;
         default: fail("unexpected event type");
     }
     return 0;
@@ -171,6 +208,24 @@ int main(int argc, char** argv) {
             write_all(f, &rec, sizeof rec, 1);
         }
         write_state(f);
+        fclose(f);
+        return 0;
+    }
+    if (!strcmp(mode, "grow")) {
+        if (argc < 6) fail("grow dump out nevents vr");
+        long nev = atol(argv[4]);
+        start_grow(dump, atof(argv[5]), argc - 6, argv + 6);
+        FILE* f = fopen(argv[3], "wb");
+        if (!f) fail("cannot open output");
+        int64_t n64 = nev;
+        write_all(f, &n64, 8, 1);
+        for (long k = 0; k < nev; k++) {
+            trace_rec rec;
+            step(&rec);
+            write_all(f, &rec, sizeof rec, 1);
+        }
+        write_state(f);
+        for (int i = 0; i < N; i++) write_all(f, &particles[i].rad, 8, 1);
         fclose(f);
         return 0;
     }

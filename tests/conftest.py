@@ -30,6 +30,7 @@ def golden_state(g):
 TRACED = ["cfg1_d1", "cfg1_r1", "n400_phi060", "tri1024_phi080", "poly500_phi050",
           "tiny_ncx3"]
 SWEEP_ONLY = ["tiny_ncx2", "tiny_ncx1"]
+GROWN = ["grow_mono200", "grow_poly150"]
 
 
 def bits(a):

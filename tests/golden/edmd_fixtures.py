@@ -21,6 +21,7 @@ REPO = os.path.dirname(os.path.dirname(HERE))
 REF_DIR = os.path.join(REPO, "oracle", "_ref")
 
 CELLCROSS, COLLISION, SCREENSHOT, THERMO, ADDINGNOISE = 0, 1, 2, 3, 4
+GROWSTOP = 9  # enum event, EDMD.h:21-34
 
 
 def box_length(n, phi, rad=0.5, aspect=1.0):
@@ -101,6 +102,33 @@ def lattice_state(n, phi, seed, rad=0.5, lattice="square", temperature=1.0):
         "Ly": float(Ly if L is None else L),
     }
     return st
+
+
+def growth_state(n, phi, seed, poly=0.0, rad=0.5, e_per=0.05):
+    """Seeded growth start (the reference's default start, EDMD.c:1691-1810,
+    with this build's generator): uniform random centres in a square box
+    sized for packing fraction phi at the target radii, Gaussian velocities
+    with E/N = e_per (EinitGrow = 0.05), masses 1.  poly > 0 draws target
+    diameters 2 rad * N(1, poly) truncated to [1 - 3 poly, 1 + 3 poly] and
+    masses (d / 2 rad)^3.  The radii in the returned state are the targets."""
+    rng = np.random.default_rng(seed)
+    if poly > 0:
+        d = np.empty(0)
+        while len(d) < n:
+            c = rng.normal(1.0, poly, 2 * n)
+            d = np.concatenate([d, c[(c >= 1 - 3 * poly) & (c <= 1 + 3 * poly)]])
+        d = d[:n]
+        r = rad * d
+        m = d ** 3
+    else:
+        r = np.full(n, rad)
+        m = np.ones(n)
+    L = math.sqrt(math.pi * float(np.sum(r * r)) / phi)
+    x = rng.uniform(0, L, n)
+    y = rng.uniform(0, L, n)
+    v = rng.standard_normal((n, 2)) * math.sqrt(e_per)
+    return {"x": x, "y": y, "vx": np.ascontiguousarray(v[:, 0]),
+            "vy": np.ascontiguousarray(v[:, 1]), "rad": r, "m": m, "Lx": L, "Ly": L}
 
 
 def scale_state(st, f):
@@ -200,6 +228,16 @@ def parse_trace(path):
     glob["ncol"] = ncol
     glob["ncross"] = ncross
     return {"trace": tr, "state": state, "glob": glob}
+
+
+def parse_grow(path):
+    """parse_trace plus the final radius of every disk."""
+    out = parse_trace(path)
+    with open(path, "rb") as f:
+        raw = f.read()
+    n = len(out["state"])
+    out["rad"] = np.frombuffer(raw, dtype="<f8", count=n, offset=len(raw) - 8 * n).copy()
+    return out
 
 
 def parse_pcf(path):

@@ -13,6 +13,9 @@ initial states and stores inputs and outputs as compressed numpy archives:
   glob_*     t, virial sums, E, px, py, ncol, ncross after the trace
   thermo     the reference's own thermo rows (saveThermo, EDMD.c:5232-5586)
   pcf_*      calculate_pcf (pcf.c:16-75) on the input positions
+  grow_*     growth starts (the radii stored are the targets; vr the rate):
+             trace through the GROWSTOP event (stopGrow, EDMD.c:4740-4785)
+             and the final state with every radius
 
 Run from the repository root after `make -C oracle ref`:
     python tests/golden/make_golden.py
@@ -71,6 +74,12 @@ CASES = [
     ("tiny_ncx1", lambda: F.lattice_state(4, 0.35, seed=8), 0),
 ]
 
+GROWTH_CASES = [
+    # name, state builder, growth rate vr, trace length (past t = 1/vr)
+    ("grow_mono200", lambda: F.growth_state(200, 0.50, seed=1), 0.1, 20000),
+    ("grow_poly150", lambda: F.growth_state(150, 0.72, seed=3, poly=0.1), 0.1 / 8, 32000),
+]
+
 THERMO_CASE = ("thermo64", lambda: F.lattice_state(64, 0.50, seed=4), 1300.0, ["-o", "100"])
 
 
@@ -119,6 +128,23 @@ def main():
             r, g = F.parse_pcf(pcf)
             arrs["pcf_r"] = r
             arrs["pcf_g"] = g
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
+        print(name, "N=%d events=%d ncol=%d" % (len(st["x"]), nev, t["glob"]["ncol"]))
+
+    for name, build, vr, nev in GROWTH_CASES:
+        st = build()
+        dump = os.path.join(tmp, name + ".dump")
+        F.write_dump(dump, st)
+        out = os.path.join(tmp, name + ".grow")
+        F.run_ref(["grow", dump, out, nev, repr(vr)])
+        t = F.parse_grow(out)
+        arrs = state_arrays(st)
+        arrs["vr"] = np.float64(vr)
+        for k in t["trace"].dtype.names:
+            arrs["trace_" + k] = t["trace"][k]
+        arrs.update(pack_state("state", t))
+        arrs["state_rad"] = t["rad"]
+        assert (t["trace"]["type"] == F.GROWSTOP).sum() == 1, name
         np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
         print(name, "N=%d events=%d ncol=%d" % (len(st["x"]), nev, t["glob"]["ncol"]))
 

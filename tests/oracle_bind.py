@@ -21,7 +21,7 @@ class Params(C.Structure):
                 ("dtime", C.c_double), ("first_thermo", C.c_double),
                 ("dtime_thermo", C.c_double), ("noise", C.c_int),
                 ("dtnoise", C.c_double), ("temperature", C.c_double),
-                ("bucket_count", C.c_int)]
+                ("bucket_count", C.c_int), ("vr", C.c_double), ("einit", C.c_double)]
 
 
 class ThermoRow(C.Structure):
@@ -65,6 +65,7 @@ def lib():
         L.orc_sync_all.argtypes = [C.c_void_p, C.c_double]
         L.orc_pcf.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double,
                               C.c_double, dp, dp]
+        L.orc_get_radii.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_tape_enable.argtypes = [C.c_void_p, C.c_int64]
         L.orc_tape_count.restype = C.c_int64
         L.orc_tape_count.argtypes = [C.c_void_p]
@@ -216,6 +217,11 @@ class Oracle:
         csx, csy = C.c_double(), C.c_double()
         lib().orc_grid(self.h, C.byref(ncx), C.byref(ncy), C.byref(csx), C.byref(csy))
         return ncx.value, ncy.value, csx.value, csy.value
+
+    def radii(self):
+        out = np.zeros(self.n)
+        lib().orc_get_radii(self.h, out.ctypes.data)
+        return out
 
     def state(self):
         n = self.n

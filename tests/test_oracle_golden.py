@@ -6,7 +6,8 @@ orientation of mirrored pairs), final states, virial sums and counters."""
 import numpy as np
 import pytest
 
-from conftest import SWEEP_ONLY, TRACED, bits, golden_state, load_golden
+from conftest import GROWN, SWEEP_ONLY, TRACED, bits, golden_state, load_golden
+import edmd_fixtures as F
 
 
 @pytest.mark.parametrize("name", TRACED + SWEEP_ONLY)
@@ -92,3 +93,33 @@ def test_calendar_shape_does_not_change_pop_order(oracle):
     lo = np.minimum(tr["i"], tr["j"])
     glo = np.minimum(g["trace_i"], g["trace_j"])
     np.testing.assert_array_equal(lo, glo)
+
+
+@pytest.mark.parametrize("name", GROWN)
+def test_growth_matches_reference(oracle, name):
+    """Growth start from radius 0 (monodisperse and polydisperse targets):
+    the trace through GROWSTOP and stopGrow's re-prediction, the final state
+    with every radius, the counters stopGrow resets."""
+    g = load_golden(name)
+    nev = len(g["trace_t"])
+    o = oracle.Oracle(golden_state(g), vr=float(g["vr"]))
+    st, tr = o.run(nev)
+    assert st == 0
+    ref = np.zeros(nev, dtype=tr.dtype)
+    for k in ("type", "i", "j", "valid", "t"):
+        ref[k] = g["trace_" + k]
+    ct, cr = F.canonical_trace(tr), F.canonical_trace(ref)
+    for k in ("type", "i", "j", "valid"):
+        np.testing.assert_array_equal(ct[k], cr[k], err_msg=k)
+    np.testing.assert_array_equal(bits(ct["t"]), bits(cr["t"]))
+    assert (tr["type"] == F.GROWSTOP).sum() == 1
+    s = o.state()
+    for k in ("x", "y", "vx", "vy", "t"):
+        np.testing.assert_array_equal(bits(s[k]), bits(g["state_" + k]), err_msg=k)
+    np.testing.assert_array_equal(s["coll"], g["state_coll"])
+    np.testing.assert_array_equal(bits(o.radii()), bits(g["state_rad"]))
+    ob = o.observables()
+    for k in ("t", "E", "px", "py"):
+        assert bits(ob[k]) == bits(g["glob_" + k]), k
+    assert ob["ncol"] == int(g["glob_ncol"])
+    assert ob["ncross"] == int(g["glob_ncross"])

@@ -270,3 +270,44 @@ def test_pcf_identities(oracle):
     x, y = rng.uniform(0, 40, 3000), rng.uniform(0, 40, 3000)
     r, g = oracle.pcf(x, y, 40.0, 40.0, 1.0, 20.0)
     assert abs(np.mean(g[2:]) - 1) < 0.02
+
+
+def test_growth_head_on_contact_time(oracle):
+    """Two equal disks growing from radius 0 at rate w each, closing at
+    speed u along x from gap g: contact when g - u t = 2 w t, so
+    t = g / (u + 2 w); the growth rule then separates them at a relative
+    normal speed of u + 4 w (restitution 1 against the moving surface)."""
+    g, u, w = 3.0, 1.0, 0.25   # vr * target radius = w
+    st = {"x": np.array([5.0, 5.0 + g]), "y": np.array([5.0, 5.0]),
+          "vx": np.array([u / 2, -u / 2]), "vy": np.zeros(2),
+          "rad": np.array([1.0, 1.0]), "m": np.ones(2), "Lx": 20.0, "Ly": 20.0}
+    o = oracle.Oracle(st, vr=w)        # vr_i = vr * 1 = w, growth stops at 1/vr = 4
+    tc = None
+    while tc is None:                  # cell crossings come first
+        code, tr = o.run(1)
+        assert code == 0
+        if tr["type"][0] == 1:
+            tc = tr["t"][0]
+    assert tc == pytest.approx(g / (u + 2 * w), rel=1e-15)
+    s = o.state()
+    assert s["vx"][1] - s["vx"][0] == pytest.approx(u + 4 * w, rel=1e-14)
+    assert s["vx"][0] + s["vx"][1] == pytest.approx(0.0, abs=1e-15)
+    assert o.radii()[0] == pytest.approx(w * tc, rel=1e-15)
+
+
+def test_growth_stop_normalises(oracle):
+    """After GROWSTOP every radius is its target (to rounding of the summed
+    growth steps), the total momentum vanishes and E/N = Einit; counters
+    restart at zero."""
+    st = F.growth_state(120, 0.6, seed=8, poly=0.1)
+    o = oracle.Oracle(st, vr=0.2, einit=1.7)
+    while o.t < 5.0:
+        code, _ = o.run(1, trace=False)
+        assert code == 0
+    r = o.radii()
+    o.sync_all(o.t)
+    np.testing.assert_allclose(o.radii(), st["rad"], rtol=1e-12)
+    ob = o.observables()
+    assert ob["E"] / 120 == pytest.approx(1.7, rel=1e-13)
+    assert abs(ob["px"]) < 1e-12 and abs(ob["py"]) < 1e-12
+    assert r.min() > 0
