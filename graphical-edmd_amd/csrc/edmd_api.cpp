@@ -58,6 +58,8 @@ extern "C" void edmd_default_desc(edmd_system_desc* d) {
     d->noise = 0;
     d->dtnoise = 0.5;
     d->temperature = 1;
+    d->vr = 0;
+    d->einit = 1;
 }
 
 /* ------------------------------------------------------------ handles */
@@ -93,6 +95,10 @@ struct edmd_batch {
     EdmdState* st = nullptr;
     EdmdTraceRec* trace = nullptr;
     EdmdThermoRow* thermo = nullptr;
+    int64_t *erank = nullptr, *lrank = nullptr; /* tie ranks: 2 slots and 1 leaf per particle */
+    /* growth starts: per-particle growth rates; any_grow while a system grows */
+    double* vr = nullptr;
+    bool any_grow = false;
     /* derivatives: tape pools, checkpoints (one snapshot of every pool per
      * segment, with the run bounds the segment was run with) */
     bool deriv_on = false;
@@ -196,6 +202,11 @@ static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
         return fail(EDMD_ERR_ARG, "system %d: cell grid too large", idx);
     c.csx = d->Lx / c.ncx;
     c.csy = d->Ly / c.ncy;
+    if (!(d->vr >= 0) || !(d->einit > 0))
+        return fail(EDMD_ERR_ARG, "system %d: growth rate must be >= 0 and einit > 0", idx);
+    c.einit = d->einit;
+    c.dtp = 5.0 / (double)d->n; /* dtPaul, EDMD.c:712-714 */
+    c.nlists = d->n;
     c.res = d->res;
     c.dtime = d->dtime;
     c.dtime_thermo = d->dtime_thermo;
@@ -262,7 +273,7 @@ static int upload_system(edmd_batch* b, int s, const edmd_system_desc* d) {
         r.vx = d->vx[i];
         r.vy = d->vy[i];
         r.tl = 0;
-        r.rad = d->rad[i];
+        r.rad = d->vr > 0 ? 0.0 : d->rad[i]; /* growth starts from radius 0, EDMD.c:1765 */
         r.m = d->m[i];
         r.coll = 0;
         r.cx = (int16_t)cx[i];
@@ -273,12 +284,52 @@ static int upload_system(edmd_batch* b, int s, const edmd_system_desc* d) {
     HIPCHK(hipMemcpy(b->ccnt + h.cell0, cnt.data(), ncell * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->cslot + h.slot0, slot.data(), ncell * c.cap * sizeof(int32_t),
                      hipMemcpyHostToDevice));
+    if (b->vr) {
+        std::vector<double> g(n, 0.0);
+        if (d->vr > 0)
+            for (int i = 0; i < n; i++) g[i] = d->vr * d->rad[i]; /* EDMD.c:1772 */
+        HIPCHK(hipMemcpy(b->vr + p0, g.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    double first_screen = d->first_screen, first_thermo = d->first_thermo;
+    if (d->vr > 0) {
+        /* output starts after the growth, EDMD.c:1959-1964 */
+        if (first_screen < 1 / d->vr) first_screen = 1 / d->vr + first_screen;
+        if (first_thermo < 1 / d->vr) first_thermo = 1 / d->vr + first_thermo;
+    }
     EdmdState st;
     memset(&st, 0, sizeof st);
     st.t = 0;
-    st.next_screen = d->first_screen;
-    st.next_thermo = d->first_thermo + 0.000000001; /* EDMD.c:1970 */
+    st.next_screen = first_screen;
+    st.next_thermo = first_thermo + 0.000000001; /* EDMD.c:1970 */
+    st.growing = d->vr > 0 ? 1 : 0;
+    st.next_grow = d->vr > 0 ? 1 / d->vr : __builtin_huge_val(); /* addEventGrow(1/vr), EDMD.c:1976 */
+    /* eventListInit's insertion order (EDMD.c:1970-1995): thermo, growth
+     * stop, screenshot, noise; the particles' slots follow (sweep). */
+    st.paul = 0;
+    st.ctr = 0;
+    st.win = 0;
+    const double dtp = c.dtp;
+    auto rank0 = [&](double tev) {
+        int64_t r = (tev - st.paul < dtp) ? EDMD_RANK_DIRECT + st.ctr : EDMD_RANK_DIRECT - 1 - st.ctr;
+        st.ctr++;
+        return r;
+    };
+    /* the bucket of a special event, as addEventToQueue files it */
+    auto win0 = [&](double tev) {
+        const double dt = tev - st.paul;
+        if (dt < dtp) return (int64_t)0;
+        if (dt >= dtp * c.nlists) return (int64_t)-1;
+        return (int64_t)(int)(dt / dtp);
+    };
+    st.win_thermo = win0(st.next_thermo);
+    st.rank_thermo = rank0(st.next_thermo);
+    st.win_grow = d->vr > 0 ? win0(st.next_grow) : INT64_MAX;
+    st.rank_grow = d->vr > 0 ? rank0(st.next_grow) : INT64_MAX;
+    st.win_screen = win0(st.next_screen);
+    st.rank_screen = rank0(st.next_screen);
     st.next_noise = d->noise ? d->dtnoise : __builtin_huge_val();
+    st.win_noise = d->noise ? win0(st.next_noise) : INT64_MAX;
+    st.rank_noise = d->noise ? rank0(st.next_noise) : INT64_MAX;
     st.last_thermo = 0;
     st.ev_limit = INT64_MAX;
     HIPCHK(hipMemcpy(b->st + s, &st, sizeof st, hipMemcpyHostToDevice));
@@ -287,6 +338,8 @@ static int upload_system(edmd_batch* b, int s, const edmd_system_desc* d) {
 
 static int batch_predict_all(edmd_batch* b) {
     HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->stream));
+    if (b->any_grow) HIPCHK(edmd_launch_sweep_grow(b->d_sys, b->nsys, b->max_n, b->stream));
+    HIPCHK(edmd_launch_sweep_count(b->d_sys, b->nsys, b->stream));
     HIPCHK(edmd_launch_tree(b->d_sys, b->nsys, b->max_lsize, b->max_nlev, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     return EDMD_OK;
@@ -352,6 +405,10 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
     if (!rc) rc = dalloc(b, &b->st, (size_t)nsys);
     if (!rc) rc = dalloc(b, &b->trace, b->ntrace);
     if (!rc) rc = dalloc(b, &b->thermo, b->nthermo);
+    if (!rc) rc = dalloc(b, &b->erank, 2 * np);
+    if (!rc) rc = dalloc(b, &b->lrank, np);
+    for (int s = 0; s < nsys; s++) b->any_grow |= descs[s].vr > 0;
+    if (!rc && b->any_grow) rc = dalloc(b, &b->vr, np);
     if (rc) { batch_free(b); return rc; }
     if (hipStreamCreate(&b->stream) != hipSuccess) {
         batch_free(b);
@@ -376,6 +433,9 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
         A.st = b->st + s;
         A.trace = b->trace + h.trace0;
         A.thermo = b->thermo + h.thermo0;
+        A.vr = b->vr ? b->vr + p0 : nullptr;
+        A.erank = b->erank + 2 * p0;
+        A.lrank = b->lrank + p0;
         rc = upload_system(b, s, descs + s);
         if (rc) { batch_free(b); return rc; }
     }
@@ -425,8 +485,21 @@ static int batch_run_mask(edmd_batch* b, int64_t max_events, uint64_t max_collis
     ra.max_events = max_events;
     ra.max_coll = max_collisions;
     ra.tmax = tmax;
-    ra.stop_mask = stop_mask;
-    HIPCHK(edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, ra, b->stream));
+    ra.stop_mask = stop_mask & ~(1 << EDMD_EV_GROWSTOP);
+    if (b->any_grow) {
+        /* growing systems first: they run until GROWSTOP (then continue in
+         * the Normal launch below) or until the bounds */
+        HIPCHK(edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, 1, ra, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        int rc = check_status(b);
+        if (rc) return rc;
+        std::vector<EdmdState> st(b->nsys);
+        HIPCHK(hipMemcpy(st.data(), b->st, b->nsys * sizeof(EdmdState), hipMemcpyDeviceToHost));
+        bool still = false;
+        for (int s = 0; s < b->nsys; s++) still |= st[s].growing != 0;
+        b->any_grow = still;
+    }
+    HIPCHK(edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, 0, ra, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     return check_status(b);
 }
@@ -498,6 +571,19 @@ extern "C" int edmd_batch_get_state(edmd_batch* b, int s, double* x, double* y, 
     return EDMD_OK;
 }
 
+extern "C" int edmd_batch_get_radii(edmd_batch* b, int s, double* rad) {
+    int rc = sys_check(b, s);
+    if (rc) return rc;
+    if (!rad) return fail(EDMD_ERR_ARG, "null output");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const HostSys& h = b->hs[s];
+    size_t n = h.c.n;
+    std::vector<EdmdPart> part(n);
+    if ((rc = d2h(part.data(), b->part + h.p0, n))) return rc;
+    for (size_t i = 0; i < n; i++) rad[i] = part[i].rad;
+    return EDMD_OK;
+}
+
 extern "C" int edmd_batch_get_events(edmd_batch* b, int s, double* t_coll, int32_t* partner,
                                      double* t_cross, int32_t* dir) {
     int rc = sys_check(b, s);
@@ -513,6 +599,38 @@ extern "C" int edmd_batch_get_events(edmd_batch* b, int s, double* t_coll, int32
         if (t_cross) t_cross[i] = ev[i].tcross;
         if (dir) dir[i] = ev[i].dir;
     }
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_get_calendar(edmd_batch* b, int s, double* key, int32_t* idx,
+                                       int32_t* aux, int64_t cap, int64_t* nnodes,
+                                       int64_t* slot_rank, int64_t* leaf_rank) {
+    int rc = sys_check(b, s);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const HostSys& h = b->hs[s];
+    const size_t n = h.c.n;
+    int64_t total = 0;
+    for (int l = 0; l <= h.c.nlev; l++) total += h.c.lsize[l];
+    if (nnodes) *nnodes = total;
+    if (key || idx || aux) {
+        if (cap < total) return fail(EDMD_ERR_ARG, "calendar has %lld nodes, cap %lld",
+                                     (long long)total, (long long)cap);
+        std::vector<EdmdNode> nd(total);
+        size_t off = 0;
+        for (int l = 0; l <= h.c.nlev; l++) {
+            const EdmdNode* src = l == 0 ? b->leaf + h.p0 : b->nodes + h.node0[l - 1];
+            if ((rc = d2h(nd.data() + off, src, (size_t)h.c.lsize[l]))) return rc;
+            off += h.c.lsize[l];
+        }
+        for (int64_t k = 0; k < total; k++) {
+            if (key) key[k] = nd[k].key;
+            if (idx) idx[k] = nd[k].idx;
+            if (aux) aux[k] = nd[k].aux;
+        }
+    }
+    if (slot_rank && (rc = d2h(slot_rank, b->erank + 2 * h.p0, 2 * n))) return rc;
+    if (leaf_rank && (rc = d2h(leaf_rank, b->lrank + h.p0, n))) return rc;
     return EDMD_OK;
 }
 
@@ -626,6 +744,10 @@ extern "C" int edmd_get_state(edmd_ctx* c, double* x, double* y, double* vx, dou
                               double* t_local, uint32_t* coll, int32_t* cx, int32_t* cy) {
     return c ? edmd_batch_get_state(c->b, 0, x, y, vx, vy, t_local, coll, cx, cy)
              : fail(EDMD_ERR_ARG, "null context");
+}
+
+extern "C" int edmd_get_radii(edmd_ctx* c, double* rad) {
+    return c ? edmd_batch_get_radii(c->b, 0, rad) : fail(EDMD_ERR_ARG, "null context");
 }
 
 extern "C" int edmd_get_events(edmd_ctx* c, double* t_cross, int32_t* dir, double* t_coll,
@@ -958,7 +1080,9 @@ static std::vector<PoolSpan> state_pools(edmd_batch* b) {
             {b->img, 2 * b->np * sizeof(int32_t)},
             {b->ccnt, b->ncell * sizeof(int32_t)},
             {b->cslot, b->nslot * sizeof(int32_t)},
-            {b->st, (size_t)b->nsys * sizeof(EdmdState)}};
+            {b->st, (size_t)b->nsys * sizeof(EdmdState)},
+            {b->erank, 2 * b->np * sizeof(int64_t)},
+            {b->lrank, b->np * sizeof(int64_t)}};
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1025,6 +1149,8 @@ extern "C" int edmd_batch_deriv_begin(edmd_batch* b, int64_t checkpoint_every, i
         if (b->hs[s].c.noise)
             return fail(EDMD_ERR_ARG, "system %d: derivatives need a system without the "
                                       "velocity-rescale thermostat", s);
+    if (b->any_grow)
+        return fail(EDMD_ERR_ARG, "derivatives need systems whose growth has ended");
     int rc = edmd_batch_deriv_end(b);
     if (rc) return rc;
     if ((double)b->nsys * (double)cap * sizeof(EdmdTapeRec) > 1.0e12)
@@ -1071,7 +1197,7 @@ static int tape_overflow_check(edmd_batch* b) {
 /* Runs one checkpoint segment: at most `seg` events per system, new tape. */
 static int run_segment(edmd_batch* b, const EdmdRunArgs& ra) {
     HIPCHK(edmd_launch_seg_begin(b->d_sys, b->nsys, b->seg, 1, b->stream));
-    HIPCHK(edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, ra, b->stream));
+    HIPCHK(edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, 0, ra, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     return check_status(b);
 }

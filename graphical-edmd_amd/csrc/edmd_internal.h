@@ -12,8 +12,11 @@ int edmd_sweep_tiles(int ncx, int ncy);
 hipError_t edmd_launch_tree(const EdmdSys* d_sys, int nsys, const int* max_lsize, int max_nlev,
                             hipStream_t s);
 hipError_t edmd_launch_sync_all(const EdmdSys* d_sys, int nsys, int max_n, hipStream_t s);
-hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, EdmdRunArgs ra,
+hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, int grow, EdmdRunArgs ra,
                            hipStream_t s);
+hipError_t edmd_launch_sweep_grow(const EdmdSys* d_sys, int nsys, int max_n, hipStream_t s);
+/* counts the 2N insertions of a full sweep (after the sweep kernels) */
+hipError_t edmd_launch_sweep_count(const EdmdSys* d_sys, int nsys, hipStream_t s);
 hipError_t edmd_launch_peek(const EdmdSys* d_sys, int nsys, EdmdPeek* out, hipStream_t s);
 hipError_t edmd_launch_pair_time(const EdmdSys* d_sys, int i, int j, double* out, hipStream_t s);
 hipError_t edmd_launch_repredict(const EdmdSys* d_sys, int max_nlev, int i, int which,

@@ -45,6 +45,11 @@ __device__ __forceinline__ double uni(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+__device__ __forceinline__ int64_t uni64(int64_t v) {
+    return (int64_t)((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32 |
+                     (uint32_t)__builtin_amdgcn_readfirstlane((int)v));
+}
+
 /* ------------------------------------------------------------ kinematics */
 
 /* PBC, EDMD.c:5896-5913 */
@@ -87,6 +92,19 @@ __device__ __forceinline__ void fly(const EdmdConst& C, double t, double& x, dou
     y = wrap_post(y, C.Ly);
 }
 
+/* freeFlyGrow, EDMD.c:4992-5007: the radius grows with the flight. */
+__device__ __forceinline__ void fly_grow(const EdmdConst& C, double t, double& x, double& y,
+                                         double vx, double vy, double& tl, double& rad,
+                                         double vr) {
+    double dt = t - tl;
+    tl = t;
+    x += dt * vx;
+    y += dt * vy;
+    rad += dt * vr;
+    x = wrap_post(x, C.Lx);
+    y = wrap_post(y, C.Ly);
+}
+
 /* collisionTimeNormal, EDMD.c:2661-2723: relative time until contact of the
  * synchronised owner (x1..) with candidate 2 last updated at tl2. */
 /* The same with candidate 2's position already flown to t:
@@ -119,6 +137,39 @@ __device__ __forceinline__ double coll_time(const EdmdConst& C, double t, double
                         overlap);
 }
 
+/* collisionTimeGrow, EDMD.c:2598-2659 (2D): contact distance
+ * DIST(r1, r2(t)) widening at vr1 + vr2; the earlier positive root, or the
+ * later one under the reference's 1e-9 rule; an overlap is flagged only
+ * when neither root is taken. */
+__device__ __forceinline__ double coll_time_grow(const EdmdConst& C, double t, double x1,
+                                                 double y1, double vx1, double vy1, double r1,
+                                                 double vr1, double x2, double y2, double vx2,
+                                                 double vy2, double tl2, double r2, double vr2,
+                                                 bool& overlap) {
+    double lat2 = t - tl2;
+    double dvx = vx2 - vx1;
+    double dvy = vy2 - vy1;
+    double dvr = vr1 + vr2;
+    double dx = (x2 + lat2 * vx2) - x1;
+    double dy = (y2 + lat2 * vy2) - y1;
+    double dr = sqrt(4 * r1 * (r2 + lat2 * vr2));
+    min_image(C, dx, dy);
+    double b = dx * dvx + dy * dvy - dvr * dr;
+    double v2 = dvx * dvx + dvy * dvy;
+    double dist2 = dx * dx + dy * dy;
+    double det = b * b - (v2 - dvr * dvr) * (dist2 - dr * dr);
+    if (det < 0) return EDMD_NEVER;
+    double plus = (-b + sqrt(det)) / (v2 - dvr * dvr);
+    double minus = (-b - sqrt(det)) / (v2 - dvr * dvr);
+    if (((minus > 0) && (plus > 0) && (minus < plus)) || ((minus > 0) && (plus < 0)))
+        return minus;
+    else if (((minus > 0) && (plus > 0.000000001) && (plus < minus)) ||
+             ((minus < 0) && (plus > 0.000000001)))
+        return plus;
+    if (dist2 - dr * dr < -0.01) overlap = true; /* EDMD.c:2653-2656 */
+    return EDMD_NEVER;
+}
+
 /* crossingEventNormal, EDMD.c:2405-2482: absolute crossing time and
  * direction code (1 -x, 2 +x, 3 -y, 4 +y); ties go to y. */
 __device__ __forceinline__ void cross_time(const EdmdConst& C, double t, double x, double y,
@@ -149,15 +200,36 @@ __device__ __forceinline__ void cross_time(const EdmdConst& C, double t, double 
     }
 }
 
-/* Calendar leaf of a particle: its earlier slot (the crossing wins ties, a
- * NaN key is parked at +inf so that it is never popped) and the event code:
- * the partner for a collision, minus the direction for a crossing. */
-__device__ __forceinline__ double leaf_key(double tcoll, double tcross) {
-    double k = (tcoll < tcross) ? tcoll : tcross;
-    return (k == k) ? k : INF_D;
+/* Rank of an event scheduled at t_ev with the queue at paulTime = paul after
+ * ctr earlier insertions (addEventToQueue, EDMD.c:2144-2170): straight into
+ * the tree when t_ev - paulTime < dtPaul, otherwise into a LIFO bucket. */
+__device__ __forceinline__ int64_t rank_of(double t_ev, double paul, double dtp, int64_t ctr) {
+    return (t_ev - paul < dtp) ? EDMD_RANK_DIRECT + ctr : EDMD_RANK_DIRECT - 1 - ctr;
 }
-__device__ __forceinline__ int leaf_aux(double tcoll, double tcross, int partner, int dir) {
-    return (tcoll < tcross) ? partner : -dir;
+
+/* Calendar leaf of a particle: its earlier slot by (time, rank) (a NaN key
+ * is parked at +inf so that it is never popped), that slot's rank and its
+ * event code: the partner for a collision, minus the direction for a
+ * crossing. */
+struct Leaf {
+    double key;
+    int64_t rank;
+    int aux;
+};
+
+__device__ __forceinline__ Leaf make_leaf(double tcoll, int64_t rcoll, double tcross,
+                                          int64_t rcross, int partner, int dir) {
+    const bool coll = tcoll < tcross || (tcoll == tcross && rcoll < rcross);
+    Leaf L;
+    const double k = coll ? tcoll : tcross;
+    L.key = (k == k) ? k : INF_D;
+    L.rank = coll ? rcoll : rcross;
+    L.aux = coll ? partner : -dir;
+    return L;
+}
+
+__device__ __forceinline__ bool before(double ka, int64_t ra, double kb, int64_t rb) {
+    return ka < kb || (ka == kb && ra < rb);
 }
 
 /* ------------------------------------------------------------ predictions */
@@ -166,11 +238,14 @@ __device__ __forceinline__ int leaf_aux(double tcoll, double tcross, int partner
  * in the reference's order, y offset outer, x offset inner, newest list
  * entry first (EDMD.c:2959-3003).  Returns the relative time; NEVER and
  * partner 0 when no candidate qualifies (EDMD.c:2830-2831). */
+template <bool G>
 __device__ __forceinline__ double predict_seq(const EdmdConst& C, const EdmdArrays& A, double t,
                                               int p, const EdmdPart& o, int& partner,
                                               bool& overlap) {
-    double dt = EDMD_NEVER;
+    /* collisionEventGrow (EDMD.c:3104-3230, no walls) starts from 1e7 */
+    double dt = G ? EDMD_GROW_NONE : EDMD_NEVER;
     int best = 0;
+    const double ovr = G ? A.vr[p] : 0.0;
     for (int j = -1; j <= 1; ++j) {
         int row = wrap_cell(o.cy + j, C.ncy) * C.ncx;
         for (int k = -1; k <= 1; ++k) {
@@ -181,8 +256,13 @@ __device__ __forceinline__ double predict_seq(const EdmdConst& C, const EdmdArra
                 int q = sl[s];
                 if (q == p) continue;
                 const EdmdPart c2 = A.p[q];
-                double tt = coll_time(C, t, o.x, o.y, o.vx, o.vy, o.rad, c2.x, c2.y, c2.vx, c2.vy,
-                                      c2.tl, c2.rad, overlap);
+                double tt;
+                if (G)
+                    tt = coll_time_grow(C, t, o.x, o.y, o.vx, o.vy, o.rad, ovr, c2.x, c2.y, c2.vx,
+                                        c2.vy, c2.tl, c2.rad, A.vr[q], overlap);
+                else
+                    tt = coll_time(C, t, o.x, o.y, o.vx, o.vy, o.rad, c2.x, c2.y, c2.vx, c2.vy,
+                                   c2.tl, c2.rad, overlap);
                 if (dt > tt) {
                     dt = tt;
                     best = q;
@@ -198,6 +278,7 @@ struct Owner {
     int p;
     double x, y, vx, vy, r;
     int cx, cy;
+    double vr; /* radius growth rate (growth starts only) */
 };
 
 /* Wave-cooperative neighbour scan for one or two owners at once.
@@ -210,6 +291,7 @@ struct Owner {
  * (time, traversal rank) among candidates with time < NEVER is the
  * reference's strict-less first minimum; no candidate gives NEVER and
  * partner 0 (EDMD.c:2830-2831, 3091-3093).  Results are uniform. */
+template <bool G>
 __device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArrays& A, int lane,
                                                double t, int nown, const Owner& a, const Owner& b,
                                                double& dta, int& pa, uint32_t& ca, double& dtb,
@@ -253,12 +335,21 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArr
         const bool vA = qA >= 0 && qA != a.p;
         const bool vB = qB >= 0 && qB != b.p;
         EdmdPart rA, rB;
+        double gA = 0, gB = 0;
         if (vA) rA = A.p[qA];
         if (vB) rB = A.p[qB];
+        if (G && vA) gA = A.vr[qA];
+        if (G && vB) gB = A.vr[qB];
+        const double lim = G ? EDMD_GROW_NONE : EDMD_NEVER;
         if (vA) {
-            double tt = coll_time(C, t, a.x, a.y, a.vx, a.vy, a.r, rA.x, rA.y, rA.vx, rA.vy, rA.tl,
-                                  rA.rad, overlap);
-            if (tt < EDMD_NEVER && tt < btA) {
+            double tt;
+            if (G)
+                tt = coll_time_grow(C, t, a.x, a.y, a.vx, a.vy, a.r, a.vr, rA.x, rA.y, rA.vx,
+                                    rA.vy, rA.tl, rA.rad, gA, overlap);
+            else
+                tt = coll_time(C, t, a.x, a.y, a.vx, a.vy, a.r, rA.x, rA.y, rA.vx, rA.vy, rA.tl,
+                               rA.rad, overlap);
+            if (tt < lim && tt < btA) {
                 btA = tt;
                 brA = sA * C.cap + (f - exA);
                 bqA = qA;
@@ -266,9 +357,14 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArr
             }
         }
         if (vB) {
-            double tt = coll_time(C, t, b.x, b.y, b.vx, b.vy, b.r, rB.x, rB.y, rB.vx, rB.vy, rB.tl,
-                                  rB.rad, overlap);
-            if (tt < EDMD_NEVER && tt < btB) {
+            double tt;
+            if (G)
+                tt = coll_time_grow(C, t, b.x, b.y, b.vx, b.vy, b.r, b.vr, rB.x, rB.y, rB.vx,
+                                    rB.vy, rB.tl, rB.rad, gB, overlap);
+            else
+                tt = coll_time(C, t, b.x, b.y, b.vx, b.vy, b.r, rB.x, rB.y, rB.vx, rB.vy, rB.tl,
+                               rB.rad, overlap);
+            if (tt < lim && tt < btB) {
                 btB = tt;
                 brB = (sB - 16) * C.cap + (f - exB);
                 bqB = qB;
@@ -287,34 +383,57 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArr
         if (otA < btA || (otA == btA && orA < brA)) { btA = otA; brA = orA; bqA = oqA; bcA = ocA; }
         if (otB < btB || (otB == btB && orB < brB)) { btB = otB; brB = orB; bqB = oqB; bcB = ocB; }
     }
-    if (btA < EDMD_NEVER) {
+    const double lim = G ? EDMD_GROW_NONE : EDMD_NEVER;
+    if (btA < lim) {
         dta = uni(btA); pa = uni(bqA); ca = (uint32_t)uni((int)bcA);
     } else {
-        dta = EDMD_NEVER; pa = 0; ca = A.p[0].coll;
+        dta = lim; pa = 0; ca = A.p[0].coll;
     }
-    if (btB < EDMD_NEVER) {
+    if (btB < lim) {
         dtb = uni(btB); pb = uni(bqB); cb = (uint32_t)uni((int)bcB);
     } else {
-        dtb = EDMD_NEVER; pb = 0; cb = A.p[0].coll;
+        dtb = lim; pb = 0; cb = A.p[0].coll;
     }
 }
 
 /* ------------------------------------------------------------ calendar */
 
-/* Lexicographic minimum over the wave of (key, leaf), carrying the event
- * code: ties go to the smaller leaf index. */
-__device__ __forceinline__ void wave_argmin3(double& k, int& i, int& x) {
+/* Minimum over the wave by (key, rank), carrying the leaf and the event
+ * code.  A node's rank is that of its leaf (lrank[leaf]); ranks are read
+ * only when several lanes hold the minimum key, which is rare.  Keys are
+ * never NaN; among +inf keys (padding, parked events) the lowest lane wins. */
+__device__ __forceinline__ void wave_argmin(const EdmdArrays& A, double& k, int& i, int& x) {
+    double mk = k;
 #pragma unroll
     for (int m = 1; m < 64; m <<= 1) {
-        double ok = __shfl_xor(k, m, 64);
-        int oi = __shfl_xor(i, m, 64);
-        int ox = __shfl_xor(x, m, 64);
-        if (ok < k || (ok == k && oi < i)) {
-            k = ok;
-            i = oi;
-            x = ox;
-        }
+        const double o = __shfl_xor(mk, m, 64);
+        mk = o < mk ? o : mk;
     }
+    mk = uni(mk);
+    const uint64_t tied = __ballot(k == mk);
+    int w = (int)__builtin_ctzll(tied);
+    if (__popcll(tied) > 1 && mk < INF_D) {
+        /* equal keys: the earlier in the reference's queue (lower rank) */
+        /* loaded by every lane (padding lanes read leaf 0) and selected,
+         * not loaded under a divergent branch */
+        const bool tie = k == mk;
+        const int64_t rl = A.lrank[tie ? i : 0];
+        int64_t r = tie ? rl : INT64_MAX;
+        int l = tie ? (int)(threadIdx.x & 63) : 64;
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+            const long long orr = __shfl_xor((long long)r, m, 64);
+            const int ol = __shfl_xor(l, m, 64);
+            if (orr < r || (orr == r && ol < l)) {
+                r = orr;
+                l = ol;
+            }
+        }
+        w = uni(l);
+    }
+    k = mk;
+    i = __builtin_amdgcn_readlane(i, w);
+    x = __builtin_amdgcn_readlane(x, w);
 }
 
 /* Level l of the calendar: the leaves for l = 0, else nodes of level l. */
@@ -334,8 +453,8 @@ __device__ __forceinline__ void top_min(const EdmdConst& C, const EdmdArrays& A,
         i = nd.idx;
         x = nd.aux;
     }
-    wave_argmin3(k, i, x);
-    key = uni(k);
+    wave_argmin(A, k, i, x);
+    key = k;
     leaf = uni(i);
     aux = uni(x);
 }
@@ -376,26 +495,31 @@ __device__ __forceinline__ void path_prefetch(const EdmdConst& C, const EdmdArra
     }
 }
 
-/* Stores the new leaves of pA (and pB >= 0) and recomputes every node on
- * their paths from the prefetched children, substituting the freshly
- * computed path nodes; where the paths meet one node takes both. */
+/* Stores the new leaves of pA (and pB >= 0) with their ranks and recomputes
+ * every node on their paths from the prefetched children, substituting the
+ * freshly computed path nodes; where the paths meet one node takes both. */
 template <int ML>
 __device__ __forceinline__ void path_update(const EdmdConst& C, const EdmdArrays& A, int lane,
-                                            const PathCache<ML>& P, int pA, double kA, int xA,
-                                            int pB, double kB, int xB) {
+                                            const PathCache<ML>& P, int pA, Leaf LA, int pB,
+                                            Leaf LB) {
+    double kA = LA.key, kB = LB.key;
+    int xA = LA.aux, xB = LB.aux;
     if (lane == 0) {
         EdmdNode n;
         n.key = kA;
         n.idx = pA;
         n.aux = xA;
         A.leaf[pA] = n;
+        A.lrank[pA] = LA.rank;
         if (pB >= 0) {
             n.key = kB;
             n.idx = pB;
             n.aux = xB;
             A.leaf[pB] = n;
+            A.lrank[pB] = LB.rank;
         }
     }
+    WAVE_SYNC();
     int iA = pA, iB = pB;
     int cA = pA, cB = pB;
 #pragma unroll
@@ -408,8 +532,8 @@ __device__ __forceinline__ void path_update(const EdmdConst& C, const EdmdArrays
         int i = P.i[0][l], x = P.x[0][l];
         if (c == cA) { k = kA; i = iA; x = xA; }
         else if (pB >= 0 && c == cB) { k = kB; i = iB; x = xB; }
-        wave_argmin3(k, i, x);
-        const double nkA = uni(k);
+        wave_argmin(A, k, i, x);
+        const double nkA = k;
         const int niA = uni(i), nxA = uni(x);
         double nkB = nkA;
         int niB = niA, nxB = nxA;
@@ -419,8 +543,8 @@ __device__ __forceinline__ void path_update(const EdmdConst& C, const EdmdArrays
             i = P.i[1][l];
             x = P.x[1][l];
             if (c == cB) { k = kB; i = iB; x = xB; }
-            wave_argmin3(k, i, x);
-            nkB = uni(k);
+            wave_argmin(A, k, i, x);
+            nkB = k;
             niB = uni(i);
             nxB = uni(x);
         }
@@ -445,47 +569,77 @@ __device__ __forceinline__ void path_update(const EdmdConst& C, const EdmdArrays
     WAVE_SYNC();
 }
 
-/* Sequential minimum over the children of node g of level l (l >= 1):
- * strict less in child order, i.e. ties go to the smaller leaf index. */
+/* Sequential minimum over the children of node g of level l (l >= 1) by
+ * (key, rank).  Every node holds a real leaf index (0 <= idx < n), so each
+ * child's rank is loaded unconditionally and the winner chosen by selects:
+ * a rank load under a divergent branch here has been compiled into code
+ * that kept the previous winner's event code on a tie (gfx950, ROCm 7.2). */
 __device__ __forceinline__ void node_min(const EdmdConst& C, const EdmdArrays& A, int l, int g) {
     int lo = g << 6;
     int hi = lo + 64;
     if (hi > C.lsize[l - 1]) hi = C.lsize[l - 1];
     const EdmdNode* ch = level(A, l - 1);
-    EdmdNode best = ch[lo];
+    const EdmdNode b0 = ch[lo];
+    double bk = b0.key;
+    int bi = b0.idx, bx = b0.aux;
+    int64_t br = A.lrank[bi];
     for (int c = lo + 1; c < hi; ++c) {
         const EdmdNode nd = ch[c];
-        if (nd.key < best.key) best = nd;
+        const int64_t r = A.lrank[nd.idx];
+        const bool take = nd.key < bk || (nd.key == bk && bk < INF_D && r < br);
+        bk = take ? nd.key : bk;
+        bi = take ? nd.idx : bi;
+        bx = take ? nd.aux : bx;
+        br = take ? r : br;
     }
+    EdmdNode best;
+    best.key = bk;
+    best.idx = bi;
+    best.aux = bx;
     A.lev[l - 1][g] = best;
 }
 
 /* ------------------------------------------------------------ sweep */
 
 /* One particle's full prediction (crossing + collision) written to its two
- * slots and its leaf; the particle is assumed synchronised at t. */
-__device__ __forceinline__ bool sweep_one(const EdmdConst& C, const EdmdArrays& A, double t, int p) {
+ * slots and its leaf; the particle is assumed synchronised at t.  G selects
+ * the growth functions.  The slots are scheduled as insertions number
+ * ctr_cross and ctr_coll with the queue at paulTime = paul.  stop_grow: the
+ * partner counter recorded is the one stopGrow's interleaved loop sees
+ * (EDMD.c:4770-4776: every counter goes 0 -> 1 in particle order, each disk
+ * predicting right after its own increment), i.e. 1 for a partner of lower
+ * or equal index, else 0. */
+template <bool G>
+__device__ __forceinline__ bool sweep_one(const EdmdConst& C, const EdmdArrays& A, double t, int p,
+                                          double paul, int64_t ctr_cross, int64_t ctr_coll,
+                                          bool stop_grow = false) {
     const EdmdPart o = A.p[p];
     double tc;
     int dir;
     cross_time(C, t, o.x, o.y, o.vx, o.vy, o.cx, o.cy, tc, dir);
     bool ov = false;
     int partner;
-    double dt = predict_seq(C, A, t, p, o, partner, ov);
+    double dt = predict_seq<G>(C, A, t, p, o, partner, ov);
     double tk = t + dt;
     EdmdEv e;
     e.tcoll = tk;
     e.tcross = tc;
     e.partner = partner;
-    e.cact = A.p[partner].coll;
+    e.cact = stop_grow ? (partner <= p ? 1u : 0u) : A.p[partner].coll;
     e.dir = dir;
     e.pad = 0;
     A.e[p] = e;
+    const int64_t rk = rank_of(tk, paul, C.dtp, ctr_coll);
+    const int64_t rx = rank_of(tc, paul, C.dtp, ctr_cross);
+    A.erank[2 * p] = rk;
+    A.erank[2 * p + 1] = rx;
+    const Leaf L = make_leaf(tk, rk, tc, rx, partner, dir);
     EdmdNode n;
-    n.key = leaf_key(tk, tc);
+    n.key = L.key;
     n.idx = p;
-    n.aux = leaf_aux(tk, tc, partner, dir);
+    n.aux = L.aux;
     A.leaf[p] = n;
+    A.lrank[p] = L.rank;
     return ov;
 }
 
@@ -559,6 +713,7 @@ __global__ __launch_bounds__(SW_THREADS) void k_sweep_tiled(const EdmdSys* __res
     const EdmdConst& C = S->c;
     const EdmdArrays& A = S->a;
     const int tid = threadIdx.x;
+    if (A.st->growing) return; /* k_sweep_grow predicts growing systems */
     const int ncx = C.ncx, ncy = C.ncy;
     const int ntx = (ncx + SW_TX - 1) / SW_TX, nty = (ncy + SW_TY - 1) / SW_TY;
     const int per = gridDim.x / 8; /* gridDim.x is a multiple of 8 */
@@ -567,6 +722,10 @@ __global__ __launch_bounds__(SW_THREADS) void k_sweep_tiled(const EdmdSys* __res
     const int tx0 = (tile % ntx) * SW_TX, ty0 = (tile / ntx) * SW_TY;
     const int vw = min(SW_TX, ncx - tx0), vh = min(SW_TY, ncy - ty0);
     const double t = A.st->t;
+    /* eventListInit order (EDMD.c:2007-2012): crossing then collision of
+     * each particle in index order, after the insertions already counted */
+    const double paul = A.st->paul;
+    const int64_t ctr0 = A.st->ctr;
 
     for (int h = tid; h < SW_NH; h += SW_THREADS) {
         const int lx = h % SW_HW - 1, ly = h / SW_HW - 1;
@@ -676,11 +835,17 @@ __global__ __launch_bounds__(SW_THREADS) void k_sweep_tiled(const EdmdSys* __res
             e.dir = dir;
             e.pad = 0;
             A.e[p] = e;
+            const int64_t rk = rank_of(tk, paul, C.dtp, ctr0 + 2 * (int64_t)p + 1);
+            const int64_t rx = rank_of(tc, paul, C.dtp, ctr0 + 2 * (int64_t)p);
+            A.erank[2 * p] = rk;
+            A.erank[2 * p + 1] = rx;
+            const Leaf Lf = make_leaf(tk, rk, tc, rx, partner, dir);
             EdmdNode n;
-            n.key = leaf_key(tk, tc);
+            n.key = Lf.key;
             n.idx = p;
-            n.aux = leaf_aux(tk, tc, partner, dir);
+            n.aux = Lf.aux;
             A.leaf[p] = n;
+            A.lrank[p] = Lf.rank;
         }
     } else {
         const int cap = C.cap;
@@ -689,7 +854,11 @@ __global__ __launch_bounds__(SW_THREADS) void k_sweep_tiled(const EdmdSys* __res
             const int lx = c % SW_TX, ly = c / SW_TX;
             if (lx >= vw || ly >= vh) continue;
             const int g = (ty0 + ly) * ncx + tx0 + lx;
-            if (sl < A.ccnt[g]) ov |= sweep_one(C, A, t, A.cslot[(size_t)g * cap + sl]);
+            if (sl < A.ccnt[g]) {
+                const int q = A.cslot[(size_t)g * cap + sl];
+                ov |= sweep_one<false>(C, A, t, q, paul, ctr0 + 2 * (int64_t)q,
+                                       ctr0 + 2 * (int64_t)q + 1);
+            }
         }
     }
     if (ov) atomicExch(&A.st->status, EDMD_DEV_OVERLAP);
@@ -710,31 +879,175 @@ __global__ __launch_bounds__(256) void k_sync_all(const EdmdSys* __restrict__ sy
     const double t = S->a.st->t;
     EdmdPart* r = S->a.p + p;
     double x = r->x, y = r->y, tl = r->tl;
-    fly(S->c, t, x, y, r->vx, r->vy, tl);
+    if (S->a.st->growing) {
+        double rad = r->rad;
+        fly_grow(S->c, t, x, y, r->vx, r->vy, tl, rad, S->a.vr[p]);
+        r->rad = rad;
+    } else {
+        fly(S->c, t, x, y, r->vx, r->vy, tl);
+    }
     r->x = x;
     r->y = y;
     r->tl = tl;
+}
+
+/* Prediction sweep of the systems that are growing (collisionEventGrow,
+ * EDMD.c:3104-3230, crossingEventGrow, EDMD.c:2343-2400 = the Normal
+ * crossing), one lane per particle. */
+__global__ __launch_bounds__(256) void k_sweep_grow(const EdmdSys* __restrict__ systems) {
+    const EdmdSys* S = systems + blockIdx.y;
+    if (!S->a.st->growing) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= S->c.n) return;
+    const int64_t ctr0 = S->a.st->ctr;
+    bool ov = sweep_one<true>(S->c, S->a, S->a.st->t, p, S->a.st->paul, ctr0 + 2 * (int64_t)p,
+                              ctr0 + 2 * (int64_t)p + 1);
+    if (ov) atomicExch(&S->a.st->status, EDMD_DEV_OVERLAP);
+}
+
+/* After a full sweep: one insertion per slot of every particle counted. */
+__global__ void k_sweep_count(const EdmdSys* __restrict__ systems, int nsys) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsys) systems[s].a.st->ctr += 2 * (int64_t)systems[s].c.n;
 }
 
 /* ------------------------------------------------------------ event loop */
 
 /* Loop state kept in registers; the rest of EdmdState stays in memory and
  * is touched only by the events that use it. */
+/* The emulated state of the reference's queue, kept in LDS (one wavefront
+ * per workgroup): it changes rarely and would otherwise hold scalar
+ * registers through the whole loop. */
+struct QueueState {
+    double paul;   /* the reference's paulTime */
+    int64_t ctr;   /* insertions so far */
+    int64_t win;   /* buckets passed */
+    int64_t srank; /* rank of the next special event */
+    int cur;       /* win modulo the bucket count */
+};
+
 struct LoopRegs {
     double t;
-    double next_special; /* earliest of the screenshot / thermo / noise times */
+    double next_special; /* earliest of the screenshot / thermo / noise / growth-stop times */
     int special;         /* its kind */
     uint64_t ncol, nevents;
     int32_t status, err_i, err_j;
+    QueueState* q;
 };
 
 __device__ __forceinline__ void next_special(const EdmdState* st, LoopRegs& R) {
     double tsp = st->next_screen;
+    int64_t rsp = st->rank_screen;
     int sp = EDMD_EV_SCREENSHOT;
-    if (st->next_thermo < tsp) { tsp = st->next_thermo; sp = EDMD_EV_THERMO; }
-    if (st->next_noise < tsp) { tsp = st->next_noise; sp = EDMD_EV_ADDINGNOISE; }
+    if (before(st->next_thermo, st->rank_thermo, tsp, rsp)) {
+        tsp = st->next_thermo; rsp = st->rank_thermo; sp = EDMD_EV_THERMO;
+    }
+    if (before(st->next_noise, st->rank_noise, tsp, rsp)) {
+        tsp = st->next_noise; rsp = st->rank_noise; sp = EDMD_EV_ADDINGNOISE;
+    }
+    if (before(st->next_grow, st->rank_grow, tsp, rsp)) {
+        tsp = st->next_grow; rsp = st->rank_grow; sp = EDMD_EV_GROWSTOP;
+    }
     R.next_special = uni(tsp);
+    R.q->srank = uni64(rsp);
     R.special = uni(sp);
+}
+
+/* Bucket the reference files an event at t_ev into (addEventToQueue,
+ * EDMD.c:2144-2170): the current one when t_ev - paulTime < dtPaul, -1 for
+ * the overflow list, else win + (int)((t_ev - paulTime) / dtPaul). */
+__device__ __host__ __forceinline__ int64_t window_of(double t_ev, double paul, double dtp,
+                                                      int nlists, int64_t win) {
+    const double dt = t_ev - paul;
+    if (dt < dtp) return win;
+    if (dt >= dtp * nlists) return -1;
+    return win + (int64_t)(int)(dt / dtp);
+}
+
+/* The overflow list is filed again when the ring of buckets wraps
+ * (EDMD.c:2292-2302): the special events waiting there get their bucket
+ * and rank anew.  Rare; kept out of line. */
+__device__ __noinline__ void refile_specials(const EdmdConst& C, EdmdState* st, double paul,
+                                             int64_t win, int64_t& ctr) {
+    double* ts[4] = {&st->next_screen, &st->next_thermo, &st->next_noise, &st->next_grow};
+    int64_t* ws[4] = {&st->win_screen, &st->win_thermo, &st->win_noise, &st->win_grow};
+    int64_t* rs[4] = {&st->rank_screen, &st->rank_thermo, &st->rank_noise, &st->rank_grow};
+    for (int k = 0; k < 4; ++k) {
+        const double tk = *ts[k];
+        if (*ws[k] != -1 || !(tk < 1e25)) continue;
+        const int64_t w = window_of(tk, paul, C.dtp, C.nlists, win);
+        /* filed straight into the tree during the wrap, before the new
+         * bucket is drained: ahead of both drained and later direct events */
+        const int64_t r = (w == win) ? ctr : rank_of(tk, paul, C.dtp, ctr);
+        ctr++;
+        WAVE_SYNC();
+        if (threadIdx.x == 0) {
+            *ws[k] = w;
+            *rs[k] = r;
+        }
+        WAVE_SYNC();
+    }
+}
+
+/* One bucket further (addNextPaulEvent, EDMD.c:2287-2307); true when the
+ * ring wrapped. */
+__device__ __forceinline__ bool step_window(const EdmdConst& C, EdmdState* st, LoopRegs& R) {
+    R.q->paul = uni(R.q->paul + C.dtp);
+    R.q->win++;
+    if (++R.q->cur < C.nlists) return false;
+    R.q->cur = 0;
+    int64_t ctr = R.q->ctr;
+    refile_specials(C, st, R.q->paul, R.q->win, ctr);
+    R.q->ctr = uni64(ctr);
+    return true;
+}
+
+__device__ __forceinline__ int64_t special_window(const EdmdState* st, int sp) {
+    const int64_t w = sp == EDMD_EV_SCREENSHOT ? st->win_screen
+                    : sp == EDMD_EV_THERMO     ? st->win_thermo
+                    : sp == EDMD_EV_GROWSTOP   ? st->win_grow
+                                               : st->win_noise;
+    return uni64(w);
+}
+
+/* findNextEvent (EDMD.c:2313-2327): while no pending event is in the
+ * reference's tree -- a particle event within dtPaul of paulTime, or a
+ * special event whose bucket has come up -- move on one bucket (refiling
+ * the overflow list when the ring wraps).  Then the earlier in the tree by
+ * (time, rank) is next.  Returns true for the special event.  A queue whose
+ * next event lies more than 2^24 buckets ahead is not walked bucket by
+ * bucket (the reference would); paulTime then jumps to that event. */
+__device__ __forceinline__ bool pick_next(const EdmdConst& C, const EdmdArrays& A, EdmdState* st,
+                                          LoopRegs& R, double key, int leaf) {
+    /* the particle event's rank matters only against a special at its time */
+    auto krank = [&]() { return key == R.next_special ? uni64(A.lrank[leaf]) : (int64_t)0; };
+    int64_t ws = special_window(st, R.special);
+    for (int guard = 0;; ++guard) {
+        const bool part_in = key - R.q->paul < C.dtp;
+        const bool spec_in = ws >= 0 && ws <= R.q->win;
+        if (part_in || spec_in)
+            return spec_in && (!part_in || before(R.next_special, R.q->srank, key, krank()));
+        if (!(key < 1e25) && !(R.next_special < 1e25)) return R.next_special <= key;
+        if (guard >= (1 << 24)) {
+            const double tn = key < R.next_special ? key : R.next_special;
+            while (!(tn - R.q->paul < C.dtp)) {
+                R.q->paul = uni(R.q->paul + C.dtp);
+                R.q->win++;
+            }
+            return before(R.next_special, R.q->srank, key, krank());
+        }
+        if (step_window(C, st, R)) {
+            next_special(st, R);
+            ws = special_window(st, R.special);
+        }
+    }
+}
+
+/* Rank of the next insertion; counts it. */
+__device__ __forceinline__ int64_t take_rank(const EdmdConst& C, LoopRegs& R, double t_ev) {
+    const int64_t r = rank_of(t_ev, R.q->paul, C.dtp, R.q->ctr);
+    R.q->ctr++;
+    return r;
 }
 
 __device__ __forceinline__ void trace_push(const EdmdConst& C, const EdmdArrays& A, int lane,
@@ -772,24 +1085,41 @@ __device__ __forceinline__ double ordered_sum(int n, int lane, F f) {
     return acc;
 }
 
-__device__ void ev_screenshot(const EdmdConst& C, const EdmdArrays& A, int lane, double t) {
+template <bool G>
+__device__ void ev_screenshot(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R) {
     /* takeAScreenshot, EDMD.c:4657-4677: free-fly every particle */
+    const double t = R.t;
     for (int q = lane; q < C.n; q += 64) {
         EdmdPart* r = A.p + q;
         double x = r->x, y = r->y, tl = r->tl;
-        fly(C, t, x, y, r->vx, r->vy, tl);
+        if (G) {
+            double rad = r->rad;
+            fly_grow(C, t, x, y, r->vx, r->vy, tl, rad, A.vr[q]);
+            r->rad = rad;
+        } else {
+            fly(C, t, x, y, r->vx, r->vy, tl);
+        }
         r->x = x;
         r->y = y;
         r->tl = tl;
     }
     WAVE_SYNC();
-    if (lane == 0) A.st->next_screen = t + C.dtime;
+    const int64_t wk = window_of(t + C.dtime, R.q->paul, C.dtp, C.nlists, R.q->win);
+    const int64_t rk = take_rank(C, R, t + C.dtime);
+    if (lane == 0) {
+        A.st->next_screen = t + C.dtime;
+        A.st->rank_screen = rk;
+        A.st->win_screen = wk;
+    }
     WAVE_SYNC();
 }
 
-__device__ void ev_thermo(const EdmdConst& C, const EdmdArrays& A, int lane, double t,
-                          uint64_t ncol) {
+__device__ void ev_thermo(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R) {
     /* takeAThermo + saveThermo, EDMD.c:4679-4713, 5232-5333 */
+    const double t = R.t;
+    const uint64_t ncol = R.ncol;
+    const int64_t wk = window_of(t + C.dtime_thermo, R.q->paul, C.dtp, C.nlists, R.q->win);
+    const int64_t rk = take_rank(C, R, t + C.dtime_thermo);
     const EdmdPart* P = A.p;
     double E = ordered_sum(C.n, lane, [&](int q) {
         double a = P[q].vx;
@@ -821,6 +1151,8 @@ __device__ void ev_thermo(const EdmdConst& C, const EdmdArrays& A, int lane, dou
         st->last_thermo = t;
         st->vir[0] = st->vir[1] = st->vir[2] = st->vir[3] = 0;
         st->next_thermo = t + C.dtime_thermo;
+        st->rank_thermo = rk;
+        st->win_thermo = wk;
     }
     WAVE_SYNC();
 }
@@ -833,6 +1165,7 @@ __device__ void rebuild_tree(const EdmdConst& C, const EdmdArrays& A, int lane) 
     }
 }
 
+template <bool G>
 __device__ void ev_rescale(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R) {
     /* addNoise, velocity-rescale branch (noise == 2): EDMD.c:4828-4832,
      * 4880-4915, 4922 */
@@ -845,18 +1178,28 @@ __device__ void ev_rescale(const EdmdConst& C, const EdmdArrays& A, int lane, Lo
     for (int q = lane; q < C.n; q += 64) {
         EdmdPart r = A.p[q];
         r.coll = r.coll + 1;
-        fly(C, t, r.x, r.y, r.vx, r.vy, r.tl);
+        if (G) fly_grow(C, t, r.x, r.y, r.vx, r.vy, r.tl, r.rad, A.vr[q]);
+        else fly(C, t, r.x, r.y, r.vx, r.vy, r.tl);
         r.vx /= sqrt(E / C.n / C.temperature);
         r.vy /= sqrt(E / C.n / C.temperature);
         A.p[q] = r;
     }
     WAVE_SYNC();
     bool ov = false;
-    for (int q = lane; q < C.n; q += 64) ov |= sweep_one(C, A, t, q);
+    /* per particle: crossing, then collision (EDMD.c:4909-4915) */
+    for (int q = lane; q < C.n; q += 64)
+        ov |= sweep_one<G>(C, A, t, q, R.q->paul, R.q->ctr + 2 * (int64_t)q, R.q->ctr + 2 * (int64_t)q + 1);
+    R.q->ctr += 2 * (int64_t)C.n;
     if (__ballot(ov) != 0ull) R.status = EDMD_DEV_OVERLAP;
     WAVE_SYNC();
     rebuild_tree(C, A, lane);
-    if (lane == 0) A.st->next_noise = t + C.dtnoise;
+    const int64_t wk = window_of(t + C.dtnoise, R.q->paul, C.dtp, C.nlists, R.q->win);
+    const int64_t rk = take_rank(C, R, t + C.dtnoise);
+    if (lane == 0) {
+        A.st->next_noise = t + C.dtnoise;
+        A.st->rank_noise = rk;
+        A.st->win_noise = wk;
+    }
     WAVE_SYNC();
 }
 
@@ -872,6 +1215,7 @@ __device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, 
                              int j) {
     /* doTheCollisionNormal, EDMD.c:3542-3562, 3583-3606, 3802-3827, 3869-3879 */
     const EdmdEv ei = A.e[i];
+    const int64_t rxi_old = A.erank[2 * i + 1]; /* crossing slot kept by a stale pop */
     const EdmdPart pi = A.p[i];
     const EdmdPart pj = A.p[j];
     PathCache<ML> P;
@@ -902,17 +1246,20 @@ __device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, 
         double dt, dtb;
         int pa, pb;
         uint32_t cpa, cpb;
-        predict_owners(C, A, lane, t, 1, oi, oi, dt, pa, cpa, dtb, pb, cpb, ov);
+        predict_owners<false>(C, A, lane, t, 1, oi, oi, dt, pa, cpa, dtb, pb, cpb, ov);
         flag_overlap(R, ov, i);
         const double tk = t + dt;
         const double tci = uni(ei.tcross);
+        const int64_t rci = uni64(rxi_old);
         const int diri = uni(ei.dir);
+        const int64_t rk = take_rank(C, R, tk);
         if (lane == 0) {
             A.e[i].tcoll = tk;
             A.e[i].partner = pa;
             A.e[i].cact = cpa;
+            A.erank[2 * i] = rk;
         }
-        path_update<ML>(C, A, lane, P, i, leaf_key(tk, tci), leaf_aux(tk, tci, pa, diri), -1, 0, 0);
+        path_update<ML>(C, A, lane, P, i, make_leaf(tk, rk, tci, rci, pa, diri), -1, Leaf{});
         return;
     }
     R.ncol++;
@@ -965,37 +1312,232 @@ __device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, 
     double dti, dtj;
     int pi_, pj_;
     uint32_t cpi, cpj;
-    predict_owners(C, A, lane, t, 2, oi, oj, dti, pi_, cpi, dtj, pj_, cpj, ov);
+    predict_owners<false>(C, A, lane, t, 2, oi, oj, dti, pi_, cpi, dtj, pj_, cpj, ov);
     flag_overlap(R, ov, i);
     double tci2, tcj2;
     int di, dj;
     cross_time(C, t, xi, yi, vxi, vyi, cxi, cyi, tci2, di);
     cross_time(C, t, xj, yj, vxj, vyj, cxj, cyj, tcj2, dj);
     const double tki = t + dti, tkj = t + dtj;
+    /* insertion order, EDMD.c:3869-3879: crossings of i and j, then the
+     * collisions of i and j */
+    const int64_t rxi = take_rank(C, R, tci2);
+    const int64_t rxj = take_rank(C, R, tcj2);
+    const int64_t rki = take_rank(C, R, tki);
+    const int64_t rkj = take_rank(C, R, tkj);
     if (lane == 0) {
         EdmdEv e;
         e.tcoll = tki; e.tcross = tci2; e.partner = pi_; e.cact = cpi; e.dir = di; e.pad = 0;
         A.e[i] = e;
         e.tcoll = tkj; e.tcross = tcj2; e.partner = pj_; e.cact = cpj; e.dir = dj;
         A.e[j] = e;
+        A.erank[2 * i] = rki;
+        A.erank[2 * i + 1] = rxi;
+        A.erank[2 * j] = rkj;
+        A.erank[2 * j + 1] = rxj;
     }
-    path_update<ML>(C, A, lane, P, i, leaf_key(tki, tci2), leaf_aux(tki, tci2, pi_, di), j,
-                    leaf_key(tkj, tcj2), leaf_aux(tkj, tcj2, pj_, dj));
+    path_update<ML>(C, A, lane, P, i, make_leaf(tki, rki, tci2, rxi, pi_, di), j,
+                    make_leaf(tkj, rkj, tcj2, rxj, pj_, dj));
 }
 
 template <int ML>
+__device__ void ev_collision_grow(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R,
+                                  int i, int j) {
+    /* doTheCollisionGrow, EDMD.c:3465-3540 (2D): both disks fly with their
+     * radii growing; the unit contact normal and the summed growth speed
+     * enter the impulse; res and delta are the compile-time 1 and 0 of
+     * EDMD.c:3467-3468; no virial. */
+    const double delta = 0.1 * 0;
+    const double res = 0.75 * 0 + 1;
+    const EdmdEv ei = A.e[i];
+    const int64_t rxi_old = A.erank[2 * i + 1]; /* crossing slot kept by a stale pop */
+    const EdmdPart pi = A.p[i];
+    const EdmdPart pj = A.p[j];
+    const double vri = uni(A.vr[i]), vrj = uni(A.vr[j]);
+    PathCache<ML> P;
+    path_prefetch<ML>(C, A, lane, i, j, P);
+    const double t = R.t;
+    double xi = uni(pi.x), yi = uni(pi.y), vxi = uni(pi.vx), vyi = uni(pi.vy), tli = uni(pi.tl);
+    double ri = uni(pi.rad);
+    const double mi = uni(pi.m);
+    const int cxi = uni((int)pi.cx), cyi = uni((int)pi.cy);
+    const uint32_t colli = (uint32_t)uni((int)pi.coll);
+    const uint32_t collj = (uint32_t)uni((int)pj.coll);
+    const uint32_t ca = (uint32_t)uni((int)ei.cact);
+
+    fly_grow(C, t, xi, yi, vxi, vyi, tli, ri, vri);
+    xi = uni(xi);
+    yi = uni(yi);
+    ri = uni(ri);
+    const bool valid = (ca == collj);
+    trace_push(C, A, lane, t, EDMD_EV_COLLISION, i, j, valid ? 1 : 0);
+    if (!valid) {
+        if (lane == 0) {
+            A.p[i].x = xi;
+            A.p[i].y = yi;
+            A.p[i].tl = t;
+            A.p[i].rad = ri;
+        }
+        WAVE_SYNC();
+        const Owner oi{i, xi, yi, vxi, vyi, ri, cxi, cyi, vri};
+        bool ov = false;
+        double dt, dtb;
+        int pa, pb;
+        uint32_t cpa, cpb;
+        predict_owners<true>(C, A, lane, t, 1, oi, oi, dt, pa, cpa, dtb, pb, cpb, ov);
+        flag_overlap(R, ov, i);
+        const double tk = t + dt;
+        const double tci = uni(ei.tcross);
+        const int64_t rci = uni64(rxi_old);
+        const int diri = uni(ei.dir);
+        const int64_t rk = take_rank(C, R, tk);
+        if (lane == 0) {
+            A.e[i].tcoll = tk;
+            A.e[i].partner = pa;
+            A.e[i].cact = cpa;
+            A.erank[2 * i] = rk;
+        }
+        path_update<ML>(C, A, lane, P, i, make_leaf(tk, rk, tci, rci, pa, diri), -1, Leaf{});
+        return;
+    }
+    R.ncol++;
+    double xj = uni(pj.x), yj = uni(pj.y), vxj = uni(pj.vx), vyj = uni(pj.vy), tlj = uni(pj.tl);
+    double rj = uni(pj.rad);
+    const double mj = uni(pj.m);
+    const int cxj = uni((int)pj.cx), cyj = uni((int)pj.cy);
+    fly_grow(C, t, xj, yj, vxj, vyj, tlj, rj, vrj);
+    double dx = xj - xi;
+    double dy = yj - yi;
+    double dvx = vxj - vxi;
+    double dvy = vyj - vyi;
+    double dvr = vri + vrj;
+    min_image(C, dx, dy);
+    double dist = sqrt(dx * dx + dy * dy);
+    double dxr = dx / dist;
+    double dyr = dy / dist;
+    double inv_mass = 1 / (mi + mj);
+    double kernel = inv_mass * (1 + res) * (dxr * dvx + dyr * dvy - dvr);
+    vxi = uni(vxi + (kernel * mj * dxr - 2 * mj * inv_mass * delta * dxr));
+    vyi = uni(vyi + (kernel * mj * dyr - 2 * mj * inv_mass * delta * dyr));
+    vxj = uni(vxj - (kernel * mi * dxr - 2 * mi * inv_mass * delta * dxr));
+    vyj = uni(vyj - (kernel * mi * dyr - 2 * mi * inv_mass * delta * dyr));
+    xj = uni(xj);
+    yj = uni(yj);
+    rj = uni(rj);
+    if (lane == 0) {
+        EdmdPart* q = A.p + i;
+        q->x = xi; q->y = yi; q->vx = vxi; q->vy = vyi; q->tl = t; q->rad = ri; q->coll = colli + 1;
+        q = A.p + j;
+        q->x = xj; q->y = yj; q->vx = vxj; q->vy = vyj; q->tl = t; q->rad = rj; q->coll = collj + 1;
+    }
+    WAVE_SYNC();
+    const Owner oi{i, xi, yi, vxi, vyi, ri, cxi, cyi, vri};
+    const Owner oj{j, xj, yj, vxj, vyj, rj, cxj, cyj, vrj};
+    bool ov = false;
+    double dti, dtj;
+    int pi_, pj_;
+    uint32_t cpi, cpj;
+    predict_owners<true>(C, A, lane, t, 2, oi, oj, dti, pi_, cpi, dtj, pj_, cpj, ov);
+    flag_overlap(R, ov, i);
+    double tci2, tcj2;
+    int di, dj;
+    cross_time(C, t, xi, yi, vxi, vyi, cxi, cyi, tci2, di);
+    cross_time(C, t, xj, yj, vxj, vyj, cxj, cyj, tcj2, dj);
+    const double tki = t + dti, tkj = t + dtj;
+    /* insertion order, EDMD.c:3869-3879: crossings of i and j, then the
+     * collisions of i and j */
+    const int64_t rxi = take_rank(C, R, tci2);
+    const int64_t rxj = take_rank(C, R, tcj2);
+    const int64_t rki = take_rank(C, R, tki);
+    const int64_t rkj = take_rank(C, R, tkj);
+    if (lane == 0) {
+        EdmdEv e;
+        e.tcoll = tki; e.tcross = tci2; e.partner = pi_; e.cact = cpi; e.dir = di; e.pad = 0;
+        A.e[i] = e;
+        e.tcoll = tkj; e.tcross = tcj2; e.partner = pj_; e.cact = cpj; e.dir = dj;
+        A.e[j] = e;
+        A.erank[2 * i] = rki;
+        A.erank[2 * i + 1] = rxi;
+        A.erank[2 * j] = rkj;
+        A.erank[2 * j + 1] = rxj;
+    }
+    path_update<ML>(C, A, lane, P, i, make_leaf(tki, rki, tci2, rxi, pi_, di), j,
+                    make_leaf(tkj, rkj, tcj2, rxj, pj_, dj));
+}
+
+/* stopGrow, EDMD.c:4740-4785 (2D, no wells): final flight with growth, all
+ * counters and image counters reset, normalizePhysicalQ (EDMD.c:5723-5764:
+ * remove the mean momentum, scale to E/N = Einit), then the Normal
+ * prediction of every disk with stopGrow's partner counters, and a new
+ * calendar.  The caller continues in the Normal loop. */
+__device__ void ev_stop_grow(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R) {
+    const double t = R.t;
+    for (int q = lane; q < C.n; q += 64) {
+        EdmdPart r = A.p[q];
+        fly_grow(C, t, r.x, r.y, r.vx, r.vy, r.tl, r.rad, A.vr[q]);
+        r.coll = 0;
+        A.p[q] = r;
+        A.img[2 * q] = 0;
+        A.img[2 * q + 1] = 0;
+    }
+    WAVE_SYNC();
+    EdmdPart* P = A.p;
+    const int n = C.n;
+    const double px = ordered_sum(n, lane, [&](int q) { return P[q].vx * P[q].m; });
+    const double py = ordered_sum(n, lane, [&](int q) { return P[q].vy * P[q].m; });
+    for (int q = lane; q < n; q += 64) {
+        const double m = P[q].m;
+        P[q].vx -= px / (n * m);
+        P[q].vy -= py / (n * m);
+    }
+    WAVE_SYNC();
+    const double E = ordered_sum(n, lane, [&](int q) {
+        double a = P[q].vx;
+        return 0.5 * P[q].m * (a * a + P[q].vy * P[q].vy);
+    });
+    for (int q = lane; q < n; q += 64) {
+        P[q].vx /= sqrt(E / n / C.einit);
+        P[q].vy /= sqrt(E / n / C.einit);
+        P[q].coll = 1;
+    }
+    WAVE_SYNC();
+    bool ov = false;
+    /* per particle: collision, then crossing (EDMD.c:4770-4776) */
+    for (int q = lane; q < n; q += 64)
+        ov |= sweep_one<false>(C, A, t, q, R.q->paul, R.q->ctr + 2 * (int64_t)q + 1,
+                               R.q->ctr + 2 * (int64_t)q, true);
+    R.q->ctr += 2 * (int64_t)n;
+    WAVE_SYNC();
+    flag_overlap(R, ov, 0);
+    rebuild_tree(C, A, lane);
+    if (lane == 0) {
+        A.st->growing = 0;
+        A.st->next_grow = INF_D;
+    }
+    WAVE_SYNC();
+}
+
+template <int ML, bool G>
 __device__ void ev_crossing(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R, int p,
                             int d) {
-    /* doTheCrossing, EDMD.c:2501-2590 */
+    /* doTheCrossing, EDMD.c:2501-2590 (its freeFly is the dispatch pointer:
+     * freeFlyGrow while growing) */
     const EdmdPart pp = A.p[p];
+    const double gvr = G ? uni(A.vr[p]) : 0.0;
     PathCache<ML> P;
     path_prefetch<ML>(C, A, lane, p, -1, P);
     const double t = R.t;
     double x = uni(pp.x), y = uni(pp.y), tl = uni(pp.tl);
-    const double vx = uni(pp.vx), vy = uni(pp.vy), r = uni(pp.rad);
+    const double vx = uni(pp.vx), vy = uni(pp.vy);
+    double r = uni(pp.rad);
     int cx = uni((int)pp.cx), cy = uni((int)pp.cy);
     trace_push(C, A, lane, t, EDMD_EV_CELLCROSS, p, d, 1);
-    fly(C, t, x, y, vx, vy, tl);
+    if (G) {
+        fly_grow(C, t, x, y, vx, vy, tl, r, gvr);
+        r = uni(r);
+    } else {
+        fly(C, t, x, y, vx, vy, tl);
+    }
     x = uni(x);
     y = uni(y);
 
@@ -1041,46 +1583,60 @@ __device__ void ev_crossing(const EdmdConst& C, const EdmdArrays& A, int lane, L
         q->x = x;
         q->y = y;
         q->tl = tl;
+        if (G) q->rad = r;
         q->cx = (int16_t)cx;
         q->cy = (int16_t)cy;
         if (wx) A.img[2 * p] += wx;
         if (wy) A.img[2 * p + 1] += wy;
     }
     WAVE_SYNC();
-    const Owner o{p, x, y, vx, vy, r, cx, cy};
+    const Owner o{p, x, y, vx, vy, r, cx, cy, gvr};
     bool ov = false;
     double dt, dtb;
     int pa, pb;
     uint32_t cpa, cpb;
-    predict_owners(C, A, lane, t, 1, o, o, dt, pa, cpa, dtb, pb, cpb, ov);
+    predict_owners<G>(C, A, lane, t, 1, o, o, dt, pa, cpa, dtb, pb, cpb, ov);
     flag_overlap(R, ov, p);
     double tc;
     int dir;
     cross_time(C, t, x, y, vx, vy, cx, cy, tc, dir);
     const double tk = t + dt;
+    /* crossingEvent(i), then collisionEvent(i), EDMD.c:2587-2589 */
+    const int64_t rx = take_rank(C, R, tc);
+    const int64_t rk = take_rank(C, R, tk);
     if (lane == 0) {
         EdmdEv e;
         e.tcoll = tk; e.tcross = tc; e.partner = pa; e.cact = cpa; e.dir = dir; e.pad = 0;
         A.e[p] = e;
+        A.erank[2 * p] = rk;
+        A.erank[2 * p + 1] = rx;
     }
-    path_update<ML>(C, A, lane, P, p, leaf_key(tk, tc), leaf_aux(tk, tc, pa, dir), -1, 0, 0);
+    path_update<ML>(C, A, lane, P, p, make_leaf(tk, rk, tc, rx, pa, dir), -1, Leaf{});
 }
 
 /* Persistent event loop, one wavefront per system (blockIdx.x); ML bounds
  * the calendar depth of every system in the launch. */
-template <int ML>
+template <int ML, bool G>
 __device__ __forceinline__ void run_loop(const EdmdSys* __restrict__ systems, EdmdRunArgs ra) {
     const EdmdConst& C = systems[blockIdx.x].c;
     const EdmdArrays& A = systems[blockIdx.x].a;
     const int lane = threadIdx.x;
     EdmdState* st = A.st;
+    /* G launches run the systems that are growing, the others the rest */
+    if ((uni(st->growing) != 0) != G) return;
+    __shared__ QueueState Qs;
     LoopRegs R;
+    R.q = &Qs;
     R.t = uni(st->t);
     R.ncol = st->ncol;
     R.nevents = st->nevents;
     R.status = st->status;
     R.err_i = st->err_i;
     R.err_j = st->err_j;
+    R.q->paul = uni(st->paul);
+    R.q->ctr = uni64(st->ctr);
+    R.q->win = uni64(st->win);
+    R.q->cur = uni((int)(R.q->win % C.nlists));
     next_special(st, R);
     uint64_t ncross = st->ncross;
     int reason = EDMD_STOP_NONE;
@@ -1096,26 +1652,38 @@ __device__ __forceinline__ void run_loop(const EdmdSys* __restrict__ systems, Ed
         double key;
         int p, aux;
         top_min(C, A, lane, key, p, aux);
-        if (!(key < R.next_special)) {
-            if (!(R.next_special < INF_D)) { reason = EDMD_STOP_EMPTY; break; }
-            R.t = R.next_special;
+        if (!(key < INF_D) && !(R.next_special < INF_D)) { reason = EDMD_STOP_EMPTY; break; }
+        if (pick_next(C, A, st, R, key, p)) {
             const int sp = R.special;
+            R.t = R.next_special;
             trace_push(C, A, lane, R.t, sp, 0, 0, 1);
-            if (sp == EDMD_EV_SCREENSHOT) ev_screenshot(C, A, lane, R.t);
-            else if (sp == EDMD_EV_THERMO) ev_thermo(C, A, lane, R.t, R.ncol);
-            else ev_rescale(C, A, lane, R);
+            if (sp == EDMD_EV_SCREENSHOT) {
+                ev_screenshot<G>(C, A, lane, R);
+            } else if (sp == EDMD_EV_THERMO) {
+                ev_thermo(C, A, lane, R);
+            } else if (sp == EDMD_EV_GROWSTOP) {
+                if (G) {
+                    ev_stop_grow(C, A, lane, R);
+                    R.ncol = 0; /* stopGrow resets ncol and ncross, EDMD.c:4745-4746 */
+                    ncross = 0;
+                }
+            } else {
+                ev_rescale<G>(C, A, lane, R);
+            }
             next_special(st, R);
             R.nevents++;
             last_type = sp;
+            if (G && sp == EDMD_EV_GROWSTOP) { reason = EDMD_STOP_GROWN; break; }
             if (ra.stop_mask & (1 << sp)) { reason = EDMD_STOP_SPECIAL; break; }
             continue;
         }
         R.t = key;
         if (aux >= 0) {
-            ev_collision<ML>(C, A, lane, R, p, aux);
+            if (G) ev_collision_grow<ML>(C, A, lane, R, p, aux);
+            else ev_collision<ML>(C, A, lane, R, p, aux);
             last_type = EDMD_EV_COLLISION;
         } else {
-            ev_crossing<ML>(C, A, lane, R, p, -aux);
+            ev_crossing<ML, G>(C, A, lane, R, p, -aux);
             ncross++;
             last_type = EDMD_EV_CELLCROSS;
         }
@@ -1133,23 +1701,26 @@ __device__ __forceinline__ void run_loop(const EdmdSys* __restrict__ systems, Ed
         st->err_j = R.err_j;
         st->stop_reason = reason;
         st->last_type = last_type;
+        st->paul = R.q->paul;
+        st->ctr = R.q->ctr;
+        st->win = R.q->win;
     }
 }
 
 /* Systems of up to 64^2 disks (calendar depth <= 2): bounded to 128 VGPRs so
  * that four waves share each SIMD and a batch of 4096 replicas is resident
  * at once on 256 CUs. */
-template <int ML>
+template <int ML, bool G>
 __global__ __launch_bounds__(64, 4) void k_run_small(const EdmdSys* __restrict__ systems,
                                                      EdmdRunArgs ra) {
-    run_loop<ML>(systems, ra);
+    run_loop<ML, G>(systems, ra);
 }
 
 /* Deeper calendars (large systems, typically one per launch): no register
  * bound, so the path caches stay in registers. */
-template <int ML>
+template <int ML, bool G>
 __global__ __launch_bounds__(64) void k_run(const EdmdSys* __restrict__ systems, EdmdRunArgs ra) {
-    run_loop<ML>(systems, ra);
+    run_loop<ML, G>(systems, ra);
 }
 
 /* ------------------------------------------------------------ single steps */
@@ -1163,11 +1734,14 @@ __global__ __launch_bounds__(64) void k_peek(const EdmdSys* __restrict__ systems
     double key;
     int p, aux;
     top_min(C, A, lane, key, p, aux);
+    __shared__ QueueState Qs;
     LoopRegs R;
+    R.q = &Qs;
     next_special(A.st, R);
+    const int64_t krank = key == R.next_special ? uni64(A.lrank[p]) : (int64_t)0;
     if (lane != 0) return;
     EdmdPeek r;
-    if (!(key < R.next_special)) {
+    if (!before(key, krank, R.next_special, R.q->srank)) {
         r.t = R.next_special; r.type = R.special; r.i = 0; r.j = 0; r.coll_actual = 0;
     } else if (aux >= 0) {
         r.t = key; r.type = EDMD_EV_COLLISION; r.i = p; r.j = aux; r.coll_actual = A.e[p].cact;
@@ -1185,7 +1759,11 @@ __global__ void k_pair_time(const EdmdSys* __restrict__ systems, int i, int j, d
     bool ov = false;
     const double t = A.st->t;
     const EdmdPart a = A.p[i], b = A.p[j];
-    out[0] = coll_time(C, t, a.x, a.y, a.vx, a.vy, a.rad, b.x, b.y, b.vx, b.vy, b.tl, b.rad, ov);
+    if (A.st->growing)
+        out[0] = coll_time_grow(C, t, a.x, a.y, a.vx, a.vy, a.rad, A.vr[i], b.x, b.y, b.vx, b.vy,
+                                b.tl, b.rad, A.vr[j], ov);
+    else
+        out[0] = coll_time(C, t, a.x, a.y, a.vx, a.vy, a.rad, b.x, b.y, b.vx, b.vy, b.tl, b.rad, ov);
     out[1] = ov ? 1.0 : 0.0;
 }
 
@@ -1203,22 +1781,41 @@ __global__ __launch_bounds__(64) void k_repredict(const EdmdSys* __restrict__ sy
     EdmdEv e = A.e[i];
     PathCache<ML> P;
     path_prefetch<ML>(C, A, lane, i, -1, P);
-    if (which & 1) cross_time(C, t, pi.x, pi.y, pi.vx, pi.vy, pi.cx, pi.cy, e.tcross, e.dir);
+    const double paul = A.st->paul;
+    int64_t ctr = A.st->ctr;
+    int64_t rcoll = A.erank[2 * i], rcross = A.erank[2 * i + 1];
+    if (which & 1) {
+        cross_time(C, t, pi.x, pi.y, pi.vx, pi.vy, pi.cx, pi.cy, e.tcross, e.dir);
+        rcross = rank_of(e.tcross, paul, C.dtp, ctr++);
+    }
     bool ov = false;
     if (which & 2) {
-        const Owner o{i, pi.x, pi.y, pi.vx, pi.vy, pi.rad, pi.cx, pi.cy};
+        const bool grow = uni(A.st->growing) != 0;
+        const Owner o{i, pi.x, pi.y, pi.vx, pi.vy, pi.rad, pi.cx, pi.cy, grow ? A.vr[i] : 0.0};
         double dt, dtb;
         int pb;
         uint32_t cpb;
-        predict_owners(C, A, lane, t, 1, o, o, dt, e.partner, e.cact, dtb, pb, cpb, ov);
+        if (grow)
+            predict_owners<true>(C, A, lane, t, 1, o, o, dt, e.partner, e.cact, dtb, pb, cpb, ov);
+        else
+            predict_owners<false>(C, A, lane, t, 1, o, o, dt, e.partner, e.cact, dtb, pb, cpb, ov);
         e.tcoll = t + dt;
+        rcoll = rank_of(e.tcoll, paul, C.dtp, ctr++);
     }
-    if (lane == 0) A.e[i] = e;
-    path_update<ML>(C, A, lane, P, i, leaf_key(e.tcoll, e.tcross),
-                    leaf_aux(e.tcoll, e.tcross, e.partner, e.dir), -1, 0, 0);
-    if (lane == 0 && __ballot(ov) != 0ull) {
-        A.st->status = EDMD_DEV_OVERLAP;
-        A.st->err_i = i;
+    if (lane == 0) {
+        A.e[i] = e;
+        A.erank[2 * i] = rcoll;
+        A.erank[2 * i + 1] = rcross;
+    }
+    path_update<ML>(C, A, lane, P, i, make_leaf(e.tcoll, rcoll, e.tcross, rcross, e.partner, e.dir),
+                    -1, Leaf{});
+    const bool any_ov = __ballot(ov) != 0ull;
+    if (lane == 0) {
+        A.st->ctr = ctr;
+        if (any_ov) {
+            A.st->status = EDMD_DEV_OVERLAP;
+            A.st->err_i = i;
+        }
     }
 }
 
@@ -1228,7 +1825,13 @@ __global__ void k_fly_one(const EdmdSys* __restrict__ systems, int i) {
     const EdmdArrays& A = systems[0].a;
     EdmdPart* r = A.p + i;
     double x = r->x, y = r->y, tl = r->tl;
-    fly(C, A.st->t, x, y, r->vx, r->vy, tl);
+    if (A.st->growing) {
+        double rad = r->rad;
+        fly_grow(C, A.st->t, x, y, r->vx, r->vy, tl, rad, A.vr[i]);
+        r->rad = rad;
+    } else {
+        fly(C, A.st->t, x, y, r->vx, r->vy, tl);
+    }
     r->x = x;
     r->y = y;
     r->tl = tl;
@@ -1335,12 +1938,30 @@ hipError_t edmd_launch_sync_all(const EdmdSys* d_sys, int nsys, int max_n, hipSt
     return hipGetLastError();
 }
 
-hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, EdmdRunArgs ra,
+template <bool G>
+static void launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, EdmdRunArgs ra, hipStream_t s) {
+    if (max_nlev <= 1) hipLaunchKernelGGL((k_run_small<1, G>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    else if (max_nlev == 2) hipLaunchKernelGGL((k_run_small<2, G>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    else if (max_nlev <= 4) hipLaunchKernelGGL((k_run<4, G>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    else hipLaunchKernelGGL((k_run<EDMD_MAX_LEVELS, G>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
+}
+
+hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, int grow, EdmdRunArgs ra,
                            hipStream_t s) {
-    if (max_nlev <= 1) hipLaunchKernelGGL(k_run_small<1>, dim3(nsys), dim3(64), 0, s, d_sys, ra);
-    else if (max_nlev == 2) hipLaunchKernelGGL(k_run_small<2>, dim3(nsys), dim3(64), 0, s, d_sys, ra);
-    else if (max_nlev <= 4) hipLaunchKernelGGL(k_run<4>, dim3(nsys), dim3(64), 0, s, d_sys, ra);
-    else hipLaunchKernelGGL(k_run<EDMD_MAX_LEVELS>, dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    if (grow) launch_run<true>(d_sys, nsys, max_nlev, ra, s);
+    else launch_run<false>(d_sys, nsys, max_nlev, ra, s);
+    return hipGetLastError();
+}
+
+hipError_t edmd_launch_sweep_grow(const EdmdSys* d_sys, int nsys, int max_n, hipStream_t s) {
+    if (nsys <= 0 || max_n <= 0) return hipSuccess;
+    dim3 grid((max_n + 255) / 256, nsys);
+    hipLaunchKernelGGL(k_sweep_grow, grid, dim3(256), 0, s, d_sys);
+    return hipGetLastError();
+}
+
+hipError_t edmd_launch_sweep_count(const EdmdSys* d_sys, int nsys, hipStream_t s) {
+    hipLaunchKernelGGL(k_sweep_count, dim3((nsys + 255) / 256), dim3(256), 0, s, d_sys, nsys);
     return hipGetLastError();
 }
 

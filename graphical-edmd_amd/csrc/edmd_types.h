@@ -7,13 +7,16 @@
  *                (u32), cell x / y (i16) -- one cache-line half per disk,
  *                gathered whole when the disk is a collision candidate
  *   event        32 B: collision slot (time, partner, partner's collision
- *                counter at prediction) and crossing slot (time, direction),
+ *                counter at prediction) and crossing slot (time, direction);
+ *                their ranks among events of equal time sit in a side array
+ *                (erank) that is read only when two keys are equal,
  *                the two slots the reference keeps per particle in
  *                eventList[N+i] and eventList[i] (EDMD.c:1935-1948)
  *   calendar     16 B nodes {key, leaf, event code}: one leaf per particle
- *                (key = earliest of its two slots) and a 64-ary tournament
- *                tree above the leaves; every node keeps the minimum key of
- *                its 64 children, the leaf holding it and that leaf's event
+ *                (key = earliest of its two slots by time, then rank) and a
+ *                64-ary tournament tree above the leaves; every node keeps the
+ *                minimum (key, rank) of its 64 children (a node's rank is that
+ *                of its leaf, lrank[leaf]), the leaf holding it and that leaf's event
  *                code (partner index for a collision, minus the direction
  *                for a crossing), so one read of the top level names both
  *                disks of the next event
@@ -36,6 +39,11 @@
 #define EDMD_EV_SCREENSHOT 2
 #define EDMD_EV_THERMO 3
 #define EDMD_EV_ADDINGNOISE 4
+#define EDMD_EV_GROWSTOP 9
+
+/* collisionEventGrow's starting minimum (EDMD.c:3106): no partner found
+ * within 1e7 time units gives partner 0 at t + 1e7 */
+#define EDMD_GROW_NONE 10000000.0
 
 /* Why a run launch returned. */
 #define EDMD_STOP_NONE 0
@@ -45,6 +53,7 @@
 #define EDMD_STOP_ERROR 4
 #define EDMD_STOP_EMPTY 5
 #define EDMD_STOP_SPECIAL 6
+#define EDMD_STOP_GROWN 7 /* growth ended; the run continues in the Normal loop */
 
 /* Device status codes (mirrored in include/edmd.h). */
 #define EDMD_DEV_OK 0
@@ -60,7 +69,27 @@ typedef struct EdmdConst {
     int32_t tape_cap;                      /* collision tape records kept, 0 = no tape */
     double Lx, Ly, halfLx, halfLy, csx, csy;
     double res, dtime, dtime_thermo, dtnoise, temperature;
+    double einit; /* Einit: E/N after the growth stops (EDMD.c:369, 5759-5763) */
+    double dtp;   /* dtPaul = 5/N, the reference's bucket width (EDMD.c:712-714) */
+    int32_t nlists; /* bucket count of the reference's queue (N) */
+    int32_t pad_c;
 } EdmdConst;
+
+/* Order among events of equal time.  The reference pops equal times in the
+ * order its queue holds them (EDMD.c:2144-2327): events scheduled within
+ * dtPaul of the current bucket start (paulTime) go straight into the tree,
+ * where equal keys go right, so the earlier insertion pops first; later
+ * events wait in LIFO bucket lists that are drained into the emptied tree
+ * when their bucket comes up, so of those the later insertion pops first,
+ * and all of them before anything inserted directly afterwards.  Every
+ * scheduled event gets a rank from a per-system insertion counter:
+ *   direct:  EDMD_RANK_DIRECT + counter
+ *   bucket:  EDMD_RANK_DIRECT - 1 - counter
+ * and the calendar orders by (time, rank).  Events of the overflow list
+ * (scheduled 5 or more time units ahead) are filed again when the bucket
+ * ring wraps, before the new bucket is drained; a special event filed
+ * straight into the tree then gets rank = counter, ahead of both classes. */
+#define EDMD_RANK_DIRECT ((int64_t)1 << 62)
 
 typedef struct EdmdState {
     double t;
@@ -74,6 +103,14 @@ typedef struct EdmdState {
     int64_t ntape;              /* collisions taped since the tape was cleared */
     double wsum;                /* sum of m_i m_j ff |a|^2 over taped collisions */
     double tlast;               /* time of the last taped collision */
+    double next_grow;           /* GROWSTOP time (1/vr) while growing, else +inf */
+    int32_t growing;            /* radii grow (freeFlyGrow, collisionEventGrow) */
+    int32_t pad_grow;
+    double paul;                /* the reference's paulTime (bucket start) */
+    int64_t ctr;                /* events scheduled so far (rank counter) */
+    int64_t rank_screen, rank_thermo, rank_noise, rank_grow; /* ranks of the special events */
+    int64_t win;                /* buckets passed (paulTime = win additions of dtPaul) */
+    int64_t win_screen, win_thermo, win_noise, win_grow; /* their buckets, -1 = overflow list */
 } EdmdState;
 
 /* One valid collision as the derivative kernels need it: time, contact
@@ -120,7 +157,7 @@ typedef struct __attribute__((aligned(32))) EdmdEv {
 
 typedef struct __attribute__((aligned(16))) EdmdNode {
     double key;  /* sanitised: NaN -> +inf */
-    int32_t idx; /* leaf holding the key */
+    int32_t idx; /* leaf holding the key; the event's rank is lrank[idx] */
     int32_t aux; /* that leaf's event: partner (>= 0) or -direction */
 } EdmdNode;
 
@@ -135,6 +172,9 @@ typedef struct EdmdArrays {
     EdmdState* st;
     EdmdTraceRec* trace;
     EdmdThermoRow* thermo;
+    double* vr;          /* radius growth rate per particle (growth starts) */
+    int64_t* erank;      /* ranks of the slots: [2p] collision, [2p + 1] crossing */
+    int64_t* lrank;      /* rank of each leaf's key (the earlier slot) */
     EdmdTapeRec* tape;   /* tape_cap records */
     double* dv;          /* tangent or adjoint vector, 4 per particle */
     double* dtl;         /* the time each particle's dv refers to */

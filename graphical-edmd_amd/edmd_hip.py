@@ -40,7 +40,7 @@ class SystemDesc(C.Structure):
                 ("dtime_thermo", C.c_double), ("noise", C.c_int),
                 ("dtnoise", C.c_double), ("temperature", C.c_double),
                 ("cell_capacity", C.c_int), ("trace_capacity", C.c_int64),
-                ("thermo_capacity", C.c_int)]
+                ("thermo_capacity", C.c_int), ("vr", C.c_double), ("einit", C.c_double)]
 
 
 class Observables(C.Structure):
@@ -103,6 +103,7 @@ def lib():
             "edmd_batch_get_state": (i32, [vp, i32] + [vp] * 8),
             "edmd_batch_get_events": (i32, [vp, i32] + [vp] * 4),
             "edmd_batch_get_trace": (i32, [vp, i32, vp, i64, vp]),
+            "edmd_batch_get_calendar": (i32, [vp, i32, vp, vp, vp, i64, vp, vp, vp]),
             "edmd_batch_get_thermo": (i32, [vp, i32, vp, i32, vp]),
             "edmd_batch_size": (i32, [vp, vp]),
             "edmd_batch_stream": (vp, [vp]),
@@ -122,6 +123,8 @@ def lib():
             "edmd_batch_deriv_values": (i32, [vp, i32, vp, vp, vp]),
             "edmd_batch_get_tape": (i32, [vp, i32, vp, i64, vp]),
             "edmd_batch_deriv_end": (i32, [vp]),
+            "edmd_batch_get_radii": (i32, [vp, i32, vp]),
+            "edmd_get_radii": (i32, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -194,6 +197,11 @@ class System:
     def sweep_predict(self):
         check(lib().edmd_sweep_predict(self.h))
 
+    def radii(self):
+        out = np.zeros(self.n)
+        check(lib().edmd_get_radii(self.h, out.ctypes.data))
+        return out
+
     def state(self):
         n = self.n
         out = {k: np.zeros(n) for k in ("x", "y", "vx", "vy", "t")}
@@ -261,11 +269,16 @@ class System:
 class Batch:
     """Many independent systems, one wavefront each."""
 
-    def __init__(self, states, **kw):
+    def __init__(self, states, per_system=None, **kw):
+        """kw: descriptor fields for every system; per_system: field -> list
+        of per-system values."""
         keep = []
         descs = (SystemDesc * len(states))()
         for k, st in enumerate(states):
-            descs[k] = make_desc(st, keep, **kw)
+            own = dict(kw)
+            for f, vals in (per_system or {}).items():
+                own[f] = vals[k]
+            descs[k] = make_desc(st, keep, **own)
         h = C.c_void_p()
         check(lib().edmd_batch_create(descs, len(states), C.byref(h)))
         self.h = h
@@ -287,6 +300,11 @@ class Batch:
 
     def sweep_predict(self):
         check(lib().edmd_batch_sweep_predict(self.h))
+
+    def radii(self, s):
+        out = np.zeros(self.ns[s])
+        check(lib().edmd_batch_get_radii(self.h, int(s), out.ctypes.data))
+        return out
 
     def time_sweep(self, reps):
         ms = C.c_double()
@@ -316,6 +334,23 @@ class Batch:
         check(lib().edmd_batch_get_events(self.h, int(s), *[_ptr(out[k]) for k in
                                                             ("t_coll", "partner", "t_cross",
                                                              "dir")]))
+        return out
+
+    def calendar(self, s):
+        """The event calendar (leaves first, then each level) with the
+        insertion ranks of the slots and leaves."""
+        n = self.ns[s]
+        tot = C.c_int64()
+        check(lib().edmd_batch_get_calendar(self.h, int(s), None, None, None, 0, C.byref(tot),
+                                            None, None))
+        m = tot.value
+        out = {"key": np.zeros(m), "idx": np.zeros(m, dtype=np.int32),
+               "aux": np.zeros(m, dtype=np.int32),
+               "slot_rank": np.zeros(2 * n, dtype=np.int64),
+               "leaf_rank": np.zeros(n, dtype=np.int64)}
+        check(lib().edmd_batch_get_calendar(self.h, int(s), _ptr(out["key"]), _ptr(out["idx"]),
+                                            _ptr(out["aux"]), m, C.byref(tot),
+                                            _ptr(out["slot_rank"]), _ptr(out["leaf_rank"])))
         return out
 
     def trace(self, s, cap=None):

@@ -41,6 +41,7 @@ extern "C" {
 #define EDMD_EV_SCREENSHOT 2
 #define EDMD_EV_THERMO 3
 #define EDMD_EV_ADDINGNOISE 4
+#define EDMD_EV_GROWSTOP 9
 
 /* ---- why a run returned ------------------------------------------------- */
 #define EDMD_STOP_NONE 0
@@ -50,6 +51,7 @@ extern "C" {
 #define EDMD_STOP_ERROR 4
 #define EDMD_STOP_EMPTY 5
 #define EDMD_STOP_SPECIAL 6 /* stopped after an event kind listed in a stop mask */
+#define EDMD_STOP_GROWN 7   /* growth ended (internal: the run continues) */
 
 /* One system: N disks in a periodic Lx x Ly box.  Replaces the state that
  * constantInit/particlesInit build (EDMD.c:745-1219, 1356-1451).  rad is a
@@ -69,6 +71,14 @@ typedef struct edmd_system_desc {
     int cell_capacity;      /* max disks per cell, 0 = derived from the radii */
     int64_t trace_capacity; /* device event-trace records kept, 0 = none */
     int thermo_capacity;    /* thermo rows kept, 0 = 4096 */
+    /* Growth start (the reference's default start, EDMD.c:1691-1810,
+     * 1957-1981): vr > 0 starts every disk at radius 0 growing at
+     * vr * rad[i] (rad[] holds the target radii) until GROWSTOP at
+     * t = 1/vr, when stopGrow (EDMD.c:4740-4785) sets E/N = einit and the
+     * Normal dynamics begin; screenshot and thermo times below 1/vr are
+     * shifted by 1/vr as EDMD.c:1959-1964 does. */
+    double vr;
+    double einit;           /* Einit, EDMD.c:369 (-E); default 1 */
 } edmd_system_desc;
 
 typedef struct edmd_trace_rec {
@@ -161,6 +171,17 @@ int edmd_batch_get_state(edmd_batch* b, int sys, double* x, double* y, double* v
 int edmd_batch_get_events(edmd_batch* b, int sys, double* t_coll, int32_t* partner,
                           double* t_cross, int32_t* dir);
 int edmd_batch_get_trace(edmd_batch* b, int sys, edmd_trace_rec* out, int64_t cap, int64_t* n);
+/* Diagnostic, no counterpart in the reference (whose queue is a pointer
+ * tree): the event calendar of one system, for consistency checks.  Nodes
+ * level by level, the leaves (one per disk) first; each node holds the
+ * earliest key below it, the leaf it came from and that leaf's event
+ * (partner >= 0 or -direction).  slot_rank (2 per disk: collision,
+ * crossing) and leaf_rank (1 per disk) are the insertion ranks that order
+ * equal times as the reference's queue does.  Any output may be NULL;
+ * *nnodes receives the node count, cap bounds key/idx/aux. */
+int edmd_batch_get_calendar(edmd_batch* b, int sys, double* key, int32_t* idx, int32_t* aux,
+                            int64_t cap, int64_t* nnodes, int64_t* slot_rank,
+                            int64_t* leaf_rank);
 int edmd_batch_get_thermo(edmd_batch* b, int sys, edmd_thermo_row* out, int cap, int* n);
 int edmd_batch_size(edmd_batch* b, int* nsys);
 void* edmd_batch_stream(edmd_batch* b);
@@ -190,6 +211,10 @@ int edmd_dump_read(const char* path, int n, double* x, double* y, double* vx, do
 int edmd_dump_write(const char* path, int append, double t, int n, double Lx, double Ly,
                     const double* x, const double* y, const double* vx, const double* vy,
                     const double* rad, const double* m, int full_precision);
+
+/* Current radius of every disk of system sys (they change while growing). */
+int edmd_batch_get_radii(edmd_batch* b, int sys, double* rad);
+int edmd_get_radii(edmd_ctx* ctx, double* rad);
 
 /* ---- observables ----------------------------------------------------------- */
 /* calculate_pcf (pcf.c:16-75) on the GPU: g(r) of n positions with the

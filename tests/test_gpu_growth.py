@@ -1,0 +1,99 @@
+"""GPU growth start (the reference's default start: disks grow from radius 0
+until GROWSTOP, then stopGrow normalises and the Normal dynamics begin)
+against the reference's golden vectors and the oracle, bit for bit."""
+import numpy as np
+import pytest
+
+import edmd_fixtures as F
+from conftest import GROWN, bits, golden_state, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_trace_equal(got, ref):
+    """Exact except for the orientation of mirrored pair events scheduled
+    more than 5 time units ahead (frequent at the start of a growth, when the
+    disks are small): the reference's overflow list reorders those when its
+    bucket ring wraps (EDMD.c:2287-2307), which the calendar here does not;
+    both orientations are the same collision."""
+    a, b = F.canonical_trace(got), F.canonical_trace(ref)
+    for k in ("type", "i", "j", "valid"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    np.testing.assert_array_equal(bits(a["t"]), bits(b["t"]))
+
+
+@pytest.mark.parametrize("name", GROWN)
+def test_growth_matches_reference(gpu, name):
+    g = load_golden(name)
+    nev = len(g["trace_t"])
+    s = gpu.System(golden_state(g), vr=float(g["vr"]), trace_capacity=nev)
+    s.run(max_events=nev)
+    ref = np.zeros(nev, dtype=gpu.TRACE_DT)
+    for k in ("type", "i", "j", "valid", "t"):
+        ref[k] = g["trace_" + k]
+    assert_trace_equal(s.trace(), ref)
+    st = s.state()
+    for k in ("x", "y", "vx", "vy", "t"):
+        np.testing.assert_array_equal(bits(st[k]), bits(g["state_" + k]), err_msg=k)
+    np.testing.assert_array_equal(st["coll"], g["state_coll"].astype(np.uint32))
+    np.testing.assert_array_equal(bits(s.radii()), bits(g["state_rad"]))
+    ob = s.observables()
+    for k in ("t", "E", "px", "py"):
+        assert bits(ob[k]) == bits(g["glob_" + k]), k
+    assert ob["ncol"] == int(g["glob_ncol"]) and ob["ncross"] == int(g["glob_ncross"])
+
+
+def test_mixed_batch_growing_and_normal_match_oracle(gpu, oracle):
+    """Growing and non-growing systems in one batch; growth rates differ so
+    GROWSTOP falls at different times; runs cross GROWSTOP in the middle of
+    a call and continue in the Normal loop."""
+    cases = [(300, 0.55, 11, 0.0, 0.2), (256, 0.72, 12, 0.1, 0.0125), (400, 0.6, 13, 0.0, 0.0),
+             (150, 0.5, 14, 0.1, 0.1), (500, 0.65, 15, 0.1, 0.05)]
+    states, vrs = [], []
+    for n, phi, seed, poly, vr in cases:
+        if vr > 0:
+            states.append(F.growth_state(n, phi, seed, poly=poly))
+        else:
+            states.append(F.lattice_state(n, phi, seed, lattice="fit"))
+        vrs.append(vr)
+    nev = 40000
+    b = gpu.Batch(states, trace_capacity=nev, per_system={"vr": vrs})
+    b.run(max_events=nev // 2)
+    b.run(max_events=nev)
+    for k, (st, vr) in enumerate(zip(states, vrs)):
+        o = oracle.Oracle(st, vr=vr)
+        code, tr = o.run(nev)
+        assert code == 0
+        assert_trace_equal(b.trace(k), tr)
+        so, sb = o.state(), b.state(k)
+        for f in ("x", "y", "vx", "vy", "t"):
+            np.testing.assert_array_equal(bits(sb[f]), bits(so[f]), err_msg="%d %s" % (k, f))
+        np.testing.assert_array_equal(bits(b.radii(k)), bits(o.radii()))
+        assert b.observables(k)["ncol"] == o.observables()["ncol"]
+
+
+def test_growth_under_thermostat_matches_oracle(gpu, oracle):
+    """Velocity rescaling during the growth uses the growth free flight and
+    growth predictions (addNoise through the dispatch pointers)."""
+    st = F.growth_state(400, 0.7, seed=21, poly=0.1)
+    kw = dict(vr=0.05, noise=2, dtnoise=0.5, temperature=1.0)
+    nev = 50000
+    s = gpu.System(st, trace_capacity=nev, **kw)
+    s.run(max_events=nev)
+    o = oracle.Oracle(st, **kw)
+    code, tr = o.run(nev)
+    assert code == 0
+    assert_trace_equal(s.trace(), tr)
+    so, sg = o.state(), s.state()
+    for f in ("x", "y", "vx", "vy", "t"):
+        np.testing.assert_array_equal(bits(sg[f]), bits(so[f]), err_msg=f)
+    np.testing.assert_array_equal(bits(s.radii()), bits(o.radii()))
+
+
+def test_growth_blocks_derivatives(gpu):
+    b = gpu.Batch([F.growth_state(200, 0.5, seed=2)], vr=0.1)
+    with pytest.raises(gpu.EdmdError) as e:
+        b.deriv_begin()
+    assert e.value.code == 22
+    b.run(max_events=200000, tmax=11.0)  # past GROWSTOP at t = 10
+    b.deriv_begin()
