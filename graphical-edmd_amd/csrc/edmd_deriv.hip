@@ -33,6 +33,7 @@
  * performs, in the same order, with no contraction, so results are bitwise
  * reproducible.
  */
+#define EDMD_DEVICE_ARRAYS 1
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 

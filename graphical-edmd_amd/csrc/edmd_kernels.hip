@@ -22,6 +22,7 @@
  * square root use the compiler's correctly rounded FP64 expansions.
  * Reference line numbers are those of src/EDMD.c.
  */
+#define EDMD_DEVICE_ARRAYS 1
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -48,6 +49,45 @@ __device__ __forceinline__ double uni(double v) {
 __device__ __forceinline__ int64_t uni64(int64_t v) {
     return (int64_t)((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32 |
                      (uint32_t)__builtin_amdgcn_readfirstlane((int)v));
+}
+
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ int64_t lane_value64(int64_t v, int l) {
+    return (int64_t)((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(v >> 32), l) << 32 |
+                     (uint32_t)__builtin_amdgcn_readlane((int)v, l));
+}
+
+/* A double moved across lanes by a DPP pattern (both halves). */
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+/* Minimum over the 64 lanes, returned on every lane.  Within each row of
+ * 16 lanes: quad swaps (lane ^ 1, lane ^ 2), then the half-row mirror
+ * (lane l <-> 7 - l within 8) and the row mirror (l <-> 15 - l), each
+ * leaving the minimum of the lanes combined so far on every lane; the four
+ * row minima are then read into scalar registers.  No LDS traffic, unlike
+ * a shuffle butterfly.  The operands must not be NaN. */
+__device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
+
+__device__ __forceinline__ double wave_min(double v) {
+    v = dmin(v, dpp_f64<0xb1>(v));  /* quad_perm [1,0,3,2] */
+    v = dmin(v, dpp_f64<0x4e>(v));  /* quad_perm [2,3,0,1] */
+    v = dmin(v, dpp_f64<0x141>(v)); /* row_half_mirror */
+    v = dmin(v, dpp_f64<0x140>(v)); /* row_mirror */
+    const double r0 = lane_value(v, 0), r1 = lane_value(v, 16);
+    const double r2 = lane_value(v, 32), r3 = lane_value(v, 48);
+    return dmin(dmin(r0, r1), dmin(r2, r3));
 }
 
 /* ------------------------------------------------------------ kinematics */
@@ -305,42 +345,52 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArr
         cell = wrap_cell(ocy + sub / 3 - 1, C.ncy) * C.ncx + wrap_cell(ocx + sub % 3 - 1, C.ncx);
         cnt = A.ccnt[cell];
     }
+    /* inclusive scan within each row of 16 lanes: DPP row shifts right by
+     * 1, 2, 4, 8, zeros shifted in at the row start */
     int incl = cnt;
-#pragma unroll
-    for (int d = 1; d < 16; d <<= 1) {
-        int v = __shfl_up(incl, d, 16);
-        if (sub >= d) incl += v;
-    }
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, true);
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xf, 0xf, true);
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xf, 0xf, true);
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xf, 0xf, true);
     const int excl = incl - cnt;
+    /* slot index of candidate f of the cell this lane read is rowbase - f
+     * (newest entry first) */
+    const int64_t rowbase = (int64_t)cell * C.cap + cnt - 1 + excl;
     const int totA = __builtin_amdgcn_readlane(incl, 8);
     const int totB = (nown == 2) ? __builtin_amdgcn_readlane(incl, 24) : 0;
     const int tot = totA > totB ? totA : totB;
 
+    /* running minimum, uniform: candidates come in traversal order, so a
+     * later pass only wins with a strictly earlier time, and within a pass
+     * the lowest lane among equal times is the earliest in traversal */
+    const double lim = G ? EDMD_GROW_NONE : EDMD_NEVER;
     double btA = INF_D, btB = INF_D;
-    int brA = 0x7fffffff, brB = 0x7fffffff, bqA = 0, bqB = 0;
+    int bqA = 0, bqB = 0;
     uint32_t bcA = 0, bcB = 0;
     for (int base = 0; base < tot; base += 64) {
         const int f = base + lane;
-        int sA = 0, sB = 16;
+        /* the cell holding candidate f: the last of the nine whose
+         * exclusive prefix is <= f (scalar reads and selects, no LDS) */
+        int64_t rbA = lane_value64(rowbase, 0), rbB = lane_value64(rowbase, 16);
 #pragma unroll
         for (int s = 1; s < 9; ++s) {
-            sA += (f >= __builtin_amdgcn_readlane(excl, s)) ? 1 : 0;
-            sB += (f >= __builtin_amdgcn_readlane(excl, 16 + s)) ? 1 : 0;
+            const int64_t ra = lane_value64(rowbase, s), rb = lane_value64(rowbase, 16 + s);
+            if (f >= __builtin_amdgcn_readlane(excl, s)) rbA = ra;
+            if (f >= __builtin_amdgcn_readlane(excl, 16 + s)) rbB = rb;
         }
-        const int cellA = __shfl(cell, sA, 64), cntA = __shfl(cnt, sA, 64), exA = __shfl(excl, sA, 64);
-        const int cellB = __shfl(cell, sB, 64), cntB = __shfl(cnt, sB, 64), exB = __shfl(excl, sB, 64);
         int qA = -1, qB = -1;
-        if (f < totA) qA = A.cslot[(size_t)cellA * C.cap + (cntA - 1 - (f - exA))];
-        if (f < totB) qB = A.cslot[(size_t)cellB * C.cap + (cntB - 1 - (f - exB))];
+        if (f < totA) qA = A.cslot[rbA - f];
+        if (f < totB) qB = A.cslot[rbB - f];
         const bool vA = qA >= 0 && qA != a.p;
         const bool vB = qB >= 0 && qB != b.p;
         EdmdPart rA, rB;
         double gA = 0, gB = 0;
-        if (vA) rA = A.p[qA];
-        if (vB) rB = A.p[qB];
+        uint32_t cA = 0, cB = 0;
+        if (vA) { rA = A.p[qA]; cA = rA.coll; }
+        if (vB) { rB = A.p[qB]; cB = rB.coll; }
         if (G && vA) gA = A.vr[qA];
         if (G && vB) gB = A.vr[qB];
-        const double lim = G ? EDMD_GROW_NONE : EDMD_NEVER;
+        double tA = INF_D, tB = INF_D;
         if (vA) {
             double tt;
             if (G)
@@ -349,12 +399,7 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArr
             else
                 tt = coll_time(C, t, a.x, a.y, a.vx, a.vy, a.r, rA.x, rA.y, rA.vx, rA.vy, rA.tl,
                                rA.rad, overlap);
-            if (tt < lim && tt < btA) {
-                btA = tt;
-                brA = sA * C.cap + (f - exA);
-                bqA = qA;
-                bcA = rA.coll;
-            }
+            if (tt < lim) tA = tt;
         }
         if (vB) {
             double tt;
@@ -364,33 +409,32 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArr
             else
                 tt = coll_time(C, t, b.x, b.y, b.vx, b.vy, b.r, rB.x, rB.y, rB.vx, rB.vy, rB.tl,
                                rB.rad, overlap);
-            if (tt < lim && tt < btB) {
-                btB = tt;
-                brB = (sB - 16) * C.cap + (f - exB);
-                bqB = qB;
-                bcB = rB.coll;
+            if (tt < lim) tB = tt;
+        }
+        const double mA = wave_min(tA);
+        if (mA < btA) {
+            const int w = (int)__builtin_ctzll(__ballot(tA == mA));
+            btA = mA;
+            bqA = __builtin_amdgcn_readlane(qA, w);
+            bcA = (uint32_t)__builtin_amdgcn_readlane((int)cA, w);
+        }
+        if (nown == 2) {
+            const double mB = wave_min(tB);
+            if (mB < btB) {
+                const int w = (int)__builtin_ctzll(__ballot(tB == mB));
+                btB = mB;
+                bqB = __builtin_amdgcn_readlane(qB, w);
+                bcB = (uint32_t)__builtin_amdgcn_readlane((int)cB, w);
             }
         }
     }
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-        double otA = __shfl_xor(btA, m, 64);
-        int orA = __shfl_xor(brA, m, 64), oqA = __shfl_xor(bqA, m, 64);
-        uint32_t ocA = (uint32_t)__shfl_xor((int)bcA, m, 64);
-        double otB = __shfl_xor(btB, m, 64);
-        int orB = __shfl_xor(brB, m, 64), oqB = __shfl_xor(bqB, m, 64);
-        uint32_t ocB = (uint32_t)__shfl_xor((int)bcB, m, 64);
-        if (otA < btA || (otA == btA && orA < brA)) { btA = otA; brA = orA; bqA = oqA; bcA = ocA; }
-        if (otB < btB || (otB == btB && orB < brB)) { btB = otB; brB = orB; bqB = oqB; bcB = ocB; }
-    }
-    const double lim = G ? EDMD_GROW_NONE : EDMD_NEVER;
     if (btA < lim) {
-        dta = uni(btA); pa = uni(bqA); ca = (uint32_t)uni((int)bcA);
+        dta = btA; pa = bqA; ca = bcA;
     } else {
         dta = lim; pa = 0; ca = A.p[0].coll;
     }
     if (btB < lim) {
-        dtb = uni(btB); pb = uni(bqB); cb = (uint32_t)uni((int)bcB);
+        dtb = btB; pb = bqB; cb = bcB;
     } else {
         dtb = lim; pb = 0; cb = A.p[0].coll;
     }
@@ -403,13 +447,7 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArr
  * only when several lanes hold the minimum key, which is rare.  Keys are
  * never NaN; among +inf keys (padding, parked events) the lowest lane wins. */
 __device__ __forceinline__ void wave_argmin(const EdmdArrays& A, double& k, int& i, int& x) {
-    double mk = k;
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-        const double o = __shfl_xor(mk, m, 64);
-        mk = o < mk ? o : mk;
-    }
-    mk = uni(mk);
+    const double mk = wave_min(k);
     const uint64_t tied = __ballot(k == mk);
     int w = (int)__builtin_ctzll(tied);
     if (__popcll(tied) > 1 && mk < INF_D) {
@@ -441,14 +479,14 @@ __device__ __forceinline__ EdmdNode* level(const EdmdArrays& A, int l) {
     return l == 0 ? A.leaf : A.lev[l - 1];
 }
 
-/* Earliest pending particle event: minimum of the top level. */
+/* Earliest pending particle event: minimum of the top level, read from
+ * `top` (the LDS copy the event loop keeps, or the level in HBM). */
 __device__ __forceinline__ void top_min(const EdmdConst& C, const EdmdArrays& A, int lane,
-                                        double& key, int& leaf, int& aux) {
-    const int top = C.nlev;
+                                        const EdmdNode* top, double& key, int& leaf, int& aux) {
     double k = INF_D;
     int i = 0x7fffffff, x = 0;
-    if (lane < C.lsize[top]) {
-        const EdmdNode nd = level(A, top)[lane];
+    if (lane < C.lsize[C.nlev]) {
+        const EdmdNode nd = top[lane];
         k = nd.key;
         i = nd.idx;
         x = nd.aux;
@@ -497,11 +535,13 @@ __device__ __forceinline__ void path_prefetch(const EdmdConst& C, const EdmdArra
 
 /* Stores the new leaves of pA (and pB >= 0) with their ranks and recomputes
  * every node on their paths from the prefetched children, substituting the
- * freshly computed path nodes; where the paths meet one node takes both. */
+ * freshly computed path nodes; where the paths meet one node takes both.
+ * With `top` (the event loop's LDS copy of the top level) the top-level
+ * nodes are written there instead of to HBM. */
 template <int ML>
 __device__ __forceinline__ void path_update(const EdmdConst& C, const EdmdArrays& A, int lane,
                                             const PathCache<ML>& P, int pA, Leaf LA, int pB,
-                                            Leaf LB) {
+                                            Leaf LB, EdmdNode* top) {
     double kA = LA.key, kB = LB.key;
     int xA = LA.aux, xB = LB.aux;
     if (lane == 0) {
@@ -511,12 +551,14 @@ __device__ __forceinline__ void path_update(const EdmdConst& C, const EdmdArrays
         n.aux = xA;
         A.leaf[pA] = n;
         A.lrank[pA] = LA.rank;
+        if (top && C.nlev == 0) top[pA] = n;
         if (pB >= 0) {
             n.key = kB;
             n.idx = pB;
             n.aux = xB;
             A.leaf[pB] = n;
             A.lrank[pB] = LB.rank;
+            if (top && C.nlev == 0) top[pB] = n;
         }
     }
     WAVE_SYNC();
@@ -553,12 +595,17 @@ __device__ __forceinline__ void path_update(const EdmdConst& C, const EdmdArrays
             n.key = nkA;
             n.idx = niA;
             n.aux = nxA;
-            A.lev[l][gA] = n;
-            if (pB >= 0 && gB != gA) {
-                n.key = nkB;
-                n.idx = niB;
-                n.aux = nxB;
-                A.lev[l][gB] = n;
+            EdmdNode nb;
+            nb.key = nkB;
+            nb.idx = niB;
+            nb.aux = nxB;
+            const bool two = pB >= 0 && gB != gA;
+            if (top && l + 1 == C.nlev) {
+                top[gA] = n;
+                if (two) top[gB] = nb;
+            } else {
+                A.lev[l][gA] = n;
+                if (two) A.lev[l][gB] = nb;
             }
         }
         kA = nkA; iA = niA; xA = nxA;
@@ -923,8 +970,26 @@ struct QueueState {
     int64_t ctr;   /* insertions so far */
     int64_t win;   /* buckets passed */
     int64_t srank; /* rank of the next special event */
+    int64_t swin;  /* bucket of the next special event (-1: overflow list) */
     int cur;       /* win modulo the bucket count */
+    /* the top level of the calendar (at most 64 nodes): read by every
+     * event, written only here while the loop runs and copied to HBM when
+     * it ends */
+    EdmdNode top[64];
 };
+
+/* Top level of the calendar between HBM and the LDS copy. */
+__device__ __forceinline__ void top_load(const EdmdConst& C, const EdmdArrays& A, int lane,
+                                         EdmdNode* top) {
+    if (lane < C.lsize[C.nlev]) top[lane] = level(A, C.nlev)[lane];
+    WAVE_SYNC();
+}
+
+__device__ __forceinline__ void top_store(const EdmdConst& C, const EdmdArrays& A, int lane,
+                                          const EdmdNode* top) {
+    if (lane < C.lsize[C.nlev]) level(A, C.nlev)[lane] = top[lane];
+    WAVE_SYNC();
+}
 
 struct LoopRegs {
     double t;
@@ -937,19 +1002,20 @@ struct LoopRegs {
 
 __device__ __forceinline__ void next_special(const EdmdState* st, LoopRegs& R) {
     double tsp = st->next_screen;
-    int64_t rsp = st->rank_screen;
+    int64_t rsp = st->rank_screen, wsp = st->win_screen;
     int sp = EDMD_EV_SCREENSHOT;
     if (before(st->next_thermo, st->rank_thermo, tsp, rsp)) {
-        tsp = st->next_thermo; rsp = st->rank_thermo; sp = EDMD_EV_THERMO;
+        tsp = st->next_thermo; rsp = st->rank_thermo; wsp = st->win_thermo; sp = EDMD_EV_THERMO;
     }
     if (before(st->next_noise, st->rank_noise, tsp, rsp)) {
-        tsp = st->next_noise; rsp = st->rank_noise; sp = EDMD_EV_ADDINGNOISE;
+        tsp = st->next_noise; rsp = st->rank_noise; wsp = st->win_noise; sp = EDMD_EV_ADDINGNOISE;
     }
     if (before(st->next_grow, st->rank_grow, tsp, rsp)) {
-        tsp = st->next_grow; rsp = st->rank_grow; sp = EDMD_EV_GROWSTOP;
+        tsp = st->next_grow; rsp = st->rank_grow; wsp = st->win_grow; sp = EDMD_EV_GROWSTOP;
     }
     R.next_special = uni(tsp);
     R.q->srank = uni64(rsp);
+    R.q->swin = uni64(wsp);
     R.special = uni(sp);
 }
 
@@ -1002,14 +1068,6 @@ __device__ __forceinline__ bool step_window(const EdmdConst& C, EdmdState* st, L
     return true;
 }
 
-__device__ __forceinline__ int64_t special_window(const EdmdState* st, int sp) {
-    const int64_t w = sp == EDMD_EV_SCREENSHOT ? st->win_screen
-                    : sp == EDMD_EV_THERMO     ? st->win_thermo
-                    : sp == EDMD_EV_GROWSTOP   ? st->win_grow
-                                               : st->win_noise;
-    return uni64(w);
-}
-
 /* findNextEvent (EDMD.c:2313-2327): while no pending event is in the
  * reference's tree -- a particle event within dtPaul of paulTime, or a
  * special event whose bucket has come up -- move on one bucket (refiling
@@ -1021,9 +1079,9 @@ __device__ __forceinline__ bool pick_next(const EdmdConst& C, const EdmdArrays& 
                                           LoopRegs& R, double key, int leaf) {
     /* the particle event's rank matters only against a special at its time */
     auto krank = [&]() { return key == R.next_special ? uni64(A.lrank[leaf]) : (int64_t)0; };
-    int64_t ws = special_window(st, R.special);
     for (int guard = 0;; ++guard) {
         const bool part_in = key - R.q->paul < C.dtp;
+        const int64_t ws = R.q->swin;
         const bool spec_in = ws >= 0 && ws <= R.q->win;
         if (part_in || spec_in)
             return spec_in && (!part_in || before(R.next_special, R.q->srank, key, krank()));
@@ -1036,10 +1094,7 @@ __device__ __forceinline__ bool pick_next(const EdmdConst& C, const EdmdArrays& 
             }
             return before(R.next_special, R.q->srank, key, krank());
         }
-        if (step_window(C, st, R)) {
-            next_special(st, R);
-            ws = special_window(st, R.special);
-        }
+        if (step_window(C, st, R)) next_special(st, R);
     }
 }
 
@@ -1193,6 +1248,7 @@ __device__ void ev_rescale(const EdmdConst& C, const EdmdArrays& A, int lane, Lo
     if (__ballot(ov) != 0ull) R.status = EDMD_DEV_OVERLAP;
     WAVE_SYNC();
     rebuild_tree(C, A, lane);
+    top_load(C, A, lane, R.q->top);
     const int64_t wk = window_of(t + C.dtnoise, R.q->paul, C.dtp, C.nlists, R.q->win);
     const int64_t rk = take_rank(C, R, t + C.dtnoise);
     if (lane == 0) {
@@ -1259,7 +1315,7 @@ __device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, 
             A.e[i].cact = cpa;
             A.erank[2 * i] = rk;
         }
-        path_update<ML>(C, A, lane, P, i, make_leaf(tk, rk, tci, rci, pa, diri), -1, Leaf{});
+        path_update<ML>(C, A, lane, P, i, make_leaf(tk, rk, tci, rci, pa, diri), -1, Leaf{}, R.q->top);
         return;
     }
     R.ncol++;
@@ -1337,7 +1393,7 @@ __device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, 
         A.erank[2 * j + 1] = rxj;
     }
     path_update<ML>(C, A, lane, P, i, make_leaf(tki, rki, tci2, rxi, pi_, di), j,
-                    make_leaf(tkj, rkj, tcj2, rxj, pj_, dj));
+                    make_leaf(tkj, rkj, tcj2, rxj, pj_, dj), R.q->top);
 }
 
 template <int ML>
@@ -1397,7 +1453,7 @@ __device__ void ev_collision_grow(const EdmdConst& C, const EdmdArrays& A, int l
             A.e[i].cact = cpa;
             A.erank[2 * i] = rk;
         }
-        path_update<ML>(C, A, lane, P, i, make_leaf(tk, rk, tci, rci, pa, diri), -1, Leaf{});
+        path_update<ML>(C, A, lane, P, i, make_leaf(tk, rk, tci, rci, pa, diri), -1, Leaf{}, R.q->top);
         return;
     }
     R.ncol++;
@@ -1462,7 +1518,7 @@ __device__ void ev_collision_grow(const EdmdConst& C, const EdmdArrays& A, int l
         A.erank[2 * j + 1] = rxj;
     }
     path_update<ML>(C, A, lane, P, i, make_leaf(tki, rki, tci2, rxi, pi_, di), j,
-                    make_leaf(tkj, rkj, tcj2, rxj, pj_, dj));
+                    make_leaf(tkj, rkj, tcj2, rxj, pj_, dj), R.q->top);
 }
 
 /* stopGrow, EDMD.c:4740-4785 (2D, no wells): final flight with growth, all
@@ -1510,6 +1566,7 @@ __device__ void ev_stop_grow(const EdmdConst& C, const EdmdArrays& A, int lane, 
     WAVE_SYNC();
     flag_overlap(R, ov, 0);
     rebuild_tree(C, A, lane);
+    top_load(C, A, lane, R.q->top);
     if (lane == 0) {
         A.st->growing = 0;
         A.st->next_grow = INF_D;
@@ -1611,7 +1668,7 @@ __device__ void ev_crossing(const EdmdConst& C, const EdmdArrays& A, int lane, L
         A.erank[2 * p] = rk;
         A.erank[2 * p + 1] = rx;
     }
-    path_update<ML>(C, A, lane, P, p, make_leaf(tk, rk, tc, rx, pa, dir), -1, Leaf{});
+    path_update<ML>(C, A, lane, P, p, make_leaf(tk, rk, tc, rx, pa, dir), -1, Leaf{}, R.q->top);
 }
 
 /* Persistent event loop, one wavefront per system (blockIdx.x); ML bounds
@@ -1638,6 +1695,7 @@ __device__ __forceinline__ void run_loop(const EdmdSys* __restrict__ systems, Ed
     R.q->win = uni64(st->win);
     R.q->cur = uni((int)(R.q->win % C.nlists));
     next_special(st, R);
+    top_load(C, A, lane, R.q->top);
     uint64_t ncross = st->ncross;
     int reason = EDMD_STOP_NONE;
     int last_type = st->last_type;
@@ -1651,7 +1709,7 @@ __device__ __forceinline__ void run_loop(const EdmdSys* __restrict__ systems, Ed
         if (!(R.t <= ra.tmax)) { reason = EDMD_STOP_TIME; break; }
         double key;
         int p, aux;
-        top_min(C, A, lane, key, p, aux);
+        top_min(C, A, lane, R.q->top, key, p, aux);
         if (!(key < INF_D) && !(R.next_special < INF_D)) { reason = EDMD_STOP_EMPTY; break; }
         if (pick_next(C, A, st, R, key, p)) {
             const int sp = R.special;
@@ -1705,6 +1763,7 @@ __device__ __forceinline__ void run_loop(const EdmdSys* __restrict__ systems, Ed
         st->ctr = R.q->ctr;
         st->win = R.q->win;
     }
+    top_store(C, A, lane, R.q->top);
 }
 
 /* Systems of up to 64^2 disks (calendar depth <= 2): bounded to 128 VGPRs so
@@ -1733,7 +1792,7 @@ __global__ __launch_bounds__(64) void k_peek(const EdmdSys* __restrict__ systems
     const int lane = threadIdx.x;
     double key;
     int p, aux;
-    top_min(C, A, lane, key, p, aux);
+    top_min(C, A, lane, level(A, C.nlev), key, p, aux);
     __shared__ QueueState Qs;
     LoopRegs R;
     R.q = &Qs;
@@ -1808,7 +1867,7 @@ __global__ __launch_bounds__(64) void k_repredict(const EdmdSys* __restrict__ sy
         A.erank[2 * i + 1] = rcross;
     }
     path_update<ML>(C, A, lane, P, i, make_leaf(e.tcoll, rcoll, e.tcross, rcross, e.partner, e.dir),
-                    -1, Leaf{});
+                    -1, Leaf{}, nullptr);
     const bool any_ov = __ballot(ov) != 0ull;
     if (lane == 0) {
         A.st->ctr = ctr;

@@ -161,25 +161,37 @@ typedef struct __attribute__((aligned(16))) EdmdNode {
     int32_t aux; /* that leaf's event: partner (>= 0) or -direction */
 } EdmdNode;
 
+/* Kernel sources (which define EDMD_DEVICE_ARRAYS) see every array as
+ * global memory (address space 1) in their device pass, so that the
+ * compiler emits global rather than flat memory instructions: flat accesses
+ * are counted against the LDS counter too, and waiting for one then waits
+ * for every LDS operation in flight.  Host code sees plain pointers; the
+ * layout is the same. */
+#if defined(__HIP_DEVICE_COMPILE__) && defined(EDMD_DEVICE_ARRAYS)
+#define EDMD_GLOBAL __attribute__((address_space(1)))
+#else
+#define EDMD_GLOBAL
+#endif
+
 typedef struct EdmdArrays {
-    EdmdPart* p;
-    EdmdEv* e;
-    EdmdNode* leaf;
-    EdmdNode* lev[EDMD_MAX_LEVELS]; /* lev[l] = level l+1 nodes */
-    int32_t* img;                   /* periodic image counters, 2 per particle */
-    int32_t* ccnt;
-    int32_t* cslot;
-    EdmdState* st;
-    EdmdTraceRec* trace;
-    EdmdThermoRow* thermo;
-    double* vr;          /* radius growth rate per particle (growth starts) */
-    int64_t* erank;      /* ranks of the slots: [2p] collision, [2p + 1] crossing */
-    int64_t* lrank;      /* rank of each leaf's key (the earlier slot) */
-    EdmdTapeRec* tape;   /* tape_cap records */
-    double* dv;          /* tangent or adjoint vector, 4 per particle */
-    double* dtl;         /* the time each particle's dv refers to */
-    double* otl;         /* particle local times when taping began */
-    EdmdDeriv* dr;
+    EdmdPart EDMD_GLOBAL* p;
+    EdmdEv EDMD_GLOBAL* e;
+    EdmdNode EDMD_GLOBAL* leaf;
+    EdmdNode EDMD_GLOBAL* lev[EDMD_MAX_LEVELS]; /* lev[l] = level l+1 nodes */
+    int32_t EDMD_GLOBAL* img;                   /* periodic image counters, 2 per particle */
+    int32_t EDMD_GLOBAL* ccnt;
+    int32_t EDMD_GLOBAL* cslot;
+    EdmdState EDMD_GLOBAL* st;
+    EdmdTraceRec EDMD_GLOBAL* trace;
+    EdmdThermoRow EDMD_GLOBAL* thermo;
+    double EDMD_GLOBAL* vr;          /* radius growth rate per particle (growth starts) */
+    int64_t EDMD_GLOBAL* erank;      /* ranks of the slots: [2p] collision, [2p + 1] crossing */
+    int64_t EDMD_GLOBAL* lrank;      /* rank of each leaf's key (the earlier slot) */
+    EdmdTapeRec EDMD_GLOBAL* tape;   /* tape_cap records */
+    double EDMD_GLOBAL* dv;          /* tangent or adjoint vector, 4 per particle */
+    double EDMD_GLOBAL* dtl;         /* the time each particle's dv refers to */
+    double EDMD_GLOBAL* otl;         /* particle local times when taping began */
+    EdmdDeriv EDMD_GLOBAL* dr;
 } EdmdArrays;
 
 typedef struct EdmdSys {
