@@ -378,8 +378,8 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
         b->np += h.c.n;
         h.cell0 = b->ncell;
         b->ncell += (size_t)h.c.ncx * h.c.ncy;
-        h.slot0 = b->nslot;
-        b->nslot += (size_t)h.c.ncx * h.c.ncy * h.c.cap;
+        h.slot0 = b->nslot; /* a multiple of 4: cell rows are read in 16-byte chunks */
+        b->nslot += ((size_t)h.c.ncx * h.c.ncy * h.c.cap + 3) & ~(size_t)3;
         for (int l = 1; l <= h.c.nlev; l++) {
             h.node0[l - 1] = b->nnode;
             b->nnode += h.c.lsize[l];

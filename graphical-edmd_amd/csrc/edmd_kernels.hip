@@ -90,6 +90,49 @@ __device__ __forceinline__ double wave_min(double v) {
     return dmin(dmin(r0, r1), dmin(r2, r3));
 }
 
+__device__ __forceinline__ int imin(int a, int b) { return b < a ? b : a; }
+
+/* Minimum of an int over the wave, on every lane: the same DPP pattern as
+ * wave_min, each step a single v_min_i32 with a DPP source. */
+__device__ __forceinline__ int wave_min_i32(int v) {
+    v = imin(v, __builtin_amdgcn_update_dpp(0x7fffffff, v, 0xb1, 0xf, 0xf, false));
+    v = imin(v, __builtin_amdgcn_update_dpp(0x7fffffff, v, 0x4e, 0xf, 0xf, false));
+    v = imin(v, __builtin_amdgcn_update_dpp(0x7fffffff, v, 0x141, 0xf, 0xf, false));
+    v = imin(v, __builtin_amdgcn_update_dpp(0x7fffffff, v, 0x140, 0xf, 0xf, false));
+    const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return imin(imin(r0, r1), imin(r2, r3));
+}
+
+/* 32-bit key of v - base that never decreases when v increases: the float
+ * nearest to v - base (both roundings are monotone), its sign-magnitude
+ * bits turned into a signed order (-0 and +0 alike).  v is never NaN. */
+__device__ __forceinline__ int order_key(double v, double base) {
+    const int b = __float_as_int((float)(v - base));
+    const int m = b & 0x7fffffff;
+    return b < 0 ? -m : m;
+}
+
+/* Lane holding the minimum of v over the wave, the lowest lane among equal
+ * values, and that minimum.  The search runs on order_key(v, base) with
+ * single-instruction DPP steps; the lane holding the exact minimum always
+ * has the least key, and only when several lanes share it are their
+ * doubles compared.  base should lie near the values for resolution (any
+ * finite base is correct). */
+__device__ __forceinline__ int wave_argmin_lane(double v, double base, double& vmin) {
+    const int k = order_key(v, base);
+    const int mk = wave_min_i32(k);
+    const uint64_t tied = __ballot(k == mk);
+    if (__popcll(tied) == 1) {
+        const int w = (int)__builtin_ctzll(tied);
+        vmin = lane_value(v, w);
+        return w;
+    }
+    const double vv = (k == mk) ? v : INF_D;
+    vmin = wave_min(vv);
+    return (int)__builtin_ctzll(__ballot(vv == vmin));
+}
+
 /* ------------------------------------------------------------ kinematics */
 
 /* PBC, EDMD.c:5896-5913 */
@@ -332,10 +375,11 @@ struct Owner {
  * reference's strict-less first minimum; no candidate gives NEVER and
  * partner 0 (EDMD.c:2830-2831, 3091-3093).  Results are uniform. */
 template <bool G>
-__device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArrays& A, int lane,
-                                               double t, int nown, const Owner& a, const Owner& b,
-                                               double& dta, int& pa, uint32_t& ca, double& dtb,
-                                               int& pb, uint32_t& cb, bool& overlap) {
+__device__ __forceinline__ void predict_owners_gather(const EdmdConst& C, const EdmdArrays& A,
+                                                      int lane, double t, int nown,
+                                                      const Owner& a, const Owner& b, double& dta,
+                                                      int& pa, uint32_t& ca, double& dtb, int& pb,
+                                                      uint32_t& cb, bool& overlap) {
     const int sub = lane & 15;
     const int grp = lane >> 4;
     int cell = 0, cnt = 0;
@@ -411,17 +455,19 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArr
                                rB.rad, overlap);
             if (tt < lim) tB = tt;
         }
-        const double mA = wave_min(tA);
+        double mA;
+        const int wA = wave_argmin_lane(tA, 0.0, mA);
         if (mA < btA) {
-            const int w = (int)__builtin_ctzll(__ballot(tA == mA));
+            const int w = wA;
             btA = mA;
             bqA = __builtin_amdgcn_readlane(qA, w);
             bcA = (uint32_t)__builtin_amdgcn_readlane((int)cA, w);
         }
         if (nown == 2) {
-            const double mB = wave_min(tB);
+            double mB;
+            const int wB = wave_argmin_lane(tB, 0.0, mB);
             if (mB < btB) {
-                const int w = (int)__builtin_ctzll(__ballot(tB == mB));
+                const int w = wB;
                 btB = mB;
                 bqB = __builtin_amdgcn_readlane(qB, w);
                 bcB = (uint32_t)__builtin_amdgcn_readlane((int)cB, w);
@@ -440,21 +486,185 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArr
     }
 }
 
+/* Largest cell capacity the staged scan handles (six 16-byte row chunks
+ * per cell), and its LDS stage: every slot of 18 cells. */
+#define EDMD_STAGE_CAP 24
+#define EDMD_STAGE_MAX (18 * EDMD_STAGE_CAP)
+
+/* Inclusive prefix sum over the wave of values held by lanes 0-31 (rows 0
+ * and 1): DPP row shifts within each row of 16, then row 0's total added
+ * to row 1 (row_bcast:15). */
+__device__ __forceinline__ int scan32(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+    return v;
+}
+
+/* Neighbour scan for one or two owners, candidates staged in LDS.
+ *
+ * Lane e < 9 reads cell e of owner a's 3x3 block, lane 9 + e cell e of
+ * owner b's (y offset outer, x offset inner): its count and its whole slot
+ * row in the same round trip.  A scan over the 18 counts places every
+ * candidate at its traversal position -- owner a's first, then owner b's,
+ * each cell's newest entry first (EDMD.c:2959-3003) -- and each lane writes
+ * its cell's entries there in LDS.  Lane f then evaluates candidate f
+ * against its owner, so the two owners share one pass when their
+ * candidates fit in 64 lanes.  Per owner, the minimum time over eligible
+ * candidates (time < NEVER), the lowest lane among equal times, and across
+ * passes only a strictly earlier time: the reference's strict-less first
+ * minimum.  Requires cap <= EDMD_STAGE_CAP, cap % 4 == 0 and cell rows
+ * aligned to 16 bytes. */
+template <bool G>
+__device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const EdmdArrays& A,
+                                                      int lane, double t, int nown,
+                                                      const Owner& a, const Owner& b, double& dta,
+                                                      int& pa, uint32_t& ca, double& dtb, int& pb,
+                                                      uint32_t& cb, bool& overlap,
+                                                      int32_t* stage) {
+    const int nent = 9 * nown;
+    const bool ent = lane < nent;
+    const bool inB = lane >= 9;
+    const int sub = inB ? lane - 9 : lane;
+    int cnt = 0;
+    int4 row[EDMD_STAGE_CAP / 4];
+    if (ent) {
+        const int ocx = inB ? b.cx : a.cx;
+        const int ocy = inB ? b.cy : a.cy;
+        const int cell =
+            wrap_cell(ocy + sub / 3 - 1, C.ncy) * C.ncx + wrap_cell(ocx + sub % 3 - 1, C.ncx);
+        const int4 EDMD_GLOBAL* src = (const int4 EDMD_GLOBAL*)(A.cslot + (int64_t)cell * C.cap);
+        cnt = A.ccnt[cell];
+#pragma unroll
+        for (int c = 0; c < EDMD_STAGE_CAP / 4; ++c)
+            if (4 * c < C.cap) row[c] = src[c];
+    }
+    const int incl = scan32(cnt);
+    const int excl = incl - cnt;
+    const int totA = __builtin_amdgcn_readlane(incl, 8);
+    const int tot = __builtin_amdgcn_readlane(incl, nent - 1);
+    if (ent) {
+        /* slot k (k-th oldest) goes to excl + cnt - 1 - k */
+        const int top = excl + cnt - 1;
+#pragma unroll
+        for (int c = 0; c < EDMD_STAGE_CAP / 4; ++c) {
+            if (4 * c >= cnt) break;
+            stage[top - 4 * c] = row[c].x;
+            if (4 * c + 1 < cnt) stage[top - 4 * c - 1] = row[c].y;
+            if (4 * c + 2 < cnt) stage[top - 4 * c - 2] = row[c].z;
+            if (4 * c + 3 < cnt) stage[top - 4 * c - 3] = row[c].w;
+        }
+    }
+    WAVE_SYNC();
+
+    const double lim = G ? EDMD_GROW_NONE : EDMD_NEVER;
+    double btA = INF_D, btB = INF_D;
+    int bqA = 0, bqB = 0;
+    uint32_t bcA = 0, bcB = 0;
+    for (int base = 0; base < tot; base += 64) {
+        const int f = base + lane;
+        const bool isB = f >= totA;
+        const int q = f < tot ? stage[f] : -1;
+        const int own = isB ? b.p : a.p;
+        const bool v = q >= 0 && q != own;
+        double tq = INF_D;
+        uint32_t cq = 0;
+        if (v) {
+            const EdmdPart r = A.p[q];
+            cq = r.coll;
+            const double ox = isB ? b.x : a.x, oy = isB ? b.y : a.y;
+            const double ovx = isB ? b.vx : a.vx, ovy = isB ? b.vy : a.vy;
+            const double orad = isB ? b.r : a.r;
+            double tt;
+            if (G) {
+                const double ovr = isB ? b.vr : a.vr;
+                tt = coll_time_grow(C, t, ox, oy, ovx, ovy, orad, ovr, r.x, r.y, r.vx, r.vy, r.tl,
+                                    r.rad, A.vr[q], overlap);
+            } else {
+                tt = coll_time(C, t, ox, oy, ovx, ovy, orad, r.x, r.y, r.vx, r.vy, r.tl, r.rad,
+                               overlap);
+            }
+            if (tt < lim) tq = tt;
+        }
+        if (base < totA) {
+            const double tA = isB ? INF_D : tq;
+            double mA;
+            const int w = wave_argmin_lane(tA, 0.0, mA);
+            if (mA < btA) {
+                btA = mA;
+                bqA = __builtin_amdgcn_readlane(q, w);
+                bcA = (uint32_t)__builtin_amdgcn_readlane((int)cq, w);
+            }
+        }
+        if (base + 64 > totA && nown == 2) {
+            const double tB = isB ? tq : INF_D;
+            double mB;
+            const int w = wave_argmin_lane(tB, 0.0, mB);
+            if (mB < btB) {
+                btB = mB;
+                bqB = __builtin_amdgcn_readlane(q, w);
+                bcB = (uint32_t)__builtin_amdgcn_readlane((int)cq, w);
+            }
+        }
+    }
+    WAVE_SYNC();
+    if (btA < lim) {
+        dta = btA; pa = bqA; ca = bcA;
+    } else {
+        dta = lim; pa = 0; ca = A.p[0].coll;
+    }
+    if (btB < lim) {
+        dtb = btB; pb = bqB; cb = bcB;
+    } else {
+        dtb = lim; pb = 0; cb = A.p[0].coll;
+    }
+}
+
+/* The staged scan where the cell capacity allows it, else the gather scan
+ * from HBM; both give the reference's result bit for bit. */
+template <bool G>
+__device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArrays& A, int lane,
+                                               double t, int nown, const Owner& a, const Owner& b,
+                                               double& dta, int& pa, uint32_t& ca, double& dtb,
+                                               int& pb, uint32_t& cb, bool& overlap,
+                                               int32_t* stage) {
+    if (C.cap <= EDMD_STAGE_CAP && (C.cap & 3) == 0)
+        predict_owners_staged<G>(C, A, lane, t, nown, a, b, dta, pa, ca, dtb, pb, cb, overlap,
+                                 stage);
+    else
+        predict_owners_gather<G>(C, A, lane, t, nown, a, b, dta, pa, ca, dtb, pb, cb, overlap);
+}
+
 /* ------------------------------------------------------------ calendar */
 
 /* Minimum over the wave by (key, rank), carrying the leaf and the event
  * code.  A node's rank is that of its leaf (lrank[leaf]); ranks are read
  * only when several lanes hold the minimum key, which is rare.  Keys are
- * never NaN; among +inf keys (padding, parked events) the lowest lane wins. */
-__device__ __forceinline__ void wave_argmin(const EdmdArrays& A, double& k, int& i, int& x) {
-    const double mk = wave_min(k);
-    const uint64_t tied = __ballot(k == mk);
+ * never NaN; among +inf keys (padding, parked events) the lowest lane wins.
+ * base: a time near the keys (the current time), for the 32-bit search. */
+__device__ __forceinline__ void wave_argmin(const EdmdArrays& A, double base, double& k, int& i,
+                                            int& x) {
+    const int kk = order_key(k, base);
+    const int mkk = wave_min_i32(kk);
+    const uint64_t near = __ballot(kk == mkk);
+    double mk;
+    uint64_t tied;
+    if (__popcll(near) == 1) {
+        tied = near;
+        mk = lane_value(k, (int)__builtin_ctzll(near));
+    } else {
+        const double kv = (kk == mkk) ? k : INF_D;
+        mk = wave_min(kv);
+        tied = __ballot(kv == mk);
+    }
     int w = (int)__builtin_ctzll(tied);
     if (__popcll(tied) > 1 && mk < INF_D) {
         /* equal keys: the earlier in the reference's queue (lower rank) */
         /* loaded by every lane (padding lanes read leaf 0) and selected,
          * not loaded under a divergent branch */
-        const bool tie = k == mk;
+        const bool tie = ((tied >> (threadIdx.x & 63)) & 1) != 0;
         const int64_t rl = A.lrank[tie ? i : 0];
         int64_t r = tie ? rl : INT64_MAX;
         int l = tie ? (int)(threadIdx.x & 63) : 64;
@@ -482,7 +692,8 @@ __device__ __forceinline__ EdmdNode* level(const EdmdArrays& A, int l) {
 /* Earliest pending particle event: minimum of the top level, read from
  * `top` (the LDS copy the event loop keeps, or the level in HBM). */
 __device__ __forceinline__ void top_min(const EdmdConst& C, const EdmdArrays& A, int lane,
-                                        const EdmdNode* top, double& key, int& leaf, int& aux) {
+                                        const EdmdNode* top, double base, double& key, int& leaf,
+                                        int& aux) {
     double k = INF_D;
     int i = 0x7fffffff, x = 0;
     if (lane < C.lsize[C.nlev]) {
@@ -491,7 +702,7 @@ __device__ __forceinline__ void top_min(const EdmdConst& C, const EdmdArrays& A,
         i = nd.idx;
         x = nd.aux;
     }
-    wave_argmin(A, k, i, x);
+    wave_argmin(A, base, k, i, x);
     key = k;
     leaf = uni(i);
     aux = uni(x);
@@ -541,7 +752,7 @@ __device__ __forceinline__ void path_prefetch(const EdmdConst& C, const EdmdArra
 template <int ML>
 __device__ __forceinline__ void path_update(const EdmdConst& C, const EdmdArrays& A, int lane,
                                             const PathCache<ML>& P, int pA, Leaf LA, int pB,
-                                            Leaf LB, EdmdNode* top) {
+                                            Leaf LB, EdmdNode* top, double base) {
     double kA = LA.key, kB = LB.key;
     int xA = LA.aux, xB = LB.aux;
     if (lane == 0) {
@@ -574,7 +785,7 @@ __device__ __forceinline__ void path_update(const EdmdConst& C, const EdmdArrays
         int i = P.i[0][l], x = P.x[0][l];
         if (c == cA) { k = kA; i = iA; x = xA; }
         else if (pB >= 0 && c == cB) { k = kB; i = iB; x = xB; }
-        wave_argmin(A, k, i, x);
+        wave_argmin(A, base, k, i, x);
         const double nkA = k;
         const int niA = uni(i), nxA = uni(x);
         double nkB = nkA;
@@ -585,7 +796,7 @@ __device__ __forceinline__ void path_update(const EdmdConst& C, const EdmdArrays
             i = P.i[1][l];
             x = P.x[1][l];
             if (c == cB) { k = kB; i = iB; x = xB; }
-            wave_argmin(A, k, i, x);
+            wave_argmin(A, base, k, i, x);
             nkB = k;
             niB = uni(i);
             nxB = uni(x);
@@ -972,6 +1183,7 @@ struct QueueState {
     int64_t srank; /* rank of the next special event */
     int64_t swin;  /* bucket of the next special event (-1: overflow list) */
     int cur;       /* win modulo the bucket count */
+    int32_t cand[EDMD_STAGE_MAX]; /* candidate stage of the neighbour scan */
     /* the top level of the calendar (at most 64 nodes): read by every
      * event, written only here while the loop runs and copied to HBM when
      * it ends */
@@ -1302,7 +1514,7 @@ __device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, 
         double dt, dtb;
         int pa, pb;
         uint32_t cpa, cpb;
-        predict_owners<false>(C, A, lane, t, 1, oi, oi, dt, pa, cpa, dtb, pb, cpb, ov);
+        predict_owners<false>(C, A, lane, t, 1, oi, oi, dt, pa, cpa, dtb, pb, cpb, ov, R.q->cand);
         flag_overlap(R, ov, i);
         const double tk = t + dt;
         const double tci = uni(ei.tcross);
@@ -1315,7 +1527,7 @@ __device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, 
             A.e[i].cact = cpa;
             A.erank[2 * i] = rk;
         }
-        path_update<ML>(C, A, lane, P, i, make_leaf(tk, rk, tci, rci, pa, diri), -1, Leaf{}, R.q->top);
+        path_update<ML>(C, A, lane, P, i, make_leaf(tk, rk, tci, rci, pa, diri), -1, Leaf{}, R.q->top, R.t);
         return;
     }
     R.ncol++;
@@ -1368,7 +1580,7 @@ __device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, 
     double dti, dtj;
     int pi_, pj_;
     uint32_t cpi, cpj;
-    predict_owners<false>(C, A, lane, t, 2, oi, oj, dti, pi_, cpi, dtj, pj_, cpj, ov);
+    predict_owners<false>(C, A, lane, t, 2, oi, oj, dti, pi_, cpi, dtj, pj_, cpj, ov, R.q->cand);
     flag_overlap(R, ov, i);
     double tci2, tcj2;
     int di, dj;
@@ -1393,7 +1605,7 @@ __device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, 
         A.erank[2 * j + 1] = rxj;
     }
     path_update<ML>(C, A, lane, P, i, make_leaf(tki, rki, tci2, rxi, pi_, di), j,
-                    make_leaf(tkj, rkj, tcj2, rxj, pj_, dj), R.q->top);
+                    make_leaf(tkj, rkj, tcj2, rxj, pj_, dj), R.q->top, R.t);
 }
 
 template <int ML>
@@ -1440,7 +1652,7 @@ __device__ void ev_collision_grow(const EdmdConst& C, const EdmdArrays& A, int l
         double dt, dtb;
         int pa, pb;
         uint32_t cpa, cpb;
-        predict_owners<true>(C, A, lane, t, 1, oi, oi, dt, pa, cpa, dtb, pb, cpb, ov);
+        predict_owners<true>(C, A, lane, t, 1, oi, oi, dt, pa, cpa, dtb, pb, cpb, ov, R.q->cand);
         flag_overlap(R, ov, i);
         const double tk = t + dt;
         const double tci = uni(ei.tcross);
@@ -1453,7 +1665,7 @@ __device__ void ev_collision_grow(const EdmdConst& C, const EdmdArrays& A, int l
             A.e[i].cact = cpa;
             A.erank[2 * i] = rk;
         }
-        path_update<ML>(C, A, lane, P, i, make_leaf(tk, rk, tci, rci, pa, diri), -1, Leaf{}, R.q->top);
+        path_update<ML>(C, A, lane, P, i, make_leaf(tk, rk, tci, rci, pa, diri), -1, Leaf{}, R.q->top, R.t);
         return;
     }
     R.ncol++;
@@ -1493,7 +1705,7 @@ __device__ void ev_collision_grow(const EdmdConst& C, const EdmdArrays& A, int l
     double dti, dtj;
     int pi_, pj_;
     uint32_t cpi, cpj;
-    predict_owners<true>(C, A, lane, t, 2, oi, oj, dti, pi_, cpi, dtj, pj_, cpj, ov);
+    predict_owners<true>(C, A, lane, t, 2, oi, oj, dti, pi_, cpi, dtj, pj_, cpj, ov, R.q->cand);
     flag_overlap(R, ov, i);
     double tci2, tcj2;
     int di, dj;
@@ -1518,7 +1730,7 @@ __device__ void ev_collision_grow(const EdmdConst& C, const EdmdArrays& A, int l
         A.erank[2 * j + 1] = rxj;
     }
     path_update<ML>(C, A, lane, P, i, make_leaf(tki, rki, tci2, rxi, pi_, di), j,
-                    make_leaf(tkj, rkj, tcj2, rxj, pj_, dj), R.q->top);
+                    make_leaf(tkj, rkj, tcj2, rxj, pj_, dj), R.q->top, R.t);
 }
 
 /* stopGrow, EDMD.c:4740-4785 (2D, no wells): final flight with growth, all
@@ -1652,7 +1864,7 @@ __device__ void ev_crossing(const EdmdConst& C, const EdmdArrays& A, int lane, L
     double dt, dtb;
     int pa, pb;
     uint32_t cpa, cpb;
-    predict_owners<G>(C, A, lane, t, 1, o, o, dt, pa, cpa, dtb, pb, cpb, ov);
+    predict_owners<G>(C, A, lane, t, 1, o, o, dt, pa, cpa, dtb, pb, cpb, ov, R.q->cand);
     flag_overlap(R, ov, p);
     double tc;
     int dir;
@@ -1668,7 +1880,7 @@ __device__ void ev_crossing(const EdmdConst& C, const EdmdArrays& A, int lane, L
         A.erank[2 * p] = rk;
         A.erank[2 * p + 1] = rx;
     }
-    path_update<ML>(C, A, lane, P, p, make_leaf(tk, rk, tc, rx, pa, dir), -1, Leaf{}, R.q->top);
+    path_update<ML>(C, A, lane, P, p, make_leaf(tk, rk, tc, rx, pa, dir), -1, Leaf{}, R.q->top, R.t);
 }
 
 /* Persistent event loop, one wavefront per system (blockIdx.x); ML bounds
@@ -1709,7 +1921,7 @@ __device__ __forceinline__ void run_loop(const EdmdSys* __restrict__ systems, Ed
         if (!(R.t <= ra.tmax)) { reason = EDMD_STOP_TIME; break; }
         double key;
         int p, aux;
-        top_min(C, A, lane, R.q->top, key, p, aux);
+        top_min(C, A, lane, R.q->top, R.t, key, p, aux);
         if (!(key < INF_D) && !(R.next_special < INF_D)) { reason = EDMD_STOP_EMPTY; break; }
         if (pick_next(C, A, st, R, key, p)) {
             const int sp = R.special;
@@ -1792,7 +2004,7 @@ __global__ __launch_bounds__(64) void k_peek(const EdmdSys* __restrict__ systems
     const int lane = threadIdx.x;
     double key;
     int p, aux;
-    top_min(C, A, lane, level(A, C.nlev), key, p, aux);
+    top_min(C, A, lane, level(A, C.nlev), A.st->t, key, p, aux);
     __shared__ QueueState Qs;
     LoopRegs R;
     R.q = &Qs;
@@ -1835,6 +2047,7 @@ __global__ __launch_bounds__(64) void k_repredict(const EdmdSys* __restrict__ sy
     const EdmdConst& C = systems[0].c;
     const EdmdArrays& A = systems[0].a;
     const int lane = threadIdx.x;
+    __shared__ int32_t stage[EDMD_STAGE_MAX];
     const double t = A.st->t;
     const EdmdPart pi = A.p[i];
     EdmdEv e = A.e[i];
@@ -1855,9 +2068,9 @@ __global__ __launch_bounds__(64) void k_repredict(const EdmdSys* __restrict__ sy
         int pb;
         uint32_t cpb;
         if (grow)
-            predict_owners<true>(C, A, lane, t, 1, o, o, dt, e.partner, e.cact, dtb, pb, cpb, ov);
+            predict_owners<true>(C, A, lane, t, 1, o, o, dt, e.partner, e.cact, dtb, pb, cpb, ov, stage);
         else
-            predict_owners<false>(C, A, lane, t, 1, o, o, dt, e.partner, e.cact, dtb, pb, cpb, ov);
+            predict_owners<false>(C, A, lane, t, 1, o, o, dt, e.partner, e.cact, dtb, pb, cpb, ov, stage);
         e.tcoll = t + dt;
         rcoll = rank_of(e.tcoll, paul, C.dtp, ctr++);
     }
@@ -1867,7 +2080,7 @@ __global__ __launch_bounds__(64) void k_repredict(const EdmdSys* __restrict__ sy
         A.erank[2 * i + 1] = rcross;
     }
     path_update<ML>(C, A, lane, P, i, make_leaf(e.tcoll, rcoll, e.tcross, rcross, e.partner, e.dir),
-                    -1, Leaf{}, nullptr);
+                    -1, Leaf{}, nullptr, t);
     const bool any_ov = __ballot(ov) != 0ull;
     if (lane == 0) {
         A.st->ctr = ctr;
