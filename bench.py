@@ -13,7 +13,7 @@ Reported beside it, in the same run:
   plain        single large system: N = 2^20 disks at phi = 0.80 on a
                triangular lattice, the same event loop on one system
                (collisions/s), and the batched/plain ratio;
-  sweep        the full prediction sweep of that system (kernel k_sweep):
+  sweep        the full prediction sweep of that system (kernel k_sweep_tiled):
                GB/s of algorithmic traffic, 80 B per disk, against the
                MI355X HBM peak (the `roofline` object);
   adjoint      derivatives of the event map: 512 replicas of N = 400 at
@@ -22,6 +22,14 @@ Reported beside it, in the same run:
                time-averaged collision virial with recomputation of every
                segment, tangent pass (finite-time growth of a random unit
                tangent), and the same with the whole tape kept in HBM;
+  dense        dense polydisperse batch with thermostat: 64 replicas of
+               N = 65536, diameters N(1, 0.1) truncated at +-3 sigma,
+               phi = 0.80, velocity-rescaling thermostat to T = 1 every 0.1
+               time units, started by growth from radius 0 (random centres,
+               growth rate 0.2: GROWSTOP at t = 5, about 31 collisions per
+               disk; the reference's own rate for phi = 0.8 is 0.0125); the
+               growth phase and then 2 * 10^5 thermostatted collisions per
+               replica are timed separately;
   cpu_baseline the reference's own CPU event loop (built with its shipped
                flags) on a bounded sample of replicas, one core.
 """
@@ -37,6 +45,10 @@ import time
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+# HBM bytes per launch of k_sweep_tiled at N = 2^20, measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE counters and calibrated against a copy of known
+# size; the summary holding it (with the commands) is committed here:
+SWEEP_TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "k_sweep_tiled_N1048576_traffic.md")
 PKG = os.path.join(HERE, "graphical-edmd_amd")
 HBM_PEAK_GBS = 8000.0
 SWEEP_BYTES_PER_DISK = 80  # x y vx vy t (40) + cell + coll + cell table (12) + slots (28)
@@ -48,6 +60,34 @@ def load_binding():
     sys.modules["edmd_hip"] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+def sweep_traffic():
+    """HBM bytes per sweep launch from the committed counter summary (its
+    line 'traffic_bytes_per_launch: <value>'), or None."""
+    try:
+        with open(SWEEP_TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    """A plain device copy of known size (1 GiB read, 1 GiB written, three
+    times), run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to scale those
+    counters to bytes."""
+    import torch
+    n = 256 * 1024 * 1024
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print("copied 3 x 1 GiB", flush=True)
 
 
 def replica_params(nrep, seed):
@@ -140,6 +180,48 @@ def adjoint_leg(E, args):
     return res
 
 
+def dense_leg(E, args):
+    """Growth start, then the thermostatted dense dynamics, of 64 large
+    polydisperse replicas in one launch."""
+    nrep, n, vr = args.dense_replicas, args.dense_n, args.dense_vr
+    t0 = time.perf_counter()
+    states = [E.init_random(n, 0.80, rad=0.5, poly=0.1, seed=args.seed + 7919 * (k + 1),
+                            e_per=0.05) for k in range(nrep)]
+    b = E.Batch(states, vr=vr, noise=2, dtnoise=0.1, temperature=1.0, einit=1.0)
+    log("dense: %d replicas of N=%d built in %.1f s" % (nrep, n, time.perf_counter() - t0))
+    res = {"workload": "%d replicas, N=%d, polydisperse N(1,0.1) diameters, phi=0.80, "
+                       "thermostat T=1 every 0.1, growth start at rate %g"
+                       % (nrep, n, vr), "growth_rate": vr}
+    # growth: up to just before GROWSTOP (stopGrow resets the counters)
+    t0 = time.perf_counter()
+    b.run(tmax=(1.0 / vr) * (1 - 1e-12), max_events=1 << 62)
+    t1 = time.perf_counter()
+    obs = [b.observables(k) for k in range(nrep)]
+    gcol = sum(o["ncol"] for o in obs)
+    gev = sum(o["nevents"] for o in obs)
+    res["growth"] = {"seconds": t1 - t0, "collisions": gcol, "events": gev,
+                     "collisions_per_s": gcol / (t1 - t0), "events_per_s": gev / (t1 - t0)}
+    log("dense growth: %d collisions in %.2f s = %.4e coll/s" % (gcol, t1 - t0, gcol / (t1 - t0)))
+    b.run(tmax=1.0 / vr + 1e-6, max_events=1 << 62)  # GROWSTOP: radii reach their targets
+    obs = [b.observables(k) for k in range(nrep)]
+    c0 = sum(o["ncol"] for o in obs)
+    e0 = sum(o["nevents"] for o in obs)
+    t0 = time.perf_counter()
+    b.run(max_collisions=args.dense_collisions, max_events=1 << 62)
+    t1 = time.perf_counter()
+    obs = [b.observables(k) for k in range(nrep)]
+    ncol = sum(o["ncol"] for o in obs) - c0
+    nev = sum(o["nevents"] for o in obs) - e0
+    res["thermostatted"] = {"seconds": t1 - t0, "collisions": ncol, "events": nev,
+                            "collisions_per_replica": args.dense_collisions,
+                            "collisions_per_s": ncol / (t1 - t0), "events_per_s": nev / (t1 - t0),
+                            "e_per_n_mean": float(np.mean([o["E"] / n for o in obs]))}
+    log("dense thermostatted: %d collisions in %.2f s = %.4e coll/s"
+        % (ncol, t1 - t0, ncol / (t1 - t0)))
+    b.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,7 +241,16 @@ def main():
     ap.add_argument("--adj-phi", type=float, default=0.60)
     ap.add_argument("--adj-collisions", type=int, default=10000)
     ap.add_argument("--adj-every", type=int, default=256)
+    ap.add_argument("--dense-replicas", type=int, default=64)
+    ap.add_argument("--dense-n", type=int, default=65536)
+    ap.add_argument("--dense-vr", type=float, default=0.2)
+    ap.add_argument("--dense-collisions", type=int, default=200000)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run the 1 GiB copy used to calibrate memory counters")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; replicas would shard across GPUs with no "
               "exchange, but --gpus %d is not supported here" % args.gpus, file=sys.stderr)
@@ -218,6 +309,10 @@ def main():
     if args.adj_replicas > 0:
         adj = adjoint_leg(E, args)
 
+    dense = None
+    if args.dense_replicas > 0:
+        dense = dense_leg(E, args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(E, ns, phis, args.seed, args.cpu_collisions, args.cpu_seconds)
@@ -249,10 +344,11 @@ def main():
         "plain_events_per_s": plain_ev,
         "batched_over_plain": value / plain,
         "sweep": {"n": args.plain_n, "ms": sweep_ms, "GB_per_s": sweep_gbs},
-        "roofline": {"kernel": "k_sweep", "bound": "hbm", "achieved": sweep_gbs,
+        "roofline": {"kernel": "k_sweep_tiled", "bound": "hbm", "achieved": sweep_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS,
-                     "traffic": None},
+                     "traffic": sweep_traffic() if args.plain_n == 1 << 20 else None},
         "adjoint": adj,
+        "dense": dense,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -987,6 +987,64 @@ extern "C" int edmd_init_lattice(int n, double phi, double rad, int lattice, uin
     return EDMD_OK;
 }
 
+extern "C" int edmd_init_random(int n, double phi, double rad, double poly, double sizeratio,
+                                double fsmall, uint64_t seed, double e_per, double* x, double* y,
+                                double* vx, double* vy, double* r, double* m, double* Lx,
+                                double* Ly) {
+    if (n < 2 || !(phi > 0) || !(phi < 1) || !(rad > 0) || !(poly >= 0) || !(poly < 1.0 / 3) ||
+        !(fsmall >= 0) || !(fsmall < 1) || !(sizeratio > 0) || !(e_per > 0) || !x || !y || !vx ||
+        !vy || !r || !m || !Lx || !Ly)
+        return fail(EDMD_ERR_ARG, "bad random-start arguments");
+    const double pi = 3.14159265358979323846;
+    uint64_t s = seed;
+    /* target radii and masses */
+    const int nsmall = (poly > 0) ? 0 : (int)(n * fsmall); /* Nsmall, EDMD.c:1703 */
+    const int nbig = n - nsmall;
+    for (int k = 0; k < n; k++) {
+        if (poly > 0) {
+            /* diameter ratio d ~ N(1, poly) truncated to [1 - 3 poly, 1 + 3 poly] */
+            double d;
+            do {
+                const double u1 = unit_open0(&s), u2 = unit_open0(&s);
+                d = 1 + poly * sqrt(-2 * log(u1)) * cos(2 * pi * u2);
+            } while (d < 1 - 3 * poly || d > 1 + 3 * poly);
+            r[k] = rad * d;
+            m[k] = d * d * d;
+        } else {
+            r[k] = k < nbig ? rad : rad * sizeratio; /* EDMD.c:1776-1789 */
+            m[k] = 1;
+        }
+    }
+    double area = 0;
+    for (int k = 0; k < n; k++) area += pi * r[k] * r[k];
+    const double L = sqrt(area / phi);
+    /* uniform centres, then Box-Muller velocities of variance e_per / m per
+     * component (EDMD.c:1719-1790 with EinitGrow = e_per) */
+    for (int k = 0; k < n; k++) {
+        x[k] = L * (unit_open0(&s) - 0.5 / 9007199254740992.0);
+        y[k] = L * (unit_open0(&s) - 0.5 / 9007199254740992.0);
+        if (x[k] >= L) x[k] = 0;
+        if (y[k] >= L) y[k] = 0;
+    }
+    for (int k = 0; k < n; k++) {
+        const double u1 = unit_open0(&s), u2 = unit_open0(&s);
+        const double a = sqrt(-2 * log(u1) / m[k] * e_per);
+        vx[k] = a * cos(2 * pi * u2);
+        vy[k] = a * sin(2 * pi * u2);
+    }
+    *Lx = *Ly = L;
+    return EDMD_OK;
+}
+
+extern "C" double edmd_growth_rate(double phi) {
+    /* optimizeGrowConstant, EDMD.c:5766-5800, for a periodic box without
+     * walls, coexistence or intruder: 0.1 below phi = 0.8, 0.1 (0.8/phi)^30
+     * above, divided by 8 above phi = 0.7 */
+    double vr = phi < 0.8 ? 0.1 : 0.1 * pow(0.8 / phi, 30);
+    if (phi > 0.7) vr /= 8;
+    return vr;
+}
+
 /* error hook for the other translation units of the library */
 int edmd_internal_fail(int code, const char* msg) { return fail(code, "%s", msg); }
 

@@ -49,7 +49,8 @@ struct Config {
     double einit = 1;
     double temperature = 0.05; /* EDMD.c:379 */
     int version = 1;
-    int lattice = 0;
+    int lattice = -1; /* -1: the reference's growth start; 0 square, 1 triangular */
+    double vr = 0;    /* growth rate of the growth start */
     double noise_period = 0;
     uint64_t max_coll = ~(uint64_t)0;
     int full_precision = 0;
@@ -131,6 +132,7 @@ extern "C" int edmd_compat_status(void) { return g_status; }
 extern "C" edmd_ctx* edmd_compat_context(void) { return g_ctx; }
 
 extern "C" void constantInit(int argc, char* argv[]) {
+    cfg = Config();
     /* option table of EDMD.c:750-797 plus this engine's extra options */
     static struct option longopt[] = {
         {"load", required_argument, NULL, 'l'},
@@ -248,17 +250,29 @@ extern "C" void particlesInit(void) {
         g_status = EDMD_ERR_ARG;
         return;
     } else {
-        if (cfg.fraction_small != 0 || cfg.aspect != 1 || cfg.control_length != 0) {
-            fprintf(stderr, "ERROR: only the monodisperse lattice start is available without -l; "
-                            "pass -x 0 (and no -a/-L/-X)\n");
+        if (cfg.aspect != 1 || cfg.control_length != 0) {
+            fprintf(stderr, "ERROR: only square boxes are available without -l (no -a/-L/-X)\n");
             g_status = EDMD_ERR_ARG;
             return;
         }
         hx.resize(N); hy.resize(N); hvx.resize(N); hvy.resize(N);
         hrad.assign(N, 1.0); /* the reference's large disks have radius 1 */
         hm.assign(N, 1.0);
-        int rc = edmd_init_lattice(N, phi, 1.0, cfg.lattice, (uint64_t)cfg.version, cfg.einit,
+        int rc;
+        if (cfg.lattice >= 0) {
+            /* monodisperse lattice start (--lattice sq|tri) */
+            rc = edmd_init_lattice(N, phi, 1.0, cfg.lattice, (uint64_t)cfg.version, cfg.einit,
                                    hx.data(), hy.data(), hvx.data(), hvy.data(), &Lx, &Ly);
+        } else {
+            /* the reference's default start (EDMD.c:1690-1810): random
+             * centres, a binary mixture of radii 1 and sizeratio (-q) with
+             * a fraction -x of small disks, growing at the rate
+             * optimizeGrowConstant picks until GROWSTOP */
+            rc = edmd_init_random(N, phi, 1.0, 0.0, cfg.sizeratio, cfg.fraction_small,
+                                  (uint64_t)cfg.version, 0.05, hx.data(), hy.data(), hvx.data(),
+                                  hvy.data(), hrad.data(), hm.data(), &Lx, &Ly);
+            cfg.vr = edmd_growth_rate(phi);
+        }
         if (rc) { report(rc, "particlesInit"); return; }
     }
     /* output names in the spirit of customName, EDMD.c:6015-6065 */
@@ -288,6 +302,8 @@ extern "C" void eventListInit(void) {
     d.dtime = cfg.dtime;
     d.first_thermo = cfg.first_thermo;
     d.dtime_thermo = cfg.dtime_thermo;
+    d.vr = cfg.load == 0 ? cfg.vr : 0.0;
+    d.einit = cfg.einit;
     if (cfg.noise_period > 0) {
         d.noise = 2;
         d.dtnoise = cfg.noise_period;

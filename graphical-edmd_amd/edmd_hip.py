@@ -110,6 +110,8 @@ def lib():
             "edmd_batch_set_stream": (i32, [vp, vp]),
             "edmd_batch_time_sweep": (i32, [vp, i32, vp]),
             "edmd_init_lattice": (i32, [i32, dbl, dbl, i32, u64, dbl] + [vp] * 6),
+            "edmd_init_random": (i32, [i32, dbl, dbl, dbl, dbl, dbl, u64, dbl] + [vp] * 8),
+            "edmd_growth_rate": (dbl, [dbl]),
             "edmd_dump_info": (i32, [C.c_char_p, vp, vp, vp]),
             "edmd_dump_read": (i32, [C.c_char_p, i32] + [vp] * 6),
             "edmd_dump_write": (i32, [C.c_char_p, i32, dbl, i32, dbl, dbl] + [vp] * 6 + [i32]),
@@ -441,6 +443,24 @@ def init_lattice(n, phi, rad=0.5, lattice=0, seed=1, temperature=1.0):
                                   vx.ctypes.data, vy.ctypes.data, C.byref(Lx), C.byref(Ly)))
     return {"x": x, "y": y, "vx": vx, "vy": vy, "rad": np.full(n, rad), "m": np.ones(n),
             "Lx": Lx.value, "Ly": Ly.value}
+
+
+def init_random(n, phi, rad=0.5, poly=0.0, sizeratio=1.0, fsmall=0.0, seed=1, e_per=0.05):
+    """Random start for a growth run (pair with vr > 0): target radii in
+    "rad", uniform centres, Gaussian velocities with variance e_per / m."""
+    a = {k: np.zeros(n) for k in ("x", "y", "vx", "vy", "rad", "m")}
+    Lx, Ly = C.c_double(), C.c_double()
+    check(lib().edmd_init_random(int(n), float(phi), float(rad), float(poly), float(sizeratio),
+                                 float(fsmall), int(seed), float(e_per),
+                                 *[a[k].ctypes.data for k in ("x", "y", "vx", "vy", "rad", "m")],
+                                 C.byref(Lx), C.byref(Ly)))
+    a["Lx"], a["Ly"] = Lx.value, Ly.value
+    return a
+
+
+def growth_rate(phi):
+    """The growth rate the reference picks for packing fraction phi."""
+    return lib().edmd_growth_rate(float(phi))
 
 
 def fp64_check(a, b):

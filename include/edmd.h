@@ -201,6 +201,21 @@ int edmd_batch_set_stream(edmd_batch* b, void* stream);
 int edmd_init_lattice(int n, double phi, double rad, int lattice, uint64_t seed,
                       double temperature, double* x, double* y, double* vx, double* vy,
                       double* Lx, double* Ly);
+/* Seeded random start for a growth run (the reference's default start,
+ * EDMD.c:1690-1810, with this library's generator): centres uniform in a
+ * square box sized so that the target radii fill the fraction phi; target
+ * radii rad for the first n - (int)(n * fsmall) disks and rad * sizeratio
+ * for the rest, masses 1 (EDMD.c:1776-1789); or, with poly > 0, radii
+ * rad * d with d ~ N(1, poly) truncated to [1 - 3 poly, 1 + 3 poly] and
+ * masses d^3; velocity components Gaussian with variance e_per / m (the
+ * reference's EinitGrow = 0.05).  Pair it with desc.vr > 0 (the disks start
+ * at radius 0).  Writes n values to each array and the box lengths. */
+int edmd_init_random(int n, double phi, double rad, double poly, double sizeratio, double fsmall,
+                     uint64_t seed, double e_per, double* x, double* y, double* vx, double* vy,
+                     double* r, double* m, double* Lx, double* Ly);
+/* optimizeGrowConstant (EDMD.c:5766-5800) for a periodic box: the growth
+ * rate the reference picks for packing fraction phi. */
+double edmd_growth_rate(double phi);
 /* LAMMPS dump reader (the load path, EDMD.c:987-995, 1387-1451; parser.c):
  * last frame, box from row 0 column 1 of BOX, columns by name prefix. */
 int edmd_dump_info(const char* path, int* n, double* Lx, double* Ly);

@@ -97,3 +97,34 @@ def test_growth_blocks_derivatives(gpu):
     assert e.value.code == 22
     b.run(max_events=200000, tmax=11.0)  # past GROWSTOP at t = 10
     b.deriv_begin()
+
+
+def test_cli_default_start_is_the_growth_start(gpu, tmp_path):
+    """`edmd -N 300 -p 0.6 -t 11.5 -D 1 --full-precision` without -l or
+    --lattice starts as the reference does by default: a binary mixture
+    (30% of radius 0.4, the rest radius 1) at random centres, growing at the
+    rate optimizeGrowConstant picks (0.1 at phi = 0.6, so GROWSTOP at t = 10);
+    screenshot times below 1/vr move by 1/vr, so the one frame is at t = 11.
+    It equals the library's run from the same seeded start."""
+    import os
+    import subprocess
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cli = os.path.join(repo, "graphical-edmd_amd", "bin", "edmd")
+    r = subprocess.run([cli, "-N", "300", "-p", "0.6", "-t", "11.5", "-D", "1",
+                        "--full-precision"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    dumps = [f for f in os.listdir(tmp_path / "dump") if f.endswith(".dump")]
+    assert len(dumps) == 1
+    frame = F.read_dump(str(tmp_path / "dump" / dumps[0]))
+    assert gpu.growth_rate(0.6) == 0.1
+    st = gpu.init_random(300, 0.6, rad=1.0, sizeratio=0.4, fsmall=0.3, seed=1, e_per=0.05)
+    s = gpu.System(st, vr=0.1, first_screen=1.0, dtime=1.0, first_thermo=1000.0,
+                   dtime_thermo=100.0, einit=1.0)
+    s.run(tmax=11.0)
+    so = s.state()
+    assert s.observables()["t"] == 11.0
+    for k in ("x", "y", "vx", "vy"):
+        np.testing.assert_array_equal(bits(frame[k]), bits(so[k]), err_msg=k)
+    np.testing.assert_array_equal(frame["rad"], st["rad"])
+    assert np.sum(st["rad"] == 0.4) == 90

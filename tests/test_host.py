@@ -83,6 +83,47 @@ def test_lattice_start(lattice, n):
     assert not np.array_equal(st3["vx"], vx)
 
 
+@pytest.mark.parametrize("kw", [dict(poly=0.1), dict(rad=1.0, sizeratio=0.4, fsmall=0.3),
+                                dict(poly=0.0)])
+def test_random_growth_start(kw):
+    """Target radii fill the box to phi exactly; truncated Gaussian
+    diameters or the binary mixture's species counts; centres inside the
+    box; velocity components of variance e_per / m; seeded."""
+    n, phi = 4000, 0.8
+    st = E.init_random(n, phi, seed=5, e_per=0.05, **kw)
+    L = st["Lx"]
+    assert st["Ly"] == L
+    assert abs(np.pi * np.sum(st["rad"] ** 2) / (L * L) - phi) < 1e-12
+    assert st["x"].min() >= 0 and st["x"].max() < L and st["y"].min() >= 0 and st["y"].max() < L
+    rad = kw.get("rad", 0.5)
+    if kw.get("poly", 0) > 0:
+        d = st["rad"] / rad
+        assert d.min() >= 0.7 and d.max() <= 1.3
+        assert abs(d.mean() - 1) < 0.01 and abs(d.std() - 0.1) < 0.01
+        np.testing.assert_allclose(st["m"], d ** 3)
+    elif kw.get("fsmall", 0) > 0:
+        nsmall = int(n * kw["fsmall"])
+        assert np.all(st["rad"][: n - nsmall] == rad)
+        assert np.all(st["rad"][n - nsmall:] == rad * kw["sizeratio"])
+        assert np.all(st["m"] == 1)
+    # <m v_x^2> = e_per per component
+    for c in ("vx", "vy"):
+        assert abs(np.mean(st["m"] * st[c] ** 2) / 0.05 - 1) < 0.06
+    st2 = E.init_random(n, phi, seed=5, e_per=0.05, **kw)
+    assert np.array_equal(st2["x"], st["x"]) and np.array_equal(st2["vy"], st["vy"])
+    st3 = E.init_random(n, phi, seed=6, e_per=0.05, **kw)
+    assert not np.array_equal(st3["x"], st["x"])
+
+
+def test_growth_rate_follows_reference_rule():
+    """optimizeGrowConstant (EDMD.c:5766-5800) for a periodic box."""
+    assert E.growth_rate(0.5) == 0.1
+    assert E.growth_rate(0.7) == 0.1
+    assert E.growth_rate(0.75) == 0.1 / 8
+    assert E.growth_rate(0.8) == 0.1 / 8
+    assert E.growth_rate(0.84) == 0.1 * (0.8 / 0.84) ** 30 / 8
+
+
 def test_lattice_rejects_impossible_packing():
     with pytest.raises(E.EdmdError) as ei:
         E.init_lattice(100, 0.85, rad=0.5, lattice=0)  # square lattice overlaps above pi/4
@@ -146,11 +187,3 @@ def test_cli_reports_missing_device():
     assert "no HIP device" in r.stderr
 
 
-def test_cli_rejects_growth_start():
-    """Without -l and without -x 0 the reference starts a binary mixture by
-    random insertion and growth; this engine reports that it has no such
-    start instead of substituting another state."""
-    r = subprocess.run([CLI, "-N", "100", "-p", "0.5", "-t", "10", "--no-output"],
-                       capture_output=True, text=True, timeout=60)
-    assert r.returncode == 22
-    assert "-x 0" in r.stderr
