@@ -192,6 +192,7 @@ __device__ __forceinline__ void fly_grow(const EdmdConst& C, double t, double& x
  * synchronised owner (x1..) with candidate 2 last updated at tl2. */
 /* The same with candidate 2's position already flown to t:
  * xt2 = x2 + (t - tl2) * vx2, yt2 likewise (EDMD.c:2667-2671). */
+template <bool WRAP = true>
 __device__ __forceinline__ double coll_time_at(const EdmdConst& C, double x1, double y1,
                                                double vx1, double vy1, double r1, double xt2,
                                                double yt2, double vx2, double vy2, double r2,
@@ -200,7 +201,7 @@ __device__ __forceinline__ double coll_time_at(const EdmdConst& C, double x1, do
     double dvy = vy2 - vy1;
     double dx = xt2 - x1;
     double dy = yt2 - y1;
-    min_image(C, dx, dy);
+    if (WRAP) min_image(C, dx, dy);
     double b = dx * dvx + dy * dvy;
     if (b > 0) return EDMD_NEVER;
     double v2 = dvx * dvx + dvy * dvy;
@@ -923,7 +924,7 @@ __device__ __forceinline__ bool sweep_one(const EdmdConst& C, const EdmdArrays& 
 #define SW_STAGE 512
 #endif
 #ifndef SW_THREADS
-#define SW_THREADS 256
+#define SW_THREADS 320 /* five waves: a tile of 8 x 8 cells holds ~260 disks at phi = 0.8 */
 #endif
 #define SW_HW (SW_TX + 2)
 #define SW_HH (SW_TY + 2)
@@ -962,6 +963,66 @@ __device__ __forceinline__ int sweep_scan(int lane, const int* v, int* out, int 
     if (2 * lane < SW_NH) out[2 * lane] = base + excl;
     if (2 * lane + 1 < SW_NH) out[2 * lane + 1] = base + excl + a;
     return __shfl(incl, 63, 64);
+}
+
+/* The predictions of a staged tile's own disks: one thread per disk,
+ * candidates from LDS in the reference's scan order (y offset outer, x
+ * offset inner, newest entry first) with the strict-less rule. */
+template <bool WRAP>
+__device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const EdmdArrays& A,
+                                                   const SweepLds& L, int tid, int ninner, double t,
+                                                   int tx0, int ty0, double paul, int64_t ctr0,
+                                                   bool& ov) {
+    for (int u = tid; u < ninner; u += SW_THREADS) {
+        const int p = L.id[u];
+        const int h = L.hcell[u];
+        const int lx = h % SW_HW - 1, ly = h / SW_HW - 1;
+        /* the owner is synchronised at t (its staged position is its own) */
+        const double ox = L.xt[u], oy = L.yt[u], ovx = L.vx[u], ovy = L.vy[u], orad = L.rad[u];
+        double tc;
+        int dir;
+        cross_time(C, t, ox, oy, ovx, ovy, tx0 + lx, ty0 + ly, tc, dir);
+        double dt = EDMD_NEVER;
+        int best = -1;
+        for (int j = -1; j <= 1; ++j) {
+            for (int k = -1; k <= 1; ++k) {
+                const int hh = (ly + 1 + j) * SW_HW + (lx + 1 + k);
+                const int st0 = L.start[hh];
+                for (int v = st0 + L.cnt[hh] - 1; v >= st0; --v) {
+                    /* the owner itself; a halo that wraps (boxes under three
+                     * tiles wide) may stage the owner's cell more than once */
+                    if (WRAP ? L.id[v] == p : v == u) continue;
+                    double tt = coll_time_at<WRAP>(C, ox, oy, ovx, ovy, orad, L.xt[v], L.yt[v],
+                                                   L.vx[v], L.vy[v], L.rad[v], ov);
+                    if (dt > tt) {
+                        dt = tt;
+                        best = v;
+                    }
+                }
+            }
+        }
+        const double tk = t + dt;
+        const int partner = best >= 0 ? L.id[best] : 0;
+        EdmdEv e;
+        e.tcoll = tk;
+        e.tcross = tc;
+        e.partner = partner;
+        e.cact = A.p[partner].coll;
+        e.dir = dir;
+        e.pad = 0;
+        A.e[p] = e;
+        const int64_t rk = rank_of(tk, paul, C.dtp, ctr0 + 2 * (int64_t)p + 1);
+        const int64_t rx = rank_of(tc, paul, C.dtp, ctr0 + 2 * (int64_t)p);
+        A.erank[2 * p] = rk;
+        A.erank[2 * p + 1] = rx;
+        const Leaf Lf = make_leaf(tk, rk, tc, rx, partner, dir);
+        EdmdNode n;
+        n.key = Lf.key;
+        n.idx = p;
+        n.aux = Lf.aux;
+        A.leaf[p] = n;
+        A.lrank[p] = Lf.rank;
+    }
 }
 
 __global__ __launch_bounds__(SW_THREADS) void k_sweep_tiled(const EdmdSys* __restrict__ systems) {
@@ -1057,54 +1118,16 @@ __global__ __launch_bounds__(SW_THREADS) void k_sweep_tiled(const EdmdSys* __res
             }
         }
         __syncthreads();
-        for (int u = tid; u < ninner; u += SW_THREADS) {
-            const int p = L.id[u];
-            const int h = L.hcell[u];
-            const int lx = h % SW_HW - 1, ly = h / SW_HW - 1;
-            /* the owner is synchronised at t (its staged position is its own) */
-            const double ox = L.xt[u], oy = L.yt[u], ovx = L.vx[u], ovy = L.vy[u], orad = L.rad[u];
-            double tc;
-            int dir;
-            cross_time(C, t, ox, oy, ovx, ovy, tx0 + lx, ty0 + ly, tc, dir);
-            double dt = EDMD_NEVER;
-            int best = -1;
-            for (int j = -1; j <= 1; ++j) {
-                for (int k = -1; k <= 1; ++k) {
-                    const int hh = (ly + 1 + j) * SW_HW + (lx + 1 + k);
-                    const int st0 = L.start[hh];
-                    for (int v = st0 + L.cnt[hh] - 1; v >= st0; --v) {
-                        if (L.id[v] == p) continue;
-                        double tt = coll_time_at(C, ox, oy, ovx, ovy, orad, L.xt[v], L.yt[v],
-                                                 L.vx[v], L.vy[v], L.rad[v], ov);
-                        if (dt > tt) {
-                            dt = tt;
-                            best = v;
-                        }
-                    }
-                }
-            }
-            const double tk = t + dt;
-            const int partner = best >= 0 ? L.id[best] : 0;
-            EdmdEv e;
-            e.tcoll = tk;
-            e.tcross = tc;
-            e.partner = partner;
-            e.cact = A.p[partner].coll;
-            e.dir = dir;
-            e.pad = 0;
-            A.e[p] = e;
-            const int64_t rk = rank_of(tk, paul, C.dtp, ctr0 + 2 * (int64_t)p + 1);
-            const int64_t rx = rank_of(tc, paul, C.dtp, ctr0 + 2 * (int64_t)p);
-            A.erank[2 * p] = rk;
-            A.erank[2 * p + 1] = rx;
-            const Leaf Lf = make_leaf(tk, rk, tc, rx, partner, dir);
-            EdmdNode n;
-            n.key = Lf.key;
-            n.idx = p;
-            n.aux = Lf.aux;
-            A.leaf[p] = n;
-            A.lrank[p] = Lf.rank;
-        }
+        /* a tile whose halo does not wrap around the box, in a box at
+         * least five cells wide: every candidate lies within two cells of
+         * its owner, so the minimum image (EDMD.c:5896-5913) changes
+         * nothing and is skipped (uniform over the workgroup) */
+        const bool nowrap = tx0 >= 1 && ty0 >= 1 && tx0 + vw <= ncx - 1 && ty0 + vh <= ncy - 1 &&
+                            ncx >= 5 && ncy >= 5;
+        if (nowrap)
+            sweep_tile_predict<false>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
+        else
+            sweep_tile_predict<true>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
     } else {
         const int cap = C.cap;
         for (int w = tid; w < SW_TX * SW_TY * cap; w += SW_THREADS) {

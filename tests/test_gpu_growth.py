@@ -121,7 +121,8 @@ def test_cli_default_start_is_the_growth_start(gpu, tmp_path):
     st = gpu.init_random(300, 0.6, rad=1.0, sizeratio=0.4, fsmall=0.3, seed=1, e_per=0.05)
     s = gpu.System(st, vr=0.1, first_screen=1.0, dtime=1.0, first_thermo=1000.0,
                    dtime_thermo=100.0, einit=1.0)
-    s.run(tmax=11.0)
+    reason, kind = s.run_until_kind(1 << 2)  # stop right after the screenshot
+    assert kind == 2
     so = s.state()
     assert s.observables()["t"] == 11.0
     for k in ("x", "y", "vx", "vy"):
