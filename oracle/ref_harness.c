@@ -68,6 +68,10 @@ static void start(const char* dump, int nextra, char** extra) {
     init_genrand(1);
     constantInit(n, args);
     configureTransverseVelocityCorrelationSampling(&dtimeThermo, &logSpacing);
+    /* the reference fixes firstThermo = 1000 (EDMD.c:248, no option);
+     * EDMD_HARNESS_FIRST_THERMO moves it for short statistical runs */
+    const char* ft = getenv("EDMD_HARNESS_FIRST_THERMO");
+    if (ft) firstThermo = atof(ft);
     particlesInit();
     cellListInit();
     eventListInit();

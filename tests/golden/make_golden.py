@@ -82,6 +82,14 @@ GROWTH_CASES = [
 
 THERMO_CASE = ("thermo64", lambda: F.lattice_state(64, 0.50, seed=4), 1300.0, ["-o", "100"])
 
+# Long run of the cfg1 system with the reference built with its shipped
+# flags (-O3 -ffast-math; its trajectory leaves the strict one after ~10^4
+# collisions): thermo rows every time unit to t = 60 (about 2 x 10^6
+# collisions) and the states at t = 30, 40, 50, 60, for statistical
+# comparisons of the reduced pressure and g(r).  The first thermo row is at
+# t = 1 (the harness moves the reference's fixed firstThermo = 1000).
+LONGRUN_CASE = ("cfg1_longrun", lambda: F.lattice_state(2000, 0.70, seed=1), [30.0, 40.0, 50.0, 60.0])
+
 
 def state_arrays(st):
     return {"x": st["x"], "y": st["y"], "vx": st["vx"], "vy": st["vy"], "rad": st["rad"],
@@ -148,6 +156,8 @@ def main():
         np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
         print(name, "N=%d events=%d ncol=%d" % (len(st["x"]), nev, t["glob"]["ncol"]))
 
+    make_longrun(tmp)
+
     name, build, tmax, extra = THERMO_CASE
     st = build()
     dump = os.path.join(tmp, name + ".dump")
@@ -166,5 +176,35 @@ def main():
     print(name, "rows=%d ncol=%d" % (len(rows), t["glob"]["ncol"]))
 
 
+def make_longrun(tmp):
+    name, build, times = LONGRUN_CASE
+    st = build()
+    dump = os.path.join(tmp, name + ".dump")
+    F.write_dump(dump, st)
+    arrs = state_arrays(st)
+    for k, tm in enumerate(times):
+        out = os.path.join(tmp, "%s_%d.state" % (name, k))
+        th = os.path.join(tmp, "%s_%d.thermo" % (name, k))
+        os.environ["EDMD_HARNESS_FIRST_THERMO"] = "1"
+        try:
+            F.run_ref(["run", dump, out, tm, th, "-o", "1"], kind="fast")
+        finally:
+            del os.environ["EDMD_HARNESS_FIRST_THERMO"]
+        t = F.parse_trace(out)
+        s = t["state"]
+        for f in ("x", "y", "vx", "vy", "t"):
+            arrs["snap%d_%s" % (k, f)] = s[f]
+        arrs["snap%d_time" % k] = np.float64(t["glob"]["t"])
+        if k == len(times) - 1:
+            rows = [list(map(float, line.split())) for line in open(th) if line.strip()]
+            arrs["thermo_rows"] = np.array([r[:8] for r in rows])
+    arrs["snap_tmax"] = np.array(times)
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
+    print(name, "rows=%d" % len(arrs["thermo_rows"]))
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["longrun"]:
+        make_longrun(tempfile.mkdtemp(prefix="edmd_golden_"))
+    else:
+        main()
