@@ -724,6 +724,56 @@ extern "C" int edmd_ctx_create(const edmd_system_desc* desc, edmd_ctx** out) {
     return EDMD_OK;
 }
 
+extern "C" int edmd_ctx_create_from_dump(const char* path, const edmd_system_desc* params,
+                                         edmd_ctx** out) {
+    if (!out) return fail(EDMD_ERR_ARG, "null output handle");
+    *out = nullptr;
+    int n = 0;
+    double Lx = 0, Ly = 0;
+    int rc = edmd_dump_info(path, &n, &Lx, &Ly);
+    if (rc) return rc;
+    std::vector<double> x(n), y(n), vx(n), vy(n), rad(n), m(n);
+    rc = edmd_dump_read(path, n, x.data(), y.data(), vx.data(), vy.data(), rad.data(), m.data());
+    if (rc) return rc;
+    edmd_system_desc d;
+    if (params) d = *params;
+    else edmd_default_desc(&d);
+    d.n = n;
+    d.Lx = Lx;
+    d.Ly = Ly;
+    d.x = x.data(); d.y = y.data(); d.vx = vx.data(); d.vy = vy.data();
+    d.rad = rad.data(); d.m = m.data();
+    return edmd_ctx_create(&d, out);
+}
+
+extern "C" int edmd_run_traced(edmd_ctx* c, int64_t max_events, uint64_t max_collisions,
+                               double tmax, edmd_trace_cb cb, void* user) {
+    if (!c) return fail(EDMD_ERR_ARG, "null context");
+    if (!cb) return fail(EDMD_ERR_ARG, "null trace callback");
+    const int64_t cap = c->b->hs[0].c.trace_cap;
+    if (cap <= 0) return fail(EDMD_ERR_ARG, "edmd_run_traced needs desc.trace_capacity > 0");
+    std::vector<edmd_trace_rec> buf((size_t)cap);
+    int64_t n = 0;
+    int rc = edmd_get_trace(c, buf.data(), cap, &n); /* drop what earlier runs left */
+    if (rc) return rc;
+    for (;;) {
+        edmd_observables o;
+        if ((rc = edmd_get_observables(c, &o))) return rc;
+        const int64_t done = (int64_t)o.nevents;
+        if (done >= max_events || o.ncol >= max_collisions || !(o.t <= tmax)) return EDMD_OK;
+        /* at most `cap` events per chunk, so the trace buffer never drops a record */
+        const int64_t bound = max_events - done > cap ? done + cap : max_events;
+        rc = edmd_run(c, bound, max_collisions, tmax);
+        int64_t got = 0;
+        int rc2 = edmd_get_trace(c, buf.data(), cap, &got);
+        if (got > 0 && cb(buf.data(), got, user) != 0) return rc ? rc : rc2;
+        if (rc) return rc;
+        if (rc2) return rc2;
+        if ((rc = edmd_get_observables(c, &o))) return rc;
+        if (o.stop_reason != EDMD_STOP_EVENTS) return EDMD_OK; /* time, collisions or empty */
+    }
+}
+
 extern "C" int edmd_ctx_destroy(edmd_ctx* c) {
     if (!c) return EDMD_OK;
     batch_free(c->b);

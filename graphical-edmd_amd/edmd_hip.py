@@ -80,6 +80,8 @@ def lib():
             "edmd_default_desc": (None, [vp]),
             "edmd_ctx_create": (i32, [vp, vp]),
             "edmd_ctx_destroy": (i32, [vp]),
+            "edmd_ctx_create_from_dump": (i32, [C.c_char_p, vp, vp]),
+            "edmd_run_traced": (i32, [vp, i64, u64, dbl, vp, vp]),
             "edmd_sweep_predict": (i32, [vp]),
             "edmd_run": (i32, [vp, i64, u64, dbl]),
             "edmd_run_until_kind": (i32, [vp, i64, u64, dbl, i32, vp, vp]),
@@ -185,8 +187,40 @@ class System:
 
     __del__ = close
 
+    @classmethod
+    def from_dump(cls, path, **kw):
+        """A system loaded from the last frame of a LAMMPS dump; kw: the other
+        descriptor fields."""
+        self = cls.__new__(cls)
+        d = SystemDesc()
+        lib().edmd_default_desc(C.byref(d))
+        for k, v in kw.items():
+            setattr(d, k, v)
+        h = C.c_void_p()
+        check(lib().edmd_ctx_create_from_dump(str(path).encode(), C.byref(d), C.byref(h)))
+        self.h = h
+        n, Lx, Ly = C.c_int(), C.c_double(), C.c_double()
+        check(lib().edmd_get_box(h, C.byref(n), C.byref(Lx), C.byref(Ly)))
+        self.n = n.value
+        self.trace_capacity = d.trace_capacity
+        return self
+
     def run(self, max_events=INT64_MAX, max_collisions=UINT64_MAX, tmax=1e300):
         check(lib().edmd_run(self.h, int(max_events), int(max_collisions), float(tmax)))
+
+    def run_traced(self, callback, max_events=INT64_MAX, max_collisions=UINT64_MAX, tmax=1e300):
+        """edmd_run_traced: callback(records) receives every handled event in
+        order (numpy records of TRACE_DT); a true return stops the run."""
+        proto = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.c_void_p)
+
+        def tramp(ptr, n, user):
+            buf = (C.c_char * (int(n) * TRACE_DT.itemsize)).from_address(ptr)
+            recs = np.frombuffer(bytes(buf), dtype=TRACE_DT)
+            return 1 if callback(recs) else 0
+
+        cb = proto(tramp)
+        check(lib().edmd_run_traced(self.h, int(max_events), int(max_collisions), float(tmax),
+                                    C.cast(cb, C.c_void_p), None))
 
     def run_until_kind(self, stop_mask, max_events=INT64_MAX, max_collisions=UINT64_MAX,
                        tmax=1e300):

@@ -112,6 +112,10 @@ typedef struct edmd_ctx edmd_ctx;
  * prediction sweep and calendar build (eventListInit, EDMD.c:1935-2012). */
 int edmd_ctx_create(const edmd_system_desc* desc, edmd_ctx** out);
 int edmd_ctx_destroy(edmd_ctx* ctx);
+/* The load path (EDMD.c:987-995, 1387-1451; parser.c): the last frame of a
+ * LAMMPS dump as the initial state, the other fields from params (NULL =
+ * edmd_default_desc). */
+int edmd_ctx_create_from_dump(const char* path, const edmd_system_desc* params, edmd_ctx** out);
 /* Full prediction sweep at the current time (kernel k_sweep): free-flies
  * every particle to the current time first, then re-predicts every
  * crossing and collision slot and rebuilds the calendar (the pattern of
@@ -121,6 +125,13 @@ int edmd_sweep_predict(edmd_ctx* ctx);
  * the system's event counter reaches max_events or its valid-collision
  * counter reaches max_collisions (both absolute counts). */
 int edmd_run(edmd_ctx* ctx, int64_t max_events, uint64_t max_collisions, double tmax);
+/* edmd_run handing every handled event to cb in order, in chunks of at most
+ * desc.trace_capacity records (which must be > 0); a nonzero return from cb
+ * stops the run after the current chunk.  The reference prints or records
+ * events in its loop (EDMD.c:534-610); this is that hook. */
+typedef int (*edmd_trace_cb)(const edmd_trace_rec* recs, int64_t n, void* user);
+int edmd_run_traced(edmd_ctx* ctx, int64_t max_events, uint64_t max_collisions, double tmax,
+                    edmd_trace_cb cb, void* user);
 int edmd_get_state(edmd_ctx* ctx, double* x, double* y, double* vx, double* vy,
                    double* t_local, uint32_t* coll, int32_t* cx, int32_t* cy);
 int edmd_get_events(edmd_ctx* ctx, double* t_cross, int32_t* dir, double* t_coll,

@@ -1,14 +1,10 @@
 """GPU parity: libedmd_hip.so (through its C ABI) against the reference's
 golden vectors and against the oracle, bit for bit.
 
-Mirrored collision events of a pair predicted from a common synchronised
-state tie exactly and either may pop first (EDMD.c:2179, 2164-2168); the
-state they produce is the same, so traces are compared with collision pairs
-unordered.  Events of exactly equal time (disks starting exactly on a cell
-edge cross at t = 0 together) are compared as sets: the reference orders
-them by its queue's insertion history.  Everything else (kinds, times,
-valid flags, states, virial sums) is compared bitwise, and final states are
-always compared bitwise."""
+Event traces are compared in raw order: kinds, disks (including which of a
+mirrored pair's two events pops first, which the calendar reproduces from
+the reference queue's insertion order, EDMD.c:2144-2327), validity and time
+bits; states, virial sums and thermo rows bitwise."""
 import ctypes as C
 import os
 import subprocess
@@ -191,6 +187,31 @@ def test_calendar_is_consistent(gpu):
         for k in range(len(states)):
             bad = check_calendar(b.calendar(k), b.events(k))
             assert not bad, "system %d after %d events: %s" % (k, nev, bad[:4])
+
+
+def test_dump_load_and_traced_run(gpu, tmp_path):
+    """edmd_ctx_create_from_dump + edmd_run_traced: the callback sees every
+    event of the reference's trace in order, in chunks no larger than the
+    trace capacity, across several calls; a true return stops the run."""
+    g = load_golden("cfg1_d1")
+    st = golden_state(g)
+    dump = str(tmp_path / "init.dump")
+    F.write_dump(dump, st)  # 17 significant digits: the state round-trips exactly
+    s = gpu.System.from_dump(dump, trace_capacity=1000)
+    assert s.n == len(st["x"])
+    chunks = []
+    s.run_traced(lambda r: chunks.append(r.copy()), max_events=7500)
+    s.run_traced(lambda r: chunks.append(r.copy()), max_events=12345)
+    assert max(len(c) for c in chunks) <= 1000
+    got = np.concatenate(chunks)
+    assert len(got) == 12345
+    ref = np.zeros(len(got), dtype=gpu.TRACE_DT)
+    for k in ("type", "i", "j", "valid", "t"):
+        ref[k] = g["trace_" + k][: len(got)]
+    assert_trace_equal(got, ref)
+    seen = []
+    s.run_traced(lambda r: seen.append(len(r)) or True, max_events=20000)
+    assert seen == [1000] and s.observables()["nevents"] == 13345
 
 
 def test_batch_collision_cap(gpu):

@@ -179,6 +179,16 @@ def test_dump_errors(tmp_path):
     assert L.edmd_dump_info(str(bad).encode(), C.byref(n), None, None) == 5
 
 
+def test_ctx_from_dump_reports_io_errors(tmp_path):
+    """edmd_ctx_create_from_dump reads the file before touching the GPU."""
+    with pytest.raises(E.EdmdError) as ei:
+        E.System.from_dump(tmp_path / "missing.dump")
+    assert ei.value.code == 5
+    L = E.lib()
+    assert L.edmd_ctx_create_from_dump(str(tmp_path / "missing.dump").encode(), None, None) == 22
+    assert L.edmd_run_traced(None, 10, 10, 1.0, None, None) == 22
+
+
 @pytest.mark.skipif(E.device_count() > 0, reason="checks the no-device path")
 def test_cli_reports_missing_device():
     r = subprocess.run([CLI, "-N", "100", "-p", "0.5", "-x", "0", "-t", "10", "--no-output"],
