@@ -332,6 +332,7 @@ static int upload_system(edmd_batch* b, int s, const edmd_system_desc* d) {
     st.rank_noise = d->noise ? rank0(st.next_noise) : INT64_MAX;
     st.last_thermo = 0;
     st.ev_limit = INT64_MAX;
+    st.coll_limit = ~(uint64_t)0;
     HIPCHK(hipMemcpy(b->st + s, &st, sizeof st, hipMemcpyHostToDevice));
     return EDMD_OK;
 }
@@ -507,6 +508,64 @@ static int batch_run_mask(edmd_batch* b, int64_t max_events, uint64_t max_collis
 extern "C" int edmd_batch_run(edmd_batch* b, int64_t max_events, uint64_t max_collisions,
                               double tmax) {
     return batch_run_mask(b, max_events, max_collisions, tmax, 0);
+}
+
+extern "C" int edmd_batch_run_epochs(edmd_batch* b, int nepochs, int64_t events_per_epoch,
+                                     uint64_t collisions_per_epoch, double tmax,
+                                     edmd_epoch_row* rows) {
+    static_assert(sizeof(edmd_epoch_row) == sizeof(EdmdEpochRow), "epoch row layout");
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    if (nepochs <= 0 || events_per_epoch <= 0 || collisions_per_epoch == 0)
+        return fail(EDMD_ERR_ARG, "epochs need nepochs > 0 and positive bounds");
+    if (b->deriv_on) return fail(EDMD_ERR_STATE, "epochs are not available while taping");
+    const int nsys = b->nsys;
+    EdmdEpochRow* d_rows = nullptr;
+    if (hipMalloc(&d_rows, (size_t)nepochs * nsys * sizeof(EdmdEpochRow)) != hipSuccess)
+        return fail(EDMD_ERR_NOMEM, "hipMalloc of %d epoch rows failed", nepochs * nsys);
+    EdmdRunArgs ra;
+    memset(&ra, 0, sizeof ra);
+    ra.max_events = INT64_MAX; /* the per-system epoch bounds apply */
+    ra.max_coll = ~(uint64_t)0;
+    ra.tmax = tmax;
+    /* one graph holds every epoch: bounds, the growing systems' launch (a
+     * no-op for systems that are not growing), the Normal launch and the
+     * counters; captured once, launched once */
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    for (int k = 0; e == hipSuccess && k < nepochs; k++) {
+        e = edmd_launch_epoch_begin(b->d_sys, nsys, events_per_epoch, collisions_per_epoch,
+                                    b->stream);
+        if (e == hipSuccess && b->any_grow)
+            e = edmd_launch_run(b->d_sys, nsys, b->max_nlev, 1, ra, b->stream);
+        if (e == hipSuccess) e = edmd_launch_run(b->d_sys, nsys, b->max_nlev, 0, ra, b->stream);
+        if (e == hipSuccess)
+            e = edmd_launch_epoch_end(b->d_sys, nsys, d_rows + (size_t)k * nsys, b->stream);
+    }
+    hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e == hipSuccess && rows)
+        e = hipMemcpy(rows, d_rows, (size_t)nepochs * nsys * sizeof(EdmdEpochRow),
+                      hipMemcpyDeviceToHost);
+    (void)hipFree(d_rows);
+    /* lift the epoch bounds again */
+    if (e == hipSuccess)
+        e = edmd_launch_epoch_begin(b->d_sys, nsys, INT64_MAX, ~(uint64_t)0, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return fail(EDMD_ERR_HIP, "epoch graph: %s", hipGetErrorString(e));
+    if (b->any_grow) {
+        std::vector<EdmdState> st(nsys);
+        HIPCHK(hipMemcpy(st.data(), b->st, nsys * sizeof(EdmdState), hipMemcpyDeviceToHost));
+        bool still = false;
+        for (int s = 0; s < nsys; s++) still |= st[s].growing != 0;
+        b->any_grow = still;
+    }
+    return check_status(b);
 }
 
 extern "C" int edmd_batch_sweep_predict(edmd_batch* b) {

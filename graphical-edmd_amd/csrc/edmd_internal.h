@@ -12,6 +12,10 @@ int edmd_sweep_tiles(int ncx, int ncy);
 hipError_t edmd_launch_tree(const EdmdSys* d_sys, int nsys, const int* max_lsize, int max_nlev,
                             hipStream_t s);
 hipError_t edmd_launch_sync_all(const EdmdSys* d_sys, int nsys, int max_n, hipStream_t s);
+hipError_t edmd_launch_epoch_begin(const EdmdSys* d_sys, int nsys, int64_t ev_inc,
+                                   uint64_t coll_inc, hipStream_t s);
+hipError_t edmd_launch_epoch_end(const EdmdSys* d_sys, int nsys, EdmdEpochRow* rows,
+                                 hipStream_t s);
 hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, int grow, EdmdRunArgs ra,
                            hipStream_t s);
 hipError_t edmd_launch_sweep_grow(const EdmdSys* d_sys, int nsys, int max_n, hipStream_t s);

@@ -1936,11 +1936,12 @@ __device__ __forceinline__ void run_loop(const EdmdSys* __restrict__ systems, Ed
     int last_type = st->last_type;
 
     const int64_t ev_limit = st->ev_limit < ra.max_events ? st->ev_limit : ra.max_events;
+    const uint64_t coll_limit = st->coll_limit < ra.max_coll ? st->coll_limit : ra.max_coll;
 
     for (;;) {
         if (R.status != EDMD_DEV_OK) { reason = EDMD_STOP_ERROR; break; }
         if ((int64_t)R.nevents >= ev_limit) { reason = EDMD_STOP_EVENTS; break; }
-        if (R.ncol >= ra.max_coll) { reason = EDMD_STOP_COLLISIONS; break; }
+        if (R.ncol >= coll_limit) { reason = EDMD_STOP_COLLISIONS; break; }
         if (!(R.t <= ra.tmax)) { reason = EDMD_STOP_TIME; break; }
         double key;
         int p, aux;
@@ -2239,6 +2240,47 @@ static void launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, EdmdRunArgs
     else if (max_nlev == 2) hipLaunchKernelGGL((k_run_small<2, G>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
     else if (max_nlev <= 4) hipLaunchKernelGGL((k_run<4, G>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
     else hipLaunchKernelGGL((k_run<EDMD_MAX_LEVELS, G>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
+}
+
+/* Epochs of a graph-captured run: the bounds of the coming epoch relative
+ * to each system's counters (one thread per system), and each system's
+ * counters when the epoch has run. */
+__global__ __launch_bounds__(256) void k_epoch_begin(const EdmdSys* __restrict__ systems, int nsys,
+                                                     int64_t ev_inc, uint64_t coll_inc) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    EdmdState EDMD_GLOBAL* st = systems[s].a.st;
+    const uint64_t ne = st->nevents, nc = st->ncol;
+    st->ev_limit = ev_inc > INT64_MAX - (int64_t)ne ? INT64_MAX : (int64_t)ne + ev_inc;
+    st->coll_limit = coll_inc > ~(uint64_t)0 - nc ? ~(uint64_t)0 : nc + coll_inc;
+}
+
+__global__ __launch_bounds__(256) void k_epoch_end(const EdmdSys* __restrict__ systems, int nsys,
+                                                   EdmdEpochRow* __restrict__ rows) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    const EdmdState EDMD_GLOBAL* st = systems[s].a.st;
+    EdmdEpochRow r;
+    r.t = st->t;
+    r.ncol = st->ncol;
+    r.ncross = st->ncross;
+    r.nevents = st->nevents;
+    r.status = st->status;
+    r.stop_reason = st->stop_reason;
+    rows[s] = r;
+}
+
+hipError_t edmd_launch_epoch_begin(const EdmdSys* d_sys, int nsys, int64_t ev_inc,
+                                   uint64_t coll_inc, hipStream_t s) {
+    hipLaunchKernelGGL(k_epoch_begin, dim3((nsys + 255) / 256), dim3(256), 0, s, d_sys, nsys,
+                       ev_inc, coll_inc);
+    return hipGetLastError();
+}
+
+hipError_t edmd_launch_epoch_end(const EdmdSys* d_sys, int nsys, EdmdEpochRow* rows,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL(k_epoch_end, dim3((nsys + 255) / 256), dim3(256), 0, s, d_sys, nsys, rows);
+    return hipGetLastError();
 }
 
 hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, int grow, EdmdRunArgs ra,

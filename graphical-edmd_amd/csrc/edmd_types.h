@@ -111,7 +111,15 @@ typedef struct EdmdState {
     int64_t rank_screen, rank_thermo, rank_noise, rank_grow; /* ranks of the special events */
     int64_t win;                /* buckets passed (paulTime = win additions of dtPaul) */
     int64_t win_screen, win_thermo, win_noise, win_grow; /* their buckets, -1 = overflow list */
+    uint64_t coll_limit;        /* per-system bound on ncol (epochs) */
 } EdmdState;
+
+/* Per-system counters at the end of one epoch of a graph-captured run. */
+typedef struct EdmdEpochRow {
+    double t;
+    uint64_t ncol, ncross, nevents;
+    int32_t status, stop_reason;
+} EdmdEpochRow;
 
 /* One valid collision as the derivative kernels need it: time, contact
  * vector a = r_j - r_i (minimum image), relative velocity c = v_j - v_i

@@ -59,6 +59,9 @@ TRACE_DT = np.dtype([("type", "<i4"), ("i", "<i4"), ("j", "<i4"), ("valid", "<i4
 THERMO_DT = np.dtype([("t", "<f8"), ("ncol", "<u8"), ("e_per_n", "<f8"), ("pressure", "<f8"),
                       ("pxx", "<f8"), ("pyy", "<f8"), ("pxy", "<f8"), ("pyx", "<f8")])
 
+EPOCH_DT = np.dtype([("t", "<f8"), ("ncol", "<u8"), ("ncross", "<u8"), ("nevents", "<u8"),
+                     ("status", "<i4"), ("stop_reason", "<i4")])
+
 TAPE_DT = np.dtype([("t", "<f8"), ("ax", "<f8"), ("ay", "<f8"), ("cx", "<f8"),
                     ("cy", "<f8"), ("ff", "<f8"), ("i", "<i4"), ("j", "<i4"),
                     ("pad", "<i4", (2,))])
@@ -101,6 +104,7 @@ def lib():
             "edmd_batch_destroy": (i32, [vp]),
             "edmd_batch_run": (i32, [vp, i64, u64, dbl]),
             "edmd_batch_sweep_predict": (i32, [vp]),
+            "edmd_batch_run_epochs": (i32, [vp, i32, i64, u64, dbl, vp]),
             "edmd_batch_get_observables": (i32, [vp, i32, vp]),
             "edmd_batch_get_state": (i32, [vp, i32] + [vp] * 8),
             "edmd_batch_get_events": (i32, [vp, i32] + [vp] * 4),
@@ -336,6 +340,16 @@ class Batch:
 
     def sweep_predict(self):
         check(lib().edmd_batch_sweep_predict(self.h))
+
+    def run_epochs(self, nepochs, events_per_epoch=INT64_MAX, collisions_per_epoch=UINT64_MAX,
+                   tmax=1e300):
+        """nepochs epochs in one hipGraph launch; returns an (nepochs, nsys)
+        array of EPOCH_DT with every system's counters after each epoch."""
+        rows = np.zeros((int(nepochs), len(self.ns)), dtype=EPOCH_DT)
+        check(lib().edmd_batch_run_epochs(self.h, int(nepochs), int(events_per_epoch),
+                                          int(collisions_per_epoch), float(tmax),
+                                          rows.ctypes.data))
+        return rows
 
     def radii(self, s):
         out = np.zeros(self.ns[s])

@@ -176,6 +176,19 @@ int edmd_batch_destroy(edmd_batch* b);
  * collision counter reaches max_collisions, or t > tmax. */
 int edmd_batch_run(edmd_batch* b, int64_t max_events, uint64_t max_collisions, double tmax);
 int edmd_batch_sweep_predict(edmd_batch* b);
+/* Epochs (the reference has none: its loop is one process per system).
+ * nepochs epochs in one hipGraph launch: each epoch lets every system handle
+ * up to events_per_epoch more events and collisions_per_epoch more valid
+ * collisions (while t <= tmax), then records its counters in
+ * rows[epoch * nsys + sys] (may be NULL).  A GROWSTOP inside an epoch
+ * resets the system's collision counter, as stopGrow does. */
+typedef struct edmd_epoch_row {
+    double t;
+    uint64_t ncol, ncross, nevents;
+    int32_t status, stop_reason;
+} edmd_epoch_row;
+int edmd_batch_run_epochs(edmd_batch* b, int nepochs, int64_t events_per_epoch,
+                          uint64_t collisions_per_epoch, double tmax, edmd_epoch_row* rows);
 int edmd_batch_get_observables(edmd_batch* b, int sys, edmd_observables* out);
 int edmd_batch_get_state(edmd_batch* b, int sys, double* x, double* y, double* vx, double* vy,
                          double* t_local, uint32_t* coll, int32_t* cx, int32_t* cy);
