@@ -7,14 +7,23 @@
  *                  collision partner over the 3x3 cell neighbourhood
  *                  (EDMD.c:2405-2482, 2661-2723, 2829-3003; invoked for all
  *                  particles at EDMD.c:2007-2012 and 4909-4915)
+ *   k_sweep_grow   the same for systems whose radii still grow
+ *                  (collisionEventGrow, EDMD.c:3104-3230)
+ *   k_sweep_count  counts a full sweep's insertions (the rank counter)
  *   k_tree_level   builds one level of the 64-ary minimum calendar
- *   k_run          persistent event loop: one wavefront per system pops the
+ *   k_run_small, k_run
+ *                  persistent event loop: one wavefront per system pops the
  *                  earliest event, applies the reference's handlers
  *                  (EDMD.c:534-610, 2501-2590, 3542-3915, 4657-4713,
- *                  4828-4923) and re-predicts with all 64 lanes
+ *                  4740-4800, 4828-4923) and re-predicts with all 64 lanes;
+ *                  instantiated for the calendar depth and for growing
+ *                  systems (the Grow dispatch of EDMD.c:1974-1990)
+ *   k_epoch_begin, k_epoch_end
+ *                  per-system bounds and counters of graph-captured epochs
  *   k_peek, k_pair_time, k_repredict, k_fly_one
  *                  single steps for the reference-compatible entry points
  *   k_sync_all     free flight of every particle to the system time
+ *   k_pcf          g(r) (pcf.c:16-75)
  *   k_fp64_check   IEEE division / square root probe for the tests
  *
  * Arithmetic: FP64 throughout, compiled with -ffp-contract=off so that every

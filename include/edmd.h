@@ -116,7 +116,7 @@ int edmd_ctx_destroy(edmd_ctx* ctx);
  * LAMMPS dump as the initial state, the other fields from params (NULL =
  * edmd_default_desc). */
 int edmd_ctx_create_from_dump(const char* path, const edmd_system_desc* params, edmd_ctx** out);
-/* Full prediction sweep at the current time (kernel k_sweep): free-flies
+/* Full prediction sweep at the current time (kernel k_sweep_tiled): free-flies
  * every particle to the current time first, then re-predicts every
  * crossing and collision slot and rebuilds the calendar (the pattern of
  * EDMD.c:2007-2012 and 4909-4915). */
