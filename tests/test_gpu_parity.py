@@ -246,6 +246,33 @@ def test_full_size_sweep_and_run(gpu, oracle):
     assert abs(og["px"]) < 1e-9 and abs(og["py"]) < 1e-9
 
 
+def test_largest_calendar_depth(gpu, oracle):
+    """2^24 + 4096 disks (phi = 0.70): the calendar has four levels above the
+    leaves (16781312 -> 262208 -> 4097 -> 65 -> 2), the deepest the engine
+    meets below 2^30 disks; the sweep and 20000 events match the oracle bit
+    for bit, and so does every disk's state."""
+    n = (1 << 24) + 4096
+    st = gpu.init_lattice(n, 0.70, rad=0.5, lattice=2, seed=77)
+    sizes = [n]
+    while sizes[-1] > 64:
+        sizes.append((sizes[-1] + 63) // 64)
+    assert len(sizes) - 1 == 4
+    s = gpu.System(st, trace_capacity=20000)
+    o = oracle.Oracle(st)
+    eg, eo = s.events(), o.events()
+    for k in ("t_cross", "t_coll"):
+        np.testing.assert_array_equal(bits(eg[k]), bits(eo[k]), err_msg=k)
+    np.testing.assert_array_equal(eg["partner"], eo["partner"])
+    del eg, eo
+    s.run(max_events=20000)
+    code, tr = o.run(20000)
+    assert code == 0
+    assert_trace_equal(s.trace(), tr)
+    sg, so = s.state(), o.state()
+    for k in ("x", "y", "vx", "vy", "t"):
+        np.testing.assert_array_equal(bits(sg[k]), bits(so[k]), err_msg=k)
+
+
 @pytest.mark.parametrize("rad", [0.5, 0.2])
 def test_tiled_sweep_matches_oracle_after_events(gpu, oracle, rad):
     """The cell-tiled sweep from a state whose disks carry different local
