@@ -113,11 +113,22 @@ struct edmd_batch {
     };
     std::vector<Ckpt> ckpts;
     void* ckpt_end = nullptr;
+    /* residency of the event loop's arrays: requested mode (EDMD_RES_*),
+     * whether every system fits one workgroup's LDS, the dynamic LDS per
+     * workgroup of the Normal [0] and growing [1] launches, the workgroups
+     * the device holds at once in LDS mode, and the mode in use */
+    int residency = EDMD_RES_AUTO;
+    bool lds_fit = false;
+    size_t lds_dyn[2] = {0, 0};
+    int64_t lds_slots = 0;
+    bool use_lds = false;
 };
 
 struct edmd_ctx {
     edmd_batch* b;
 };
+
+static int choose_residency(edmd_batch* b);
 
 template <class T>
 static int dalloc(edmd_batch* b, T** p, size_t n) {
@@ -451,6 +462,7 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
     }
     rc = batch_predict_all(b);
     if (!rc) rc = check_status(b);
+    if (!rc) rc = choose_residency(b);
     if (rc) { batch_free(b); return rc; }
     *out = b;
     return EDMD_OK;
@@ -478,6 +490,82 @@ extern "C" int edmd_batch_set_stream(edmd_batch* b, void* stream) {
     return EDMD_OK;
 }
 
+/* Decides where the event loop keeps each system's arrays.  LDS residency
+ * needs a calendar of depth <= 2 and every system's arrays (edmd_lds_layout)
+ * plus the loop's static LDS within one workgroup's LDS; in auto mode it is
+ * used when, in addition, the whole batch is resident on the device at once
+ * (one wavefront per CU-slot, no second round of workgroups). */
+static int choose_residency(edmd_batch* b) {
+    b->lds_fit = false;
+    b->lds_slots = 0;
+    b->use_lds = false;
+    b->lds_dyn[0] = b->lds_dyn[1] = 0;
+    for (int s = 0; s < b->nsys; s++)
+        for (int g = 0; g < 2; g++)
+            b->lds_dyn[g] = std::max(b->lds_dyn[g], (size_t)edmd_lds_layout(&b->hs[s].c, g).total);
+    if (b->max_nlev <= 2) {
+        int dev = 0, maxblk = 0, ncu = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&maxblk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        bool fit = true;
+        int64_t slots = INT64_MAX;
+        for (int g = 0; g < (b->any_grow ? 2 : 1) && fit; g++) {
+            int per_cu = 0;
+            size_t stat = 0;
+            hipError_t e = edmd_lds_occupancy(b->max_nlev, g, b->lds_dyn[g], &per_cu, &stat);
+            if (e != hipSuccess || per_cu < 1 || stat + b->lds_dyn[g] > (size_t)maxblk) {
+                (void)hipGetLastError();
+                fit = false;
+                break;
+            }
+            slots = std::min(slots, (int64_t)per_cu * ncu);
+        }
+        if (fit) {
+            b->lds_fit = true;
+            b->lds_slots = slots;
+        }
+    }
+    if (b->residency == EDMD_RES_LDS && !b->lds_fit)
+        return fail(EDMD_ERR_ARG,
+                    "LDS residency needs calendars of depth <= 2 and %zu bytes of LDS per workgroup",
+                    b->lds_dyn[1] > b->lds_dyn[0] ? b->lds_dyn[1] : b->lds_dyn[0]);
+    b->use_lds = b->residency == EDMD_RES_LDS ||
+                 (b->residency == EDMD_RES_AUTO && b->lds_fit && b->nsys <= b->lds_slots);
+    return EDMD_OK;
+}
+
+/* One launch of the event loop over every system (grow: the growing ones). */
+static hipError_t launch_loop(edmd_batch* b, int grow, const EdmdRunArgs& ra) {
+    if (b->use_lds)
+        return edmd_launch_run_lds(b->d_sys, b->nsys, b->max_nlev, grow, b->lds_dyn[grow ? 1 : 0],
+                                   ra, b->stream);
+    return edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, grow, ra, b->stream);
+}
+
+extern "C" int edmd_batch_set_residency(edmd_batch* b, int mode) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    if (mode != EDMD_RES_AUTO && mode != EDMD_RES_HBM && mode != EDMD_RES_LDS)
+        return fail(EDMD_ERR_ARG, "unknown residency mode %d", mode);
+    const int old = b->residency;
+    b->residency = mode;
+    int rc = choose_residency(b);
+    if (rc) {
+        b->residency = old;
+        (void)choose_residency(b);
+    }
+    return rc;
+}
+
+extern "C" int edmd_batch_get_residency(edmd_batch* b, int* in_lds, int64_t* lds_bytes,
+                                        int64_t* lds_slots) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    if (in_lds) *in_lds = b->use_lds ? 1 : 0;
+    if (lds_bytes) *lds_bytes = (int64_t)(b->any_grow ? b->lds_dyn[1] : b->lds_dyn[0]);
+    if (lds_slots) *lds_slots = b->lds_fit ? b->lds_slots : 0;
+    return EDMD_OK;
+}
+
 static int batch_run_mask(edmd_batch* b, int64_t max_events, uint64_t max_collisions, double tmax,
                           int stop_mask) {
     if (!b) return fail(EDMD_ERR_ARG, "null batch");
@@ -490,7 +578,7 @@ static int batch_run_mask(edmd_batch* b, int64_t max_events, uint64_t max_collis
     if (b->any_grow) {
         /* growing systems first: they run until GROWSTOP (then continue in
          * the Normal launch below) or until the bounds */
-        HIPCHK(edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, 1, ra, b->stream));
+        HIPCHK(launch_loop(b, 1, ra));
         HIPCHK(hipStreamSynchronize(b->stream));
         int rc = check_status(b);
         if (rc) return rc;
@@ -500,7 +588,7 @@ static int batch_run_mask(edmd_batch* b, int64_t max_events, uint64_t max_collis
         for (int s = 0; s < b->nsys; s++) still |= st[s].growing != 0;
         b->any_grow = still;
     }
-    HIPCHK(edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, 0, ra, b->stream));
+    HIPCHK(launch_loop(b, 0, ra));
     HIPCHK(hipStreamSynchronize(b->stream));
     return check_status(b);
 }
@@ -537,8 +625,8 @@ extern "C" int edmd_batch_run_epochs(edmd_batch* b, int nepochs, int64_t events_
         e = edmd_launch_epoch_begin(b->d_sys, nsys, events_per_epoch, collisions_per_epoch,
                                     b->stream);
         if (e == hipSuccess && b->any_grow)
-            e = edmd_launch_run(b->d_sys, nsys, b->max_nlev, 1, ra, b->stream);
-        if (e == hipSuccess) e = edmd_launch_run(b->d_sys, nsys, b->max_nlev, 0, ra, b->stream);
+            e = launch_loop(b, 1, ra);
+        if (e == hipSuccess) e = launch_loop(b, 0, ra);
         if (e == hipSuccess)
             e = edmd_launch_epoch_end(b->d_sys, nsys, d_rows + (size_t)k * nsys, b->stream);
     }
@@ -1364,7 +1452,7 @@ static int tape_overflow_check(edmd_batch* b) {
 /* Runs one checkpoint segment: at most `seg` events per system, new tape. */
 static int run_segment(edmd_batch* b, const EdmdRunArgs& ra) {
     HIPCHK(edmd_launch_seg_begin(b->d_sys, b->nsys, b->seg, 1, b->stream));
-    HIPCHK(edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, 0, ra, b->stream));
+    HIPCHK(launch_loop(b, 0, ra));
     HIPCHK(hipStreamSynchronize(b->stream));
     return check_status(b);
 }

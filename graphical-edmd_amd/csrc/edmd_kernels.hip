@@ -331,8 +331,8 @@ __device__ __forceinline__ bool before(double ka, int64_t ra, double kb, int64_t
  * in the reference's order, y offset outer, x offset inner, newest list
  * entry first (EDMD.c:2959-3003).  Returns the relative time; NEVER and
  * partner 0 when no candidate qualifies (EDMD.c:2830-2831). */
-template <bool G>
-__device__ __forceinline__ double predict_seq(const EdmdConst& C, const EdmdArrays& A, double t,
+template <bool G, class AR>
+__device__ __forceinline__ double predict_seq(const EdmdConst& C, const AR& A, double t,
                                               int p, const EdmdPart& o, int& partner,
                                               bool& overlap) {
     /* collisionEventGrow (EDMD.c:3104-3230, no walls) starts from 1e7 */
@@ -344,7 +344,7 @@ __device__ __forceinline__ double predict_seq(const EdmdConst& C, const EdmdArra
         for (int k = -1; k <= 1; ++k) {
             int c = row + wrap_cell(o.cx + k, C.ncx);
             int cnt = A.ccnt[c];
-            const int32_t* sl = A.cslot + (size_t)c * C.cap;
+            const auto sl = A.cslot + (size_t)c * C.cap;
             for (int s = cnt - 1; s >= 0; --s) {
                 int q = sl[s];
                 if (q == p) continue;
@@ -384,8 +384,8 @@ struct Owner {
  * (time, traversal rank) among candidates with time < NEVER is the
  * reference's strict-less first minimum; no candidate gives NEVER and
  * partner 0 (EDMD.c:2830-2831, 3091-3093).  Results are uniform. */
-template <bool G>
-__device__ __forceinline__ void predict_owners_gather(const EdmdConst& C, const EdmdArrays& A,
+template <bool G, class AR>
+__device__ __forceinline__ void predict_owners_gather(const EdmdConst& C, const AR& A,
                                                       int lane, double t, int nown,
                                                       const Owner& a, const Owner& b, double& dta,
                                                       int& pa, uint32_t& ca, double& dtb, int& pb,
@@ -527,8 +527,8 @@ __device__ __forceinline__ int scan32(int v) {
  * passes only a strictly earlier time: the reference's strict-less first
  * minimum.  Requires cap <= EDMD_STAGE_CAP, cap % 4 == 0 and cell rows
  * aligned to 16 bytes. */
-template <bool G>
-__device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const EdmdArrays& A,
+template <bool G, class AR>
+__device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const AR& A,
                                                       int lane, double t, int nown,
                                                       const Owner& a, const Owner& b, double& dta,
                                                       int& pa, uint32_t& ca, double& dtb, int& pb,
@@ -545,7 +545,7 @@ __device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const 
         const int ocy = inB ? b.cy : a.cy;
         const int cell =
             wrap_cell(ocy + sub / 3 - 1, C.ncy) * C.ncx + wrap_cell(ocx + sub % 3 - 1, C.ncx);
-        const int4 EDMD_GLOBAL* src = (const int4 EDMD_GLOBAL*)(A.cslot + (int64_t)cell * C.cap);
+        const int4* src = (const int4*)(A.cslot + (int64_t)cell * C.cap);
         cnt = A.ccnt[cell];
 #pragma unroll
         for (int c = 0; c < EDMD_STAGE_CAP / 4; ++c)
@@ -634,8 +634,8 @@ __device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const 
 
 /* The staged scan where the cell capacity allows it, else the gather scan
  * from HBM; both give the reference's result bit for bit. */
-template <bool G>
-__device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArrays& A, int lane,
+template <bool G, class AR>
+__device__ __forceinline__ void predict_owners(const EdmdConst& C, const AR& A, int lane,
                                                double t, int nown, const Owner& a, const Owner& b,
                                                double& dta, int& pa, uint32_t& ca, double& dtb,
                                                int& pb, uint32_t& cb, bool& overlap,
@@ -654,7 +654,8 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const EdmdArr
  * only when several lanes hold the minimum key, which is rare.  Keys are
  * never NaN; among +inf keys (padding, parked events) the lowest lane wins.
  * base: a time near the keys (the current time), for the 32-bit search. */
-__device__ __forceinline__ void wave_argmin(const EdmdArrays& A, double base, double& k, int& i,
+template <class AR>
+__device__ __forceinline__ void wave_argmin(const AR& A, double base, double& k, int& i,
                                             int& x) {
     const int kk = order_key(k, base);
     const int mkk = wave_min_i32(kk);
@@ -695,13 +696,15 @@ __device__ __forceinline__ void wave_argmin(const EdmdArrays& A, double base, do
 }
 
 /* Level l of the calendar: the leaves for l = 0, else nodes of level l. */
-__device__ __forceinline__ EdmdNode* level(const EdmdArrays& A, int l) {
+template <class AR>
+__device__ __forceinline__ auto level(const AR& A, int l) -> decltype(A.leaf) {
     return l == 0 ? A.leaf : A.lev[l - 1];
 }
 
 /* Earliest pending particle event: minimum of the top level, read from
  * `top` (the LDS copy the event loop keeps, or the level in HBM). */
-__device__ __forceinline__ void top_min(const EdmdConst& C, const EdmdArrays& A, int lane,
+template <class AR>
+__device__ __forceinline__ void top_min(const EdmdConst& C, const AR& A, int lane,
                                         const EdmdNode* top, double base, double& key, int& leaf,
                                         int& aux) {
     double k = INF_D;
@@ -728,8 +731,8 @@ struct PathCache {
     int x[2][ML];
 };
 
-template <int ML>
-__device__ __forceinline__ void path_prefetch(const EdmdConst& C, const EdmdArrays& A, int lane,
+template <int ML, class AR>
+__device__ __forceinline__ void path_prefetch(const EdmdConst& C, const AR& A, int lane,
                                               int pA, int pB, PathCache<ML>& P) {
 #pragma unroll
     for (int l = 0; l < ML; ++l) {
@@ -759,8 +762,8 @@ __device__ __forceinline__ void path_prefetch(const EdmdConst& C, const EdmdArra
  * freshly computed path nodes; where the paths meet one node takes both.
  * With `top` (the event loop's LDS copy of the top level) the top-level
  * nodes are written there instead of to HBM. */
-template <int ML>
-__device__ __forceinline__ void path_update(const EdmdConst& C, const EdmdArrays& A, int lane,
+template <int ML, class AR>
+__device__ __forceinline__ void path_update(const EdmdConst& C, const AR& A, int lane,
                                             const PathCache<ML>& P, int pA, Leaf LA, int pB,
                                             Leaf LB, EdmdNode* top, double base) {
     double kA = LA.key, kB = LB.key;
@@ -842,11 +845,12 @@ __device__ __forceinline__ void path_update(const EdmdConst& C, const EdmdArrays
  * child's rank is loaded unconditionally and the winner chosen by selects:
  * a rank load under a divergent branch here has been compiled into code
  * that kept the previous winner's event code on a tie (gfx950, ROCm 7.2). */
-__device__ __forceinline__ void node_min(const EdmdConst& C, const EdmdArrays& A, int l, int g) {
+template <class AR>
+__device__ __forceinline__ void node_min(const EdmdConst& C, const AR& A, int l, int g) {
     int lo = g << 6;
     int hi = lo + 64;
     if (hi > C.lsize[l - 1]) hi = C.lsize[l - 1];
-    const EdmdNode* ch = level(A, l - 1);
+    const auto ch = level(A, l - 1);
     const EdmdNode b0 = ch[lo];
     double bk = b0.key;
     int bi = b0.idx, bx = b0.aux;
@@ -877,8 +881,8 @@ __device__ __forceinline__ void node_min(const EdmdConst& C, const EdmdArrays& A
  * (EDMD.c:4770-4776: every counter goes 0 -> 1 in particle order, each disk
  * predicting right after its own increment), i.e. 1 for a partner of lower
  * or equal index, else 0. */
-template <bool G>
-__device__ __forceinline__ bool sweep_one(const EdmdConst& C, const EdmdArrays& A, double t, int p,
+template <bool G, class AR>
+__device__ __forceinline__ bool sweep_one(const EdmdConst& C, const AR& A, double t, int p,
                                           double paul, int64_t ctr_cross, int64_t ctr_coll,
                                           bool stop_grow = false) {
     const EdmdPart o = A.p[p];
@@ -977,8 +981,8 @@ __device__ __forceinline__ int sweep_scan(int lane, const int* v, int* out, int 
 /* The predictions of a staged tile's own disks: one thread per disk,
  * candidates from LDS in the reference's scan order (y offset outer, x
  * offset inner, newest entry first) with the strict-less rule. */
-template <bool WRAP>
-__device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const EdmdArrays& A,
+template <bool WRAP, class AR>
+__device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR& A,
                                                    const SweepLds& L, int tid, int ninner, double t,
                                                    int tx0, int ty0, double paul, int64_t ctr0,
                                                    bool& ov) {
@@ -1223,13 +1227,15 @@ struct QueueState {
 };
 
 /* Top level of the calendar between HBM and the LDS copy. */
-__device__ __forceinline__ void top_load(const EdmdConst& C, const EdmdArrays& A, int lane,
+template <class AR>
+__device__ __forceinline__ void top_load(const EdmdConst& C, const AR& A, int lane,
                                          EdmdNode* top) {
     if (lane < C.lsize[C.nlev]) top[lane] = level(A, C.nlev)[lane];
     WAVE_SYNC();
 }
 
-__device__ __forceinline__ void top_store(const EdmdConst& C, const EdmdArrays& A, int lane,
+template <class AR>
+__device__ __forceinline__ void top_store(const EdmdConst& C, const AR& A, int lane,
                                           const EdmdNode* top) {
     if (lane < C.lsize[C.nlev]) level(A, C.nlev)[lane] = top[lane];
     WAVE_SYNC();
@@ -1319,7 +1325,8 @@ __device__ __forceinline__ bool step_window(const EdmdConst& C, EdmdState* st, L
  * (time, rank) is next.  Returns true for the special event.  A queue whose
  * next event lies more than 2^24 buckets ahead is not walked bucket by
  * bucket (the reference would); paulTime then jumps to that event. */
-__device__ __forceinline__ bool pick_next(const EdmdConst& C, const EdmdArrays& A, EdmdState* st,
+template <class AR>
+__device__ __forceinline__ bool pick_next(const EdmdConst& C, const AR& A, EdmdState* st,
                                           LoopRegs& R, double key, int leaf) {
     /* the particle event's rank matters only against a special at its time */
     auto krank = [&]() { return key == R.next_special ? uni64(A.lrank[leaf]) : (int64_t)0; };
@@ -1349,7 +1356,8 @@ __device__ __forceinline__ int64_t take_rank(const EdmdConst& C, LoopRegs& R, do
     return r;
 }
 
-__device__ __forceinline__ void trace_push(const EdmdConst& C, const EdmdArrays& A, int lane,
+template <class AR>
+__device__ __forceinline__ void trace_push(const EdmdConst& C, const AR& A, int lane,
                                            double t, int type, int i, int j, int valid) {
     if (C.trace_cap <= 0) return;
     if (lane == 0) {
@@ -1384,12 +1392,12 @@ __device__ __forceinline__ double ordered_sum(int n, int lane, F f) {
     return acc;
 }
 
-template <bool G>
-__device__ void ev_screenshot(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R) {
+template <bool G, class AR>
+__device__ void ev_screenshot(const EdmdConst& C, const AR& A, int lane, LoopRegs& R) {
     /* takeAScreenshot, EDMD.c:4657-4677: free-fly every particle */
     const double t = R.t;
     for (int q = lane; q < C.n; q += 64) {
-        EdmdPart* r = A.p + q;
+        const auto r = A.p + q;
         double x = r->x, y = r->y, tl = r->tl;
         if (G) {
             double rad = r->rad;
@@ -1413,13 +1421,14 @@ __device__ void ev_screenshot(const EdmdConst& C, const EdmdArrays& A, int lane,
     WAVE_SYNC();
 }
 
-__device__ void ev_thermo(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R) {
+template <class AR>
+__device__ void ev_thermo(const EdmdConst& C, const AR& A, int lane, LoopRegs& R) {
     /* takeAThermo + saveThermo, EDMD.c:4679-4713, 5232-5333 */
     const double t = R.t;
     const uint64_t ncol = R.ncol;
     const int64_t wk = window_of(t + C.dtime_thermo, R.q->paul, C.dtp, C.nlists, R.q->win);
     const int64_t rk = take_rank(C, R, t + C.dtime_thermo);
-    const EdmdPart* P = A.p;
+    const auto P = A.p;
     double E = ordered_sum(C.n, lane, [&](int q) {
         double a = P[q].vx;
         return 0.5 * P[q].m * (a * a + P[q].vy * P[q].vy);
@@ -1457,18 +1466,19 @@ __device__ void ev_thermo(const EdmdConst& C, const EdmdArrays& A, int lane, Loo
 }
 
 /* Rebuilds every level of the calendar after a full re-prediction. */
-__device__ void rebuild_tree(const EdmdConst& C, const EdmdArrays& A, int lane) {
+template <class AR>
+__device__ void rebuild_tree(const EdmdConst& C, const AR& A, int lane) {
     for (int l = 1; l <= C.nlev; ++l) {
         for (int g = lane; g < C.lsize[l]; g += 64) node_min(C, A, l, g);
         WAVE_SYNC();
     }
 }
 
-template <bool G>
-__device__ void ev_rescale(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R) {
+template <bool G, class AR>
+__device__ void ev_rescale(const EdmdConst& C, const AR& A, int lane, LoopRegs& R) {
     /* addNoise, velocity-rescale branch (noise == 2): EDMD.c:4828-4832,
      * 4880-4915, 4922 */
-    const EdmdPart* P = A.p;
+    const auto P = A.p;
     const double t = R.t;
     double E = ordered_sum(C.n, lane, [&](int q) {
         double a = P[q].vx;
@@ -1510,8 +1520,8 @@ __device__ __forceinline__ void flag_overlap(LoopRegs& R, bool ov, int p) {
     }
 }
 
-template <int ML>
-__device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R, int i,
+template <int ML, class AR>
+__device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs& R, int i,
                              int j) {
     /* doTheCollisionNormal, EDMD.c:3542-3562, 3583-3606, 3802-3827, 3869-3879 */
     const EdmdEv ei = A.e[i];
@@ -1588,7 +1598,7 @@ __device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, 
         st->vir[1] += w * dy * dy;
         st->vir[2] += w * dx * dy;
         st->vir[3] += w * dy * dx;
-        EdmdPart* q = A.p + i;
+        auto q = A.p + i;
         q->x = xi; q->y = yi; q->vx = vxi; q->vy = vyi; q->tl = t; q->coll = colli + 1;
         q = A.p + j;
         q->x = xj; q->y = yj; q->vx = vxj; q->vy = vyj; q->tl = t; q->coll = collj + 1;
@@ -1640,8 +1650,8 @@ __device__ void ev_collision(const EdmdConst& C, const EdmdArrays& A, int lane, 
                     make_leaf(tkj, rkj, tcj2, rxj, pj_, dj), R.q->top, R.t);
 }
 
-template <int ML>
-__device__ void ev_collision_grow(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R,
+template <int ML, class AR>
+__device__ void ev_collision_grow(const EdmdConst& C, const AR& A, int lane, LoopRegs& R,
                                   int i, int j) {
     /* doTheCollisionGrow, EDMD.c:3465-3540 (2D): both disks fly with their
      * radii growing; the unit contact normal and the summed growth speed
@@ -1725,7 +1735,7 @@ __device__ void ev_collision_grow(const EdmdConst& C, const EdmdArrays& A, int l
     yj = uni(yj);
     rj = uni(rj);
     if (lane == 0) {
-        EdmdPart* q = A.p + i;
+        auto q = A.p + i;
         q->x = xi; q->y = yi; q->vx = vxi; q->vy = vyi; q->tl = t; q->rad = ri; q->coll = colli + 1;
         q = A.p + j;
         q->x = xj; q->y = yj; q->vx = vxj; q->vy = vyj; q->tl = t; q->rad = rj; q->coll = collj + 1;
@@ -1770,7 +1780,8 @@ __device__ void ev_collision_grow(const EdmdConst& C, const EdmdArrays& A, int l
  * remove the mean momentum, scale to E/N = Einit), then the Normal
  * prediction of every disk with stopGrow's partner counters, and a new
  * calendar.  The caller continues in the Normal loop. */
-__device__ void ev_stop_grow(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R) {
+template <class AR>
+__device__ void ev_stop_grow(const EdmdConst& C, const AR& A, int lane, LoopRegs& R) {
     const double t = R.t;
     for (int q = lane; q < C.n; q += 64) {
         EdmdPart r = A.p[q];
@@ -1781,7 +1792,7 @@ __device__ void ev_stop_grow(const EdmdConst& C, const EdmdArrays& A, int lane, 
         A.img[2 * q + 1] = 0;
     }
     WAVE_SYNC();
-    EdmdPart* P = A.p;
+    const auto P = A.p;
     const int n = C.n;
     const double px = ordered_sum(n, lane, [&](int q) { return P[q].vx * P[q].m; });
     const double py = ordered_sum(n, lane, [&](int q) { return P[q].vy * P[q].m; });
@@ -1818,8 +1829,8 @@ __device__ void ev_stop_grow(const EdmdConst& C, const EdmdArrays& A, int lane, 
     WAVE_SYNC();
 }
 
-template <int ML, bool G>
-__device__ void ev_crossing(const EdmdConst& C, const EdmdArrays& A, int lane, LoopRegs& R, int p,
+template <int ML, bool G, class AR>
+__device__ void ev_crossing(const EdmdConst& C, const AR& A, int lane, LoopRegs& R, int p,
                             int d) {
     /* doTheCrossing, EDMD.c:2501-2590 (its freeFly is the dispatch pointer:
      * freeFlyGrow while growing) */
@@ -1880,7 +1891,7 @@ __device__ void ev_crossing(const EdmdConst& C, const EdmdArrays& A, int lane, L
     if (lane == 0) {
         A.cslot[(size_t)c1 * C.cap + n1] = p; /* newest entry: visited first */
         A.ccnt[c1] = n1 + 1;
-        EdmdPart* q = A.p + p;
+        const auto q = A.p + p;
         q->x = x;
         q->y = y;
         q->tl = tl;
@@ -1915,16 +1926,14 @@ __device__ void ev_crossing(const EdmdConst& C, const EdmdArrays& A, int lane, L
     path_update<ML>(C, A, lane, P, p, make_leaf(tk, rk, tc, rx, pa, dir), -1, Leaf{}, R.q->top, R.t);
 }
 
-/* Persistent event loop, one wavefront per system (blockIdx.x); ML bounds
- * the calendar depth of every system in the launch. */
-template <int ML, bool G>
-__device__ __forceinline__ void run_loop(const EdmdSys* __restrict__ systems, EdmdRunArgs ra) {
-    const EdmdConst& C = systems[blockIdx.x].c;
-    const EdmdArrays& A = systems[blockIdx.x].a;
+/* Persistent event loop of one system on one wavefront; ML bounds the
+ * calendar depth of every system in the launch.  A holds the system's arrays
+ * in HBM (EdmdArrays) or in LDS (EdmdArraysL); the caller has checked that
+ * the system's growth phase matches G. */
+template <int ML, bool G, class AR>
+__device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRunArgs ra) {
     const int lane = threadIdx.x;
     EdmdState* st = A.st;
-    /* G launches run the systems that are growing, the others the rest */
-    if ((uni(st->growing) != 0) != G) return;
     __shared__ QueueState Qs;
     LoopRegs R;
     R.q = &Qs;
@@ -2017,14 +2026,146 @@ __device__ __forceinline__ void run_loop(const EdmdSys* __restrict__ systems, Ed
 template <int ML, bool G>
 __global__ __launch_bounds__(64, 4) void k_run_small(const EdmdSys* __restrict__ systems,
                                                      EdmdRunArgs ra) {
-    run_loop<ML, G>(systems, ra);
+    const EdmdSys& S = systems[blockIdx.x];
+    /* G launches run the systems that are growing, the others the rest */
+    if ((uni(S.a.st->growing) != 0) != G) return;
+    run_loop<ML, G>(S.c, S.a, ra);
 }
 
 /* Deeper calendars (large systems, typically one per launch): no register
  * bound, so the path caches stay in registers. */
 template <int ML, bool G>
 __global__ __launch_bounds__(64) void k_run(const EdmdSys* __restrict__ systems, EdmdRunArgs ra) {
-    run_loop<ML, G>(systems, ra);
+    const EdmdSys& S = systems[blockIdx.x];
+    if ((uni(S.a.st->growing) != 0) != G) return;
+    run_loop<ML, G>(S.c, S.a, ra);
+}
+
+/* ----------------------------------------------------- LDS-resident systems */
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define EDMD_LDS __attribute__((address_space(3)))
+#else
+#define EDMD_LDS
+#endif
+
+/* The arrays of a system whose event loop runs out of LDS.  Everything the
+ * loop reads and writes on each event (particles, event slots, calendar
+ * nodes, ranks, image counters, cell lists, growth rates) sits in the
+ * workgroup's LDS for the whole launch; the per-system record and the
+ * output buffers (trace, thermo rows, tape, derivative state) stay in HBM.
+ * Fields mean what the fields of EdmdArrays of the same name mean. */
+struct EdmdArraysL {
+    EdmdPart EDMD_LDS* p;
+    EdmdEv EDMD_LDS* e;
+    EdmdNode EDMD_LDS* leaf;
+    EdmdNode EDMD_LDS* lev[EDMD_MAX_LEVELS];
+    int32_t EDMD_LDS* img;
+    int32_t EDMD_LDS* ccnt;
+    int32_t EDMD_LDS* cslot;
+    EdmdState EDMD_GLOBAL* st;
+    EdmdTraceRec EDMD_GLOBAL* trace;
+    EdmdThermoRow EDMD_GLOBAL* thermo;
+    double EDMD_LDS* vr;
+    int64_t EDMD_LDS* erank;
+    int64_t EDMD_LDS* lrank;
+    EdmdTapeRec EDMD_GLOBAL* tape;
+    double EDMD_GLOBAL* dv;
+    double EDMD_GLOBAL* dtl;
+    double EDMD_GLOBAL* otl;
+    EdmdDeriv EDMD_GLOBAL* dr;
+};
+
+/* Copies one array between HBM and LDS with the whole wavefront, in 16-,
+ * 8- or 4-byte words as the HBM address and the length allow (LDS offsets
+ * are 16-byte aligned; every length is a multiple of 4). */
+template <bool IN>
+__device__ __forceinline__ void lds_xfer(unsigned char EDMD_LDS* l, const void EDMD_GLOBAL* gv,
+                                         uint32_t bytes, int lane) {
+    unsigned char EDMD_GLOBAL* g = (unsigned char EDMD_GLOBAL*)gv;
+    const uint32_t al = uni((int)((uint32_t)(uintptr_t)g | bytes));
+    if ((al & 15) == 0) {
+        int4 EDMD_LDS* L = (int4 EDMD_LDS*)l;
+        int4 EDMD_GLOBAL* H = (int4 EDMD_GLOBAL*)g;
+        for (uint32_t i = lane; i < bytes / 16; i += 64) {
+            if (IN) L[i] = H[i]; else H[i] = L[i];
+        }
+    } else if ((al & 7) == 0) {
+        int2 EDMD_LDS* L = (int2 EDMD_LDS*)l;
+        int2 EDMD_GLOBAL* H = (int2 EDMD_GLOBAL*)g;
+        for (uint32_t i = lane; i < bytes / 8; i += 64) {
+            if (IN) L[i] = H[i]; else H[i] = L[i];
+        }
+    } else {
+        int EDMD_LDS* L = (int EDMD_LDS*)l;
+        int EDMD_GLOBAL* H = (int EDMD_GLOBAL*)g;
+        for (uint32_t i = lane; i < bytes / 4; i += 64) {
+            if (IN) L[i] = H[i]; else H[i] = L[i];
+        }
+    }
+}
+
+/* Moves a system's loop arrays into LDS (IN) or back to HBM (!IN); the
+ * growth rates are read-only and only come in. */
+template <bool IN, bool G>
+__device__ __forceinline__ void lds_system(const EdmdConst& C, const EdmdArrays& H,
+                                           const EdmdArraysL& A, int lane) {
+    const uint32_t n = (uint32_t)C.n, ncell = (uint32_t)(C.ncx * C.ncy);
+    lds_xfer<IN>((unsigned char EDMD_LDS*)A.p, H.p, n * 64, lane);
+    lds_xfer<IN>((unsigned char EDMD_LDS*)A.e, H.e, n * 32, lane);
+    lds_xfer<IN>((unsigned char EDMD_LDS*)A.leaf, H.leaf, n * 16, lane);
+    for (int l = 0; l < C.nlev; l++)
+        lds_xfer<IN>((unsigned char EDMD_LDS*)A.lev[l], H.lev[l], (uint32_t)C.lsize[l + 1] * 16, lane);
+    lds_xfer<IN>((unsigned char EDMD_LDS*)A.erank, H.erank, n * 16, lane);
+    lds_xfer<IN>((unsigned char EDMD_LDS*)A.lrank, H.lrank, n * 8, lane);
+    lds_xfer<IN>((unsigned char EDMD_LDS*)A.img, H.img, n * 8, lane);
+    lds_xfer<IN>((unsigned char EDMD_LDS*)A.ccnt, H.ccnt, ncell * 4, lane);
+    lds_xfer<IN>((unsigned char EDMD_LDS*)A.cslot, H.cslot, ncell * (uint32_t)C.cap * 4, lane);
+    if (IN && G) lds_xfer<true>((unsigned char EDMD_LDS*)A.vr, H.vr, n * 8, lane);
+}
+
+/* One workgroup (one wavefront) per system with the system's loop arrays
+ * resident in LDS: they come in from HBM when the launch starts, the event
+ * loop runs on them, and they go back when it stops.  The dynamic LDS of
+ * the launch holds the largest system's arrays (edmd_lds_layout). */
+template <int ML, bool G>
+__global__ __launch_bounds__(64) void k_run_lds(const EdmdSys* __restrict__ systems,
+                                                EdmdRunArgs ra) {
+    extern __shared__ __align__(64) unsigned char lds_pool[];
+    const EdmdSys& S = systems[blockIdx.x];
+    const EdmdConst& C = S.c;
+    const EdmdArrays& H = S.a;
+    const int lane = threadIdx.x;
+    if ((uni(H.st->growing) != 0) != G) return;
+    const EdmdLdsLayout L = edmd_lds_layout(&C, G ? 1 : 0);
+    unsigned char EDMD_LDS* base = (unsigned char EDMD_LDS*)lds_pool;
+    /* the pointer table itself lives in LDS: the loop indexes lev[] with
+     * run-time levels, which would put a register copy in scratch */
+    __shared__ EdmdArraysL A;
+    A.p = (EdmdPart EDMD_LDS*)(base + L.p);
+    A.e = (EdmdEv EDMD_LDS*)(base + L.e);
+    A.leaf = (EdmdNode EDMD_LDS*)(base + L.leaf);
+    for (int l = 0; l < EDMD_MAX_LEVELS; l++) A.lev[l] = (EdmdNode EDMD_LDS*)(base + L.lev[l]);
+    A.img = (int32_t EDMD_LDS*)(base + L.img);
+    A.ccnt = (int32_t EDMD_LDS*)(base + L.ccnt);
+    A.cslot = (int32_t EDMD_LDS*)(base + L.cslot);
+    A.vr = (double EDMD_LDS*)(base + L.vr);
+    A.erank = (int64_t EDMD_LDS*)(base + L.erank);
+    A.lrank = (int64_t EDMD_LDS*)(base + L.lrank);
+    A.st = H.st;
+    A.trace = H.trace;
+    A.thermo = H.thermo;
+    A.tape = H.tape;
+    A.dv = H.dv;
+    A.dtl = H.dtl;
+    A.otl = H.otl;
+    A.dr = H.dr;
+    WAVE_SYNC();
+    lds_system<true, G>(C, H, A, lane);
+    WAVE_SYNC();
+    run_loop<ML, G>(C, A, ra);
+    WAVE_SYNC();
+    lds_system<false, G>(C, H, A, lane);
 }
 
 /* ------------------------------------------------------------ single steps */
@@ -2296,6 +2437,42 @@ hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, int gro
                            hipStream_t s) {
     if (grow) launch_run<true>(d_sys, nsys, max_nlev, ra, s);
     else launch_run<false>(d_sys, nsys, max_nlev, ra, s);
+    return hipGetLastError();
+}
+
+/* The LDS-resident loop exists for calendars of depth <= 2 (up to 4096
+ * disks, far more than one workgroup's LDS holds). */
+static const void* lds_kernel(int max_nlev, int grow) {
+    if (max_nlev <= 1) return grow ? (const void*)k_run_lds<1, true> : (const void*)k_run_lds<1, false>;
+    if (max_nlev == 2) return grow ? (const void*)k_run_lds<2, true> : (const void*)k_run_lds<2, false>;
+    return nullptr;
+}
+
+hipError_t edmd_lds_occupancy(int max_nlev, int grow, size_t dyn_bytes, int* blocks_per_cu,
+                              size_t* static_bytes) {
+    const void* f = lds_kernel(max_nlev, grow);
+    *blocks_per_cu = 0;
+    *static_bytes = 0;
+    if (!f) return hipErrorInvalidValue;
+    hipFuncAttributes fa;
+    hipError_t e = hipFuncGetAttributes(&fa, f);
+    if (e != hipSuccess) return e;
+    *static_bytes = fa.sharedSizeBytes;
+    e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_bytes);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, 64, dyn_bytes);
+}
+
+hipError_t edmd_launch_run_lds(const EdmdSys* d_sys, int nsys, int max_nlev, int grow,
+                               size_t dyn_bytes, EdmdRunArgs ra, hipStream_t s) {
+    if (max_nlev > 2) return hipErrorInvalidValue;
+    if (grow) {
+        if (max_nlev <= 1) hipLaunchKernelGGL((k_run_lds<1, true>), dim3(nsys), dim3(64), dyn_bytes, s, d_sys, ra);
+        else hipLaunchKernelGGL((k_run_lds<2, true>), dim3(nsys), dim3(64), dyn_bytes, s, d_sys, ra);
+    } else {
+        if (max_nlev <= 1) hipLaunchKernelGGL((k_run_lds<1, false>), dim3(nsys), dim3(64), dyn_bytes, s, d_sys, ra);
+        else hipLaunchKernelGGL((k_run_lds<2, false>), dim3(nsys), dim3(64), dyn_bytes, s, d_sys, ra);
+    }
     return hipGetLastError();
 }
 

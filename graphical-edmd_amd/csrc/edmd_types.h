@@ -114,6 +114,43 @@ typedef struct EdmdState {
     uint64_t coll_limit;        /* per-system bound on ncol (epochs) */
 } EdmdState;
 
+/* LDS-resident runs: byte offsets of a system's arrays in its workgroup's
+ * LDS (16-byte aligned), shared by the host (to size the launch) and the
+ * device (to carve the pool).  Particle, event, calendar, rank, image and
+ * cell arrays are resident; the state, trace, thermo rows and tape stay in
+ * HBM. */
+#if defined(__HIPCC__)
+#define EDMD_HD __host__ __device__
+#else
+#define EDMD_HD
+#endif
+typedef struct EdmdLdsLayout {
+    uint32_t p, e, leaf, lev[EDMD_MAX_LEVELS], erank, lrank, img, ccnt, cslot, vr, total;
+} EdmdLdsLayout;
+
+static inline EDMD_HD uint64_t edmd_lds_align(uint64_t b) { return (b + 15) & ~(uint64_t)15; }
+
+static inline EDMD_HD EdmdLdsLayout edmd_lds_layout(const EdmdConst* c, int grow) {
+    EdmdLdsLayout L;
+    uint64_t o = 0;
+    const uint64_t n = (uint64_t)c->n, ncell = (uint64_t)c->ncx * (uint64_t)c->ncy;
+    L.p = (uint32_t)o;     o = edmd_lds_align(o + n * 64);
+    L.e = (uint32_t)o;     o = edmd_lds_align(o + n * 32);
+    L.leaf = (uint32_t)o;  o = edmd_lds_align(o + n * 16);
+    for (int l = 0; l < EDMD_MAX_LEVELS; l++) {
+        L.lev[l] = (uint32_t)o;
+        if (l < c->nlev) o = edmd_lds_align(o + (uint64_t)c->lsize[l + 1] * 16);
+    }
+    L.erank = (uint32_t)o; o = edmd_lds_align(o + n * 16);
+    L.lrank = (uint32_t)o; o = edmd_lds_align(o + n * 8);
+    L.img = (uint32_t)o;   o = edmd_lds_align(o + n * 8);
+    L.ccnt = (uint32_t)o;  o = edmd_lds_align(o + ncell * 4);
+    L.cslot = (uint32_t)o; o = edmd_lds_align(o + ncell * (uint64_t)c->cap * 4);
+    L.vr = (uint32_t)o;    o = edmd_lds_align(o + (grow ? n * 8 : 0));
+    L.total = o > 0xffffffffu ? 0xffffffffu : (uint32_t)o;
+    return L;
+}
+
 /* Per-system counters at the end of one epoch of a graph-captured run. */
 typedef struct EdmdEpochRow {
     double t;

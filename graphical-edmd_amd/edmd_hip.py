@@ -23,6 +23,7 @@ STOP_NAMES = {0: "none", 1: "events", 2: "collisions", 3: "time", 4: "error", 5:
 NEVER = 1e26
 INT64_MAX = (1 << 63) - 1
 UINT64_MAX = (1 << 64) - 1
+RES_AUTO, RES_HBM, RES_LDS = 0, 1, 2  # event-loop array residency (edmd_batch_set_residency)
 
 
 class EdmdError(RuntimeError):
@@ -114,6 +115,8 @@ def lib():
             "edmd_batch_size": (i32, [vp, vp]),
             "edmd_batch_stream": (vp, [vp]),
             "edmd_batch_set_stream": (i32, [vp, vp]),
+            "edmd_batch_set_residency": (i32, [vp, i32]),
+            "edmd_batch_get_residency": (i32, [vp, vp, vp, vp]),
             "edmd_batch_time_sweep": (i32, [vp, i32, vp]),
             "edmd_init_lattice": (i32, [i32, dbl, dbl, i32, u64, dbl] + [vp] * 6),
             "edmd_init_random": (i32, [i32, dbl, dbl, dbl, dbl, dbl, u64, dbl] + [vp] * 8),
@@ -411,8 +414,27 @@ class Batch:
                                          C.byref(n)))
         return out[:n.value]
 
+    def thermo(self, s):
+        n = C.c_int()
+        check(lib().edmd_batch_get_thermo(self.h, int(s), None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=THERMO_DT)
+        check(lib().edmd_batch_get_thermo(self.h, int(s), out.ctypes.data, n.value, C.byref(n)))
+        return out[:n.value]
+
     def stream(self):
         return lib().edmd_batch_stream(self.h)
+
+    def set_residency(self, mode):
+        """Where the event loop keeps the systems' arrays: RES_AUTO, RES_HBM
+        or RES_LDS (EDMD_ERR_ARG when a system does not fit LDS)."""
+        check(lib().edmd_batch_set_residency(self.h, int(mode)))
+
+    def residency(self):
+        """{'lds': mode in use is LDS, 'lds_bytes': dynamic LDS per workgroup
+        in LDS mode, 'lds_slots': workgroups resident at once in LDS mode}"""
+        a, b, c = C.c_int(), C.c_int64(), C.c_int64()
+        check(lib().edmd_batch_get_residency(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"lds": bool(a.value), "lds_bytes": b.value, "lds_slots": c.value}
 
     # ---- derivatives of the event map
 

@@ -212,6 +212,26 @@ void* edmd_batch_stream(edmd_batch* b);
 /* Uses the caller's HIP stream (hipStream_t) for all later work. */
 int edmd_batch_set_stream(edmd_batch* b, void* stream);
 
+/* Where the event loop keeps each system's arrays while it runs.  No
+ * counterpart in the reference (one system in host memory): the batched
+ * form of its step loop, mainLoop (EDMD.c:5595-5630).  EDMD_RES_HBM: in
+ * device memory, one wavefront per system.  EDMD_RES_LDS: each system's
+ * particles, event slots, calendar, ranks and cell lists are copied into
+ * its workgroup's LDS when a run launches and back when it stops (calendar
+ * depth <= 2 and the largest system's arrays within one workgroup's LDS,
+ * EDMD_ERR_ARG otherwise).  EDMD_RES_AUTO (the default): LDS when every
+ * system fits and the whole batch is resident on the device at once, HBM
+ * otherwise.  Results are identical in every mode. */
+#define EDMD_RES_AUTO 0
+#define EDMD_RES_HBM 1
+#define EDMD_RES_LDS 2
+int edmd_batch_set_residency(edmd_batch* b, int mode);
+/* The mode in use (*in_lds 1 = LDS), the dynamic LDS per workgroup the LDS
+ * mode takes, and the workgroups the device holds at once in that mode
+ * (0 when the systems do not fit).  Any output may be NULL. */
+int edmd_batch_get_residency(edmd_batch* b, int* in_lds, int64_t* lds_bytes,
+                             int64_t* lds_slots);
+
 /* ---- initial states and files -------------------------------------------- */
 /* Seeded lattice start: lattice 0 = square (first n sites of a
  * ceil(sqrt n)^2 grid), 1 = triangular (the reference's Hex layout,
