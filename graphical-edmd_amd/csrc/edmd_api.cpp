@@ -117,6 +117,8 @@ struct edmd_batch {
      * whether every system fits one workgroup's LDS, the dynamic LDS per
      * workgroup of the Normal [0] and growing [1] launches, the workgroups
      * the device holds at once in LDS mode, and the mode in use */
+    /* tile shape of the prediction sweep (edmd_sweep_shape) */
+    int sweep_variant = 0;
     int residency = EDMD_RES_AUTO;
     bool lds_fit = false;
     size_t lds_dyn[2] = {0, 0};
@@ -129,6 +131,8 @@ struct edmd_ctx {
 };
 
 static int choose_residency(edmd_batch* b);
+static void set_sweep_variant(edmd_batch* b, int v);
+static int default_sweep_variant(const edmd_batch* b);
 
 template <class T>
 static int dalloc(edmd_batch* b, T** p, size_t n) {
@@ -349,7 +353,7 @@ static int upload_system(edmd_batch* b, int s, const edmd_system_desc* d) {
 }
 
 static int batch_predict_all(edmd_batch* b) {
-    HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->stream));
+    HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->sweep_variant, b->stream));
     if (b->any_grow) HIPCHK(edmd_launch_sweep_grow(b->d_sys, b->nsys, b->max_n, b->stream));
     HIPCHK(edmd_launch_sweep_count(b->d_sys, b->nsys, b->stream));
     HIPCHK(edmd_launch_tree(b->d_sys, b->nsys, b->max_lsize, b->max_nlev, b->stream));
@@ -401,10 +405,10 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
         h.thermo0 = b->nthermo;
         b->nthermo += h.c.thermo_cap;
         b->max_n = std::max(b->max_n, h.c.n);
-        b->max_tiles = std::max(b->max_tiles, edmd_sweep_tiles(h.c.ncx, h.c.ncy));
         b->max_nlev = std::max(b->max_nlev, h.c.nlev);
         for (int l = 0; l <= h.c.nlev; l++) b->max_lsize[l] = std::max(b->max_lsize[l], h.c.lsize[l]);
     }
+    set_sweep_variant(b, default_sweep_variant(b));
     int rc = EDMD_OK;
     const size_t np = b->np;
     if (!rc) rc = dalloc(b, &b->part, np);
@@ -490,6 +494,44 @@ extern "C" int edmd_batch_set_stream(edmd_batch* b, void* stream) {
     return EDMD_OK;
 }
 
+/* Sweep tile shape: the grid covers the largest system's tiles. */
+static void set_sweep_variant(edmd_batch* b, int v) {
+    b->sweep_variant = v;
+    b->max_tiles = 0;
+    for (int s = 0; s < b->nsys; s++)
+        b->max_tiles = std::max(b->max_tiles, edmd_sweep_tiles(b->hs[s].c.ncx, b->hs[s].c.ncy, v));
+}
+
+/* Default sweep shape from the disks per cell rho of the batch: a tile's
+ * disks take ceil(rho TX TY / 64) wave passes of the candidate loop, so
+ * the cost per disk is that many passes of 64 over rho TX TY, times a
+ * staging term for the halo; shapes whose staged disks would come near
+ * STAGE are skipped, and idle waves break ties. */
+static int default_sweep_variant(const edmd_batch* b) {
+    double nd = 0, nc = 0;
+    for (int s = 0; s < b->nsys; s++) {
+        nd += b->hs[s].c.n;
+        nc += (double)b->hs[s].c.ncx * b->hs[s].c.ncy;
+    }
+    const double rho = nd / nc;
+    int best = 0;
+    double bc = 1e300;
+    for (int v = 0; v < EDMD_SWEEP_VARIANTS; v++) {
+        const EdmdSweepShape sh = edmd_sweep_shape(v);
+        const double cells = (double)sh.tx * sh.ty, halo = (double)(sh.tx + 2) * (sh.ty + 2);
+        const double owners = rho * cells;
+        if (rho * halo * 1.15 > sh.stage || owners <= 0) continue;
+        const double waves = ceil(owners / 64);
+        const double idle = std::max(0.0, sh.threads / 64.0 - waves);
+        const double cost = waves * 64 / owners * (1 + 0.1 * halo / cells) + 0.01 * idle;
+        if (cost < bc) {
+            bc = cost;
+            best = v;
+        }
+    }
+    return best;
+}
+
 /* Decides where the event loop keeps each system's arrays.  LDS residency
  * needs a calendar of depth <= 2 and every system's arrays (edmd_lds_layout)
  * plus the loop's static LDS within one workgroup's LDS; in auto mode it is
@@ -564,6 +606,117 @@ extern "C" int edmd_batch_get_residency(edmd_batch* b, int* in_lds, int64_t* lds
     if (lds_bytes) *lds_bytes = (int64_t)(b->any_grow ? b->lds_dyn[1] : b->lds_dyn[0]);
     if (lds_slots) *lds_slots = b->lds_fit ? b->lds_slots : 0;
     return EDMD_OK;
+}
+
+extern "C" int edmd_batch_set_sweep_shape(edmd_batch* b, int shape) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    if (shape < 0 || shape >= EDMD_SWEEP_VARIANTS)
+        return fail(EDMD_ERR_ARG, "sweep shape %d out of range [0, %d)", shape, EDMD_SWEEP_VARIANTS);
+    set_sweep_variant(b, shape);
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_get_sweep_shape(edmd_batch* b, int* shape, int* tile_x, int* tile_y,
+                                          int* threads) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    const EdmdSweepShape sh = edmd_sweep_shape(b->sweep_variant);
+    if (shape) *shape = b->sweep_variant;
+    if (tile_x) *tile_x = sh.tx;
+    if (tile_y) *tile_y = sh.ty;
+    if (threads) *threads = sh.threads;
+    return EDMD_OK;
+}
+
+static int check_status(edmd_batch* b);
+static size_t ckpt_bytes(edmd_batch* b);
+static int ckpt_copy(edmd_batch* b, void* mem, bool save);
+
+/* Times every sweep shape and, when the systems fit LDS, a trial run in
+ * each residency, on the batch itself: its state is saved first and put
+ * back after every trial, so the batch continues exactly as if untuned. */
+extern "C" int edmd_batch_autotune(edmd_batch* b, int sweep_reps, int64_t trial_events,
+                                   edmd_tune_report* out) {
+    static_assert(EDMD_TUNE_SWEEP_SHAPES == EDMD_SWEEP_VARIANTS, "sweep shape count");
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    if (sweep_reps < 0 || trial_events < 0) return fail(EDMD_ERR_ARG, "negative trial sizes");
+    if (b->deriv_on) return fail(EDMD_ERR_STATE, "autotuning is not available while taping");
+    edmd_tune_report r;
+    memset(&r, 0, sizeof r);
+    for (int v = 0; v < EDMD_TUNE_SWEEP_SHAPES; v++) r.sweep_ms[v] = -1;
+    r.loop_ms_hbm = r.loop_ms_lds = -1;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    void* snap = nullptr;
+    if (hipMalloc(&snap, ckpt_bytes(b)) != hipSuccess)
+        return fail(EDMD_ERR_NOMEM, "hipMalloc of %zu bytes for the tuning snapshot failed",
+                    ckpt_bytes(b));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    int rc = e == hipSuccess ? ckpt_copy(b, snap, true) : fail(EDMD_ERR_HIP, "hipEventCreate failed");
+    float ms = 0;
+    if (!rc && sweep_reps > 0) {
+        int best = b->sweep_variant;
+        double bt = 1e300;
+        for (int v = 0; v < EDMD_SWEEP_VARIANTS && e == hipSuccess; v++) {
+            set_sweep_variant(b, v);
+            e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, v, b->stream);
+            if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+            for (int k = 0; k < sweep_reps && e == hipSuccess; k++)
+                e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, v, b->stream);
+            if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess) {
+                r.sweep_ms[v] = ms / sweep_reps;
+                if (r.sweep_ms[v] < bt) {
+                    bt = r.sweep_ms[v];
+                    best = v;
+                }
+            }
+        }
+        set_sweep_variant(b, best);
+        if (e != hipSuccess) rc = fail(EDMD_ERR_HIP, "sweep trial: %s", hipGetErrorString(e));
+    }
+    if (!rc) rc = ckpt_copy(b, snap, false);
+    if (!rc && trial_events > 0 && b->lds_fit) {
+        EdmdRunArgs ra;
+        memset(&ra, 0, sizeof ra);
+        ra.max_events = INT64_MAX; /* the per-system trial bound applies */
+        ra.max_coll = ~(uint64_t)0;
+        ra.tmax = 1e300;
+        double t[2] = {0, 0};
+        for (int m = 0; m < 2 && !rc; m++) {
+            b->use_lds = m == 1;
+            e = edmd_launch_epoch_begin(b->d_sys, b->nsys, trial_events, ~(uint64_t)0, b->stream);
+            if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+            if (e == hipSuccess && b->any_grow) e = launch_loop(b, 1, ra);
+            if (e == hipSuccess) e = launch_loop(b, 0, ra);
+            if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) rc = fail(EDMD_ERR_HIP, "loop trial: %s", hipGetErrorString(e));
+            t[m] = ms;
+            if (!rc) rc = check_status(b);
+            int rc2 = ckpt_copy(b, snap, false);
+            if (!rc) rc = rc2;
+        }
+        if (!rc) {
+            r.loop_ms_hbm = t[0];
+            r.loop_ms_lds = t[1];
+            b->residency = t[1] < t[0] ? EDMD_RES_LDS : EDMD_RES_HBM;
+        }
+        int rc3 = choose_residency(b);
+        if (!rc) rc = rc3;
+    }
+    if (hipStreamSynchronize(b->stream) != hipSuccess && !rc)
+        rc = fail(EDMD_ERR_HIP, "tuning: stream synchronisation failed");
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(snap);
+    r.sweep_shape = b->sweep_variant;
+    r.in_lds = b->use_lds ? 1 : 0;
+    if (out) *out = r;
+    return rc;
 }
 
 static int batch_run_mask(edmd_batch* b, int64_t max_events, uint64_t max_collisions, double tmax,
@@ -670,7 +823,8 @@ extern "C" int edmd_batch_time_sweep(edmd_batch* b, int reps, double* avg_ms) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, b->stream));
-    for (int r = 0; r < reps; r++) HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->stream));
+    for (int r = 0; r < reps; r++)
+        HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->sweep_variant, b->stream));
     HIPCHK(hipEventRecord(e1, b->stream));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;

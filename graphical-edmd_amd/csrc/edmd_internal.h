@@ -7,8 +7,15 @@
 
 #include "edmd_types.h"
 
-hipError_t edmd_launch_sweep_tiled(const EdmdSys* d_sys, int nsys, int max_tiles, hipStream_t s);
-int edmd_sweep_tiles(int ncx, int ncy);
+/* Tile shape of sweep variant v (tx = 0 for an unknown variant). */
+typedef struct EdmdSweepShape {
+    int tx, ty, threads, stage;
+} EdmdSweepShape;
+#define EDMD_SWEEP_VARIANTS 8
+EdmdSweepShape edmd_sweep_shape(int variant);
+hipError_t edmd_launch_sweep_tiled(const EdmdSys* d_sys, int nsys, int max_tiles, int variant,
+                                   hipStream_t s);
+int edmd_sweep_tiles(int ncx, int ncy, int variant);
 hipError_t edmd_launch_tree(const EdmdSys* d_sys, int nsys, const int* max_lsize, int max_nlev,
                             hipStream_t s);
 hipError_t edmd_launch_sync_all(const EdmdSys* d_sys, int nsys, int max_n, hipStream_t s);

@@ -916,43 +916,60 @@ __device__ __forceinline__ bool sweep_one(const EdmdConst& C, const AR& A, doubl
 }
 
 /* Cell-tiled sweep: one workgroup per SW_TX x SW_TY cells.  The tile and a
- * one-cell halo (wrapped periodically) are staged into LDS as SoA, the
- * tile's own disks first; each thread then predicts one disk of the tile
- * from LDS, scanning its 3x3 cells in the reference's order (y offset
- * outer, x offset inner, newest list entry first, EDMD.c:2959-3003) with
- * the strict-less rule, so the result is that of predict_seq bit for bit.
- * Every disk record is read from HBM by the one tile that owns it and the
- * (at most eight) tiles whose halo holds it, instead of once per neighbour
- * scan that touches it.  Workgroups are dealt to the eight XCDs round
- * robin; tiles are renumbered so that each XCD sweeps a contiguous band of
- * tile rows and halo re-reads hit its own L2.  A tile whose staged disks
- * exceed SW_STAGE falls back to predict_seq from global memory. */
-#ifndef SW_TX
-#define SW_TX 8
-#endif
-#ifndef SW_TY
-#define SW_TY 8
-#endif
-#ifndef SW_STAGE
-#define SW_STAGE 512
-#endif
-#ifndef SW_THREADS
-#define SW_THREADS 320 /* five waves: a tile of 8 x 8 cells holds ~260 disks at phi = 0.8 */
-#endif
-#define SW_HW (SW_TX + 2)
-#define SW_HH (SW_TY + 2)
-#define SW_NH (SW_HW * SW_HH)
-static_assert(SW_NH <= 128 && SW_NH <= 255, "staged cells: two per lane of wave 0, u8 index");
-static_assert(SW_THREADS >= 64 && SW_THREADS % 64 == 0, "whole waves");
+ * one-cell halo (wrapped periodically) are staged into LDS cell by cell in
+ * row-major order, newest list entry first within a cell, so the three
+ * cells of one row of a disk's 3x3 neighbourhood are one contiguous run
+ * that lists the candidates in the reference's scan order (y offset outer,
+ * x offset inner, newest entry first, EDMD.c:2959-3003).  Each thread
+ * predicts one of the tile's disks over its three runs as one flat
+ * candidate loop whose trip count is uniform over the wave; the loop keeps
+ * only the cheap terms (approach b, contact d, discriminant) and marks the
+ * candidates that can collide, and the roots of the marked ones are solved
+ * afterwards in the same order with the strict-less rule, so the result is
+ * that of predict_seq bit for bit.  Every disk record is read from HBM by
+ * the one tile that owns it and the (at most eight) tiles whose halo holds
+ * it, instead of once per neighbour scan that touches it.  Workgroups are
+ * dealt to the eight XCDs round robin; tiles are renumbered so that each
+ * XCD sweeps a contiguous band of tile rows and halo re-reads hit its own
+ * L2.  A tile whose staged disks exceed SW_STAGE falls back to predict_seq
+ * from global memory. */
+/* Tile shapes of the sweep: TX x TY cells per workgroup of THREADS
+ * threads; STAGE staged disks at most (tile plus halo). */
+template <int TX_, int TY_, int THREADS_>
+struct SweepShape {
+    static constexpr int TX = TX_, TY = TY_, THREADS = THREADS_;
+    static constexpr int HW = TX + 2, HH = TY + 2, NH = HW * HH;
+    static constexpr int STAGE = 512;
+    static_assert(NH <= 255, "staged cells carry a u8 index");
+    static_assert(THREADS >= 64 && THREADS % 64 == 0, "whole waves");
+};
 
+/* The shapes a batch chooses from (edmd_batch_autotune times them; the
+ * default is picked from the disks per cell).  At phi = 0.8 with unit
+ * diameter a cell holds ~4.1 disks, at phi = 0.7 ~3.6, with the reference
+ * CLI's radius 1 about one. */
+using SweepV0 = SweepShape<8, 8, 320>;
+using SweepV1 = SweepShape<6, 10, 256>;
+using SweepV2 = SweepShape<10, 6, 256>;
+using SweepV3 = SweepShape<8, 7, 256>;
+using SweepV4 = SweepShape<8, 6, 192>;
+using SweepV5 = SweepShape<8, 8, 256>;
+using SweepV6 = SweepShape<8, 8, 64>;
+using SweepV7 = SweepShape<16, 8, 128>;
+
+template <class S>
 struct SweepLds {
-    /* staged disks: position flown to the sweep time, velocity, radius */
-    double xt[SW_STAGE], yt[SW_STAGE], vx[SW_STAGE], vy[SW_STAGE], rad[SW_STAGE];
-    int32_t id[SW_STAGE];
-    uint8_t hcell[SW_STAGE]; /* staged cell of every staged disk */
-    int32_t cnt[SW_NH];
-    int32_t start[SW_NH];
-    int32_t gcell[SW_NH];
+    /* staged disks: position flown to the sweep time, velocity, radius,
+     * index, staged cell */
+    double2 pos[S::STAGE], vel[S::STAGE];
+    double rad[S::STAGE];
+    int32_t id[S::STAGE];
+    uint8_t hcell[S::STAGE];
+    int16_t own[S::STAGE];     /* staged index of each of the tile's own disks */
+    int32_t cnt[S::NH];
+    int32_t start[S::NH + 1];  /* first staged index of each cell; [S::NH] = total */
+    int32_t ostart[S::NH];     /* first own-list index of each tile cell */
+    int32_t gcell[S::NH];
     int32_t ninner, total;
 };
 
@@ -961,59 +978,139 @@ __device__ __forceinline__ int mod_cell(int a, int nc) {
     return r < 0 ? r + nc : r;
 }
 
-/* Exclusive scan of v over the SW_NH staged cells by wave 0 (two cells per
- * lane); returns the total.  Cells are ordered h = (ly + 1) * SW_HW + lx + 1. */
+/* Exclusive scan of v over the NH staged cells by one wave (PER
+ * consecutive cells per lane); returns the total.  Cells are ordered
+ * h = (ly + 1) * HW + lx + 1. */
+template <int NH>
 __device__ __forceinline__ int sweep_scan(int lane, const int* v, int* out, int base) {
-    int a = (2 * lane < SW_NH) ? v[2 * lane] : 0;
-    int b = (2 * lane + 1 < SW_NH) ? v[2 * lane + 1] : 0;
-    int incl = a + b;
+    constexpr int PER = (NH + 63) / 64;
+    int val[PER];
+    int sum = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int h = PER * lane + i;
+        val[i] = h < NH ? v[h] : 0;
+        sum += val[i];
+    }
+    int incl = sum;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         int u = __shfl_up(incl, d, 64);
         if (lane >= d) incl += u;
     }
-    const int excl = incl - a - b;
-    if (2 * lane < SW_NH) out[2 * lane] = base + excl;
-    if (2 * lane + 1 < SW_NH) out[2 * lane + 1] = base + excl + a;
+    int run = base + incl - sum;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int h = PER * lane + i;
+        if (h < NH) out[h] = run;
+        run += val[i];
+    }
     return __shfl(incl, 63, 64);
 }
 
-/* The predictions of a staged tile's own disks: one thread per disk,
- * candidates from LDS in the reference's scan order (y offset outer, x
- * offset inner, newest entry first) with the strict-less rule. */
-template <bool WRAP, class AR>
+/* The terms of collisionTime (EDMD.c:2661-2725) that decide whether a pair
+ * can collide: relative position (minimum image when WRAP) and velocity,
+ * approach b, |dv|^2, contact term d and discriminant, in the reference's
+ * operation order. */
+template <bool WRAP>
+__device__ __forceinline__ void pair_terms(const EdmdConst& C, double ox, double oy, double ovx,
+                                           double ovy, double orad, const double2 pv,
+                                           const double2 vv, double r2, double& b, double& v2,
+                                           double& d, double& det) {
+    const double dvx = vv.x - ovx;
+    const double dvy = vv.y - ovy;
+    double dx = pv.x - ox;
+    double dy = pv.y - oy;
+    if (WRAP) min_image(C, dx, dy);
+    b = dx * dvx + dy * dvy;
+    v2 = dvx * dvx + dvy * dvy;
+    d = dx * dx + dy * dy - 4 * orad * r2; /* DIST2, EDMD.c:35-43 */
+    det = b * b - v2 * d;
+}
+
+/* The predictions of a staged tile's own disks, one thread per disk.  The
+ * candidates of a disk are its three row runs, k = 0 .. tot-1 in scan
+ * order.  Pass 1 runs k over the wave's largest tot (uniform) and marks in
+ * a 256-bit mask the candidates with b <= 0 and det >= 0 (every other one
+ * gives the reference's "never", which the strict-less minimum never
+ * takes), flagging overlaps as collisionTime does; pass 2 solves the marked
+ * roots in increasing k.  A disk with more than 256 candidates solves its
+ * roots in pass 1 instead. */
+template <class S, bool WRAP, class AR>
 __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR& A,
-                                                   const SweepLds& L, int tid, int ninner, double t,
+                                                   const SweepLds<S>& L, int tid, int ninner, double t,
                                                    int tx0, int ty0, double paul, int64_t ctr0,
                                                    bool& ov) {
-    for (int u = tid; u < ninner; u += SW_THREADS) {
-        const int p = L.id[u];
-        const int h = L.hcell[u];
-        const int lx = h % SW_HW - 1, ly = h / SW_HW - 1;
+    const int lane = tid & 63;
+    for (int u0 = tid - lane; u0 < ninner; u0 += S::THREADS) { /* uniform over the wave */
+        const int u = u0 + lane;
+        const bool act = u < ninner;
+        const int me = L.own[act ? u : 0]; /* idle lanes shadow the first disk */
+        const int h = L.hcell[me];
+        const int lx = h % S::HW - 1, ly = h / S::HW - 1;
         /* the owner is synchronised at t (its staged position is its own) */
-        const double ox = L.xt[u], oy = L.yt[u], ovx = L.vx[u], ovy = L.vy[u], orad = L.rad[u];
+        const double2 opos = L.pos[me], ovel = L.vel[me];
+        const double ox = opos.x, oy = opos.y, ovx = ovel.x, ovy = ovel.y, orad = L.rad[me];
+        const int p = L.id[me];
+        /* row runs: cells (lx - 1 .. lx + 1, ly + j) are staged contiguously */
+        const int h0 = ly * S::HW + lx;
+        const int a0 = L.start[h0], n0 = L.start[h0 + 3] - a0;
+        const int a1 = L.start[h0 + S::HW], n1 = L.start[h0 + S::HW + 3] - a1;
+        const int a2 = L.start[h0 + 2 * S::HW], n2 = L.start[h0 + 2 * S::HW + 3] - a2;
+        const int c1 = n0, c2 = n0 + n1;
+        const int o0 = a0, o1 = a1 - c1, o2 = a2 - c2;
+        const int tot = act ? c2 + n2 : 0;
+        const int ktot = -wave_min_i32(-tot);
+        const bool direct = tot > 256;
+        uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+        double dt = EDMD_NEVER;
+        int best = -1;
+        for (int k = 0; k < ktot; ++k) {
+            if (k >= tot) continue;
+            const int v = k + (k < c1 ? o0 : (k < c2 ? o1 : o2));
+            /* the owner itself; a halo that wraps (boxes under three tiles
+             * wide) may stage the owner's cell more than once */
+            if (WRAP ? L.id[v] == p : v == me) continue;
+            double b, v2, d, det;
+            pair_terms<WRAP>(C, ox, oy, ovx, ovy, orad, L.pos[v], L.vel[v], L.rad[v], b, v2, d,
+                             det);
+            if (b > 0) continue;
+            if (d < -0.01) ov = true; /* EDMD.c:2708-2717 */
+            if (det < 0) continue;
+            if (direct) {
+                const double tt = (-b - sqrt(det)) / v2;
+                if (dt > tt) {
+                    dt = tt;
+                    best = v;
+                }
+            } else {
+                const uint64_t bit = (uint64_t)1 << (k & 63);
+                if (k < 64) m0 |= bit;
+                else if (k < 128) m1 |= bit;
+                else if (k < 192) m2 |= bit;
+                else m3 |= bit;
+            }
+        }
+        while (m0 | m1 | m2 | m3) {
+            int k;
+            if (m0) { k = __builtin_ctzll(m0); m0 &= m0 - 1; }
+            else if (m1) { k = 64 + __builtin_ctzll(m1); m1 &= m1 - 1; }
+            else if (m2) { k = 128 + __builtin_ctzll(m2); m2 &= m2 - 1; }
+            else { k = 192 + __builtin_ctzll(m3); m3 &= m3 - 1; }
+            const int v = k + (k < c1 ? o0 : (k < c2 ? o1 : o2));
+            double b, v2, d, det;
+            pair_terms<WRAP>(C, ox, oy, ovx, ovy, orad, L.pos[v], L.vel[v], L.rad[v], b, v2, d,
+                             det);
+            const double tt = (-b - sqrt(det)) / v2;
+            if (dt > tt) {
+                dt = tt;
+                best = v;
+            }
+        }
+        if (!act) continue;
         double tc;
         int dir;
         cross_time(C, t, ox, oy, ovx, ovy, tx0 + lx, ty0 + ly, tc, dir);
-        double dt = EDMD_NEVER;
-        int best = -1;
-        for (int j = -1; j <= 1; ++j) {
-            for (int k = -1; k <= 1; ++k) {
-                const int hh = (ly + 1 + j) * SW_HW + (lx + 1 + k);
-                const int st0 = L.start[hh];
-                for (int v = st0 + L.cnt[hh] - 1; v >= st0; --v) {
-                    /* the owner itself; a halo that wraps (boxes under three
-                     * tiles wide) may stage the owner's cell more than once */
-                    if (WRAP ? L.id[v] == p : v == u) continue;
-                    double tt = coll_time_at<WRAP>(C, ox, oy, ovx, ovy, orad, L.xt[v], L.yt[v],
-                                                   L.vx[v], L.vy[v], L.rad[v], ov);
-                    if (dt > tt) {
-                        dt = tt;
-                        best = v;
-                    }
-                }
-            }
-        }
         const double tk = t + dt;
         const int partner = best >= 0 ? L.id[best] : 0;
         EdmdEv e;
@@ -1038,96 +1135,92 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
     }
 }
 
-__global__ __launch_bounds__(SW_THREADS) void k_sweep_tiled(const EdmdSys* __restrict__ systems) {
-    __shared__ SweepLds L;
-    __shared__ int vtmp[SW_NH];
-    const EdmdSys* S = systems + blockIdx.y;
-    const EdmdConst& C = S->c;
-    const EdmdArrays& A = S->a;
+template <class S>
+__global__ __launch_bounds__(S::THREADS) void k_sweep_tiled(const EdmdSys* __restrict__ systems) {
+    __shared__ SweepLds<S> L;
+    __shared__ int vtmp[S::NH];
+    const EdmdSys* Sy = systems + blockIdx.y;
+    const EdmdConst& C = Sy->c;
+    const EdmdArrays& A = Sy->a;
     const int tid = threadIdx.x;
     if (A.st->growing) return; /* k_sweep_grow predicts growing systems */
     const int ncx = C.ncx, ncy = C.ncy;
-    const int ntx = (ncx + SW_TX - 1) / SW_TX, nty = (ncy + SW_TY - 1) / SW_TY;
+    const int ntx = (ncx + S::TX - 1) / S::TX, nty = (ncy + S::TY - 1) / S::TY;
     const int per = gridDim.x / 8; /* gridDim.x is a multiple of 8 */
     const int tile = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
     if (tile >= ntx * nty) return; /* uniform over the workgroup */
-    const int tx0 = (tile % ntx) * SW_TX, ty0 = (tile / ntx) * SW_TY;
-    const int vw = min(SW_TX, ncx - tx0), vh = min(SW_TY, ncy - ty0);
+    const int tx0 = (tile % ntx) * S::TX, ty0 = (tile / ntx) * S::TY;
+    const int vw = min(S::TX, ncx - tx0), vh = min(S::TY, ncy - ty0);
     const double t = A.st->t;
     /* eventListInit order (EDMD.c:2007-2012): crossing then collision of
      * each particle in index order, after the insertions already counted */
     const double paul = A.st->paul;
     const int64_t ctr0 = A.st->ctr;
 
-    for (int h = tid; h < SW_NH; h += SW_THREADS) {
-        const int lx = h % SW_HW - 1, ly = h / SW_HW - 1;
+    for (int h = tid; h < S::NH; h += S::THREADS) {
+        const int lx = h % S::HW - 1, ly = h / S::HW - 1;
         const int g = mod_cell(ty0 + ly, ncy) * ncx + mod_cell(tx0 + lx, ncx);
         L.gcell[h] = g;
         L.cnt[h] = A.ccnt[g];
     }
     __syncthreads();
-    /* tile cells first, then the halo ring and clipped cells */
+    /* staged order: cells row-major (h), each cell newest entry first;
+     * the tile's own cells also get consecutive places in the owner list */
     if (tid < 64) {
-        for (int h = 2 * tid; h < 2 * tid + 2 && h < SW_NH; ++h) {
-            const int lx = h % SW_HW - 1, ly = h / SW_HW - 1;
+        constexpr int PER = (S::NH + 63) / 64;
+        for (int h = PER * tid; h < PER * tid + PER && h < S::NH; ++h) {
+            const int lx = h % S::HW - 1, ly = h / S::HW - 1;
             const bool own = lx >= 0 && lx < vw && ly >= 0 && ly < vh;
             vtmp[h] = own ? L.cnt[h] : 0;
         }
-        const int nin = sweep_scan(tid, vtmp, vtmp, 0);
-        int other[2] = {0, 0};
-        int sown[2] = {0, 0};
-        for (int k = 0; k < 2; ++k) {
-            const int h = 2 * tid + k;
-            if (h < SW_NH) {
-                const int lx = h % SW_HW - 1, ly = h / SW_HW - 1;
-                const bool own = lx >= 0 && lx < vw && ly >= 0 && ly < vh;
-                sown[k] = own ? vtmp[h] : -1;
-                other[k] = own ? 0 : L.cnt[h];
-            }
-        }
-        for (int k = 0; k < 2; ++k)
-            if (2 * tid + k < SW_NH) vtmp[2 * tid + k] = other[k];
-        const int nout = sweep_scan(tid, vtmp, vtmp, nin);
-        for (int k = 0; k < 2; ++k) {
-            const int h = 2 * tid + k;
-            if (h < SW_NH) L.start[h] = sown[k] >= 0 ? sown[k] : vtmp[h];
-        }
+        const int nin = sweep_scan<S::NH>(tid, vtmp, L.ostart, 0);
+        const int tot = sweep_scan<S::NH>(tid, L.cnt, L.start, 0);
         if (tid == 0) {
+            L.start[S::NH] = tot;
             L.ninner = nin;
-            L.total = nin + nout;
+            L.total = tot;
         }
     }
     __syncthreads();
     const int total = L.total, ninner = L.ninner;
     bool ov = false;
-    if (total <= SW_STAGE) {
+    if (total <= S::STAGE) {
         const int cap = C.cap;
-        for (int h = tid; h < SW_NH; h += SW_THREADS) {
+        for (int h = tid; h < S::NH; h += S::THREADS) {
             const int st0 = L.start[h], c = L.cnt[h];
+            const int lx = h % S::HW - 1, ly = h / S::HW - 1;
+            const bool own = lx >= 0 && lx < vw && ly >= 0 && ly < vh;
             for (int sl = 0; sl < c; ++sl) L.hcell[st0 + sl] = (uint8_t)h;
+            if (own)
+                for (int sl = 0; sl < c; ++sl) L.own[L.ostart[h] + sl] = (int16_t)(st0 + sl);
         }
         __syncthreads();
         /* two staged disks per thread and pass: both slot reads, then both
-         * record reads, are in flight together */
-        for (int u0 = tid; u0 < total; u0 += 2 * SW_THREADS) {
-            const int u1 = u0 + SW_THREADS;
+         * record reads, are in flight together; staged place st0 + i holds
+         * list slot cnt - 1 - i */
+        for (int u0 = tid; u0 < total; u0 += 2 * S::THREADS) {
+            const int u1 = u0 + S::THREADS;
             const int h0 = L.hcell[u0];
-            const int q0 = A.cslot[(size_t)L.gcell[h0] * cap + (u0 - L.start[h0])];
+            const int q0 = A.cslot[(size_t)L.gcell[h0] * cap + (L.start[h0 + 1] - 1 - u0)];
             int q1 = -1;
             if (u1 < total) {
                 const int h1 = L.hcell[u1];
-                q1 = A.cslot[(size_t)L.gcell[h1] * cap + (u1 - L.start[h1])];
+                q1 = A.cslot[(size_t)L.gcell[h1] * cap + (L.start[h1 + 1] - 1 - u1)];
             }
             const EdmdPart r0 = A.p[q0];
             EdmdPart r1;
             if (q1 >= 0) r1 = A.p[q1];
             const double lat0 = t - r0.tl;
-            L.xt[u0] = r0.x + lat0 * r0.vx; L.yt[u0] = r0.y + lat0 * r0.vy;
-            L.vx[u0] = r0.vx; L.vy[u0] = r0.vy; L.rad[u0] = r0.rad; L.id[u0] = q0;
+            L.pos[u0] = make_double2(r0.x + lat0 * r0.vx, r0.y + lat0 * r0.vy);
+            L.vel[u0] = make_double2(r0.vx, r0.vy);
+            L.rad[u0] = r0.rad;
+            L.id[u0] = q0;
             if (q1 >= 0) {
                 const double lat1 = t - r1.tl;
-                L.xt[u1] = r1.x + lat1 * r1.vx; L.yt[u1] = r1.y + lat1 * r1.vy;
-                L.vx[u1] = r1.vx; L.vy[u1] = r1.vy; L.rad[u1] = r1.rad; L.id[u1] = q1;
+                L.pos[u1] = make_double2(r1.x + lat1 * r1.vx, r1.y + lat1 * r1.vy);
+                L.vel[u1] = make_double2(r1.vx, r1.vy);
+                L.rad[u1] = r1.rad;
+                L.id[u1] = q1;
             }
         }
         __syncthreads();
@@ -1138,14 +1231,14 @@ __global__ __launch_bounds__(SW_THREADS) void k_sweep_tiled(const EdmdSys* __res
         const bool nowrap = tx0 >= 1 && ty0 >= 1 && tx0 + vw <= ncx - 1 && ty0 + vh <= ncy - 1 &&
                             ncx >= 5 && ncy >= 5;
         if (nowrap)
-            sweep_tile_predict<false>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
+            sweep_tile_predict<S, false>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
         else
-            sweep_tile_predict<true>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
+            sweep_tile_predict<S, true>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
     } else {
         const int cap = C.cap;
-        for (int w = tid; w < SW_TX * SW_TY * cap; w += SW_THREADS) {
+        for (int w = tid; w < S::TX * S::TY * cap; w += S::THREADS) {
             const int c = w / cap, sl = w - c * cap;
-            const int lx = c % SW_TX, ly = c / SW_TX;
+            const int lx = c % S::TX, ly = c / S::TX;
             if (lx >= vw || ly >= vh) continue;
             const int g = (ty0 + ly) * ncx + tx0 + lx;
             if (sl < A.ccnt[g]) {
@@ -2356,15 +2449,52 @@ __global__ void k_fp64_check(int n, const double* __restrict__ a, const double* 
 
 /* ------------------------------------------------------------ launchers */
 
-hipError_t edmd_launch_sweep_tiled(const EdmdSys* d_sys, int nsys, int max_tiles, hipStream_t s) {
-    if (nsys <= 0 || max_tiles <= 0) return hipSuccess;
+template <class S>
+static void launch_sweep(const EdmdSys* d_sys, int nsys, int max_tiles, hipStream_t s) {
     dim3 grid((max_tiles + 7) / 8 * 8, nsys);
-    hipLaunchKernelGGL(k_sweep_tiled, grid, dim3(SW_THREADS), 0, s, d_sys);
+    hipLaunchKernelGGL((k_sweep_tiled<S>), grid, dim3(S::THREADS), 0, s, d_sys);
+}
+
+hipError_t edmd_launch_sweep_tiled(const EdmdSys* d_sys, int nsys, int max_tiles, int variant,
+                                   hipStream_t s) {
+    if (nsys <= 0 || max_tiles <= 0) return hipSuccess;
+    switch (variant) {
+        case 0: launch_sweep<SweepV0>(d_sys, nsys, max_tiles, s); break;
+        case 1: launch_sweep<SweepV1>(d_sys, nsys, max_tiles, s); break;
+        case 2: launch_sweep<SweepV2>(d_sys, nsys, max_tiles, s); break;
+        case 3: launch_sweep<SweepV3>(d_sys, nsys, max_tiles, s); break;
+        case 4: launch_sweep<SweepV4>(d_sys, nsys, max_tiles, s); break;
+        case 5: launch_sweep<SweepV5>(d_sys, nsys, max_tiles, s); break;
+        case 6: launch_sweep<SweepV6>(d_sys, nsys, max_tiles, s); break;
+        case 7: launch_sweep<SweepV7>(d_sys, nsys, max_tiles, s); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-int edmd_sweep_tiles(int ncx, int ncy) {
-    return ((ncx + SW_TX - 1) / SW_TX) * ((ncy + SW_TY - 1) / SW_TY);
+template <class S>
+static EdmdSweepShape shape_of() {
+    return EdmdSweepShape{S::TX, S::TY, S::THREADS, S::STAGE};
+}
+
+EdmdSweepShape edmd_sweep_shape(int variant) {
+    switch (variant) {
+        case 0: return shape_of<SweepV0>();
+        case 1: return shape_of<SweepV1>();
+        case 2: return shape_of<SweepV2>();
+        case 3: return shape_of<SweepV3>();
+        case 4: return shape_of<SweepV4>();
+        case 5: return shape_of<SweepV5>();
+        case 6: return shape_of<SweepV6>();
+        case 7: return shape_of<SweepV7>();
+        default: return EdmdSweepShape{0, 0, 0, 0};
+    }
+}
+
+int edmd_sweep_tiles(int ncx, int ncy, int variant) {
+    const EdmdSweepShape sh = edmd_sweep_shape(variant);
+    if (sh.tx <= 0) return 0;
+    return ((ncx + sh.tx - 1) / sh.tx) * ((ncy + sh.ty - 1) / sh.ty);
 }
 
 hipError_t edmd_launch_tree(const EdmdSys* d_sys, int nsys, const int* max_lsize, int max_nlev,

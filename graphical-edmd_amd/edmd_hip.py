@@ -63,6 +63,11 @@ THERMO_DT = np.dtype([("t", "<f8"), ("ncol", "<u8"), ("e_per_n", "<f8"), ("press
 EPOCH_DT = np.dtype([("t", "<f8"), ("ncol", "<u8"), ("ncross", "<u8"), ("nevents", "<u8"),
                      ("status", "<i4"), ("stop_reason", "<i4")])
 
+
+class TuneReport(C.Structure):
+    _fields_ = [("sweep_ms", C.c_double * 8), ("sweep_shape", C.c_int), ("in_lds", C.c_int),
+                ("loop_ms_hbm", C.c_double), ("loop_ms_lds", C.c_double)]
+
 TAPE_DT = np.dtype([("t", "<f8"), ("ax", "<f8"), ("ay", "<f8"), ("cx", "<f8"),
                     ("cy", "<f8"), ("ff", "<f8"), ("i", "<i4"), ("j", "<i4"),
                     ("pad", "<i4", (2,))])
@@ -117,6 +122,9 @@ def lib():
             "edmd_batch_set_stream": (i32, [vp, vp]),
             "edmd_batch_set_residency": (i32, [vp, i32]),
             "edmd_batch_get_residency": (i32, [vp, vp, vp, vp]),
+            "edmd_batch_set_sweep_shape": (i32, [vp, i32]),
+            "edmd_batch_get_sweep_shape": (i32, [vp, vp, vp, vp, vp]),
+            "edmd_batch_autotune": (i32, [vp, i32, i64, vp]),
             "edmd_batch_time_sweep": (i32, [vp, i32, vp]),
             "edmd_init_lattice": (i32, [i32, dbl, dbl, i32, u64, dbl] + [vp] * 6),
             "edmd_init_random": (i32, [i32, dbl, dbl, dbl, dbl, dbl, u64, dbl] + [vp] * 8),
@@ -428,6 +436,24 @@ class Batch:
         """Where the event loop keeps the systems' arrays: RES_AUTO, RES_HBM
         or RES_LDS (EDMD_ERR_ARG when a system does not fit LDS)."""
         check(lib().edmd_batch_set_residency(self.h, int(mode)))
+
+    def set_sweep_shape(self, shape):
+        check(lib().edmd_batch_set_sweep_shape(self.h, int(shape)))
+
+    def sweep_shape(self):
+        """{'shape', 'tile_x', 'tile_y', 'threads'} of the prediction sweep."""
+        v = [C.c_int() for _ in range(4)]
+        check(lib().edmd_batch_get_sweep_shape(self.h, *[C.byref(x) for x in v]))
+        return dict(zip(("shape", "tile_x", "tile_y", "threads"), [x.value for x in v]))
+
+    def autotune(self, sweep_reps=10, trial_events=2000):
+        """Times every sweep shape and both residencies on this batch (state
+        saved and restored) and keeps the fastest; returns the report."""
+        r = TuneReport()
+        check(lib().edmd_batch_autotune(self.h, int(sweep_reps), int(trial_events), C.byref(r)))
+        return {"sweep_ms": list(r.sweep_ms), "sweep_shape": r.sweep_shape,
+                "in_lds": bool(r.in_lds), "loop_ms_hbm": r.loop_ms_hbm,
+                "loop_ms_lds": r.loop_ms_lds}
 
     def residency(self):
         """{'lds': mode in use is LDS, 'lds_bytes': dynamic LDS per workgroup

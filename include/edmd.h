@@ -232,6 +232,30 @@ int edmd_batch_set_residency(edmd_batch* b, int mode);
 int edmd_batch_get_residency(edmd_batch* b, int* in_lds, int64_t* lds_bytes,
                              int64_t* lds_slots);
 
+/* Tile shape of the prediction sweep (collisionEventNormal and
+ * crossingEventNormal for every disk, as eventListInit runs them,
+ * EDMD.c:2007-2012): shape s tiles the cell grid into tile_x x tile_y
+ * cells per workgroup of `threads` threads.  Every shape gives identical
+ * predictions; the default is chosen from the disks per cell. */
+#define EDMD_TUNE_SWEEP_SHAPES 8
+int edmd_batch_set_sweep_shape(edmd_batch* b, int shape);
+int edmd_batch_get_sweep_shape(edmd_batch* b, int* shape, int* tile_x, int* tile_y, int* threads);
+
+/* Autotuner, no counterpart in the reference: times sweep_reps sweeps in
+ * every tile shape and, when every system fits LDS, a trial of
+ * trial_events events per system in each residency (EDMD_RES_HBM and
+ * EDMD_RES_LDS), then keeps the fastest of each.  The batch's state is
+ * saved before and restored after every trial, so results are unchanged.
+ * sweep_reps = 0 or trial_events = 0 skips that part.  Not while taping. */
+typedef struct edmd_tune_report {
+    double sweep_ms[EDMD_TUNE_SWEEP_SHAPES]; /* per shape, < 0 when not timed */
+    int sweep_shape;                         /* the shape kept */
+    int in_lds;                              /* the residency kept (1 = LDS) */
+    double loop_ms_hbm, loop_ms_lds;         /* trial times, < 0 when not run */
+} edmd_tune_report;
+int edmd_batch_autotune(edmd_batch* b, int sweep_reps, int64_t trial_events,
+                        edmd_tune_report* out);
+
 /* ---- initial states and files -------------------------------------------- */
 /* Seeded lattice start: lattice 0 = square (first n sites of a
  * ceil(sqrt n)^2 grid), 1 = triangular (the reference's Hex layout,

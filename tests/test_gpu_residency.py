@@ -25,6 +25,16 @@ def small_states(k0=0, count=6):
     return out
 
 
+def polydisperse(n, seed):
+    """Square lattice at phi 0.5 (spacing 1.25) with radii uniform in
+    [0.4, 0.6]: no overlaps."""
+    st = F.lattice_state(n, 0.5, seed=seed, lattice="fit")
+    rng = np.random.default_rng(seed)
+    st["rad"] = rng.uniform(0.4, 0.6, n)
+    st["m"] = (st["rad"] / 0.5) ** 3
+    return st
+
+
 def pair(gpu, states, **kw):
     """The same batch twice: LDS-resident and device-memory loops."""
     a = gpu.Batch(states, **kw)
@@ -183,3 +193,72 @@ def test_lds_epochs_and_taped_adjoint(gpu):
         np.testing.assert_array_equal(bits(ga[s]), bits(gb[s]))
     a.deriv_end()
     b.deriv_end()
+
+
+def test_every_sweep_shape_predicts_identically(gpu):
+    """Every tile shape of the prediction sweep gives the same event slots
+    and calendar, bit for bit: unit-diameter disks at phi 0.7 and 0.8, the
+    reference CLI's radius 1, a polydisperse system and a box three cells
+    wide, after 500 events; and boxes two cells wide (the halo wraps onto
+    the tile itself) at the start.  Events in two-cell boxes are left out:
+    the reference's own dynamics break down there (the oracle reaches
+    negative times and overlaps within a few hundred events)."""
+    moving = [F.lattice_state(2000, 0.7, seed=61, lattice="fit"),
+              F.lattice_state(3000, 0.8, seed=62, lattice="fit"),
+              F.lattice_state(900, 0.5, seed=63, rad=1.0, lattice="fit"),
+              polydisperse(700, 64),
+              F.lattice_state(20, 0.4, seed=4, lattice="fit")]
+    still = [F.lattice_state(12, 0.3, seed=2, lattice="fit"),
+             F.lattice_state(14, 0.35, seed=6, lattice="fit"),
+             F.lattice_state(3000, 0.8, seed=67, lattice="fit")]
+    ref = None
+    for shape in range(8):
+        got = []
+        for states, nev in ((moving, 500), (still, 0)):
+            b = gpu.Batch(states)
+            if nev:
+                b.run(max_events=nev)
+            b.set_sweep_shape(shape)
+            sh = b.sweep_shape()
+            assert sh["shape"] == shape and sh["tile_x"] > 0 and sh["threads"] % 64 == 0
+            b.sweep_predict()
+            got += [(b.events(s), b.calendar(s)) for s in range(len(states))]
+        if ref is None:
+            ref = got
+            continue
+        for s in range(len(got)):
+            for part in (0, 1):
+                for k in ref[s][part]:
+                    np.testing.assert_array_equal(got[s][part][k], ref[s][part][k],
+                                                  err_msg="shape %d system %d %s" % (shape, s, k))
+    with pytest.raises(gpu.EdmdError):
+        b.set_sweep_shape(8)
+
+
+def test_autotune_restores_state(gpu):
+    """The autotuner times every sweep shape and both residencies on the
+    batch itself; afterwards the batch continues exactly as an untuned twin
+    (traces, states and calendars bit for bit)."""
+    states = small_states(120, 5)
+    a = gpu.Batch(states, trace_capacity=8000)
+    b = gpu.Batch(states, trace_capacity=8000)
+    a.run(max_events=3000)
+    b.run(max_events=3000)
+    rep = a.autotune(sweep_reps=3, trial_events=1500)
+    print("autotune report", rep, flush=True)
+    assert all(ms > 0 for ms in rep["sweep_ms"])
+    assert rep["loop_ms_hbm"] > 0 and rep["loop_ms_lds"] > 0
+    assert rep["sweep_shape"] == a.sweep_shape()["shape"]
+    assert rep["in_lds"] == a.residency()["lds"]
+    assert rep["sweep_ms"][rep["sweep_shape"]] == min(rep["sweep_ms"])
+    assert_batches_equal(a, b, len(states))
+    a.run(max_events=8000)
+    b.run(max_events=8000)
+    assert_batches_equal(a, b, len(states))
+    a.sweep_predict()
+    b.sweep_predict()
+    assert_batches_equal(a, b, len(states), traces=False)
+    # a system too large for LDS: sweep shapes only
+    big = gpu.Batch([F.lattice_state(5000, 0.75, seed=8, lattice="fit")])
+    rep = big.autotune(sweep_reps=2, trial_events=1000)
+    assert rep["loop_ms_hbm"] < 0 and rep["loop_ms_lds"] < 0 and not rep["in_lds"]
