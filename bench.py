@@ -32,6 +32,12 @@ Reported beside it, in the same run:
                replica are timed separately;
   cpu_baseline the reference's own CPU event loop (built with its shipped
                flags) on a bounded sample of replicas, one core.
+
+Before each leg's timed region every batch is autotuned
+(edmd_batch_autotune: each sweep tile shape timed, and for batches whose
+systems fit one workgroup's LDS a trial run from HBM and from LDS; state
+saved and restored); --no-tune keeps the defaults.  The choices are
+reported under "tuning".
 """
 import argparse
 import importlib.util
@@ -140,6 +146,22 @@ def cpu_baseline(E, ns, phis, base_seed, coll, seconds):
                       "%d collisions each, one process" % (k, coll)}
 
 
+def tune(b, args, sweep_reps, trial_events):
+    """Autotunes batch b unless --no-tune; returns what was kept."""
+    if args.no_tune:
+        rep = None
+    else:
+        rep = b.autotune(sweep_reps=sweep_reps, trial_events=trial_events)
+    sh = b.sweep_shape()
+    out = {"sweep_tile": [sh["tile_x"], sh["tile_y"]], "sweep_threads": sh["threads"],
+           "lds_resident": b.residency()["lds"]}
+    if rep is not None:
+        out["sweep_ms_by_shape"] = [round(x, 5) for x in rep["sweep_ms"]]
+        if rep["loop_ms_hbm"] >= 0:
+            out["trial_ms_hbm_lds"] = [round(rep["loop_ms_hbm"], 3), round(rep["loop_ms_lds"], 3)]
+    return out
+
+
 def adjoint_leg(E, args):
     """Forward run with checkpoints, adjoint and tangent passes with
     recomputation, then the whole-tape variant, on the same replicas."""
@@ -156,6 +178,7 @@ def adjoint_leg(E, args):
            "checkpoint_every": args.adj_every}
     for mode, every, cap in (("checkpointed", args.adj_every, 0), ("whole_tape", 0, ncoll)):
         b = E.Batch(states)
+        res.setdefault("tuning", tune(b, args, 2, 2000))
         b.deriv_begin(checkpoint_every=every, tape_capacity=cap)
         t0 = time.perf_counter()
         b.deriv_run(max_collisions=ncoll, max_events=1 << 62)
@@ -203,6 +226,7 @@ def dense_leg(E, args):
                      "collisions_per_s": gcol / (t1 - t0), "events_per_s": gev / (t1 - t0)}
     log("dense growth: %d collisions in %.2f s = %.4e coll/s" % (gcol, t1 - t0, gcol / (t1 - t0)))
     b.run(tmax=1.0 / vr + 1e-6, max_events=1 << 62)  # GROWSTOP: radii reach their targets
+    res["tuning"] = tune(b, args, 3, 0)
     obs = [b.observables(k) for k in range(nrep)]
     c0 = sum(o["ncol"] for o in obs)
     e0 = sum(o["nevents"] for o in obs)
@@ -236,6 +260,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-collisions", type=int, default=100000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default sweep shapes and residency (no autotuning)")
     ap.add_argument("--adj-replicas", type=int, default=512)
     ap.add_argument("--adj-n", type=int, default=400)
     ap.add_argument("--adj-phi", type=float, default=0.60)
@@ -247,6 +273,9 @@ def main():
     ap.add_argument("--dense-collisions", type=int, default=200000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the 1 GiB copy used to calibrate memory counters")
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="only build the N = 2^20 system, tune it and time --sweep-reps sweeps "
+                         "(the workload of the sweep profiles)")
     args = ap.parse_args()
     if args.calibrate_copy:
         copy_calibration()
@@ -260,6 +289,12 @@ def main():
     if E.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.sweep_only:
+        big = E.Batch([E.init_lattice(args.plain_n, 0.80, rad=0.5, lattice=1, seed=args.seed)])
+        t = tune(big, args, 5, 0)
+        ms = big.time_sweep(args.sweep_reps)
+        print(json.dumps({"sweep_ms": ms, "n": args.plain_n, "tuning": t}), flush=True)
+        return
 
     # ---- batched replicas -------------------------------------------------
     ns, phis = replica_params(args.replicas, args.seed)
@@ -269,6 +304,7 @@ def main():
     b = E.Batch(states)
     log("batch of %d replicas (%d disks) built in %.1f s" % (len(states), int(ns.sum()),
                                                             time.perf_counter() - t0))
+    tuning = {"batched": tune(b, args, 2, 2000)}
     cps = args.coll_per_step
     for w in range(args.warmup):
         b.run(max_collisions=cps * (w + 1), max_events=1 << 62)
@@ -290,6 +326,7 @@ def main():
     # ---- plain single large system: sweep roofline and event loop ---------
     st2 = E.init_lattice(args.plain_n, 0.80, rad=0.5, lattice=1, seed=args.seed)
     big = E.Batch([st2])
+    tuning["plain"] = tune(big, args, 5, 0)
     sweep_ms = big.time_sweep(args.sweep_reps)
     sweep_bytes = SWEEP_BYTES_PER_DISK * args.plain_n
     sweep_gbs = sweep_bytes / (sweep_ms * 1e-3) / 1e9
@@ -343,7 +380,9 @@ def main():
         "plain_collisions_per_s": plain,
         "plain_events_per_s": plain_ev,
         "batched_over_plain": value / plain,
-        "sweep": {"n": args.plain_n, "ms": sweep_ms, "GB_per_s": sweep_gbs},
+        "sweep": {"n": args.plain_n, "ms": sweep_ms, "GB_per_s": sweep_gbs,
+                  "tile": tuning["plain"]["sweep_tile"]},
+        "tuning": tuning,
         "roofline": {"kernel": "k_sweep_tiled", "bound": "hbm", "achieved": sweep_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS,
                      "traffic": sweep_traffic() if args.plain_n == 1 << 20 else None},

@@ -655,7 +655,8 @@ extern "C" int edmd_batch_autotune(edmd_batch* b, int sweep_reps, int64_t trial_
     int rc = e == hipSuccess ? ckpt_copy(b, snap, true) : fail(EDMD_ERR_HIP, "hipEventCreate failed");
     float ms = 0;
     if (!rc && sweep_reps > 0) {
-        int best = b->sweep_variant;
+        const int cur = b->sweep_variant;
+        int best = cur;
         double bt = 1e300;
         for (int v = 0; v < EDMD_SWEEP_VARIANTS && e == hipSuccess; v++) {
             set_sweep_variant(b, v);
@@ -674,6 +675,9 @@ extern "C" int edmd_batch_autotune(edmd_batch* b, int sweep_reps, int64_t trial_
                 }
             }
         }
+        /* the shape in use stays unless another is more than 3% faster:
+         * shapes within timing noise of each other do not flip */
+        if (e == hipSuccess && r.sweep_ms[cur] > 0 && bt > 0.97 * r.sweep_ms[cur]) best = cur;
         set_sweep_variant(b, best);
         if (e != hipSuccess) rc = fail(EDMD_ERR_HIP, "sweep trial: %s", hipGetErrorString(e));
     }

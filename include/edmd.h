@@ -244,7 +244,8 @@ int edmd_batch_get_sweep_shape(edmd_batch* b, int* shape, int* tile_x, int* tile
 /* Autotuner, no counterpart in the reference: times sweep_reps sweeps in
  * every tile shape and, when every system fits LDS, a trial of
  * trial_events events per system in each residency (EDMD_RES_HBM and
- * EDMD_RES_LDS), then keeps the fastest of each.  The batch's state is
+ * EDMD_RES_LDS), then keeps the fastest of each (the sweep shape in use
+ * stays unless another is more than 3% faster).  The batch's state is
  * saved before and restored after every trial, so results are unchanged.
  * sweep_reps = 0 or trial_events = 0 skips that part.  Not while taping. */
 typedef struct edmd_tune_report {

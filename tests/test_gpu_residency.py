@@ -250,7 +250,8 @@ def test_autotune_restores_state(gpu):
     assert rep["loop_ms_hbm"] > 0 and rep["loop_ms_lds"] > 0
     assert rep["sweep_shape"] == a.sweep_shape()["shape"]
     assert rep["in_lds"] == a.residency()["lds"]
-    assert rep["sweep_ms"][rep["sweep_shape"]] == min(rep["sweep_ms"])
+    # the kept shape is the fastest, or the previous one within 3% of it
+    assert rep["sweep_ms"][rep["sweep_shape"]] <= min(rep["sweep_ms"]) / 0.97
     assert_batches_equal(a, b, len(states))
     a.run(max_events=8000)
     b.run(max_events=8000)
