@@ -1,8 +1,9 @@
 /*
  * HIP kernels of libedmd_hip.so for gfx950 (CDNA4, 64-lane wavefronts).
  *
- *   k_sweep_tiled  prediction sweep: one workgroup per 8x8-cell tile stages
- *                  the tile and its halo in LDS; one lane per particle
+ *   k_sweep_tiled  prediction sweep: one workgroup per tile of cells (eight
+ *                  tile shapes) stages the tile and its halo in LDS; one
+ *                  lane per particle
  *                  computes the next cell crossing and the earliest
  *                  collision partner over the 3x3 cell neighbourhood
  *                  (EDMD.c:2405-2482, 2661-2723, 2829-3003; invoked for all
@@ -18,6 +19,8 @@
  *                  4740-4800, 4828-4923) and re-predicts with all 64 lanes;
  *                  instantiated for the calendar depth and for growing
  *                  systems (the Grow dispatch of EDMD.c:1974-1990)
+ *   k_run_lds      the same loop with each system's arrays copied into its
+ *                  workgroup's LDS for the duration of the launch
  *   k_epoch_begin, k_epoch_end
  *                  per-system bounds and counters of graph-captured epochs
  *   k_peek, k_pair_time, k_repredict, k_fly_one
