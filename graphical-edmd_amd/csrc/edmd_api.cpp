@@ -60,6 +60,10 @@ extern "C" void edmd_default_desc(edmd_system_desc* d) {
     d->temperature = 1;
     d->vr = 0;
     d->einit = 1;
+    d->wall_x = 0;
+    d->wall_y = 0;
+    d->res_wall = 1; /* resW, EDMD.c:372 */
+    d->res_tan = 0;  /* resT, EDMD.c:373 */
 }
 
 /* ------------------------------------------------------------ handles */
@@ -124,6 +128,7 @@ struct edmd_batch {
     size_t lds_dyn[2] = {0, 0};
     int64_t lds_slots = 0;
     bool use_lds = false;
+    bool any_walls = false; /* some system has walls: the loop's wall variant */
 };
 
 struct edmd_ctx {
@@ -220,6 +225,17 @@ static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
     if (!(d->vr >= 0) || !(d->einit > 0))
         return fail(EDMD_ERR_ARG, "system %d: growth rate must be >= 0 and einit > 0", idx);
     c.einit = d->einit;
+    if ((d->wall_x != 0 && d->wall_x != 1) || (d->wall_y != 0 && d->wall_y != 1))
+        return fail(EDMD_ERR_ARG, "system %d: wall flags must be 0 or 1", idx);
+    if ((d->wall_x || d->wall_y) && d->vr > 0)
+        return fail(EDMD_ERR_ARG, "system %d: walls need a start without growth (vr = 0)", idx);
+    if (!(d->res_wall >= 0) || !(d->res_tan == d->res_tan))
+        return fail(EDMD_ERR_ARG, "system %d: bad wall restitution", idx);
+    c.walls = (d->wall_x ? 1 : 0) | (d->wall_y ? 2 : 0);
+    c.res_wall = d->res_wall;
+    c.res_tan = d->res_tan;
+    c.mihx = d->wall_x ? __builtin_huge_val() : c.halfLx;
+    c.mihy = d->wall_y ? __builtin_huge_val() : c.halfLy;
     c.dtp = 5.0 / (double)d->n; /* dtPaul, EDMD.c:712-714 */
     c.nlists = d->n;
     c.res = d->res;
@@ -424,6 +440,7 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
     if (!rc) rc = dalloc(b, &b->erank, 2 * np);
     if (!rc) rc = dalloc(b, &b->lrank, np);
     for (int s = 0; s < nsys; s++) b->any_grow |= descs[s].vr > 0;
+    for (int s = 0; s < nsys; s++) b->any_walls |= b->hs[s].c.walls != 0;
     if (!rc && b->any_grow) rc = dalloc(b, &b->vr, np);
     if (rc) { batch_free(b); return rc; }
     if (hipStreamCreate(&b->stream) != hipSuccess) {
@@ -555,7 +572,8 @@ static int choose_residency(edmd_batch* b) {
         for (int g = 0; g < (b->any_grow ? 2 : 1) && fit; g++) {
             int per_cu = 0;
             size_t stat = 0;
-            hipError_t e = edmd_lds_occupancy(b->max_nlev, g, b->lds_dyn[g], &per_cu, &stat);
+            hipError_t e = edmd_lds_occupancy(b->max_nlev, g, b->any_walls ? 1 : 0, b->lds_dyn[g],
+                                              &per_cu, &stat);
             if (e != hipSuccess || per_cu < 1 || stat + b->lds_dyn[g] > (size_t)maxblk) {
                 (void)hipGetLastError();
                 fit = false;
@@ -579,10 +597,11 @@ static int choose_residency(edmd_batch* b) {
 
 /* One launch of the event loop over every system (grow: the growing ones). */
 static hipError_t launch_loop(edmd_batch* b, int grow, const EdmdRunArgs& ra) {
+    const int walls = b->any_walls ? 1 : 0;
     if (b->use_lds)
-        return edmd_launch_run_lds(b->d_sys, b->nsys, b->max_nlev, grow, b->lds_dyn[grow ? 1 : 0],
-                                   ra, b->stream);
-    return edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, grow, ra, b->stream);
+        return edmd_launch_run_lds(b->d_sys, b->nsys, b->max_nlev, grow, walls,
+                                   b->lds_dyn[grow ? 1 : 0], ra, b->stream);
+    return edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, grow, walls, ra, b->stream);
 }
 
 extern "C" int edmd_batch_set_residency(edmd_batch* b, int mode) {
@@ -1564,6 +1583,9 @@ extern "C" int edmd_batch_deriv_begin(edmd_batch* b, int64_t checkpoint_every, i
                                       "velocity-rescale thermostat", s);
     if (b->any_grow)
         return fail(EDMD_ERR_ARG, "derivatives need systems whose growth has ended");
+    for (int s = 0; s < b->nsys; s++)
+        if (b->hs[s].c.walls)
+            return fail(EDMD_ERR_ARG, "system %d: derivatives need a system without walls", s);
     int rc = edmd_batch_deriv_end(b);
     if (rc) return rc;
     if ((double)b->nsys * (double)cap * sizeof(EdmdTapeRec) > 1.0e12)

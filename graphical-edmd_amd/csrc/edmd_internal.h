@@ -23,17 +23,18 @@ hipError_t edmd_launch_epoch_begin(const EdmdSys* d_sys, int nsys, int64_t ev_in
                                    uint64_t coll_inc, hipStream_t s);
 hipError_t edmd_launch_epoch_end(const EdmdSys* d_sys, int nsys, EdmdEpochRow* rows,
                                  hipStream_t s);
-hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, int grow, EdmdRunArgs ra,
+hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, int grow, int walls,
+                           EdmdRunArgs ra,
                            hipStream_t s);
 /* Event loop with each system's arrays resident in LDS (calendar depth <= 2):
  * dyn_bytes of dynamic LDS per workgroup, at least edmd_lds_layout's total
  * for every system of the launch. */
-hipError_t edmd_launch_run_lds(const EdmdSys* d_sys, int nsys, int max_nlev, int grow,
+hipError_t edmd_launch_run_lds(const EdmdSys* d_sys, int nsys, int max_nlev, int grow, int walls,
                                size_t dyn_bytes, EdmdRunArgs ra, hipStream_t s);
 /* Workgroups of the LDS-resident loop one CU holds with dyn_bytes of dynamic
  * LDS each, and the loop's static LDS. */
-hipError_t edmd_lds_occupancy(int max_nlev, int grow, size_t dyn_bytes, int* blocks_per_cu,
-                              size_t* static_bytes);
+hipError_t edmd_lds_occupancy(int max_nlev, int grow, int walls, size_t dyn_bytes,
+                              int* blocks_per_cu, size_t* static_bytes);
 hipError_t edmd_launch_sweep_grow(const EdmdSys* d_sys, int nsys, int max_n, hipStream_t s);
 /* counts the 2N insertions of a full sweep (after the sweep kernels) */
 hipError_t edmd_launch_sweep_count(const EdmdSys* d_sys, int nsys, hipStream_t s);

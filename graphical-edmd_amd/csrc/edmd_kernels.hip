@@ -149,10 +149,54 @@ __device__ __forceinline__ int wave_argmin_lane(double v, double base, double& v
 
 /* PBC, EDMD.c:5896-5913 */
 __device__ __forceinline__ void min_image(const EdmdConst& C, double& dx, double& dy) {
-    if (dx >= C.halfLx) dx -= C.Lx;
-    else if (dx < -C.halfLx) dx += C.Lx;
-    if (dy >= C.halfLy) dy -= C.Ly;
-    else if (dy < -C.halfLy) dy += C.Ly;
+    /* mihx / mihy are halfLx / halfLy, or +inf across walls (no image) */
+    if (dx >= C.mihx) dx -= C.Lx;
+    else if (dx < -C.mihx) dx += C.Lx;
+    if (dy >= C.mihy) dy -= C.Ly;
+    else if (dy < -C.mihy) dy += C.Ly;
+}
+
+/* The wall part of collisionEventNormal (EDMD.c:2845-2895, no field, mid or
+ * circular wall; logTime is the identity without damping): a disk in a
+ * boundary cell moving towards the wall gets the time to contact, the
+ * starting value of the strict-less minimum over the neighbours.  The x
+ * test sets xy = 0 whenever it applies, also when the y wall stays the
+ * earlier, as the reference does.  Returns whether a wall time was set. */
+__device__ __forceinline__ bool wall_time(const EdmdConst& C, double x, double y, double vx,
+                                          double vy, double r, int cx, int cy, double& dt,
+                                          int& xy) {
+    bool wall = false;
+    dt = EDMD_NEVER;
+    xy = 2;
+    if (C.walls & 2) {
+        if (cy == 0 && vy < 0) {
+            dt = -(y - r) / vy;
+            wall = true;
+            xy = 1;
+        } else if (cy == C.ncy - 1 && vy > 0) {
+            dt = (C.Ly - y - r) / vy;
+            wall = true;
+            xy = 1;
+        }
+    }
+    if (C.walls & 1) {
+        if (cx == 0 && vx < 0) {
+            const double dw = -(x - r) / vx;
+            if (dt > dw) {
+                wall = true;
+                dt = dw;
+            }
+            xy = 0;
+        } else if (cx == C.ncx - 1 && vx > 0) {
+            const double dw = (C.Lx - x - r) / vx;
+            if (dt > dw) {
+                wall = true;
+                dt = dw;
+            }
+            xy = 0;
+        }
+    }
+    return wall;
 }
 
 /* PBCinsideCellX/Y, EDMD.c:5922-5936 */
@@ -334,7 +378,7 @@ __device__ __forceinline__ bool before(double ka, int64_t ra, double kb, int64_t
  * in the reference's order, y offset outer, x offset inner, newest list
  * entry first (EDMD.c:2959-3003).  Returns the relative time; NEVER and
  * partner 0 when no candidate qualifies (EDMD.c:2830-2831). */
-template <bool G, class AR>
+template <bool G, bool W = true, class AR>
 __device__ __forceinline__ double predict_seq(const EdmdConst& C, const AR& A, double t,
                                               int p, const EdmdPart& o, int& partner,
                                               bool& overlap) {
@@ -342,6 +386,9 @@ __device__ __forceinline__ double predict_seq(const EdmdConst& C, const AR& A, d
     double dt = G ? EDMD_GROW_NONE : EDMD_NEVER;
     int best = 0;
     const double ovr = G ? A.vr[p] : 0.0;
+    int xy = 2;
+    bool wall = false;
+    if (W && !G && C.walls) wall = wall_time(C, o.x, o.y, o.vx, o.vy, o.rad, o.cx, o.cy, dt, xy);
     for (int j = -1; j <= 1; ++j) {
         int row = wrap_cell(o.cy + j, C.ncy) * C.ncx;
         for (int k = -1; k <= 1; ++k) {
@@ -362,11 +409,12 @@ __device__ __forceinline__ double predict_seq(const EdmdConst& C, const AR& A, d
                 if (dt > tt) {
                     dt = tt;
                     best = q;
+                    wall = false;
                 }
             }
         }
     }
-    partner = best;
+    partner = wall ? EDMD_WALL_CODE(xy) : best;
     return dt;
 }
 
@@ -637,7 +685,7 @@ __device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const 
 
 /* The staged scan where the cell capacity allows it, else the gather scan
  * from HBM; both give the reference's result bit for bit. */
-template <bool G, class AR>
+template <bool G, bool W = true, class AR>
 __device__ __forceinline__ void predict_owners(const EdmdConst& C, const AR& A, int lane,
                                                double t, int nown, const Owner& a, const Owner& b,
                                                double& dta, int& pa, uint32_t& ca, double& dtb,
@@ -648,6 +696,23 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const AR& A, 
                                  stage);
     else
         predict_owners_gather<G>(C, A, lane, t, nown, a, b, dta, pa, ca, dtb, pb, cb, overlap);
+    if (W && !G && C.walls) {
+        /* the wall time starts the reference's minimum: a neighbour
+         * replaces it only when strictly earlier */
+        double dw;
+        int xy;
+        if (wall_time(C, a.x, a.y, a.vx, a.vy, a.r, a.cx, a.cy, dw, xy) && !(dta < dw)) {
+            dta = dw;
+            pa = EDMD_WALL_CODE(xy);
+            ca = 0;
+        }
+        if (nown == 2 && wall_time(C, b.x, b.y, b.vx, b.vy, b.r, b.cx, b.cy, dw, xy) &&
+            !(dtb < dw)) {
+            dtb = dw;
+            pb = EDMD_WALL_CODE(xy);
+            cb = 0;
+        }
+    }
 }
 
 /* ------------------------------------------------------------ calendar */
@@ -884,7 +949,7 @@ __device__ __forceinline__ void node_min(const EdmdConst& C, const AR& A, int l,
  * (EDMD.c:4770-4776: every counter goes 0 -> 1 in particle order, each disk
  * predicting right after its own increment), i.e. 1 for a partner of lower
  * or equal index, else 0. */
-template <bool G, class AR>
+template <bool G, bool W = true, class AR>
 __device__ __forceinline__ bool sweep_one(const EdmdConst& C, const AR& A, double t, int p,
                                           double paul, int64_t ctr_cross, int64_t ctr_coll,
                                           bool stop_grow = false) {
@@ -894,13 +959,13 @@ __device__ __forceinline__ bool sweep_one(const EdmdConst& C, const AR& A, doubl
     cross_time(C, t, o.x, o.y, o.vx, o.vy, o.cx, o.cy, tc, dir);
     bool ov = false;
     int partner;
-    double dt = predict_seq<G>(C, A, t, p, o, partner, ov);
+    double dt = predict_seq<G, W>(C, A, t, p, o, partner, ov);
     double tk = t + dt;
     EdmdEv e;
     e.tcoll = tk;
     e.tcross = tc;
     e.partner = partner;
-    e.cact = stop_grow ? (partner <= p ? 1u : 0u) : A.p[partner].coll;
+    e.cact = stop_grow ? (partner <= p ? 1u : 0u) : (partner >= 0 ? A.p[partner].coll : 0u);
     e.dir = dir;
     e.pad = 0;
     A.e[p] = e;
@@ -1068,6 +1133,9 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
         uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
         double dt = EDMD_NEVER;
         int best = -1;
+        int wxy = 2;
+        bool wall = false;
+        if (C.walls) wall = wall_time(C, ox, oy, ovx, ovy, orad, tx0 + lx, ty0 + ly, dt, wxy);
         for (int k = 0; k < ktot; ++k) {
             if (k >= tot) continue;
             const int v = k + (k < c1 ? o0 : (k < c2 ? o1 : o2));
@@ -1115,12 +1183,12 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
         int dir;
         cross_time(C, t, ox, oy, ovx, ovy, tx0 + lx, ty0 + ly, tc, dir);
         const double tk = t + dt;
-        const int partner = best >= 0 ? L.id[best] : 0;
+        const int partner = best >= 0 ? L.id[best] : (wall ? EDMD_WALL_CODE(wxy) : 0);
         EdmdEv e;
         e.tcoll = tk;
         e.tcross = tc;
         e.partner = partner;
-        e.cact = A.p[partner].coll;
+        e.cact = partner >= 0 ? A.p[partner].coll : 0u;
         e.dir = dir;
         e.pad = 0;
         A.e[p] = e;
@@ -1549,8 +1617,17 @@ __device__ void ev_thermo(const EdmdConst& C, const AR& A, int lane, LoopRegs& R
             row.pxy = -1 * v2 / (area * dT) + exy / area;
             row.pyx = -1 * v3 / (area * dT) + eyx / area;
             row.pressure = (-1 / (dT)) * (v0 + v1) / (2 * area) + E / area;
+            row.pwall = 0;
+            if (C.walls) {
+                /* EDMD.c:5428-5451: dp / (deltaTime * perimeter) */
+                double perimeter = 0;
+                if (C.walls & 1) perimeter += 2 * C.Ly;
+                if (C.walls & 2) perimeter += 2 * C.Lx;
+                row.pwall = st->dpw / (dT * perimeter);
+            }
             A.thermo[k] = row;
         }
+        if (C.walls) st->dpw = 0;
         st->nthermo = k + 1;
         st->last_thermo = t;
         st->vir[0] = st->vir[1] = st->vir[2] = st->vir[3] = 0;
@@ -1570,7 +1647,7 @@ __device__ void rebuild_tree(const EdmdConst& C, const AR& A, int lane) {
     }
 }
 
-template <bool G, class AR>
+template <bool G, bool W, class AR>
 __device__ void ev_rescale(const EdmdConst& C, const AR& A, int lane, LoopRegs& R) {
     /* addNoise, velocity-rescale branch (noise == 2): EDMD.c:4828-4832,
      * 4880-4915, 4922 */
@@ -1593,7 +1670,7 @@ __device__ void ev_rescale(const EdmdConst& C, const AR& A, int lane, LoopRegs& 
     bool ov = false;
     /* per particle: crossing, then collision (EDMD.c:4909-4915) */
     for (int q = lane; q < C.n; q += 64)
-        ov |= sweep_one<G>(C, A, t, q, R.q->paul, R.q->ctr + 2 * (int64_t)q, R.q->ctr + 2 * (int64_t)q + 1);
+        ov |= sweep_one<G, W>(C, A, t, q, R.q->paul, R.q->ctr + 2 * (int64_t)q, R.q->ctr + 2 * (int64_t)q + 1);
     R.q->ctr += 2 * (int64_t)C.n;
     if (__ballot(ov) != 0ull) R.status = EDMD_DEV_OVERLAP;
     WAVE_SYNC();
@@ -1616,7 +1693,7 @@ __device__ __forceinline__ void flag_overlap(LoopRegs& R, bool ov, int p) {
     }
 }
 
-template <int ML, class AR>
+template <int ML, bool W, class AR>
 __device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs& R, int i,
                              int j) {
     /* doTheCollisionNormal, EDMD.c:3542-3562, 3583-3606, 3802-3827, 3869-3879 */
@@ -1652,7 +1729,7 @@ __device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs
         double dt, dtb;
         int pa, pb;
         uint32_t cpa, cpb;
-        predict_owners<false>(C, A, lane, t, 1, oi, oi, dt, pa, cpa, dtb, pb, cpb, ov, R.q->cand);
+        predict_owners<false, W>(C, A, lane, t, 1, oi, oi, dt, pa, cpa, dtb, pb, cpb, ov, R.q->cand);
         flag_overlap(R, ov, i);
         const double tk = t + dt;
         const double tci = uni(ei.tcross);
@@ -1718,7 +1795,7 @@ __device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs
     double dti, dtj;
     int pi_, pj_;
     uint32_t cpi, cpj;
-    predict_owners<false>(C, A, lane, t, 2, oi, oj, dti, pi_, cpi, dtj, pj_, cpj, ov, R.q->cand);
+    predict_owners<false, W>(C, A, lane, t, 2, oi, oj, dti, pi_, cpi, dtj, pj_, cpj, ov, R.q->cand);
     flag_overlap(R, ov, i);
     double tci2, tcj2;
     int di, dj;
@@ -1925,21 +2002,29 @@ __device__ void ev_stop_grow(const EdmdConst& C, const AR& A, int lane, LoopRegs
     WAVE_SYNC();
 }
 
-template <int ML, bool G, class AR>
-__device__ void ev_crossing(const EdmdConst& C, const AR& A, int lane, LoopRegs& R, int p,
-                            int d) {
-    /* doTheCrossing, EDMD.c:2501-2590 (its freeFly is the dispatch pointer:
-     * freeFlyGrow while growing) */
+template <int ML, bool G, bool W, class AR>
+__device__ void ev_single(const EdmdConst& C, const AR& A, int lane, LoopRegs& R, int p,
+                          int d) {
+    /* The events that change one disk.  d = 1-4: a cell crossing in that
+     * direction (doTheCrossing, EDMD.c:2501-2590; its freeFly is the
+     * dispatch pointer, freeFlyGrow while growing).  d = 5 + xy: a wall hit
+     * (Normal dynamics only; doTheWallNormal, EDMD.c:3384-3430, 3454-3460,
+     * without mid, thermal or shear walls): counted as a collision, the
+     * momentum given to the wall accumulated for the thermo row, the normal
+     * velocity reversed times resW and the tangential one scaled by resT.
+     * Both then predict the crossing slot and the collision slot again, in
+     * that order. */
+    const bool wall = W && !G && d >= 5;
     const EdmdPart pp = A.p[p];
     const double gvr = G ? uni(A.vr[p]) : 0.0;
     PathCache<ML> P;
     path_prefetch<ML>(C, A, lane, p, -1, P);
     const double t = R.t;
     double x = uni(pp.x), y = uni(pp.y), tl = uni(pp.tl);
-    const double vx = uni(pp.vx), vy = uni(pp.vy);
+    double vx = uni(pp.vx), vy = uni(pp.vy);
     double r = uni(pp.rad);
     int cx = uni((int)pp.cx), cy = uni((int)pp.cy);
-    trace_push(C, A, lane, t, EDMD_EV_CELLCROSS, p, d, 1);
+    trace_push(C, A, lane, t, wall ? EDMD_EV_WALL : EDMD_EV_CELLCROSS, p, wall ? d - 5 : d, 1);
     if (G) {
         fly_grow(C, t, x, y, vx, vy, tl, r, gvr);
         r = uni(r);
@@ -1949,67 +2034,98 @@ __device__ void ev_crossing(const EdmdConst& C, const AR& A, int lane, LoopRegs&
     x = uni(x);
     y = uni(y);
 
-    /* removeFromCell: drop p, keeping the order of the others */
-    const int c0 = cy * C.ncx + cx;
-    const int n0 = A.ccnt[c0];
-    int v = (lane < n0) ? A.cslot[(size_t)c0 * C.cap + lane] : -1;
-    uint64_t hit = __ballot(v == p);
-    int pos = hit ? (int)__builtin_ctzll(hit) : n0;
-    if (lane > pos && lane < n0) A.cslot[(size_t)c0 * C.cap + lane - 1] = v;
-    if (lane == 0) A.ccnt[c0] = n0 - 1;
-
-    int wx = 0, wy = 0; /* periodic image change */
-    switch (d) {
-        case 1:
-            if (cx - 1 == -1) { cx = C.ncx - 1; wx = -1; } else cx -= 1;
-            break;
-        case 2:
-            if (cx + 1 == C.ncx) { cx = 0; wx = 1; } else cx += 1;
-            break;
-        case 3:
-            if (cy - 1 == -1) { cy = C.ncy - 1; wy = -1; } else cy -= 1;
-            break;
-        default:
-            if (cy + 1 == C.ncy) { cy = 0; wy = 1; } else cy += 1;
-            break;
-    }
-    WAVE_SYNC();
-    const int c1 = cy * C.ncx + cx;
-    const int n1 = A.ccnt[c1];
-    if (n1 >= C.cap) {
-        if (R.status == EDMD_DEV_OK) {
-            R.status = EDMD_DEV_CELL_OVERFLOW;
-            R.err_i = p;
-            R.err_j = c1;
+    if (wall) {
+        R.ncol++;
+        const double m = uni(pp.m);
+        double dp;
+        /* the tangential product is made +0.0 where it is -0.0 (resT = 0
+         * times a negative velocity): the reference's crossing prediction
+         * divides by a -0.0 velocity and flies the disk to t = -inf */
+        if (d == 5) {
+            dp = fabs((C.res_wall + 1) * m * vx);
+            vx = -C.res_wall * vx;
+            vy = C.res_tan * vy + 0.0;
+        } else {
+            dp = fabs((C.res_wall + 1) * m * vy);
+            vy = -C.res_wall * vy;
+            vx = C.res_tan * vx + 0.0;
         }
-        return;
+        vx = uni(vx);
+        vy = uni(vy);
+        if (lane == 0) {
+            A.st->dpw += dp;
+            const auto q = A.p + p;
+            q->x = x;
+            q->y = y;
+            q->tl = tl;
+            q->vx = vx;
+            q->vy = vy;
+            q->coll = pp.coll + 1;
+        }
+        WAVE_SYNC();
+    } else {
+        /* removeFromCell: drop p, keeping the order of the others */
+        const int c0 = cy * C.ncx + cx;
+        const int n0 = A.ccnt[c0];
+        int v = (lane < n0) ? A.cslot[(size_t)c0 * C.cap + lane] : -1;
+        uint64_t hit = __ballot(v == p);
+        int pos = hit ? (int)__builtin_ctzll(hit) : n0;
+        if (lane > pos && lane < n0) A.cslot[(size_t)c0 * C.cap + lane - 1] = v;
+        if (lane == 0) A.ccnt[c0] = n0 - 1;
+
+        int wx = 0, wy = 0; /* periodic image change */
+        switch (d) {
+            case 1:
+                if (cx - 1 == -1) { cx = C.ncx - 1; wx = -1; } else cx -= 1;
+                break;
+            case 2:
+                if (cx + 1 == C.ncx) { cx = 0; wx = 1; } else cx += 1;
+                break;
+            case 3:
+                if (cy - 1 == -1) { cy = C.ncy - 1; wy = -1; } else cy -= 1;
+                break;
+            default:
+                if (cy + 1 == C.ncy) { cy = 0; wy = 1; } else cy += 1;
+                break;
+        }
+        WAVE_SYNC();
+        const int c1 = cy * C.ncx + cx;
+        const int n1 = A.ccnt[c1];
+        if (n1 >= C.cap) {
+            if (R.status == EDMD_DEV_OK) {
+                R.status = EDMD_DEV_CELL_OVERFLOW;
+                R.err_i = p;
+                R.err_j = c1;
+            }
+            return;
+        }
+        if (lane == 0) {
+            A.cslot[(size_t)c1 * C.cap + n1] = p; /* newest entry: visited first */
+            A.ccnt[c1] = n1 + 1;
+            const auto q = A.p + p;
+            q->x = x;
+            q->y = y;
+            q->tl = tl;
+            if (G) q->rad = r;
+            q->cx = (int16_t)cx;
+            q->cy = (int16_t)cy;
+            if (wx) A.img[2 * p] += wx;
+            if (wy) A.img[2 * p + 1] += wy;
+        }
+        WAVE_SYNC();
     }
-    if (lane == 0) {
-        A.cslot[(size_t)c1 * C.cap + n1] = p; /* newest entry: visited first */
-        A.ccnt[c1] = n1 + 1;
-        const auto q = A.p + p;
-        q->x = x;
-        q->y = y;
-        q->tl = tl;
-        if (G) q->rad = r;
-        q->cx = (int16_t)cx;
-        q->cy = (int16_t)cy;
-        if (wx) A.img[2 * p] += wx;
-        if (wy) A.img[2 * p + 1] += wy;
-    }
-    WAVE_SYNC();
     const Owner o{p, x, y, vx, vy, r, cx, cy, gvr};
     bool ov = false;
     double dt, dtb;
     int pa, pb;
     uint32_t cpa, cpb;
-    predict_owners<G>(C, A, lane, t, 1, o, o, dt, pa, cpa, dtb, pb, cpb, ov, R.q->cand);
+    predict_owners<G, W>(C, A, lane, t, 1, o, o, dt, pa, cpa, dtb, pb, cpb, ov, R.q->cand);
     flag_overlap(R, ov, p);
     double tc;
     int dir;
     cross_time(C, t, x, y, vx, vy, cx, cy, tc, dir);
     const double tk = t + dt;
-    /* crossingEvent(i), then collisionEvent(i), EDMD.c:2587-2589 */
+    /* crossingEvent(i), then collisionEvent(i), EDMD.c:2587-2589, 3458-3459 */
     const int64_t rx = take_rank(C, R, tc);
     const int64_t rk = take_rank(C, R, tk);
     if (lane == 0) {
@@ -2026,7 +2142,7 @@ __device__ void ev_crossing(const EdmdConst& C, const AR& A, int lane, LoopRegs&
  * calendar depth of every system in the launch.  A holds the system's arrays
  * in HBM (EdmdArrays) or in LDS (EdmdArraysL); the caller has checked that
  * the system's growth phase matches G. */
-template <int ML, bool G, class AR>
+template <int ML, bool G, bool W, class AR>
 __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRunArgs ra) {
     const int lane = threadIdx.x;
     EdmdState* st = A.st;
@@ -2076,7 +2192,7 @@ __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRu
                     ncross = 0;
                 }
             } else {
-                ev_rescale<G>(C, A, lane, R);
+                ev_rescale<G, W>(C, A, lane, R);
             }
             next_special(st, R);
             R.nevents++;
@@ -2088,12 +2204,18 @@ __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRu
         R.t = key;
         if (aux >= 0) {
             if (G) ev_collision_grow<ML>(C, A, lane, R, p, aux);
-            else ev_collision<ML>(C, A, lane, R, p, aux);
+            else ev_collision<ML, W>(C, A, lane, R, p, aux);
             last_type = EDMD_EV_COLLISION;
         } else {
-            ev_crossing<ML, G>(C, A, lane, R, p, -aux);
-            ncross++;
-            last_type = EDMD_EV_CELLCROSS;
+            /* a crossing (code -1..-4) or a wall hit (-5 - xy) */
+            const bool wall = W && !G && aux <= EDMD_WALL_CODE(0);
+            ev_single<ML, G, W>(C, A, lane, R, p, wall ? 5 + EDMD_WALL_CODE(0) - aux : -aux);
+            if (wall) {
+                last_type = EDMD_EV_WALL;
+            } else {
+                ncross++;
+                last_type = EDMD_EV_CELLCROSS;
+            }
         }
         R.nevents++;
         if (ra.stop_mask & (1 << last_type)) { reason = EDMD_STOP_SPECIAL; break; }
@@ -2119,22 +2241,22 @@ __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRu
 /* Systems of up to 64^2 disks (calendar depth <= 2): bounded to 128 VGPRs so
  * that four waves share each SIMD and a batch of 4096 replicas is resident
  * at once on 256 CUs. */
-template <int ML, bool G>
+template <int ML, bool G, bool W>
 __global__ __launch_bounds__(64, 4) void k_run_small(const EdmdSys* __restrict__ systems,
                                                      EdmdRunArgs ra) {
     const EdmdSys& S = systems[blockIdx.x];
     /* G launches run the systems that are growing, the others the rest */
     if ((uni(S.a.st->growing) != 0) != G) return;
-    run_loop<ML, G>(S.c, S.a, ra);
+    run_loop<ML, G, W>(S.c, S.a, ra);
 }
 
 /* Deeper calendars (large systems, typically one per launch): no register
  * bound, so the path caches stay in registers. */
-template <int ML, bool G>
+template <int ML, bool G, bool W>
 __global__ __launch_bounds__(64) void k_run(const EdmdSys* __restrict__ systems, EdmdRunArgs ra) {
     const EdmdSys& S = systems[blockIdx.x];
     if ((uni(S.a.st->growing) != 0) != G) return;
-    run_loop<ML, G>(S.c, S.a, ra);
+    run_loop<ML, G, W>(S.c, S.a, ra);
 }
 
 /* ----------------------------------------------------- LDS-resident systems */
@@ -2224,7 +2346,7 @@ __device__ __forceinline__ void lds_system(const EdmdConst& C, const EdmdArrays&
  * resident in LDS: they come in from HBM when the launch starts, the event
  * loop runs on them, and they go back when it stops.  The dynamic LDS of
  * the launch holds the largest system's arrays (edmd_lds_layout). */
-template <int ML, bool G>
+template <int ML, bool G, bool W>
 __global__ __launch_bounds__(64) void k_run_lds(const EdmdSys* __restrict__ systems,
                                                 EdmdRunArgs ra) {
     extern __shared__ __align__(64) unsigned char lds_pool[];
@@ -2259,7 +2381,7 @@ __global__ __launch_bounds__(64) void k_run_lds(const EdmdSys* __restrict__ syst
     WAVE_SYNC();
     lds_system<true, G>(C, H, A, lane);
     WAVE_SYNC();
-    run_loop<ML, G>(C, A, ra);
+    run_loop<ML, G, W>(C, A, ra);
     WAVE_SYNC();
     lds_system<false, G>(C, H, A, lane);
 }
@@ -2286,6 +2408,8 @@ __global__ __launch_bounds__(64) void k_peek(const EdmdSys* __restrict__ systems
         r.t = R.next_special; r.type = R.special; r.i = 0; r.j = 0; r.coll_actual = 0;
     } else if (aux >= 0) {
         r.t = key; r.type = EDMD_EV_COLLISION; r.i = p; r.j = aux; r.coll_actual = A.e[p].cact;
+    } else if (aux <= EDMD_WALL_CODE(0)) {
+        r.t = key; r.type = EDMD_EV_WALL; r.i = p; r.j = EDMD_WALL_CODE(0) - aux; r.coll_actual = 0;
     } else {
         r.t = key; r.type = EDMD_EV_CELLCROSS; r.i = p; r.j = -aux; r.coll_actual = 0;
     }
@@ -2517,12 +2641,13 @@ hipError_t edmd_launch_sync_all(const EdmdSys* d_sys, int nsys, int max_n, hipSt
     return hipGetLastError();
 }
 
-template <bool G>
+/* W: some system of the launch has walls (never together with growth). */
+template <bool G, bool W>
 static void launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, EdmdRunArgs ra, hipStream_t s) {
-    if (max_nlev <= 1) hipLaunchKernelGGL((k_run_small<1, G>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
-    else if (max_nlev == 2) hipLaunchKernelGGL((k_run_small<2, G>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
-    else if (max_nlev <= 4) hipLaunchKernelGGL((k_run<4, G>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
-    else hipLaunchKernelGGL((k_run<EDMD_MAX_LEVELS, G>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    if (max_nlev <= 1) hipLaunchKernelGGL((k_run_small<1, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    else if (max_nlev == 2) hipLaunchKernelGGL((k_run_small<2, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    else if (max_nlev <= 4) hipLaunchKernelGGL((k_run<4, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    else hipLaunchKernelGGL((k_run<EDMD_MAX_LEVELS, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
 }
 
 /* Epochs of a graph-captured run: the bounds of the coming epoch relative
@@ -2566,24 +2691,29 @@ hipError_t edmd_launch_epoch_end(const EdmdSys* d_sys, int nsys, EdmdEpochRow* r
     return hipGetLastError();
 }
 
-hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, int grow, EdmdRunArgs ra,
-                           hipStream_t s) {
-    if (grow) launch_run<true>(d_sys, nsys, max_nlev, ra, s);
-    else launch_run<false>(d_sys, nsys, max_nlev, ra, s);
+hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, int grow, int walls,
+                           EdmdRunArgs ra, hipStream_t s) {
+    if (grow) launch_run<true, false>(d_sys, nsys, max_nlev, ra, s);
+    else if (walls) launch_run<false, true>(d_sys, nsys, max_nlev, ra, s);
+    else launch_run<false, false>(d_sys, nsys, max_nlev, ra, s);
     return hipGetLastError();
 }
 
 /* The LDS-resident loop exists for calendars of depth <= 2 (up to 4096
  * disks, far more than one workgroup's LDS holds). */
-static const void* lds_kernel(int max_nlev, int grow) {
-    if (max_nlev <= 1) return grow ? (const void*)k_run_lds<1, true> : (const void*)k_run_lds<1, false>;
-    if (max_nlev == 2) return grow ? (const void*)k_run_lds<2, true> : (const void*)k_run_lds<2, false>;
+static const void* lds_kernel(int max_nlev, int grow, int walls) {
+    if (max_nlev <= 1)
+        return grow ? (const void*)k_run_lds<1, true, false>
+                    : walls ? (const void*)k_run_lds<1, false, true> : (const void*)k_run_lds<1, false, false>;
+    if (max_nlev == 2)
+        return grow ? (const void*)k_run_lds<2, true, false>
+                    : walls ? (const void*)k_run_lds<2, false, true> : (const void*)k_run_lds<2, false, false>;
     return nullptr;
 }
 
-hipError_t edmd_lds_occupancy(int max_nlev, int grow, size_t dyn_bytes, int* blocks_per_cu,
-                              size_t* static_bytes) {
-    const void* f = lds_kernel(max_nlev, grow);
+hipError_t edmd_lds_occupancy(int max_nlev, int grow, int walls, size_t dyn_bytes,
+                              int* blocks_per_cu, size_t* static_bytes) {
+    const void* f = lds_kernel(max_nlev, grow, walls);
     *blocks_per_cu = 0;
     *static_bytes = 0;
     if (!f) return hipErrorInvalidValue;
@@ -2596,16 +2726,19 @@ hipError_t edmd_lds_occupancy(int max_nlev, int grow, size_t dyn_bytes, int* blo
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, 64, dyn_bytes);
 }
 
-hipError_t edmd_launch_run_lds(const EdmdSys* d_sys, int nsys, int max_nlev, int grow,
+template <bool G, bool W>
+static void launch_lds(const EdmdSys* d_sys, int nsys, int max_nlev, size_t dyn_bytes,
+                       EdmdRunArgs ra, hipStream_t s) {
+    if (max_nlev <= 1) hipLaunchKernelGGL((k_run_lds<1, G, W>), dim3(nsys), dim3(64), dyn_bytes, s, d_sys, ra);
+    else hipLaunchKernelGGL((k_run_lds<2, G, W>), dim3(nsys), dim3(64), dyn_bytes, s, d_sys, ra);
+}
+
+hipError_t edmd_launch_run_lds(const EdmdSys* d_sys, int nsys, int max_nlev, int grow, int walls,
                                size_t dyn_bytes, EdmdRunArgs ra, hipStream_t s) {
     if (max_nlev > 2) return hipErrorInvalidValue;
-    if (grow) {
-        if (max_nlev <= 1) hipLaunchKernelGGL((k_run_lds<1, true>), dim3(nsys), dim3(64), dyn_bytes, s, d_sys, ra);
-        else hipLaunchKernelGGL((k_run_lds<2, true>), dim3(nsys), dim3(64), dyn_bytes, s, d_sys, ra);
-    } else {
-        if (max_nlev <= 1) hipLaunchKernelGGL((k_run_lds<1, false>), dim3(nsys), dim3(64), dyn_bytes, s, d_sys, ra);
-        else hipLaunchKernelGGL((k_run_lds<2, false>), dim3(nsys), dim3(64), dyn_bytes, s, d_sys, ra);
-    }
+    if (grow) launch_lds<true, false>(d_sys, nsys, max_nlev, dyn_bytes, ra, s);
+    else if (walls) launch_lds<false, true>(d_sys, nsys, max_nlev, dyn_bytes, ra, s);
+    else launch_lds<false, false>(d_sys, nsys, max_nlev, dyn_bytes, ra, s);
     return hipGetLastError();
 }
 

@@ -39,7 +39,13 @@
 #define EDMD_EV_SCREENSHOT 2
 #define EDMD_EV_THERMO 3
 #define EDMD_EV_ADDINGNOISE 4
+#define EDMD_EV_WALL 5
 #define EDMD_EV_GROWSTOP 9
+
+/* A collision slot holding a wall event: partner -5 - xy (xy = 0: wall
+ * normal to x, 1: normal to y); the leaf's event code is the same value,
+ * below the crossing codes -1..-4. */
+#define EDMD_WALL_CODE(xy) (-5 - (xy))
 
 /* collisionEventGrow's starting minimum (EDMD.c:3106): no partner found
  * within 1e7 time units gives partner 0 at t + 1e7 */
@@ -72,7 +78,10 @@ typedef struct EdmdConst {
     double einit; /* Einit: E/N after the growth stops (EDMD.c:369, 5759-5763) */
     double dtp;   /* dtPaul = 5/N, the reference's bucket width (EDMD.c:712-714) */
     int32_t nlists; /* bucket count of the reference's queue (N) */
-    int32_t pad_c;
+    int32_t walls;  /* bit 0: walls normal to x, bit 1: normal to y */
+    double res_wall, res_tan; /* resW, resT (EDMD.c:372-373) */
+    /* minimum-image half widths: halfLx / halfLy, +inf across walls */
+    double mihx, mihy;
 } EdmdConst;
 
 /* Order among events of equal time.  The reference pops equal times in the
@@ -112,6 +121,7 @@ typedef struct EdmdState {
     int64_t win;                /* buckets passed (paulTime = win additions of dtPaul) */
     int64_t win_screen, win_thermo, win_noise, win_grow; /* their buckets, -1 = overflow list */
     uint64_t coll_limit;        /* per-system bound on ncol (epochs) */
+    double dpw;                 /* momentum given to the walls since the last thermo row */
 } EdmdState;
 
 /* LDS-resident runs: byte offsets of a system's arrays in its workgroup's
@@ -185,6 +195,7 @@ typedef struct EdmdThermoRow {
     double t;
     uint64_t ncol;
     double e_per_n, pressure, pxx, pyy, pxy, pyx;
+    double pwall;
 } EdmdThermoRow;
 
 typedef struct __attribute__((aligned(64))) EdmdPart {

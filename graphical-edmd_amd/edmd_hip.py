@@ -18,6 +18,7 @@ HEADER_PATHS = [os.path.join(os.path.dirname(HERE), "include", "edmd.h"),
 
 EDMD_OK = 0
 EV_CELLCROSS, EV_COLLISION, EV_SCREENSHOT, EV_THERMO, EV_ADDINGNOISE = 0, 1, 2, 3, 4
+EV_WALL = 5  # j: 0 = wall normal to x, 1 = normal to y; a wall slot's partner is -5 - j
 STOP_NAMES = {0: "none", 1: "events", 2: "collisions", 3: "time", 4: "error", 5: "empty",
               6: "special"}
 NEVER = 1e26
@@ -41,7 +42,9 @@ class SystemDesc(C.Structure):
                 ("dtime_thermo", C.c_double), ("noise", C.c_int),
                 ("dtnoise", C.c_double), ("temperature", C.c_double),
                 ("cell_capacity", C.c_int), ("trace_capacity", C.c_int64),
-                ("thermo_capacity", C.c_int), ("vr", C.c_double), ("einit", C.c_double)]
+                ("thermo_capacity", C.c_int), ("vr", C.c_double), ("einit", C.c_double),
+                ("wall_x", C.c_int), ("wall_y", C.c_int), ("res_wall", C.c_double),
+                ("res_tan", C.c_double)]
 
 
 class Observables(C.Structure):
@@ -58,7 +61,8 @@ class Observables(C.Structure):
 TRACE_DT = np.dtype([("type", "<i4"), ("i", "<i4"), ("j", "<i4"), ("valid", "<i4"),
                      ("t", "<f8")])
 THERMO_DT = np.dtype([("t", "<f8"), ("ncol", "<u8"), ("e_per_n", "<f8"), ("pressure", "<f8"),
-                      ("pxx", "<f8"), ("pyy", "<f8"), ("pxy", "<f8"), ("pyx", "<f8")])
+                      ("pxx", "<f8"), ("pyy", "<f8"), ("pxy", "<f8"), ("pyx", "<f8"),
+                      ("pwall", "<f8")])
 
 EPOCH_DT = np.dtype([("t", "<f8"), ("ncol", "<u8"), ("ncross", "<u8"), ("nevents", "<u8"),
                      ("status", "<i4"), ("stop_reason", "<i4")])

@@ -41,6 +41,7 @@ extern "C" {
 #define EDMD_EV_SCREENSHOT 2
 #define EDMD_EV_THERMO 3
 #define EDMD_EV_ADDINGNOISE 4
+#define EDMD_EV_WALL 5 /* trace j / peek j: 0 = wall normal to x, 1 = normal to y */
 #define EDMD_EV_GROWSTOP 9
 
 /* ---- why a run returned ------------------------------------------------- */
@@ -79,7 +80,20 @@ typedef struct edmd_system_desc {
      * shifted by 1/vr as EDMD.c:1959-1964 does. */
     double vr;
     double einit;           /* Einit, EDMD.c:369 (-E); default 1 */
+    /* Hard walls (addWallx / addWally of the reference's graphics build,
+     * EDMD.c:267-268; its command-line build fixes them at 0): walls at
+     * x = 0, Lx and/or y = 0, Ly, no minimum image across them
+     * (EDMD.c:5896-5913).  A hit (EDMD.c:3384-3430) reverses the normal
+     * velocity times res_wall (resW, default 1) and scales the tangential
+     * one by res_tan (resT, default 0), and counts as a collision.  Not with
+     * a growth start (vr > 0) or taping. */
+    int wall_x, wall_y;
+    double res_wall, res_tan;
 } edmd_system_desc;
+
+/* A collision slot holding a wall event reports partner -5 - xy
+ * (edmd_batch_get_events / edmd_get_events). */
+#define EDMD_WALL_PARTNER(xy) (-5 - (xy))
 
 typedef struct edmd_trace_rec {
     int32_t type, i, j, valid; /* crossing: j = direction 1..4; collision: j = partner */
@@ -90,6 +104,7 @@ typedef struct edmd_thermo_row {
     double t;
     uint64_t ncol;
     double e_per_n, pressure, pxx, pyy, pxy, pyx; /* saveThermo, EDMD.c:5232-5333 */
+    double pwall; /* wall pressure dp / (dt * perimeter), EDMD.c:5428-5451; 0 without walls */
 } edmd_thermo_row;
 
 typedef struct edmd_observables {

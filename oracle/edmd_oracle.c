@@ -60,16 +60,22 @@ struct orc_ctx {
     /* growth start: per-disk radius growth rate, set while growing */
     double* vr;
     int growing;
+    double dp; /* momentum given to the walls since the last thermo row */
 };
 
 /* ------------------------------------------------------------ kinematics */
 
-/* PBC, EDMD.c:5896-5913: minimum image with >= L/2 and < -L/2. */
+/* PBC, EDMD.c:5896-5913: minimum image with >= L/2 and < -L/2, none
+ * across walls. */
 static void min_image(const orc_ctx* s, double* dx, double* dy) {
-    if (*dx >= s->halfLx) *dx -= s->Lx;
-    else if (*dx < -s->halfLx) *dx += s->Lx;
-    if (*dy >= s->halfLy) *dy -= s->Ly;
-    else if (*dy < -s->halfLy) *dy += s->Ly;
+    if (!s->prm.wall_x) {
+        if (*dx >= s->halfLx) *dx -= s->Lx;
+        else if (*dx < -s->halfLx) *dx += s->Lx;
+    }
+    if (!s->prm.wall_y) {
+        if (*dy >= s->halfLy) *dy -= s->Ly;
+        else if (*dy < -s->halfLy) *dy += s->Ly;
+    }
 }
 
 /* PBCinsideCellX/Y, EDMD.c:5922-5936. */
@@ -147,7 +153,7 @@ static void queue_add(orc_ctx* s, int e) {
         return;
     }
     int idx;
-    if (dt >= s->dt_paul * s->n_lists) {
+    if (!(dt < s->dt_paul * s->n_lists)) { /* also NaN: parked in the overflow list */
         idx = s->n_lists;
     } else {
         idx = s->cur_list + (int)(dt / s->dt_paul);
@@ -362,6 +368,39 @@ static void predict_collision(orc_ctx* s, int i, double* t_abs, int* partner) {
     int best = 0;
     double dt = NEVER;
     int X = s->cx[i], Y = s->cy[i];
+    /* walls first (EDMD.c:2845-2895, no field; logTime is the identity
+     * without damping): a disk in a boundary cell moving towards the wall
+     * gets the time to contact.  The x test sets xy = 0 whenever it applies,
+     * also when the y wall stays the earlier, as the reference does. */
+    int wall = 0, xy = 2;
+    if (s->prm.wall_y) {
+        if (Y == 0 && s->vy[i] < 0) {
+            dt = -(s->y[i] - s->rad[i]) / s->vy[i];
+            wall = 1;
+            xy = 1;
+        } else if (Y == s->ncy - 1 && s->vy[i] > 0) {
+            dt = (s->Ly - s->y[i] - s->rad[i]) / s->vy[i];
+            wall = 1;
+            xy = 1;
+        }
+    }
+    if (s->prm.wall_x) {
+        if (X == 0 && s->vx[i] < 0) {
+            double dtw = -(s->x[i] - s->rad[i]) / s->vx[i];
+            if (dt > dtw) {
+                wall = 1;
+                dt = dtw;
+            }
+            xy = 0;
+        } else if (X == s->ncx - 1 && s->vx[i] > 0) {
+            double dtw = (s->Lx - s->x[i] - s->rad[i]) / s->vx[i];
+            if (dt > dtw) {
+                wall = 1;
+                dt = dtw;
+            }
+            xy = 0;
+        }
+    }
     for (int j = -1; j <= 1; j++) {
         for (int k = -1; k <= 1; k++) {
             int c = wrap_cell(Y + j, s->ncy) * s->ncx + wrap_cell(X + k, s->ncx);
@@ -371,12 +410,13 @@ static void predict_collision(orc_ctx* s, int i, double* t_abs, int* partner) {
                 if (dt > dtt) {
                     best = p;
                     dt = dtt;
+                    wall = 0;
                 }
             }
         }
     }
     *t_abs = s->t + dt;
-    *partner = best;
+    *partner = wall ? ORC_WALL_PARTNER(xy) : best;
 }
 
 static void predict_crossing(const orc_ctx* s, int i, double* t_abs, int* dir) {
@@ -424,10 +464,17 @@ static void schedule_collision(orc_ctx* s, int i) {
     int partner;
     predict_collision(s, i, &tc, &partner);
     ev_node* e = &s->ev[s->n + i];
-    e->j = partner;
-    e->type = ORC_COLLISION;
+    if (partner < 0) {
+        /* addWallEvent, EDMD.c:3321-3330 */
+        e->j = -5 - partner;
+        e->type = ORC_WALL;
+        e->coll_actual = 0;
+    } else {
+        e->j = partner;
+        e->type = ORC_COLLISION;
+        e->coll_actual = s->coll[partner];
+    }
     e->t = tc;
-    e->coll_actual = s->coll[partner];
     queue_add(s, s->n + i);
 }
 
@@ -582,6 +629,36 @@ static void do_crossing(orc_ctx* s, const ev_node* ev) {
     schedule_collision(s, i);
 }
 
+static void do_wall(orc_ctx* s, const ev_node* ev) {
+    /* doTheWallNormal, EDMD.c:3384-3430, 3454-3460 (no mid wall, thermal
+     * wall or shear wall): counted as a collision, momentum to the wall,
+     * normal velocity reversed times resW, tangential scaled by resT, then
+     * new crossing and collision slots.  One deliberate difference: the
+     * scaled tangential velocity is made +0.0 where the product gives -0.0
+     * (resT = 0, the reference's default, times a negative velocity).  The
+     * reference's crossingEventNormal (EDMD.c:2416-2425) treats -0.0 as
+     * non-negative and divides by it, so its crossing time is -inf and the
+     * disk is flown to t = -inf (positions NaN); +0.0 gives +inf, no
+     * crossing in that direction, as for any disk at rest along it. */
+    int i = ev->i, xy = ev->j;
+    s->ncol++;
+    free_fly(s, i);
+    double rw = s->prm.res_wall, rt = s->prm.res_tan;
+    if (xy == 0) {
+        s->dp += fabs((rw + 1) * s->m[i] * s->vx[i]);
+        s->vx[i] = -rw * s->vx[i];
+        s->vy[i] = rt * s->vy[i] + 0.0;
+    } else {
+        s->dp += fabs((rw + 1) * s->m[i] * s->vy[i]);
+        s->vy[i] = -rw * s->vy[i];
+        s->vx[i] = rt * s->vx[i] + 0.0;
+    }
+    s->coll[i]++;
+    queue_remove(s, i);
+    schedule_crossing(s, i);
+    schedule_collision(s, i);
+}
+
 static void do_screenshot(orc_ctx* s) {
     /* takeAScreenshot, EDMD.c:4657-4677: free-fly everyone (a state update
      * point), reschedule; the dump write has no effect on the state. */
@@ -613,6 +690,14 @@ static void do_thermo(orc_ctx* s) {
     row.pyx = -1 * s->vir_yx / (area * dT) + eyx / area;
     row.pressure = (-1 / (dT)) * (s->vir_xx + s->vir_yy) / (2 * area) + s->E / area;
     s->vir_xx = s->vir_yy = s->vir_xy = s->vir_yx = 0;
+    row.pwall = 0;
+    if (s->prm.wall_x || s->prm.wall_y) {
+        double perimeter = 0;
+        if (s->prm.wall_x) perimeter += 2 * s->Ly;
+        if (s->prm.wall_y) perimeter += 2 * s->Lx;
+        row.pwall = s->dp / (dT * perimeter);
+        s->dp = 0;
+    }
     if (s->nrows == s->caprows) {
         int cap = s->caprows ? 2 * s->caprows : 16;
         orc_thermo_row* r = realloc(s->rows, (size_t)cap * sizeof *r);
@@ -698,6 +783,7 @@ static int step(orc_ctx* s, orc_trace* rec) {
     switch (ev->type) {
         case ORC_COLLISION: do_collision(s, ev); break;
         case ORC_CELLCROSS: s->ncross++; do_crossing(s, ev); break;
+        case ORC_WALL: do_wall(s, ev); break;
         case ORC_SCREENSHOT: do_screenshot(s); break;
         case ORC_THERMO: do_thermo(s); break;
         case ORC_ADDINGNOISE: do_rescale(s); break;
@@ -723,6 +809,10 @@ void orc_default_params(orc_params* p) {
     p->bucket_count = 0;
     p->vr = 0;
     p->einit = 1;
+    p->wall_x = 0;
+    p->wall_y = 0;
+    p->res_wall = 1; /* resW, EDMD.c:372 */
+    p->res_tan = 0;  /* resT, EDMD.c:373 */
 }
 
 orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* y,
@@ -734,6 +824,8 @@ orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* 
     s->n = n;
     if (prm) s->prm = *prm;
     else orc_default_params(&s->prm);
+    /* walls only with the Normal dispatch (doTheWallGrow is not restated) */
+    if ((s->prm.wall_x || s->prm.wall_y) && s->prm.vr > 0) { free(s); return NULL; }
     /* boxConstantHelper, EDMD.c:679-715 */
     s->Lx = Lx;
     s->Ly = Ly;
@@ -908,7 +1000,9 @@ void orc_get_events(const orc_ctx* s, double* t_cross, int32_t* dir, double* t_c
         if (t_cross) t_cross[i] = s->ev[i].t;
         if (dir) dir[i] = s->ev[i].j;
         if (t_coll) t_coll[i] = s->ev[n + i].t;
-        if (partner) partner[i] = s->ev[n + i].j;
+        if (partner)
+            partner[i] = s->ev[n + i].type == ORC_WALL ? ORC_WALL_PARTNER(s->ev[n + i].j)
+                                                       : s->ev[n + i].j;
         if (coll_actual) coll_actual[i] = s->ev[n + i].coll_actual;
     }
 }

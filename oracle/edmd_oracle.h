@@ -26,7 +26,12 @@ extern "C" {
 
 /* Event kinds, numbered as the reference's enum event (EDMD.h:21-34). */
 enum { ORC_CELLCROSS = 0, ORC_COLLISION = 1, ORC_SCREENSHOT = 2, ORC_THERMO = 3,
-       ORC_ADDINGNOISE = 4, ORC_GROWSTOP = 9 };
+       ORC_ADDINGNOISE = 4, ORC_WALL = 5, ORC_GROWSTOP = 9 };
+
+/* A collision slot holding a wall event reports partner -5 - xy (xy = 0:
+ * a wall normal to x, 1: normal to y), apart from the disks (>= 0) and the
+ * crossing directions (1-4). */
+#define ORC_WALL_PARTNER(xy) (-5 - (xy))
 
 /* Status codes. */
 enum { ORC_OK = 0, ORC_ERR_OVERLAP = 3, ORC_ERR_ARG = 22, ORC_ERR_NOMEM = 12 };
@@ -47,6 +52,14 @@ typedef struct {
      * rescales velocities to E/N = einit. */
     double vr;
     double einit;        /* Einit (EDMD.c:369, -E) */
+    /* Hard walls (the reference's addWallx / addWally, EDMD.c:267-268, which
+     * its command-line build fixes at 0, EDMD.c:290-291): walls at x = 0, Lx
+     * and/or y = 0, Ly; the minimum image is off across them
+     * (EDMD.c:5896-5913).  A wall hit reverses the normal velocity times
+     * res_wall and scales the tangential one by res_tan (resW, resT,
+     * EDMD.c:372-373, 3384-3430). */
+    int wall_x, wall_y;
+    double res_wall, res_tan;
 } orc_params;
 
 typedef struct {
@@ -63,6 +76,7 @@ typedef struct {
     double e_per_n;
     double pressure;
     double pxx, pyy, pxy, pyx;
+    double pwall; /* wall pressure dp / (dt * perimeter), 0 without walls (EDMD.c:5428-5451) */
 } orc_thermo_row;
 
 typedef struct orc_ctx orc_ctx;

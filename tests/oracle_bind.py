@@ -21,13 +21,15 @@ class Params(C.Structure):
                 ("dtime", C.c_double), ("first_thermo", C.c_double),
                 ("dtime_thermo", C.c_double), ("noise", C.c_int),
                 ("dtnoise", C.c_double), ("temperature", C.c_double),
-                ("bucket_count", C.c_int), ("vr", C.c_double), ("einit", C.c_double)]
+                ("bucket_count", C.c_int), ("vr", C.c_double), ("einit", C.c_double),
+                ("wall_x", C.c_int), ("wall_y", C.c_int), ("res_wall", C.c_double),
+                ("res_tan", C.c_double)]
 
 
 class ThermoRow(C.Structure):
     _fields_ = [("t", C.c_double), ("ncol", C.c_uint64), ("e_per_n", C.c_double),
                 ("pressure", C.c_double), ("pxx", C.c_double), ("pyy", C.c_double),
-                ("pxy", C.c_double), ("pyx", C.c_double)]
+                ("pxy", C.c_double), ("pyx", C.c_double), ("pwall", C.c_double)]
 
 
 TRACE_DT = np.dtype([("type", "<i4"), ("i", "<i4"), ("j", "<i4"),
@@ -256,7 +258,8 @@ class Oracle:
         arr = (ThermoRow * max(n, 1))()
         lib().orc_thermo_rows(self.h, arr, n)
         return [dict(t=r.t, ncol=r.ncol, e_per_n=r.e_per_n, pressure=r.pressure,
-                     pxx=r.pxx, pyy=r.pyy, pxy=r.pxy, pyx=r.pyx) for r in arr[:n]]
+                     pxx=r.pxx, pyy=r.pyy, pxy=r.pxy, pyx=r.pyx, pwall=r.pwall)
+                for r in arr[:n]]
 
     def collision_time(self, i, j):
         return lib().orc_collision_time(self.h, i, j)

@@ -1,0 +1,151 @@
+"""Hard walls on the GPU against the oracle, bit for bit.
+
+Walls are the reference's addWallx / addWally (EDMD.c:2845-2895, 3321-3330,
+3384-3430, 5428-5451, 5896-5913), which its command-line build compiles out;
+the oracle's restatement is pinned by tests/test_oracle_walls.py.  Traces
+are compared in raw order (kind, disks or wall, validity, time bits), states
+and thermo rows (with the wall pressure) bitwise."""
+import numpy as np
+import pytest
+
+import edmd_fixtures as F
+from conftest import bits
+
+pytestmark = pytest.mark.gpu
+
+WALL = 5
+QUIET = dict(first_screen=1e9, dtime=1e9, first_thermo=1e9)
+
+
+def walled(n, phi, seed, walls, temperature=1.0):
+    """A fitted-lattice start pressed affinely into [r + 0.01, L - r - 0.01]
+    along each walled direction (the lattice's outer rows may sit closer
+    than a radius to the box edge; the squeeze leaves every pair apart)."""
+    st = F.lattice_state(n, phi, seed=seed, lattice="fit", temperature=temperature)
+    r, L = 0.5, st["Lx"]
+    lo, span = r + 0.01, L - 2 * (r + 0.01)
+    for key, flag in (("x", "wall_x"), ("y", "wall_y")):
+        if walls.get(flag):
+            st[key] = lo + st[key] / L * span
+    dx = st["x"][:, None] - st["x"][None, :]
+    dy = st["y"][:, None] - st["y"][None, :]
+    if not walls.get("wall_x"):
+        dx -= L * np.round(dx / L)
+    if not walls.get("wall_y"):
+        dy -= L * np.round(dy / L)
+    d2 = dx * dx + dy * dy
+    np.fill_diagonal(d2, 9.0)
+    assert d2.min() > 1.0
+    return st
+
+
+def assert_trace_equal(got, ref):
+    assert len(got) == len(ref)
+    for k in ("type", "i", "j", "valid"):
+        bad = np.nonzero(got[k] != ref[k])[0]
+        assert len(bad) == 0, "%s differs first at event %d" % (k, bad[0])
+    bad = np.nonzero(bits(got["t"]) != bits(ref["t"]))[0]
+    assert len(bad) == 0, "time differs first at event %d" % bad[0]
+
+
+def assert_state_equal(got, ref):
+    for k in ("x", "y", "vx", "vy", "t"):
+        np.testing.assert_array_equal(bits(got[k]), bits(ref[k]), err_msg=k)
+    np.testing.assert_array_equal(got["coll"].astype(np.uint64), ref["coll"].astype(np.uint64))
+    np.testing.assert_array_equal(got["cx"], ref["cx"])
+    np.testing.assert_array_equal(got["cy"], ref["cy"])
+
+
+CASES = [
+    # (n, phi, seed, walls and restitution)
+    (400, 0.5, 21, dict(wall_y=1, res_tan=1.0)),              # specular channel
+    (400, 0.6, 22, dict(wall_x=1, wall_y=1)),                  # closed box, resW 1, resT 0
+    (600, 0.45, 23, dict(wall_x=1, res_wall=0.9, res_tan=0.5)),
+    (1500, 0.7, 24, dict(wall_y=1, res_tan=1.0)),              # gather-free staged scan, deeper calendar
+]
+
+
+@pytest.mark.parametrize("n,phi,seed,walls", CASES)
+def test_wall_traces_match_oracle(gpu, oracle, n, phi, seed, walls):
+    st = walled(n, phi, seed, walls)
+    nev = 20000
+    s = gpu.System(st, trace_capacity=nev, **walls, **QUIET)
+    ev0 = s.events()
+    o = oracle.Oracle(st, **walls, **QUIET)
+    oe = o.events()
+    # initial sweep: wall slots report partner -5 - xy in both
+    for k in ("t_coll", "t_cross"):
+        np.testing.assert_array_equal(bits(ev0[k]), bits(oe[k]), err_msg=k)
+    np.testing.assert_array_equal(ev0["partner"], oe["partner"])
+    np.testing.assert_array_equal(ev0["dir"], oe["dir"])
+    s.run(max_events=nev)
+    code, tr = o.run(nev)
+    assert code == 0
+    g = s.trace()
+    assert np.sum(g["type"] == WALL) > 50
+    assert_trace_equal(g, tr)
+    assert_state_equal(s.state(), o.state())
+    assert s.observables()["ncol"] == o.observables()["ncol"]
+
+
+def test_wall_pressure_rows_and_thermostat_match_oracle(gpu, oracle):
+    """Thermo rows with the wall pressure, and the velocity-rescale
+    thermostat (full re-predictions with walls) in a closed box."""
+    st = walled(500, 0.55, 31, dict(wall_x=1, wall_y=1), temperature=2.0)
+    kw = dict(wall_x=1, wall_y=1, res_tan=1.0, first_thermo=0.5, dtime_thermo=0.5,
+              noise=2, dtnoise=0.3, temperature=1.5, first_screen=1e9, dtime=1e9)
+    nev = 30000
+    s = gpu.System(st, trace_capacity=nev, thermo_capacity=512, **kw)
+    s.run(max_events=nev)
+    o = oracle.Oracle(st, **kw)
+    code, tr = o.run(nev)
+    assert code == 0
+    assert_trace_equal(s.trace(), tr)
+    rows, orows = s.thermo(), o.thermo_rows()
+    assert len(rows) == len(orows) > 10
+    for r, q in zip(rows, orows):
+        for k in ("t", "e_per_n", "pressure", "pxx", "pyy", "pxy", "pyx", "pwall"):
+            assert bits(np.float64(r[k])) == bits(np.float64(q[k])), k
+        assert r["pwall"] > 0
+
+
+def test_wall_batch_mixed_residencies_and_shapes(gpu, oracle):
+    """A batch mixing wall and periodic systems in one launch (the loop's
+    wall variant serves both), from HBM and from LDS, every sweep shape:
+    each system evolves as the oracle evolves it alone."""
+    specs = [(300, 0.5, 41, dict(wall_y=1, res_tan=1.0)), (350, 0.6, 42, {}),
+             (280, 0.55, 43, dict(wall_x=1, wall_y=1)), (320, 0.65, 44, dict(wall_x=1))]
+    states = [walled(n, phi, sd, w) for n, phi, sd, w in specs]
+    per = {k: [w.get(k, d) for *_, w in specs]
+           for k, d in (("wall_x", 0), ("wall_y", 0), ("res_tan", 0.0))}
+    nev = 12000
+    refs = []
+    for (n, phi, sd, w), st in zip(specs, states):
+        o = oracle.Oracle(st, **w, **QUIET)
+        code, tr = o.run(nev)
+        assert code == 0
+        refs.append((tr, o.state()))
+    for mode, shape in ((gpu.RES_HBM, 0), (gpu.RES_LDS, 3), (gpu.RES_LDS, 7)):
+        b = gpu.Batch(states, per_system=per, trace_capacity=nev, **QUIET)
+        b.set_residency(mode)
+        b.set_sweep_shape(shape)
+        b.sweep_predict()  # the shape under test predicts the starting slots again
+        b2 = gpu.Batch(states, per_system=per, trace_capacity=nev, **QUIET)
+        for s in range(len(states)):
+            for k, v in b.events(s).items():
+                np.testing.assert_array_equal(v, b2.events(s)[k], err_msg=k)
+        b.run(max_events=nev)
+        for s, (tr, so) in enumerate(refs):
+            assert_trace_equal(b.trace(s), tr)
+            assert_state_equal(b.state(s), so)
+
+
+def test_wall_arguments_are_checked(gpu):
+    st = F.lattice_state(100, 0.4, seed=5, lattice="fit")
+    with pytest.raises(gpu.EdmdError):
+        gpu.System(st, wall_x=1, vr=0.1)
+    with pytest.raises(gpu.EdmdError):
+        gpu.System(st, wall_y=2)
+    b = gpu.Batch([st], wall_y=1)
+    with pytest.raises(gpu.EdmdError):
+        b.deriv_begin(checkpoint_every=256)

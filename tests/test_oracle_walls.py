@@ -1,0 +1,138 @@
+"""The oracle's hard walls (addWallx / addWally of the reference's graphics
+build, EDMD.c:2845-2895, 3321-3330, 3384-3430, 5428-5451, 5896-5913) against
+what the mathematics fixes.
+
+The reference's command-line build compiles the walls out (its addWallx and
+addWally are constants 0, EDMD.c:290-291), so no reference vector exists for
+them; these checks pin the restatement instead: exact wall-contact times and
+bounce periods, the reflection rule, conservation laws and the ideal-gas
+limit of the wall pressure."""
+import numpy as np
+import pytest
+
+import edmd_fixtures as F
+
+WALL = 5
+
+
+def disks(pos, vel, L=20.0, rad=0.5):
+    pos, vel = np.asarray(pos, float), np.asarray(vel, float)
+    return {"x": pos[:, 0].copy(), "y": pos[:, 1].copy(), "vx": vel[:, 0].copy(),
+            "vy": vel[:, 1].copy(), "rad": np.full(len(pos), rad), "m": np.ones(len(pos)),
+            "Lx": L, "Ly": L}
+
+
+def test_bounce_times_closed_form(oracle):
+    """A disk falling straight onto the y walls of a channel hits them at
+    t = y0 - r, then every (L - 2r) / |vy|, alternating walls; the other
+    disk rests ten units away."""
+    st = disks([(3.0, 7.25), (13.0, 10.0)], [(0.0, -1.0), (0.0, 0.0)])
+    o = oracle.Oracle(st, wall_y=1, res_tan=1.0, first_screen=1e9, dtime=1e9,
+                      first_thermo=1e9)
+    code, tr = o.run(400)
+    assert code == 0
+    w = tr[tr["type"] == WALL]
+    assert len(w) >= 8 and np.all(w["i"] == 0) and np.all(w["j"] == 1)
+    k = np.arange(len(w))
+    np.testing.assert_allclose(w["t"], 6.75 + 19.0 * k, rtol=1e-13)
+    # every hit counts as a collision; velocities flip exactly
+    s = o.state()
+    assert s["vy"][0] == (-1.0 if len(w) % 2 == 0 else 1.0) and s["vx"][0] == 0.0
+    assert o.observables()["ncol"] == len(w)
+
+
+def test_corner_quirk_and_tangential_damping(oracle):
+    """Defaults resW = 1, resT = 0: a hit keeps the normal speed and
+    removes the tangential one.  In a corner cell moving towards both walls
+    the x branch sets xy = 0 even when the y wall is the earlier
+    (EDMD.c:2875-2894): the event comes at the y-wall time and reflects
+    vx, and resT = 0 stops the disk in y."""
+    st = disks([(1.5, 0.9), (15.0, 15.0)], [(-0.5, -1.0), (0.0, 0.3)])
+    o = oracle.Oracle(st, wall_x=1, wall_y=1, first_screen=1e9, dtime=1e9, first_thermo=1e9)
+    t, partner = o.predict_collision(0)
+    # y wall at 0.4, x wall at 2.0: the y wall time, labelled xy = 0
+    assert t == 0.4 and partner == -5
+    code, tr = o.run(1)
+    assert tr[0]["type"] == WALL and tr[0]["j"] == 0 and tr[0]["t"] == 0.4
+    s = o.state()
+    assert s["vx"][0] == 0.5 and s["vy"][0] == 0.0
+    # a disk in a non-corner cell: tangential velocity removed, normal reversed
+    st = disks([(10.0, 0.9), (15.0, 15.0)], [(0.3, -2.0), (0.0, 0.3)])
+    o = oracle.Oracle(st, wall_y=1, first_screen=1e9, dtime=1e9, first_thermo=1e9)
+    code, tr = o.run(1)
+    assert tr[0]["type"] == WALL and tr[0]["j"] == 1 and tr[0]["t"] == 0.2
+    s = o.state()
+    assert s["vx"][0] == 0.0 and s["vy"][0] == 2.0
+    # a negative tangential velocity times resT = 0 gives +0.0, not the
+    # reference's -0.0 (whose crossing time would be -inf, EDMD.c:2416-2425)
+    st2 = disks([(10.5, 0.9), (15.0, 15.0)], [(-0.3, -2.0), (0.0, 0.3)])
+    o = oracle.Oracle(st2, wall_y=1, first_screen=1e9, dtime=1e9, first_thermo=1e9)
+    o.run(1)
+    s = o.state()
+    assert s["vx"][0] == 0.0 and not np.signbit(s["vx"][0])
+    t, d = o.predict_crossing(0)
+    assert d == 4 and t == pytest.approx(0.2 + (2.0 - 0.5) / 2.0, rel=1e-12)
+    # res_wall scales the normal speed
+    o = oracle.Oracle(st, wall_y=1, res_wall=0.5, res_tan=1.0, first_screen=1e9, dtime=1e9,
+                      first_thermo=1e9)
+    o.run(1)
+    s = o.state()
+    assert s["vx"][0] == 0.3 and s["vy"][0] == 1.0
+
+
+def test_no_minimum_image_across_walls(oracle):
+    """Two disks on either side of a y wall see each other only through the
+    box, not across the wall; across the periodic x boundary they do."""
+    st = disks([(5.0, 0.6), (5.0, 19.4)], [(0.0, -0.1), (0.0, 0.1)])
+    o = oracle.Oracle(st, wall_y=1)
+    assert o.collision_time(0, 1) == 1e26
+    o = oracle.Oracle(st)
+    # across the periodic boundary: gap 0.2 closing at 0.2
+    assert o.collision_time(0, 1) == pytest.approx(1.0, rel=1e-12)
+    st = disks([(0.6, 5.0), (19.4, 5.0)], [(-0.1, 0.0), (0.1, 0.0)])
+    o = oracle.Oracle(st, wall_y=1)
+    assert o.collision_time(0, 1) == pytest.approx(1.0, rel=1e-12)
+
+
+def test_channel_conserves_energy_and_x_momentum(oracle):
+    """Specular elastic walls (resW = resT = 1) in y: energy and the x
+    momentum stay at their initial values through thousands of wall hits and
+    disk collisions; the y momentum does not (the walls take it)."""
+    st = F.lattice_state(400, 0.5, seed=11, lattice="fit")
+    o = oracle.Oracle(st, wall_y=1, res_tan=1.0, first_screen=1e9, dtime=1e9, first_thermo=1e9)
+    e0 = o.observables()
+    code, tr = o.run(40000)
+    assert code == 0
+    assert np.sum(tr["type"] == WALL) > 300
+    e1 = o.observables()
+    assert abs(e1["E"] / e0["E"] - 1) < 1e-12
+    assert abs(e1["px"] - e0["px"]) < 1e-10
+    assert abs(e1["py"] - e0["py"]) > 1e-3
+
+
+def test_wall_pressure_ideal_gas_limit(oracle):
+    """A dilute gas (phi = 0.02) between y walls: the wall pressure of the
+    thermo rows (momentum to the walls per time per perimeter) approaches
+    the ideal-gas value N T / A with T = E / N (2D, E = sum m v^2 / 2), with
+    the low-density excluded-volume correction Z = 1 + 2 phi."""
+    st = F.lattice_state(300, 0.02, seed=12, lattice="fit")
+    o = oracle.Oracle(st, wall_y=1, res_tan=1.0, first_screen=1e9, dtime=1e9,
+                      first_thermo=20.0, dtime_thermo=20.0)
+    o.run_until(2000.0)
+    rows = o.thermo_rows()
+    pw = np.array([r["pwall"] for r in rows[1:]])
+    assert len(pw) > 50
+    A = st["Lx"] * st["Ly"]
+    ideal = 300 * o.observables()["E"] / 300 / A * (1 + 2 * 0.02)
+    se = pw.std(ddof=1) / np.sqrt(len(pw))
+    assert abs(pw.mean() - ideal) < max(4 * se, 0.03 * ideal), (pw.mean(), ideal, se)
+    # no walls: the column is zero
+    o = oracle.Oracle(st, first_thermo=20.0, dtime_thermo=20.0)
+    o.run_until(100.0)
+    assert all(r["pwall"] == 0 for r in o.thermo_rows())
+
+
+def test_walls_with_growth_are_rejected(oracle):
+    st = F.lattice_state(50, 0.3, seed=1, lattice="fit")
+    with pytest.raises(ValueError):
+        oracle.Oracle(st, wall_x=1, vr=0.1)
