@@ -1128,38 +1128,62 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
         const int c1 = n0, c2 = n0 + n1;
         const int o0 = a0, o1 = a1 - c1, o2 = a2 - c2;
         const int tot = act ? c2 + n2 : 0;
-        const int ktot = -wave_min_i32(-tot);
         const bool direct = tot > 256;
-        uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+        const int tot1 = direct ? 0 : tot;
+        const int ktot = -wave_min_i32(-tot1);
+        uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0, cur = 0;
         double dt = EDMD_NEVER;
         int best = -1;
         int wxy = 2;
         bool wall = false;
         if (C.walls) wall = wall_time(C, ox, oy, ovx, ovy, orad, tx0 + lx, ty0 + ly, dt, wxy);
+        /* pass 1 without branches: every lane evaluates its k-th candidate
+         * (a lane past its count re-reads its own disk, masked out); the
+         * marks of 64 consecutive k collect in cur and go to their word
+         * when k leaves it (k is uniform over the wave) */
         for (int k = 0; k < ktot; ++k) {
-            if (k >= tot) continue;
-            const int v = k + (k < c1 ? o0 : (k < c2 ? o1 : o2));
-            /* the owner itself; a halo that wraps (boxes under three tiles
-             * wide) may stage the owner's cell more than once */
-            if (WRAP ? L.id[v] == p : v == me) continue;
+            const bool in = k < tot1;
+            const int vk = k + (k < c1 ? o0 : (k < c2 ? o1 : o2));
+            const int v = in ? vk : me;
             double b, v2, d, det;
             pair_terms<WRAP>(C, ox, oy, ovx, ovy, orad, L.pos[v], L.vel[v], L.rad[v], b, v2, d,
                              det);
-            if (b > 0) continue;
-            if (d < -0.01) ov = true; /* EDMD.c:2708-2717 */
-            if (det < 0) continue;
-            if (direct) {
-                const double tt = (-b - sqrt(det)) / v2;
-                if (dt > tt) {
-                    dt = tt;
-                    best = v;
+            /* the owner itself; a halo that wraps (boxes under three tiles
+             * wide) may stage the owner's cell more than once */
+            const bool self = WRAP ? L.id[v] == p : v == me;
+            const bool ok = in && !self && !(b > 0);
+            ov = ov || (ok && d < -0.01); /* EDMD.c:2708-2717 */
+            const bool mark = ok && !(det < 0);
+            cur |= mark ? ((uint64_t)1 << (k & 63)) : (uint64_t)0;
+            if ((k & 63) == 63 || k == ktot - 1) {
+                switch (k >> 6) {
+                    case 0: m0 = cur; break;
+                    case 1: m1 = cur; break;
+                    case 2: m2 = cur; break;
+                    default: m3 = cur; break;
                 }
-            } else {
-                const uint64_t bit = (uint64_t)1 << (k & 63);
-                if (k < 64) m0 |= bit;
-                else if (k < 128) m1 |= bit;
-                else if (k < 192) m2 |= bit;
-                else m3 |= bit;
+                cur = 0;
+            }
+        }
+        /* disks with more than 256 candidates solve their roots in order as
+         * they go */
+        if (__ballot(direct)) {
+            if (direct) {
+                for (int k = 0; k < tot; ++k) {
+                    const int v = k + (k < c1 ? o0 : (k < c2 ? o1 : o2));
+                    if (WRAP ? L.id[v] == p : v == me) continue;
+                    double b, v2, d, det;
+                    pair_terms<WRAP>(C, ox, oy, ovx, ovy, orad, L.pos[v], L.vel[v], L.rad[v], b,
+                                     v2, d, det);
+                    if (b > 0) continue;
+                    if (d < -0.01) ov = true;
+                    if (det < 0) continue;
+                    const double tt = (-b - sqrt(det)) / v2;
+                    if (dt > tt) {
+                        dt = tt;
+                        best = v;
+                    }
+                }
             }
         }
         while (m0 | m1 | m2 | m3) {
