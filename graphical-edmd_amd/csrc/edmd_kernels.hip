@@ -268,6 +268,30 @@ __device__ __forceinline__ double coll_time_at(const EdmdConst& C, double x1, do
     return (-b - sqrt(det)) / v2;
 }
 
+/* collisionTimeNormal as coll_time computes it, without branches: every
+ * term is evaluated and the result selected, so a wavefront whose lanes hold
+ * different candidates runs one instruction stream.  Bitwise the same
+ * result; the overlap test applies where the reference reaches it (b <= 0). */
+__device__ __forceinline__ double coll_time_nb(const EdmdConst& C, double t, double x1, double y1,
+                                               double vx1, double vy1, double r1, double x2,
+                                               double y2, double vx2, double vy2, double tl2,
+                                               double r2, bool& overlap) {
+    const double lat2 = t - tl2;
+    const double dvx = vx2 - vx1;
+    const double dvy = vy2 - vy1;
+    double dx = (x2 + lat2 * vx2) - x1;
+    double dy = (y2 + lat2 * vy2) - y1;
+    min_image(C, dx, dy);
+    const double b = dx * dvx + dy * dvy;
+    const double v2 = dvx * dvx + dvy * dvy;
+    const double d = dx * dx + dy * dy - 4 * r1 * r2; /* DIST2, EDMD.c:35-43 */
+    const double det = b * b - v2 * d;
+    const bool approach = !(b > 0);
+    overlap = overlap || (approach && d < -0.01); /* EDMD.c:2708-2717 */
+    const double root = (-b - sqrt(det)) / v2;
+    return (approach && !(det < 0)) ? root : EDMD_NEVER;
+}
+
 __device__ __forceinline__ double coll_time(const EdmdConst& C, double t, double x1, double y1,
                                             double vx1, double vy1, double r1, double x2,
                                             double y2, double vx2, double vy2, double tl2,
@@ -502,7 +526,7 @@ __device__ __forceinline__ void predict_owners_gather(const EdmdConst& C, const 
                 tt = coll_time_grow(C, t, a.x, a.y, a.vx, a.vy, a.r, a.vr, rA.x, rA.y, rA.vx,
                                     rA.vy, rA.tl, rA.rad, gA, overlap);
             else
-                tt = coll_time(C, t, a.x, a.y, a.vx, a.vy, a.r, rA.x, rA.y, rA.vx, rA.vy, rA.tl,
+                tt = coll_time_nb(C, t, a.x, a.y, a.vx, a.vy, a.r, rA.x, rA.y, rA.vx, rA.vy, rA.tl,
                                rA.rad, overlap);
             if (tt < lim) tA = tt;
         }
@@ -512,7 +536,7 @@ __device__ __forceinline__ void predict_owners_gather(const EdmdConst& C, const 
                 tt = coll_time_grow(C, t, b.x, b.y, b.vx, b.vy, b.r, b.vr, rB.x, rB.y, rB.vx,
                                     rB.vy, rB.tl, rB.rad, gB, overlap);
             else
-                tt = coll_time(C, t, b.x, b.y, b.vx, b.vy, b.r, rB.x, rB.y, rB.vx, rB.vy, rB.tl,
+                tt = coll_time_nb(C, t, b.x, b.y, b.vx, b.vy, b.r, rB.x, rB.y, rB.vx, rB.vy, rB.tl,
                                rB.rad, overlap);
             if (tt < lim) tB = tt;
         }
@@ -644,8 +668,8 @@ __device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const 
                 tt = coll_time_grow(C, t, ox, oy, ovx, ovy, orad, ovr, r.x, r.y, r.vx, r.vy, r.tl,
                                     r.rad, A.vr[q], overlap);
             } else {
-                tt = coll_time(C, t, ox, oy, ovx, ovy, orad, r.x, r.y, r.vx, r.vy, r.tl, r.rad,
-                               overlap);
+                tt = coll_time_nb(C, t, ox, oy, ovx, ovy, orad, r.x, r.y, r.vx, r.vy, r.tl, r.rad,
+                                  overlap);
             }
             if (tt < lim) tq = tt;
         }
