@@ -64,6 +64,8 @@ extern "C" void edmd_default_desc(edmd_system_desc* d) {
     d->wall_y = 0;
     d->res_wall = 1; /* resW, EDMD.c:372 */
     d->res_tan = 0;  /* resT, EDMD.c:373 */
+    d->add_field = 0;
+    d->field = -0.02; /* EDMD.c:366 */
 }
 
 /* ------------------------------------------------------------ handles */
@@ -231,6 +233,14 @@ static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
         return fail(EDMD_ERR_ARG, "system %d: walls need a start without growth (vr = 0)", idx);
     if (!(d->res_wall >= 0) || !(d->res_tan == d->res_tan))
         return fail(EDMD_ERR_ARG, "system %d: bad wall restitution", idx);
+    if (d->add_field != 0 && d->add_field != 1)
+        return fail(EDMD_ERR_ARG, "system %d: add_field must be 0 or 1", idx);
+    if (d->add_field && !(d->field < 0))
+        return fail(EDMD_ERR_ARG, "system %d: the field must be negative", idx);
+    if (d->add_field && d->vr > 0)
+        return fail(EDMD_ERR_ARG, "system %d: a field needs a start without growth", idx);
+    c.field_on = d->add_field;
+    c.field = d->field;
     c.walls = (d->wall_x ? 1 : 0) | (d->wall_y ? 2 : 0);
     c.res_wall = d->res_wall;
     c.res_tan = d->res_tan;
@@ -440,7 +450,7 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
     if (!rc) rc = dalloc(b, &b->erank, 2 * np);
     if (!rc) rc = dalloc(b, &b->lrank, np);
     for (int s = 0; s < nsys; s++) b->any_grow |= descs[s].vr > 0;
-    for (int s = 0; s < nsys; s++) b->any_walls |= b->hs[s].c.walls != 0;
+    for (int s = 0; s < nsys; s++) b->any_walls |= b->hs[s].c.walls != 0 || b->hs[s].c.field_on;
     if (!rc && b->any_grow) rc = dalloc(b, &b->vr, np);
     if (rc) { batch_free(b); return rc; }
     if (hipStreamCreate(&b->stream) != hipSuccess) {
@@ -1584,8 +1594,8 @@ extern "C" int edmd_batch_deriv_begin(edmd_batch* b, int64_t checkpoint_every, i
     if (b->any_grow)
         return fail(EDMD_ERR_ARG, "derivatives need systems whose growth has ended");
     for (int s = 0; s < b->nsys; s++)
-        if (b->hs[s].c.walls)
-            return fail(EDMD_ERR_ARG, "system %d: derivatives need a system without walls", s);
+        if (b->hs[s].c.walls || b->hs[s].c.field_on)
+            return fail(EDMD_ERR_ARG, "system %d: derivatives need a system without walls or field", s);
     int rc = edmd_batch_deriv_end(b);
     if (rc) return rc;
     if ((double)b->nsys * (double)cap * sizeof(EdmdTapeRec) > 1.0e12)

@@ -168,7 +168,25 @@ __device__ __forceinline__ bool wall_time(const EdmdConst& C, double x, double y
     bool wall = false;
     dt = EDMD_NEVER;
     xy = 2;
-    if (C.walls & 2) {
+    if ((C.walls & 2) && C.field_on) {
+        /* EDMD.c:2846-2860: the bottom wall from the bottom row whatever the
+         * velocity; the top wall for any disk moving up (NaN when it cannot
+         * reach it) */
+        const double f = C.field;
+        if (cy == 0) {
+            dt = (-vy - sqrt(vy * vy - 2 * f * (y - r))) / f;
+            wall = true;
+            xy = 1;
+        }
+        if (vy > 0) {
+            const double dt2 = (-vy + sqrt(vy * vy - 2 * f * (y - (C.Ly - r)))) / f;
+            if (!__builtin_isnan(dt2) && dt2 < dt) {
+                dt = dt2;
+                wall = true;
+                xy = 1;
+            }
+        }
+    } else if (C.walls & 2) {
         if (cy == 0 && vy < 0) {
             dt = -(y - r) / vy;
             wall = true;
@@ -229,6 +247,24 @@ __device__ __forceinline__ void fly(const EdmdConst& C, double t, double& x, dou
     y += dt * vy;
     x = wrap_post(x, C.Lx);
     y = wrap_post(y, C.Ly);
+}
+
+/* freeFlyNormal with the field when the system has one (EDMD.c:4957-4963:
+ * parabolic in y, vy advanced); F = false compiles the field out. */
+template <bool F>
+__device__ __forceinline__ void fly_v(const EdmdConst& C, double t, double& x, double& y,
+                                      double vx, double& vy, double& tl) {
+    if (F && C.field_on) {
+        const double dt = t - tl;
+        tl = t;
+        x += dt * vx;
+        y += dt * vy + 0.5 * C.field * dt * dt;
+        vy += dt * C.field;
+        x = wrap_post(x, C.Lx);
+        y = wrap_post(y, C.Ly);
+    } else {
+        fly(C, t, x, y, vx, vy, tl);
+    }
 }
 
 /* freeFlyGrow, EDMD.c:4992-5007: the radius grows with the flight. */
@@ -336,6 +372,7 @@ __device__ __forceinline__ double coll_time_grow(const EdmdConst& C, double t, d
 
 /* crossingEventNormal, EDMD.c:2405-2482: absolute crossing time and
  * direction code (1 -x, 2 +x, 3 -y, 4 +y); ties go to y. */
+template <bool F = true>
 __device__ __forceinline__ void cross_time(const EdmdConst& C, double t, double x, double y,
                                            double vx, double vy, int cx, int cy, double& tout,
                                            int& dir) {
@@ -348,7 +385,20 @@ __device__ __forceinline__ void cross_time(const EdmdConst& C, double t, double 
         tx = wrap_inside((1 + cx) * C.csx - x, C.halfLx, C.Lx) / vx;
         xx = 2;
     }
-    if (vy < 0) {
+    if (F && C.field_on) {
+        /* EDMD.c:2428-2442: up through the top edge when fast enough, else
+         * down through the bottom edge */
+        const double f = C.field;
+        double dy = wrap_inside(y - (1 + cy) * C.csy, C.halfLy, C.Ly);
+        if (dy < 0 && vy > sqrt(2 * f * dy)) {
+            ty = (-vy + sqrt(vy * vy - 2 * f * dy)) / f;
+            yy = 4;
+        } else {
+            dy = wrap_inside(y - cy * C.csy, C.halfLy, C.Ly);
+            ty = (-vy - sqrt(vy * vy - 2 * f * dy)) / f;
+            yy = 3;
+        }
+    } else if (vy < 0) {
         ty = wrap_inside(cy * C.csy - y, C.halfLy, C.Ly) / vy;
         yy = 3;
     } else {
@@ -422,7 +472,17 @@ __device__ __forceinline__ double predict_seq(const EdmdConst& C, const AR& A, d
             for (int s = cnt - 1; s >= 0; --s) {
                 int q = sl[s];
                 if (q == p) continue;
-                const EdmdPart c2 = A.p[q];
+                EdmdPart c2 = A.p[q];
+                if (W && !G && C.field_on) {
+                    /* collisionTimeNormal flies the neighbour first under
+                     * the field, a state update (EDMD.c:2664-2666) */
+                    fly_v<true>(C, t, c2.x, c2.y, c2.vx, c2.vy, c2.tl);
+                    const auto w = A.p + q;
+                    w->x = c2.x;
+                    w->y = c2.y;
+                    w->vy = c2.vy;
+                    w->tl = c2.tl;
+                }
                 double tt;
                 if (G)
                     tt = coll_time_grow(C, t, o.x, o.y, o.vx, o.vy, o.rad, ovr, c2.x, c2.y, c2.vx,
@@ -459,7 +519,7 @@ struct Owner {
  * (time, traversal rank) among candidates with time < NEVER is the
  * reference's strict-less first minimum; no candidate gives NEVER and
  * partner 0 (EDMD.c:2830-2831, 3091-3093).  Results are uniform. */
-template <bool G, class AR>
+template <bool G, bool W = true, class AR>
 __device__ __forceinline__ void predict_owners_gather(const EdmdConst& C, const AR& A,
                                                       int lane, double t, int nown,
                                                       const Owner& a, const Owner& b, double& dta,
@@ -517,6 +577,19 @@ __device__ __forceinline__ void predict_owners_gather(const EdmdConst& C, const 
         uint32_t cA = 0, cB = 0;
         if (vA) { rA = A.p[qA]; cA = rA.coll; }
         if (vB) { rB = A.p[qB]; cB = rB.coll; }
+        if (W && !G && C.field_on) {
+            /* the neighbours flown to t and stored (EDMD.c:2664-2666) */
+            if (vA) {
+                fly_v<true>(C, t, rA.x, rA.y, rA.vx, rA.vy, rA.tl);
+                const auto w = A.p + qA;
+                w->x = rA.x; w->y = rA.y; w->vy = rA.vy; w->tl = rA.tl;
+            }
+            if (vB) {
+                fly_v<true>(C, t, rB.x, rB.y, rB.vx, rB.vy, rB.tl);
+                const auto w = A.p + qB;
+                w->x = rB.x; w->y = rB.y; w->vy = rB.vy; w->tl = rB.tl;
+            }
+        }
         if (G && vA) gA = A.vr[qA];
         if (G && vB) gB = A.vr[qB];
         double tA = INF_D, tB = INF_D;
@@ -602,7 +675,7 @@ __device__ __forceinline__ int scan32(int v) {
  * passes only a strictly earlier time: the reference's strict-less first
  * minimum.  Requires cap <= EDMD_STAGE_CAP, cap % 4 == 0 and cell rows
  * aligned to 16 bytes. */
-template <bool G, class AR>
+template <bool G, bool W = true, class AR>
 __device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const AR& A,
                                                       int lane, double t, int nown,
                                                       const Owner& a, const Owner& b, double& dta,
@@ -657,7 +730,17 @@ __device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const 
         double tq = INF_D;
         uint32_t cq = 0;
         if (v) {
-            const EdmdPart r = A.p[q];
+            EdmdPart r = A.p[q];
+            if (W && !G && C.field_on) {
+                /* the neighbour flown to t and stored (EDMD.c:2664-2666); a
+                 * disk listed for both owners is written twice alike */
+                fly_v<true>(C, t, r.x, r.y, r.vx, r.vy, r.tl);
+                const auto w = A.p + q;
+                w->x = r.x;
+                w->y = r.y;
+                w->vy = r.vy;
+                w->tl = r.tl;
+            }
             cq = r.coll;
             const double ox = isB ? b.x : a.x, oy = isB ? b.y : a.y;
             const double ovx = isB ? b.vx : a.vx, ovy = isB ? b.vy : a.vy;
@@ -716,10 +799,10 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const AR& A, 
                                                int& pb, uint32_t& cb, bool& overlap,
                                                int32_t* stage) {
     if (C.cap <= EDMD_STAGE_CAP && (C.cap & 3) == 0)
-        predict_owners_staged<G>(C, A, lane, t, nown, a, b, dta, pa, ca, dtb, pb, cb, overlap,
-                                 stage);
+        predict_owners_staged<G, W>(C, A, lane, t, nown, a, b, dta, pa, ca, dtb, pb, cb, overlap,
+                                    stage);
     else
-        predict_owners_gather<G>(C, A, lane, t, nown, a, b, dta, pa, ca, dtb, pb, cb, overlap);
+        predict_owners_gather<G, W>(C, A, lane, t, nown, a, b, dta, pa, ca, dtb, pb, cb, overlap);
     if (W && !G && C.walls) {
         /* the wall time starts the reference's minimum: a neighbour
          * replaces it only when strictly earlier */
@@ -980,7 +1063,7 @@ __device__ __forceinline__ bool sweep_one(const EdmdConst& C, const AR& A, doubl
     const EdmdPart o = A.p[p];
     double tc;
     int dir;
-    cross_time(C, t, o.x, o.y, o.vx, o.vy, o.cx, o.cy, tc, dir);
+    cross_time<W>(C, t, o.x, o.y, o.vx, o.vy, o.cx, o.cy, tc, dir);
     bool ov = false;
     int partner;
     double dt = predict_seq<G, W>(C, A, t, p, o, partner, ov);
@@ -1390,7 +1473,9 @@ __global__ __launch_bounds__(256) void k_sync_all(const EdmdSys* __restrict__ sy
         fly_grow(S->c, t, x, y, r->vx, r->vy, tl, rad, S->a.vr[p]);
         r->rad = rad;
     } else {
-        fly(S->c, t, x, y, r->vx, r->vy, tl);
+        double vy = r->vy;
+        fly_v<true>(S->c, t, x, y, r->vx, vy, tl);
+        r->vy = vy;
     }
     r->x = x;
     r->y = y;
@@ -1604,7 +1689,7 @@ __device__ __forceinline__ double ordered_sum(int n, int lane, F f) {
     return acc;
 }
 
-template <bool G, class AR>
+template <bool G, bool W, class AR>
 __device__ void ev_screenshot(const EdmdConst& C, const AR& A, int lane, LoopRegs& R) {
     /* takeAScreenshot, EDMD.c:4657-4677: free-fly every particle */
     const double t = R.t;
@@ -1616,7 +1701,9 @@ __device__ void ev_screenshot(const EdmdConst& C, const AR& A, int lane, LoopReg
             fly_grow(C, t, x, y, r->vx, r->vy, tl, rad, A.vr[q]);
             r->rad = rad;
         } else {
-            fly(C, t, x, y, r->vx, r->vy, tl);
+            double vy = r->vy;
+            fly_v<W>(C, t, x, y, r->vx, vy, tl);
+            if (W) r->vy = vy;
         }
         r->x = x;
         r->y = y;
@@ -1709,7 +1796,7 @@ __device__ void ev_rescale(const EdmdConst& C, const AR& A, int lane, LoopRegs& 
         EdmdPart r = A.p[q];
         r.coll = r.coll + 1;
         if (G) fly_grow(C, t, r.x, r.y, r.vx, r.vy, r.tl, r.rad, A.vr[q]);
-        else fly(C, t, r.x, r.y, r.vx, r.vy, r.tl);
+        else fly_v<W>(C, t, r.x, r.y, r.vx, r.vy, r.tl);
         r.vx /= sqrt(E / C.n / C.temperature);
         r.vy /= sqrt(E / C.n / C.temperature);
         A.p[q] = r;
@@ -1759,9 +1846,10 @@ __device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs
     const uint32_t collj = (uint32_t)uni((int)pj.coll);
     const uint32_t ca = (uint32_t)uni((int)ei.cact);
 
-    fly(C, t, xi, yi, vxi, vyi, tli);
+    fly_v<W>(C, t, xi, yi, vxi, vyi, tli);
     xi = uni(xi);
     yi = uni(yi);
+    vyi = uni(vyi);
     const bool valid = (ca == collj);
     trace_push(C, A, lane, t, EDMD_EV_COLLISION, i, j, valid ? 1 : 0);
     if (!valid) {
@@ -1769,6 +1857,7 @@ __device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs
         if (lane == 0) {
             A.p[i].x = xi;
             A.p[i].y = yi;
+            A.p[i].vy = vyi;
             A.p[i].tl = t;
         }
         WAVE_SYNC();
@@ -1797,7 +1886,8 @@ __device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs
     double xj = uni(pj.x), yj = uni(pj.y), vxj = uni(pj.vx), vyj = uni(pj.vy), tlj = uni(pj.tl);
     const double rj = uni(pj.rad), mj = uni(pj.m);
     const int cxj = uni((int)pj.cx), cyj = uni((int)pj.cy);
-    fly(C, t, xj, yj, vxj, vyj, tlj);
+    fly_v<W>(C, t, xj, yj, vxj, vyj, tlj);
+    vyj = uni(vyj);
     double dx = xj - xi;
     double dy = yj - yi;
     double dvx = vxj - vxi;
@@ -1847,8 +1937,8 @@ __device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs
     flag_overlap(R, ov, i);
     double tci2, tcj2;
     int di, dj;
-    cross_time(C, t, xi, yi, vxi, vyi, cxi, cyi, tci2, di);
-    cross_time(C, t, xj, yj, vxj, vyj, cxj, cyj, tcj2, dj);
+    cross_time<W>(C, t, xi, yi, vxi, vyi, cxi, cyi, tci2, di);
+    cross_time<W>(C, t, xj, yj, vxj, vyj, cxj, cyj, tcj2, dj);
     const double tki = t + dti, tkj = t + dtj;
     /* insertion order, EDMD.c:3869-3879: crossings of i and j, then the
      * collisions of i and j */
@@ -1972,8 +2062,8 @@ __device__ void ev_collision_grow(const EdmdConst& C, const AR& A, int lane, Loo
     flag_overlap(R, ov, i);
     double tci2, tcj2;
     int di, dj;
-    cross_time(C, t, xi, yi, vxi, vyi, cxi, cyi, tci2, di);
-    cross_time(C, t, xj, yj, vxj, vyj, cxj, cyj, tcj2, dj);
+    cross_time<false>(C, t, xi, yi, vxi, vyi, cxi, cyi, tci2, di);
+    cross_time<false>(C, t, xj, yj, vxj, vyj, cxj, cyj, tcj2, dj);
     const double tki = t + dti, tkj = t + dtj;
     /* insertion order, EDMD.c:3869-3879: crossings of i and j, then the
      * collisions of i and j */
@@ -2036,7 +2126,7 @@ __device__ void ev_stop_grow(const EdmdConst& C, const AR& A, int lane, LoopRegs
     bool ov = false;
     /* per particle: collision, then crossing (EDMD.c:4770-4776) */
     for (int q = lane; q < n; q += 64)
-        ov |= sweep_one<false>(C, A, t, q, R.q->paul, R.q->ctr + 2 * (int64_t)q + 1,
+        ov |= sweep_one<false, false>(C, A, t, q, R.q->paul, R.q->ctr + 2 * (int64_t)q + 1,
                                R.q->ctr + 2 * (int64_t)q, true);
     R.q->ctr += 2 * (int64_t)n;
     WAVE_SYNC();
@@ -2077,10 +2167,11 @@ __device__ void ev_single(const EdmdConst& C, const AR& A, int lane, LoopRegs& R
         fly_grow(C, t, x, y, vx, vy, tl, r, gvr);
         r = uni(r);
     } else {
-        fly(C, t, x, y, vx, vy, tl);
+        fly_v<W>(C, t, x, y, vx, vy, tl);
     }
     x = uni(x);
     y = uni(y);
+    vy = uni(vy);
 
     if (wall) {
         R.ncol++;
@@ -2154,6 +2245,7 @@ __device__ void ev_single(const EdmdConst& C, const AR& A, int lane, LoopRegs& R
             q->x = x;
             q->y = y;
             q->tl = tl;
+            if (W) q->vy = vy; /* advanced by the field's flight */
             if (G) q->rad = r;
             q->cx = (int16_t)cx;
             q->cy = (int16_t)cy;
@@ -2171,7 +2263,7 @@ __device__ void ev_single(const EdmdConst& C, const AR& A, int lane, LoopRegs& R
     flag_overlap(R, ov, p);
     double tc;
     int dir;
-    cross_time(C, t, x, y, vx, vy, cx, cy, tc, dir);
+    cross_time<W>(C, t, x, y, vx, vy, cx, cy, tc, dir);
     const double tk = t + dt;
     /* crossingEvent(i), then collisionEvent(i), EDMD.c:2587-2589, 3458-3459 */
     const int64_t rx = take_rank(C, R, tc);
@@ -2230,7 +2322,7 @@ __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRu
             R.t = R.next_special;
             trace_push(C, A, lane, R.t, sp, 0, 0, 1);
             if (sp == EDMD_EV_SCREENSHOT) {
-                ev_screenshot<G>(C, A, lane, R);
+                ev_screenshot<G, W>(C, A, lane, R);
             } else if (sp == EDMD_EV_THERMO) {
                 ev_thermo(C, A, lane, R);
             } else if (sp == EDMD_EV_GROWSTOP) {
@@ -2544,7 +2636,9 @@ __global__ void k_fly_one(const EdmdSys* __restrict__ systems, int i) {
         fly_grow(C, A.st->t, x, y, r->vx, r->vy, tl, rad, A.vr[i]);
         r->rad = rad;
     } else {
-        fly(C, A.st->t, x, y, r->vx, r->vy, tl);
+        double vy = r->vy;
+        fly_v<true>(C, A.st->t, x, y, r->vx, vy, tl);
+        r->vy = vy;
     }
     r->x = x;
     r->y = y;

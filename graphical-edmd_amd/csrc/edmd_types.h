@@ -82,6 +82,9 @@ typedef struct EdmdConst {
     double res_wall, res_tan; /* resW, resT (EDMD.c:372-373) */
     /* minimum-image half widths: halfLx / halfLy, +inf across walls */
     double mihx, mihy;
+    int32_t field_on; /* uniform field along y (addField) */
+    int32_t pad_f;
+    double field;     /* its value (< 0) */
 } EdmdConst;
 
 /* Order among events of equal time.  The reference pops equal times in the

@@ -89,6 +89,14 @@ typedef struct edmd_system_desc {
      * a growth start (vr > 0) or taping. */
     int wall_x, wall_y;
     double res_wall, res_tan;
+    /* Uniform field along y (addField / field of the graphics build,
+     * EDMD.c:265, 366, -f): free flight parabolic in y (EDMD.c:4957-4963),
+     * every neighbour flown to the current time when its collision time is
+     * taken (EDMD.c:2664-2666), crossings and y-wall hits solving the
+     * quadratic (EDMD.c:2428-2442, 2846-2860).  field < 0 (default -0.02).
+     * Not with a growth start or taping. */
+    int add_field;
+    double field;
 } edmd_system_desc;
 
 /* A collision slot holding a wall event reports partner -5 - xy

@@ -92,12 +92,19 @@ static double wrap_post(double v, double L) {
     return v;
 }
 
-/* freeFlyNormal, EDMD.c:4954-4990 (no field, no damping). */
+/* freeFlyNormal, EDMD.c:4954-4990 (no damping): straight, or parabolic in
+ * y under the field. */
 static void free_fly(orc_ctx* s, int p) {
     double dt = s->t - s->tl[p];
     s->tl[p] = s->t;
-    s->x[p] += dt * s->vx[p];
-    s->y[p] += dt * s->vy[p];
+    if (s->prm.add_field) {
+        s->x[p] += dt * s->vx[p];
+        s->y[p] += dt * s->vy[p] + 0.5 * s->prm.field * dt * dt;
+        s->vy[p] += dt * s->prm.field;
+    } else {
+        s->x[p] += dt * s->vx[p];
+        s->y[p] += dt * s->vy[p];
+    }
     s->x[p] = wrap_post(s->x[p], s->Lx);
     s->y[p] = wrap_post(s->y[p], s->Ly);
 }
@@ -284,7 +291,10 @@ static int wrap_cell(int a, int nc) {
 /* ------------------------------------------------------------ predictions */
 
 static double coll_time(orc_ctx* s, int i, int j) {
-    /* collisionTimeNormal, EDMD.c:2661-2723 (no damping, field or shear). */
+    /* collisionTimeNormal, EDMD.c:2661-2723 (no damping or shear); under the
+     * field the neighbour is flown to the current time first, a state
+     * update of the neighbour (EDMD.c:2664-2666). */
+    if (s->prm.add_field) free_fly(s, j);
     double lat2 = s->t - s->tl[j];
     double dvx = s->vx[j] - s->vx[i];
     double dvy = s->vy[j] - s->vy[i];
@@ -373,7 +383,25 @@ static void predict_collision(orc_ctx* s, int i, double* t_abs, int* partner) {
      * gets the time to contact.  The x test sets xy = 0 whenever it applies,
      * also when the y wall stays the earlier, as the reference does. */
     int wall = 0, xy = 2;
-    if (s->prm.wall_y) {
+    if (s->prm.wall_y && s->prm.add_field) {
+        /* EDMD.c:2846-2860: the bottom wall from the bottom row whatever the
+         * velocity, the top wall for any disk moving up (NaN when it cannot
+         * reach it) */
+        double f = s->prm.field, vy = s->vy[i];
+        if (Y == 0) {
+            dt = (-vy - sqrt(vy * vy - 2 * f * (s->y[i] - s->rad[i]))) / f;
+            wall = 1;
+            xy = 1;
+        }
+        if (vy > 0) {
+            double dt2 = (-vy + sqrt(vy * vy - 2 * f * (s->y[i] - (s->Ly - s->rad[i])))) / f;
+            if (!isnan(dt2) && dt2 < dt) {
+                dt = dt2;
+                wall = 1;
+                xy = 1;
+            }
+        }
+    } else if (s->prm.wall_y) {
         if (Y == 0 && s->vy[i] < 0) {
             dt = -(s->y[i] - s->rad[i]) / s->vy[i];
             wall = 1;
@@ -430,7 +458,20 @@ static void predict_crossing(const orc_ctx* s, int i, double* t_abs, int* dir) {
         tx = wrap_inside((1 + s->cx[i]) * s->csx - s->x[i], s->halfLx, s->Lx) / s->vx[i];
         xx = 2;
     }
-    if (s->vy[i] < 0) {
+    if (s->prm.add_field) {
+        /* EDMD.c:2428-2442: up through the top edge when fast enough,
+         * else down through the bottom edge */
+        double f = s->prm.field, vy = s->vy[i];
+        double dy = wrap_inside(s->y[i] - (1 + s->cy[i]) * s->csy, s->halfLy, s->Ly);
+        if (dy < 0 && vy > sqrt(2 * f * dy)) {
+            ty = (-vy + sqrt(vy * vy - 2 * f * dy)) / f;
+            yy = 4;
+        } else {
+            dy = wrap_inside(s->y[i] - s->cy[i] * s->csy, s->halfLy, s->Ly);
+            ty = (-vy - sqrt(vy * vy - 2 * f * dy)) / f;
+            yy = 3;
+        }
+    } else if (s->vy[i] < 0) {
         ty = wrap_inside(s->cy[i] * s->csy - s->y[i], s->halfLy, s->Ly) / s->vy[i];
         yy = 3;
     } else {
@@ -813,6 +854,8 @@ void orc_default_params(orc_params* p) {
     p->wall_y = 0;
     p->res_wall = 1; /* resW, EDMD.c:372 */
     p->res_tan = 0;  /* resT, EDMD.c:373 */
+    p->add_field = 0;
+    p->field = -0.02; /* EDMD.c:366 */
 }
 
 orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* y,
@@ -825,7 +868,10 @@ orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* 
     if (prm) s->prm = *prm;
     else orc_default_params(&s->prm);
     /* walls only with the Normal dispatch (doTheWallGrow is not restated) */
-    if ((s->prm.wall_x || s->prm.wall_y) && s->prm.vr > 0) { free(s); return NULL; }
+    if ((s->prm.wall_x || s->prm.wall_y || s->prm.add_field) && s->prm.vr > 0) {
+        free(s);
+        return NULL;
+    }
     /* boxConstantHelper, EDMD.c:679-715 */
     s->Lx = Lx;
     s->Ly = Ly;

@@ -60,6 +60,13 @@ typedef struct {
      * EDMD.c:372-373, 3384-3430). */
     int wall_x, wall_y;
     double res_wall, res_tan;
+    /* Uniform field along y (addField / field, EDMD.c:265, 366, -f; the
+     * command-line build fixes addField at 0): free flight is parabolic in
+     * y (EDMD.c:4957-4963), every neighbour is flown to the current time
+     * when its collision time is taken (EDMD.c:2664-2666), crossings and
+     * y-wall hits solve the quadratic (EDMD.c:2428-2442, 2846-2860). */
+    int add_field;
+    double field;        /* must be < 0, default -0.02 (EDMD.c:365-366) */
 } orc_params;
 
 typedef struct {

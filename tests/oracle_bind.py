@@ -23,7 +23,7 @@ class Params(C.Structure):
                 ("dtnoise", C.c_double), ("temperature", C.c_double),
                 ("bucket_count", C.c_int), ("vr", C.c_double), ("einit", C.c_double),
                 ("wall_x", C.c_int), ("wall_y", C.c_int), ("res_wall", C.c_double),
-                ("res_tan", C.c_double)]
+                ("res_tan", C.c_double), ("add_field", C.c_int), ("field", C.c_double)]
 
 
 class ThermoRow(C.Structure):

@@ -149,3 +149,63 @@ def test_wall_arguments_are_checked(gpu):
     b = gpu.Batch([st], wall_y=1)
     with pytest.raises(gpu.EdmdError):
         b.deriv_begin(checkpoint_every=256)
+
+
+# ---- the uniform field along y (addField)
+
+FIELD_CASES = [
+    (400, 0.4, 51, dict(wall_y=1, res_tan=1.0, add_field=1, field=-0.05)),   # gravity, channel
+    (350, 0.5, 52, dict(wall_x=1, wall_y=1, add_field=1, field=-0.1)),       # closed box, resT 0
+    (300, 0.5, 53, dict(add_field=1, field=-0.02)),                          # periodic, free fall
+]
+
+
+@pytest.mark.parametrize("n,phi,seed,kw", FIELD_CASES)
+def test_field_traces_match_oracle(gpu, oracle, n, phi, seed, kw):
+    """Parabolic flights, the neighbours flown (and stored) whenever their
+    collision time is taken, quadratic crossing and wall times: traces and
+    states bit for bit against the oracle, from HBM and from LDS."""
+    st = walled(n, phi, seed, kw)
+    nev = 20000
+    o = oracle.Oracle(st, **kw, **QUIET)
+    code, tr = o.run(nev)
+    assert code == 0
+    so = o.state()
+    for mode in (gpu.RES_HBM, gpu.RES_LDS):
+        b = gpu.Batch([st], trace_capacity=nev, **kw, **QUIET)
+        b.set_residency(mode)
+        b.run(max_events=nev)
+        assert_trace_equal(b.trace(0), tr)
+        assert_state_equal(b.state(0), so)
+
+
+def test_field_with_thermostat_and_screenshots(gpu, oracle):
+    """Screenshots (everyone flown), thermo rows and velocity rescales under
+    the field, in a channel with walls."""
+    kw = dict(wall_y=1, res_tan=1.0, add_field=1, field=-0.04, first_screen=0.7, dtime=0.7,
+              first_thermo=0.5, dtime_thermo=0.5, noise=2, dtnoise=0.9, temperature=1.2)
+    st = walled(450, 0.45, 54, kw)
+    nev = 25000
+    s = gpu.System(st, trace_capacity=nev, thermo_capacity=512, **kw)
+    s.run(max_events=nev)
+    o = oracle.Oracle(st, **kw)
+    code, tr = o.run(nev)
+    assert code == 0
+    assert_trace_equal(s.trace(), tr)
+    assert_state_equal(s.state(), o.state())
+    rows, orows = s.thermo(), o.thermo_rows()
+    assert len(rows) == len(orows) > 5
+    for r, q in zip(rows, orows):
+        for k in ("t", "e_per_n", "pressure", "pwall"):
+            assert bits(np.float64(r[k])) == bits(np.float64(q[k])), k
+
+
+def test_field_arguments_are_checked(gpu):
+    st = F.lattice_state(100, 0.4, seed=5, lattice="fit")
+    with pytest.raises(gpu.EdmdError):
+        gpu.System(st, add_field=1, field=0.1)
+    with pytest.raises(gpu.EdmdError):
+        gpu.System(st, add_field=1, vr=0.1)
+    b = gpu.Batch([st], add_field=1)
+    with pytest.raises(gpu.EdmdError):
+        b.deriv_begin(checkpoint_every=256)

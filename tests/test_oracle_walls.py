@@ -13,6 +13,7 @@ import pytest
 import edmd_fixtures as F
 
 WALL = 5
+QUIET = dict(first_screen=1e9, dtime=1e9, first_thermo=1e9)
 
 
 def disks(pos, vel, L=20.0, rad=0.5):
@@ -136,3 +137,69 @@ def test_walls_with_growth_are_rejected(oracle):
     st = F.lattice_state(50, 0.3, seed=1, lattice="fit")
     with pytest.raises(ValueError):
         oracle.Oracle(st, wall_x=1, vr=0.1)
+
+
+# ---- the uniform field along y (addField, EDMD.c:2428-2442, 2664-2666,
+# 2846-2860, 4957-4963)
+
+def test_field_bounce_closed_form(oracle):
+    """A disk dropped from rest at height h above contact in a field f < 0
+    between y walls (resW = 1) hits the floor at t = sqrt(2h/|f|) and then
+    every 2 sqrt(2h/|f|): the flights are exact parabolas and the bounce
+    keeps the speed."""
+    f = -0.5
+    h = 6.0
+    st = disks([(3.0, 0.5 + h), (13.0, 10.0)], [(0.0, 0.0), (0.0, 0.0)])
+    o = oracle.Oracle(st, wall_y=1, add_field=1, field=f, res_tan=1.0, **QUIET)
+    code, tr = o.run(300)
+    assert code == 0
+    w = tr[(tr["type"] == WALL) & (tr["i"] == 0)]
+    assert len(w) >= 5 and np.all(w["j"] == 1)
+    t1 = np.sqrt(2 * h / -f)
+    np.testing.assert_allclose(w["t"], t1 * (1 + 2 * np.arange(len(w))), rtol=1e-12)
+    # the resting disk falls too and bounces with its own period
+    w1 = tr[(tr["type"] == WALL) & (tr["i"] == 1)]
+    np.testing.assert_allclose(w1["t"][0], np.sqrt(2 * 9.5 / -f), rtol=1e-12)
+
+
+def test_field_crossing_closed_form(oracle):
+    """Crossing times under the field against the other root formula: up
+    through the top edge when vy > sqrt(2 f dy), else down through the
+    bottom edge."""
+    f = -0.3
+    for vy, expect_dir in ((2.0, 4), (0.2, 3), (-1.0, 3)):
+        st = disks([(3.0, 4.7), (13.0, 13.0)], [(0.0, vy), (0.0, 0.0)])
+        o = oracle.Oracle(st, add_field=1, field=f, **QUIET)
+        t, d = o.predict_crossing(0)
+        assert d == expect_dir
+        # cell 2 spans [4, 6): top edge 1.3 above, bottom edge 0.7 below
+        g = 1.3 if d == 4 else -0.7
+        # root of g = vy t + f t^2 / 2 in the cancellation-free form
+        disc = np.sqrt(vy * vy + 2 * f * g)
+        tt = 2 * g / (vy + disc) if d == 4 else 2 * g / (vy - disc)
+        assert t == pytest.approx(tt, rel=1e-12)
+
+
+def test_field_box_conserves_total_energy(oracle):
+    """Elastic disks (res 1) in a closed box under the field with specular
+    walls (resW = resT = 1): kinetic plus potential energy (-m f y) stays
+    constant through collisions, wall hits and parabolic flights."""
+    f = -0.05
+    st = F.lattice_state(200, 0.4, seed=17, lattice="fit")
+    L = st["Lx"]
+    st["x"] = 0.51 + st["x"] / L * (L - 1.02)
+    st["y"] = 0.51 + st["y"] / L * (L - 1.02)
+    o = oracle.Oracle(st, wall_x=1, wall_y=1, res_tan=1.0, add_field=1, field=f, **QUIET)
+
+    def total():
+        s = o.state()
+        T = o.t
+        dt = T - s["t"]
+        vy = s["vy"] + dt * f
+        y = s["y"] + dt * s["vy"] + 0.5 * f * dt * dt
+        return 0.5 * np.sum(s["vx"] ** 2 + vy ** 2) - f * np.sum(y)
+
+    e0 = total()
+    code, tr = o.run(30000)
+    assert code == 0 and np.sum(tr["type"] == WALL) > 200
+    assert abs(total() / e0 - 1) < 1e-11
