@@ -66,6 +66,8 @@ extern "C" void edmd_default_desc(edmd_system_desc* d) {
     d->res_tan = 0;  /* resT, EDMD.c:373 */
     d->add_field = 0;
     d->field = -0.02; /* EDMD.c:366 */
+    d->damping = 0;
+    d->gamma = 0.02; /* gamm, EDMD.c:382 */
 }
 
 /* ------------------------------------------------------------ handles */
@@ -130,7 +132,7 @@ struct edmd_batch {
     size_t lds_dyn[2] = {0, 0};
     int64_t lds_slots = 0;
     bool use_lds = false;
-    bool any_walls = false; /* some system has walls: the loop's wall variant */
+    bool any_walls = false; /* some system has walls, the field or damping: the kernels' variants for them */
 };
 
 struct edmd_ctx {
@@ -239,8 +241,16 @@ static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
         return fail(EDMD_ERR_ARG, "system %d: the field must be negative", idx);
     if (d->add_field && d->vr > 0)
         return fail(EDMD_ERR_ARG, "system %d: a field needs a start without growth", idx);
+    if (d->damping != 0 && d->damping != 1)
+        return fail(EDMD_ERR_ARG, "system %d: damping must be 0 or 1", idx);
+    if (d->damping && !(d->gamma > 0))
+        return fail(EDMD_ERR_ARG, "system %d: the damping rate must be positive", idx);
+    if (d->damping && (d->add_field || d->vr > 0))
+        return fail(EDMD_ERR_ARG, "system %d: damping goes without the field or growth", idx);
     c.field_on = d->add_field;
     c.field = d->field;
+    c.damping = d->damping;
+    c.gamma = d->gamma;
     c.walls = (d->wall_x ? 1 : 0) | (d->wall_y ? 2 : 0);
     c.res_wall = d->res_wall;
     c.res_tan = d->res_tan;
@@ -379,7 +389,7 @@ static int upload_system(edmd_batch* b, int s, const edmd_system_desc* d) {
 }
 
 static int batch_predict_all(edmd_batch* b) {
-    HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->sweep_variant, b->stream));
+    HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->sweep_variant, b->any_walls ? 1 : 0, b->stream));
     if (b->any_grow) HIPCHK(edmd_launch_sweep_grow(b->d_sys, b->nsys, b->max_n, b->stream));
     HIPCHK(edmd_launch_sweep_count(b->d_sys, b->nsys, b->stream));
     HIPCHK(edmd_launch_tree(b->d_sys, b->nsys, b->max_lsize, b->max_nlev, b->stream));
@@ -450,7 +460,8 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
     if (!rc) rc = dalloc(b, &b->erank, 2 * np);
     if (!rc) rc = dalloc(b, &b->lrank, np);
     for (int s = 0; s < nsys; s++) b->any_grow |= descs[s].vr > 0;
-    for (int s = 0; s < nsys; s++) b->any_walls |= b->hs[s].c.walls != 0 || b->hs[s].c.field_on;
+    for (int s = 0; s < nsys; s++)
+        b->any_walls |= b->hs[s].c.walls != 0 || b->hs[s].c.field_on || b->hs[s].c.damping;
     if (!rc && b->any_grow) rc = dalloc(b, &b->vr, np);
     if (rc) { batch_free(b); return rc; }
     if (hipStreamCreate(&b->stream) != hipSuccess) {
@@ -689,10 +700,10 @@ extern "C" int edmd_batch_autotune(edmd_batch* b, int sweep_reps, int64_t trial_
         double bt = 1e300;
         for (int v = 0; v < EDMD_SWEEP_VARIANTS && e == hipSuccess; v++) {
             set_sweep_variant(b, v);
-            e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, v, b->stream);
+            e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, v, b->any_walls ? 1 : 0, b->stream);
             if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
             for (int k = 0; k < sweep_reps && e == hipSuccess; k++)
-                e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, v, b->stream);
+                e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, v, b->any_walls ? 1 : 0, b->stream);
             if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
@@ -857,7 +868,7 @@ extern "C" int edmd_batch_time_sweep(edmd_batch* b, int reps, double* avg_ms) {
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, b->stream));
     for (int r = 0; r < reps; r++)
-        HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->sweep_variant, b->stream));
+        HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->sweep_variant, b->any_walls ? 1 : 0, b->stream));
     HIPCHK(hipEventRecord(e1, b->stream));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
@@ -1594,8 +1605,9 @@ extern "C" int edmd_batch_deriv_begin(edmd_batch* b, int64_t checkpoint_every, i
     if (b->any_grow)
         return fail(EDMD_ERR_ARG, "derivatives need systems whose growth has ended");
     for (int s = 0; s < b->nsys; s++)
-        if (b->hs[s].c.walls || b->hs[s].c.field_on)
-            return fail(EDMD_ERR_ARG, "system %d: derivatives need a system without walls or field", s);
+        if (b->hs[s].c.walls || b->hs[s].c.field_on || b->hs[s].c.damping)
+            return fail(EDMD_ERR_ARG,
+                        "system %d: derivatives need a system without walls, field or damping", s);
     int rc = edmd_batch_deriv_end(b);
     if (rc) return rc;
     if ((double)b->nsys * (double)cap * sizeof(EdmdTapeRec) > 1.0e12)

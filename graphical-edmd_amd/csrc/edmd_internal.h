@@ -13,8 +13,10 @@ typedef struct EdmdSweepShape {
 } EdmdSweepShape;
 #define EDMD_SWEEP_VARIANTS 8
 EdmdSweepShape edmd_sweep_shape(int variant);
+/* ext: some system has walls, the field or damping (the variant of the
+ * kernel that handles them) */
 hipError_t edmd_launch_sweep_tiled(const EdmdSys* d_sys, int nsys, int max_tiles, int variant,
-                                   hipStream_t s);
+                                   int ext, hipStream_t s);
 int edmd_sweep_tiles(int ncx, int ncy, int variant);
 hipError_t edmd_launch_tree(const EdmdSys* d_sys, int nsys, const int* max_lsize, int max_nlev,
                             hipStream_t s);

@@ -156,6 +156,55 @@ __device__ __forceinline__ void min_image(const EdmdConst& C, double& dx, double
     else if (dy < -C.mihy) dy += C.Ly;
 }
 
+/* e^x from its Taylor series: x = k ln2 + r, |r| <= ln2/2, ln2 in two
+ * parts so that k ln2 is exact (Cody and Waite's reduction), e^r by the
+ * nested form 1 + r(1 + r/2(1 + r/3(... (1 + r/17)))) (truncation below
+ * 2e-24 relative), times 2^k; within 4 ulp.  Only +, *, rint and ldexp,
+ * so the oracle's restatement of the same steps gives the same bits. */
+__device__ __forceinline__ double edmd_exp(double x) {
+    if (x != x) return x;
+    if (x > 709.5) return __builtin_huge_val();
+    if (x < -745.5) return 0.0;
+    const double k = rint(x * 1.4426950408889634); /* x / ln2, to nearest */
+    const double r = (x - k * 0x1.62e42fefa3800p-1) - k * 0x1.ef35793c76730p-45;
+    double p = 1.0;
+#pragma unroll
+    for (int n = 17; n >= 1; --n) p = 1.0 + (r * p) * (1.0 / n);
+    return ldexp(p, (int)k);
+}
+
+/* log x from log m = 2 atanh s = 2(s + s^3/3 + ... + s^23/23),
+ * s = (m - 1)/(m + 1), x = m 2^e with m in [sqrt(1/2), sqrt(2)), plus
+ * e ln2 in two parts; within 4 ulp, the oracle's steps. */
+__device__ __forceinline__ double edmd_log(double x) {
+    if (x != x || x < 0) return __builtin_nan("");
+    if (x == 0) return -__builtin_huge_val();
+    if (x == __builtin_huge_val()) return x;
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.70710678118654752) {
+        m *= 2;
+        e -= 1;
+    }
+    const double s = (m - 1) / (m + 1), s2 = s * s;
+    double q = 1.0 / 23;
+#pragma unroll
+    for (int n = 21; n >= 1; n -= 2) q = (1.0 / n) + s2 * q;
+    const double lm = 2 * s * q;
+    return e * 0x1.62e42fefa3800p-1 + (e * 0x1.ef35793c76730p-45 + lm);
+}
+
+/* logTime, EDMD.c:5709-5721: under damping, the time at which the damped
+ * motion has covered what the undamped one covers in t; never when it
+ * cannot.  F = false compiles damping out. */
+template <bool F>
+__device__ __forceinline__ double log_time(const EdmdConst& C, double t) {
+    if (!(F && C.damping)) return t;
+    const double var = (1 - t * C.gamma);
+    if (var < 0) return EDMD_NEVER;
+    return -edmd_log(var) / C.gamma;
+}
+
 /* The wall part of collisionEventNormal (EDMD.c:2845-2895, no field, mid or
  * circular wall; logTime is the identity without damping): a disk in a
  * boundary cell moving towards the wall gets the time to contact, the
@@ -188,25 +237,25 @@ __device__ __forceinline__ bool wall_time(const EdmdConst& C, double x, double y
         }
     } else if (C.walls & 2) {
         if (cy == 0 && vy < 0) {
-            dt = -(y - r) / vy;
+            dt = log_time<true>(C, -(y - r) / vy);
             wall = true;
             xy = 1;
         } else if (cy == C.ncy - 1 && vy > 0) {
-            dt = (C.Ly - y - r) / vy;
+            dt = log_time<true>(C, (C.Ly - y - r) / vy);
             wall = true;
             xy = 1;
         }
     }
     if (C.walls & 1) {
         if (cx == 0 && vx < 0) {
-            const double dw = -(x - r) / vx;
+            const double dw = log_time<true>(C, -(x - r) / vx);
             if (dt > dw) {
                 wall = true;
                 dt = dw;
             }
             xy = 0;
         } else if (cx == C.ncx - 1 && vx > 0) {
-            const double dw = (C.Lx - x - r) / vx;
+            const double dw = log_time<true>(C, (C.Lx - x - r) / vx);
             if (dt > dw) {
                 wall = true;
                 dt = dw;
@@ -249,12 +298,24 @@ __device__ __forceinline__ void fly(const EdmdConst& C, double t, double& x, dou
     y = wrap_post(y, C.Ly);
 }
 
-/* freeFlyNormal with the field when the system has one (EDMD.c:4957-4963:
- * parabolic in y, vy advanced); F = false compiles the field out. */
+/* freeFlyNormal for a system with the field (EDMD.c:4957-4963: parabolic
+ * in y, vy advanced) or damping (EDMD.c:4965-4975: velocities times
+ * exp(-gamma dt)); F = false compiles both out. */
 template <bool F>
 __device__ __forceinline__ void fly_v(const EdmdConst& C, double t, double& x, double& y,
-                                      double vx, double& vy, double& tl) {
-    if (F && C.field_on) {
+                                      double& vx, double& vy, double& tl) {
+    if (F && C.damping) {
+        const double dt = t - tl;
+        tl = t;
+        const double g = C.gamma;
+        const double ex = edmd_exp(-g * dt);
+        x += vx / g * (1 - ex);
+        y += vy / g * (1 - ex);
+        vx *= ex;
+        vy *= ex;
+        x = wrap_post(x, C.Lx);
+        y = wrap_post(y, C.Ly);
+    } else if (F && C.field_on) {
         const double dt = t - tl;
         tl = t;
         x += dt * vx;
@@ -379,10 +440,10 @@ __device__ __forceinline__ void cross_time(const EdmdConst& C, double t, double 
     double tx, ty;
     int xx, yy;
     if (vx < 0) {
-        tx = wrap_inside(cx * C.csx - x, C.halfLx, C.Lx) / vx;
+        tx = log_time<F>(C, wrap_inside(cx * C.csx - x, C.halfLx, C.Lx) / vx);
         xx = 1;
     } else {
-        tx = wrap_inside((1 + cx) * C.csx - x, C.halfLx, C.Lx) / vx;
+        tx = log_time<F>(C, wrap_inside((1 + cx) * C.csx - x, C.halfLx, C.Lx) / vx);
         xx = 2;
     }
     if (F && C.field_on) {
@@ -399,10 +460,10 @@ __device__ __forceinline__ void cross_time(const EdmdConst& C, double t, double 
             yy = 3;
         }
     } else if (vy < 0) {
-        ty = wrap_inside(cy * C.csy - y, C.halfLy, C.Ly) / vy;
+        ty = log_time<F>(C, wrap_inside(cy * C.csy - y, C.halfLy, C.Ly) / vy);
         yy = 3;
     } else {
-        ty = wrap_inside((1 + cy) * C.csy - y, C.halfLy, C.Ly) / vy;
+        ty = log_time<F>(C, wrap_inside((1 + cy) * C.csy - y, C.halfLy, C.Ly) / vy);
         yy = 4;
     }
     if (tx < ty) {
@@ -473,13 +534,14 @@ __device__ __forceinline__ double predict_seq(const EdmdConst& C, const AR& A, d
                 int q = sl[s];
                 if (q == p) continue;
                 EdmdPart c2 = A.p[q];
-                if (W && !G && C.field_on) {
+                if (W && !G && (C.field_on || C.damping)) {
                     /* collisionTimeNormal flies the neighbour first under
-                     * the field, a state update (EDMD.c:2664-2666) */
+                     * the field or damping, a state update (EDMD.c:2664-2666) */
                     fly_v<true>(C, t, c2.x, c2.y, c2.vx, c2.vy, c2.tl);
                     const auto w = A.p + q;
                     w->x = c2.x;
                     w->y = c2.y;
+                    w->vx = c2.vx;
                     w->vy = c2.vy;
                     w->tl = c2.tl;
                 }
@@ -488,8 +550,9 @@ __device__ __forceinline__ double predict_seq(const EdmdConst& C, const AR& A, d
                     tt = coll_time_grow(C, t, o.x, o.y, o.vx, o.vy, o.rad, ovr, c2.x, c2.y, c2.vx,
                                         c2.vy, c2.tl, c2.rad, A.vr[q], overlap);
                 else
-                    tt = coll_time(C, t, o.x, o.y, o.vx, o.vy, o.rad, c2.x, c2.y, c2.vx, c2.vy,
-                                   c2.tl, c2.rad, overlap);
+                    tt = log_time<W && !G>(
+                        C, coll_time(C, t, o.x, o.y, o.vx, o.vy, o.rad, c2.x, c2.y, c2.vx, c2.vy,
+                                     c2.tl, c2.rad, overlap));
                 if (dt > tt) {
                     dt = tt;
                     best = q;
@@ -577,17 +640,17 @@ __device__ __forceinline__ void predict_owners_gather(const EdmdConst& C, const 
         uint32_t cA = 0, cB = 0;
         if (vA) { rA = A.p[qA]; cA = rA.coll; }
         if (vB) { rB = A.p[qB]; cB = rB.coll; }
-        if (W && !G && C.field_on) {
+        if (W && !G && (C.field_on || C.damping)) {
             /* the neighbours flown to t and stored (EDMD.c:2664-2666) */
             if (vA) {
                 fly_v<true>(C, t, rA.x, rA.y, rA.vx, rA.vy, rA.tl);
                 const auto w = A.p + qA;
-                w->x = rA.x; w->y = rA.y; w->vy = rA.vy; w->tl = rA.tl;
+                w->x = rA.x; w->y = rA.y; w->vx = rA.vx; w->vy = rA.vy; w->tl = rA.tl;
             }
             if (vB) {
                 fly_v<true>(C, t, rB.x, rB.y, rB.vx, rB.vy, rB.tl);
                 const auto w = A.p + qB;
-                w->x = rB.x; w->y = rB.y; w->vy = rB.vy; w->tl = rB.tl;
+                w->x = rB.x; w->y = rB.y; w->vx = rB.vx; w->vy = rB.vy; w->tl = rB.tl;
             }
         }
         if (G && vA) gA = A.vr[qA];
@@ -599,8 +662,8 @@ __device__ __forceinline__ void predict_owners_gather(const EdmdConst& C, const 
                 tt = coll_time_grow(C, t, a.x, a.y, a.vx, a.vy, a.r, a.vr, rA.x, rA.y, rA.vx,
                                     rA.vy, rA.tl, rA.rad, gA, overlap);
             else
-                tt = coll_time_nb(C, t, a.x, a.y, a.vx, a.vy, a.r, rA.x, rA.y, rA.vx, rA.vy, rA.tl,
-                               rA.rad, overlap);
+                tt = log_time<W>(C, coll_time_nb(C, t, a.x, a.y, a.vx, a.vy, a.r, rA.x, rA.y,
+                                                 rA.vx, rA.vy, rA.tl, rA.rad, overlap));
             if (tt < lim) tA = tt;
         }
         if (vB) {
@@ -609,8 +672,8 @@ __device__ __forceinline__ void predict_owners_gather(const EdmdConst& C, const 
                 tt = coll_time_grow(C, t, b.x, b.y, b.vx, b.vy, b.r, b.vr, rB.x, rB.y, rB.vx,
                                     rB.vy, rB.tl, rB.rad, gB, overlap);
             else
-                tt = coll_time_nb(C, t, b.x, b.y, b.vx, b.vy, b.r, rB.x, rB.y, rB.vx, rB.vy, rB.tl,
-                               rB.rad, overlap);
+                tt = log_time<W>(C, coll_time_nb(C, t, b.x, b.y, b.vx, b.vy, b.r, rB.x, rB.y,
+                                                 rB.vx, rB.vy, rB.tl, rB.rad, overlap));
             if (tt < lim) tB = tt;
         }
         double mA;
@@ -731,13 +794,14 @@ __device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const 
         uint32_t cq = 0;
         if (v) {
             EdmdPart r = A.p[q];
-            if (W && !G && C.field_on) {
+            if (W && !G && (C.field_on || C.damping)) {
                 /* the neighbour flown to t and stored (EDMD.c:2664-2666); a
                  * disk listed for both owners is written twice alike */
                 fly_v<true>(C, t, r.x, r.y, r.vx, r.vy, r.tl);
                 const auto w = A.p + q;
                 w->x = r.x;
                 w->y = r.y;
+                w->vx = r.vx;
                 w->vy = r.vy;
                 w->tl = r.tl;
             }
@@ -751,8 +815,8 @@ __device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const 
                 tt = coll_time_grow(C, t, ox, oy, ovx, ovy, orad, ovr, r.x, r.y, r.vx, r.vy, r.tl,
                                     r.rad, A.vr[q], overlap);
             } else {
-                tt = coll_time_nb(C, t, ox, oy, ovx, ovy, orad, r.x, r.y, r.vx, r.vy, r.tl, r.rad,
-                                  overlap);
+                tt = log_time<W>(C, coll_time_nb(C, t, ox, oy, ovx, ovy, orad, r.x, r.y, r.vx,
+                                                 r.vy, r.tl, r.rad, overlap));
             }
             if (tt < lim) tq = tt;
         }
@@ -1211,7 +1275,7 @@ __device__ __forceinline__ void pair_terms(const EdmdConst& C, double ox, double
  * takes), flagging overlaps as collisionTime does; pass 2 solves the marked
  * roots in increasing k.  A disk with more than 256 candidates solves its
  * roots in pass 1 instead. */
-template <class S, bool WRAP, class AR>
+template <class S, bool WRAP, bool X, class AR>
 __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR& A,
                                                    const SweepLds<S>& L, int tid, int ninner, double t,
                                                    int tx0, int ty0, double paul, int64_t ctr0,
@@ -1243,7 +1307,7 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
         int best = -1;
         int wxy = 2;
         bool wall = false;
-        if (C.walls) wall = wall_time(C, ox, oy, ovx, ovy, orad, tx0 + lx, ty0 + ly, dt, wxy);
+        if (X && C.walls) wall = wall_time(C, ox, oy, ovx, ovy, orad, tx0 + lx, ty0 + ly, dt, wxy);
         /* pass 1 without branches: every lane evaluates its k-th candidate
          * (a lane past its count re-reads its own disk, masked out); the
          * marks of 64 consecutive k collect in cur and go to their word
@@ -1285,7 +1349,7 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
                     if (b > 0) continue;
                     if (d < -0.01) ov = true;
                     if (det < 0) continue;
-                    const double tt = (-b - sqrt(det)) / v2;
+                    const double tt = log_time<X>(C, (-b - sqrt(det)) / v2);
                     if (dt > tt) {
                         dt = tt;
                         best = v;
@@ -1303,7 +1367,7 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
             double b, v2, d, det;
             pair_terms<WRAP>(C, ox, oy, ovx, ovy, orad, L.pos[v], L.vel[v], L.rad[v], b, v2, d,
                              det);
-            const double tt = (-b - sqrt(det)) / v2;
+            const double tt = log_time<X>(C, (-b - sqrt(det)) / v2);
             if (dt > tt) {
                 dt = tt;
                 best = v;
@@ -1312,7 +1376,7 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
         if (!act) continue;
         double tc;
         int dir;
-        cross_time(C, t, ox, oy, ovx, ovy, tx0 + lx, ty0 + ly, tc, dir);
+        cross_time<X>(C, t, ox, oy, ovx, ovy, tx0 + lx, ty0 + ly, tc, dir);
         const double tk = t + dt;
         const int partner = best >= 0 ? L.id[best] : (wall ? EDMD_WALL_CODE(wxy) : 0);
         EdmdEv e;
@@ -1337,7 +1401,7 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
     }
 }
 
-template <class S>
+template <class S, bool X>
 __global__ __launch_bounds__(S::THREADS) void k_sweep_tiled(const EdmdSys* __restrict__ systems) {
     __shared__ SweepLds<S> L;
     __shared__ int vtmp[S::NH];
@@ -1433,9 +1497,9 @@ __global__ __launch_bounds__(S::THREADS) void k_sweep_tiled(const EdmdSys* __res
         const bool nowrap = tx0 >= 1 && ty0 >= 1 && tx0 + vw <= ncx - 1 && ty0 + vh <= ncy - 1 &&
                             ncx >= 5 && ncy >= 5;
         if (nowrap)
-            sweep_tile_predict<S, false>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
+            sweep_tile_predict<S, false, X>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
         else
-            sweep_tile_predict<S, true>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
+            sweep_tile_predict<S, true, X>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
     } else {
         const int cap = C.cap;
         for (int w = tid; w < S::TX * S::TY * cap; w += S::THREADS) {
@@ -1445,7 +1509,7 @@ __global__ __launch_bounds__(S::THREADS) void k_sweep_tiled(const EdmdSys* __res
             const int g = (ty0 + ly) * ncx + tx0 + lx;
             if (sl < A.ccnt[g]) {
                 const int q = A.cslot[(size_t)g * cap + sl];
-                ov |= sweep_one<false>(C, A, t, q, paul, ctr0 + 2 * (int64_t)q,
+                ov |= sweep_one<false, X>(C, A, t, q, paul, ctr0 + 2 * (int64_t)q,
                                        ctr0 + 2 * (int64_t)q + 1);
             }
         }
@@ -1473,8 +1537,9 @@ __global__ __launch_bounds__(256) void k_sync_all(const EdmdSys* __restrict__ sy
         fly_grow(S->c, t, x, y, r->vx, r->vy, tl, rad, S->a.vr[p]);
         r->rad = rad;
     } else {
-        double vy = r->vy;
-        fly_v<true>(S->c, t, x, y, r->vx, vy, tl);
+        double vx = r->vx, vy = r->vy;
+        fly_v<true>(S->c, t, x, y, vx, vy, tl);
+        r->vx = vx;
         r->vy = vy;
     }
     r->x = x;
@@ -1701,9 +1766,12 @@ __device__ void ev_screenshot(const EdmdConst& C, const AR& A, int lane, LoopReg
             fly_grow(C, t, x, y, r->vx, r->vy, tl, rad, A.vr[q]);
             r->rad = rad;
         } else {
-            double vy = r->vy;
-            fly_v<W>(C, t, x, y, r->vx, vy, tl);
-            if (W) r->vy = vy;
+            double vx = r->vx, vy = r->vy;
+            fly_v<W>(C, t, x, y, vx, vy, tl);
+            if (W) {
+                r->vx = vx;
+                r->vy = vy;
+            }
         }
         r->x = x;
         r->y = y;
@@ -1849,6 +1917,7 @@ __device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs
     fly_v<W>(C, t, xi, yi, vxi, vyi, tli);
     xi = uni(xi);
     yi = uni(yi);
+    vxi = uni(vxi);
     vyi = uni(vyi);
     const bool valid = (ca == collj);
     trace_push(C, A, lane, t, EDMD_EV_COLLISION, i, j, valid ? 1 : 0);
@@ -1857,6 +1926,7 @@ __device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs
         if (lane == 0) {
             A.p[i].x = xi;
             A.p[i].y = yi;
+            A.p[i].vx = vxi;
             A.p[i].vy = vyi;
             A.p[i].tl = t;
         }
@@ -1887,6 +1957,7 @@ __device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs
     const double rj = uni(pj.rad), mj = uni(pj.m);
     const int cxj = uni((int)pj.cx), cyj = uni((int)pj.cy);
     fly_v<W>(C, t, xj, yj, vxj, vyj, tlj);
+    vxj = uni(vxj);
     vyj = uni(vyj);
     double dx = xj - xi;
     double dy = yj - yi;
@@ -2171,6 +2242,7 @@ __device__ void ev_single(const EdmdConst& C, const AR& A, int lane, LoopRegs& R
     }
     x = uni(x);
     y = uni(y);
+    vx = uni(vx);
     vy = uni(vy);
 
     if (wall) {
@@ -2245,7 +2317,10 @@ __device__ void ev_single(const EdmdConst& C, const AR& A, int lane, LoopRegs& R
             q->x = x;
             q->y = y;
             q->tl = tl;
-            if (W) q->vy = vy; /* advanced by the field's flight */
+            if (W) { /* changed by the flight under the field or damping */
+                q->vx = vx;
+                q->vy = vy;
+            }
             if (G) q->rad = r;
             q->cx = (int16_t)cx;
             q->cy = (int16_t)cy;
@@ -2636,8 +2711,9 @@ __global__ void k_fly_one(const EdmdSys* __restrict__ systems, int i) {
         fly_grow(C, A.st->t, x, y, r->vx, r->vy, tl, rad, A.vr[i]);
         r->rad = rad;
     } else {
-        double vy = r->vy;
-        fly_v<true>(C, A.st->t, x, y, r->vx, vy, tl);
+        double vx = r->vx, vy = r->vy;
+        fly_v<true>(C, A.st->t, x, y, vx, vy, tl);
+        r->vx = vx;
         r->vy = vy;
     }
     r->x = x;
@@ -2719,23 +2795,24 @@ __global__ void k_fp64_check(int n, const double* __restrict__ a, const double* 
 /* ------------------------------------------------------------ launchers */
 
 template <class S>
-static void launch_sweep(const EdmdSys* d_sys, int nsys, int max_tiles, hipStream_t s) {
+static void launch_sweep(const EdmdSys* d_sys, int nsys, int max_tiles, int ext, hipStream_t s) {
     dim3 grid((max_tiles + 7) / 8 * 8, nsys);
-    hipLaunchKernelGGL((k_sweep_tiled<S>), grid, dim3(S::THREADS), 0, s, d_sys);
+    if (ext) hipLaunchKernelGGL((k_sweep_tiled<S, true>), grid, dim3(S::THREADS), 0, s, d_sys);
+    else hipLaunchKernelGGL((k_sweep_tiled<S, false>), grid, dim3(S::THREADS), 0, s, d_sys);
 }
 
 hipError_t edmd_launch_sweep_tiled(const EdmdSys* d_sys, int nsys, int max_tiles, int variant,
-                                   hipStream_t s) {
+                                   int ext, hipStream_t s) {
     if (nsys <= 0 || max_tiles <= 0) return hipSuccess;
     switch (variant) {
-        case 0: launch_sweep<SweepV0>(d_sys, nsys, max_tiles, s); break;
-        case 1: launch_sweep<SweepV1>(d_sys, nsys, max_tiles, s); break;
-        case 2: launch_sweep<SweepV2>(d_sys, nsys, max_tiles, s); break;
-        case 3: launch_sweep<SweepV3>(d_sys, nsys, max_tiles, s); break;
-        case 4: launch_sweep<SweepV4>(d_sys, nsys, max_tiles, s); break;
-        case 5: launch_sweep<SweepV5>(d_sys, nsys, max_tiles, s); break;
-        case 6: launch_sweep<SweepV6>(d_sys, nsys, max_tiles, s); break;
-        case 7: launch_sweep<SweepV7>(d_sys, nsys, max_tiles, s); break;
+        case 0: launch_sweep<SweepV0>(d_sys, nsys, max_tiles, ext, s); break;
+        case 1: launch_sweep<SweepV1>(d_sys, nsys, max_tiles, ext, s); break;
+        case 2: launch_sweep<SweepV2>(d_sys, nsys, max_tiles, ext, s); break;
+        case 3: launch_sweep<SweepV3>(d_sys, nsys, max_tiles, ext, s); break;
+        case 4: launch_sweep<SweepV4>(d_sys, nsys, max_tiles, ext, s); break;
+        case 5: launch_sweep<SweepV5>(d_sys, nsys, max_tiles, ext, s); break;
+        case 6: launch_sweep<SweepV6>(d_sys, nsys, max_tiles, ext, s); break;
+        case 7: launch_sweep<SweepV7>(d_sys, nsys, max_tiles, ext, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -83,8 +83,9 @@ typedef struct EdmdConst {
     /* minimum-image half widths: halfLx / halfLy, +inf across walls */
     double mihx, mihy;
     int32_t field_on; /* uniform field along y (addField) */
-    int32_t pad_f;
+    int32_t damping;  /* velocities decay as exp(-gamma t) in flight */
     double field;     /* its value (< 0) */
+    double gamma;     /* damping rate (gamm) */
 } EdmdConst;
 
 /* Order among events of equal time.  The reference pops equal times in the

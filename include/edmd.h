@@ -97,6 +97,16 @@ typedef struct edmd_system_desc {
      * Not with a growth start or taping. */
     int add_field;
     double field;
+    /* Damping (damping / gamm of the graphics build, EDMD.c:294, 382):
+     * velocities decay as exp(-gamma t) in flight (EDMD.c:4965-4975), a
+     * predicted time t of the undamped motion becomes -log(1 - gamma t) /
+     * gamma, never when gamma t >= 1 (logTime, EDMD.c:5709-5721), and
+     * neighbours are flown when their collision time is taken.  gamma > 0
+     * (default 0.02).  exp and log are this library's own (a Taylor series
+     * and an atanh series with Cody-Waite reduction), not the C library's.
+     * Not with the field, a growth start or taping. */
+    int damping;
+    double gamma;
 } edmd_system_desc;
 
 /* A collision slot holding a wall event reports partner -5 - xy

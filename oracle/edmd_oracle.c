@@ -92,12 +92,73 @@ static double wrap_post(double v, double L) {
     return v;
 }
 
-/* freeFlyNormal, EDMD.c:4954-4990 (no damping): straight, or parabolic in
- * y under the field. */
+/* e^x, from its Taylor series: x = k ln2 + r, |r| <= ln2/2, with ln2 in
+ * two parts so that k ln2 is exact (Cody and Waite's reduction); e^r by
+ * the nested form 1 + r(1 + r/2(1 + r/3(... (1 + r/17)))), whose
+ * truncation error is below r^18/18! < 2e-24 relative; times 2^k.  Only
+ * +, *, rint and ldexp, so any IEEE double machine computes the same
+ * bits (the GPU library evaluates the same steps). */
+static const double ORC_LN2_HI = 0x1.62e42fefa3800p-1; /* 43 significant bits */
+static const double ORC_LN2_LO = 0x1.ef35793c76730p-45;
+static const double ORC_INV[24] = {0, 1.0, 1.0 / 2, 1.0 / 3, 1.0 / 4, 1.0 / 5, 1.0 / 6, 1.0 / 7,
+                                   1.0 / 8, 1.0 / 9, 1.0 / 10, 1.0 / 11, 1.0 / 12, 1.0 / 13,
+                                   1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17, 1.0 / 18, 1.0 / 19,
+                                   1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23};
+
+double orc_exp(double x) {
+    if (x != x) return x;
+    if (x > 709.5) return HUGE_VAL;
+    if (x < -745.5) return 0.0;
+    const double k = rint(x * 1.4426950408889634); /* x / ln2, to nearest */
+    const double r = (x - k * ORC_LN2_HI) - k * ORC_LN2_LO;
+    double p = 1.0;
+    for (int n = 17; n >= 1; --n) p = 1.0 + (r * p) * ORC_INV[n];
+    return ldexp(p, (int)k);
+}
+
+/* log x, from the series log m = 2 atanh s = 2 (s + s^3/3 + s^5/5 + ...),
+ * s = (m - 1)/(m + 1), for x = m 2^e with m in [sqrt(1/2), sqrt(2)) so
+ * |s| <= 0.1716 and the terms to s^23/23 leave less than 1e-19 relative;
+ * plus e ln2 in two parts.  Same steps on the GPU. */
+double orc_log(double x) {
+    if (x != x || x < 0) return NAN;
+    if (x == 0) return -HUGE_VAL;
+    if (x == HUGE_VAL) return x;
+    int e;
+    double m = frexp(x, &e); /* [0.5, 1) */
+    if (m < 0.70710678118654752) {
+        m *= 2;
+        e -= 1;
+    }
+    const double s = (m - 1) / (m + 1), s2 = s * s;
+    double q = ORC_INV[23];
+    for (int n = 21; n >= 1; n -= 2) q = ORC_INV[n] + s2 * q;
+    const double lm = 2 * s * q;
+    return e * ORC_LN2_HI + (e * ORC_LN2_LO + lm);
+}
+
+/* logTime, EDMD.c:5709-5721: the time at which the damped motion has gone
+ * as far as the undamped one in t_lin. */
+static double log_time(const orc_ctx* s, double t_lin) {
+    if (!s->prm.damping) return t_lin;
+    double var = (1 - t_lin * s->prm.gamma);
+    if (var < 0) return NEVER;
+    return -orc_log(var) / s->prm.gamma;
+}
+
+/* freeFlyNormal, EDMD.c:4954-4990: straight, parabolic in y under the
+ * field, exponentially slowed under damping. */
 static void free_fly(orc_ctx* s, int p) {
     double dt = s->t - s->tl[p];
     s->tl[p] = s->t;
-    if (s->prm.add_field) {
+    if (s->prm.damping) {
+        double g = s->prm.gamma;
+        double ex = orc_exp(-g * dt);
+        s->x[p] += s->vx[p] / g * (1 - ex);
+        s->y[p] += s->vy[p] / g * (1 - ex);
+        s->vx[p] *= ex;
+        s->vy[p] *= ex;
+    } else if (s->prm.add_field) {
         s->x[p] += dt * s->vx[p];
         s->y[p] += dt * s->vy[p] + 0.5 * s->prm.field * dt * dt;
         s->vy[p] += dt * s->prm.field;
@@ -294,7 +355,7 @@ static double coll_time(orc_ctx* s, int i, int j) {
     /* collisionTimeNormal, EDMD.c:2661-2723 (no damping or shear); under the
      * field the neighbour is flown to the current time first, a state
      * update of the neighbour (EDMD.c:2664-2666). */
-    if (s->prm.add_field) free_fly(s, j);
+    if (s->prm.add_field || s->prm.damping) free_fly(s, j);
     double lat2 = s->t - s->tl[j];
     double dvx = s->vx[j] - s->vx[i];
     double dvy = s->vy[j] - s->vy[i];
@@ -311,7 +372,7 @@ static double coll_time(orc_ctx* s, int i, int j) {
         if (s->status == ORC_OK) s->status = ORC_ERR_OVERLAP;
     }
     if (det < 0) return NEVER;
-    return (-b - sqrt(det)) / v2;
+    return log_time(s, (-b - sqrt(det)) / v2);
 }
 
 static double coll_time_grow(orc_ctx* s, int i, int j) {
@@ -403,25 +464,25 @@ static void predict_collision(orc_ctx* s, int i, double* t_abs, int* partner) {
         }
     } else if (s->prm.wall_y) {
         if (Y == 0 && s->vy[i] < 0) {
-            dt = -(s->y[i] - s->rad[i]) / s->vy[i];
+            dt = log_time(s, -(s->y[i] - s->rad[i]) / s->vy[i]);
             wall = 1;
             xy = 1;
         } else if (Y == s->ncy - 1 && s->vy[i] > 0) {
-            dt = (s->Ly - s->y[i] - s->rad[i]) / s->vy[i];
+            dt = log_time(s, (s->Ly - s->y[i] - s->rad[i]) / s->vy[i]);
             wall = 1;
             xy = 1;
         }
     }
     if (s->prm.wall_x) {
         if (X == 0 && s->vx[i] < 0) {
-            double dtw = -(s->x[i] - s->rad[i]) / s->vx[i];
+            double dtw = log_time(s, -(s->x[i] - s->rad[i]) / s->vx[i]);
             if (dt > dtw) {
                 wall = 1;
                 dt = dtw;
             }
             xy = 0;
         } else if (X == s->ncx - 1 && s->vx[i] > 0) {
-            double dtw = (s->Lx - s->x[i] - s->rad[i]) / s->vx[i];
+            double dtw = log_time(s, (s->Lx - s->x[i] - s->rad[i]) / s->vx[i]);
             if (dt > dtw) {
                 wall = 1;
                 dt = dtw;
@@ -452,10 +513,11 @@ static void predict_crossing(const orc_ctx* s, int i, double* t_abs, int* dir) {
     double tx, ty;
     int xx, yy;
     if (s->vx[i] < 0) {
-        tx = wrap_inside(s->cx[i] * s->csx - s->x[i], s->halfLx, s->Lx) / s->vx[i];
+        tx = log_time(s, wrap_inside(s->cx[i] * s->csx - s->x[i], s->halfLx, s->Lx) / s->vx[i]);
         xx = 1;
     } else {
-        tx = wrap_inside((1 + s->cx[i]) * s->csx - s->x[i], s->halfLx, s->Lx) / s->vx[i];
+        tx = log_time(s, wrap_inside((1 + s->cx[i]) * s->csx - s->x[i], s->halfLx, s->Lx) /
+                             s->vx[i]);
         xx = 2;
     }
     if (s->prm.add_field) {
@@ -472,10 +534,11 @@ static void predict_crossing(const orc_ctx* s, int i, double* t_abs, int* dir) {
             yy = 3;
         }
     } else if (s->vy[i] < 0) {
-        ty = wrap_inside(s->cy[i] * s->csy - s->y[i], s->halfLy, s->Ly) / s->vy[i];
+        ty = log_time(s, wrap_inside(s->cy[i] * s->csy - s->y[i], s->halfLy, s->Ly) / s->vy[i]);
         yy = 3;
     } else {
-        ty = wrap_inside((1 + s->cy[i]) * s->csy - s->y[i], s->halfLy, s->Ly) / s->vy[i];
+        ty = log_time(s, wrap_inside((1 + s->cy[i]) * s->csy - s->y[i], s->halfLy, s->Ly) /
+                             s->vy[i]);
         yy = 4;
     }
     if (tx < ty) {
@@ -856,6 +919,8 @@ void orc_default_params(orc_params* p) {
     p->res_tan = 0;  /* resT, EDMD.c:373 */
     p->add_field = 0;
     p->field = -0.02; /* EDMD.c:366 */
+    p->damping = 0;
+    p->gamma = 0.02; /* gamm, EDMD.c:382 */
 }
 
 orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* y,
@@ -868,7 +933,11 @@ orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* 
     if (prm) s->prm = *prm;
     else orc_default_params(&s->prm);
     /* walls only with the Normal dispatch (doTheWallGrow is not restated) */
-    if ((s->prm.wall_x || s->prm.wall_y || s->prm.add_field) && s->prm.vr > 0) {
+    if ((s->prm.add_field && s->prm.damping) || (s->prm.damping && !(s->prm.gamma > 0))) {
+        free(s);
+        return NULL;
+    }
+    if ((s->prm.wall_x || s->prm.wall_y || s->prm.add_field || s->prm.damping) && s->prm.vr > 0) {
         free(s);
         return NULL;
     }

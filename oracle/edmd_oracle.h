@@ -67,6 +67,16 @@ typedef struct {
      * y-wall hits solve the quadratic (EDMD.c:2428-2442, 2846-2860). */
     int add_field;
     double field;        /* must be < 0, default -0.02 (EDMD.c:365-366) */
+    /* Damping (damping / gamm, EDMD.c:294, 382; fixed at 0 in the
+     * command-line build): velocities decay as exp(-gamma t) in flight
+     * (EDMD.c:4965-4975), every predicted time t_lin of the undamped motion
+     * becomes logTime(t_lin) = -log(1 - gamma t_lin) / gamma, never when
+     * gamma t_lin >= 1 (EDMD.c:5709-5721), and neighbours are flown before
+     * their collision time is taken (EDMD.c:2664-2666).  exp and log are
+     * this oracle's own (orc_exp, orc_log), the same algorithm the GPU
+     * library uses, not the C library's. */
+    int damping;
+    double gamma;        /* gamm, default 0.02 */
 } orc_params;
 
 typedef struct {
@@ -161,6 +171,10 @@ double orc_tangent(int n, const double* m, const double* tl0, const orc_tape_rec
 void orc_adjoint(int n, const double* m, const double* tl0, const orc_tape_rec* tape,
                  int64_t ntape, double t_end, double lw, double lt, const double* lam_end,
                  double* grad);
+/* The exponential and logarithm used under damping (see orc_params). */
+double orc_exp(double x);
+double orc_log(double x);
+
 /* Sum of w_k = m_i m_j ff |a|^2 over the taped collisions, in tape order. */
 double orc_tape_wsum(const orc_ctx* s);
 

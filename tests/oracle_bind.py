@@ -23,7 +23,8 @@ class Params(C.Structure):
                 ("dtnoise", C.c_double), ("temperature", C.c_double),
                 ("bucket_count", C.c_int), ("vr", C.c_double), ("einit", C.c_double),
                 ("wall_x", C.c_int), ("wall_y", C.c_int), ("res_wall", C.c_double),
-                ("res_tan", C.c_double), ("add_field", C.c_int), ("field", C.c_double)]
+                ("res_tan", C.c_double), ("add_field", C.c_int), ("field", C.c_double),
+                ("damping", C.c_int), ("gamma", C.c_double)]
 
 
 class ThermoRow(C.Structure):
@@ -81,6 +82,10 @@ def lib():
         L.orc_replay_fd.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp, C.c_double,
                                     C.c_void_p, C.c_int64, C.c_double] + [dp] * 6 + [
                                     C.c_double, C.c_double, C.c_double, C.c_void_p]
+        L.orc_exp.restype = C.c_double
+        L.orc_exp.argtypes = [C.c_double]
+        L.orc_log.restype = C.c_double
+        L.orc_log.argtypes = [C.c_double]
         L.orc_tape_wsum.restype = C.c_double
         L.orc_tape_wsum.argtypes = [C.c_void_p]
         L.orc_replay.restype = C.c_double

@@ -209,3 +209,47 @@ def test_field_arguments_are_checked(gpu):
     b = gpu.Batch([st], add_field=1)
     with pytest.raises(gpu.EdmdError):
         b.deriv_begin(checkpoint_every=256)
+
+
+# ---- damping (damping / gamm)
+
+DAMP_CASES = [
+    (400, 0.5, 61, dict(damping=1, gamma=0.02)),                           # periodic
+    (350, 0.55, 62, dict(damping=1, gamma=0.05, wall_y=1, res_tan=1.0)),  # channel
+]
+
+
+@pytest.mark.parametrize("n,phi,seed,kw", DAMP_CASES)
+def test_damping_traces_match_oracle(gpu, oracle, n, phi, seed, kw):
+    """Exponentially slowed flights (the library's own exp), logTime of
+    every predicted time (its own log), neighbours flown and stored when
+    scanned: traces and states bit for bit against the oracle, from HBM and
+    from LDS, every sweep shape giving the same starting slots."""
+    st = walled(n, phi, seed, kw)
+    nev = 15000
+    o = oracle.Oracle(st, **kw, **QUIET)
+    oe = o.events()
+    code, tr = o.run(nev)
+    assert code == 0
+    so = o.state()
+    for mode, shape in ((gpu.RES_HBM, 1), (gpu.RES_LDS, 6)):
+        b = gpu.Batch([st], trace_capacity=nev, **kw, **QUIET)
+        ev = b.events(0)
+        for k in ("t_coll", "t_cross"):
+            np.testing.assert_array_equal(bits(ev[k]), bits(oe[k]), err_msg=k)
+        b.set_residency(mode)
+        b.set_sweep_shape(shape)
+        b.run(max_events=nev)
+        assert_trace_equal(b.trace(0), tr)
+        assert_state_equal(b.state(0), so)
+
+
+def test_damping_arguments_are_checked(gpu):
+    st = F.lattice_state(100, 0.4, seed=5, lattice="fit")
+    with pytest.raises(gpu.EdmdError):
+        gpu.System(st, damping=1, gamma=0.0)
+    with pytest.raises(gpu.EdmdError):
+        gpu.System(st, damping=1, add_field=1)
+    b = gpu.Batch([st], damping=1)
+    with pytest.raises(gpu.EdmdError):
+        b.deriv_begin(checkpoint_every=256)

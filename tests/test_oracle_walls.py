@@ -203,3 +203,58 @@ def test_field_box_conserves_total_energy(oracle):
     code, tr = o.run(30000)
     assert code == 0 and np.sum(tr["type"] == WALL) > 200
     assert abs(total() / e0 - 1) < 1e-11
+
+
+# ---- damping (damping / gamm, EDMD.c:4965-4975, 5709-5721, 2664-2666)
+
+def test_own_exp_and_log_are_accurate(oracle):
+    """The oracle's exp and log (the steps the GPU library also runs) agree
+    with exact values to 4 ulp over the ranges the dynamics use (log adds
+    e ln2 to log m, which cancels partly for x just below 1)."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 40
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([rng.uniform(-40, 5, 4000), rng.uniform(-1e-3, 1e-3, 500), [0.0, -1e-300]])
+    for x in xs:
+        got = oracle.lib().orc_exp(float(x))
+        ref = float(Decimal(float(x)).exp())
+        assert abs(got - ref) <= 4 * np.spacing(ref), (x, got, ref)
+    ys = np.concatenate([rng.uniform(1e-6, 1.0, 4000), rng.uniform(1.0, 1e6, 500), [1.0, 0.5]])
+    for y in ys:
+        got = oracle.lib().orc_log(float(y))
+        ref = float(Decimal(float(y)).ln())
+        assert abs(got - ref) <= 4 * np.spacing(abs(ref)) + (5e-324 if ref == 0 else 0), (y, got, ref)
+    assert oracle.lib().orc_log(1.0) == 0.0 and oracle.lib().orc_exp(0.0) == 1.0
+
+
+def test_damped_head_on_and_crossing_closed_form(oracle):
+    """Under damping the separation of two disks closing at u shrinks by
+    u (1 - e^{-g t}) / g: contact at t = -ln(1 - g gap / u) / g, never when
+    g gap >= u; a crossing a distance D ahead comes at -ln(1 - g D / v) / g."""
+    g = 0.05
+    st = disks([(5.0, 5.0), (7.5, 5.0)], [(0.5, 0.0), (-0.5, 0.0)])  # gap 1.5, closing 1
+    o = oracle.Oracle(st, damping=1, gamma=g, **QUIET)
+    assert o.collision_time(0, 1) == pytest.approx(-np.log(1 - g * 1.5) / g, rel=1e-13)
+    o = oracle.Oracle(st, damping=1, gamma=1.0, **QUIET)
+    assert o.collision_time(0, 1) == 1e26
+    st = disks([(4.5, 5.0), (15.0, 15.0)], [(0.8, 0.0), (0.0, 0.0)])  # cell [4, 6): 1.5 ahead
+    o = oracle.Oracle(st, damping=1, gamma=g, **QUIET)
+    t, d = o.predict_crossing(0)
+    assert d == 2 and t == pytest.approx(-np.log(1 - g * 1.5 / 0.8) / g, rel=1e-13)
+
+
+def test_damped_energy_decays_exactly(oracle):
+    """Elastic collisions keep the kinetic energy at the instant and every
+    flight scales velocities by e^{-g dt}, so the energy of the disks flown
+    to a common time T is E0 e^{-2 g T}."""
+    g = 0.03
+    st = F.lattice_state(300, 0.5, seed=19, lattice="fit")
+    o = oracle.Oracle(st, damping=1, gamma=g, **QUIET)
+    e0 = o.observables()["E"]
+    code, tr = o.run(15000)
+    assert code == 0
+    T = o.t
+    lib = oracle.lib()
+    lib.orc_sync_all(o.h, T)
+    e1 = o.observables()["E"]
+    assert abs(e1 / (e0 * np.exp(-2 * g * T)) - 1) < 1e-10, (e1, e0, T)
