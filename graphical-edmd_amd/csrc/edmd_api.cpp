@@ -270,7 +270,9 @@ static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
     c.trace_cap = (int32_t)d->trace_capacity;
     c.thermo_cap = d->thermo_capacity > 0 ? d->thermo_capacity : 4096;
     double rmin = d->rad[0];
+    c.mono = 1;
     for (int i = 0; i < d->n; i++) {
+        if (memcmp(&d->rad[i], &d->rad[0], sizeof(double)) != 0) c.mono = 0;
         if (!(d->rad[i] > 0) || !(d->m[i] > 0))
             return fail(EDMD_ERR_ARG, "system %d: particle %d has non-positive radius or mass", idx, i);
         rmin = std::min(rmin, d->rad[i]);

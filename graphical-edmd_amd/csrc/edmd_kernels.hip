@@ -1275,7 +1275,7 @@ __device__ __forceinline__ void pair_terms(const EdmdConst& C, double ox, double
  * takes), flagging overlaps as collisionTime does; pass 2 solves the marked
  * roots in increasing k.  A disk with more than 256 candidates solves its
  * roots in pass 1 instead. */
-template <class S, bool WRAP, bool X, class AR>
+template <class S, bool WRAP, bool X, bool MONO, class AR>
 __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR& A,
                                                    const SweepLds<S>& L, int tid, int ninner, double t,
                                                    int tx0, int ty0, double paul, int64_t ctr0,
@@ -1317,8 +1317,8 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
             const int vk = k + (k < c1 ? o0 : (k < c2 ? o1 : o2));
             const int v = in ? vk : me;
             double b, v2, d, det;
-            pair_terms<WRAP>(C, ox, oy, ovx, ovy, orad, L.pos[v], L.vel[v], L.rad[v], b, v2, d,
-                             det);
+            pair_terms<WRAP>(C, ox, oy, ovx, ovy, orad, L.pos[v], L.vel[v],
+                             MONO ? orad : L.rad[v], b, v2, d, det);
             /* the owner itself; a halo that wraps (boxes under three tiles
              * wide) may stage the owner's cell more than once */
             const bool self = WRAP ? L.id[v] == p : v == me;
@@ -1344,8 +1344,8 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
                     const int v = k + (k < c1 ? o0 : (k < c2 ? o1 : o2));
                     if (WRAP ? L.id[v] == p : v == me) continue;
                     double b, v2, d, det;
-                    pair_terms<WRAP>(C, ox, oy, ovx, ovy, orad, L.pos[v], L.vel[v], L.rad[v], b,
-                                     v2, d, det);
+                    pair_terms<WRAP>(C, ox, oy, ovx, ovy, orad, L.pos[v], L.vel[v],
+                                     MONO ? orad : L.rad[v], b, v2, d, det);
                     if (b > 0) continue;
                     if (d < -0.01) ov = true;
                     if (det < 0) continue;
@@ -1365,8 +1365,8 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
             else { k = 192 + __builtin_ctzll(m3); m3 &= m3 - 1; }
             const int v = k + (k < c1 ? o0 : (k < c2 ? o1 : o2));
             double b, v2, d, det;
-            pair_terms<WRAP>(C, ox, oy, ovx, ovy, orad, L.pos[v], L.vel[v], L.rad[v], b, v2, d,
-                             det);
+            pair_terms<WRAP>(C, ox, oy, ovx, ovy, orad, L.pos[v], L.vel[v],
+                             MONO ? orad : L.rad[v], b, v2, d, det);
             const double tt = log_time<X>(C, (-b - sqrt(det)) / v2);
             if (dt > tt) {
                 dt = tt;
@@ -1496,10 +1496,14 @@ __global__ __launch_bounds__(S::THREADS) void k_sweep_tiled(const EdmdSys* __res
          * nothing and is skipped (uniform over the workgroup) */
         const bool nowrap = tx0 >= 1 && ty0 >= 1 && tx0 + vw <= ncx - 1 && ty0 + vh <= ncy - 1 &&
                             ncx >= 5 && ncy >= 5;
-        if (nowrap)
-            sweep_tile_predict<S, false, X>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
+        /* equal radii: the contact term 4 r r of every pair is the owner's
+         * own (bitwise the same product), no radius read per candidate */
+        if (nowrap && C.mono)
+            sweep_tile_predict<S, false, X, true>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
+        else if (nowrap)
+            sweep_tile_predict<S, false, X, false>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
         else
-            sweep_tile_predict<S, true, X>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
+            sweep_tile_predict<S, true, X, false>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
     } else {
         const int cap = C.cap;
         for (int w = tid; w < S::TX * S::TY * cap; w += S::THREADS) {

@@ -86,6 +86,8 @@ typedef struct EdmdConst {
     int32_t damping;  /* velocities decay as exp(-gamma t) in flight */
     double field;     /* its value (< 0) */
     double gamma;     /* damping rate (gamm) */
+    int32_t mono;     /* every disk has the same radius (bitwise) */
+    int32_t pad_m;
 } EdmdConst;
 
 /* Order among events of equal time.  The reference pops equal times in the
