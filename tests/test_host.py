@@ -197,3 +197,40 @@ def test_cli_reports_missing_device():
     assert "no HIP device" in r.stderr
 
 
+
+
+def test_python_mirrors_match_the_c_header(tmp_path):
+    """The ctypes mirrors of the public structs (descriptor, thermo row,
+    observables, tune report) have the C header's sizes and field offsets:
+    compiled with the host C compiler, no GPU needed."""
+    import ctypes as C
+    fields = {
+        "edmd_system_desc": (E.SystemDesc, ["n", "Lx", "x", "res", "noise", "cell_capacity",
+                                            "trace_capacity", "vr", "einit", "wall_x", "wall_y",
+                                            "res_wall", "res_tan", "add_field", "field",
+                                            "damping", "gamma"]),
+        "edmd_observables": (E.Observables, ["t", "ncol", "E", "py", "status", "err_j"]),
+        "edmd_tune_report": (E.TuneReport, ["sweep_ms", "sweep_shape", "in_lds", "loop_ms_hbm",
+                                            "loop_ms_lds"]),
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "edmd.h"', 'int main(void) {']
+    for name, (_, fl) in fields.items():
+        lines.append('printf("%%s %%zu\\n", "%s", sizeof(%s));' % (name, name))
+        for f in fl:
+            lines.append('printf("%%s.%%s %%zu\\n", "%s", "%s", offsetof(%s, %s));' % (name, f, name, f))
+    lines.append('printf("thermo %zu pwall %zu\\n", sizeof(edmd_thermo_row), '
+                 'offsetof(edmd_thermo_row, pwall));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(REPO, "include"), "-o", str(exe),
+                    str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = dict(l.rsplit(" ", 1) for l in out if l and not l.startswith("thermo"))
+    for name, (cls, fl) in fields.items():
+        assert int(got[name]) == C.sizeof(cls), name
+        for f in fl:
+            assert int(got["%s.%s" % (name, f)]) == getattr(cls, f).offset, (name, f)
+    th = [l for l in out if l.startswith("thermo")][0].split()
+    assert int(th[1]) == E.THERMO_DT.itemsize and int(th[3]) == E.THERMO_DT.fields["pwall"][1]
