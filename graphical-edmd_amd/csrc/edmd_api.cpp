@@ -132,7 +132,8 @@ struct edmd_batch {
     size_t lds_dyn[2] = {0, 0};
     int64_t lds_slots = 0;
     bool use_lds = false;
-    bool any_walls = false; /* some system has walls, the field or damping: the kernels' variants for them */
+    /* some system has walls, the field or damping: the kernels' variants for them */
+    bool any_walls = false;
 };
 
 struct edmd_ctx {
@@ -391,7 +392,8 @@ static int upload_system(edmd_batch* b, int s, const edmd_system_desc* d) {
 }
 
 static int batch_predict_all(edmd_batch* b) {
-    HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->sweep_variant, b->any_walls ? 1 : 0, b->stream));
+    HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->sweep_variant,
+                                   b->any_walls ? 1 : 0, b->stream));
     if (b->any_grow) HIPCHK(edmd_launch_sweep_grow(b->d_sys, b->nsys, b->max_n, b->stream));
     HIPCHK(edmd_launch_sweep_count(b->d_sys, b->nsys, b->stream));
     HIPCHK(edmd_launch_tree(b->d_sys, b->nsys, b->max_lsize, b->max_nlev, b->stream));
@@ -702,10 +704,12 @@ extern "C" int edmd_batch_autotune(edmd_batch* b, int sweep_reps, int64_t trial_
         double bt = 1e300;
         for (int v = 0; v < EDMD_SWEEP_VARIANTS && e == hipSuccess; v++) {
             set_sweep_variant(b, v);
-            e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, v, b->any_walls ? 1 : 0, b->stream);
+            e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, v, b->any_walls ? 1 : 0,
+                                        b->stream);
             if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
             for (int k = 0; k < sweep_reps && e == hipSuccess; k++)
-                e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, v, b->any_walls ? 1 : 0, b->stream);
+                e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, v,
+                                            b->any_walls ? 1 : 0, b->stream);
             if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
@@ -870,7 +874,8 @@ extern "C" int edmd_batch_time_sweep(edmd_batch* b, int reps, double* avg_ms) {
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, b->stream));
     for (int r = 0; r < reps; r++)
-        HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->sweep_variant, b->any_walls ? 1 : 0, b->stream));
+        HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->sweep_variant,
+                                       b->any_walls ? 1 : 0, b->stream));
     HIPCHK(hipEventRecord(e1, b->stream));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
