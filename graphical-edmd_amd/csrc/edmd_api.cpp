@@ -1137,6 +1137,43 @@ extern "C" int edmd_sweep_predict(edmd_ctx* c) {
     return c ? edmd_batch_sweep_predict(c->b) : fail(EDMD_ERR_ARG, "null context");
 }
 
+extern "C" int edmd_load_dump(const char* path, edmd_ctx** out) {
+    return edmd_ctx_create_from_dump(path, nullptr, out);
+}
+
+/* Derivatives of a context's single system: the batch calls on its batch. */
+extern "C" int edmd_adjoint_begin(edmd_ctx* c, int64_t checkpoint_every, int64_t tape_capacity) {
+    return c ? edmd_batch_deriv_begin(c->b, checkpoint_every, tape_capacity)
+             : fail(EDMD_ERR_ARG, "null context");
+}
+
+extern "C" int edmd_adjoint_run(edmd_ctx* c, int64_t max_events, uint64_t max_collisions,
+                                double tmax) {
+    return c ? edmd_batch_deriv_run(c->b, max_events, max_collisions, tmax)
+             : fail(EDMD_ERR_ARG, "null context");
+}
+
+extern "C" int edmd_adjoint_tangent(edmd_ctx* c, const double* tan_in, double* tan_out,
+                                    double* dw, double* dtlast) {
+    return c ? edmd_batch_tangent(c->b, tan_in, tan_out, dw, dtlast)
+             : fail(EDMD_ERR_ARG, "null context");
+}
+
+extern "C" int edmd_adjoint_gradient(edmd_ctx* c, double lw, double lt, const double* lam_end,
+                                     double* grad) {
+    return c ? edmd_batch_adjoint(c->b, &lw, &lt, lam_end, grad)
+             : fail(EDMD_ERR_ARG, "null context");
+}
+
+extern "C" int edmd_adjoint_values(edmd_ctx* c, double* w, double* tlast, int64_t* records) {
+    return c ? edmd_batch_deriv_values(c->b, 0, w, tlast, records)
+             : fail(EDMD_ERR_ARG, "null context");
+}
+
+extern "C" int edmd_adjoint_end(edmd_ctx* c) {
+    return c ? edmd_batch_deriv_end(c->b) : fail(EDMD_ERR_ARG, "null context");
+}
+
 extern "C" int edmd_run(edmd_ctx* c, int64_t max_events, uint64_t max_collisions, double tmax) {
     return c ? edmd_batch_run(c->b, max_events, max_collisions, tmax)
              : fail(EDMD_ERR_ARG, "null context");

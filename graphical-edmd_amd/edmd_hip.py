@@ -149,6 +149,13 @@ def lib():
             "edmd_batch_deriv_end": (i32, [vp]),
             "edmd_batch_get_radii": (i32, [vp, i32, vp]),
             "edmd_get_radii": (i32, [vp, vp]),
+            "edmd_load_dump": (i32, [C.c_char_p, vp]),
+            "edmd_adjoint_begin": (i32, [vp, i64, i64]),
+            "edmd_adjoint_run": (i32, [vp, i64, u64, dbl]),
+            "edmd_adjoint_tangent": (i32, [vp] * 5),
+            "edmd_adjoint_gradient": (i32, [vp, dbl, dbl, vp, vp]),
+            "edmd_adjoint_values": (i32, [vp] * 4),
+            "edmd_adjoint_end": (i32, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -217,7 +224,10 @@ class System:
         for k, v in kw.items():
             setattr(d, k, v)
         h = C.c_void_p()
-        check(lib().edmd_ctx_create_from_dump(str(path).encode(), C.byref(d), C.byref(h)))
+        if kw:
+            check(lib().edmd_ctx_create_from_dump(str(path).encode(), C.byref(d), C.byref(h)))
+        else:
+            check(lib().edmd_load_dump(str(path).encode(), C.byref(h)))
         self.h = h
         n, Lx, Ly = C.c_int(), C.c_double(), C.c_double()
         check(lib().edmd_get_box(h, C.byref(n), C.byref(Lx), C.byref(Ly)))
@@ -320,6 +330,42 @@ class System:
 
     def free_fly(self, i):
         check(lib().edmd_free_fly(self.h, int(i)))
+
+    # ---- derivatives of the event map (edmd_adjoint_*)
+
+    def adjoint_begin(self, checkpoint_every=256, tape_capacity=0):
+        check(lib().edmd_adjoint_begin(self.h, int(checkpoint_every), int(tape_capacity)))
+
+    def adjoint_run(self, max_events=INT64_MAX, max_collisions=UINT64_MAX, tmax=1e300):
+        check(lib().edmd_adjoint_run(self.h, int(max_events), int(max_collisions),
+                                     float(tmax)))
+
+    def adjoint_tangent(self, tan_in):
+        """tan_in: (n, 4) (dx dy dvx dvy per disk). Returns (final tangent,
+        dW, d t_last)."""
+        t = _f64(np.asarray(tan_in, dtype=np.float64).reshape(self.n, 4))
+        out = np.zeros_like(t)
+        dw, dtl = C.c_double(), C.c_double()
+        check(lib().edmd_adjoint_tangent(self.h, _ptr(t), _ptr(out), C.byref(dw),
+                                         C.byref(dtl)))
+        return out, dw.value, dtl.value
+
+    def adjoint_gradient(self, lw=0.0, lt=0.0, lam_end=None):
+        """Gradient ((n, 4)) of lw W + lt t_last + <lam_end, final state>
+        with respect to the taping-origin state."""
+        le = None if lam_end is None else _f64(
+            np.asarray(lam_end, dtype=np.float64).reshape(self.n, 4))
+        g = np.zeros((self.n, 4))
+        check(lib().edmd_adjoint_gradient(self.h, float(lw), float(lt), _ptr(le), _ptr(g)))
+        return g
+
+    def adjoint_values(self):
+        w, tl, k = C.c_double(), C.c_double(), C.c_int64()
+        check(lib().edmd_adjoint_values(self.h, C.byref(w), C.byref(tl), C.byref(k)))
+        return w.value, tl.value, k.value
+
+    def adjoint_end(self):
+        check(lib().edmd_adjoint_end(self.h))
 
 
 class Batch:

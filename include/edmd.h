@@ -149,6 +149,8 @@ int edmd_ctx_destroy(edmd_ctx* ctx);
  * LAMMPS dump as the initial state, the other fields from params (NULL =
  * edmd_default_desc). */
 int edmd_ctx_create_from_dump(const char* path, const edmd_system_desc* params, edmd_ctx** out);
+/* edmd_ctx_create_from_dump with the default parameters. */
+int edmd_load_dump(const char* path, edmd_ctx** out);
 /* Full prediction sweep at the current time (kernel k_sweep_tiled): free-flies
  * every particle to the current time first, then re-predicts every
  * crossing and collision slot and rebuilds the calendar (the pattern of
@@ -396,6 +398,19 @@ int edmd_batch_deriv_values(edmd_batch* b, int sys, double* w, double* tlast, in
 int edmd_batch_get_tape(edmd_batch* b, int sys, edmd_tape_rec* out, int64_t cap, int64_t* n);
 /* Stops taping and frees tapes and checkpoints. */
 int edmd_batch_deriv_end(edmd_batch* b);
+
+/* The same for the single system of a context (no counterpart in the
+ * reference, which has no derivatives): begin / run / tangent / gradient /
+ * values / end, with the arguments of the batch calls for one system
+ * (lw and lt are scalars here). */
+int edmd_adjoint_begin(edmd_ctx* ctx, int64_t checkpoint_every, int64_t tape_capacity);
+int edmd_adjoint_run(edmd_ctx* ctx, int64_t max_events, uint64_t max_collisions, double tmax);
+int edmd_adjoint_tangent(edmd_ctx* ctx, const double* tan_in, double* tan_out, double* dw,
+                         double* dtlast);
+int edmd_adjoint_gradient(edmd_ctx* ctx, double lw, double lt, const double* lam_end,
+                          double* grad);
+int edmd_adjoint_values(edmd_ctx* ctx, double* w, double* tlast, int64_t* records);
+int edmd_adjoint_end(edmd_ctx* ctx);
 
 /* ---- diagnostics ----------------------------------------------------------- */
 /* q[k] = a[k] / b[k] and s[k] = sqrt(a[k]) evaluated on the GPU, for the

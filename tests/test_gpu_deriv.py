@@ -92,6 +92,36 @@ def test_tangent_and_adjoint_match_oracle(full_tape, oracle):
         np.testing.assert_array_equal(bits(grad[k]), bits(g), err_msg="adjoint %d" % k)
 
 
+def test_context_adjoint_matches_oracle(gpu, oracle, tmp_path):
+    """edmd_load_dump + edmd_adjoint_*: the single-system context path gives
+    the oracle's tape values, tangent and gradient bit for bit, here through
+    checkpointed recomputation (every 300 events)."""
+    st = golden_state(load_golden("poly500_phi050"))
+    dump = str(tmp_path / "init.dump")
+    F.write_dump(dump, st)  # 17 significant digits: the state round-trips exactly
+    s = gpu.System.from_dump(dump)
+    ncoll = 3000
+    s.adjoint_begin(checkpoint_every=300)
+    s.adjoint_run(max_collisions=ncoll)
+    o, s0, tape = oracle_taped(oracle, st, ncoll)
+    w, tl, _ = s.adjoint_values()
+    assert bits(w) == bits(o.tape_wsum()) and tl == tape["t"][-1]
+    rng = np.random.default_rng(3)
+    tin = rng.standard_normal((len(st["x"]), 4))
+    tout, dw, dtl = s.adjoint_tangent(tin)
+    tan = tin.copy()
+    rdw, rdtl = oracle.tangent(st["m"], s0["t"], tape, o.t, tan)
+    np.testing.assert_array_equal(bits(tout), bits(tan))
+    assert bits(dw) == bits(rdw) and bits(dtl) == bits(rdtl)
+    lam = rng.standard_normal((len(st["x"]), 4))
+    g = s.adjoint_gradient(1.25, -0.75, lam)
+    rg = oracle.adjoint(st["m"], s0["t"], tape, o.t, 1.25, -0.75, lam)
+    np.testing.assert_array_equal(bits(g), bits(rg))
+    s.adjoint_end()
+    with pytest.raises(gpu.EdmdError):
+        s.adjoint_values()
+
+
 @pytest.mark.parametrize("every", [256, 97])
 def test_checkpointed_recomputation_equals_full_tape(gpu, full_tape, every):
     """Checkpoint every K events, recompute each segment's tape in the

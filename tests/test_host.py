@@ -187,6 +187,16 @@ def test_ctx_from_dump_reports_io_errors(tmp_path):
     L = E.lib()
     assert L.edmd_ctx_create_from_dump(str(tmp_path / "missing.dump").encode(), None, None) == 22
     assert L.edmd_run_traced(None, 10, 10, 1.0, None, None) == 22
+    assert L.edmd_load_dump(str(tmp_path / "missing.dump").encode(), None) == 22
+    h = C.c_void_p()
+    assert L.edmd_load_dump(str(tmp_path / "missing.dump").encode(), C.byref(h)) == 5
+    assert not h.value
+    assert L.edmd_adjoint_begin(None, 0, 0) == 22
+    assert L.edmd_adjoint_run(None, 10, 10, 1.0) == 22
+    assert L.edmd_adjoint_tangent(None, None, None, None, None) == 22
+    assert L.edmd_adjoint_gradient(None, 1.0, 0.0, None, None) == 22
+    assert L.edmd_adjoint_values(None, None, None, None) == 22
+    assert L.edmd_adjoint_end(None) == 22
 
 
 @pytest.mark.skipif(E.device_count() > 0, reason="checks the no-device path")
