@@ -1267,6 +1267,16 @@ __device__ __forceinline__ void pair_terms(const EdmdConst& C, double ox, double
     det = b * b - v2 * d;
 }
 
+/* The bits of the 64-bit mark word starting at candidate base that stay
+ * set: candidates k < tot other than kself. */
+__device__ __forceinline__ uint64_t keep_bits(int tot, int kself, int base) {
+    const int r = tot - base;
+    uint64_t m = r >= 64 ? ~(uint64_t)0 : (r <= 0 ? (uint64_t)0 : ((uint64_t)1 << r) - 1);
+    const int sb = kself - base;
+    if (sb >= 0 && sb < 64) m &= ~((uint64_t)1 << sb);
+    return m;
+}
+
 /* The predictions of a staged tile's own disks, one thread per disk.  The
  * candidates of a disk are its three row runs, k = 0 .. tot-1 in scan
  * order.  Pass 1 runs k over the wave's largest tot (uniform) and marks in
@@ -1308,10 +1318,45 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
         int wxy = 2;
         bool wall = false;
         if (X && C.walls) wall = wall_time(C, ox, oy, ovx, ovy, orad, tx0 + lx, ty0 + ly, dt, wxy);
-        /* pass 1 without branches: every lane evaluates its k-th candidate
-         * (a lane past its count re-reads its own disk, masked out); the
-         * marks of 64 consecutive k collect in cur and go to their word
+        /* pass 1 without branches: every lane evaluates its k-th candidate;
+         * the marks of consecutive k collect in cur and go to their word
          * when k leaves it (k is uniform over the wave) */
+        if constexpr (!WRAP) {
+            /* The staged index advances by one per k and moves to the next
+             * row run where that run starts; a lane past its own count reads
+             * a staged disk clamped into the stage.  Only the discriminant
+             * terms stay in the loop: the owner itself (k = kself) and the
+             * k past the count are cleared from the marks afterwards, and
+             * the overlap test moves to pass 2, since a pair with d < 0 and
+             * b <= 0 always has det >= 0 and is marked. */
+            int v = a0 - 1;
+            uint32_t c32 = 0;
+            for (int k = 0; k < ktot; ++k) {
+                v = v + 1;
+                v = k == c1 ? a1 : v;
+                v = k == c2 ? a2 : v;
+                const int vc = min(v, S::STAGE - 1);
+                double b, v2, d, det;
+                pair_terms<false>(C, ox, oy, ovx, ovy, orad, L.pos[vc], L.vel[vc],
+                                  MONO ? orad : L.rad[vc], b, v2, d, det);
+                if (!(b > 0) && !(det < 0)) c32 |= 1u << (k & 31);
+                if ((k & 31) == 31 || k == ktot - 1) {
+                    const uint64_t c = (k & 32) ? (uint64_t)c32 << 32 : (uint64_t)c32;
+                    switch (k >> 6) {
+                        case 0: m0 |= c; break;
+                        case 1: m1 |= c; break;
+                        case 2: m2 |= c; break;
+                        default: m3 |= c; break;
+                    }
+                    c32 = 0;
+                }
+            }
+            const int kself = c1 + (me - a1); /* the owner lies in its middle row run */
+            m0 &= keep_bits(tot1, kself, 0);
+            m1 &= keep_bits(tot1, kself, 64);
+            m2 &= keep_bits(tot1, kself, 128);
+            m3 &= keep_bits(tot1, kself, 192);
+        } else
         for (int k = 0; k < ktot; ++k) {
             const bool in = k < tot1;
             const int vk = k + (k < c1 ? o0 : (k < c2 ? o1 : o2));
@@ -1367,6 +1412,7 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
             double b, v2, d, det;
             pair_terms<WRAP>(C, ox, oy, ovx, ovy, orad, L.pos[v], L.vel[v],
                              MONO ? orad : L.rad[v], b, v2, d, det);
+            if (!WRAP) ov = ov || d < -0.01; /* EDMD.c:2708-2717 */
             const double tt = log_time<X>(C, (-b - sqrt(det)) / v2);
             if (dt > tt) {
                 dt = tt;
