@@ -27,7 +27,7 @@ def golden_state(g):
             "m": g["m"], "Lx": float(g["Lx"]), "Ly": float(g["Ly"])}
 
 
-TRACED = ["cfg1_d1", "cfg1_r1", "n400_phi060", "tri1024_phi080", "poly500_phi050",
+TRACED = ["sq2000_phi070", "sq2000_phi070_r1", "n400_phi060", "tri1024_phi080", "poly500_phi050",
           "tiny_ncx3"]
 SWEEP_ONLY = ["tiny_ncx2", "tiny_ncx1"]
 GROWN = ["grow_mono200", "grow_poly150"]

@@ -60,8 +60,8 @@ def poly_state(n, phi, seed, mean_rad=0.5, rel_sd=0.1, trunc=2.0):
 
 CASES = [
     # name, state builder, trace length
-    ("cfg1_d1", lambda: F.lattice_state(2000, 0.70, seed=1), 20000),
-    ("cfg1_r1", lambda: F.scale_state(F.lattice_state(2000, 0.70, seed=1), 2.0), 10000),
+    ("sq2000_phi070", lambda: F.lattice_state(2000, 0.70, seed=1), 20000),
+    ("sq2000_phi070_r1", lambda: F.scale_state(F.lattice_state(2000, 0.70, seed=1), 2.0), 10000),
     ("n400_phi060", lambda: F.lattice_state(400, 0.60, seed=2), 20000),
     ("tri1024_phi080", lambda: F.lattice_state(1024, 0.80, seed=5, lattice="tri"), 10000),
     ("poly500_phi050", lambda: poly_state(500, 0.50, seed=6), 10000),
@@ -82,13 +82,13 @@ GROWTH_CASES = [
 
 THERMO_CASE = ("thermo64", lambda: F.lattice_state(64, 0.50, seed=4), 1300.0, ["-o", "100"])
 
-# Long run of the cfg1 system with the reference built with its shipped
+# Long run of the N = 2000, phi = 0.70 square-lattice system with the reference built with its shipped
 # flags (-O3 -ffast-math; its trajectory leaves the strict one after ~10^4
 # collisions): thermo rows every time unit to t = 60 (about 2 x 10^6
 # collisions) and the states at t = 30, 40, 50, 60, for statistical
 # comparisons of the reduced pressure and g(r).  The first thermo row is at
 # t = 1 (the harness moves the reference's fixed firstThermo = 1000).
-LONGRUN_CASE = ("cfg1_longrun", lambda: F.lattice_state(2000, 0.70, seed=1), [30.0, 40.0, 50.0, 60.0])
+LONGRUN_CASE = ("sq2000_phi070_longrun", lambda: F.lattice_state(2000, 0.70, seed=1), [30.0, 40.0, 50.0, 60.0])
 
 
 def state_arrays(st):
@@ -130,7 +130,7 @@ def main():
         for k in trace.dtype.names:
             arrs["trace_" + k] = trace[k]
         arrs.update(pack_state("state", t))
-        if name == "cfg1_d1":
+        if name == "sq2000_phi070":
             pcf = os.path.join(tmp, name + ".pcf")
             F.run_ref(["pcf", dump, pcf, 0.1, st["Lx"] / 2])
             r, g = F.parse_pcf(pcf)

@@ -1,8 +1,8 @@
 """Long-run observables against the reference built with its shipped flags.
 
-The cfg1 system (N = 2000, diameter 1, phi = 0.70) runs to t = 60, about
+The square-lattice system of N = 2000, diameter 1, phi = 0.70 runs to t = 60, about
 1.3 x 10^6 valid collisions, on the GPU and in the reference compiled with
--O3 -ffast-math (tests/golden/make_golden.py, case cfg1_longrun).  That
+-O3 -ffast-math (tests/golden/make_golden.py, case sq2000_phi070_longrun).  That
 build's trajectory leaves the strict one after ~10^4 collisions, so the two
 runs are independent samples of the same equilibrium beyond the first few
 time units: the reduced pressure (thermo rows every time unit,
@@ -25,7 +25,7 @@ def synced(x, y, vx, vy, tl, T, L):
 
 @pytest.fixture(scope="module")
 def runs(gpu):
-    g = load_golden("cfg1_longrun")
+    g = load_golden("sq2000_phi070_longrun")
     st = golden_state(g)
     s = gpu.System(st, first_thermo=1.0, dtime_thermo=1.0, thermo_capacity=128,
                    first_screen=1e9, dtime=1e9)

@@ -178,7 +178,7 @@ def test_calendar_is_consistent(gpu):
     lattice hold many exact ties (mirrored pairs), the growth start crosses
     its stop event, the 5000-disk system has a three-level calendar."""
     from calendar_check import check_calendar
-    states = [golden_state(load_golden("cfg1_d1")), F.lattice_state(1024, 0.8, 5, lattice="tri"),
+    states = [golden_state(load_golden("sq2000_phi070")), F.lattice_state(1024, 0.8, 5, lattice="tri"),
               F.growth_state(300, 0.55, 11), F.lattice_state(5000, 0.7, 9, lattice="fit")]
     b = gpu.Batch(states, per_system={"vr": [0.0, 0.0, 0.2, 0.0]})
     for nev in (0, 300, 30000):
@@ -193,7 +193,7 @@ def test_dump_load_and_traced_run(gpu, tmp_path):
     """edmd_ctx_create_from_dump + edmd_run_traced: the callback sees every
     event of the reference's trace in order, in chunks no larger than the
     trace capacity, across several calls; a true return stops the run."""
-    g = load_golden("cfg1_d1")
+    g = load_golden("sq2000_phi070")
     st = golden_state(g)
     dump = str(tmp_path / "init.dump")
     F.write_dump(dump, st)  # 17 significant digits: the state round-trips exactly
@@ -441,7 +441,7 @@ def test_cli_writes_reference_outputs(gpu, oracle, tmp_path):
 def test_pcf_matches_reference(gpu, oracle):
     """calculate_pcf (pcf.c:16-75) on the GPU: bit-exact with the reference's
     golden histogram and with the oracle on an evolved state."""
-    g = load_golden("cfg1_d1")
+    g = load_golden("sq2000_phi070")
     L = float(g["Lx"])
     r, gr = gpu.pcf(g["x"], g["y"], L, L, 0.1, L / 2)
     np.testing.assert_array_equal(bits(r), bits(g["pcf_r"]))

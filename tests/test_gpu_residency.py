@@ -112,7 +112,7 @@ def test_lds_loop_equals_hbm_loop(gpu):
 
 
 def test_lds_loop_matches_oracle(gpu, oracle):
-    """The cfg4-sized system (400 disks, phi 0.60) in an LDS-resident loop:
+    """A system of 400 disks at phi 0.60 in an LDS-resident loop:
     trace and state bit for bit as the oracle's over 10^4 events."""
     st = F.lattice_state(400, 0.6, seed=4, lattice="fit")
     nev = 10000

@@ -75,7 +75,7 @@ def test_thermo_run_matches_reference(oracle):
 
 
 def test_pcf_matches_reference(oracle):
-    g = load_golden("cfg1_d1")
+    g = load_golden("sq2000_phi070")
     r, gr = oracle.pcf(g["x"], g["y"], float(g["Lx"]), float(g["Ly"]), 0.1, float(g["Lx"]) / 2)
     np.testing.assert_array_equal(bits(r), bits(g["pcf_r"]))
     np.testing.assert_array_equal(bits(gr), bits(g["pcf_g"]))
