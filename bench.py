@@ -276,6 +276,9 @@ def main():
     ap.add_argument("--sweep-only", action="store_true",
                     help="only build the N = 2^20 system, tune it and time --sweep-reps sweeps "
                          "(the workload of the sweep profiles)")
+    ap.add_argument("--melt", type=float, default=0.0,
+                    help="with --sweep-only: first run the system for this many valid "
+                         "collisions per disk (the melt of the N = 2^20 configuration is 50)")
     args = ap.parse_args()
     if args.calibrate_copy:
         copy_calibration()
@@ -291,9 +294,24 @@ def main():
         sys.exit(1)
     if args.sweep_only:
         big = E.Batch([E.init_lattice(args.plain_n, 0.80, rad=0.5, lattice=1, seed=args.seed)])
-        t = tune(big, args, 5, 0)
-        ms = big.time_sweep(args.sweep_reps)
-        print(json.dumps({"sweep_ms": ms, "n": args.plain_n, "tuning": t}), flush=True)
+        out = {"n": args.plain_n}
+        if args.melt > 0:
+            target = int(args.melt * args.plain_n)
+            chunk = 5000000
+            t0 = time.perf_counter()
+            done = 0
+            while done < target:
+                done = min(target, done + chunk)
+                big.run(max_collisions=done, max_events=1 << 62)
+                log("melt: %d of %d collisions, %.0f s" % (done, target, time.perf_counter() - t0))
+            el = time.perf_counter() - t0
+            ob = big.observables(0)
+            out["melt"] = {"collisions": int(ob["ncol"]), "events": int(ob["nevents"]),
+                           "seconds": el, "collisions_per_s": ob["ncol"] / el, "t": ob["t"]}
+            big.sweep_predict()
+        out["tuning"] = tune(big, args, 5, 0)
+        out["sweep_ms"] = big.time_sweep(args.sweep_reps)
+        print(json.dumps(out), flush=True)
         return
 
     # ---- batched replicas -------------------------------------------------
