@@ -337,6 +337,32 @@ def test_overlap_is_reported(gpu):
     assert ei.value.code == 3
 
 
+def test_overlap_is_reported_in_interior_tiles(gpu):
+    """An overlapping pair in the middle of a large box, where the sweep tile's
+    halo does not wrap: reported when the disks approach; not when they
+    recede, under every sweep shape, since collisionTime returns never for
+    b > 0 before its overlap test (EDMD.c:2694-2717)."""
+    st = F.lattice_state(20000, 0.2, seed=3)
+    c = int(np.argmin((st["x"] - st["Lx"] / 2) ** 2 + (st["y"] - st["Ly"] / 2) ** 2))
+    m = (c + 5000) % 20000
+    st["x"][m] = st["x"][c] + 0.5  # spacing ~1.98: no other disk within a diameter
+    st["y"][m] = st["y"][c]
+    for sign in (1.0, -1.0):
+        s = dict(st, vx=st["vx"].copy(), vy=st["vy"].copy())
+        s["vx"][c], s["vy"][c] = sign, 0.0
+        s["vx"][m], s["vy"][m] = -sign, 0.0
+        if sign > 0:
+            with pytest.raises(gpu.EdmdError) as ei:
+                gpu.Batch([s])
+            assert ei.value.code == 3
+        else:
+            b = gpu.Batch([s])  # L = 280: 140 cells a side, the pair far from every edge
+            for v in range(8):
+                b.set_sweep_shape(v)
+                b.sweep_predict()
+            assert b.observables(0)["status"] == 0
+
+
 def test_cell_overflow_is_reported(gpu):
     st = F.lattice_state(900, 0.7, seed=1)
     s0 = gpu.System(st)
