@@ -304,6 +304,50 @@ def test_tiled_sweep_matches_oracle_after_events(gpu, oracle, rad):
         assert bits(eg["t_coll"][i]) == bits(tk) and eg["partner"][i] == p, i
 
 
+def clustered_state(seed=5):
+    """Unit-diameter disks on a square lattice of spacing 2 in an 80 x 80
+    box, with a 10 x 10 block of disks of radius 0.05 (spacing 0.12) in a
+    hole at the centre: the block's disks have ~100 candidates in their
+    3 x 3 cells, while every tile still fits the sweep's stage."""
+    rng = np.random.default_rng(seed)
+    L = 80.0
+    g = np.arange(40) * 2.0 + 1.0
+    bx, by = [a.ravel() for a in np.meshgrid(g, g)]
+    keep = (bx - L / 2) ** 2 + (by - L / 2) ** 2 > 3.0 ** 2
+    bx, by = bx[keep], by[keep]
+    c = (np.arange(10) - 4.5) * 0.12
+    sx, sy = [a.ravel() + L / 2 for a in np.meshgrid(c, c)]
+    x = np.concatenate([bx, sx])
+    y = np.concatenate([by, sy])
+    rad = np.concatenate([np.full(len(bx), 0.5), np.full(len(sx), 0.05)])
+    n = len(x)
+    v = rng.standard_normal((n, 2))
+    v -= v.mean(axis=0)
+    return {"x": x, "y": y, "vx": v[:, 0].copy(), "vy": v[:, 1].copy(), "rad": rad,
+            "m": np.ones(n), "Lx": L, "Ly": L}
+
+
+def test_sweep_with_crowded_neighbourhoods_matches_oracle(gpu, oracle):
+    """Disks with more than 64 candidates in tiles whose halo does not wrap
+    (the candidate marks span several 64-bit words): every sweep tile shape
+    gives the oracle's predictions bit for bit."""
+    st = clustered_state()
+    i, j = np.triu_indices(len(st["x"]), 1)
+    gap = np.hypot(st["x"][i] - st["x"][j], st["y"][i] - st["y"][j]) - st["rad"][i] - st["rad"][j]
+    assert gap.min() > 0.01
+    o = oracle.Oracle(st)
+    eo = o.events()
+    b = gpu.Batch([st])
+    for v in range(8):
+        b.set_sweep_shape(v)
+        b.sweep_predict()
+        eg = b.events(0)
+        for k in ("t_cross", "t_coll"):
+            np.testing.assert_array_equal(bits(eg[k]), bits(eo[k]), err_msg="shape %d %s" % (v, k))
+        np.testing.assert_array_equal(eg["partner"], eo["partner"], err_msg="shape %d" % v)
+        np.testing.assert_array_equal(eg["dir"], eo["dir"], err_msg="shape %d" % v)
+
+
 def test_thermostat_rescale_matches_oracle(gpu, oracle):
     """Velocity-rescale thermostat events (EDMD.c:4828-4923) every 0.05: the
     trace through ~30 rescales and full re-predictions matches the oracle,
