@@ -271,7 +271,9 @@ static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
     c.trace_cap = (int32_t)d->trace_capacity;
     c.thermo_cap = d->thermo_capacity > 0 ? d->thermo_capacity : 4096;
     double rmin = d->rad[0];
-    c.mono = 1;
+    /* equal radii bit for bit; a growth start's radii reach their targets
+     * through different products (rad + vr*dt per disk), so they are not */
+    c.mono = d->vr > 0 ? 0 : 1;
     for (int i = 0; i < d->n; i++) {
         if (memcmp(&d->rad[i], &d->rad[0], sizeof(double)) != 0) c.mono = 0;
         if (!(d->rad[i] > 0) || !(d->m[i] > 0))
@@ -867,23 +869,43 @@ extern "C" int edmd_batch_sweep_predict(edmd_batch* b) {
     return check_status(b);
 }
 
+/* Times the prediction sweep on the batch's current state.  The sweep
+ * rewrites every event slot, leaf and rank without rebuilding the calendar
+ * above the leaves, so the state is saved first and put back afterwards:
+ * the batch continues exactly as if the call had not been made. */
 extern "C" int edmd_batch_time_sweep(edmd_batch* b, int reps, double* avg_ms) {
     if (!b || reps <= 0 || !avg_ms) return fail(EDMD_ERR_ARG, "bad arguments");
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, b->stream));
-    for (int r = 0; r < reps; r++)
-        HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->sweep_variant,
-                                       b->any_walls ? 1 : 0, b->stream));
-    HIPCHK(hipEventRecord(e1, b->stream));
-    HIPCHK(hipEventSynchronize(e1));
+    if (b->deriv_on) return fail(EDMD_ERR_STATE, "sweep timing is not available while taping");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    void* snap = nullptr;
+    if (hipMalloc(&snap, ckpt_bytes(b)) != hipSuccess)
+        return fail(EDMD_ERR_NOMEM, "hipMalloc of %zu bytes for the timing snapshot failed",
+                    ckpt_bytes(b));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    int rc = e == hipSuccess ? ckpt_copy(b, snap, true) : fail(EDMD_ERR_HIP, "hipEventCreate failed");
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *avg_ms = ms / reps;
-    return check_status(b);
+    if (!rc) {
+        e = hipEventRecord(e0, b->stream);
+        for (int r = 0; r < reps && e == hipSuccess; r++)
+            e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->sweep_variant,
+                                        b->any_walls ? 1 : 0, b->stream);
+        if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) rc = fail(EDMD_ERR_HIP, "sweep timing: %s", hipGetErrorString(e));
+    }
+    if (!rc) rc = check_status(b);
+    int rc2 = ckpt_copy(b, snap, false);
+    if (!rc) rc = rc2;
+    if (hipStreamSynchronize(b->stream) != hipSuccess && !rc)
+        rc = fail(EDMD_ERR_HIP, "sweep timing: stream synchronisation failed");
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(snap);
+    if (!rc) *avg_ms = ms / reps;
+    return rc;
 }
 
 static int sys_check(edmd_batch* b, int s) {

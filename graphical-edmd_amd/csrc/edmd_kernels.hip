@@ -1536,11 +1536,16 @@ __global__ __launch_bounds__(S::THREADS) void k_sweep_tiled(const EdmdSys* __res
             }
         }
         __syncthreads();
-        /* a tile whose halo does not wrap around the box, in a box at
-         * least five cells wide: every candidate lies within two cells of
-         * its owner, so the minimum image (EDMD.c:5896-5913) changes
-         * nothing and is skipped (uniform over the workgroup) */
-        const bool nowrap = tx0 >= 1 && ty0 >= 1 && tx0 + vw <= ncx - 1 && ty0 + vh <= ncy - 1 &&
+        /* a tile whose halo stays off the seam, in a box at least five
+         * cells wide: every candidate lies within two cells of its owner,
+         * so the minimum image (EDMD.c:5896-5913) changes nothing and is
+         * skipped (uniform over the workgroup).  The last cell column and
+         * row are seam cells: a leftward (downward) crossing out of cell 0
+         * can leave x (y) at +0 there (PBCpostX wraps only x < 0,
+         * EDMD.c:5948-5953), and a disk with t_local = t keeps it when
+         * synchronised, so a halo holding them needs the minimum image.
+         * Column and row 0 are only ever halo of tiles that wrap. */
+        const bool nowrap = tx0 >= 1 && ty0 >= 1 && tx0 + vw <= ncx - 2 && ty0 + vh <= ncy - 2 &&
                             ncx >= 5 && ncy >= 5;
         /* equal radii: the contact term 4 r r of every pair is the owner's
          * own (bitwise the same product), no radius read per candidate */

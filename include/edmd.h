@@ -417,7 +417,10 @@ int edmd_adjoint_end(edmd_ctx* ctx);
  * IEEE-rounding tests of the FP64 operations the engine relies on. */
 int edmd_fp64_check(int n, const double* a, const double* b, double* q, double* s);
 /* Average duration (ms) of `reps` prediction sweeps of every system of the
- * batch, timed with HIP events on the batch's stream. */
+ * batch, timed with HIP events on the batch's stream.  The sweeps re-predict
+ * every event slot; the batch's state is saved before and restored after
+ * them (one device allocation of the state's size for the call), so a run
+ * that follows continues as if the call had not been made. */
 int edmd_batch_time_sweep(edmd_batch* b, int reps, double* avg_ms);
 
 #ifdef __cplusplus

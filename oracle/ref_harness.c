@@ -24,6 +24,15 @@
  *                                      1770-1773) and eventListInit takes its
  *                                      growth branch (EDMD.c:1957-1981);
  *                                      event trace, final state and radii
+ *   noise <dump> <out> <nevents> <T> <dtnoise> <vr>
+ *                                      velocity-rescale thermostat: after the
+ *                                      start (a growth start when vr > 0) the
+ *                                      harness sets T and dtnoise (plain
+ *                                      globals, EDMD.c:383, 386) and
+ *                                      schedules the first ADDINGNOISE event
+ *                                      at dtnoise (addEventNoise, EDMD.c:
+ *                                      4814-4822); event trace, final state,
+ *                                      radii and E after every rescale
  * Extra arguments after the mode arguments are passed to constantInit.
  */
 #define main edmd_reference_main
@@ -109,6 +118,10 @@ static void start_grow(const char* dump, double rate, int nextra, char** extra) 
     open_sinks();
 }
 
+/* E right after each rescale (noise mode), for the invariant E/N = T. */
+static double* rescale_e;
+static long n_rescale, cap_rescale;
+
 /* One pass of the dispatch switch of EDMD.c:534-610. */
 static int step(trace_rec* rec) {
     nextEvent = findNextEvent();
@@ -125,7 +138,18 @@ static int step(trace_rec* rec) {
         case COLLISION: doTheCollision(); break;
         case WALL: doTheWall(); break;
         case CELLCROSS: ncross++; doTheCrossing(); break;
-        case ADDINGNOISE: addNoise(); break;
+        case ADDINGNOISE:
+            /* This build fixes noise at 0 (EDMD.c:289), so addNoise() takes
+             * the velocity-rescale branch (v /= sqrt(E/N/T), EDMD.c:4880-
+             * 4901) without the physicalQ() it makes first for noise == 2
+             * (EDMD.c:4830-4832): make that call here. */
+            physicalQ();
+            addNoise();
+            if (n_rescale < cap_rescale) {
+                physicalQ();
+                rescale_e[n_rescale++] = E;
+            }
+            break;
         case SCREENSHOT: takeAScreenshot(); break;
         case THERMO: takeAThermo(); break;
         case GROWSTOP: stopGrow(); break;
@@ -230,6 +254,39 @@ int main(int argc, char** argv) {
         }
         write_state(f);
         for (int i = 0; i < N; i++) write_all(f, &particles[i].rad, 8, 1);
+        fclose(f);
+        return 0;
+    }
+    if (!strcmp(mode, "noise")) {
+        if (argc < 8) fail("noise dump out nevents T dtnoise vr");
+        long nev = atol(argv[4]);
+        double temp = atof(argv[5]);
+        double period = atof(argv[6]);
+        double rate = atof(argv[7]);
+        if (rate > 0)
+            start_grow(dump, rate, argc - 8, argv + 8);
+        else
+            start(dump, argc - 8, argv + 8);
+        T = temp;
+        dtnoise = period;
+        addEventNoise(dtnoise);
+        cap_rescale = nev;
+        rescale_e = calloc((size_t)nev + 1, sizeof(double));
+        if (!rescale_e) fail("out of memory");
+        FILE* f = fopen(argv[3], "wb");
+        if (!f) fail("cannot open output");
+        int64_t n64 = nev;
+        write_all(f, &n64, 8, 1);
+        for (long k = 0; k < nev; k++) {
+            trace_rec rec;
+            step(&rec);
+            write_all(f, &rec, sizeof rec, 1);
+        }
+        write_state(f);
+        for (int i = 0; i < N; i++) write_all(f, &particles[i].rad, 8, 1);
+        int64_t nr = n_rescale;
+        write_all(f, &nr, 8, 1);
+        write_all(f, rescale_e, 8, (size_t)n_rescale);
         fclose(f);
         return 0;
     }

@@ -31,6 +31,13 @@ TRACED = ["sq2000_phi070", "sq2000_phi070_r1", "n400_phi060", "tri1024_phi080", 
           "tiny_ncx3"]
 SWEEP_ONLY = ["tiny_ncx2", "tiny_ncx1"]
 GROWN = ["grow_mono200", "grow_poly150"]
+RESCALED = ["rescale_n400", "rescale_res09_n400", "rescale_grow_poly400"]
+
+
+def rescale_params(g):
+    """Oracle / engine keyword arguments of a thermostat golden vector."""
+    return dict(noise=2, dtnoise=float(g["dtnoise"]), temperature=float(g["temperature"]),
+                vr=float(g["vr"]), res=float(g["res"]))
 
 
 def bits(a):

@@ -240,6 +240,20 @@ def parse_grow(path):
     return out
 
 
+def parse_noise(path):
+    """parse_grow plus E right after every velocity rescale."""
+    out = parse_trace(path)
+    with open(path, "rb") as f:
+        raw = f.read()
+    n = len(out["state"])
+    off = 8 + len(out["trace"]) * TRACE_REC.itemsize + 4 + n * STATE_REC.itemsize + 16 + 64
+    out["rad"] = np.frombuffer(raw, dtype="<f8", count=n, offset=off).copy()
+    off += 8 * n
+    nr, = struct.unpack_from("<q", raw, off)
+    out["rescale_E"] = np.frombuffer(raw, dtype="<f8", count=nr, offset=off + 8).copy()
+    return out
+
+
 def parse_pcf(path):
     with open(path, "rb") as f:
         raw = f.read()

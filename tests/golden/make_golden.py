@@ -16,6 +16,9 @@ initial states and stores inputs and outputs as compressed numpy archives:
   grow_*     growth starts (the radii stored are the targets; vr the rate):
              trace through the GROWSTOP event (stopGrow, EDMD.c:4740-4785)
              and the final state with every radius
+  rescale_*  velocity-rescale thermostat (addNoise, EDMD.c:4828-4923): trace
+             through every rescale, final state and radii, E after each
+             rescale
 
 Run from the repository root after `make -C oracle ref`:
     python tests/golden/make_golden.py
@@ -78,6 +81,21 @@ GROWTH_CASES = [
     # name, state builder, growth rate vr, trace length (past t = 1/vr)
     ("grow_mono200", lambda: F.growth_state(200, 0.50, seed=1), 0.1, 20000),
     ("grow_poly150", lambda: F.growth_state(150, 0.72, seed=3, poly=0.1), 0.1 / 8, 32000),
+]
+
+NOISE_CASES = [
+    # name, state builder, T, dtnoise, growth rate vr (0: no growth), extra
+    # constantInit options, trace length.  addNoise's velocity-rescale branch
+    # (EDMD.c:4880-4901) run by the reference: a hot start rescaled to a lower
+    # T; inelastic disks (-r 0.9) so that every rescale moves E; a growth
+    # start thermostatted while it grows (addNoise through the growth
+    # pointers) and past GROWSTOP.
+    ("rescale_n400", lambda: F.lattice_state(400, 0.60, seed=77, temperature=2.0), 1.5, 0.05,
+     0.0, [], 25000),
+    ("rescale_res09_n400", lambda: F.lattice_state(400, 0.60, seed=78), 1.0, 0.1, 0.0,
+     ["-r", "0.9"], 25000),
+    ("rescale_grow_poly400", lambda: F.growth_state(400, 0.70, seed=21, poly=0.1), 1.0, 0.5,
+     0.05, [], 50000),
 ]
 
 THERMO_CASE = ("thermo64", lambda: F.lattice_state(64, 0.50, seed=4), 1300.0, ["-o", "100"])
@@ -156,6 +174,7 @@ def main():
         np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
         print(name, "N=%d events=%d ncol=%d" % (len(st["x"]), nev, t["glob"]["ncol"]))
 
+    make_noise(tmp)
     make_longrun(tmp)
 
     name, build, tmax, extra = THERMO_CASE
@@ -174,6 +193,31 @@ def main():
     arrs["dtime_thermo"] = np.float64(float(extra[1]))
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
     print(name, "rows=%d ncol=%d" % (len(rows), t["glob"]["ncol"]))
+
+
+def make_noise(tmp):
+    for name, build, temp, dtn, vr, extra, nev in NOISE_CASES:
+        st = build()
+        dump = os.path.join(tmp, name + ".dump")
+        F.write_dump(dump, st)
+        out = os.path.join(tmp, name + ".noise")
+        F.run_ref(["noise", dump, out, nev, repr(temp), repr(dtn), repr(vr)] + extra)
+        t = F.parse_noise(out)
+        arrs = state_arrays(st)
+        arrs["temperature"] = np.float64(temp)
+        arrs["dtnoise"] = np.float64(dtn)
+        arrs["vr"] = np.float64(vr)
+        arrs["res"] = np.float64(float(extra[1]) if extra[:1] == ["-r"] else 1.0)
+        for k in t["trace"].dtype.names:
+            arrs["trace_" + k] = t["trace"][k]
+        arrs.update(pack_state("state", t))
+        arrs["state_rad"] = t["rad"]
+        arrs["rescale_E"] = t["rescale_E"]
+        nres = int((t["trace"]["type"] == F.ADDINGNOISE).sum())
+        assert nres == len(t["rescale_E"]) and nres >= 20, (name, nres)
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
+        print(name, "N=%d events=%d rescales=%d ncol=%d" % (len(st["x"]), nev, nres,
+                                                            t["glob"]["ncol"]))
 
 
 def make_longrun(tmp):
@@ -206,5 +250,7 @@ def make_longrun(tmp):
 if __name__ == "__main__":
     if sys.argv[1:] == ["longrun"]:
         make_longrun(tempfile.mkdtemp(prefix="edmd_golden_"))
+    elif sys.argv[1:] == ["noise"]:
+        make_noise(tempfile.mkdtemp(prefix="edmd_golden_"))
     else:
         main()

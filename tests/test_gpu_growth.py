@@ -129,3 +129,30 @@ def test_cli_default_start_is_the_growth_start(gpu, tmp_path):
         np.testing.assert_array_equal(bits(frame[k]), bits(so[k]), err_msg=k)
     np.testing.assert_array_equal(frame["rad"], st["rad"])
     assert np.sum(st["rad"] == 0.4) == 90
+
+
+def test_sweep_after_monodisperse_growth_matches_oracle(gpu, oracle):
+    """A monodisperse growth start ends with radii that differ in their last
+    bits (each disk's radius is advanced by its own flights), so the sweep
+    after GROWSTOP must use each pair's own contact term DIST2(r_i, r_j)
+    (EDMD.c:35-43), not the owner's 4 r_i r_i.  The box has interior tiles
+    for every tile shape; every disk is compared."""
+    st = F.growth_state(3000, 0.5, seed=31)
+    vr = 0.1
+    o = oracle.Oracle(st, vr=vr)
+    code, done = o.run_until(10.5)
+    assert code == 0 and o.t > 10.0
+    assert len(np.unique(o.radii())) > 1
+    o.sync_all(o.t)
+    from test_gpu_parity import assert_sweep_equals_oracle
+    for v in (1, 5):
+        b = gpu.Batch([st], vr=vr)
+        b.run(max_events=done)
+        assert b.observables(0)["nevents"] == done
+        b.set_sweep_shape(v)
+        b.sweep_predict()
+        sb, so = b.state(0), o.state()
+        for k in ("x", "y", "vx", "vy"):
+            np.testing.assert_array_equal(bits(sb[k]), bits(so[k]), err_msg=k)
+        np.testing.assert_array_equal(bits(b.radii(0)), bits(o.radii()))
+        assert_sweep_equals_oracle(b.events(0), o, "shape %d" % v)

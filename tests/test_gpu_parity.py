@@ -297,11 +297,88 @@ def test_tiled_sweep_matches_oracle_after_events(gpu, oracle, rad):
     for k in ("x", "y", "vx", "vy"):
         np.testing.assert_array_equal(bits(sg[k]), bits(so[k]), err_msg=k)
     # the oracle's re-prediction of every disk at t, in the sweep's form
-    for i in range(0, len(st["x"]), 97):
-        tc, d = o.predict_crossing(i)
-        tk, p = o.predict_collision(i)
-        assert bits(eg["t_cross"][i]) == bits(tc) and eg["dir"][i] == d, i
-        assert bits(eg["t_coll"][i]) == bits(tk) and eg["partner"][i] == p, i
+    assert_sweep_equals_oracle(eg, o)
+
+
+def assert_sweep_equals_oracle(eg, o, label=""):
+    n = len(eg["t_coll"])
+    tc = np.empty(n)
+    tk = np.empty(n)
+    d = np.empty(n, dtype=np.int64)
+    p = np.empty(n, dtype=np.int64)
+    for i in range(n):
+        tc[i], d[i] = o.predict_crossing(i)
+        tk[i], p[i] = o.predict_collision(i)
+    np.testing.assert_array_equal(bits(eg["t_cross"]), bits(tc), err_msg=label + " t_cross")
+    np.testing.assert_array_equal(eg["dir"], d, err_msg=label + " dir")
+    np.testing.assert_array_equal(bits(eg["t_coll"]), bits(tk), err_msg=label + " t_coll")
+    np.testing.assert_array_equal(eg["partner"], p, err_msg=label + " partner")
+
+
+def seam_state():
+    """A 98.5-wide box (49 x 49 cells: 48 is a multiple of the tile widths
+    6, 8 and 16 and of the tile heights 6 and 8, so a tile ends on the
+    column and row next to the seam cells).  Disks S start at x = 0.125
+    moving left at exactly -1: their crossing at t = 0.125 leaves x = +0
+    (PBCpostX wraps only x < 0, EDMD.c:5948-5953) in the last cell column.
+    Disks O in the column before it move right towards S's image.  The same
+    in y.  A sparse background lattice fills the rest."""
+    L = 98.5
+    xs, ys, vxs, vys = [], [], [], []
+    rows = np.arange(3.0, 95.0, 6.0)
+    for y0 in rows:
+        xs += [0.125, L - 2.5]
+        ys += [y0, y0]
+        vxs += [-1.0, 1.0]
+        vys += [0.0, 0.0]
+    for x0 in rows + 1.5:
+        xs += [x0, x0]
+        ys += [0.125, L - 2.5]
+        vxs += [0.0, 0.0]
+        vys += [-1.0, 1.0]
+    rng = np.random.default_rng(3)
+    for gx in np.arange(8.0, 90.0, 5.0):
+        for gy in np.arange(6.0 + 2.5, 90.0, 6.0):
+            xs.append(gx)
+            ys.append(gy)
+            v = rng.standard_normal(2)
+            vxs.append(v[0])
+            vys.append(v[1])
+    n = len(xs)
+    return {"x": np.array(xs), "y": np.array(ys), "vx": np.array(vxs), "vy": np.array(vys),
+            "rad": np.full(n, 0.5), "m": np.ones(n), "Lx": L, "Ly": L}
+
+
+def test_sweep_after_seam_crossings_matches_oracle(gpu, oracle):
+    """Disks that crossed the periodic seam and still sit at +0 in the last
+    cell column (or row) are halo candidates of the tile that ends next to
+    them; the sweep must take their minimum image there, as collisionTime
+    does (EDMD.c:2661-2723 via PBC, EDMD.c:5896-5913).  Every tile shape,
+    every disk."""
+    st = seam_state()
+    # stop right after the seam crossings at t = 0.125: the crossed disks
+    # keep t_local = t, so synchronising them leaves them at +0
+    code, tr = oracle.Oracle(st).run(4000)
+    assert code == 0
+    done = int(np.sum(tr["t"] <= 0.125))
+    o = oracle.Oracle(st)
+    code, _ = o.run(done, trace=False)
+    assert code == 0 and o.t == 0.125
+    o.sync_all(o.t)
+    s0 = o.state()
+    n_seam = int(np.sum((s0["cx"] == 48) & (s0["x"] == 0)) + np.sum((s0["cy"] == 48) & (s0["y"] == 0)))
+    assert n_seam >= 20, n_seam
+    for v in range(8):
+        b = gpu.Batch([st])
+        b.run(max_events=done)
+        assert b.observables(0)["nevents"] == done
+        b.set_sweep_shape(v)
+        b.sweep_predict()
+        sb = b.state(0)
+        so = o.state()
+        for k in ("x", "y", "vx", "vy"):
+            np.testing.assert_array_equal(bits(sb[k]), bits(so[k]), err_msg="shape %d %s" % (v, k))
+        assert_sweep_equals_oracle(b.events(0), o, "shape %d" % v)
 
 
 def clustered_state(seed=5):
@@ -572,3 +649,23 @@ def test_graph_epochs_equal_stepwise_runs(gpu):
     n0 = int(rows[-1, 0]["nevents"])
     a.run(max_events=n0 + 100)
     assert a.observables(0)["nevents"] == n0 + 100
+
+
+def test_time_sweep_leaves_the_run_unchanged(gpu):
+    """Timing the sweep in the middle of a run (the sweep rewrites every
+    event slot and leaf) puts the state back: the run continues bit for bit
+    as one that was never timed."""
+    st = F.lattice_state(4000, 0.75, seed=12, lattice="fit")
+    a = gpu.Batch([st], trace_capacity=30000)
+    b = gpu.Batch([st], trace_capacity=30000)
+    a.run(max_events=10000)
+    b.run(max_events=10000)
+    ms = b.time_sweep(3)
+    assert ms > 0
+    a.run(max_events=30000)
+    b.run(max_events=30000)
+    ta, tb = a.trace(0), b.trace(0)
+    np.testing.assert_array_equal(ta["i"], tb["i"])
+    np.testing.assert_array_equal(bits(ta["t"]), bits(tb["t"]))
+    for k in ("x", "y", "vx", "vy", "t"):
+        np.testing.assert_array_equal(bits(a.state(0)[k]), bits(b.state(0)[k]), err_msg=k)

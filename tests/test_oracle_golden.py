@@ -6,7 +6,8 @@ orientation of mirrored pairs), final states, virial sums and counters."""
 import numpy as np
 import pytest
 
-from conftest import GROWN, SWEEP_ONLY, TRACED, bits, golden_state, load_golden
+from conftest import (GROWN, RESCALED, SWEEP_ONLY, TRACED, bits, golden_state, load_golden,
+                      rescale_params)
 import edmd_fixtures as F
 
 
@@ -123,3 +124,38 @@ def test_growth_matches_reference(oracle, name):
         assert bits(ob[k]) == bits(g["glob_" + k]), k
     assert ob["ncol"] == int(g["glob_ncol"])
     assert ob["ncross"] == int(g["glob_ncross"])
+
+
+@pytest.mark.parametrize("name", RESCALED)
+def test_rescale_thermostat_matches_reference(oracle, name):
+    """The velocity-rescale thermostat (addNoise, EDMD.c:4828-4923) against
+    the reference itself: a hot start rescaled to a lower T, inelastic disks
+    whose every rescale moves E, and a growth start thermostatted while it
+    grows.  Raw trace order (the growth case canonically, as the growth
+    vectors), every rescale's E, final state and radii bit for bit."""
+    g = load_golden(name)
+    nev = len(g["trace_t"])
+    o = oracle.Oracle(golden_state(g), **rescale_params(g))
+    st, tr = o.run(nev)
+    assert st == 0
+    ref = np.zeros(nev, dtype=tr.dtype)
+    for k in ("type", "i", "j", "valid", "t"):
+        ref[k] = g["trace_" + k]
+    if float(g["vr"]) > 0:
+        tr, ref = F.canonical_trace(tr), F.canonical_trace(ref)
+    for k in ("type", "i", "j", "valid"):
+        np.testing.assert_array_equal(tr[k], ref[k], err_msg=k)
+    np.testing.assert_array_equal(bits(tr["t"]), bits(ref["t"]))
+    s = o.state()
+    for k in ("x", "y", "vx", "vy", "t"):
+        np.testing.assert_array_equal(bits(s[k]), bits(g["state_" + k]), err_msg=k)
+    np.testing.assert_array_equal(s["coll"], g["state_coll"])
+    np.testing.assert_array_equal(bits(o.radii()), bits(g["state_rad"]))
+    ob = o.observables()
+    for k in ("t", "E", "px", "py", "virXX", "virYY"):
+        assert bits(ob[k]) == bits(g["glob_" + k]), k
+    assert ob["ncol"] == int(g["glob_ncol"])
+    assert ob["ncross"] == int(g["glob_ncross"])
+    # E/N = T right after every rescale, in the reference's own arithmetic
+    e = g["rescale_E"] / len(g["x"])
+    np.testing.assert_allclose(e, float(g["temperature"]), rtol=1e-13)
