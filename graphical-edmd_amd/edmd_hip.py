@@ -140,6 +140,7 @@ def lib():
             "edmd_fp64_check": (i32, [i32, vp, vp, vp, vp]),
             "edmd_pcf": (i32, [i32, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]),
             "edmd_batch_pcf": (i32, [vp, i32, i32, dbl, dbl, vp, vp, vp]),
+            "edmd_pcf_noalloc": (i32, [i32, vp, vp, dbl, dbl, dbl, dbl, vp, vp, vp]),
             "edmd_batch_deriv_begin": (i32, [vp, i64, i64]),
             "edmd_batch_deriv_run": (i32, [vp, i64, u64, dbl]),
             "edmd_batch_tangent": (i32, [vp] * 5),

@@ -348,6 +348,19 @@ int edmd_pcf(int n, const double* x, const double* y, double Lx, double Ly, doub
  * disks free-flown to the current time without changing the state. */
 int edmd_batch_pcf(edmd_batch* b, int sys, int extrapolate, double dr, double max_r, double* r,
                    double* g, int* nbins);
+/* edmd_pcf without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  x, y (n <= 2^22
+ * positions) and r, g are DEVICE pointers the caller owns; the pair counts
+ * live in a fixed per-GPU array of the library (at most
+ * EDMD_PCF_NOALLOC_MAX_BINS bins, 256 KiB per GPU, held until the process
+ * ends; more bins are an EDMD_ERR_ARG error).  Runs on the null stream of
+ * the current device and waits for it.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU race.  The results equal edmd_pcf's
+ * bit for bit. */
+#define EDMD_PCF_NOALLOC_MAX_BINS 32768
+int edmd_pcf_noalloc(int n, const double* x, const double* y, double Lx, double Ly, double dr,
+                     double max_r, double* r, double* g, int* nbins);
 
 /* ---- derivatives of the event map ------------------------------------------ */
 /* The reference computes no derivatives; these entry points are new.  For
