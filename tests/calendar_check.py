@@ -41,15 +41,23 @@ def check_calendar(cal, ev):
     for l in range(1, len(sizes)):
         base = off
         off += sizes[l - 1]
-        for g in range(sizes[l]):
-            lo, hi = base + 64 * g, base + min(64 * g + 64, sizes[l - 1])
-            best = lo
-            for c in range(lo + 1, hi):
-                if key[c] < key[best] or (key[c] == key[best] and key[best] < np.inf
-                                          and lr[idx[c]] < lr[idx[best]]):
-                    best = c
-            k = off + g
-            if (key[k], idx[k], aux[k]) != (key[best], idx[best], aux[best]):
-                bad.append("level %d node %d: (%r, %d, %d) expected (%r, %d, %d)"
-                           % (l, g, key[k], idx[k], aux[k], key[best], idx[best], aux[best]))
+        m, ng = sizes[l - 1], sizes[l]
+        # children of node g: base + 64 g .. base + min(64 g + 64, m), padded
+        pos = 64 * np.arange(ng)[:, None] + np.arange(64)[None, :]
+        real = pos < m
+        c = base + np.minimum(pos, m - 1)
+        ck = np.where(real, key[c], np.inf)
+        kmin = ck.min(axis=1)
+        tied = real & (ck == kmin[:, None])
+        # equal finite keys: the lower leaf rank; +inf keys: the first child
+        rank = np.where(np.isfinite(kmin)[:, None], lr[idx[c]], 0)
+        score = np.where(tied, rank, np.iinfo(np.int64).max)
+        tied &= score == score.min(axis=1)[:, None]
+        best = c[np.arange(ng), np.argmax(tied, axis=1)]
+        k = off + np.arange(ng)
+        wrong = (key[k] != key[best]) | (idx[k] != idx[best]) | (aux[k] != aux[best])
+        for g in np.nonzero(wrong)[0]:
+            b = best[g]
+            bad.append("level %d node %d: (%r, %d, %d) expected (%r, %d, %d)"
+                       % (l, g, key[k[g]], idx[k[g]], aux[k[g]], key[b], idx[b], aux[b]))
     return bad
