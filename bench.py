@@ -107,43 +107,87 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(E, ns, phis, base_seed, coll, seconds):
-    """The reference's event loop (oracle/_ref/ref_fast: the reference's own
-    sources built with its Makefile flags) on the first replicas of the same
-    workload, one process on one core, until ~`seconds` of work."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores():
+    """Cores this process may use, at most 16 (one GPU's share of a box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _write_dump(E, st, path):
+    args = [st[f].ctypes.data for f in ("x", "y", "vx", "vy", "rad", "m")]
+    E.check(E.lib().edmd_dump_write(path.encode(), 0, 0.0, len(st["x"]), st["Lx"], st["Ly"],
+                                    *args, 1))
+
+
+def cpu_baseline(E, ns, phis, base_seed, coll, seconds, plain_n):
+    """The reference's event loop on the host cores, in the same run: one
+    process per core, each on its own replica of the batched workload
+    (oracle/_ref/ref_fast, the reference's sources built with its Makefile
+    flags), each stopping after `coll` collisions or `seconds`; then one
+    process on the N = plain_n, phi = 0.80 triangular-lattice system of the
+    plain GPU leg.  Falls back to the oracle (one core) where the reference
+    build is absent."""
     ref = os.path.join(HERE, "oracle", "_ref", "ref_fast")
-    kind = "reference"
-    if not os.path.exists(ref):
-        kind = "port"
     tmp = tempfile.mkdtemp(prefix="edmd_cpu_")
-    total_coll = 0
-    total_s = 0.0
-    k = 0
-    while total_s < seconds and k < len(ns):
-        st = E.init_lattice(int(ns[k]), float(phis[k]), rad=0.5, lattice=2,
-                            seed=base_seed + k)
-        if kind == "reference":
-            path = os.path.join(tmp, "r%d.dump" % k)
-            args = [st[f].ctypes.data for f in ("x", "y", "vx", "vy", "rad", "m")]
-            E.check(E.lib().edmd_dump_write(path.encode(), 0, 0.0, len(st["x"]), st["Lx"],
-                                            st["Ly"], *args, 1))
-            r = subprocess.run([ref, "bench", path, str(coll)], capture_output=True, text=True,
-                               timeout=600, check=True)
-            res = json.loads(r.stderr.strip().splitlines()[-1])
-            total_coll += res["ncol"]
-            total_s += res["seconds"]
-        else:
-            sys.path.insert(0, os.path.join(HERE, "tests"))
-            import oracle_bind
-            o = oracle_bind.Oracle(st)
-            t0 = time.perf_counter()
-            o.run_collisions(coll)
-            total_s += time.perf_counter() - t0
-            total_coll += o.observables()["ncol"]
-        k += 1
-    return {"value": total_coll / total_s, "unit": "collisions/s", "cores": 1, "kind": kind,
-            "sample": "%d replicas of the batched workload (N~U[256,2048], phi~U[0.5,0.75]), "
-                      "%d collisions each, one process" % (k, coll)}
+    if not os.path.exists(ref):
+        sys.path.insert(0, os.path.join(HERE, "tests"))
+        import oracle_bind
+        st = E.init_lattice(int(ns[0]), float(phis[0]), rad=0.5, lattice=2, seed=base_seed)
+        o = oracle_bind.Oracle(st)
+        t0 = time.perf_counter()
+        o.run_collisions(coll)
+        el = time.perf_counter() - t0
+        v = o.observables()["ncol"] / el
+        return {"value": v, "unit": "collisions/s", "cores": 1, "kind": "port",
+                "per_core": v, "aggregate": v, "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+                "sample": "replica 0 of the batched workload, %d collisions, the oracle" % coll}
+    cores = host_cores()
+    procs = []
+    for k in range(cores):
+        st = E.init_lattice(int(ns[k]), float(phis[k]), rad=0.5, lattice=2, seed=base_seed + k)
+        path = os.path.join(tmp, "r%d.dump" % k)
+        _write_dump(E, st, path)
+        procs.append(subprocess.Popen([ref, "bench", path, str(coll), str(seconds)],
+                                      stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                                      text=True))
+    rates = []
+    for pr in procs:
+        _, err = pr.communicate(timeout=10 * seconds + 120)
+        if pr.returncode != 0:
+            raise RuntimeError("reference CPU run failed: " + err[-500:])
+        rates.append(json.loads(err.strip().splitlines()[-1])["coll_per_s"])
+    plain = None
+    if plain_n > 0:
+        st = E.init_lattice(plain_n, 0.80, rad=0.5, lattice=1, seed=base_seed)
+        path = os.path.join(tmp, "plain.dump")
+        _write_dump(E, st, path)
+        r = subprocess.run([ref, "bench", path, str(1 << 40), str(seconds)], capture_output=True,
+                           text=True, timeout=20 * seconds + 300, check=True)
+        plain = json.loads(r.stderr.strip().splitlines()[-1])["coll_per_s"]
+        os.remove(path)
+    agg = float(sum(rates))
+    return {"value": agg, "unit": "collisions/s", "cores": cores, "kind": "reference",
+            "per_core": agg / cores, "per_core_min": min(rates), "per_core_max": max(rates),
+            "aggregate": agg, "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "plain_cpu_collisions_per_s": plain,
+            "sample": "%d processes at once, one per core, each on its own replica of the "
+                      "batched workload (N~U[256,2048], phi~U[0.5,0.75]) for %d collisions or "
+                      "%g s; plain: one process on the N=%d phi=0.80 triangular lattice for %g s"
+                      % (cores, coll, seconds, plain_n, seconds)}
 
 
 def tune(b, args, sweep_reps, trial_events):
@@ -257,8 +301,8 @@ def main():
     ap.add_argument("--plain-n", type=int, default=1 << 20)
     ap.add_argument("--plain-collisions", type=int, default=300000)
     ap.add_argument("--sweep-reps", type=int, default=20)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-collisions", type=int, default=100000)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-collisions", type=int, default=20000000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default sweep shapes and residency (no autotuning)")
@@ -370,8 +414,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(E, ns, phis, args.seed, args.cpu_collisions, args.cpu_seconds)
-        log("cpu baseline: %.4e coll/s (%s)" % (cpu["value"], cpu["kind"]))
+        cpu = cpu_baseline(E, ns, phis, args.seed, args.cpu_collisions, args.cpu_seconds,
+                           args.plain_n)
+        log("cpu baseline: %.4e coll/s aggregate on %d cores (%s)"
+            % (cpu["value"], cpu["cores"], cpu["kind"]))
 
     out = {
         "metric": "collisions/sec, batched replicas vs plain single system; prediction GB/s",
@@ -407,6 +453,9 @@ def main():
         "adjoint": adj,
         "dense": dense,
         "cpu_baseline": cpu,
+        "plain_over_cpu_core": (plain / cpu["plain_cpu_collisions_per_s"]
+                                if cpu and cpu.get("plain_cpu_collisions_per_s") else None),
+        "batched_over_cpu_aggregate": value / cpu["aggregate"] if cpu else None,
     }
     print(json.dumps(out))
 

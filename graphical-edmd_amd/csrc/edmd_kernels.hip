@@ -145,6 +145,53 @@ __device__ __forceinline__ int wave_argmin_lane(double v, double base, double& v
     return (int)__builtin_ctzll(__ballot(vv == vmin));
 }
 
+/* Two independent int minima over the wave at once: the same DPP steps as
+ * wave_min_i32 with the two chains interleaved, so each DPP step's latency
+ * is covered by the other chain's step. */
+__device__ __forceinline__ void wave_min_i32x2(int& a, int& b) {
+    a = imin(a, __builtin_amdgcn_update_dpp(0x7fffffff, a, 0xb1, 0xf, 0xf, false));
+    b = imin(b, __builtin_amdgcn_update_dpp(0x7fffffff, b, 0xb1, 0xf, 0xf, false));
+    a = imin(a, __builtin_amdgcn_update_dpp(0x7fffffff, a, 0x4e, 0xf, 0xf, false));
+    b = imin(b, __builtin_amdgcn_update_dpp(0x7fffffff, b, 0x4e, 0xf, 0xf, false));
+    a = imin(a, __builtin_amdgcn_update_dpp(0x7fffffff, a, 0x141, 0xf, 0xf, false));
+    b = imin(b, __builtin_amdgcn_update_dpp(0x7fffffff, b, 0x141, 0xf, 0xf, false));
+    a = imin(a, __builtin_amdgcn_update_dpp(0x7fffffff, a, 0x140, 0xf, 0xf, false));
+    b = imin(b, __builtin_amdgcn_update_dpp(0x7fffffff, b, 0x140, 0xf, 0xf, false));
+    const int a0 = __builtin_amdgcn_readlane(a, 0), a1 = __builtin_amdgcn_readlane(a, 16);
+    const int a2 = __builtin_amdgcn_readlane(a, 32), a3 = __builtin_amdgcn_readlane(a, 48);
+    const int b0 = __builtin_amdgcn_readlane(b, 0), b1 = __builtin_amdgcn_readlane(b, 16);
+    const int b2 = __builtin_amdgcn_readlane(b, 32), b3 = __builtin_amdgcn_readlane(b, 48);
+    a = imin(imin(a0, a1), imin(a2, a3));
+    b = imin(imin(b0, b1), imin(b2, b3));
+}
+
+/* wave_argmin_lane of two values at once (the minima of two owners over the
+ * same candidate pass): the key searches run interleaved; each result is
+ * the one wave_argmin_lane gives for that value alone. */
+__device__ __forceinline__ void wave_argmin_lane2(double va, double vb, double base, double& ma,
+                                                  int& wa, double& mb, int& wb) {
+    const int ka = order_key(va, base), kb = order_key(vb, base);
+    int mka = ka, mkb = kb;
+    wave_min_i32x2(mka, mkb);
+    const uint64_t ta = __ballot(ka == mka), tb = __ballot(kb == mkb);
+    if (__popcll(ta) == 1) {
+        wa = (int)__builtin_ctzll(ta);
+        ma = lane_value(va, wa);
+    } else {
+        const double vv = (ka == mka) ? va : INF_D;
+        ma = wave_min(vv);
+        wa = (int)__builtin_ctzll(__ballot(vv == ma));
+    }
+    if (__popcll(tb) == 1) {
+        wb = (int)__builtin_ctzll(tb);
+        mb = lane_value(vb, wb);
+    } else {
+        const double vv = (kb == mkb) ? vb : INF_D;
+        mb = wave_min(vv);
+        wb = (int)__builtin_ctzll(__ballot(vv == mb));
+    }
+}
+
 /* ------------------------------------------------------------ kinematics */
 
 /* PBC, EDMD.c:5896-5913 */
@@ -738,7 +785,7 @@ __device__ __forceinline__ int scan32(int v) {
  * passes only a strictly earlier time: the reference's strict-less first
  * minimum.  Requires cap <= EDMD_STAGE_CAP, cap % 4 == 0 and cell rows
  * aligned to 16 bytes. */
-template <bool G, bool W = true, class AR>
+template <bool G, bool W = true, bool PF = false, class AR>
 __device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const AR& A,
                                                       int lane, double t, int nown,
                                                       const Owner& a, const Owner& b, double& dta,
@@ -784,16 +831,37 @@ __device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const 
     double btA = INF_D, btB = INF_D;
     int bqA = 0, bqB = 0;
     uint32_t bcA = 0, bcB = 0;
+    /* PF: the next pass's candidate records are read while this pass is
+     * evaluated (one fewer dependent round trip per extra pass; the records
+     * cost registers, so only the loop of large systems does it) */
+    EdmdPart rnext;
+    int qnext = -1;
+    if (PF) {
+        qnext = lane < tot ? stage[lane] : -1;
+        const int own0 = lane >= totA ? b.p : a.p;
+        if (qnext >= 0 && qnext != own0) rnext = A.p[qnext];
+    }
     for (int base = 0; base < tot; base += 64) {
         const int f = base + lane;
         const bool isB = f >= totA;
-        const int q = f < tot ? stage[f] : -1;
+        int q;
+        EdmdPart rcur;
+        if (PF) {
+            q = qnext;
+            rcur = rnext;
+            const int fn = f + 64;
+            qnext = fn < tot ? stage[fn] : -1;
+            const int ownn = fn >= totA ? b.p : a.p;
+            if (qnext >= 0 && qnext != ownn) rnext = A.p[qnext];
+        } else {
+            q = f < tot ? stage[f] : -1;
+        }
         const int own = isB ? b.p : a.p;
         const bool v = q >= 0 && q != own;
         double tq = INF_D;
         uint32_t cq = 0;
         if (v) {
-            EdmdPart r = A.p[q];
+            EdmdPart r = PF ? rcur : A.p[q];
             if (W && !G && (C.field_on || C.damping)) {
                 /* the neighbour flown to t and stored (EDMD.c:2664-2666); a
                  * disk listed for both owners is written twice alike */
@@ -820,8 +888,24 @@ __device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const 
             }
             if (tt < lim) tq = tt;
         }
-        if (base < totA) {
-            const double tA = isB ? INF_D : tq;
+        const bool hasA = base < totA, hasB = base + 64 > totA && nown == 2;
+        const double tA = isB ? INF_D : tq;
+        const double tB = isB ? tq : INF_D;
+        if (hasA && hasB) {
+            double mA, mB;
+            int wA, wB;
+            wave_argmin_lane2(tA, tB, 0.0, mA, wA, mB, wB);
+            if (mA < btA) {
+                btA = mA;
+                bqA = __builtin_amdgcn_readlane(q, wA);
+                bcA = (uint32_t)__builtin_amdgcn_readlane((int)cq, wA);
+            }
+            if (mB < btB) {
+                btB = mB;
+                bqB = __builtin_amdgcn_readlane(q, wB);
+                bcB = (uint32_t)__builtin_amdgcn_readlane((int)cq, wB);
+            }
+        } else if (hasA) {
             double mA;
             const int w = wave_argmin_lane(tA, 0.0, mA);
             if (mA < btA) {
@@ -829,9 +913,7 @@ __device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const 
                 bqA = __builtin_amdgcn_readlane(q, w);
                 bcA = (uint32_t)__builtin_amdgcn_readlane((int)cq, w);
             }
-        }
-        if (base + 64 > totA && nown == 2) {
-            const double tB = isB ? tq : INF_D;
+        } else if (hasB) {
             double mB;
             const int w = wave_argmin_lane(tB, 0.0, mB);
             if (mB < btB) {
@@ -856,14 +938,14 @@ __device__ __forceinline__ void predict_owners_staged(const EdmdConst& C, const 
 
 /* The staged scan where the cell capacity allows it, else the gather scan
  * from HBM; both give the reference's result bit for bit. */
-template <bool G, bool W = true, class AR>
+template <bool G, bool W = true, bool PF = false, class AR>
 __device__ __forceinline__ void predict_owners(const EdmdConst& C, const AR& A, int lane,
                                                double t, int nown, const Owner& a, const Owner& b,
                                                double& dta, int& pa, uint32_t& ca, double& dtb,
                                                int& pb, uint32_t& cb, bool& overlap,
                                                int32_t* stage) {
     if (C.cap <= EDMD_STAGE_CAP && (C.cap & 3) == 0)
-        predict_owners_staged<G, W>(C, A, lane, t, nown, a, b, dta, pa, ca, dtb, pb, cb, overlap,
+        predict_owners_staged<G, W, PF>(C, A, lane, t, nown, a, b, dta, pa, ca, dtb, pb, cb, overlap,
                                     stage);
     else
         predict_owners_gather<G, W>(C, A, lane, t, nown, a, b, dta, pa, ca, dtb, pb, cb, overlap);
@@ -893,11 +975,11 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const AR& A, 
  * only when several lanes hold the minimum key, which is rare.  Keys are
  * never NaN; among +inf keys (padding, parked events) the lowest lane wins.
  * base: a time near the keys (the current time), for the 32-bit search. */
+/* The part of wave_argmin after the key search: kk is this lane's order
+ * key, mkk the minimum key over the wave. */
 template <class AR>
-__device__ __forceinline__ void wave_argmin(const AR& A, double base, double& k, int& i,
-                                            int& x) {
-    const int kk = order_key(k, base);
-    const int mkk = wave_min_i32(kk);
+__device__ __forceinline__ void wave_argmin_finish(const AR& A, int kk, int mkk, double& k,
+                                                   int& i, int& x) {
     const uint64_t near = __ballot(kk == mkk);
     double mk;
     uint64_t tied;
@@ -932,6 +1014,25 @@ __device__ __forceinline__ void wave_argmin(const AR& A, double base, double& k,
     k = mk;
     i = __builtin_amdgcn_readlane(i, w);
     x = __builtin_amdgcn_readlane(x, w);
+}
+
+template <class AR>
+__device__ __forceinline__ void wave_argmin(const AR& A, double base, double& k, int& i,
+                                            int& x) {
+    const int kk = order_key(k, base);
+    wave_argmin_finish(A, kk, wave_min_i32(kk), k, i, x);
+}
+
+/* wave_argmin of two independent (key, leaf, code) sets, the key searches
+ * interleaved; each result is the one wave_argmin gives for its set. */
+template <class AR>
+__device__ __forceinline__ void wave_argmin2(const AR& A, double base, double& ka, int& ia,
+                                             int& xa, double& kb, int& ib, int& xb) {
+    const int kka = order_key(ka, base), kkb = order_key(kb, base);
+    int ma = kka, mb = kkb;
+    wave_min_i32x2(ma, mb);
+    wave_argmin_finish(A, kka, ma, ka, ia, xa);
+    wave_argmin_finish(A, kkb, mb, kb, ib, xb);
 }
 
 /* Level l of the calendar: the leaves for l = 0, else nodes of level l. */
@@ -1037,21 +1138,29 @@ __device__ __forceinline__ void path_update(const EdmdConst& C, const AR& A, int
         int i = P.i[0][l], x = P.x[0][l];
         if (c == cA) { k = kA; i = iA; x = xA; }
         else if (pB >= 0 && c == cB) { k = kB; i = iB; x = xB; }
-        wave_argmin(A, base, k, i, x);
-        const double nkA = k;
-        const int niA = uni(i), nxA = uni(x);
-        double nkB = nkA;
-        int niB = niA, nxB = nxA;
+        double nkA, nkB;
+        int niA, nxA, niB, nxB;
         if (pB >= 0 && gB != gA) {
-            c = (gB << 6) + lane;
-            k = P.k[1][l];
-            i = P.i[1][l];
-            x = P.x[1][l];
-            if (c == cB) { k = kB; i = iB; x = xB; }
+            /* two different nodes: both minima at once */
+            const int c2 = (gB << 6) + lane;
+            double k2 = P.k[1][l];
+            int i2 = P.i[1][l], x2 = P.x[1][l];
+            if (c2 == cB) { k2 = kB; i2 = iB; x2 = xB; }
+            wave_argmin2(A, base, k, i, x, k2, i2, x2);
+            nkB = k2;
+            niB = uni(i2);
+            nxB = uni(x2);
+            nkA = k;
+            niA = uni(i);
+            nxA = uni(x);
+        } else {
             wave_argmin(A, base, k, i, x);
-            nkB = k;
-            niB = uni(i);
-            nxB = uni(x);
+            nkA = k;
+            niA = uni(i);
+            nxA = uni(x);
+            nkB = nkA;
+            niB = niA;
+            nxB = nxA;
         }
         if (lane == 0) {
             EdmdNode n;
@@ -1630,12 +1739,6 @@ __global__ void k_sweep_count(const EdmdSys* __restrict__ systems, int nsys) {
  * per workgroup): it changes rarely and would otherwise hold scalar
  * registers through the whole loop. */
 struct QueueState {
-    double paul;   /* the reference's paulTime */
-    int64_t ctr;   /* insertions so far */
-    int64_t win;   /* buckets passed */
-    int64_t srank; /* rank of the next special event */
-    int64_t swin;  /* bucket of the next special event (-1: overflow list) */
-    int cur;       /* win modulo the bucket count */
     int32_t cand[EDMD_STAGE_MAX]; /* candidate stage of the neighbour scan */
     /* the top level of the calendar (at most 64 nodes): read by every
      * event, written only here while the loop runs and copied to HBM when
@@ -1664,7 +1767,17 @@ struct LoopRegs {
     int special;         /* its kind */
     uint64_t ncol, nevents;
     int32_t status, err_i, err_j;
-    QueueState* q;
+    /* the queue's position (uniform, kept in scalar registers) */
+    double paul;   /* the reference's paulTime */
+    int64_t ctr;   /* insertions so far */
+    int64_t win;   /* buckets passed */
+    int64_t srank; /* rank of the next special event */
+    int64_t swin;  /* bucket of the next special event (-1: overflow list) */
+    int cur;       /* win modulo the bucket count */
+    /* collision virial sums since the last thermo row (collTermX, collTermY,
+     * collTermXY, collTermYX), accumulated here in the reference's order */
+    double vir[4];
+    QueueState* q; /* LDS: candidate stage and the calendar's top level */
 };
 
 __device__ __forceinline__ void next_special(const EdmdState* st, LoopRegs& R) {
@@ -1681,8 +1794,8 @@ __device__ __forceinline__ void next_special(const EdmdState* st, LoopRegs& R) {
         tsp = st->next_grow; rsp = st->rank_grow; wsp = st->win_grow; sp = EDMD_EV_GROWSTOP;
     }
     R.next_special = uni(tsp);
-    R.q->srank = uni64(rsp);
-    R.q->swin = uni64(wsp);
+    R.srank = uni64(rsp);
+    R.swin = uni64(wsp);
     R.special = uni(sp);
 }
 
@@ -1725,13 +1838,13 @@ __device__ __noinline__ void refile_specials(const EdmdConst& C, EdmdState* st, 
 /* One bucket further (addNextPaulEvent, EDMD.c:2287-2307); true when the
  * ring wrapped. */
 __device__ __forceinline__ bool step_window(const EdmdConst& C, EdmdState* st, LoopRegs& R) {
-    R.q->paul = uni(R.q->paul + C.dtp);
-    R.q->win++;
-    if (++R.q->cur < C.nlists) return false;
-    R.q->cur = 0;
-    int64_t ctr = R.q->ctr;
-    refile_specials(C, st, R.q->paul, R.q->win, ctr);
-    R.q->ctr = uni64(ctr);
+    R.paul = uni(R.paul + C.dtp);
+    R.win++;
+    if (++R.cur < C.nlists) return false;
+    R.cur = 0;
+    int64_t ctr = R.ctr;
+    refile_specials(C, st, R.paul, R.win, ctr);
+    R.ctr = uni64(ctr);
     return true;
 }
 
@@ -1748,19 +1861,19 @@ __device__ __forceinline__ bool pick_next(const EdmdConst& C, const AR& A, EdmdS
     /* the particle event's rank matters only against a special at its time */
     auto krank = [&]() { return key == R.next_special ? uni64(A.lrank[leaf]) : (int64_t)0; };
     for (int guard = 0;; ++guard) {
-        const bool part_in = key - R.q->paul < C.dtp;
-        const int64_t ws = R.q->swin;
-        const bool spec_in = ws >= 0 && ws <= R.q->win;
+        const bool part_in = key - R.paul < C.dtp;
+        const int64_t ws = R.swin;
+        const bool spec_in = ws >= 0 && ws <= R.win;
         if (part_in || spec_in)
-            return spec_in && (!part_in || before(R.next_special, R.q->srank, key, krank()));
+            return spec_in && (!part_in || before(R.next_special, R.srank, key, krank()));
         if (!(key < 1e25) && !(R.next_special < 1e25)) return R.next_special <= key;
         if (guard >= (1 << 24)) {
             const double tn = key < R.next_special ? key : R.next_special;
-            while (!(tn - R.q->paul < C.dtp)) {
-                R.q->paul = uni(R.q->paul + C.dtp);
-                R.q->win++;
+            while (!(tn - R.paul < C.dtp)) {
+                R.paul = uni(R.paul + C.dtp);
+                R.win++;
             }
-            return before(R.next_special, R.q->srank, key, krank());
+            return before(R.next_special, R.srank, key, krank());
         }
         if (step_window(C, st, R)) next_special(st, R);
     }
@@ -1768,8 +1881,8 @@ __device__ __forceinline__ bool pick_next(const EdmdConst& C, const AR& A, EdmdS
 
 /* Rank of the next insertion; counts it. */
 __device__ __forceinline__ int64_t take_rank(const EdmdConst& C, LoopRegs& R, double t_ev) {
-    const int64_t r = rank_of(t_ev, R.q->paul, C.dtp, R.q->ctr);
-    R.q->ctr++;
+    const int64_t r = rank_of(t_ev, R.paul, C.dtp, R.ctr);
+    R.ctr++;
     return r;
 }
 
@@ -1810,7 +1923,8 @@ __device__ __forceinline__ double ordered_sum(int n, int lane, F f) {
 }
 
 template <bool G, bool W, class AR>
-__device__ void ev_screenshot(const EdmdConst& C, const AR& A, int lane, LoopRegs& R) {
+__device__ __forceinline__ void ev_screenshot(const EdmdConst& C, const AR& A, int lane,
+                                              LoopRegs& R) {
     /* takeAScreenshot, EDMD.c:4657-4677: free-fly every particle */
     const double t = R.t;
     for (int q = lane; q < C.n; q += 64) {
@@ -1833,7 +1947,7 @@ __device__ void ev_screenshot(const EdmdConst& C, const AR& A, int lane, LoopReg
         r->tl = tl;
     }
     WAVE_SYNC();
-    const int64_t wk = window_of(t + C.dtime, R.q->paul, C.dtp, C.nlists, R.q->win);
+    const int64_t wk = window_of(t + C.dtime, R.paul, C.dtp, C.nlists, R.win);
     const int64_t rk = take_rank(C, R, t + C.dtime);
     if (lane == 0) {
         A.st->next_screen = t + C.dtime;
@@ -1844,11 +1958,11 @@ __device__ void ev_screenshot(const EdmdConst& C, const AR& A, int lane, LoopReg
 }
 
 template <class AR>
-__device__ void ev_thermo(const EdmdConst& C, const AR& A, int lane, LoopRegs& R) {
+__device__ __forceinline__ void ev_thermo(const EdmdConst& C, const AR& A, int lane, LoopRegs& R) {
     /* takeAThermo + saveThermo, EDMD.c:4679-4713, 5232-5333 */
     const double t = R.t;
     const uint64_t ncol = R.ncol;
-    const int64_t wk = window_of(t + C.dtime_thermo, R.q->paul, C.dtp, C.nlists, R.q->win);
+    const int64_t wk = window_of(t + C.dtime_thermo, R.paul, C.dtp, C.nlists, R.win);
     const int64_t rk = take_rank(C, R, t + C.dtime_thermo);
     const auto P = A.p;
     double E = ordered_sum(C.n, lane, [&](int q) {
@@ -1862,7 +1976,7 @@ __device__ void ev_thermo(const EdmdConst& C, const AR& A, int lane, LoopRegs& R
     EdmdState* st = A.st;
     const double area = C.Lx * C.Ly;
     const double dT = t - st->last_thermo;
-    const double v0 = st->vir[0], v1 = st->vir[1], v2 = st->vir[2], v3 = st->vir[3];
+    const double v0 = R.vir[0], v1 = R.vir[1], v2 = R.vir[2], v3 = R.vir[3];
     const int k = st->nthermo;
     if (lane == 0) {
         if (k < C.thermo_cap) {
@@ -1888,11 +2002,11 @@ __device__ void ev_thermo(const EdmdConst& C, const AR& A, int lane, LoopRegs& R
         if (C.walls) st->dpw = 0;
         st->nthermo = k + 1;
         st->last_thermo = t;
-        st->vir[0] = st->vir[1] = st->vir[2] = st->vir[3] = 0;
         st->next_thermo = t + C.dtime_thermo;
         st->rank_thermo = rk;
         st->win_thermo = wk;
     }
+    R.vir[0] = R.vir[1] = R.vir[2] = R.vir[3] = 0;
     WAVE_SYNC();
 }
 
@@ -1906,11 +2020,12 @@ __device__ void rebuild_tree(const EdmdConst& C, const AR& A, int lane) {
 }
 
 template <bool G, bool W, class AR>
-__device__ void ev_rescale(const EdmdConst& C, const AR& A, int lane, LoopRegs& R) {
+__device__ __forceinline__ bool ev_rescale(const EdmdConst& C, const AR& A, int lane,
+                                           LoopRegs& R) {
+    const double t = R.t;
     /* addNoise, velocity-rescale branch (noise == 2): EDMD.c:4828-4832,
      * 4880-4915, 4922 */
     const auto P = A.p;
-    const double t = R.t;
     double E = ordered_sum(C.n, lane, [&](int q) {
         double a = P[q].vx;
         return 0.5 * P[q].m * (a * a + P[q].vy * P[q].vy);
@@ -1928,13 +2043,13 @@ __device__ void ev_rescale(const EdmdConst& C, const AR& A, int lane, LoopRegs& 
     bool ov = false;
     /* per particle: crossing, then collision (EDMD.c:4909-4915) */
     for (int q = lane; q < C.n; q += 64)
-        ov |= sweep_one<G, W>(C, A, t, q, R.q->paul, R.q->ctr + 2 * (int64_t)q, R.q->ctr + 2 * (int64_t)q + 1);
-    R.q->ctr += 2 * (int64_t)C.n;
-    if (__ballot(ov) != 0ull) R.status = EDMD_DEV_OVERLAP;
+        ov |= sweep_one<G, W>(C, A, t, q, R.paul, R.ctr + 2 * (int64_t)q, R.ctr + 2 * (int64_t)q + 1);
+    R.ctr += 2 * (int64_t)C.n;
+    const bool any_ov = __ballot(ov) != 0ull;
     WAVE_SYNC();
     rebuild_tree(C, A, lane);
     top_load(C, A, lane, R.q->top);
-    const int64_t wk = window_of(t + C.dtnoise, R.q->paul, C.dtp, C.nlists, R.q->win);
+    const int64_t wk = window_of(t + C.dtnoise, R.paul, C.dtp, C.nlists, R.win);
     const int64_t rk = take_rank(C, R, t + C.dtnoise);
     if (lane == 0) {
         A.st->next_noise = t + C.dtnoise;
@@ -1942,6 +2057,7 @@ __device__ void ev_rescale(const EdmdConst& C, const AR& A, int lane, LoopRegs& 
         A.st->win_noise = wk;
     }
     WAVE_SYNC();
+    return any_ov;
 }
 
 __device__ __forceinline__ void flag_overlap(LoopRegs& R, bool ov, int p) {
@@ -1952,7 +2068,7 @@ __device__ __forceinline__ void flag_overlap(LoopRegs& R, bool ov, int p) {
 }
 
 template <int ML, bool W, class AR>
-__device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs& R, int i,
+__device__ __forceinline__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs& R, int i,
                              int j) {
     /* doTheCollisionNormal, EDMD.c:3542-3562, 3583-3606, 3802-3827, 3869-3879 */
     const EdmdEv ei = A.e[i];
@@ -1991,7 +2107,8 @@ __device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs
         double dt, dtb;
         int pa, pb;
         uint32_t cpa, cpb;
-        predict_owners<false, W>(C, A, lane, t, 1, oi, oi, dt, pa, cpa, dtb, pb, cpb, ov, R.q->cand);
+        predict_owners<false, W, (ML > 2 && !W)>(C, A, lane, t, 1, oi, oi, dt, pa, cpa, dtb, pb, cpb,
+                                                 ov, R.q->cand);
         flag_overlap(R, ov, i);
         const double tk = t + dt;
         const double tci = uni(ei.tcross);
@@ -2029,12 +2146,12 @@ __device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs
     vyj = uni(vyj - ff * mi * dy);
     xj = uni(xj);
     yj = uni(yj);
+    R.vir[0] += w * dx * dx;
+    R.vir[1] += w * dy * dy;
+    R.vir[2] += w * dx * dy;
+    R.vir[3] += w * dy * dx;
     if (lane == 0) {
         EdmdState* st = A.st;
-        st->vir[0] += w * dx * dx;
-        st->vir[1] += w * dy * dy;
-        st->vir[2] += w * dx * dy;
-        st->vir[3] += w * dy * dx;
         auto q = A.p + i;
         q->x = xi; q->y = yi; q->vx = vxi; q->vy = vyi; q->tl = t; q->coll = colli + 1;
         q = A.p + j;
@@ -2059,7 +2176,8 @@ __device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs
     double dti, dtj;
     int pi_, pj_;
     uint32_t cpi, cpj;
-    predict_owners<false, W>(C, A, lane, t, 2, oi, oj, dti, pi_, cpi, dtj, pj_, cpj, ov, R.q->cand);
+    predict_owners<false, W, (ML > 2 && !W)>(C, A, lane, t, 2, oi, oj, dti, pi_, cpi, dtj, pj_, cpj,
+                                             ov, R.q->cand);
     flag_overlap(R, ov, i);
     double tci2, tcj2;
     int di, dj;
@@ -2088,7 +2206,7 @@ __device__ void ev_collision(const EdmdConst& C, const AR& A, int lane, LoopRegs
 }
 
 template <int ML, class AR>
-__device__ void ev_collision_grow(const EdmdConst& C, const AR& A, int lane, LoopRegs& R,
+__device__ __forceinline__ void ev_collision_grow(const EdmdConst& C, const AR& A, int lane, LoopRegs& R,
                                   int i, int j) {
     /* doTheCollisionGrow, EDMD.c:3465-3540 (2D): both disks fly with their
      * radii growing; the unit contact normal and the summed growth speed
@@ -2218,7 +2336,8 @@ __device__ void ev_collision_grow(const EdmdConst& C, const AR& A, int lane, Loo
  * prediction of every disk with stopGrow's partner counters, and a new
  * calendar.  The caller continues in the Normal loop. */
 template <class AR>
-__device__ void ev_stop_grow(const EdmdConst& C, const AR& A, int lane, LoopRegs& R) {
+__device__ __forceinline__ bool ev_stop_grow(const EdmdConst& C, const AR& A, int lane,
+                                             LoopRegs& R) {
     const double t = R.t;
     for (int q = lane; q < C.n; q += 64) {
         EdmdPart r = A.p[q];
@@ -2252,11 +2371,11 @@ __device__ void ev_stop_grow(const EdmdConst& C, const AR& A, int lane, LoopRegs
     bool ov = false;
     /* per particle: collision, then crossing (EDMD.c:4770-4776) */
     for (int q = lane; q < n; q += 64)
-        ov |= sweep_one<false, false>(C, A, t, q, R.q->paul, R.q->ctr + 2 * (int64_t)q + 1,
-                               R.q->ctr + 2 * (int64_t)q, true);
-    R.q->ctr += 2 * (int64_t)n;
+        ov |= sweep_one<false, false>(C, A, t, q, R.paul, R.ctr + 2 * (int64_t)q + 1,
+                               R.ctr + 2 * (int64_t)q, true);
+    R.ctr += 2 * (int64_t)n;
     WAVE_SYNC();
-    flag_overlap(R, ov, 0);
+    const bool any_ov = __ballot(ov) != 0ull;
     rebuild_tree(C, A, lane);
     top_load(C, A, lane, R.q->top);
     if (lane == 0) {
@@ -2264,10 +2383,11 @@ __device__ void ev_stop_grow(const EdmdConst& C, const AR& A, int lane, LoopRegs
         A.st->next_grow = INF_D;
     }
     WAVE_SYNC();
+    return any_ov;
 }
 
 template <int ML, bool G, bool W, class AR>
-__device__ void ev_single(const EdmdConst& C, const AR& A, int lane, LoopRegs& R, int p,
+__device__ __forceinline__ void ev_single(const EdmdConst& C, const AR& A, int lane, LoopRegs& R, int p,
                           int d) {
     /* The events that change one disk.  d = 1-4: a cell crossing in that
      * direction (doTheCrossing, EDMD.c:2501-2590; its freeFly is the
@@ -2389,7 +2509,8 @@ __device__ void ev_single(const EdmdConst& C, const AR& A, int lane, LoopRegs& R
     double dt, dtb;
     int pa, pb;
     uint32_t cpa, cpb;
-    predict_owners<G, W>(C, A, lane, t, 1, o, o, dt, pa, cpa, dtb, pb, cpb, ov, R.q->cand);
+    predict_owners<G, W, (ML > 2 && !W)>(C, A, lane, t, 1, o, o, dt, pa, cpa, dtb, pb, cpb, ov,
+                                         R.q->cand);
     flag_overlap(R, ov, p);
     double tc;
     int dir;
@@ -2425,10 +2546,11 @@ __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRu
     R.status = st->status;
     R.err_i = st->err_i;
     R.err_j = st->err_j;
-    R.q->paul = uni(st->paul);
-    R.q->ctr = uni64(st->ctr);
-    R.q->win = uni64(st->win);
-    R.q->cur = uni((int)(R.q->win % C.nlists));
+    R.paul = uni(st->paul);
+    R.ctr = uni64(st->ctr);
+    R.win = uni64(st->win);
+    R.cur = uni((int)(R.win % C.nlists));
+    for (int k = 0; k < 4; ++k) R.vir[k] = uni(st->vir[k]);
     next_special(st, R);
     top_load(C, A, lane, R.q->top);
     uint64_t ncross = st->ncross;
@@ -2457,12 +2579,13 @@ __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRu
                 ev_thermo(C, A, lane, R);
             } else if (sp == EDMD_EV_GROWSTOP) {
                 if (G) {
-                    ev_stop_grow(C, A, lane, R);
+                    flag_overlap(R, ev_stop_grow(C, A, lane, R), 0);
                     R.ncol = 0; /* stopGrow resets ncol and ncross, EDMD.c:4745-4746 */
                     ncross = 0;
                 }
             } else {
-                ev_rescale<G, W>(C, A, lane, R);
+                if (ev_rescale<G, W>(C, A, lane, R) && R.status == EDMD_DEV_OK)
+                    R.status = EDMD_DEV_OVERLAP;
             }
             next_special(st, R);
             R.nevents++;
@@ -2501,9 +2624,10 @@ __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRu
         st->err_j = R.err_j;
         st->stop_reason = reason;
         st->last_type = last_type;
-        st->paul = R.q->paul;
-        st->ctr = R.q->ctr;
-        st->win = R.q->win;
+        st->paul = R.paul;
+        st->ctr = R.ctr;
+        st->win = R.win;
+        for (int k = 0; k < 4; ++k) st->vir[k] = R.vir[k];
     }
     top_store(C, A, lane, R.q->top);
 }
@@ -2674,7 +2798,7 @@ __global__ __launch_bounds__(64) void k_peek(const EdmdSys* __restrict__ systems
     const int64_t krank = key == R.next_special ? uni64(A.lrank[p]) : (int64_t)0;
     if (lane != 0) return;
     EdmdPeek r;
-    if (!before(key, krank, R.next_special, R.q->srank)) {
+    if (!before(key, krank, R.next_special, R.srank)) {
         r.t = R.next_special; r.type = R.special; r.i = 0; r.j = 0; r.coll_actual = 0;
     } else if (aux >= 0) {
         r.t = key; r.type = EDMD_EV_COLLISION; r.i = p; r.j = aux; r.coll_actual = A.e[p].cact;
