@@ -107,6 +107,9 @@ struct edmd_batch {
     /* growth starts: per-particle growth rates; any_grow while a system grows */
     double* vr = nullptr;
     bool any_grow = false;
+    /* results of the step-level calls (peek, pair time), allocated once */
+    EdmdPeek* step_peek = nullptr;
+    double* step_pair = nullptr;
     /* derivatives: tape pools, checkpoints (one snapshot of every pool per
      * segment, with the run bounds the segment was run with) */
     bool deriv_on = false;
@@ -461,6 +464,8 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
     if (!rc) rc = dalloc(b, &b->ccnt, b->ncell);
     if (!rc) rc = dalloc(b, &b->cslot, b->nslot);
     if (!rc) rc = dalloc(b, &b->st, (size_t)nsys);
+    if (!rc) rc = dalloc(b, &b->step_peek, 1);
+    if (!rc) rc = dalloc(b, &b->step_pair, 2);
     if (!rc) rc = dalloc(b, &b->trace, b->ntrace);
     if (!rc) rc = dalloc(b, &b->thermo, b->nthermo);
     if (!rc) rc = dalloc(b, &b->erank, 2 * np);
@@ -1262,13 +1267,11 @@ extern "C" int edmd_run_until_kind(edmd_ctx* c, int64_t max_events, uint64_t max
 extern "C" int edmd_peek_event(edmd_ctx* c, double* t, int* kind, int* i, int* j,
                                uint32_t* coll_actual) {
     if (!c) return fail(EDMD_ERR_ARG, "null context");
-    EdmdPeek* d = nullptr;
-    HIPCHK(hipMalloc(&d, sizeof(EdmdPeek)));
+    EdmdPeek* d = c->b->step_peek;
     EdmdPeek h;
     hipError_t e = edmd_launch_peek(c->b->d_sys, 1, d, c->b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, c->b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->b->stream);
-    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost);
-    (void)hipFree(d);
     if (e != hipSuccess) return fail(EDMD_ERR_HIP, "peek failed: %s", hipGetErrorString(e));
     if (t) *t = h.t;
     if (kind) *kind = h.type;
@@ -1288,13 +1291,11 @@ extern "C" int edmd_collision_time(edmd_ctx* c, int i, int j, double* dt) {
     int rc = particle_check(c, i);
     if (!rc) rc = particle_check(c, j);
     if (rc) return rc;
-    double* d = nullptr;
-    HIPCHK(hipMalloc(&d, 2 * sizeof(double)));
+    double* d = c->b->step_pair;
     double h[2] = {0, 0};
     hipError_t e = edmd_launch_pair_time(c->b->d_sys, i, j, d, c->b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, c->b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->b->stream);
-    if (e == hipSuccess) e = hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost);
-    (void)hipFree(d);
     if (e != hipSuccess) return fail(EDMD_ERR_HIP, "pair time failed: %s", hipGetErrorString(e));
     if (dt) *dt = h[0];
     if (h[1] != 0) return fail(EDMD_ERR_OVERLAP, "particles %d and %d overlap", i, j);
@@ -1308,6 +1309,51 @@ extern "C" int edmd_repredict(edmd_ctx* c, int i, int which) {
     HIPCHK(edmd_launch_repredict(c->b->d_sys, c->b->max_nlev, i, which, c->b->stream));
     HIPCHK(hipStreamSynchronize(c->b->stream));
     return check_status(c->b);
+}
+
+/* addEventToQueue / removeEventFromQueue on one eventList slot. */
+static int queue_edit(edmd_ctx* c, int kind, int i, int j, double t, uint32_t cact, int mode) {
+    if (!c) return fail(EDMD_ERR_ARG, "null context");
+    const int n = c->b->hs[0].c.n;
+    switch (kind) {
+        case EDMD_EV_COLLISION:
+            if (mode != 2 && (j < 0 || j >= n))
+                return fail(EDMD_ERR_ARG, "collision partner %d out of range", j);
+            break;
+        case EDMD_EV_CELLCROSS:
+            if (mode != 2 && (j < 1 || j > 4))
+                return fail(EDMD_ERR_ARG, "crossing direction %d is not 1..4", j);
+            break;
+        case EDMD_EV_WALL:
+            if (mode != 2 && (j < 0 || j > 1)) return fail(EDMD_ERR_ARG, "wall side %d is not 0 or 1", j);
+            break;
+        case EDMD_EV_SCREENSHOT:
+        case EDMD_EV_THERMO:
+        case EDMD_EV_ADDINGNOISE:
+            i = 0;
+            break;
+        default:
+            return fail(EDMD_ERR_ARG, "event kind %d cannot be queued by hand", kind);
+    }
+    if (i < 0 || i >= n) return fail(EDMD_ERR_ARG, "particle %d out of range", i);
+    if (mode != 2 && t != t) return fail(EDMD_ERR_ARG, "event time is NaN");
+    HIPCHK(edmd_launch_queue_edit(c->b->d_sys, c->b->max_nlev, kind, i, j, t, cact, mode,
+                                  c->b->stream));
+    HIPCHK(hipStreamSynchronize(c->b->stream));
+    return EDMD_OK;
+}
+
+extern "C" int edmd_queue_add(edmd_ctx* c, int kind, int i, int j, double t,
+                              uint32_t coll_actual) {
+    return queue_edit(c, kind, i, j, t, coll_actual, 0);
+}
+
+extern "C" int edmd_add_collision_event(edmd_ctx* c, int i, int j, double t) {
+    return queue_edit(c, EDMD_EV_COLLISION, i, j, t, 0, 1);
+}
+
+extern "C" int edmd_queue_remove(edmd_ctx* c, int kind, int i) {
+    return queue_edit(c, kind, i, 0, 0.0, 0, 2);
 }
 
 extern "C" int edmd_free_fly(edmd_ctx* c, int i) {

@@ -31,9 +31,11 @@ double res = 1;
 unsigned long int ncol = 0, ncross = 0;
 int Nxcells = 0, Nycells = 0;
 node* nextEvent = nullptr;
+node** eventList = nullptr;
 void (*collisionEvent)(int) = collisionEventNormal;
 void (*crossingEvent)(int) = crossingEventNormal;
 void (*doTheCollision)(void) = doTheCollisionNormal;
+void (*freeFly)(int) = freeFlyNormal;
 }
 
 namespace {
@@ -61,6 +63,11 @@ Config cfg;
 edmd_ctx* g_ctx = nullptr;
 int g_status = EDMD_OK;
 node g_node;
+/* the reference's eventList (EDMD.c:1935-1945): 2N + 10 nodes, slot i the
+ * crossing of particle i, N + i its collision, 2N + 1 the screenshot,
+ * 2N + 2 the thermostat, 2N + 3 the thermo row, 2N + 5 the growth stop */
+std::vector<node> g_nodes;
+std::vector<node*> g_list;
 std::vector<double> hx, hy, hvx, hvy, hrad, hm;
 std::string dump_path, thermo_path;
 
@@ -315,12 +322,29 @@ extern "C" void eventListInit(void) {
     edmd_get_grid(g_ctx, &ncx, &ncy, nullptr, nullptr, nullptr);
     Nxcells = ncx;
     Nycells = ncy;
+    g_nodes.assign(2 * (size_t)N + 10, node{});
+    g_list.resize(g_nodes.size());
+    for (size_t k = 0; k < g_nodes.size(); k++) g_list[k] = &g_nodes[k];
+    for (int i = 0; i < N; i++) {
+        g_nodes[i].i = i;
+        g_nodes[i].type = EDMD_EV_CELLCROSS;
+        g_nodes[N + i].i = i;
+        g_nodes[N + i].type = EDMD_EV_COLLISION;
+    }
+    g_nodes[2 * N + 1].type = EDMD_EV_SCREENSHOT;
+    g_nodes[2 * N + 2].type = EDMD_EV_ADDINGNOISE;
+    g_nodes[2 * N + 3].type = EDMD_EV_THERMO;
+    g_nodes[2 * N + 5].type = EDMD_EV_GROWSTOP;
+    eventList = g_list.data();
     refresh();
 }
 
 extern "C" void freeArrays(void) {
     if (g_ctx) edmd_ctx_destroy(g_ctx);
     g_ctx = nullptr;
+    eventList = nullptr;
+    g_list.clear();
+    g_nodes.clear();
 }
 
 extern "C" void physicalQ(void) { refresh(); }
@@ -342,9 +366,85 @@ extern "C" node* findNextEvent(void) {
     return &g_node;
 }
 
+namespace {
+/* The (kind, particle) a node of eventList stands for, from its place in
+ * the list; other nodes (the findNextEvent snapshot, a caller's own) from
+ * their fields.  False for nodes that name no queue slot. */
+bool node_slot(const node* n, int* kind, int* i) {
+    if (!n) return false;
+    if (!g_nodes.empty() && n >= g_nodes.data() && n < g_nodes.data() + g_nodes.size()) {
+        const long k = n - g_nodes.data();
+        if (k < N) { *kind = EDMD_EV_CELLCROSS; *i = (int)k; return true; }
+        if (k < 2 * (long)N) {
+            *kind = n->type == EDMD_EV_WALL ? EDMD_EV_WALL : EDMD_EV_COLLISION;
+            *i = (int)(k - N);
+            return true;
+        }
+        *i = 0;
+        switch (k - 2 * (long)N) {
+            case 1: *kind = EDMD_EV_SCREENSHOT; return true;
+            case 2: *kind = EDMD_EV_ADDINGNOISE; return true;
+            case 3: *kind = EDMD_EV_THERMO; return true;
+            default: return false;
+        }
+    }
+    *kind = n->type;
+    *i = n->i;
+    return true;
+}
+}  // namespace
+
 extern "C" void removeEventFromQueue(node* toRemove) {
-    /* the GPU calendar replaces a slot when its event is handled */
-    (void)toRemove;
+    /* The event findNextEvent returned is taken out by the handler that
+     * runs it on the GPU (main's loop removes it right before dispatch,
+     * EDMD.c:532-534); any other slot is taken out now. */
+    if (!g_ctx) { g_status = EDMD_ERR_STATE; return; }
+    if (toRemove == &g_node) return;
+    int kind, i;
+    if (!node_slot(toRemove, &kind, &i)) {
+        fprintf(stderr, "removeEventFromQueue: not a node of eventList\n");
+        g_status = EDMD_ERR_ARG;
+        return;
+    }
+    report(edmd_queue_remove(g_ctx, kind, i), "removeEventFromQueue");
+}
+
+extern "C" void addEventToQueue(node* toAdd) {
+    /* files the node as its fields say (addEventToQueue, EDMD.c:2144-2170):
+     * time t, partner or direction j, partner counter collActual */
+    if (!g_ctx) { g_status = EDMD_ERR_STATE; return; }
+    int kind, i;
+    if (!node_slot(toAdd, &kind, &i)) {
+        fprintf(stderr, "addEventToQueue: not a node of eventList\n");
+        g_status = EDMD_ERR_ARG;
+        return;
+    }
+    report(edmd_queue_add(g_ctx, kind, i, toAdd->j, toAdd->t, (uint32_t)toAdd->collActual),
+           "addEventToQueue");
+}
+
+extern "C" void addCollisionEvent(int i, int j, double tc) {
+    /* addCollisionEvent, EDMD.c:3286-3297 */
+    if (!g_ctx) { g_status = EDMD_ERR_STATE; return; }
+    if (i >= 0 && i < N && !g_nodes.empty()) {
+        node* nd = &g_nodes[N + i];
+        nd->t = tc;
+        nd->j = j;
+        nd->type = EDMD_EV_COLLISION;
+    }
+    report(edmd_add_collision_event(g_ctx, i, j, tc), "addCollisionEvent");
+}
+
+extern "C" void addCrossingEvent(int i, int j, double tc) {
+    /* addCrossingEvent, EDMD.c:2487-2496: j is the direction code 1..4 */
+    if (!g_ctx) { g_status = EDMD_ERR_STATE; return; }
+    if (i >= 0 && i < N && !g_nodes.empty()) {
+        node* nd = &g_nodes[i];
+        nd->t = tc;
+        nd->j = j;
+        nd->type = EDMD_EV_CELLCROSS;
+    }
+    report(edmd_queue_add(g_ctx, EDMD_EV_CELLCROSS, i, j, tc, 0), "addCrossingEvent");
 }
 
 extern "C" void doTheCollisionNormal(void) { step_one(EDMD_EV_COLLISION, "doTheCollisionNormal"); }

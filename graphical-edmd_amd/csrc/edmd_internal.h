@@ -45,6 +45,10 @@ hipError_t edmd_launch_pair_time(const EdmdSys* d_sys, int i, int j, double* out
 hipError_t edmd_launch_repredict(const EdmdSys* d_sys, int max_nlev, int i, int which,
                                  hipStream_t s);
 hipError_t edmd_launch_fly_one(const EdmdSys* d_sys, int i, hipStream_t s);
+/* addEventToQueue / removeEventFromQueue of one eventList slot (mode 0:
+ * file with cact, 1: file with the partner's counter, 2: remove) */
+hipError_t edmd_launch_queue_edit(const EdmdSys* d_sys, int max_nlev, int kind, int i, int j,
+                                  double t, uint32_t cact, int mode, hipStream_t s);
 hipError_t edmd_launch_pcf(const double* x, const double* y, const EdmdPart* part, int n, double t,
                            int extrap, double Lx, double Ly, double dr, int nb,
                            unsigned long long* hist, hipStream_t s);

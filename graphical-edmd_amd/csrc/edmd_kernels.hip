@@ -2879,6 +2879,74 @@ __global__ __launch_bounds__(64) void k_repredict(const EdmdSys* __restrict__ sy
     }
 }
 
+/* addEventToQueue / removeEventFromQueue (EDMD.c:2144-2221) on one slot of
+ * the reference's eventList, as a caller fills it (addCollisionEvent,
+ * addCrossingEvent, addWallEvent, addEvent*: EDMD.c:2487-2496, 3286-3297,
+ * 3321-3330, 4421-4440, 4814-4822).  mode 0: file the slot at time t with
+ * partner / direction / wall side j and partner counter cact; mode 1: the
+ * same with the partner's current counter (addCollisionEvent); mode 2:
+ * take the slot out (it never pops until filed again).  A filed slot takes
+ * the next insertion rank, as the reference's queue orders it. */
+template <int ML>
+__global__ __launch_bounds__(64) void k_queue_edit(const EdmdSys* __restrict__ systems, int kind,
+                                                  int i, int j, double t, uint32_t cact,
+                                                  int mode) {
+    const EdmdConst& C = systems[0].c;
+    const EdmdArrays& A = systems[0].a;
+    const int lane = threadIdx.x;
+    EdmdState* st = A.st;
+    const double paul = st->paul;
+    int64_t ctr = st->ctr;
+    const bool remove = mode == 2;
+    if (kind == EDMD_EV_SCREENSHOT || kind == EDMD_EV_THERMO || kind == EDMD_EV_ADDINGNOISE) {
+        const double tn = remove ? INF_D : t;
+        const int64_t rk = remove ? INT64_MAX : rank_of(t, paul, C.dtp, ctr);
+        const int64_t wk = remove ? INT64_MAX : window_of(t, paul, C.dtp, C.nlists, st->win);
+        if (!remove) ctr++;
+        if (lane == 0) {
+            if (kind == EDMD_EV_SCREENSHOT) {
+                st->next_screen = tn; st->rank_screen = rk; st->win_screen = wk;
+            } else if (kind == EDMD_EV_THERMO) {
+                st->next_thermo = tn; st->rank_thermo = rk; st->win_thermo = wk;
+            } else {
+                st->next_noise = tn; st->rank_noise = rk; st->win_noise = wk;
+            }
+            st->ctr = ctr;
+        }
+        return;
+    }
+    EdmdEv e = A.e[i];
+    int64_t rcoll = A.erank[2 * i], rcross = A.erank[2 * i + 1];
+    PathCache<ML> P;
+    path_prefetch<ML>(C, A, lane, i, -1, P);
+    if (kind == EDMD_EV_CELLCROSS) {
+        if (remove) {
+            e.tcross = INF_D;
+        } else {
+            e.tcross = t;
+            e.dir = j;
+            rcross = rank_of(t, paul, C.dtp, ctr++);
+        }
+    } else {
+        if (remove) {
+            e.tcoll = INF_D;
+        } else {
+            e.tcoll = t;
+            e.partner = kind == EDMD_EV_WALL ? EDMD_WALL_CODE(j) : j;
+            e.cact = kind == EDMD_EV_WALL ? 0u : (mode == 1 ? A.p[j].coll : cact);
+            rcoll = rank_of(t, paul, C.dtp, ctr++);
+        }
+    }
+    if (lane == 0) {
+        A.e[i] = e;
+        A.erank[2 * i] = rcoll;
+        A.erank[2 * i + 1] = rcross;
+    }
+    path_update<ML>(C, A, lane, P, i, make_leaf(e.tcoll, rcoll, e.tcross, rcross, e.partner, e.dir),
+                    -1, Leaf{}, nullptr, st->t);
+    if (lane == 0) st->ctr = ctr;
+}
+
 /* freeFlyNormal(p) at the system time, EDMD.c:4954-4990. */
 __global__ void k_fly_one(const EdmdSys* __restrict__ systems, int i) {
     const EdmdConst& C = systems[0].c;
@@ -3168,6 +3236,18 @@ hipError_t edmd_launch_repredict(const EdmdSys* d_sys, int max_nlev, int i, int 
     else if (max_nlev <= 4)
         hipLaunchKernelGGL(k_repredict<4>, dim3(1), dim3(64), 0, s, d_sys, i, which);
     else hipLaunchKernelGGL(k_repredict<EDMD_MAX_LEVELS>, dim3(1), dim3(64), 0, s, d_sys, i, which);
+    return hipGetLastError();
+}
+
+hipError_t edmd_launch_queue_edit(const EdmdSys* d_sys, int max_nlev, int kind, int i, int j,
+                                  double t, uint32_t cact, int mode, hipStream_t s) {
+    if (max_nlev <= 1)
+        hipLaunchKernelGGL(k_queue_edit<1>, dim3(1), dim3(64), 0, s, d_sys, kind, i, j, t, cact, mode);
+    else if (max_nlev <= 4)
+        hipLaunchKernelGGL(k_queue_edit<4>, dim3(1), dim3(64), 0, s, d_sys, kind, i, j, t, cact, mode);
+    else
+        hipLaunchKernelGGL(k_queue_edit<EDMD_MAX_LEVELS>, dim3(1), dim3(64), 0, s, d_sys, kind, i, j,
+                           t, cact, mode);
     return hipGetLastError();
 }
 

@@ -111,6 +111,9 @@ def lib():
             "edmd_collision_time": (i32, [vp, i32, i32, vp]),
             "edmd_repredict": (i32, [vp, i32, i32]),
             "edmd_free_fly": (i32, [vp, i32]),
+            "edmd_queue_add": (i32, [vp, i32, i32, i32, dbl, C.c_uint32]),
+            "edmd_add_collision_event": (i32, [vp, i32, i32, dbl]),
+            "edmd_queue_remove": (i32, [vp, i32, i32]),
             "edmd_batch_create": (i32, [vp, i32, vp]),
             "edmd_batch_destroy": (i32, [vp]),
             "edmd_batch_run": (i32, [vp, i64, u64, dbl]),
@@ -333,6 +336,16 @@ class System:
         check(lib().edmd_free_fly(self.h, int(i)))
 
     # ---- derivatives of the event map (edmd_adjoint_*)
+
+    def queue_add(self, kind, i, j, t, coll_actual=0):
+        """edmd_queue_add: addEventToQueue of the eventList slot of (kind, i)."""
+        check(lib().edmd_queue_add(self.h, int(kind), int(i), int(j), float(t), int(coll_actual)))
+
+    def add_collision_event(self, i, j, t):
+        check(lib().edmd_add_collision_event(self.h, int(i), int(j), float(t)))
+
+    def queue_remove(self, kind, i):
+        check(lib().edmd_queue_remove(self.h, int(kind), int(i)))
 
     def adjoint_begin(self, checkpoint_every=256, tape_capacity=0):
         check(lib().edmd_adjoint_begin(self.h, int(checkpoint_every), int(tape_capacity)))

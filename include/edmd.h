@@ -190,6 +190,8 @@ int edmd_run_until_kind(edmd_ctx* ctx, int64_t max_events, uint64_t max_collisio
 /* findNextEvent (EDMD.c:2313-2327): the pending event that would be handled
  * next, without handling it.  For a crossing j is the direction code. */
 int edmd_peek_event(edmd_ctx* ctx, double* t, int* kind, int* i, int* j, uint32_t* coll_actual);
+/* (edmd_peek_event and edmd_collision_time use a result buffer allocated
+ * with the context, no allocation per call.) */
 /* collisionTimeNormal(p_i, p_j) (EDMD.c:2661-2723) at the current time:
  * relative time, EDMD_NEVER-valued 1e26 when the pair does not meet;
  * EDMD_ERR_OVERLAP when the pair overlaps by more than the tolerance. */
@@ -199,6 +201,24 @@ int edmd_collision_time(edmd_ctx* ctx, int i, int j, double* dt);
 int edmd_repredict(edmd_ctx* ctx, int i, int which);
 /* freeFlyNormal(p_i) to the current time, EDMD.c:4954-4990. */
 int edmd_free_fly(edmd_ctx* ctx, int i);
+/* addEventToQueue(node*) (EDMD.c:2144-2170) of the reference's eventList
+ * slot of (kind, i), filled as the reference's callers fill it:
+ * EDMD_EV_COLLISION: particle i's collision slot, partner j, partner counter
+ * coll_actual (addCollisionEvent, EDMD.c:3286-3297, sets it to the
+ * partner's counter); EDMD_EV_WALL: that slot as a wall hit on side j
+ * (addWallEvent, EDMD.c:3321-3330); EDMD_EV_CELLCROSS: i's crossing slot,
+ * direction j = 1..4 (addCrossingEvent, EDMD.c:2487-2496);
+ * EDMD_EV_SCREENSHOT / _THERMO / _ADDINGNOISE: that scheduled event (i, j
+ * unused).  The event is filed at absolute time t and takes the next
+ * insertion rank, so equal times pop in the reference's order.  A slot
+ * filed twice keeps the second filing. */
+int edmd_queue_add(edmd_ctx* ctx, int kind, int i, int j, double t, uint32_t coll_actual);
+/* addCollisionEvent(i, j, t) (EDMD.c:3286-3297): edmd_queue_add of i's
+ * collision slot with j's current collision counter. */
+int edmd_add_collision_event(edmd_ctx* ctx, int i, int j, double t);
+/* removeEventFromQueue(node*) (EDMD.c:2209-2221) of the slot of (kind, i):
+ * its event never pops until the slot is filed again. */
+int edmd_queue_remove(edmd_ctx* ctx, int kind, int i);
 
 /* ---- batched replicas ---------------------------------------------------- */
 typedef struct edmd_batch edmd_batch;

@@ -43,10 +43,20 @@ extern double phi, Lx, Ly, t, tmax, E, px, py, pressure, res;
 extern unsigned long int ncol, ncross;
 extern int Nxcells, Nycells;
 extern node* nextEvent;
+/* The reference's eventList (EDMD.c:488, 1935-1945), 2N + 10 nodes after
+ * eventListInit: slot i is particle i's crossing, N + i its collision,
+ * 2N + 1 the screenshot, 2N + 2 the thermostat, 2N + 3 the thermo row.
+ * The nodes name the device slots they stand for: removeEventFromQueue and
+ * addEventToQueue act on that slot (the latter with the node's t, j and
+ * collActual, as the reference's callers fill them); addCollisionEvent and
+ * addCrossingEvent also record t and j in the node.  Their fields are not
+ * refreshed by the events the GPU handles. */
+extern node** eventList;
 /* The reference's dispatch pointers (EDMD.c:317-321) for the Normal model. */
 extern void (*collisionEvent)(int);
 extern void (*crossingEvent)(int);
 extern void (*doTheCollision)(void);
+extern void (*freeFly)(int);
 
 /* CLI options of constantInit (EDMD.c:750-977), same letters and long names.
  * Options of model variants this engine does not implement are accepted and
@@ -64,7 +74,12 @@ void eventListInit(void);
 void freeArrays(void);
 void physicalQ(void);
 node* findNextEvent(void);
+/* The node findNextEvent returned is taken out by the handler that runs its
+ * event (main removes it right before dispatching it, EDMD.c:532-534). */
 void removeEventFromQueue(node* toRemove);
+void addEventToQueue(node* toAdd);
+void addCollisionEvent(int i, int j, double t);
+void addCrossingEvent(int i, int j, double t);
 void doTheCollisionNormal(void);
 void doTheCrossing(void);
 void takeAScreenshot(void);

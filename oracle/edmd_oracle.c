@@ -1046,6 +1046,58 @@ int orc_init_events(orc_ctx* s) {
     return s->status;
 }
 
+/* The slot of the reference's eventList a kind of event uses for particle
+ * i: i (crossing), N + i (collision, wall), 2N + 1 screenshot, 2N + 2
+ * noise, 2N + 3 thermo (EDMD.c:1935-2019, 2487-2496, 3286-3297, 3321-3330,
+ * 4421-4440, 4814-4822); -1 for anything else. */
+static int event_slot(const orc_ctx* s, int kind, int i) {
+    switch (kind) {
+        case ORC_CELLCROSS: return (i >= 0 && i < s->n) ? i : -1;
+        case ORC_COLLISION:
+        case ORC_WALL: return (i >= 0 && i < s->n) ? s->n + i : -1;
+        case ORC_SCREENSHOT: return 2 * s->n + 1;
+        case ORC_ADDINGNOISE: return 2 * s->n + 2;
+        case ORC_THERMO: return 2 * s->n + 3;
+        default: return -1;
+    }
+}
+
+int orc_add_event(orc_ctx* s, int kind, int i, int j, double t, uint64_t coll_actual) {
+    /* addEventToQueue (EDMD.c:2144-2170) of the slot's node after the
+     * caller has filled it, as addCollisionEvent / addCrossingEvent /
+     * addWallEvent / addEvent* fill it (EDMD.c:2487-2496, 3286-3297,
+     * 3321-3330, 4421-4440, 4814-4822). */
+    int e = event_slot(s, kind, i);
+    if (e < 0) return ORC_ERR_ARG;
+    ev_node* nd = &s->ev[e];
+    nd->type = kind;
+    nd->t = t;
+    nd->j = j;
+    if (kind == ORC_COLLISION) nd->coll_actual = coll_actual;
+    if (kind == ORC_WALL) nd->coll_actual = 0;
+    if (e < 2 * s->n) nd->i = i;
+    queue_add(s, e);
+    return ORC_OK;
+}
+
+int orc_remove_event(orc_ctx* s, int kind, int i) {
+    /* removeEventFromQueue, EDMD.c:2209-2221 */
+    int e = event_slot(s, kind, i);
+    if (e < 0) return ORC_ERR_ARG;
+    queue_remove(s, e);
+    return ORC_OK;
+}
+
+uint64_t orc_coll_count(const orc_ctx* s, int i) { return s->coll[i]; }
+
+/* crossingEvent(i) / collisionEvent(i) of the Normal model (EDMD.c:
+ * 2405-2496, 2829-3093): predict and file the slot; the caller has taken
+ * the old one out. */
+void orc_crossing_event(orc_ctx* s, int i) { schedule_crossing(s, i); }
+void orc_collision_event(orc_ctx* s, int i) { schedule_collision(s, i); }
+/* freeFly(p_i) at the current time, EDMD.c:4954-4990 */
+void orc_free_fly(orc_ctx* s, int i) { fly(s, i); }
+
 int orc_run(orc_ctx* s, int64_t nevents, orc_trace* trace, int64_t* done) {
     int64_t k = 0;
     int st = s->status;

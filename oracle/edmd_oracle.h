@@ -103,6 +103,21 @@ orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* 
                     const double* vx, const double* vy, const double* rad,
                     const double* m, const orc_params* prm);
 void orc_destroy(orc_ctx* s);
+/* addEventToQueue of the eventList slot of (kind, i) filled with time t,
+ * partner or direction j and, for a collision, the partner counter
+ * coll_actual; kinds ORC_CELLCROSS, ORC_COLLISION, ORC_WALL (j = xy),
+ * ORC_SCREENSHOT, ORC_THERMO, ORC_ADDINGNOISE.  removeEventFromQueue of that
+ * slot; the slot must be queued. */
+int orc_add_event(orc_ctx* s, int kind, int i, int j, double t, uint64_t coll_actual);
+int orc_remove_event(orc_ctx* s, int kind, int i);
+/* particles[i].coll */
+uint64_t orc_coll_count(const orc_ctx* s, int i);
+/* crossingEvent(i), collisionEvent(i) (predict and file; the old slot must
+ * have been removed) and freeFly(p_i) at the current time */
+void orc_crossing_event(orc_ctx* s, int i);
+void orc_collision_event(orc_ctx* s, int i);
+void orc_free_fly(orc_ctx* s, int i);
+
 /* eventListInit (EDMD.c:1935-2019): thermo, screenshot, optional noise
  * event, then crossing and collision prediction for every particle. */
 int orc_init_events(orc_ctx* s);

@@ -69,6 +69,13 @@ def lib():
         L.orc_pcf.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double,
                               C.c_double, dp, dp]
         L.orc_get_radii.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_add_event.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64]
+        L.orc_remove_event.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_coll_count.restype = C.c_uint64
+        L.orc_coll_count.argtypes = [C.c_void_p, C.c_int]
+        L.orc_crossing_event.argtypes = [C.c_void_p, C.c_int]
+        L.orc_collision_event.argtypes = [C.c_void_p, C.c_int]
+        L.orc_free_fly.argtypes = [C.c_void_p, C.c_int]
         L.orc_tape_enable.argtypes = [C.c_void_p, C.c_int64]
         L.orc_tape_count.restype = C.c_int64
         L.orc_tape_count.argtypes = [C.c_void_p]
@@ -283,6 +290,28 @@ class Oracle:
 
     def sync_all(self, t):
         lib().orc_sync_all(self.h, t)
+
+    def add_event(self, kind, i, j, t, coll_actual=None):
+        """addEventToQueue of the eventList slot of (kind, i); a collision
+        takes j's current counter unless coll_actual is given."""
+        if coll_actual is None:
+            coll_actual = lib().orc_coll_count(self.h, j) if kind == 1 else 0
+        assert lib().orc_add_event(self.h, kind, i, j, t, coll_actual) == 0
+
+    def remove_event(self, kind, i):
+        assert lib().orc_remove_event(self.h, kind, i) == 0
+
+    def coll_count(self, i):
+        return lib().orc_coll_count(self.h, i)
+
+    def crossing_event(self, i):
+        lib().orc_crossing_event(self.h, i)
+
+    def collision_event(self, i):
+        lib().orc_collision_event(self.h, i)
+
+    def free_fly(self, i):
+        lib().orc_free_fly(self.h, i)
 
     def tape_enable(self, capacity):
         assert lib().orc_tape_enable(self.h, capacity) == 0

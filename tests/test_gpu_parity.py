@@ -669,3 +669,162 @@ def test_time_sweep_leaves_the_run_unchanged(gpu):
     np.testing.assert_array_equal(bits(ta["t"]), bits(tb["t"]))
     for k in ("x", "y", "vx", "vy", "t"):
         np.testing.assert_array_equal(bits(a.state(0)[k]), bits(b.state(0)[k]), err_msg=k)
+
+
+def test_queue_edits_match_oracle(gpu, oracle):
+    """addEventToQueue / removeEventFromQueue on slots the caller fills
+    (EDMD.c:2144-2221, with addCollisionEvent's and addCrossingEvent's
+    filling, EDMD.c:2487-2496, 3286-3297): a collision and a crossing taken
+    out and filed again by hand (same time and partner or direction, a new
+    insertion rank), two collisions filed with a stale partner counter (they
+    pop invalid and the disk predicts anew), the thermo row moved; then both
+    run on and the traces and states agree bit for bit."""
+    st = F.lattice_state(400, 0.6, seed=31)
+    nev = 3000
+    s = gpu.System(st, trace_capacity=2 * nev)
+    o = oracle.Oracle(st)
+    s.run(max_events=nev)
+    code, tr0 = o.run(nev)
+    assert code == 0
+    t = o.t
+    assert bits(s.observables()["t"]) == bits(t)
+    ev = o.events()
+    COLL, CROSS, THERMO = 1, 0, 3
+    p5, t5 = int(ev["partner"][5]), float(ev["t_coll"][5])
+    d7, t7 = int(ev["dir"][7]), float(ev["t_cross"][7])
+    edits = [("remove", COLL, 5), ("coll", 5, p5, t5),
+             ("remove", COLL, 9), ("coll_ca", 9, 40, t + 1e-4, 0),
+             ("remove", CROSS, 7), ("cross", 7, d7, t7),
+             ("remove", THERMO, 0), ("thermo", t + 1.5e-4),
+             ("remove", COLL, 11), ("coll_ca", 11, 12, t + 2.5e-4, 0)]
+    for e in edits:
+        if e[0] == "remove":
+            s.queue_remove(e[1], e[2])
+            o.remove_event(e[1], e[2])
+        elif e[0] == "coll":
+            s.add_collision_event(e[1], e[2], e[3])
+            o.add_event(COLL, e[1], e[2], e[3])
+        elif e[0] == "coll_ca":
+            s.queue_add(COLL, e[1], e[2], e[3], e[4])
+            o.add_event(COLL, e[1], e[2], e[3], e[4])
+        elif e[0] == "cross":
+            s.queue_add(CROSS, e[1], e[2], e[3])
+            o.add_event(CROSS, e[1], e[2], e[3])
+        elif e[0] == "thermo":
+            s.queue_add(THERMO, 0, 0, e[1])
+            o.add_event(THERMO, 0, 0, e[1])
+    s.run(max_events=2 * nev)
+    code, tr = o.run(nev)
+    assert code == 0
+    gt = s.trace()[nev:]
+    assert len(gt) == nev
+    for f in ("type", "i", "j", "valid"):
+        np.testing.assert_array_equal(gt[f], tr[f], err_msg=f)
+    np.testing.assert_array_equal(bits(gt["t"]), bits(tr["t"]))
+    # the stale filings popped invalid, the thermo row at its new time
+    hand = [(r["type"], r["i"], r["j"], r["valid"]) for r in tr[:50]]
+    assert (1, 9, 40, 0) in hand and (1, 11, 12, 0) in hand
+    assert (3, 0, 0, 1) in hand
+    so, sg = o.state(), s.state()
+    for k in ("x", "y", "vx", "vy", "t"):
+        np.testing.assert_array_equal(bits(sg[k]), bits(so[k]), err_msg=k)
+    assert len(s.thermo()) == 1 and bits(s.thermo()[0]["t"]) == bits(t + 1.5e-4)
+    with pytest.raises(gpu.EdmdError):
+        s.queue_add(0, 3, 7, t + 1)  # direction 7
+    with pytest.raises(gpu.EdmdError):
+        s.queue_add(1, 3, 400, t + 1)  # partner out of range
+
+
+def test_compat_gui_pattern_matches_oracle(gpu, oracle, tmp_path):
+    """The reference's other caller (graphics.c:156-176) changes a disk by
+    hand: freeFly through the dispatch pointer, removeEventFromQueue of its
+    eventList slots, crossingEvent / collisionEvent; plus addCollisionEvent,
+    addCrossingEvent and addEventToQueue of an eventList node.  Driven
+    through the compatibility layer against the oracle doing the same."""
+    L = gpu.lib()
+    n = 300
+    st = F.lattice_state(n, 0.6, seed=19)
+    dump = str(tmp_path / "init.dump")
+    F.write_dump(dump, st)
+    L.constantInit.argtypes = [C.c_int, C.POINTER(C.c_char_p)]
+    L.findNextEvent.restype = C.c_void_p
+    L.edmd_compat_status.restype = C.c_int
+    args = [b"edmd", b"-l", dump.encode(), b"-t", b"1e9", b"--no-output"]
+    L.constantInit(len(args), (C.c_char_p * len(args))(*args))
+    for f in (L.particlesInit, L.cellListInit, L.eventListInit, L.physicalQ):
+        f()
+    assert L.edmd_compat_status() == 0
+
+    class Node(C.Structure):
+        _fields_ = [("lft", C.c_void_p), ("rgt", C.c_void_p), ("top", C.c_void_p),
+                    ("i", C.c_int), ("j", C.c_int), ("q", C.c_int), ("t", C.c_double),
+                    ("type", C.c_int), ("collActual", C.c_ulong)]
+
+    event_list = C.POINTER(C.c_void_p).in_dll(L, "eventList")
+    fn_int = C.CFUNCTYPE(None, C.c_int)
+    free_fly = fn_int.in_dll(L, "freeFly")
+    crossing_event = fn_int.in_dll(L, "crossingEvent")
+    collision_event = fn_int.in_dll(L, "collisionEvent")
+    L.removeEventFromQueue.argtypes = [C.c_void_p]
+    L.addEventToQueue.argtypes = [C.c_void_p]
+    L.addCollisionEvent.argtypes = [C.c_int, C.c_int, C.c_double]
+    L.addCrossingEvent.argtypes = [C.c_int, C.c_int, C.c_double]
+    handlers = {0: L.doTheCrossing, 1: L.doTheCollisionNormal, 2: L.takeAScreenshot,
+                3: L.takeAThermo}
+
+    def loop(k):
+        got = []
+        for _ in range(k):
+            nd = Node.from_address(L.findNextEvent())
+            got.append((nd.type, nd.i, nd.j, nd.t))
+            L.removeEventFromQueue(C.addressof(nd))
+            handlers[nd.type]()
+            assert L.edmd_compat_status() == 0
+        return got
+
+    def same(got, tr):
+        for (ty, i, j, tt), r in zip(got, tr):
+            assert ty == r["type"] and bits(tt) == bits(r["t"])
+            if ty == 1:
+                assert sorted((i, j)) == sorted((r["i"], r["j"]))
+            else:
+                assert (i, j) == (r["i"], r["j"])
+
+    o = oracle.Oracle(st)
+    same(loop(200), o.run(200)[1])
+    t = C.c_double.in_dll(L, "t").value
+    assert bits(t) == bits(o.t)
+    # graphics.c:156-176 on disk 12: fly, kick (none here), re-predict
+    p = 12
+    free_fly(p)
+    o.free_fly(p)
+    L.removeEventFromQueue(event_list[p])
+    crossing_event(p)
+    L.removeEventFromQueue(event_list[n + p])
+    collision_event(p)
+    o.remove_event(0, p)
+    o.crossing_event(p)
+    o.remove_event(1, p)
+    o.collision_event(p)
+    # slots filed again through the reference's own add functions, with
+    # the times, partners and directions they hold (a new insertion rank)
+    ev = o.events()
+    L.removeEventFromQueue(event_list[n + 20])
+    L.addCollisionEvent(20, int(ev["partner"][20]), float(ev["t_coll"][20]))
+    o.remove_event(1, 20)
+    o.add_event(1, 20, int(ev["partner"][20]), float(ev["t_coll"][20]))
+    L.removeEventFromQueue(event_list[30])
+    L.addCrossingEvent(30, int(ev["dir"][30]), float(ev["t_cross"][30]))
+    o.remove_event(0, 30)
+    o.add_event(0, 30, int(ev["dir"][30]), float(ev["t_cross"][30]))
+    thermo = Node.from_address(event_list[2 * n + 3])
+    L.removeEventFromQueue(event_list[2 * n + 3])
+    thermo.t = t + 0.0015
+    thermo.j = 0
+    L.addEventToQueue(event_list[2 * n + 3])
+    o.remove_event(3, 0)
+    o.add_event(3, 0, 0, t + 0.0015)
+    assert L.edmd_compat_status() == 0
+    same(loop(300), o.run(300)[1])
+    assert C.c_ulong.in_dll(L, "ncol").value == o.observables()["ncol"]
+    L.freeArrays()

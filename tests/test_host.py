@@ -49,6 +49,10 @@ def test_compat_globals_exported():
         C.c_double.in_dll(L, name)
     for name in ("ncol", "ncross"):
         C.c_ulong.in_dll(L, name)
+    # eventList (a node** set by eventListInit) and the dispatch pointers
+    assert not C.c_void_p.in_dll(L, "eventList").value
+    for name in ("freeFly", "collisionEvent", "crossingEvent", "doTheCollision"):
+        assert C.c_void_p.in_dll(L, name).value
 
 
 @pytest.mark.skipif(E.device_count() > 0, reason="checks the no-device path")
