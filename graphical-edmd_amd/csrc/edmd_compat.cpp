@@ -6,6 +6,7 @@
  */
 #include <getopt.h>
 #include <math.h>
+#include <time.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -57,9 +58,56 @@ struct Config {
     uint64_t max_coll = ~(uint64_t)0;
     int full_precision = 0;
     int no_output = 0;
+    int64_t trace_capacity = 0;
 };
 
 Config cfg;
+
+/* MT19937 (Matsumoto and Nishimura, ACM TOMACS 8(1), 1998; the 2002
+ * initialisation by a 32-bit seed), the random stream the reference's
+ * starts draw from (init_genrand / genrand of mersenne.c), with its draw
+ * genrand() = k / 2^32 for the 32-bit output k and drand(a, b) =
+ * genrand() (b - a) + a (EDMD.c:5699-5701).  One stream for the process, as
+ * in the reference. */
+struct Mt19937 {
+    static constexpr int NN = 624, MM = 397;
+    uint32_t mt[NN];
+    int mti = NN + 1;
+    void seed(uint32_t s) {
+        mt[0] = s;
+        for (mti = 1; mti < NN; mti++)
+            mt[mti] = 1812433253u * (mt[mti - 1] ^ (mt[mti - 1] >> 30)) + (uint32_t)mti;
+    }
+    uint32_t next() {
+        static const uint32_t mag01[2] = {0x0u, 0x9908b0dfu};
+        uint32_t y;
+        if (mti >= NN) {
+            if (mti == NN + 1) seed(5489u);
+            int kk;
+            for (kk = 0; kk < NN - MM; kk++) {
+                y = (mt[kk] & 0x80000000u) | (mt[kk + 1] & 0x7fffffffu);
+                mt[kk] = mt[kk + MM] ^ (y >> 1) ^ mag01[y & 1u];
+            }
+            for (; kk < NN - 1; kk++) {
+                y = (mt[kk] & 0x80000000u) | (mt[kk + 1] & 0x7fffffffu);
+                mt[kk] = mt[kk + (MM - NN)] ^ (y >> 1) ^ mag01[y & 1u];
+            }
+            y = (mt[NN - 1] & 0x80000000u) | (mt[0] & 0x7fffffffu);
+            mt[NN - 1] = mt[MM - 1] ^ (y >> 1) ^ mag01[y & 1u];
+            mti = 0;
+        }
+        y = mt[mti++];
+        y ^= (y >> 11);
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= (y >> 18);
+        return y;
+    }
+    double genrand() { return next() * (1.0 / 4294967296.0); }
+    double drand(double a, double b) { return (genrand() * (b - a)) + a; }
+};
+Mt19937 g_mt;
+
 edmd_ctx* g_ctx = nullptr;
 int g_status = EDMD_OK;
 node g_node;
@@ -140,6 +188,9 @@ extern "C" edmd_ctx* edmd_compat_context(void) { return g_ctx; }
 
 extern "C" void constantInit(int argc, char* argv[]) {
     cfg = Config();
+    /* the reference's main seeds the stream with the clock right before
+     * constantInit (EDMD.c:500); -v seeds it again with the version */
+    g_mt.seed((uint32_t)time(NULL));
     /* option table of EDMD.c:750-797 plus this engine's extra options */
     static struct option longopt[] = {
         {"load", required_argument, NULL, 'l'},
@@ -189,6 +240,7 @@ extern "C" void constantInit(int argc, char* argv[]) {
         {"max-collisions", required_argument, NULL, 3005},
         {"full-precision", no_argument, NULL, 3006},
         {"no-output", no_argument, NULL, 3007},
+        {"trace-capacity", required_argument, NULL, 3008},
         {NULL, 0, NULL, 0}};
     optind = 1;
     int c;
@@ -214,7 +266,10 @@ extern "C" void constantInit(int argc, char* argv[]) {
             case 'L': cfg.control_length = 1; cfg.load = 0; sscanf(optarg, "%lf", &Ly); break;
             case 'X': cfg.control_length = 2; cfg.load = 0; sscanf(optarg, "%lf", &Lx); break;
             case 'o': sscanf(optarg, "%lf", &cfg.dtime_thermo); break;
-            case 'v': sscanf(optarg, "%d", &cfg.version); break;
+            case 'v':
+                sscanf(optarg, "%d", &cfg.version);
+                g_mt.seed((uint32_t)cfg.version); /* init_genrand(version), EDMD.c:903-906 */
+                break;
             case 3001: cfg.lattice = (strcmp(optarg, "tri") == 0) ? 1 : 0; break;
             case 3002:
                 sscanf(optarg, "%lf", &cfg.noise_period);
@@ -224,6 +279,7 @@ extern "C" void constantInit(int argc, char* argv[]) {
             case 3005: cfg.max_coll = strtoull(optarg, NULL, 10); break;
             case 3006: cfg.full_precision = 1; break;
             case 3007: cfg.no_output = 1; break;
+            case 3008: cfg.trace_capacity = strtoll(optarg, NULL, 10); break;
             case '?': break;
             default:
                 fprintf(stderr, "WARNING: option -%c (long code %d) belongs to a model variant this "
@@ -232,6 +288,160 @@ extern "C" void constantInit(int argc, char* argv[]) {
         }
     }
     g_status = EDMD_OK;
+}
+
+namespace {
+
+/* cos and sin of one angle as the reference's build evaluates them: its
+ * compiler turns the pair cos(a), sin(a) of the Box-Muller draws
+ * (EDMD.c:1375-1376, 1800-1801) into one call of the C library's sincos,
+ * whose results can differ from separate cos and sin calls in the last
+ * bit. */
+void ref_sincos(double a, double* s, double* c) { sincos(a, s, c); }
+
+/* physicalQ and normalizePhysicalQ (EDMD.c:5968-5990, 5723-5764; periodic
+ * box, no shear) on the host arrays, in the reference's order. */
+void host_physical_q(double* E_out, double* px_out, double* py_out) {
+    double ppx = 0, ppy = 0, ee = 0;
+    for (int i = 0; i < N; i++) {
+        ppx += hvx[i] * hm[i];
+        ppy += hvy[i] * hm[i];
+        const double vx = hvx[i];
+        ee += 0.5 * hm[i] * (vx * vx + hvy[i] * hvy[i]);
+    }
+    *E_out = ee;
+    *px_out = ppx;
+    *py_out = ppy;
+}
+
+void host_normalize(double einit) {
+    double ee, ppx, ppy;
+    host_physical_q(&ee, &ppx, &ppy);
+    for (int i = 0; i < N; i++) {
+        hvx[i] -= ppx / (N * hm[i]);
+        hvy[i] -= ppy / (N * hm[i]);
+    }
+    host_physical_q(&ee, &ppx, &ppy);
+    for (int i = 0; i < N; i++) {
+        hvx[i] /= sqrt(ee / N / einit);
+        hvy[i] /= sqrt(ee / N / einit);
+    }
+}
+
+/* The text load path (load == 1, EDMD.c:980-986, 1366-1386): a line
+ * "N Lx [Ly]", then one line "type x y rad m" per disk; velocities drawn
+ * from the stream (Box-Muller, E/N = Einit per component pair), then
+ * normalizePhysicalQ.  Returns nonzero on error. */
+int text_start() {
+    FILE* f = fopen(cfg.file.c_str(), "r");
+    if (!f) {
+        fprintf(stderr, "ERROR: cannot open %s\n", cfg.file.c_str());
+        g_status = EDMD_ERR_IO;
+        return 1;
+    }
+    char line[4096];
+    int n = 0;
+    double lx = 0, ly = 0;
+    int nv = fgets(line, sizeof line, f) ? sscanf(line, "%d %lf %lf", &n, &lx, &ly) : 0;
+    if (nv == 2) ly = lx;
+    if (nv < 2 || n < 2 || !(lx > 0) || !(ly > 0)) {
+        fclose(f);
+        fprintf(stderr, "ERROR: %s: first line is not \"N Lx [Ly]\"\n", cfg.file.c_str());
+        g_status = EDMD_ERR_IO;
+        return 1;
+    }
+    N = n;
+    Lx = lx;
+    Ly = ly;
+    hx.resize(n); hy.resize(n); hvx.resize(n); hvy.resize(n); hrad.resize(n); hm.resize(n);
+    int nbig = 0;
+    for (int i = 0; i < n; i++) {
+        int type = 0;
+        if (!fgets(line, sizeof line, f) ||
+            sscanf(line, "%d %lf %lf %lf %lf", &type, &hx[i], &hy[i], &hrad[i], &hm[i]) != 5) {
+            fclose(f);
+            fprintf(stderr, "ERROR: %s: disk line %d is not \"type x y rad m\"\n",
+                    cfg.file.c_str(), i + 1);
+            g_status = EDMD_ERR_IO;
+            return 1;
+        }
+        nbig += type;
+        const double u1 = g_mt.drand(0, 1);
+        const double u2 = g_mt.drand(0, 1);
+        double sn, cs;
+        ref_sincos(2 * M_PI * u2, &sn, &cs);
+        hvx[i] = sqrt(-2 * log(u1) / hm[i] * cfg.einit) * cs;
+        hvy[i] = sqrt(-2 * log(u1) / hm[i] * cfg.einit) * sn;
+    }
+    fclose(f);
+    host_normalize(cfg.einit);
+    const int nsmall = N - nbig;
+    cfg.sizeratio = hrad[N - 1];
+    phi = M_PI * (cfg.sizeratio * cfg.sizeratio * nsmall + nbig) / (Lx * Ly);
+    return 0;
+}
+
+/* The reference's default start (EDMD.c:1119-1184 box, 1690-1810
+ * particlesInit, load == 0, periodic box, no polydispersity): the box from
+ * N, phi and the aspect ratio (or Ly / Lx given by -L / -X), the growth
+ * rate of optimizeGrowConstant, and per disk, from the stream: x, y
+ * uniform in the box, radius 0 growing towards 1 (the first N - Nsmall
+ * disks) or sizeratio, mass 1, Box-Muller velocities of energy EinitGrow =
+ * 0.05 per disk, and one more draw (the charge draw, unused here). */
+int random_start() {
+    const double r2 = (1 + cfg.fraction_small * (cfg.sizeratio * cfg.sizeratio - 1));
+    if (cfg.control_length == 0) {
+        Lx = sqrt(M_PI * N / (phi * cfg.aspect) * r2);
+        Ly = Lx * cfg.aspect;
+    } else if (cfg.control_length == 1) {
+        Lx = Ly / cfg.aspect;
+    } else {
+        Ly = Lx * cfg.aspect;
+        N = (int)(phi * Lx * Ly / (M_PI * r2));
+        phi = M_PI * N * r2 / (Lx * Ly);
+    }
+    if (N < 2 || !(Lx > 0) || !(Ly > 0)) {
+        fprintf(stderr, "ERROR: no box for N = %d, Lx = %g, Ly = %g\n", N, Lx, Ly);
+        g_status = EDMD_ERR_ARG;
+        return 1;
+    }
+    cfg.vr = edmd_growth_rate(phi); /* optimizeGrowConstant, EDMD.c:5766-5799 */
+    const double einit_grow = 0.05;
+    const int nsmall = (int)(N * cfg.fraction_small);
+    const int nbig = N - nsmall;
+    hx.resize(N); hy.resize(N); hvx.resize(N); hvy.resize(N); hrad.resize(N); hm.resize(N);
+    for (int i = 0; i < N; i++) {
+        hx[i] = g_mt.drand(0, Lx);
+        hy[i] = g_mt.drand(0, Ly);
+        /* the target radius: the growth rate of disk i is vr * target */
+        hrad[i] = i < nbig ? 1.0 : cfg.sizeratio;
+        hm[i] = 1;
+        const double u1 = g_mt.drand(0, 1);
+        const double u2 = g_mt.drand(0, 1);
+        double sn, cs;
+        ref_sincos(2 * M_PI * u2, &sn, &cs);
+        hvx[i] = sqrt(-2 * log(u1) / hm[i] * einit_grow) * cs;
+        hvy[i] = sqrt(-2 * log(u1) / hm[i] * einit_grow) * sn;
+        (void)g_mt.drand(0, 1); /* randomTemp, the charge draw, EDMD.c:1806 */
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int edmd_compat_particles(double* x, double* y, double* vx, double* vy, double* rad,
+                                     double* m, double* vr) {
+    if (hx.empty() || (int)hx.size() != N) return EDMD_ERR_STATE;
+    for (int i = 0; i < N; i++) {
+        if (x) x[i] = hx[i];
+        if (y) y[i] = hy[i];
+        if (vx) vx[i] = hvx[i];
+        if (vy) vy[i] = hvy[i];
+        if (rad) rad[i] = hrad[i];
+        if (m) m[i] = hm[i];
+        if (vr) vr[i] = cfg.load == 0 && cfg.lattice < 0 ? cfg.vr * hrad[i] : 0.0;
+    }
+    return EDMD_OK;
 }
 
 extern "C" void particlesInit(void) {
@@ -252,35 +462,22 @@ extern "C" void particlesInit(void) {
         for (int i = 0; i < n; i++) area += M_PI * hrad[i] * hrad[i];
         phi = area / (Lx * Ly);
     } else if (cfg.load == 1) {
-        fprintf(stderr, "ERROR: the text load format (load == 1) draws velocities from the "
-                        "reference's MT19937 stream; load a LAMMPS dump (*.dump) instead\n");
-        g_status = EDMD_ERR_ARG;
-        return;
-    } else {
+        if (text_start()) return;
+    } else if (cfg.lattice >= 0) {
+        /* monodisperse lattice start (--lattice sq|tri) */
         if (cfg.aspect != 1 || cfg.control_length != 0) {
-            fprintf(stderr, "ERROR: only square boxes are available without -l (no -a/-L/-X)\n");
+            fprintf(stderr, "ERROR: the lattice starts are square (no -a/-L/-X)\n");
             g_status = EDMD_ERR_ARG;
             return;
         }
         hx.resize(N); hy.resize(N); hvx.resize(N); hvy.resize(N);
         hrad.assign(N, 1.0); /* the reference's large disks have radius 1 */
         hm.assign(N, 1.0);
-        int rc;
-        if (cfg.lattice >= 0) {
-            /* monodisperse lattice start (--lattice sq|tri) */
-            rc = edmd_init_lattice(N, phi, 1.0, cfg.lattice, (uint64_t)cfg.version, cfg.einit,
+        int rc = edmd_init_lattice(N, phi, 1.0, cfg.lattice, (uint64_t)cfg.version, cfg.einit,
                                    hx.data(), hy.data(), hvx.data(), hvy.data(), &Lx, &Ly);
-        } else {
-            /* the reference's default start (EDMD.c:1690-1810): random
-             * centres, a binary mixture of radii 1 and sizeratio (-q) with
-             * a fraction -x of small disks, growing at the rate
-             * optimizeGrowConstant picks until GROWSTOP */
-            rc = edmd_init_random(N, phi, 1.0, 0.0, cfg.sizeratio, cfg.fraction_small,
-                                  (uint64_t)cfg.version, 0.05, hx.data(), hy.data(), hvx.data(),
-                                  hvy.data(), hrad.data(), hm.data(), &Lx, &Ly);
-            cfg.vr = edmd_growth_rate(phi);
-        }
         if (rc) { report(rc, "particlesInit"); return; }
+    } else {
+        if (random_start()) return;
     }
     /* output names in the spirit of customName, EDMD.c:6015-6065 */
     char base[400];
@@ -309,7 +506,8 @@ extern "C" void eventListInit(void) {
     d.dtime = cfg.dtime;
     d.first_thermo = cfg.first_thermo;
     d.dtime_thermo = cfg.dtime_thermo;
-    d.vr = cfg.load == 0 ? cfg.vr : 0.0;
+    d.vr = cfg.load == 0 && cfg.lattice < 0 ? cfg.vr : 0.0;
+    d.trace_capacity = cfg.trace_capacity;
     d.einit = cfg.einit;
     if (cfg.noise_period > 0) {
         d.noise = 2;

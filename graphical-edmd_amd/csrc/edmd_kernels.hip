@@ -1312,6 +1312,7 @@ struct SweepLds {
     double2 pos[S::STAGE], vel[S::STAGE];
     double rad[S::STAGE];
     int32_t id[S::STAGE];
+    uint32_t coll[S::STAGE]; /* collision counter (the partner counter a prediction records) */
     uint8_t hcell[S::STAGE];
     int16_t own[S::STAGE];     /* staged index of each of the tile's own disks */
     int32_t cnt[S::NH];
@@ -1538,7 +1539,9 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
         e.tcoll = tk;
         e.tcross = tc;
         e.partner = partner;
-        e.cact = partner >= 0 ? A.p[partner].coll : 0u;
+        /* the partner's counter from the stage; no partner: disk 0's
+         * (EDMD.c:3091-3093 files partner 0) */
+        e.cact = best >= 0 ? L.coll[best] : (partner >= 0 ? A.p[partner].coll : 0u);
         e.dir = dir;
         e.pad = 0;
         A.e[p] = e;
@@ -1636,12 +1639,14 @@ __global__ __launch_bounds__(S::THREADS) void k_sweep_tiled(const EdmdSys* __res
             L.vel[u0] = make_double2(r0.vx, r0.vy);
             L.rad[u0] = r0.rad;
             L.id[u0] = q0;
+            L.coll[u0] = r0.coll;
             if (q1 >= 0) {
                 const double lat1 = t - r1.tl;
                 L.pos[u1] = make_double2(r1.x + lat1 * r1.vx, r1.y + lat1 * r1.vy);
                 L.vel[u1] = make_double2(r1.vx, r1.vy);
                 L.rad[u1] = r1.rad;
                 L.id[u1] = q1;
+                L.coll[u1] = r1.coll;
             }
         }
         __syncthreads();

@@ -66,7 +66,16 @@ extern void (*freeFly)(int);
  *   --first-screen T, --first-thermo T
  *   --max-collisions C     also stop after C valid collisions
  *   --full-precision       dump values with 17 significant digits
- *   --no-output            write no dump or thermo files */
+ *   --no-output            write no dump or thermo files
+ *   --trace-capacity K     keep the first K handled events (edmd_get_trace
+ *                          on edmd_compat_context())
+ * Without -l the start is the reference's own (particlesInit, EDMD.c:
+ * 1690-1810: the box from -N/-p/-a or -L/-X, random centres, Box-Muller
+ * velocities, growth at optimizeGrowConstant's rate) drawing from the
+ * reference's MT19937 stream, seeded by -v as the reference seeds it, so
+ * that a given command line starts from the reference's disks bit for
+ * bit; -l with a file not ending in 'p' is the reference's text format
+ * ("N Lx [Ly]", then "type x y rad m" per disk). */
 void constantInit(int argc, char* argv[]);
 void particlesInit(void);
 void cellListInit(void);
@@ -99,6 +108,12 @@ edmd_ctx* edmd_compat_context(void);
  * collision cap, or (when stop_mask != 0) right after an event of a kind in
  * stop_mask; returns the kind of the last event handled or -1 on error. */
 int edmd_compat_run(int stop_mask);
+/* The disks particlesInit made, on the host, before eventListInit: x, y,
+ * velocities, radius (a growth start's targets: 1 or sizeratio), mass and
+ * growth rate (vr times the target; 0 without growth).  Any pointer may be
+ * NULL.  EDMD_ERR_STATE before particlesInit. */
+int edmd_compat_particles(double* x, double* y, double* vx, double* vy, double* rad, double* m,
+                          double* vr);
 /* Output settings parsed by constantInit, for the command-line host. */
 int edmd_compat_outputs(const char** dump_path, const char** thermo_path, int* full_precision);
 

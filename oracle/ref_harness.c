@@ -24,6 +24,13 @@
  *                                      1770-1773) and eventListInit takes its
  *                                      growth branch (EDMD.c:1957-1981);
  *                                      event trace, final state and radii
+ *   cli   <out> <nevents> <args...>   the reference's own start from its
+ *                                      command line (no -l dump: its random
+ *                                      growth start or its text load path,
+ *                                      particlesInit, EDMD.c:1354-1838); the
+ *                                      state right after particlesInit (with
+ *                                      every disk's growth rate), the event
+ *                                      trace, final state and radii
  *   noise <dump> <out> <nevents> <T> <dtnoise> <vr>
  *                                      velocity-rescale thermostat: after the
  *                                      start (a growth start when vr > 0) the
@@ -114,6 +121,34 @@ static void start_grow(const char* dump, double rate, int nextra, char** extra) 
     load = 0;
     eventListInit();
     load = saved;
+    physicalQ();
+    open_sinks();
+}
+
+/* The command line as given (the reference's main, EDMD.c:498-518, with
+ * init_genrand(1) standing in for its time(NULL) seed, which -v replaces);
+ * the particles right after particlesInit are written to f. */
+static void start_cli(int nargs, char** cli, FILE* f) {
+    char* args[64];
+    int n = 0;
+    args[n++] = "edmd";
+    for (int k = 0; k < nargs && n < 63; k++) args[n++] = cli[k];
+    args[n] = NULL;
+    init_genrand(1);
+    constantInit(n, args);
+    configureTransverseVelocityCorrelationSampling(&dtimeThermo, &logSpacing);
+    particlesInit();
+    int32_t n32 = N;
+    double box[3] = {Lx, Ly, vr};
+    fwrite(&n32, 4, 1, f);
+    fwrite(box, 8, 3, f);
+    for (int i = 0; i < N; i++) {
+        particle* p = particles + i;
+        double d[7] = {p->x, p->y, p->vx, p->vy, p->rad, p->m, p->vr};
+        fwrite(d, 8, 7, f);
+    }
+    cellListInit();
+    eventListInit();
     physicalQ();
     open_sinks();
 }
@@ -245,6 +280,25 @@ int main(int argc, char** argv) {
         start_grow(dump, atof(argv[5]), argc - 6, argv + 6);
         FILE* f = fopen(argv[3], "wb");
         if (!f) fail("cannot open output");
+        int64_t n64 = nev;
+        write_all(f, &n64, 8, 1);
+        for (long k = 0; k < nev; k++) {
+            trace_rec rec;
+            step(&rec);
+            write_all(f, &rec, sizeof rec, 1);
+        }
+        write_state(f);
+        for (int i = 0; i < N; i++) write_all(f, &particles[i].rad, 8, 1);
+        fclose(f);
+        return 0;
+    }
+    if (!strcmp(mode, "cli")) {
+        /* cli <out> <nevents> <args...>: argv[2] is the output here */
+        if (argc < 5) fail("cli out nevents args...");
+        long nev = atol(argv[3]);
+        FILE* f = fopen(argv[2], "wb");
+        if (!f) fail("cannot open output");
+        start_cli(argc - 4, argv + 4, f);
         int64_t n64 = nev;
         write_all(f, &n64, 8, 1);
         for (long k = 0; k < nev; k++) {

@@ -254,6 +254,36 @@ def parse_noise(path):
     return out
 
 
+def parse_cli(path):
+    """The cli mode's output: the box, the growth rate and the disks right
+    after particlesInit, then what parse_grow reads."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    n, = struct.unpack_from("<i", raw, 0)
+    Lx, Ly, vr = struct.unpack_from("<3d", raw, 4)
+    init = np.frombuffer(raw, dtype="<f8", count=7 * n, offset=28).reshape(n, 7).copy()
+    off = 28 + 56 * n
+    tmp = path + ".rest"
+    with open(tmp, "wb") as f:
+        f.write(raw[off:])
+    out = parse_grow(tmp)
+    os.remove(tmp)
+    out.update(Lx=Lx, Ly=Ly, vr=vr,
+               init={k: init[:, c].copy() for c, k in
+                     enumerate(("x", "y", "vx", "vy", "rad", "m", "vr"))})
+    return out
+
+
+def write_text_state(path, st, types):
+    """The reference's text start format (EDMD.c:980-986, 1366-1386): a line
+    "N Lx Ly", then "type x y rad m" per disk, values at 17 digits."""
+    with open(path, "w") as f:
+        f.write("%d %.17g %.17g\n" % (len(st["x"]), st["Lx"], st["Ly"]))
+        for k in range(len(st["x"])):
+            f.write("%d %.17g %.17g %.17g %.17g\n" % (types[k], st["x"][k], st["y"][k],
+                                                     st["rad"][k], st["m"][k]))
+
+
 def parse_pcf(path):
     with open(path, "rb") as f:
         raw = f.read()

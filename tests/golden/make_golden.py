@@ -19,6 +19,9 @@ initial states and stores inputs and outputs as compressed numpy archives:
   rescale_*  velocity-rescale thermostat (addNoise, EDMD.c:4828-4923): trace
              through every rescale, final state and radii, E after each
              rescale
+  cli_*      the reference's own starts from its command line (random
+             growth start, text load): the disks right after particlesInit,
+             trace, final state and radii
 
 Run from the repository root after `make -C oracle ref`:
     python tests/golden/make_golden.py
@@ -98,6 +101,32 @@ NOISE_CASES = [
      0.05, [], 50000),
 ]
 
+def text_state():
+    """A text-format start (radius-1 units, as the reference's own): 196
+    disks on a square lattice of spacing 2.6, every third one of radius 0.5
+    (type 0, mass 0.125), the others of radius 1 (type 1, mass 1)."""
+    side, a = 14, 2.6
+    k = np.arange(side * side)
+    big = (k % 3) != 2
+    st = {"x": (k % side + 0.5) * a, "y": (k // side + 0.5) * a,
+          "rad": np.where(big, 1.0, 0.5), "m": np.where(big, 1.0, 0.125),
+          "Lx": side * a, "Ly": side * a}
+    return st, big.astype(int)
+
+
+CLI_CASES = [
+    # name, command line (no -l dump: the reference's own start), trace length
+    # random growth start, default binary mixture (sizeratio 0.4, 30 % small)
+    ("cli_n300_phi060", ["-N", "300", "-p", "0.6", "-v", "7"], 40000),
+    # monodisperse growth start in a box twice as high as wide
+    ("cli_n256_phi055_aspect2", ["-N", "256", "-p", "0.55", "-x", "0", "-a", "2", "-v", "3"],
+     40000),
+    # a box given by its width (-X): N follows from phi
+    ("cli_lx40_phi050", ["-X", "40", "-p", "0.5", "-x", "0.2", "-q", "0.6", "-v", "11"], 30000),
+    # the text load path: velocities from the stream, normalizePhysicalQ
+    ("cli_text196", ["-l", "@TEXT@", "-v", "5", "-E", "1.3"], 20000),
+]
+
 THERMO_CASE = ("thermo64", lambda: F.lattice_state(64, 0.50, seed=4), 1300.0, ["-o", "100"])
 
 # Long run of the N = 2000, phi = 0.70 square-lattice system with the reference built with its shipped
@@ -175,6 +204,7 @@ def main():
         print(name, "N=%d events=%d ncol=%d" % (len(st["x"]), nev, t["glob"]["ncol"]))
 
     make_noise(tmp)
+    make_cli(tmp)
     make_longrun(tmp)
 
     name, build, tmax, extra = THERMO_CASE
@@ -220,6 +250,35 @@ def make_noise(tmp):
                                                             t["glob"]["ncol"]))
 
 
+def make_cli(tmp):
+    for name, args, nev in CLI_CASES:
+        arrs = {}
+        if "@TEXT@" in args:
+            st, types = text_state()
+            path = os.path.join(tmp, name + ".txt")
+            F.write_text_state(path, st, types)
+            args = [path if a == "@TEXT@" else a for a in args]
+            arrs["text_x"], arrs["text_y"] = st["x"], st["y"]
+            arrs["text_rad"], arrs["text_m"] = st["rad"], st["m"]
+            arrs["text_type"] = types.astype(np.int32)
+            arrs["text_Lx"], arrs["text_Ly"] = np.float64(st["Lx"]), np.float64(st["Ly"])
+        out = os.path.join(tmp, name + ".cli")
+        F.run_ref(["cli", out, nev] + args)
+        t = F.parse_cli(out)
+        arrs["args"] = np.array(" ".join(a for a in args if not a.endswith(".txt")))
+        arrs["Lx"], arrs["Ly"], arrs["vr"] = np.float64(t["Lx"]), np.float64(t["Ly"]), np.float64(t["vr"])
+        for k, v in t["init"].items():
+            arrs["init_" + k] = v
+        for k in t["trace"].dtype.names:
+            arrs["trace_" + k] = t["trace"][k]
+        arrs.update(pack_state("state", t))
+        arrs["state_rad"] = t["rad"]
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
+        ngs = int((t["trace"]["type"] == F.GROWSTOP).sum())
+        print(name, "N=%d Lx=%.3f Ly=%.3f vr=%g events=%d growstop=%d ncol=%d"
+              % (len(t["init"]["x"]), t["Lx"], t["Ly"], t["vr"], nev, ngs, t["glob"]["ncol"]))
+
+
 def make_longrun(tmp):
     name, build, times = LONGRUN_CASE
     st = build()
@@ -252,5 +311,7 @@ if __name__ == "__main__":
         make_longrun(tempfile.mkdtemp(prefix="edmd_golden_"))
     elif sys.argv[1:] == ["noise"]:
         make_noise(tempfile.mkdtemp(prefix="edmd_golden_"))
+    elif sys.argv[1:] == ["cli"]:
+        make_cli(tempfile.mkdtemp(prefix="edmd_golden_"))
     else:
         main()
