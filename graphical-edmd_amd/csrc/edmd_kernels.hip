@@ -351,7 +351,17 @@ __device__ __forceinline__ void fly(const EdmdConst& C, double t, double& x, dou
 template <bool F>
 __device__ __forceinline__ void fly_v(const EdmdConst& C, double t, double& x, double& y,
                                       double& vx, double& vy, double& tl) {
-    if (F && C.damping) {
+    /* the field first: with both on the flight is the field's alone
+     * (EDMD.c:4957-4975), while logTime still applies the damping */
+    if (F && C.field_on) {
+        const double dt = t - tl;
+        tl = t;
+        x += dt * vx;
+        y += dt * vy + 0.5 * C.field * dt * dt;
+        vy += dt * C.field;
+        x = wrap_post(x, C.Lx);
+        y = wrap_post(y, C.Ly);
+    } else if (F && C.damping) {
         const double dt = t - tl;
         tl = t;
         const double g = C.gamma;
@@ -360,14 +370,6 @@ __device__ __forceinline__ void fly_v(const EdmdConst& C, double t, double& x, d
         y += vy / g * (1 - ex);
         vx *= ex;
         vy *= ex;
-        x = wrap_post(x, C.Lx);
-        y = wrap_post(y, C.Ly);
-    } else if (F && C.field_on) {
-        const double dt = t - tl;
-        tl = t;
-        x += dt * vx;
-        y += dt * vy + 0.5 * C.field * dt * dt;
-        vy += dt * C.field;
         x = wrap_post(x, C.Lx);
         y = wrap_post(y, C.Ly);
     } else {

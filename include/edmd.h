@@ -104,7 +104,10 @@ typedef struct edmd_system_desc {
      * neighbours are flown when their collision time is taken.  gamma > 0
      * (default 0.02).  exp and log are this library's own (a Taylor series
      * and an atanh series with Cody-Waite reduction), not the C library's.
-     * Not with the field, a growth start or taping. */
+     * With the field as well, the flight is the field's alone (its branch
+     * of freeFlyNormal comes first, EDMD.c:4957-4975) while logTime still
+     * converts the straight-line times of x crossings, x walls and
+     * collisions.  Not with a growth start or taping. */
     int damping;
     double gamma;
 } edmd_system_desc;

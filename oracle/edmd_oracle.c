@@ -147,21 +147,22 @@ static double log_time(const orc_ctx* s, double t_lin) {
 }
 
 /* freeFlyNormal, EDMD.c:4954-4990: straight, parabolic in y under the
- * field, exponentially slowed under damping. */
+ * field, exponentially slowed under damping; with both on, the field's
+ * flight (its branch comes first, EDMD.c:4957). */
 static void free_fly(orc_ctx* s, int p) {
     double dt = s->t - s->tl[p];
     s->tl[p] = s->t;
-    if (s->prm.damping) {
+    if (s->prm.add_field) {
+        s->x[p] += dt * s->vx[p];
+        s->y[p] += dt * s->vy[p] + 0.5 * s->prm.field * dt * dt;
+        s->vy[p] += dt * s->prm.field;
+    } else if (s->prm.damping) {
         double g = s->prm.gamma;
         double ex = orc_exp(-g * dt);
         s->x[p] += s->vx[p] / g * (1 - ex);
         s->y[p] += s->vy[p] / g * (1 - ex);
         s->vx[p] *= ex;
         s->vy[p] *= ex;
-    } else if (s->prm.add_field) {
-        s->x[p] += dt * s->vx[p];
-        s->y[p] += dt * s->vy[p] + 0.5 * s->prm.field * dt * dt;
-        s->vy[p] += dt * s->prm.field;
     } else {
         s->x[p] += dt * s->vx[p];
         s->y[p] += dt * s->vy[p];
@@ -933,7 +934,7 @@ orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* 
     if (prm) s->prm = *prm;
     else orc_default_params(&s->prm);
     /* walls only with the Normal dispatch (doTheWallGrow is not restated) */
-    if ((s->prm.add_field && s->prm.damping) || (s->prm.damping && !(s->prm.gamma > 0))) {
+    if (s->prm.damping && !(s->prm.gamma > 0)) {
         free(s);
         return NULL;
     }

@@ -216,6 +216,9 @@ def test_field_arguments_are_checked(gpu):
 DAMP_CASES = [
     (400, 0.5, 61, dict(damping=1, gamma=0.02)),                           # periodic
     (350, 0.55, 62, dict(damping=1, gamma=0.05, wall_y=1, res_tan=1.0)),  # channel
+    # the field with damping: the field's flight, damped x and collision times
+    (380, 0.5, 63, dict(damping=1, gamma=0.03, add_field=1, field=-0.05, wall_y=1, res_tan=1.0)),
+    (300, 0.45, 64, dict(damping=1, gamma=0.01, add_field=1, field=-0.02, wall_x=1, wall_y=1)),
 ]
 
 
@@ -249,7 +252,7 @@ def test_damping_arguments_are_checked(gpu):
     with pytest.raises(gpu.EdmdError):
         gpu.System(st, damping=1, gamma=0.0)
     with pytest.raises(gpu.EdmdError):
-        gpu.System(st, damping=1, add_field=1)
+        gpu.System(st, damping=1, vr=0.1)
     b = gpu.Batch([st], damping=1)
     with pytest.raises(gpu.EdmdError):
         b.deriv_begin(checkpoint_every=256)

@@ -258,3 +258,30 @@ def test_damped_energy_decays_exactly(oracle):
     lib.orc_sync_all(o.h, T)
     e1 = o.observables()["E"]
     assert abs(e1 / (e0 * np.exp(-2 * g * T)) - 1) < 1e-10, (e1, e0, T)
+
+
+# ---- the field with damping (freeFlyNormal takes the field's branch first)
+
+def test_field_with_damping_closed_form(oracle):
+    """With both on, an x crossing comes at the damped time
+    -ln(1 - g D / vx) / g (logTime), but the flight to it is the field's:
+    x advances by vx t undamped, y falls by f t^2 / 2, vx keeps its value
+    and vy gains f t (EDMD.c:4957-4975).  Taking the damped flight instead
+    would leave the disk exactly on the cell edge with vx e^{-g t}."""
+    g, f = 0.05, -0.02
+    st = disks([(4.5, 5.0), (15.0, 15.0)], [(0.8, 0.0), (0.0, 0.0)])  # cell [4, 6): 1.5 ahead
+    o = oracle.Oracle(st, damping=1, gamma=g, add_field=1, field=f, **QUIET)
+    t1 = -np.log(1 - g * 1.5 / 0.8) / g
+    tc, d = o.predict_crossing(0)
+    assert d == 2 and tc == pytest.approx(t1, rel=1e-13)
+    code, tr = o.run(1)
+    assert code == 0 and tr["type"][0] == F.CELLCROSS and tr["i"][0] == 0
+    oracle.lib().orc_sync_all(o.h, o.t)
+    s = o.state()
+    assert s["x"][0] == pytest.approx(4.5 + 0.8 * t1, rel=1e-13)
+    assert s["x"][0] > 6.0 + 0.05
+    assert s["vx"][0] == 0.8
+    assert s["y"][0] == pytest.approx(5.0 + 0.5 * f * t1 * t1, rel=1e-13)
+    assert s["vy"][0] == pytest.approx(f * t1, rel=1e-13)
+    # the resting disk falls under the field, undamped
+    assert s["vy"][1] == pytest.approx(f * t1, rel=1e-13)
