@@ -235,8 +235,6 @@ static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
     c.einit = d->einit;
     if ((d->wall_x != 0 && d->wall_x != 1) || (d->wall_y != 0 && d->wall_y != 1))
         return fail(EDMD_ERR_ARG, "system %d: wall flags must be 0 or 1", idx);
-    if ((d->wall_x || d->wall_y) && d->vr > 0)
-        return fail(EDMD_ERR_ARG, "system %d: walls need a start without growth (vr = 0)", idx);
     if (!(d->res_wall >= 0) || !(d->res_tan == d->res_tan))
         return fail(EDMD_ERR_ARG, "system %d: bad wall restitution", idx);
     if (d->add_field != 0 && d->add_field != 1)

@@ -313,6 +313,49 @@ __device__ __forceinline__ bool wall_time(const EdmdConst& C, double x, double y
     return wall;
 }
 
+/* The wall slot of collisionEventGrow (EDMD.c:3118-3153): the surface of a
+ * growing disk in a boundary cell closes on the wall at v -/+ vr; no
+ * logTime.  dt starts at the growth version's 1e7; as in wall_time, the x
+ * test sets xy = 0 whenever it applies. */
+__device__ __forceinline__ bool wall_time_grow(const EdmdConst& C, double x, double y, double vx,
+                                               double vy, double r, double vr, int cx, int cy,
+                                               double& dt, int& xy) {
+    bool wall = false;
+    dt = EDMD_GROW_NONE;
+    xy = 2;
+    if (C.walls & 2) {
+        const double vyp = vy + vr, vym = vy - vr;
+        if (cy == 0 && vym < 0) {
+            dt = -(y - r) / vym;
+            wall = true;
+            xy = 1;
+        } else if (cy == C.ncy - 1 && vyp > 0) {
+            dt = (C.Ly - y - r) / vyp;
+            wall = true;
+            xy = 1;
+        }
+    }
+    if (C.walls & 1) {
+        const double vxp = vx + vr, vxm = vx - vr;
+        if (cx == 0 && vxm < 0) {
+            const double dw = -(x - r) / vxm;
+            if (dt > dw) {
+                wall = true;
+                dt = dw;
+            }
+            xy = 0;
+        } else if (cx == C.ncx - 1 && vxp > 0) {
+            const double dw = (C.Lx - x - r) / vxp;
+            if (dt > dw) {
+                wall = true;
+                dt = dw;
+            }
+            xy = 0;
+        }
+    }
+    return wall;
+}
+
 /* PBCinsideCellX/Y, EDMD.c:5922-5936 */
 __device__ __forceinline__ double wrap_inside(double d, double half, double L) {
     if (d >= half) return d - L;
@@ -573,6 +616,8 @@ __device__ __forceinline__ double predict_seq(const EdmdConst& C, const AR& A, d
     int xy = 2;
     bool wall = false;
     if (W && !G && C.walls) wall = wall_time(C, o.x, o.y, o.vx, o.vy, o.rad, o.cx, o.cy, dt, xy);
+    if (W && G && C.walls)
+        wall = wall_time_grow(C, o.x, o.y, o.vx, o.vy, o.rad, ovr, o.cx, o.cy, dt, xy);
     for (int j = -1; j <= 1; ++j) {
         int row = wrap_cell(o.cy + j, C.ncy) * C.ncx;
         for (int k = -1; k <= 1; ++k) {
@@ -951,18 +996,21 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const AR& A, 
                                     stage);
     else
         predict_owners_gather<G, W>(C, A, lane, t, nown, a, b, dta, pa, ca, dtb, pb, cb, overlap);
-    if (W && !G && C.walls) {
+    if (W && C.walls) {
         /* the wall time starts the reference's minimum: a neighbour
          * replaces it only when strictly earlier */
+        auto wt = [&](const Owner& o, double& dw, int& xy) {
+            return G ? wall_time_grow(C, o.x, o.y, o.vx, o.vy, o.r, o.vr, o.cx, o.cy, dw, xy)
+                     : wall_time(C, o.x, o.y, o.vx, o.vy, o.r, o.cx, o.cy, dw, xy);
+        };
         double dw;
         int xy;
-        if (wall_time(C, a.x, a.y, a.vx, a.vy, a.r, a.cx, a.cy, dw, xy) && !(dta < dw)) {
+        if (wt(a, dw, xy) && !(dta < dw)) {
             dta = dw;
             pa = EDMD_WALL_CODE(xy);
             ca = 0;
         }
-        if (nown == 2 && wall_time(C, b.x, b.y, b.vx, b.vy, b.r, b.cx, b.cy, dw, xy) &&
-            !(dtb < dw)) {
+        if (nown == 2 && wt(b, dw, xy) && !(dtb < dw)) {
             dtb = dw;
             pb = EDMD_WALL_CODE(xy);
             cb = 0;
@@ -2342,7 +2390,7 @@ __device__ __forceinline__ void ev_collision_grow(const EdmdConst& C, const AR& 
  * remove the mean momentum, scale to E/N = Einit), then the Normal
  * prediction of every disk with stopGrow's partner counters, and a new
  * calendar.  The caller continues in the Normal loop. */
-template <class AR>
+template <bool W, class AR>
 __device__ __forceinline__ bool ev_stop_grow(const EdmdConst& C, const AR& A, int lane,
                                              LoopRegs& R) {
     const double t = R.t;
@@ -2378,7 +2426,7 @@ __device__ __forceinline__ bool ev_stop_grow(const EdmdConst& C, const AR& A, in
     bool ov = false;
     /* per particle: collision, then crossing (EDMD.c:4770-4776) */
     for (int q = lane; q < n; q += 64)
-        ov |= sweep_one<false, false>(C, A, t, q, R.paul, R.ctr + 2 * (int64_t)q + 1,
+        ov |= sweep_one<false, W>(C, A, t, q, R.paul, R.ctr + 2 * (int64_t)q + 1,
                                R.ctr + 2 * (int64_t)q, true);
     R.ctr += 2 * (int64_t)n;
     WAVE_SYNC();
@@ -2405,7 +2453,7 @@ __device__ __forceinline__ void ev_single(const EdmdConst& C, const AR& A, int l
      * velocity reversed times resW and the tangential one scaled by resT.
      * Both then predict the crossing slot and the collision slot again, in
      * that order. */
-    const bool wall = W && !G && d >= 5;
+    const bool wall = W && d >= 5;
     const EdmdPart pp = A.p[p];
     const double gvr = G ? uni(A.vr[p]) : 0.0;
     PathCache<ML> P;
@@ -2427,7 +2475,33 @@ __device__ __forceinline__ void ev_single(const EdmdConst& C, const AR& A, int l
     vx = uni(vx);
     vy = uni(vy);
 
-    if (wall) {
+    if (wall && G) {
+        /* doTheWallGrow, EDMD.c:3333-3382: a collision, no momentum to the
+         * wall; the normal velocity turned away with 2 vr added (for y,
+         * which way by the disk's height) */
+        R.ncol++;
+        if (d == 5) {
+            const double sg = vx < 0 ? -1.0 : (vx > 0 ? 1.0 : 0.0);
+            vx = -sg * (fabs(vx) + 2 * gvr);
+        } else if (y > C.Ly - 3) {
+            vy = -vy - 2 * gvr;
+        } else {
+            vy = -vy + 2 * gvr;
+        }
+        vx = uni(vx);
+        vy = uni(vy);
+        if (lane == 0) {
+            const auto q = A.p + p;
+            q->x = x;
+            q->y = y;
+            q->tl = tl;
+            q->vx = vx;
+            q->vy = vy;
+            q->rad = r;
+            q->coll = pp.coll + 1;
+        }
+        WAVE_SYNC();
+    } else if (wall) {
         R.ncol++;
         const double m = uni(pp.m);
         double dp;
@@ -2586,7 +2660,7 @@ __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRu
                 ev_thermo(C, A, lane, R);
             } else if (sp == EDMD_EV_GROWSTOP) {
                 if (G) {
-                    flag_overlap(R, ev_stop_grow(C, A, lane, R), 0);
+                    flag_overlap(R, ev_stop_grow<W>(C, A, lane, R), 0);
                     R.ncol = 0; /* stopGrow resets ncol and ncross, EDMD.c:4745-4746 */
                     ncross = 0;
                 }
@@ -2608,7 +2682,7 @@ __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRu
             last_type = EDMD_EV_COLLISION;
         } else {
             /* a crossing (code -1..-4) or a wall hit (-5 - xy) */
-            const bool wall = W && !G && aux <= EDMD_WALL_CODE(0);
+            const bool wall = W && aux <= EDMD_WALL_CODE(0);
             ev_single<ML, G, W>(C, A, lane, R, p, wall ? 5 + EDMD_WALL_CODE(0) - aux : -aux);
             if (wall) {
                 last_type = EDMD_EV_WALL;
@@ -3166,7 +3240,8 @@ hipError_t edmd_launch_epoch_end(const EdmdSys* d_sys, int nsys, EdmdEpochRow* r
 
 hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, int grow, int walls,
                            EdmdRunArgs ra, hipStream_t s) {
-    if (grow) launch_run<true, false>(d_sys, nsys, max_nlev, ra, s);
+    if (grow && walls) launch_run<true, true>(d_sys, nsys, max_nlev, ra, s);
+    else if (grow) launch_run<true, false>(d_sys, nsys, max_nlev, ra, s);
     else if (walls) launch_run<false, true>(d_sys, nsys, max_nlev, ra, s);
     else launch_run<false, false>(d_sys, nsys, max_nlev, ra, s);
     return hipGetLastError();
@@ -3176,10 +3251,10 @@ hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, int gro
  * disks, far more than one workgroup's LDS holds). */
 static const void* lds_kernel(int max_nlev, int grow, int walls) {
     if (max_nlev <= 1)
-        return grow ? (const void*)k_run_lds<1, true, false>
+        return grow ? (walls ? (const void*)k_run_lds<1, true, true> : (const void*)k_run_lds<1, true, false>)
                     : walls ? (const void*)k_run_lds<1, false, true> : (const void*)k_run_lds<1, false, false>;
     if (max_nlev == 2)
-        return grow ? (const void*)k_run_lds<2, true, false>
+        return grow ? (walls ? (const void*)k_run_lds<2, true, true> : (const void*)k_run_lds<2, true, false>)
                     : walls ? (const void*)k_run_lds<2, false, true> : (const void*)k_run_lds<2, false, false>;
     return nullptr;
 }
@@ -3209,7 +3284,8 @@ static void launch_lds(const EdmdSys* d_sys, int nsys, int max_nlev, size_t dyn_
 hipError_t edmd_launch_run_lds(const EdmdSys* d_sys, int nsys, int max_nlev, int grow, int walls,
                                size_t dyn_bytes, EdmdRunArgs ra, hipStream_t s) {
     if (max_nlev > 2) return hipErrorInvalidValue;
-    if (grow) launch_lds<true, false>(d_sys, nsys, max_nlev, dyn_bytes, ra, s);
+    if (grow && walls) launch_lds<true, true>(d_sys, nsys, max_nlev, dyn_bytes, ra, s);
+    else if (grow) launch_lds<true, false>(d_sys, nsys, max_nlev, dyn_bytes, ra, s);
     else if (walls) launch_lds<false, true>(d_sys, nsys, max_nlev, dyn_bytes, ra, s);
     else launch_lds<false, false>(d_sys, nsys, max_nlev, dyn_bytes, ra, s);
     return hipGetLastError();

@@ -85,8 +85,11 @@ typedef struct edmd_system_desc {
      * x = 0, Lx and/or y = 0, Ly, no minimum image across them
      * (EDMD.c:5896-5913).  A hit (EDMD.c:3384-3430) reverses the normal
      * velocity times res_wall (resW, default 1) and scales the tangential
-     * one by res_tan (resT, default 0), and counts as a collision.  Not with
-     * a growth start (vr > 0) or taping. */
+     * one by res_tan (resT, default 0), and counts as a collision.  During
+     * a growth start the wall slot closes the disk's surface at v -/+ vr on
+     * the wall (EDMD.c:3118-3153) and a hit turns the normal velocity away
+     * with 2 vr added, no momentum counted (doTheWallGrow, EDMD.c:
+     * 3333-3382); after GROWSTOP the walls above apply.  Not with taping. */
     int wall_x, wall_y;
     double res_wall, res_tan;
     /* Uniform field along y (addField / field of the graphics build,

@@ -408,11 +408,47 @@ static double coll_time_grow(orc_ctx* s, int i, int j) {
 }
 
 static void predict_collision_grow(orc_ctx* s, int i, double* t_abs, int* partner) {
-    /* collisionEventGrow, EDMD.c:3104-3230 without walls: the minimum
-     * starts at 1e7, not never, and is scanned as in the Normal version. */
+    /* collisionEventGrow, EDMD.c:3104-3230 (hard walls, no circular or mid
+     * wall): the minimum starts at 1e7, not never, and is scanned as in the
+     * Normal version. */
     int best = 0;
     double dt = 10000000;
     int X = s->cx[i], Y = s->cy[i];
+    /* walls first (EDMD.c:3118-3153): the surface of a disk in a boundary
+     * cell closes on the wall at v -/+ vr; no logTime.  The x test sets
+     * xy = 0 whenever it applies, as in the Normal version. */
+    int wall = 0, xy = 2;
+    const double vr = s->vr[i];
+    if (s->prm.wall_y) {
+        double vyp = s->vy[i] + vr, vym = s->vy[i] - vr;
+        if (Y == 0 && vym < 0) {
+            dt = -(s->y[i] - s->rad[i]) / vym;
+            wall = 1;
+            xy = 1;
+        } else if (Y == s->ncy - 1 && vyp > 0) {
+            dt = (s->Ly - s->y[i] - s->rad[i]) / vyp;
+            wall = 1;
+            xy = 1;
+        }
+    }
+    if (s->prm.wall_x) {
+        double vxp = s->vx[i] + vr, vxm = s->vx[i] - vr;
+        if (X == 0 && vxm < 0) {
+            double dtw = -(s->x[i] - s->rad[i]) / vxm;
+            if (dt > dtw) {
+                wall = 1;
+                dt = dtw;
+            }
+            xy = 0;
+        } else if (X == s->ncx - 1 && vxp > 0) {
+            double dtw = (s->Lx - s->x[i] - s->rad[i]) / vxp;
+            if (dt > dtw) {
+                wall = 1;
+                dt = dtw;
+            }
+            xy = 0;
+        }
+    }
     for (int j = -1; j <= 1; j++) {
         for (int k = -1; k <= 1; k++) {
             int c = wrap_cell(Y + j, s->ncy) * s->ncx + wrap_cell(X + k, s->ncx);
@@ -422,12 +458,13 @@ static void predict_collision_grow(orc_ctx* s, int i, double* t_abs, int* partne
                 if (dt > dtt) {
                     best = p;
                     dt = dtt;
+                    wall = 0;
                 }
             }
         }
     }
     *t_abs = s->t + dt;
-    *partner = best;
+    *partner = wall ? ORC_WALL_PARTNER(xy) : best;
 }
 
 static void predict_collision(orc_ctx* s, int i, double* t_abs, int* partner) {
@@ -734,7 +771,35 @@ static void do_crossing(orc_ctx* s, const ev_node* ev) {
     schedule_collision(s, i);
 }
 
+static double sign_of(double x) { return (x < 0) ? -1 : (x > 0); }
+
+static void do_wall_grow(orc_ctx* s, const ev_node* ev) {
+    /* doTheWallGrow, EDMD.c:3333-3382 (hard walls): counted as a collision,
+     * no momentum to the wall; the normal velocity turned away from the
+     * wall with the growth rate added twice, which way for y decided by
+     * the disk's height (y > Ly - 3: the top wall). */
+    int i = ev->i, xy = ev->j;
+    s->ncol++;
+    free_fly_grow(s, i);
+    const double vr = s->vr[i];
+    if (xy == 0) {
+        s->vx[i] = -sign_of(s->vx[i]) * (fabs(s->vx[i]) + 2 * vr);
+    } else if (s->y[i] > s->Ly - 3) {
+        s->vy[i] = -s->vy[i] - 2 * vr;
+    } else {
+        s->vy[i] = -s->vy[i] + 2 * vr;
+    }
+    s->coll[i]++;
+    queue_remove(s, i);
+    schedule_crossing(s, i);
+    schedule_collision(s, i);
+}
+
 static void do_wall(orc_ctx* s, const ev_node* ev) {
+    if (s->growing) {
+        do_wall_grow(s, ev);
+        return;
+    }
     /* doTheWallNormal, EDMD.c:3384-3430, 3454-3460 (no mid wall, thermal
      * wall or shear wall): counted as a collision, momentum to the wall,
      * normal velocity reversed times resW, tangential scaled by resT, then
@@ -933,12 +998,12 @@ orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* 
     s->n = n;
     if (prm) s->prm = *prm;
     else orc_default_params(&s->prm);
-    /* walls only with the Normal dispatch (doTheWallGrow is not restated) */
+    /* the field and damping only with the Normal dispatch */
     if (s->prm.damping && !(s->prm.gamma > 0)) {
         free(s);
         return NULL;
     }
-    if ((s->prm.wall_x || s->prm.wall_y || s->prm.add_field || s->prm.damping) && s->prm.vr > 0) {
+    if ((s->prm.add_field || s->prm.damping) && s->prm.vr > 0) {
         free(s);
         return NULL;
     }

@@ -105,7 +105,10 @@ def test_cli_default_start_is_the_growth_start(gpu, tmp_path):
     (30% of radius 0.4, the rest radius 1) at random centres, growing at the
     rate optimizeGrowConstant picks (0.1 at phi = 0.6, so GROWSTOP at t = 10);
     screenshot times below 1/vr move by 1/vr, so the one frame is at t = 11.
-    It equals the library's run from the same seeded start."""
+    It equals the library's run from the disks particlesInit makes for the
+    same command line (the reference's MT19937 start)."""
+    import ctypes as C
+    from test_start_reference import start
     import os
     import subprocess
     repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -118,7 +121,17 @@ def test_cli_default_start_is_the_growth_start(gpu, tmp_path):
     assert len(dumps) == 1
     frame = F.read_dump(str(tmp_path / "dump" / dumps[0]))
     assert gpu.growth_rate(0.6) == 0.1
-    st = gpu.init_random(300, 0.6, rad=1.0, sizeratio=0.4, fsmall=0.3, seed=1, e_per=0.05)
+    L = gpu.lib()
+    assert start(L, ["-N", "300", "-p", "0.6", "--no-output"]) == 0
+    st = {k: np.zeros(300) for k in ("x", "y", "vx", "vy", "rad", "m", "vr")}
+    L.edmd_compat_particles.argtypes = [C.c_void_p] * 7
+    assert L.edmd_compat_particles(*[st[k].ctypes.data for k in
+                                     ("x", "y", "vx", "vy", "rad", "m", "vr")]) == 0
+    st["Lx"] = C.c_double.in_dll(L, "Lx").value
+    st["Ly"] = C.c_double.in_dll(L, "Ly").value
+    L.freeArrays()
+    np.testing.assert_array_equal(st["vr"], 0.1 * st["rad"])
+    del st["vr"]
     s = gpu.System(st, vr=0.1, first_screen=1.0, dtime=1.0, first_thermo=1000.0,
                    dtime_thermo=100.0, einit=1.0)
     reason, kind = s.run_until_kind(1 << 2)  # stop right after the screenshot

@@ -143,8 +143,6 @@ def test_wall_batch_mixed_residencies_and_shapes(gpu, oracle):
 def test_wall_arguments_are_checked(gpu):
     st = F.lattice_state(100, 0.4, seed=5, lattice="fit")
     with pytest.raises(gpu.EdmdError):
-        gpu.System(st, wall_x=1, vr=0.1)
-    with pytest.raises(gpu.EdmdError):
         gpu.System(st, wall_y=2)
     b = gpu.Batch([st], wall_y=1)
     with pytest.raises(gpu.EdmdError):
@@ -256,3 +254,42 @@ def test_damping_arguments_are_checked(gpu):
     b = gpu.Batch([st], damping=1)
     with pytest.raises(gpu.EdmdError):
         b.deriv_begin(checkpoint_every=256)
+
+
+# ---- hard walls under a growth start (collisionEventGrow, doTheWallGrow)
+
+GROW_WALL_CASES = [
+    (300, 0.55, 71, 0.0, 0.1, dict(wall_x=1)),
+    (350, 0.6, 72, 0.1, 0.05, dict(wall_y=1)),
+    (400, 0.65, 73, 0.0, 0.1, dict(wall_x=1, wall_y=1, res_tan=1.0)),
+]
+
+
+@pytest.mark.parametrize("n,phi,seed,poly,vr,kw", GROW_WALL_CASES)
+def test_walls_under_growth_match_oracle(gpu, oracle, n, phi, seed, poly, vr, kw):
+    """Disks growing from radius 0 in a box with walls: wall slots with the
+    surface closing at v -/+ vr, wall hits that add 2 vr, then GROWSTOP and
+    the Normal walls (resW, resT, momentum to the walls): traces (mirrored
+    pairs unordered, as for every growth start) and states, radii included,
+    bit for bit against the oracle, from HBM and from LDS."""
+    st = F.growth_state(n, phi, seed, poly=poly)
+    nev = 40000
+    o = oracle.Oracle(st, vr=vr, **kw, **QUIET)
+    code, tr = o.run(nev)
+    assert code == 0
+    assert np.sum(tr["type"] == WALL) > 20
+    assert np.sum(tr["type"] == F.GROWSTOP) == 1
+    grown = np.nonzero(tr["type"] == F.GROWSTOP)[0][0]
+    assert np.sum(tr["type"][:grown] == WALL) > 5 and np.sum(tr["type"][grown:] == WALL) > 5
+    so = o.state()
+    for mode in (gpu.RES_HBM, gpu.RES_LDS):
+        b = gpu.Batch([st], trace_capacity=nev, vr=vr, **kw, **QUIET)
+        b.set_residency(mode)
+        b.run(max_events=nev)
+        a, c = F.canonical_trace(b.trace(0)), F.canonical_trace(tr)
+        for k in ("type", "i", "j", "valid"):
+            np.testing.assert_array_equal(a[k], c[k], err_msg=k)
+        np.testing.assert_array_equal(bits(a["t"]), bits(c["t"]))
+        assert_state_equal(b.state(0), so)
+        np.testing.assert_array_equal(bits(b.radii(0)), bits(o.radii()))
+        assert b.observables(0)["ncol"] == o.observables()["ncol"]

@@ -133,10 +133,12 @@ def test_wall_pressure_ideal_gas_limit(oracle):
     assert all(r["pwall"] == 0 for r in o.thermo_rows())
 
 
-def test_walls_with_growth_are_rejected(oracle):
+def test_field_or_damping_with_growth_are_rejected(oracle):
     st = F.lattice_state(50, 0.3, seed=1, lattice="fit")
     with pytest.raises(ValueError):
-        oracle.Oracle(st, wall_x=1, vr=0.1)
+        oracle.Oracle(st, add_field=1, vr=0.1)
+    with pytest.raises(ValueError):
+        oracle.Oracle(st, damping=1, vr=0.1)
 
 
 # ---- the uniform field along y (addField, EDMD.c:2428-2442, 2664-2666,
@@ -285,3 +287,35 @@ def test_field_with_damping_closed_form(oracle):
     assert s["vy"][0] == pytest.approx(f * t1, rel=1e-13)
     # the resting disk falls under the field, undamped
     assert s["vy"][1] == pytest.approx(f * t1, rel=1e-13)
+
+
+# ---- hard walls under a growth start (collisionEventGrow / doTheWallGrow)
+
+def test_walls_under_growth_closed_form(oracle):
+    """A growing disk's surface closes on a wall at |v -/+ vr| (EDMD.c:
+    3118-3153): from x = 1.2 at vx = -0.3 with vr = 0.1 (target 0.5 at rate
+    0.2) the left wall comes at 1.2 / 0.4 = 3; the top wall from y = 18.8
+    at vy = 0.2 at 1.2 / 0.3 = 4; the bottom wall from y = 0.6 at vy = -0.1
+    at 0.6 / 0.2 = 3.  A hit (EDMD.c:3333-3382) counts as a collision and
+    turns the normal velocity away with 2 vr added: vx = 0.3 + 0.2; at the
+    top (y > Ly - 3) vy = -0.2 - 0.2; at the bottom vy = 0.1 + 0.2.  No
+    momentum goes to the walls during growth."""
+    st = disks([(1.2, 9.0), (9.0, 18.8), (15.0, 0.6)],
+               [(-0.3, 0.0), (0.0, 0.2), (0.0, -0.1)])
+    o = oracle.Oracle(st, wall_x=1, wall_y=1, vr=0.2, **QUIET)
+    for i, (tw, xy) in enumerate([(3.0, 0), (4.0, 1), (3.0, 1)]):
+        t, p = o.predict_collision(i)
+        assert p == -5 - xy and t == pytest.approx(tw, rel=1e-14), (i, t, p)
+    code, tr = o.run(3)
+    assert code == 0
+    assert list(tr["type"]) == [WALL] * 3
+    assert sorted(tr["i"][:2]) == [0, 2] and tr["i"][2] == 1
+    oracle.lib().orc_sync_all(o.h, o.t)
+    s = o.state()
+    assert s["vx"][0] == pytest.approx(0.5, rel=1e-15) and s["vy"][0] == 0.0
+    assert s["vy"][1] == pytest.approx(-0.4, rel=1e-15)
+    assert s["vy"][2] == pytest.approx(0.3, rel=1e-15)
+    ob = o.observables()
+    assert ob["ncol"] == 3
+    # the radii grew with the flights: 0.1 per unit time up to t = 4
+    np.testing.assert_allclose(o.radii(), 0.4, rtol=1e-14)
