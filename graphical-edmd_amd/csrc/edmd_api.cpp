@@ -241,14 +241,10 @@ static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
         return fail(EDMD_ERR_ARG, "system %d: add_field must be 0 or 1", idx);
     if (d->add_field && !(d->field < 0))
         return fail(EDMD_ERR_ARG, "system %d: the field must be negative", idx);
-    if (d->add_field && d->vr > 0)
-        return fail(EDMD_ERR_ARG, "system %d: a field needs a start without growth", idx);
     if (d->damping != 0 && d->damping != 1)
         return fail(EDMD_ERR_ARG, "system %d: damping must be 0 or 1", idx);
     if (d->damping && !(d->gamma > 0))
         return fail(EDMD_ERR_ARG, "system %d: the damping rate must be positive", idx);
-    if (d->damping && d->vr > 0)
-        return fail(EDMD_ERR_ARG, "system %d: damping goes without growth", idx);
     c.field_on = d->add_field;
     c.field = d->field;
     c.damping = d->damping;

@@ -1286,7 +1286,8 @@ __device__ __forceinline__ bool sweep_one(const EdmdConst& C, const AR& A, doubl
     const EdmdPart o = A.p[p];
     double tc;
     int dir;
-    cross_time<W>(C, t, o.x, o.y, o.vx, o.vy, o.cx, o.cy, tc, dir);
+    /* crossingEventGrow while growing: no field, no logTime */
+    cross_time<W && !G>(C, t, o.x, o.y, o.vx, o.vy, o.cx, o.cy, tc, dir);
     bool ov = false;
     int partner;
     double dt = predict_seq<G, W>(C, A, t, p, o, partner, ov);
@@ -2595,7 +2596,7 @@ __device__ __forceinline__ void ev_single(const EdmdConst& C, const AR& A, int l
     flag_overlap(R, ov, p);
     double tc;
     int dir;
-    cross_time<W>(C, t, x, y, vx, vy, cx, cy, tc, dir);
+    cross_time<W && !G>(C, t, x, y, vx, vy, cx, cy, tc, dir);
     const double tk = t + dt;
     /* crossingEvent(i), then collisionEvent(i), EDMD.c:2587-2589, 3458-3459 */
     const int64_t rx = take_rank(C, R, tc);

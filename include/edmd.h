@@ -97,7 +97,9 @@ typedef struct edmd_system_desc {
      * every neighbour flown to the current time when its collision time is
      * taken (EDMD.c:2664-2666), crossings and y-wall hits solving the
      * quadratic (EDMD.c:2428-2442, 2846-2860).  field < 0 (default -0.02).
-     * Not with a growth start or taping. */
+     * During a growth start the Grow dispatch ignores it (straight flights
+     * and crossings, EDMD.c:2343-2400, 4992-5007) until GROWSTOP.  Not with
+     * taping. */
     int add_field;
     double field;
     /* Damping (damping / gamm of the graphics build, EDMD.c:294, 382):
@@ -110,7 +112,10 @@ typedef struct edmd_system_desc {
      * With the field as well, the flight is the field's alone (its branch
      * of freeFlyNormal comes first, EDMD.c:4957-4975) while logTime still
      * converts the straight-line times of x crossings, x walls and
-     * collisions.  Not with a growth start or taping. */
+     * collisions; that pairing of damped times with undamped flights can
+     * end in overlaps and NaN states, as in the reference.  During a growth
+     * start damping waits for GROWSTOP, as the field does.  Not with
+     * taping. */
     int damping;
     double gamma;
 } edmd_system_desc;

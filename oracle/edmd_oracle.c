@@ -546,19 +546,27 @@ static void predict_collision(orc_ctx* s, int i, double* t_abs, int* partner) {
     *partner = wall ? ORC_WALL_PARTNER(xy) : best;
 }
 
+/* logTime where the crossing uses it: crossingEventGrow (EDMD.c:2343-2400)
+ * takes the straight-line times as they are. */
+static double cross_log_time(const orc_ctx* s, double t_lin) {
+    return s->growing ? t_lin : log_time(s, t_lin);
+}
+
 static void predict_crossing(const orc_ctx* s, int i, double* t_abs, int* dir) {
-    /* crossingEventNormal, EDMD.c:2405-2482: ties go to y. */
+    /* crossingEventNormal, EDMD.c:2405-2482, or crossingEventGrow while
+     * growing (no field, no damping): ties go to y. */
     double tx, ty;
     int xx, yy;
     if (s->vx[i] < 0) {
-        tx = log_time(s, wrap_inside(s->cx[i] * s->csx - s->x[i], s->halfLx, s->Lx) / s->vx[i]);
+        tx = cross_log_time(s, wrap_inside(s->cx[i] * s->csx - s->x[i], s->halfLx, s->Lx) /
+                                   s->vx[i]);
         xx = 1;
     } else {
-        tx = log_time(s, wrap_inside((1 + s->cx[i]) * s->csx - s->x[i], s->halfLx, s->Lx) /
-                             s->vx[i]);
+        tx = cross_log_time(s, wrap_inside((1 + s->cx[i]) * s->csx - s->x[i], s->halfLx, s->Lx) /
+                                   s->vx[i]);
         xx = 2;
     }
-    if (s->prm.add_field) {
+    if (s->prm.add_field && !s->growing) {
         /* EDMD.c:2428-2442: up through the top edge when fast enough,
          * else down through the bottom edge */
         double f = s->prm.field, vy = s->vy[i];
@@ -572,11 +580,12 @@ static void predict_crossing(const orc_ctx* s, int i, double* t_abs, int* dir) {
             yy = 3;
         }
     } else if (s->vy[i] < 0) {
-        ty = log_time(s, wrap_inside(s->cy[i] * s->csy - s->y[i], s->halfLy, s->Ly) / s->vy[i]);
+        ty = cross_log_time(s, wrap_inside(s->cy[i] * s->csy - s->y[i], s->halfLy, s->Ly) /
+                                   s->vy[i]);
         yy = 3;
     } else {
-        ty = log_time(s, wrap_inside((1 + s->cy[i]) * s->csy - s->y[i], s->halfLy, s->Ly) /
-                             s->vy[i]);
+        ty = cross_log_time(s, wrap_inside((1 + s->cy[i]) * s->csy - s->y[i], s->halfLy, s->Ly) /
+                                   s->vy[i]);
         yy = 4;
     }
     if (tx < ty) {
@@ -998,12 +1007,7 @@ orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* 
     s->n = n;
     if (prm) s->prm = *prm;
     else orc_default_params(&s->prm);
-    /* the field and damping only with the Normal dispatch */
     if (s->prm.damping && !(s->prm.gamma > 0)) {
-        free(s);
-        return NULL;
-    }
-    if ((s->prm.add_field || s->prm.damping) && s->prm.vr > 0) {
         free(s);
         return NULL;
     }

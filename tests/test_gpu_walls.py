@@ -202,8 +202,6 @@ def test_field_arguments_are_checked(gpu):
     st = F.lattice_state(100, 0.4, seed=5, lattice="fit")
     with pytest.raises(gpu.EdmdError):
         gpu.System(st, add_field=1, field=0.1)
-    with pytest.raises(gpu.EdmdError):
-        gpu.System(st, add_field=1, vr=0.1)
     b = gpu.Batch([st], add_field=1)
     with pytest.raises(gpu.EdmdError):
         b.deriv_begin(checkpoint_every=256)
@@ -233,6 +231,9 @@ def test_damping_traces_match_oracle(gpu, oracle, n, phi, seed, kw):
     code, tr = o.run(nev)
     assert code == 0
     so = o.state()
+    # the field with damping pairs damped times with undamped flights, which
+    # can end in overlaps and NaN states; these runs stay finite
+    assert all(np.all(np.isfinite(so[k])) for k in ("x", "y", "vx", "vy"))
     for mode, shape in ((gpu.RES_HBM, 1), (gpu.RES_LDS, 6)):
         b = gpu.Batch([st], trace_capacity=nev, **kw, **QUIET)
         ev = b.events(0)
@@ -249,8 +250,6 @@ def test_damping_arguments_are_checked(gpu):
     st = F.lattice_state(100, 0.4, seed=5, lattice="fit")
     with pytest.raises(gpu.EdmdError):
         gpu.System(st, damping=1, gamma=0.0)
-    with pytest.raises(gpu.EdmdError):
-        gpu.System(st, damping=1, vr=0.1)
     b = gpu.Batch([st], damping=1)
     with pytest.raises(gpu.EdmdError):
         b.deriv_begin(checkpoint_every=256)
@@ -262,6 +261,10 @@ GROW_WALL_CASES = [
     (300, 0.55, 71, 0.0, 0.1, dict(wall_x=1)),
     (350, 0.6, 72, 0.1, 0.05, dict(wall_y=1)),
     (400, 0.65, 73, 0.0, 0.1, dict(wall_x=1, wall_y=1, res_tan=1.0)),
+    # the field and damping wait for GROWSTOP (the Grow dispatch ignores them)
+    (350, 0.55, 74, 0.0, 0.1, dict(wall_y=1, res_tan=1.0, add_field=1, field=-0.05)),
+    (300, 0.6, 75, 0.1, 0.1, dict(wall_x=1, wall_y=1, damping=1, gamma=0.02)),
+    (320, 0.5, 76, 0.0, 0.1, dict(wall_x=1, wall_y=1, res_tan=1.0, add_field=1, field=-0.03)),
 ]
 
 

@@ -133,12 +133,36 @@ def test_wall_pressure_ideal_gas_limit(oracle):
     assert all(r["pwall"] == 0 for r in o.thermo_rows())
 
 
-def test_field_or_damping_with_growth_are_rejected(oracle):
-    st = F.lattice_state(50, 0.3, seed=1, lattice="fit")
-    with pytest.raises(ValueError):
-        oracle.Oracle(st, add_field=1, vr=0.1)
-    with pytest.raises(ValueError):
-        oracle.Oracle(st, damping=1, vr=0.1)
+def test_field_and_damping_wait_for_growstop(oracle):
+    """While the disks grow the Grow dispatch ignores the field and damping:
+    straight flights (freeFlyGrow, EDMD.c:4992-5007) and straight crossing
+    times without logTime (crossingEventGrow, EDMD.c:2343-2400).  A disk
+    at y = 5 rising at 0.5 leaves its cell [4, 6) at t = 2, not at the
+    parabola's 2.76 (field -0.1) nor the damped 2.23 (gamma 0.05); after
+    GROWSTOP at t = 10 the same disk gets the Normal times."""
+    st = disks([(9.0, 5.0), (15.0, 15.0)], [(0.0, 0.5), (0.0, 0.0)])
+    for kw in (dict(add_field=1, field=-0.1), dict(damping=1, gamma=0.05)):
+        o = oracle.Oracle(st, vr=0.1, **kw, **QUIET)
+        t, d = o.predict_crossing(0)
+        assert d == 4 and t == pytest.approx(2.0, rel=1e-15), kw
+        code, tr = o.run(1)
+        assert code == 0 and tr["type"][0] == F.CELLCROSS and tr["t"][0] == t
+        oracle.lib().orc_sync_all(o.h, o.t)
+        s = o.state()
+        assert s["vy"][0] == 0.5 and s["y"][0] == pytest.approx(6.0, rel=1e-15)
+    # after GROWSTOP (t = 1 at rate 1): the field's parabola from the state
+    # stopGrow left (velocities rescaled to E/N = 1)
+    f = -0.1
+    o = oracle.Oracle(st, vr=1.0, add_field=1, field=f, **QUIET)
+    code, tr = o.run(1)
+    assert code == 0 and tr["type"][0] == F.GROWSTOP
+    s = o.state()
+    y, vy = s["y"][0], s["vy"][0]
+    assert 4.0 < y < 6.0 and vy > 0
+    t, d = o.predict_crossing(0)
+    assert d == 4
+    assert t - o.t == pytest.approx((-vy + np.sqrt(vy * vy - 2 * f * (y - 6.0))) / f, rel=1e-12)
+    assert abs((t - o.t) - (6.0 - y) / vy) > 1e-3
 
 
 # ---- the uniform field along y (addField, EDMD.c:2428-2442, 2664-2666,
