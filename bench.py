@@ -198,11 +198,12 @@ def tune(b, args, sweep_reps, trial_events):
         rep = b.autotune(sweep_reps=sweep_reps, trial_events=trial_events)
     sh = b.sweep_shape()
     out = {"sweep_tile": [sh["tile_x"], sh["tile_y"]], "sweep_threads": sh["threads"],
-           "lds_resident": b.residency()["lds"]}
+           "lds_resident": b.residency()["lds"], "loop_systems_per_group": b.loop_waves()}
     if rep is not None:
         out["sweep_ms_by_shape"] = [round(x, 5) for x in rep["sweep_ms"]]
         if rep["loop_ms_hbm"] >= 0:
             out["trial_ms_hbm_lds"] = [round(rep["loop_ms_hbm"], 3), round(rep["loop_ms_lds"], 3)]
+            out["trial_ms_hbm_2_4_per_group"] = [round(x, 3) for x in rep["loop_ms_packed"]]
     return out
 
 

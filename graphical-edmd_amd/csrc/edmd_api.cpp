@@ -137,6 +137,10 @@ struct edmd_batch {
     bool use_lds = false;
     /* some system has walls, the field or damping: the kernels' variants for them */
     bool any_walls = false;
+    /* systems per workgroup of the HBM-resident loop (1, 2 or 4; more than
+     * one only for calendars of depth <= 2 without walls, field or damping,
+     * and not for the growing systems' launch) */
+    int loop_wpb = 1;
 };
 
 struct edmd_ctx {
@@ -621,13 +625,36 @@ static int choose_residency(edmd_batch* b) {
     return EDMD_OK;
 }
 
+/* Whether the HBM-resident loop may hold several systems per workgroup. */
+static bool packable(const edmd_batch* b) { return !b->any_walls && b->max_nlev <= 2; }
+
 /* One launch of the event loop over every system (grow: the growing ones). */
 static hipError_t launch_loop(edmd_batch* b, int grow, const EdmdRunArgs& ra) {
     const int walls = b->any_walls ? 1 : 0;
     if (b->use_lds)
         return edmd_launch_run_lds(b->d_sys, b->nsys, b->max_nlev, grow, walls,
                                    b->lds_dyn[grow ? 1 : 0], ra, b->stream);
-    return edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, grow, walls, ra, b->stream);
+    const int wpb = (!grow && packable(b)) ? b->loop_wpb : 1;
+    return edmd_launch_run(b->d_sys, b->nsys, b->max_nlev, grow, walls, wpb, ra, b->stream);
+}
+
+extern "C" int edmd_batch_set_loop_waves(edmd_batch* b, int systems_per_group) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    if (systems_per_group != 1 && systems_per_group != 2 && systems_per_group != 4)
+        return fail(EDMD_ERR_ARG, "systems per workgroup must be 1, 2 or 4 (got %d)",
+                    systems_per_group);
+    if (systems_per_group > 1 && !packable(b))
+        return fail(EDMD_ERR_ARG,
+                    "several systems per workgroup need calendars of depth <= 2 and no walls, "
+                    "field or damping");
+    b->loop_wpb = systems_per_group;
+    return EDMD_OK;
+}
+
+extern "C" int edmd_batch_get_loop_waves(edmd_batch* b, int* systems_per_group) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    if (systems_per_group) *systems_per_group = b->loop_wpb;
+    return EDMD_OK;
 }
 
 extern "C" int edmd_batch_set_residency(edmd_batch* b, int mode) {
@@ -688,7 +715,7 @@ extern "C" int edmd_batch_autotune(edmd_batch* b, int sweep_reps, int64_t trial_
     edmd_tune_report r;
     memset(&r, 0, sizeof r);
     for (int v = 0; v < EDMD_TUNE_SWEEP_SHAPES; v++) r.sweep_ms[v] = -1;
-    r.loop_ms_hbm = r.loop_ms_lds = -1;
+    r.loop_ms_hbm = r.loop_ms_lds = r.loop_ms_packed[0] = r.loop_ms_packed[1] = -1;
     HIPCHK(hipStreamSynchronize(b->stream));
     void* snap = nullptr;
     if (hipMalloc(&snap, ckpt_bytes(b)) != hipSuccess)
@@ -729,15 +756,19 @@ extern "C" int edmd_batch_autotune(edmd_batch* b, int sweep_reps, int64_t trial_
         if (e != hipSuccess) rc = fail(EDMD_ERR_HIP, "sweep trial: %s", hipGetErrorString(e));
     }
     if (!rc) rc = ckpt_copy(b, snap, false);
-    if (!rc && trial_events > 0 && b->lds_fit) {
+    if (!rc && trial_events > 0 && (b->lds_fit || packable(b))) {
         EdmdRunArgs ra;
         memset(&ra, 0, sizeof ra);
         ra.max_events = INT64_MAX; /* the per-system trial bound applies */
         ra.max_coll = ~(uint64_t)0;
         ra.tmax = 1e300;
-        double t[2] = {0, 0};
-        for (int m = 0; m < 2 && !rc; m++) {
-            b->use_lds = m == 1;
+        /* trials: HBM with 1, 2 and 4 systems per workgroup, then LDS */
+        const int wpbs[4] = {1, 2, 4, 1};
+        double t[4] = {-1, -1, -1, -1};
+        for (int m = 0; m < 4 && !rc; m++) {
+            if (m == 3 ? !b->lds_fit : (m > 0 && !packable(b))) continue;
+            b->use_lds = m == 3;
+            b->loop_wpb = wpbs[m];
             e = edmd_launch_epoch_begin(b->d_sys, b->nsys, trial_events, ~(uint64_t)0, b->stream);
             if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
             if (e == hipSuccess && b->any_grow) e = launch_loop(b, 1, ra);
@@ -751,10 +782,16 @@ extern "C" int edmd_batch_autotune(edmd_batch* b, int sweep_reps, int64_t trial_
             int rc2 = ckpt_copy(b, snap, false);
             if (!rc) rc = rc2;
         }
+        int best = 0;
+        for (int m = 1; m < 4; m++)
+            if (t[m] > 0 && t[m] < t[best]) best = m;
+        b->loop_wpb = wpbs[best];
         if (!rc) {
             r.loop_ms_hbm = t[0];
-            r.loop_ms_lds = t[1];
-            b->residency = t[1] < t[0] ? EDMD_RES_LDS : EDMD_RES_HBM;
+            r.loop_ms_packed[0] = t[1];
+            r.loop_ms_packed[1] = t[2];
+            r.loop_ms_lds = t[3];
+            b->residency = best == 3 ? EDMD_RES_LDS : EDMD_RES_HBM;
         }
         int rc3 = choose_residency(b);
         if (!rc) rc = rc3;
@@ -766,6 +803,7 @@ extern "C" int edmd_batch_autotune(edmd_batch* b, int sweep_reps, int64_t trial_
     (void)hipFree(snap);
     r.sweep_shape = b->sweep_variant;
     r.in_lds = b->use_lds ? 1 : 0;
+    r.loop_systems_per_group = b->loop_wpb;
     if (out) *out = r;
     return rc;
 }

@@ -26,7 +26,7 @@ hipError_t edmd_launch_epoch_begin(const EdmdSys* d_sys, int nsys, int64_t ev_in
 hipError_t edmd_launch_epoch_end(const EdmdSys* d_sys, int nsys, EdmdEpochRow* rows,
                                  hipStream_t s);
 hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, int grow, int walls,
-                           EdmdRunArgs ra,
+                           int wpb, EdmdRunArgs ra,
                            hipStream_t s);
 /* Event loop with each system's arrays resident in LDS (calendar depth <= 2):
  * dyn_bytes of dynamic LDS per workgroup, at least edmd_lds_layout's total

@@ -1883,7 +1883,7 @@ __device__ __noinline__ void refile_specials(const EdmdConst& C, EdmdState* st, 
         const int64_t r = (w == win) ? ctr : rank_of(tk, paul, C.dtp, ctr);
         ctr++;
         WAVE_SYNC();
-        if (threadIdx.x == 0) {
+        if ((threadIdx.x & 63) == 0) {
             *ws[k] = w;
             *rs[k] = r;
         }
@@ -2615,13 +2615,15 @@ __device__ __forceinline__ void ev_single(const EdmdConst& C, const AR& A, int l
  * calendar depth of every system in the launch.  A holds the system's arrays
  * in HBM (EdmdArrays) or in LDS (EdmdArraysL); the caller has checked that
  * the system's growth phase matches G. */
-template <int ML, bool G, bool W, class AR>
+template <int ML, bool G, bool W, int WPB = 1, class AR>
 __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRunArgs ra) {
-    const int lane = threadIdx.x;
+    /* WPB wavefronts per workgroup, each looping over its own system with
+     * its own LDS queue state */
+    const int lane = threadIdx.x & 63;
     EdmdState* st = A.st;
-    __shared__ QueueState Qs;
+    __shared__ QueueState Qs[WPB];
     LoopRegs R;
-    R.q = &Qs;
+    R.q = &Qs[WPB > 1 ? uni((int)(threadIdx.x >> 6)) : 0];
     R.t = uni(st->t);
     R.ncol = st->ncol;
     R.nevents = st->nevents;
@@ -2717,13 +2719,17 @@ __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRu
 /* Systems of up to 64^2 disks (calendar depth <= 2): bounded to 128 VGPRs so
  * that four waves share each SIMD and a batch of 4096 replicas is resident
  * at once on 256 CUs. */
-template <int ML, bool G, bool W>
-__global__ __launch_bounds__(64, 4) void k_run_small(const EdmdSys* __restrict__ systems,
-                                                     EdmdRunArgs ra) {
-    const EdmdSys& S = systems[blockIdx.x];
+template <int ML, bool G, bool W, int WPB = 1>
+__global__ __launch_bounds__(64 * WPB, 4) void k_run_small(const EdmdSys* __restrict__ systems,
+                                                           EdmdRunArgs ra) {
+    /* WPB systems per workgroup, one per wavefront (the launch's systems
+     * per workgroup are tuned; the last workgroup may hold fewer) */
+    const int sys = WPB > 1 ? (int)blockIdx.x * WPB + uni((int)(threadIdx.x >> 6)) : (int)blockIdx.x;
+    if (WPB > 1 && sys >= ra.nsys) return;
+    const EdmdSys& S = systems[sys];
     /* G launches run the systems that are growing, the others the rest */
     if ((uni(S.a.st->growing) != 0) != G) return;
-    run_loop<ML, G, W>(S.c, S.a, ra);
+    run_loop<ML, G, W, WPB>(S.c, S.a, ra);
 }
 
 /* Deeper calendars (large systems, typically one per launch): no register
@@ -3192,6 +3198,7 @@ hipError_t edmd_launch_sync_all(const EdmdSys* d_sys, int nsys, int max_n, hipSt
 /* W: some system of the launch has walls (never together with growth). */
 template <bool G, bool W>
 static void launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, EdmdRunArgs ra, hipStream_t s) {
+    ra.nsys = nsys;
     if (max_nlev <= 1) hipLaunchKernelGGL((k_run_small<1, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
     else if (max_nlev == 2) hipLaunchKernelGGL((k_run_small<2, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
     else if (max_nlev <= 4) hipLaunchKernelGGL((k_run<4, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
@@ -3239,9 +3246,24 @@ hipError_t edmd_launch_epoch_end(const EdmdSys* d_sys, int nsys, EdmdEpochRow* r
     return hipGetLastError();
 }
 
+/* Systems of calendar depth <= 2 without growth, walls, field or damping,
+ * wpb (2 or 4) systems per workgroup. */
+template <int WPB>
+static void launch_run_packed(const EdmdSys* d_sys, int nsys, int max_nlev, EdmdRunArgs ra,
+                              hipStream_t s) {
+    ra.nsys = nsys;
+    const dim3 grid((nsys + WPB - 1) / WPB), block(64 * WPB);
+    if (max_nlev <= 1) hipLaunchKernelGGL((k_run_small<1, false, false, WPB>), grid, block, 0, s, d_sys, ra);
+    else hipLaunchKernelGGL((k_run_small<2, false, false, WPB>), grid, block, 0, s, d_sys, ra);
+}
+
 hipError_t edmd_launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, int grow, int walls,
-                           EdmdRunArgs ra, hipStream_t s) {
-    if (grow && walls) launch_run<true, true>(d_sys, nsys, max_nlev, ra, s);
+                           int wpb, EdmdRunArgs ra, hipStream_t s) {
+    if (wpb != 1 && (grow || walls || max_nlev > 2 || (wpb != 2 && wpb != 4)))
+        return hipErrorInvalidValue;
+    if (wpb == 2) launch_run_packed<2>(d_sys, nsys, max_nlev, ra, s);
+    else if (wpb == 4) launch_run_packed<4>(d_sys, nsys, max_nlev, ra, s);
+    else if (grow && walls) launch_run<true, true>(d_sys, nsys, max_nlev, ra, s);
     else if (grow) launch_run<true, false>(d_sys, nsys, max_nlev, ra, s);
     else if (walls) launch_run<false, true>(d_sys, nsys, max_nlev, ra, s);
     else launch_run<false, false>(d_sys, nsys, max_nlev, ra, s);

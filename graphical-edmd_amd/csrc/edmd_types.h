@@ -266,7 +266,7 @@ typedef struct EdmdRunArgs {
     uint64_t max_coll;  /* absolute bound on the valid-collision counter */
     double tmax;        /* the reference's loop bound: run while t <= tmax */
     int32_t stop_mask;  /* stop after handling an event of kind k when bit k is set */
-    int32_t pad;
+    int32_t nsys;       /* systems in the launch (workgroups holding several) */
 } EdmdRunArgs;
 
 /* Next pending event as the reference's findNextEvent would return it. */

@@ -71,7 +71,8 @@ EPOCH_DT = np.dtype([("t", "<f8"), ("ncol", "<u8"), ("ncross", "<u8"), ("nevents
 
 class TuneReport(C.Structure):
     _fields_ = [("sweep_ms", C.c_double * 8), ("sweep_shape", C.c_int), ("in_lds", C.c_int),
-                ("loop_ms_hbm", C.c_double), ("loop_ms_lds", C.c_double)]
+                ("loop_ms_hbm", C.c_double), ("loop_ms_lds", C.c_double),
+                ("loop_ms_packed", C.c_double * 2), ("loop_systems_per_group", C.c_int)]
 
 TAPE_DT = np.dtype([("t", "<f8"), ("ax", "<f8"), ("ay", "<f8"), ("cx", "<f8"),
                     ("cy", "<f8"), ("ff", "<f8"), ("i", "<i4"), ("j", "<i4"),
@@ -133,6 +134,8 @@ def lib():
             "edmd_batch_set_sweep_shape": (i32, [vp, i32]),
             "edmd_batch_get_sweep_shape": (i32, [vp, vp, vp, vp, vp]),
             "edmd_batch_autotune": (i32, [vp, i32, i64, vp]),
+            "edmd_batch_set_loop_waves": (i32, [vp, i32]),
+            "edmd_batch_get_loop_waves": (i32, [vp, vp]),
             "edmd_batch_time_sweep": (i32, [vp, i32, vp]),
             "edmd_init_lattice": (i32, [i32, dbl, dbl, i32, u64, dbl] + [vp] * 6),
             "edmd_init_random": (i32, [i32, dbl, dbl, dbl, dbl, dbl, u64, dbl] + [vp] * 8),
@@ -512,13 +515,24 @@ class Batch:
         return dict(zip(("shape", "tile_x", "tile_y", "threads"), [x.value for x in v]))
 
     def autotune(self, sweep_reps=10, trial_events=2000):
-        """Times every sweep shape and both residencies on this batch (state
-        saved and restored) and keeps the fastest; returns the report."""
+        """Times every sweep shape and every event-loop configuration (arrays
+        in HBM with 1, 2, 4 systems per workgroup; arrays in LDS) on this
+        batch (state saved and restored) and keeps the fastest; returns the
+        report."""
         r = TuneReport()
         check(lib().edmd_batch_autotune(self.h, int(sweep_reps), int(trial_events), C.byref(r)))
         return {"sweep_ms": list(r.sweep_ms), "sweep_shape": r.sweep_shape,
                 "in_lds": bool(r.in_lds), "loop_ms_hbm": r.loop_ms_hbm,
-                "loop_ms_lds": r.loop_ms_lds}
+                "loop_ms_lds": r.loop_ms_lds, "loop_ms_packed": list(r.loop_ms_packed),
+                "loop_systems_per_group": r.loop_systems_per_group}
+
+    def set_loop_waves(self, systems_per_group):
+        check(lib().edmd_batch_set_loop_waves(self.h, int(systems_per_group)))
+
+    def loop_waves(self):
+        v = C.c_int()
+        check(lib().edmd_batch_get_loop_waves(self.h, C.byref(v)))
+        return v.value
 
     def residency(self):
         """{'lds': mode in use is LDS, 'lds_bytes': dynamic LDS per workgroup

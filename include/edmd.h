@@ -307,18 +307,31 @@ int edmd_batch_get_residency(edmd_batch* b, int* in_lds, int64_t* lds_bytes,
 int edmd_batch_set_sweep_shape(edmd_batch* b, int shape);
 int edmd_batch_get_sweep_shape(edmd_batch* b, int* shape, int* tile_x, int* tile_y, int* threads);
 
+/* Systems per workgroup of the event loop with arrays in HBM, no
+ * counterpart in the reference: 1 (default), 2 or 4, one wavefront per
+ * system either way.  More than one only for calendars of depth <= 2 (up to
+ * 4096 disks) without walls, field or damping; the growing systems' launch
+ * always takes one.  Results do not depend on it. */
+int edmd_batch_set_loop_waves(edmd_batch* b, int systems_per_group);
+int edmd_batch_get_loop_waves(edmd_batch* b, int* systems_per_group);
+
 /* Autotuner, no counterpart in the reference: times sweep_reps sweeps in
- * every tile shape and, when every system fits LDS, a trial of
- * trial_events events per system in each residency (EDMD_RES_HBM and
- * EDMD_RES_LDS), then keeps the fastest of each (the sweep shape in use
- * stays unless another is more than 3% faster).  The batch's state is
- * saved before and restored after every trial, so results are unchanged.
- * sweep_reps = 0 or trial_events = 0 skips that part.  Not while taping. */
+ * every tile shape and a trial of trial_events events per system in each
+ * event-loop configuration -- arrays in HBM with 1, 2 and 4 systems per
+ * workgroup (2 and 4 where edmd_batch_set_loop_waves allows them) and, when
+ * every system fits LDS, arrays in LDS -- then keeps the fastest sweep
+ * shape (the one in use stays unless another is more than 3% faster) and
+ * the fastest loop configuration.  The batch's state is saved before and
+ * restored after every trial, so results are unchanged.  sweep_reps = 0 or
+ * trial_events = 0 skips that part.  Not while taping. */
 typedef struct edmd_tune_report {
     double sweep_ms[EDMD_TUNE_SWEEP_SHAPES]; /* per shape, < 0 when not timed */
     int sweep_shape;                         /* the shape kept */
     int in_lds;                              /* the residency kept (1 = LDS) */
-    double loop_ms_hbm, loop_ms_lds;         /* trial times, < 0 when not run */
+    double loop_ms_hbm, loop_ms_lds;         /* trial times (HBM: one system per
+                                                workgroup), < 0 when not run */
+    double loop_ms_packed[2];                /* HBM, 2 and 4 systems per workgroup */
+    int loop_systems_per_group;              /* the systems per workgroup kept */
 } edmd_tune_report;
 int edmd_batch_autotune(edmd_batch* b, int sweep_reps, int64_t trial_events,
                         edmd_tune_report* out);

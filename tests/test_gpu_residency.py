@@ -260,6 +260,52 @@ def test_autotune_restores_state(gpu):
     b.sweep_predict()
     assert_batches_equal(a, b, len(states), traces=False)
     # a system too large for LDS: sweep shapes only
+    # a system too large for LDS: HBM configurations only
     big = gpu.Batch([F.lattice_state(5000, 0.75, seed=8, lattice="fit")])
     rep = big.autotune(sweep_reps=2, trial_events=1000)
-    assert rep["loop_ms_hbm"] < 0 and rep["loop_ms_lds"] < 0 and not rep["in_lds"]
+    assert rep["loop_ms_lds"] < 0 and not rep["in_lds"]
+    assert rep["loop_ms_hbm"] > 0 and all(ms > 0 for ms in rep["loop_ms_packed"])
+    assert rep["loop_systems_per_group"] == big.loop_waves() in (1, 2, 4)
+    assert rep["in_lds"] or rep["loop_systems_per_group"] in (1, 2, 4)
+
+
+def test_systems_per_workgroup_give_identical_results(gpu, oracle):
+    """The HBM loop with 1, 2 or 4 systems per workgroup (one wavefront
+    each, the last workgroup partly empty: 13 systems, two of them growing
+    and run by their own launch) gives the same traces, states and
+    calendars bit for bit, and every system matches the oracle."""
+    states = small_states(140, 11)
+    vrs = [0.0] * 11
+    for k, n in ((0, 300), (1, 420)):
+        states.insert(3 * k + 2, F.growth_state(n, 0.6, seed=900 + k))
+        vrs.insert(3 * k + 2, 0.1)
+    nev = 12000
+    out = []
+    for wpb in (1, 2, 4):
+        b = gpu.Batch(states, trace_capacity=nev, per_system={"vr": vrs})
+        b.set_residency(gpu.RES_HBM)
+        b.set_loop_waves(wpb)
+        assert b.loop_waves() == wpb
+        b.run(max_events=nev // 3)
+        b.run(max_events=nev)
+        out.append(b)
+    for b in out[1:]:
+        assert_batches_equal(out[0], b, len(states), traces=False)
+    # traces are drained when read: each batch's once
+    tr = [[b.trace(s) for s in range(len(states))] for b in out]
+    for s in range(len(states)):
+        assert len(tr[0][s]) == nev
+        for w in (1, 2):
+            np.testing.assert_array_equal(tr[w][s], tr[0][s], err_msg="system %d" % s)
+    for k in (0, 2, 5, 12):
+        o = oracle.Oracle(states[k], vr=vrs[k])
+        code, tr = o.run(nev)
+        assert code == 0
+        so, sb = o.state(), out[2].state(k)
+        for f in ("x", "y", "vx", "vy", "t"):
+            np.testing.assert_array_equal(bits(sb[f]), bits(so[f]), err_msg="%d %s" % (k, f))
+    walled = gpu.Batch(states[:2], wall_y=1)
+    with pytest.raises(gpu.EdmdError):
+        walled.set_loop_waves(2)
+    with pytest.raises(gpu.EdmdError):
+        out[0].set_loop_waves(3)
