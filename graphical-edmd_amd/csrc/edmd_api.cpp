@@ -68,6 +68,7 @@ extern "C" void edmd_default_desc(edmd_system_desc* d) {
     d->field = -0.02; /* EDMD.c:366 */
     d->damping = 0;
     d->gamma = 0.02; /* gamm, EDMD.c:382 */
+    d->wall_circle = 0;
 }
 
 /* ------------------------------------------------------------ handles */
@@ -253,11 +254,15 @@ static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
     c.field = d->field;
     c.damping = d->damping;
     c.gamma = d->gamma;
-    c.walls = (d->wall_x ? 1 : 0) | (d->wall_y ? 2 : 0);
+    if (d->wall_circle != 0 && d->wall_circle != 1)
+        return fail(EDMD_ERR_ARG, "system %d: wall_circle must be 0 or 1", idx);
+    if (d->wall_circle && d->add_field)
+        return fail(EDMD_ERR_ARG, "system %d: a circular wall goes without the field", idx);
+    c.walls = (d->wall_x ? 1 : 0) | (d->wall_y ? 2 : 0) | (d->wall_circle ? 4 : 0);
     c.res_wall = d->res_wall;
     c.res_tan = d->res_tan;
-    c.mihx = d->wall_x ? __builtin_huge_val() : c.halfLx;
-    c.mihy = d->wall_y ? __builtin_huge_val() : c.halfLy;
+    c.mihx = (d->wall_x || d->wall_circle) ? __builtin_huge_val() : c.halfLx;
+    c.mihy = (d->wall_y || d->wall_circle) ? __builtin_huge_val() : c.halfLy;
     c.dtp = 5.0 / (double)d->n; /* dtPaul, EDMD.c:712-714 */
     c.nlists = d->n;
     c.res = d->res;

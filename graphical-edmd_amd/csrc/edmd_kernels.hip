@@ -241,6 +241,61 @@ __device__ __forceinline__ double edmd_log(double x) {
     return e * 0x1.62e42fefa3800p-1 + (e * 0x1.ef35793c76730p-45 + lm);
 }
 
+/* sin and cos of an angle in [-pi, pi] (atan2's range): theta = k pi/2 + r,
+ * |r| <= pi/4, pi/2 in three parts (33, 33 and 53 bits, Cody and Waite) so
+ * that k times the first two is exact; sin r and cos r from their Taylor
+ * series to r^19/19! and r^20/20! in nested form; the quadrant k mod 4
+ * picks and signs them.  The oracle runs the same steps; both are within
+ * 2 ulp of the C library. */
+__device__ __forceinline__ void edmd_sincos(double th, double& sn, double& cs) {
+    const double k = rint(th * 0x1.45f306dc9c883p-1); /* th * 2/pi, to nearest */
+    const double r = ((th - k * 0x1.921fb54400000p+0) - k * 0x1.0b4611a600000p-34) -
+                     k * 0x1.3198a2e037073p-69;
+    const double r2 = r * r;
+    double ps = 1.0, pc = 1.0;
+#pragma unroll
+    for (int n = 19; n >= 3; n -= 2) ps = 1.0 - (r2 * ((1.0 / n) * (1.0 / (n - 1)))) * ps;
+#pragma unroll
+    for (int n = 20; n >= 2; n -= 2) pc = 1.0 - (r2 * ((1.0 / n) * (1.0 / (n - 1)))) * pc;
+    const double sr = r * ps, cr = pc;
+    switch (((int)k) & 3) {
+        case 0: sn = sr; cs = cr; break;
+        case 1: sn = cr; cs = -sr; break;
+        case 2: sn = -sr; cs = -cr; break;
+        default: sn = -cr; cs = sr; break;
+    }
+}
+
+/* atan z for z in [0, 1]: about 0.4 or 0.8 (atan z = atan c + atan w,
+ * w = (z - c)/(1 + z c), |w| <= 0.2), then w (1 - w^2 (1/3 - w^2 (1/5 -
+ * ... w^2/23))). */
+__device__ __forceinline__ double edmd_atan01(double z) {
+    double base = 0, w = z;
+    if (z > 0.6) {
+        w = (z - 0.8) / (1 + z * 0.8);
+        base = 0x1.5977a5103ea93p-1; /* atan(0.8) */
+    } else if (z > 0.2) {
+        w = (z - 0.4) / (1 + z * 0.4);
+        base = 0x1.85a376b677dc0p-2; /* atan(0.4) */
+    }
+    const double w2 = w * w;
+    double q = 1.0 / 23;
+#pragma unroll
+    for (int n = 21; n >= 3; n -= 2) q = (1.0 / n) - w2 * q;
+    return base + w * (1 - w2 * q);
+}
+
+/* atan2(y, x) in [-pi, pi], pi/2 and pi as head + tail. */
+__device__ __forceinline__ double edmd_atan2(double y, double x) {
+    const double ax = fabs(x), ay = fabs(y);
+    double a;
+    if (ay > ax) a = (0x1.921fb54442d18p+0 - edmd_atan01(ax / ay)) + 0x1.1a62633145c07p-54;
+    else if (ax > 0) a = edmd_atan01(ay / ax);
+    else a = 0;
+    if (__builtin_signbit(x)) a = (0x1.921fb54442d18p+1 - a) + 0x1.1a62633145c07p-53;
+    return __builtin_signbit(y) ? -a : a;
+}
+
 /* logTime, EDMD.c:5709-5721: under damping, the time at which the damped
  * motion has covered what the undamped one covers in t; never when it
  * cannot.  F = false compiles damping out. */
@@ -310,6 +365,18 @@ __device__ __forceinline__ bool wall_time(const EdmdConst& C, double x, double y
             xy = 0;
         }
     }
+    if (C.walls & 4) {
+        /* the circle (EDMD.c:2938-2954, no field): the centre reaches the
+         * circle of radius halfLx - r, for every disk; the time replaces
+         * the other walls' */
+        const double y0 = y - C.halfLx, x0 = x - C.halfLx;
+        const double Cc = (x0 * x0 + y0 * y0) - (C.halfLx - r) * (C.halfLx - r);
+        const double A = (vx * vx + vy * vy);
+        const double B = 2 * (vx * x0 + vy * y0);
+        dt = (-B + sqrt(B * B - 4 * A * Cc)) / (2 * A);
+        wall = true;
+        xy = 2;
+    }
     return wall;
 }
 
@@ -352,6 +419,16 @@ __device__ __forceinline__ bool wall_time_grow(const EdmdConst& C, double x, dou
             }
             xy = 0;
         }
+    }
+    if (C.walls & 4) {
+        /* EDMD.c:3158-3172: the surface closes on the circle at |v| - vr */
+        const double y0 = y - C.halfLx, x0 = x - C.halfLx;
+        const double A = vx * vx + vy * vy - vr * vr;
+        const double B = 2 * (vx * x0 + vy * y0 + (C.halfLx - r) * vr);
+        const double Cc = (x0 * x0 + y0 * y0) - (C.halfLx - r) * (C.halfLx - r);
+        dt = (-B + sqrt(B * B - 4 * A * Cc)) / (2 * A);
+        wall = true;
+        xy = 2;
     }
     return wall;
 }
@@ -2030,7 +2107,8 @@ __device__ __forceinline__ void ev_thermo(const EdmdConst& C, const AR& A, int l
     double exy = ordered_sum(C.n, lane, [&](int q) { return P[q].m * P[q].vx * P[q].vy; });
     double eyx = ordered_sum(C.n, lane, [&](int q) { return P[q].m * P[q].vy * P[q].vx; });
     EdmdState* st = A.st;
-    const double area = C.Lx * C.Ly;
+    /* EDMD.c:5253-5256: the disc's area inside a circular wall */
+    const double area = (C.walls & 4) ? 3.14159265358979323846 * C.halfLx * C.halfLx : C.Lx * C.Ly;
     const double dT = t - st->last_thermo;
     const double v0 = R.vir[0], v1 = R.vir[1], v2 = R.vir[2], v3 = R.vir[3];
     const int k = st->nthermo;
@@ -2049,6 +2127,7 @@ __device__ __forceinline__ void ev_thermo(const EdmdConst& C, const AR& A, int l
             if (C.walls) {
                 /* EDMD.c:5428-5451: dp / (deltaTime * perimeter) */
                 double perimeter = 0;
+                if (C.walls & 4) perimeter = 3.14159265358979323846 * C.Lx;
                 if (C.walls & 1) perimeter += 2 * C.Ly;
                 if (C.walls & 2) perimeter += 2 * C.Lx;
                 row.pwall = st->dpw / (dT * perimeter);
@@ -2406,12 +2485,31 @@ __device__ __forceinline__ bool ev_stop_grow(const EdmdConst& C, const AR& A, in
     WAVE_SYNC();
     const auto P = A.p;
     const int n = C.n;
-    const double px = ordered_sum(n, lane, [&](int q) { return P[q].vx * P[q].m; });
-    const double py = ordered_sum(n, lane, [&](int q) { return P[q].vy * P[q].m; });
-    for (int q = lane; q < n; q += 64) {
-        const double m = P[q].m;
-        P[q].vx -= px / (n * m);
-        P[q].vy -= py / (n * m);
+    if (!(W && (C.walls & 4))) {
+        const double px = ordered_sum(n, lane, [&](int q) { return P[q].vx * P[q].m; });
+        const double py = ordered_sum(n, lane, [&](int q) { return P[q].vy * P[q].m; });
+        for (int q = lane; q < n; q += 64) {
+            const double m = P[q].m;
+            P[q].vx -= px / (n * m);
+            P[q].vy -= py / (n * m);
+        }
+    } else {
+        /* in a circle, the angular momentum about the centre
+         * (EDMD.c:5730-5749): dvT accumulates -m (x0 vy - y0 vx), dvM m r */
+        const double h = C.halfLx;
+        const double dvT = -ordered_sum(n, lane, [&](int q) {
+            return P[q].m * (-P[q].vx * (P[q].y - h) + P[q].vy * (P[q].x - h));
+        });
+        const double dvM = ordered_sum(n, lane, [&](int q) {
+            return P[q].m * sqrt((P[q].x - h) * (P[q].x - h) + (P[q].y - h) * (P[q].y - h));
+        });
+        const double dv = dvT / dvM;
+        for (int q = lane; q < n; q += 64) {
+            const double x0 = P[q].x - h, y0 = P[q].y - h;
+            const double r = sqrt(x0 * x0 + y0 * y0);
+            P[q].vx += -dv * y0 / r;
+            P[q].vy += dv * x0 / r;
+        }
     }
     WAVE_SYNC();
     const double E = ordered_sum(n, lane, [&](int q) {
@@ -2484,10 +2582,16 @@ __device__ __forceinline__ void ev_single(const EdmdConst& C, const AR& A, int l
         if (d == 5) {
             const double sg = vx < 0 ? -1.0 : (vx > 0 ? 1.0 : 0.0);
             vx = -sg * (fabs(vx) + 2 * gvr);
-        } else if (y > C.Ly - 3) {
-            vy = -vy - 2 * gvr;
+        } else if (d == 6) {
+            if (y > C.Ly - 3) vy = -vy - 2 * gvr;
+            else vy = -vy + 2 * gvr;
         } else {
-            vy = -vy + 2 * gvr;
+            /* the circle: reflected about the normal, 2 vr added along it */
+            double nx, ny;
+            edmd_sincos(edmd_atan2(y - C.halfLx, x - C.halfLx), ny, nx);
+            const double mix = -(1 + C.res_wall) * (vx * nx + vy * ny);
+            vx += (mix - 2 * gvr) * nx;
+            vy += (mix - 2 * gvr) * ny;
         }
         vx = uni(vx);
         vy = uni(vy);
@@ -2513,10 +2617,19 @@ __device__ __forceinline__ void ev_single(const EdmdConst& C, const AR& A, int l
             dp = fabs((C.res_wall + 1) * m * vx);
             vx = -C.res_wall * vx;
             vy = C.res_tan * vy + 0.0;
-        } else {
+        } else if (d == 6) {
             dp = fabs((C.res_wall + 1) * m * vy);
             vy = -C.res_wall * vy;
             vx = C.res_tan * vx + 0.0;
+        } else {
+            /* the circle (EDMD.c:3441-3454): the normal velocity reversed
+             * times resW, |m mix| to the wall */
+            double nx, ny;
+            edmd_sincos(edmd_atan2(y - C.halfLx, x - C.halfLx), ny, nx);
+            const double mix = -(1 + C.res_wall) * (vx * nx + vy * ny);
+            vx += mix * nx;
+            vy += mix * ny;
+            dp = fabs(m * mix);
         }
         vx = uni(vx);
         vy = uni(vy);

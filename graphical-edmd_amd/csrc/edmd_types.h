@@ -78,7 +78,7 @@ typedef struct EdmdConst {
     double einit; /* Einit: E/N after the growth stops (EDMD.c:369, 5759-5763) */
     double dtp;   /* dtPaul = 5/N, the reference's bucket width (EDMD.c:712-714) */
     int32_t nlists; /* bucket count of the reference's queue (N) */
-    int32_t walls;  /* bit 0: walls normal to x, bit 1: normal to y */
+    int32_t walls;  /* bit 0: walls normal to x, bit 1: normal to y, bit 2: the circle */
     double res_wall, res_tan; /* resW, resT (EDMD.c:372-373) */
     /* minimum-image half widths: halfLx / halfLy, +inf across walls */
     double mihx, mihy;

@@ -45,7 +45,7 @@ class SystemDesc(C.Structure):
                 ("thermo_capacity", C.c_int), ("vr", C.c_double), ("einit", C.c_double),
                 ("wall_x", C.c_int), ("wall_y", C.c_int), ("res_wall", C.c_double),
                 ("res_tan", C.c_double), ("add_field", C.c_int), ("field", C.c_double),
-                ("damping", C.c_int), ("gamma", C.c_double)]
+                ("damping", C.c_int), ("gamma", C.c_double), ("wall_circle", C.c_int)]
 
 
 class Observables(C.Structure):

@@ -118,6 +118,21 @@ typedef struct edmd_system_desc {
      * taping. */
     int damping;
     double gamma;
+    /* Circular wall (addCircularWall of the graphics build, EDMD.c:270): a
+     * circle of radius Lx/2 about (Lx/2, Lx/2) (the reference takes Lx/2
+     * for both coordinates; boxes are square); a disk's centre meets the
+     * circle of radius Lx/2 - rad (EDMD.c:2938-2954; with growth, its
+     * surface at |v| - vr, EDMD.c:3158-3172), for every disk, replacing the
+     * other walls' times; a hit reflects the velocity about the normal at
+     * the angle atan2 of the disk's position (EDMD.c:3441-3454, growth
+     * EDMD.c:3358-3376), this library's own atan2 / sin / cos (within 2 ulp
+     * of the C library's, the same steps on the CPU checker).  No minimum
+     * image; thermo rows take the area pi (Lx/2)^2 and the wall perimeter
+     * pi Lx; at GROWSTOP the angular momentum about the centre is removed
+     * instead of the momentum (EDMD.c:5723-5764).  The disks must start
+     * inside the circle.  Not with the field (the quartic of quartic.c) or
+     * taping. */
+    int wall_circle;
 } edmd_system_desc;
 
 /* A collision slot holding a wall event reports partner -5 - xy

@@ -68,11 +68,11 @@ struct orc_ctx {
 /* PBC, EDMD.c:5896-5913: minimum image with >= L/2 and < -L/2, none
  * across walls. */
 static void min_image(const orc_ctx* s, double* dx, double* dy) {
-    if (!s->prm.wall_x) {
+    if (!s->prm.wall_x && !s->prm.wall_circle) {
         if (*dx >= s->halfLx) *dx -= s->Lx;
         else if (*dx < -s->halfLx) *dx += s->Lx;
     }
-    if (!s->prm.wall_y) {
+    if (!s->prm.wall_y && !s->prm.wall_circle) {
         if (*dy >= s->halfLy) *dy -= s->Ly;
         else if (*dy < -s->halfLy) *dy += s->Ly;
     }
@@ -135,6 +135,70 @@ double orc_log(double x) {
     for (int n = 21; n >= 1; n -= 2) q = ORC_INV[n] + s2 * q;
     const double lm = 2 * s * q;
     return e * ORC_LN2_HI + (e * ORC_LN2_LO + lm);
+}
+
+/* sin and cos of an angle in [-pi, pi] (atan2's range): theta = k pi/2 + r,
+ * |r| <= pi/4, with pi/2 in three parts (33, 33 and 53 bits, Cody and Waite)
+ * so that k times the first two is exact; sin r and cos r from their Taylor
+ * series to r^19/19! and r^20/20! (truncation below 1e-19 relative) in the
+ * nested forms r (1 - r^2/(2 3) (1 - r^2/(4 5) (...))) and
+ * 1 - r^2/(1 2) (1 - r^2/(3 4) (...)); the quadrant k mod 4 picks and signs
+ * them.  Constants: oracle/trig_constants.py.  Same steps on the GPU. */
+static const double ORC_PIO2_1 = 0x1.921fb54400000p+0;
+static const double ORC_PIO2_2 = 0x1.0b4611a600000p-34;
+static const double ORC_PIO2_3 = 0x1.3198a2e037073p-69;
+static const double ORC_PIO2_HI = 0x1.921fb54442d18p+0, ORC_PIO2_LO = 0x1.1a62633145c07p-54;
+static const double ORC_PI_HI = 0x1.921fb54442d18p+1, ORC_PI_LO = 0x1.1a62633145c07p-53;
+static const double ORC_2_OVER_PI = 0x1.45f306dc9c883p-1;
+static const double ORC_ATAN_04 = 0x1.85a376b677dc0p-2; /* atan(0.4) */
+static const double ORC_ATAN_08 = 0x1.5977a5103ea93p-1; /* atan(0.8) */
+
+void orc_sincos(double th, double* sn, double* cs) {
+    const double k = rint(th * ORC_2_OVER_PI);
+    const double r = ((th - k * ORC_PIO2_1) - k * ORC_PIO2_2) - k * ORC_PIO2_3;
+    const double r2 = r * r;
+    double ps = 1.0, pc = 1.0;
+    for (int n = 19; n >= 3; n -= 2) ps = 1.0 - (r2 * (ORC_INV[n] * ORC_INV[n - 1])) * ps;
+    for (int n = 20; n >= 2; n -= 2) pc = 1.0 - (r2 * (ORC_INV[n] * ORC_INV[n - 1])) * pc;
+    const double sr = r * ps, cr = pc;
+    switch (((int)k) & 3) {
+        case 0: *sn = sr; *cs = cr; break;
+        case 1: *sn = cr; *cs = -sr; break;
+        case 2: *sn = -sr; *cs = -cr; break;
+        default: *sn = -cr; *cs = sr; break;
+    }
+}
+
+/* atan z for z in [0, 1]: about 0.4 or 0.8 (atan z = atan c + atan w,
+ * w = (z - c)/(1 + z c)) so that |w| <= 0.2, then the series
+ * w (1 - w^2 (1/3 - w^2 (1/5 - ... w^2/23))), truncation below 1e-18
+ * relative. */
+static double orc_atan01(double z) {
+    double base = 0, w = z;
+    if (z > 0.6) {
+        w = (z - 0.8) / (1 + z * 0.8);
+        base = ORC_ATAN_08;
+    } else if (z > 0.2) {
+        w = (z - 0.4) / (1 + z * 0.4);
+        base = ORC_ATAN_04;
+    }
+    const double w2 = w * w;
+    double q = ORC_INV[23];
+    for (int n = 21; n >= 3; n -= 2) q = ORC_INV[n] - w2 * q;
+    return base + w * (1 - w2 * q);
+}
+
+/* atan2(y, x) in [-pi, pi]: the angle of the smaller over the larger
+ * coordinate, reflected about pi/4 and pi/2 and signed by quadrant (pi/2
+ * and pi as head + tail).  Same steps on the GPU. */
+double orc_atan2(double y, double x) {
+    const double ax = fabs(x), ay = fabs(y);
+    double a;
+    if (ay > ax) a = (ORC_PIO2_HI - orc_atan01(ax / ay)) + ORC_PIO2_LO;
+    else if (ax > 0) a = orc_atan01(ay / ax);
+    else a = 0;
+    if (signbit(x)) a = (ORC_PI_HI - a) + ORC_PI_LO;
+    return signbit(y) ? -a : a;
 }
 
 /* logTime, EDMD.c:5709-5721: the time at which the damped motion has gone
@@ -449,6 +513,17 @@ static void predict_collision_grow(orc_ctx* s, int i, double* t_abs, int* partne
             xy = 0;
         }
     }
+    if (s->prm.wall_circle) {
+        /* EDMD.c:3158-3172: the surface closes on the circle at |v| - vr */
+        double y0 = s->y[i] - s->halfLx, x0 = s->x[i] - s->halfLx;
+        double v0x = s->vx[i], v0y = s->vy[i], rad = s->rad[i];
+        double A = v0x * v0x + v0y * v0y - vr * vr;
+        double B = 2 * (v0x * x0 + v0y * y0 + (s->halfLx - rad) * vr);
+        double Cc = (x0 * x0 + y0 * y0) - (s->halfLx - rad) * (s->halfLx - rad);
+        dt = (-B + sqrt(B * B - 4 * A * Cc)) / (2 * A);
+        wall = 1;
+        xy = 2;
+    }
     for (int j = -1; j <= 1; j++) {
         for (int k = -1; k <= 1; k++) {
             int c = wrap_cell(Y + j, s->ncy) * s->ncx + wrap_cell(X + k, s->ncx);
@@ -527,6 +602,18 @@ static void predict_collision(orc_ctx* s, int i, double* t_abs, int* partner) {
             }
             xy = 0;
         }
+    }
+    if (s->prm.wall_circle) {
+        /* EDMD.c:2938-2954 (no field): the centre reaches the circle of
+         * radius halfLx - rad; the time replaces any earlier wall's */
+        double y0 = s->y[i] - s->halfLx, x0 = s->x[i] - s->halfLx, rad = s->rad[i];
+        double Cc = (x0 * x0 + y0 * y0) - (s->halfLx - rad) * (s->halfLx - rad);
+        double v0x = s->vx[i], v0y = s->vy[i];
+        double A = (v0x * v0x + v0y * v0y);
+        double B = 2 * (v0x * x0 + v0y * y0);
+        dt = (-B + sqrt(B * B - 4 * A * Cc)) / (2 * A);
+        wall = 1;
+        xy = 2;
     }
     for (int j = -1; j <= 1; j++) {
         for (int k = -1; k <= 1; k++) {
@@ -793,10 +880,16 @@ static void do_wall_grow(orc_ctx* s, const ev_node* ev) {
     const double vr = s->vr[i];
     if (xy == 0) {
         s->vx[i] = -sign_of(s->vx[i]) * (fabs(s->vx[i]) + 2 * vr);
-    } else if (s->y[i] > s->Ly - 3) {
-        s->vy[i] = -s->vy[i] - 2 * vr;
+    } else if (xy == 1) {
+        if (s->y[i] > s->Ly - 3) s->vy[i] = -s->vy[i] - 2 * vr;
+        else s->vy[i] = -s->vy[i] + 2 * vr;
     } else {
-        s->vy[i] = -s->vy[i] + 2 * vr;
+        /* the circle: reflected about the normal, 2 vr added along it */
+        double nx, ny;
+        orc_sincos(orc_atan2(s->y[i] - s->halfLx, s->x[i] - s->halfLx), &ny, &nx);
+        double mix = -(1 + s->prm.res_wall) * (s->vx[i] * nx + s->vy[i] * ny);
+        s->vx[i] += (mix - 2 * vr) * nx;
+        s->vy[i] += (mix - 2 * vr) * ny;
     }
     s->coll[i]++;
     queue_remove(s, i);
@@ -827,10 +920,19 @@ static void do_wall(orc_ctx* s, const ev_node* ev) {
         s->dp += fabs((rw + 1) * s->m[i] * s->vx[i]);
         s->vx[i] = -rw * s->vx[i];
         s->vy[i] = rt * s->vy[i] + 0.0;
-    } else {
+    } else if (xy == 1) {
         s->dp += fabs((rw + 1) * s->m[i] * s->vy[i]);
         s->vy[i] = -rw * s->vy[i];
         s->vx[i] = rt * s->vx[i] + 0.0;
+    } else {
+        /* the circle (EDMD.c:3441-3454): the normal velocity reversed
+         * times resW, the momentum |m mix| to the wall */
+        double nx, ny;
+        orc_sincos(orc_atan2(s->y[i] - s->halfLx, s->x[i] - s->halfLx), &ny, &nx);
+        double mix = -(1 + rw) * (s->vx[i] * nx + s->vy[i] * ny);
+        s->vx[i] += mix * nx;
+        s->vy[i] += mix * ny;
+        s->dp += fabs(s->m[i] * mix);
     }
     s->coll[i]++;
     queue_remove(s, i);
@@ -849,6 +951,7 @@ static void do_thermo(orc_ctx* s) {
     /* takeAThermo + saveThermo, EDMD.c:4679-4713, 5232-5333 */
     physical_q(s);
     double area = s->Lx * s->Ly;
+    if (s->prm.wall_circle) area = M_PI * s->halfLx * s->halfLx; /* EDMD.c:5254-5256 */
     double dT = s->t - s->last_thermo;
     s->last_thermo = s->t;
     double ex = 0, ey = 0, exy = 0, eyx = 0;
@@ -870,8 +973,9 @@ static void do_thermo(orc_ctx* s) {
     row.pressure = (-1 / (dT)) * (s->vir_xx + s->vir_yy) / (2 * area) + s->E / area;
     s->vir_xx = s->vir_yy = s->vir_xy = s->vir_yx = 0;
     row.pwall = 0;
-    if (s->prm.wall_x || s->prm.wall_y) {
+    if (s->prm.wall_x || s->prm.wall_y || s->prm.wall_circle) {
         double perimeter = 0;
+        if (s->prm.wall_circle) perimeter = M_PI * s->Lx;
         if (s->prm.wall_x) perimeter += 2 * s->Ly;
         if (s->prm.wall_y) perimeter += 2 * s->Lx;
         row.pwall = s->dp / (dT * perimeter);
@@ -906,12 +1010,29 @@ static void do_rescale(orc_ctx* s) {
 }
 
 static void normalize_physical_q(orc_ctx* s) {
-    /* normalizePhysicalQ, EDMD.c:5723-5764 (no circular wall): remove the
-     * mean momentum, then scale to E/N = Einit. */
-    physical_q(s);
-    for (int i = 0; i < s->n; i++) {
-        s->vx[i] -= s->px / (s->n * s->m[i]);
-        s->vy[i] -= s->py / (s->n * s->m[i]);
+    /* normalizePhysicalQ, EDMD.c:5723-5764: remove the mean momentum (in a
+     * circle, the angular momentum about the centre instead), then scale
+     * to E/N = Einit. */
+    if (!s->prm.wall_circle) {
+        physical_q(s);
+        for (int i = 0; i < s->n; i++) {
+            s->vx[i] -= s->px / (s->n * s->m[i]);
+            s->vy[i] -= s->py / (s->n * s->m[i]);
+        }
+    } else {
+        const double h = s->halfLx;
+        double dvT = 0, dvM = 0;
+        for (int i = 0; i < s->n; i++) {
+            double r = sqrt((s->x[i] - h) * (s->x[i] - h) + (s->y[i] - h) * (s->y[i] - h));
+            dvT -= s->m[i] * (-s->vx[i] * (s->y[i] - h) + s->vy[i] * (s->x[i] - h));
+            dvM += s->m[i] * r;
+        }
+        double dv = dvT / dvM;
+        for (int i = 0; i < s->n; i++) {
+            double r = sqrt((s->x[i] - h) * (s->x[i] - h) + (s->y[i] - h) * (s->y[i] - h));
+            s->vx[i] += -dv * (s->y[i] - h) / r;
+            s->vy[i] += dv * (s->x[i] - h) / r;
+        }
     }
     physical_q(s);
     for (int i = 0; i < s->n; i++) {
@@ -996,6 +1117,7 @@ void orc_default_params(orc_params* p) {
     p->field = -0.02; /* EDMD.c:366 */
     p->damping = 0;
     p->gamma = 0.02; /* gamm, EDMD.c:382 */
+    p->wall_circle = 0;
 }
 
 orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* y,
@@ -1007,7 +1129,7 @@ orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* 
     s->n = n;
     if (prm) s->prm = *prm;
     else orc_default_params(&s->prm);
-    if (s->prm.damping && !(s->prm.gamma > 0)) {
+    if ((s->prm.damping && !(s->prm.gamma > 0)) || (s->prm.wall_circle && s->prm.add_field)) {
         free(s);
         return NULL;
     }

@@ -77,6 +77,16 @@ typedef struct {
      * library uses, not the C library's. */
     int damping;
     double gamma;        /* gamm, default 0.02 */
+    /* Circular wall (addCircularWall, EDMD.c:270): a circle of radius
+     * halfLx about (halfLx, halfLx) (the reference uses halfLx for both
+     * coordinates); a disk's centre meets the circle of radius
+     * halfLx - rad (EDMD.c:2938-2954, growth EDMD.c:3158-3172); a hit
+     * reflects the velocity about the normal at atan2 of the disk's
+     * position (EDMD.c:3441-3454, growth 3358-3376), with this oracle's
+     * own atan2 / sin / cos; no minimum image (EDMD.c:5896-5913); the area
+     * of the thermo row is pi halfLx^2 and the wall perimeter pi Lx.  Not
+     * with the field (whose circle needs the quartic of quartic.c). */
+    int wall_circle;
 } orc_params;
 
 typedef struct {
@@ -189,6 +199,10 @@ void orc_adjoint(int n, const double* m, const double* tl0, const orc_tape_rec* 
 /* The exponential and logarithm used under damping (see orc_params). */
 double orc_exp(double x);
 double orc_log(double x);
+/* The library's own sin/cos (angles in [-pi, pi]) and atan2, the circular
+ * wall's normal. */
+void orc_sincos(double th, double* sn, double* cs);
+double orc_atan2(double y, double x);
 
 /* Sum of w_k = m_i m_j ff |a|^2 over the taped collisions, in tape order. */
 double orc_tape_wsum(const orc_ctx* s);

@@ -24,7 +24,7 @@ class Params(C.Structure):
                 ("bucket_count", C.c_int), ("vr", C.c_double), ("einit", C.c_double),
                 ("wall_x", C.c_int), ("wall_y", C.c_int), ("res_wall", C.c_double),
                 ("res_tan", C.c_double), ("add_field", C.c_int), ("field", C.c_double),
-                ("damping", C.c_int), ("gamma", C.c_double)]
+                ("damping", C.c_int), ("gamma", C.c_double), ("wall_circle", C.c_int)]
 
 
 class ThermoRow(C.Structure):
@@ -93,6 +93,9 @@ def lib():
         L.orc_exp.argtypes = [C.c_double]
         L.orc_log.restype = C.c_double
         L.orc_log.argtypes = [C.c_double]
+        L.orc_sincos.argtypes = [C.c_double, C.c_void_p, C.c_void_p]
+        L.orc_atan2.restype = C.c_double
+        L.orc_atan2.argtypes = [C.c_double, C.c_double]
         L.orc_tape_wsum.restype = C.c_double
         L.orc_tape_wsum.argtypes = [C.c_void_p]
         L.orc_replay.restype = C.c_double

@@ -296,3 +296,72 @@ def test_walls_under_growth_match_oracle(gpu, oracle, n, phi, seed, poly, vr, kw
         assert_state_equal(b.state(0), so)
         np.testing.assert_array_equal(bits(b.radii(0)), bits(o.radii()))
         assert b.observables(0)["ncol"] == o.observables()["ncol"]
+
+
+# ---- the circular wall (addCircularWall)
+
+def in_circle(st, margin=1.0):
+    """The disks of st whose centres lie more than `margin` inside the
+    circle of radius Lx/2 about (Lx/2, Lx/2)."""
+    c = st["Lx"] / 2
+    keep = (st["x"] - c) ** 2 + (st["y"] - c) ** 2 < (c - margin) ** 2
+    return {k: (v[keep] if isinstance(v, np.ndarray) else v) for k, v in st.items()}
+
+
+CIRCLE_CASES = [
+    (500, 0.55, 81, 0.0, 0.0, dict()),
+    (450, 0.5, 82, 0.0, 0.0, dict(res_wall=0.9)),
+    (400, 0.5, 83, 0.0, 0.0, dict(damping=1, gamma=0.01)),
+    (450, 0.5, 84, 0.1, 0.1, dict()),                     # growth start through GROWSTOP
+]
+
+
+@pytest.mark.parametrize("n,phi,seed,poly,vr,kw", CIRCLE_CASES)
+def test_circular_wall_matches_oracle(gpu, oracle, n, phi, seed, poly, vr, kw):
+    """Disks inside a circular wall: every disk's circle time, hits that
+    reflect about the normal at the library's own atan2 / sin / cos (the
+    checker runs the same steps), no minimum image, thermo rows over the
+    disc's area with the wall pressure over pi Lx, growth starts through
+    GROWSTOP's angular-momentum removal: traces, states and thermo rows bit
+    for bit against the oracle, from HBM and from LDS."""
+    kw = dict(kw, wall_circle=1, first_screen=1e9, dtime=1e9, first_thermo=0.5,
+              dtime_thermo=0.5)
+    if vr > 0:
+        st = in_circle(F.growth_state(n, phi, seed, poly=poly))
+    else:
+        st = in_circle(F.lattice_state(n, phi, seed, lattice="fit"))
+    nev = 20000
+    o = oracle.Oracle(st, vr=vr, **kw)
+    code, tr = o.run(nev)
+    assert code == 0
+    assert np.sum(tr["type"] == WALL) > 100
+    if vr > 0:
+        assert np.sum(tr["type"] == F.GROWSTOP) == 1
+    so, orows = o.state(), o.thermo_rows()
+    assert len(orows) > 3 and any(r["pwall"] > 0 for r in orows)
+    for mode in (gpu.RES_HBM, gpu.RES_LDS):
+        b = gpu.Batch([st], trace_capacity=nev, thermo_capacity=256, vr=vr, **kw)
+        b.set_residency(mode)
+        b.run(max_events=nev)
+        g = b.trace(0)
+        if vr > 0:
+            g, c = F.canonical_trace(g), F.canonical_trace(tr)
+        else:
+            c = tr
+        for k in ("type", "i", "j", "valid"):
+            np.testing.assert_array_equal(g[k], c[k], err_msg=k)
+        np.testing.assert_array_equal(bits(g["t"]), bits(c["t"]))
+        assert_state_equal(b.state(0), so)
+        rows = b.thermo(0)
+        assert len(rows) == len(orows)
+        for r, q in zip(rows, orows):
+            for k in ("t", "e_per_n", "pressure", "pwall"):
+                assert bits(np.float64(r[k])) == bits(np.float64(q[k])), k
+
+
+def test_circular_wall_arguments_are_checked(gpu):
+    st = in_circle(F.lattice_state(200, 0.5, seed=5, lattice="fit"))
+    with pytest.raises(gpu.EdmdError):
+        gpu.System(st, wall_circle=1, add_field=1)
+    with pytest.raises(gpu.EdmdError):
+        gpu.System(st, wall_circle=2)

@@ -7,6 +7,8 @@ addWally are constants 0, EDMD.c:290-291), so no reference vector exists for
 them; these checks pin the restatement instead: exact wall-contact times and
 bounce periods, the reflection rule, conservation laws and the ideal-gas
 limit of the wall pressure."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -343,3 +345,95 @@ def test_walls_under_growth_closed_form(oracle):
     assert ob["ncol"] == 3
     # the radii grew with the flights: 0.1 per unit time up to t = 4
     np.testing.assert_allclose(o.radii(), 0.4, rtol=1e-14)
+
+
+# ---- the circular wall (addCircularWall, EDMD.c:2938-2954, 3158-3172,
+# 3358-3376, 3441-3454, 5723-5764)
+
+def test_own_trig_is_accurate(oracle):
+    """The oracle's atan2 / sin / cos (the steps the GPU library also runs)
+    agree with the C library to 2 ulp over the angles the circular wall
+    meets; atan2 keeps IEEE's quadrants and signed zeros."""
+    L = oracle.lib()
+    rng = np.random.default_rng(7)
+    s, c = C.c_double(), C.c_double()
+    for th in np.concatenate([rng.uniform(-np.pi, np.pi, 20000), [0.0, np.pi, -np.pi, np.pi / 2]]):
+        L.orc_sincos(float(th), C.byref(s), C.byref(c))
+        assert abs(s.value - np.sin(th)) <= 2 * np.spacing(max(abs(np.sin(th)), 1e-300)), th
+        assert abs(c.value - np.cos(th)) <= 2 * np.spacing(max(abs(np.cos(th)), 1e-300)), th
+    ys = rng.normal(size=20000) * np.exp(rng.uniform(-5, 5, 20000))
+    xs = rng.normal(size=20000) * np.exp(rng.uniform(-5, 5, 20000))
+    for y, x in zip(ys, xs):
+        r = np.arctan2(y, x)
+        assert abs(L.orc_atan2(float(y), float(x)) - r) <= 2 * np.spacing(abs(r)), (y, x)
+    for y, x in [(0.0, 1.0), (0.0, -1.0), (-0.0, -1.0), (1.0, 0.0), (-1.0, 0.0), (-0.0, 2.0)]:
+        assert L.orc_atan2(y, x) == np.arctan2(y, x)
+        assert np.signbit(L.orc_atan2(y, x)) == np.signbit(np.arctan2(y, x))
+
+
+def test_circular_wall_closed_form(oracle):
+    """In a circle of radius 10 about (10, 10) a disk of radius 0.5 at
+    (11, 10) moving along +x at 1 meets the wall when its centre is 9.5 out,
+    at t = 8.5, comes back along the diameter (the normal is exactly +x)
+    and meets the far side 19 later; a hit at an oblique angle keeps the
+    speed and reverses the normal component; the pressure row divides by
+    the area pi 10^2 and the wall momentum by the perimeter pi 20."""
+    st = disks([(11.0, 10.0), (10.0, 5.0)], [(1.0, 0.0), (0.0, 0.01)])
+    o = oracle.Oracle(st, wall_circle=1, **QUIET)
+    t, p = o.predict_collision(0)
+    assert p == -5 - 2 and t == pytest.approx(8.5, rel=1e-15)
+    o.run_until(9.0)
+    oracle.lib().orc_sync_all(o.h, o.t)
+    s = o.state()
+    assert s["vx"][0] == -1.0 and s["vy"][0] == 0.0
+    t, p = o.predict_collision(0)
+    assert p == -7 and t == pytest.approx(27.5, rel=1e-14)
+    # oblique: from (10, 12) at (1, 0.5)
+    st = disks([(10.0, 12.0), (10.0, 5.0)], [(1.0, 0.5), (0.0, 0.01)])
+    o = oracle.Oracle(st, wall_circle=1, **QUIET)
+    t, p = o.predict_collision(0)
+    assert p == -7
+    code, tr = o.run(40)
+    k = np.nonzero((tr["type"] == WALL) & (tr["i"] == 0))[0][0]
+    o2 = oracle.Oracle(st, wall_circle=1, **QUIET)
+    o2.run(int(k) + 1)
+    oracle.lib().orc_sync_all(o2.h, o2.t)
+    s = o2.state()
+    x, y = s["x"][0] - 10, s["y"][0] - 10
+    assert np.hypot(x, y) == pytest.approx(9.5, rel=1e-12)
+    n = np.array([x, y]) / np.hypot(x, y)
+    v1 = np.array([s["vx"][0], s["vy"][0]])
+    assert np.hypot(*v1) == pytest.approx(np.hypot(1.0, 0.5), rel=1e-14)
+    assert v1 @ n == pytest.approx(-(np.array([1.0, 0.5]) @ n), rel=1e-12)
+    assert v1 @ n < 0
+
+
+def test_circular_wall_under_growth_and_growstop(oracle):
+    """A growing disk at the centre moving at v meets the circle when
+    v t + vr t = R: t = 10 / 1.05 (target 0.5 at rate 0.1); the hit turns
+    vx to v - 2 v - 2 vr = -1.1.  At GROWSTOP the circle's normalisation
+    removes the angular momentum about the centre (EDMD.c:5730-5749) and
+    scales to E/N = Einit."""
+    st = disks([(10.0, 10.0), (10.0, 6.0)], [(1.0, 0.0), (0.3, -0.2)])
+    o = oracle.Oracle(st, wall_circle=1, vr=0.1, **QUIET)
+    t, p = o.predict_collision(0)
+    assert p == -7 and t == pytest.approx(10 / 1.05, rel=1e-14)
+    o.run_until(t + 0.01)
+    oracle.lib().orc_sync_all(o.h, o.t)
+    assert o.state()["vx"][0] == pytest.approx(-1.1, rel=1e-14)
+    st = F.growth_state(200, 0.4, seed=41)
+    c = st["Lx"] / 2
+    keep = (st["x"] - c) ** 2 + (st["y"] - c) ** 2 < (c - 1.0) ** 2
+    st = {k: (v[keep] if isinstance(v, np.ndarray) else v) for k, v in st.items()}
+    o = oracle.Oracle(st, wall_circle=1, vr=0.1, **QUIET)
+    code, tr = o.run(20000)
+    assert code == 0 and np.sum(tr["type"] == F.GROWSTOP) == 1
+    s = o.state()
+    k = np.nonzero(tr["type"] == F.GROWSTOP)[0][0]
+    o2 = oracle.Oracle(st, wall_circle=1, vr=0.1, **QUIET)
+    o2.run(int(k) + 1)
+    s = o2.state()
+    Lz = np.sum(st["m"] * ((s["x"] - c) * s["vy"] - (s["y"] - c) * s["vx"]))
+    E = 0.5 * np.sum(st["m"] * (s["vx"] ** 2 + s["vy"] ** 2))
+    assert abs(Lz) < 1e-10 * np.sum(st["m"] * np.hypot(s["vx"], s["vy"]) * c)
+    assert E / len(s["x"]) == pytest.approx(1.0, rel=1e-12)
