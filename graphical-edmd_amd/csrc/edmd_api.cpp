@@ -69,6 +69,9 @@ extern "C" void edmd_default_desc(edmd_system_desc* d) {
     d->damping = 0;
     d->gamma = 0.02; /* gamm, EDMD.c:382 */
     d->wall_circle = 0;
+    d->mid_wall = 0;
+    d->type = nullptr;
+    d->sizeratio = 0.4; /* EDMD.c:83 */
 }
 
 /* ------------------------------------------------------------ handles */
@@ -107,6 +110,7 @@ struct edmd_batch {
     int64_t *erank = nullptr, *lrank = nullptr; /* tie ranks: 2 slots and 1 leaf per particle */
     /* growth starts: per-particle growth rates; any_grow while a system grows */
     double* vr = nullptr;
+    int32_t* ptype = nullptr; /* disk types, when a system has the mid wall */
     bool any_grow = false;
     /* results of the step-level calls (peek, pair time), allocated once */
     EdmdPeek* step_peek = nullptr;
@@ -258,7 +262,11 @@ static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
         return fail(EDMD_ERR_ARG, "system %d: wall_circle must be 0 or 1", idx);
     if (d->wall_circle && d->add_field)
         return fail(EDMD_ERR_ARG, "system %d: a circular wall goes without the field", idx);
-    c.walls = (d->wall_x ? 1 : 0) | (d->wall_y ? 2 : 0) | (d->wall_circle ? 4 : 0);
+    if (d->mid_wall != 0 && d->mid_wall != 1)
+        return fail(EDMD_ERR_ARG, "system %d: mid_wall must be 0 or 1", idx);
+    c.walls = (d->wall_x ? 1 : 0) | (d->wall_y ? 2 : 0) | (d->wall_circle ? 4 : 0) |
+              (d->mid_wall ? 8 : 0);
+    c.sizeratio = d->sizeratio;
     c.res_wall = d->res_wall;
     c.res_tan = d->res_tan;
     c.mihx = (d->wall_x || d->wall_circle) ? __builtin_huge_val() : c.halfLx;
@@ -346,6 +354,12 @@ static int upload_system(edmd_batch* b, int s, const edmd_system_desc* d) {
     HIPCHK(hipMemcpy(b->ccnt + h.cell0, cnt.data(), ncell * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->cslot + h.slot0, slot.data(), ncell * c.cap * sizeof(int32_t),
                      hipMemcpyHostToDevice));
+    if (b->ptype) {
+        std::vector<int32_t> ty(n, 0);
+        if (d->type)
+            for (int i = 0; i < n; i++) ty[i] = d->type[i];
+        HIPCHK(hipMemcpy(b->ptype + p0, ty.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
     if (b->vr) {
         std::vector<double> g(n, 0.0);
         if (d->vr > 0)
@@ -477,6 +491,9 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
     for (int s = 0; s < nsys; s++)
         b->any_walls |= b->hs[s].c.walls != 0 || b->hs[s].c.field_on || b->hs[s].c.damping;
     if (!rc && b->any_grow) rc = dalloc(b, &b->vr, np);
+    bool any_mid = false;
+    for (int s = 0; s < nsys; s++) any_mid |= descs[s].mid_wall != 0;
+    if (!rc && any_mid) rc = dalloc(b, &b->ptype, np);
     if (rc) { batch_free(b); return rc; }
     if (hipStreamCreate(&b->stream) != hipSuccess) {
         batch_free(b);
@@ -502,6 +519,7 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
         A.trace = b->trace + h.trace0;
         A.thermo = b->thermo + h.thermo0;
         A.vr = b->vr ? b->vr + p0 : nullptr;
+        A.ptype = b->ptype ? b->ptype + p0 : nullptr;
         A.erank = b->erank + 2 * p0;
         A.lrank = b->lrank + p0;
         rc = upload_system(b, s, descs + s);

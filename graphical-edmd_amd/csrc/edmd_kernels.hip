@@ -315,7 +315,7 @@ __device__ __forceinline__ double log_time(const EdmdConst& C, double t) {
  * earlier, as the reference does.  Returns whether a wall time was set. */
 __device__ __forceinline__ bool wall_time(const EdmdConst& C, double x, double y, double vx,
                                           double vy, double r, int cx, int cy, double& dt,
-                                          int& xy) {
+                                          int& xy, int typ = 0) {
     bool wall = false;
     dt = EDMD_NEVER;
     xy = 2;
@@ -365,6 +365,26 @@ __device__ __forceinline__ bool wall_time(const EdmdConst& C, double x, double y
             xy = 0;
         }
     }
+    if ((C.walls & 8) && typ == 1) {
+        /* the mid wall for the solute (EDMD.c:2897-2915): from within 4 of
+         * x = Lx/2, moving towards it */
+        const double h = C.Lx / 2;
+        if (x > h - 4 && x < h && vx > 0) {
+            const double dw = log_time<true>(C, (h - x - r) / vx);
+            if (dt > dw) {
+                wall = true;
+                dt = dw;
+            }
+            xy = 0;
+        } else if (x < h + 4 && x > h && vx < 0) {
+            const double dw = log_time<true>(C, ((x - r) - h) / (-vx));
+            if (dt > dw) {
+                wall = true;
+                dt = dw;
+            }
+            xy = 0;
+        }
+    }
     if (C.walls & 4) {
         /* the circle (EDMD.c:2938-2954, no field): the centre reaches the
          * circle of radius halfLx - r, for every disk; the time replaces
@@ -386,7 +406,8 @@ __device__ __forceinline__ bool wall_time(const EdmdConst& C, double x, double y
  * test sets xy = 0 whenever it applies. */
 __device__ __forceinline__ bool wall_time_grow(const EdmdConst& C, double x, double y, double vx,
                                                double vy, double r, double vr, int cx, int cy,
-                                               double& dt, int& xy) {
+                                               double& dt, int& xy,
+                                               int typ = 0) {
     bool wall = false;
     dt = EDMD_GROW_NONE;
     xy = 2;
@@ -413,6 +434,26 @@ __device__ __forceinline__ bool wall_time_grow(const EdmdConst& C, double x, dou
             xy = 0;
         } else if (cx == C.ncx - 1 && vxp > 0) {
             const double dw = (C.Lx - x - r) / vxp;
+            if (dt > dw) {
+                wall = true;
+                dt = dw;
+            }
+            xy = 0;
+        }
+    }
+    if ((C.walls & 8) && typ == 1) {
+        /* EDMD.c:3175-3195 */
+        const double h = C.Lx / 2;
+        const double vxp = vx + vr, vxm = vx - vr;
+        if (x > h - 4 && x < h && vxp > 0) {
+            const double dw = (h - x - r) / (vxp);
+            if (dt > dw) {
+                wall = true;
+                dt = dw;
+            }
+            xy = 0;
+        } else if (x < h + 4 && x > h && vxm < 0) {
+            const double dw = ((x - r) - h) / (-vxm);
             if (dt > dw) {
                 wall = true;
                 dt = dw;
@@ -692,9 +733,11 @@ __device__ __forceinline__ double predict_seq(const EdmdConst& C, const AR& A, d
     const double ovr = G ? A.vr[p] : 0.0;
     int xy = 2;
     bool wall = false;
-    if (W && !G && C.walls) wall = wall_time(C, o.x, o.y, o.vx, o.vy, o.rad, o.cx, o.cy, dt, xy);
+    const int typ = (W && (C.walls & 8)) ? A.ptype[p] : 0;
+    if (W && !G && C.walls)
+        wall = wall_time(C, o.x, o.y, o.vx, o.vy, o.rad, o.cx, o.cy, dt, xy, typ);
     if (W && G && C.walls)
-        wall = wall_time_grow(C, o.x, o.y, o.vx, o.vy, o.rad, ovr, o.cx, o.cy, dt, xy);
+        wall = wall_time_grow(C, o.x, o.y, o.vx, o.vy, o.rad, ovr, o.cx, o.cy, dt, xy, typ);
     for (int j = -1; j <= 1; ++j) {
         int row = wrap_cell(o.cy + j, C.ncy) * C.ncx;
         for (int k = -1; k <= 1; ++k) {
@@ -1077,8 +1120,9 @@ __device__ __forceinline__ void predict_owners(const EdmdConst& C, const AR& A, 
         /* the wall time starts the reference's minimum: a neighbour
          * replaces it only when strictly earlier */
         auto wt = [&](const Owner& o, double& dw, int& xy) {
-            return G ? wall_time_grow(C, o.x, o.y, o.vx, o.vy, o.r, o.vr, o.cx, o.cy, dw, xy)
-                     : wall_time(C, o.x, o.y, o.vx, o.vy, o.r, o.cx, o.cy, dw, xy);
+            const int typ = (C.walls & 8) ? A.ptype[o.p] : 0;
+            return G ? wall_time_grow(C, o.x, o.y, o.vx, o.vy, o.r, o.vr, o.cx, o.cy, dw, xy, typ)
+                     : wall_time(C, o.x, o.y, o.vx, o.vy, o.r, o.cx, o.cy, dw, xy, typ);
         };
         double dw;
         int xy;
@@ -1555,7 +1599,9 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
         int best = -1;
         int wxy = 2;
         bool wall = false;
-        if (X && C.walls) wall = wall_time(C, ox, oy, ovx, ovy, orad, tx0 + lx, ty0 + ly, dt, wxy);
+        if (X && C.walls)
+            wall = wall_time(C, ox, oy, ovx, ovy, orad, tx0 + lx, ty0 + ly, dt, wxy,
+                             (C.walls & 8) ? A.ptype[p] : 0);
         /* pass 1 without branches: every lane evaluates its k-th candidate;
          * the marks of consecutive k collect in cur and go to their word
          * when k leaves it (k is uniform over the wave) */
@@ -2106,6 +2152,22 @@ __device__ __forceinline__ void ev_thermo(const EdmdConst& C, const AR& A, int l
     double ey = ordered_sum(C.n, lane, [&](int q) { return P[q].m * P[q].vy * P[q].vy; });
     double exy = ordered_sum(C.n, lane, [&](int q) { return P[q].m * P[q].vx * P[q].vy; });
     double eyx = ordered_sum(C.n, lane, [&](int q) { return P[q].m * P[q].vy * P[q].vx; });
+    /* the mid wall's columns (EDMD.c:5453-5477), from the stored disks in
+     * particle order; type-0 disks on the other side add +0 */
+    double nl = 0, nr = 0, ns = 0, tl = 0, tr = 0, tsol = 0;
+    if (C.walls & 8) {
+        const auto T = A.ptype;
+        const double h = C.halfLx;
+        auto temp = [&](int q) {
+            return 0.5 * P[q].m * (P[q].vx * P[q].vx + P[q].vy * P[q].vy);
+        };
+        nl = ordered_sum(C.n, lane, [&](int q) { return (T[q] == 0 && P[q].x < h) ? 1.0 : 0.0; });
+        nr = ordered_sum(C.n, lane, [&](int q) { return (T[q] == 0 && P[q].x > h) ? 1.0 : 0.0; });
+        ns = ordered_sum(C.n, lane, [&](int q) { return T[q] != 0 ? 1.0 : 0.0; });
+        tl = ordered_sum(C.n, lane, [&](int q) { return (T[q] == 0 && P[q].x < h) ? temp(q) : 0.0; });
+        tr = ordered_sum(C.n, lane, [&](int q) { return (T[q] == 0 && P[q].x > h) ? temp(q) : 0.0; });
+        tsol = ordered_sum(C.n, lane, [&](int q) { return T[q] != 0 ? temp(q) : 0.0; });
+    }
     EdmdState* st = A.st;
     /* EDMD.c:5253-5256: the disc's area inside a circular wall */
     const double area = (C.walls & 4) ? 3.14159265358979323846 * C.halfLx * C.halfLx : C.Lx * C.Ly;
@@ -2124,7 +2186,20 @@ __device__ __forceinline__ void ev_thermo(const EdmdConst& C, const AR& A, int l
             row.pyx = -1 * v3 / (area * dT) + eyx / area;
             row.pressure = (-1 / (dT)) * (v0 + v1) / (2 * area) + E / area;
             row.pwall = 0;
-            if (C.walls) {
+            row.pmid = row.dp_lr = row.pack_left = row.pack_right = 0;
+            row.temp_left = row.temp_right = row.temp_solute = 0;
+            if (C.walls & 8) {
+                const double ratio = 3.14159265358979323846 / (C.halfLx * C.Ly);
+                const double sr = C.sizeratio;
+                row.pmid = st->dpm / (dT * C.Ly);
+                row.dp_lr = -(st->dpr - st->dpl) / (dT * C.Ly);
+                row.pack_left = ratio * (double)(int)nl * sr * sr;
+                row.pack_right = ratio * (double)(int)nr * sr * sr;
+                row.temp_left = tl / (int)nl;
+                row.temp_right = tr / (int)nr;
+                row.temp_solute = tsol / (int)ns;
+            }
+            if (C.walls & 7) {
                 /* EDMD.c:5428-5451: dp / (deltaTime * perimeter) */
                 double perimeter = 0;
                 if (C.walls & 4) perimeter = 3.14159265358979323846 * C.Lx;
@@ -2134,7 +2209,8 @@ __device__ __forceinline__ void ev_thermo(const EdmdConst& C, const AR& A, int l
             }
             A.thermo[k] = row;
         }
-        if (C.walls) st->dpw = 0;
+        if (C.walls & 7) st->dpw = 0;
+        if (C.walls & 8) st->dpl = st->dpm = st->dpr = 0;
         st->nthermo = k + 1;
         st->last_thermo = t;
         st->next_thermo = t + C.dtime_thermo;
@@ -2613,8 +2689,19 @@ __device__ __forceinline__ void ev_single(const EdmdConst& C, const AR& A, int l
         /* the tangential product is made +0.0 where it is -0.0 (resT = 0
          * times a negative velocity): the reference's crossing prediction
          * divides by a -0.0 velocity and flies the disk to t = -inf */
+        double dpl = 0, dpm = 0, dpr = 0;
         if (d == 5) {
             dp = fabs((C.res_wall + 1) * m * vx);
+            if (C.walls & 8) {
+                /* EDMD.c:3393-3409: the side walls within 3 of the box's edges */
+                if (vx < 0) {
+                    if (x - 3 < 0) dpl = dp;
+                    else dpm = dp;
+                } else {
+                    if (x + 3 > C.Lx) dpr = dp;
+                    else dpm = dp;
+                }
+            }
             vx = -C.res_wall * vx;
             vy = C.res_tan * vy + 0.0;
         } else if (d == 6) {
@@ -2635,6 +2722,11 @@ __device__ __forceinline__ void ev_single(const EdmdConst& C, const AR& A, int l
         vy = uni(vy);
         if (lane == 0) {
             A.st->dpw += dp;
+            if (C.walls & 8) {
+                A.st->dpl += dpl;
+                A.st->dpm += dpm;
+                A.st->dpr += dpr;
+            }
             const auto q = A.p + p;
             q->x = x;
             q->y = y;
@@ -2887,6 +2979,7 @@ struct EdmdArraysL {
     double EDMD_GLOBAL* dtl;
     double EDMD_GLOBAL* otl;
     EdmdDeriv EDMD_GLOBAL* dr;
+    int32_t EDMD_GLOBAL* ptype;
 };
 
 /* Copies one array between HBM and LDS with the whole wavefront, in 16-,
@@ -2973,6 +3066,7 @@ __global__ __launch_bounds__(64) void k_run_lds(const EdmdSys* __restrict__ syst
     A.dtl = H.dtl;
     A.otl = H.otl;
     A.dr = H.dr;
+    A.ptype = H.ptype;
     WAVE_SYNC();
     lds_system<true, G>(C, H, A, lane);
     WAVE_SYNC();

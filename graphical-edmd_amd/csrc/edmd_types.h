@@ -78,7 +78,9 @@ typedef struct EdmdConst {
     double einit; /* Einit: E/N after the growth stops (EDMD.c:369, 5759-5763) */
     double dtp;   /* dtPaul = 5/N, the reference's bucket width (EDMD.c:712-714) */
     int32_t nlists; /* bucket count of the reference's queue (N) */
-    int32_t walls;  /* bit 0: walls normal to x, bit 1: normal to y, bit 2: the circle */
+    int32_t walls;  /* bit 0: walls normal to x, bit 1: normal to y, bit 2: the circle,
+                       bit 3: the mid wall */
+    double sizeratio; /* the mid wall's packing factor */
     double res_wall, res_tan; /* resW, resT (EDMD.c:372-373) */
     /* minimum-image half widths: halfLx / halfLy, +inf across walls */
     double mihx, mihy;
@@ -128,6 +130,7 @@ typedef struct EdmdState {
     int64_t win_screen, win_thermo, win_noise, win_grow; /* their buckets, -1 = overflow list */
     uint64_t coll_limit;        /* per-system bound on ncol (epochs) */
     double dpw;                 /* momentum given to the walls since the last thermo row */
+    double dpl, dpm, dpr;       /* dpLeft, dpMid, dpRight of the mid wall (EDMD.c:146-148) */
 } EdmdState;
 
 /* LDS-resident runs: byte offsets of a system's arrays in its workgroup's
@@ -202,6 +205,7 @@ typedef struct EdmdThermoRow {
     uint64_t ncol;
     double e_per_n, pressure, pxx, pyy, pxy, pyx;
     double pwall;
+    double pmid, dp_lr, pack_left, pack_right, temp_left, temp_right, temp_solute;
 } EdmdThermoRow;
 
 typedef struct __attribute__((aligned(64))) EdmdPart {
@@ -254,6 +258,7 @@ typedef struct EdmdArrays {
     double EDMD_GLOBAL* dtl;         /* the time each particle's dv refers to */
     double EDMD_GLOBAL* otl;         /* particle local times when taping began */
     EdmdDeriv EDMD_GLOBAL* dr;
+    int32_t EDMD_GLOBAL* ptype;      /* disk types (mid wall) */
 } EdmdArrays;
 
 typedef struct EdmdSys {

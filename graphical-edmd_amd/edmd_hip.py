@@ -45,7 +45,8 @@ class SystemDesc(C.Structure):
                 ("thermo_capacity", C.c_int), ("vr", C.c_double), ("einit", C.c_double),
                 ("wall_x", C.c_int), ("wall_y", C.c_int), ("res_wall", C.c_double),
                 ("res_tan", C.c_double), ("add_field", C.c_int), ("field", C.c_double),
-                ("damping", C.c_int), ("gamma", C.c_double), ("wall_circle", C.c_int)]
+                ("damping", C.c_int), ("gamma", C.c_double), ("wall_circle", C.c_int),
+                ("mid_wall", C.c_int), ("type", C.c_void_p), ("sizeratio", C.c_double)]
 
 
 class Observables(C.Structure):
@@ -63,7 +64,9 @@ TRACE_DT = np.dtype([("type", "<i4"), ("i", "<i4"), ("j", "<i4"), ("valid", "<i4
                      ("t", "<f8")])
 THERMO_DT = np.dtype([("t", "<f8"), ("ncol", "<u8"), ("e_per_n", "<f8"), ("pressure", "<f8"),
                       ("pxx", "<f8"), ("pyy", "<f8"), ("pxy", "<f8"), ("pyx", "<f8"),
-                      ("pwall", "<f8")])
+                      ("pwall", "<f8"), ("pmid", "<f8"), ("dp_lr", "<f8"), ("pack_left", "<f8"),
+                      ("pack_right", "<f8"), ("temp_left", "<f8"), ("temp_right", "<f8"),
+                      ("temp_solute", "<f8")])
 
 EPOCH_DT = np.dtype([("t", "<f8"), ("ncol", "<u8"), ("ncross", "<u8"), ("nevents", "<u8"),
                      ("status", "<i4"), ("stop_reason", "<i4")])
@@ -197,6 +200,10 @@ def make_desc(st, keep, **kw):
     d.Ly = float(st["Ly"])
     for k, a in arrs.items():
         setattr(d, k, a.ctypes.data)
+    if "type" in st:
+        ty = np.ascontiguousarray(st["type"], dtype=np.int32)
+        keep.append(ty)
+        d.type = ty.ctypes.data
     for k, v in kw.items():
         setattr(d, k, v)
     return d

@@ -133,6 +133,18 @@ typedef struct edmd_system_desc {
      * inside the circle.  Not with the field (the quartic of quartic.c) or
      * taping. */
     int wall_circle;
+    /* Mid wall (addMidWall of the graphics build, EDMD.c:269): a wall at
+     * x = Lx/2 for the disks of type 1 only (the solute; type 0, the
+     * solvent, passes through), met from within 4 of it (EDMD.c:2897-2915;
+     * with growth, at v +/- vr, EDMD.c:3175-3195); a hit is an x-wall hit
+     * whose momentum also goes to dpLeft, dpMid or dpRight by the disk's
+     * position (EDMD.c:3393-3409); thermo rows carry the mid-wall columns
+     * of EDMD.c:5453-5477.  type: one int per disk (NULL: all 0), copied;
+     * sizeratio: the solvent's packing factor in those columns (EDMD.c:83,
+     * default 0.4). */
+    int mid_wall;
+    const int32_t* type;
+    double sizeratio;
 } edmd_system_desc;
 
 /* A collision slot holding a wall event reports partner -5 - xy
@@ -149,6 +161,11 @@ typedef struct edmd_thermo_row {
     uint64_t ncol;
     double e_per_n, pressure, pxx, pyy, pxy, pyx; /* saveThermo, EDMD.c:5232-5333 */
     double pwall; /* wall pressure dp / (dt * perimeter), EDMD.c:5428-5451; 0 without walls */
+    /* mid wall (EDMD.c:5453-5477), 0 without it: dpMid / (dt Ly),
+     * -(dpRight - dpLeft) / (dt Ly), the solvent's packing pi n sizeratio^2
+     * / (Lx/2 Ly) left and right of x = Lx/2, its mean kinetic energy left
+     * and right, the solute's */
+    double pmid, dp_lr, pack_left, pack_right, temp_left, temp_right, temp_solute;
 } edmd_thermo_row;
 
 typedef struct edmd_observables {

@@ -61,6 +61,8 @@ struct orc_ctx {
     double* vr;
     int growing;
     double dp; /* momentum given to the walls since the last thermo row */
+    int* type; /* disk types (the mid wall stops type 1) */
+    double dp_left, dp_mid, dp_right; /* dpLeft, dpMid, dpRight (EDMD.c:146-148) */
 };
 
 /* ------------------------------------------------------------ kinematics */
@@ -513,6 +515,26 @@ static void predict_collision_grow(orc_ctx* s, int i, double* t_abs, int* partne
             xy = 0;
         }
     }
+    if (s->prm.mid_wall && s->type[i] == 1) {
+        /* EDMD.c:3175-3195 */
+        double xi = s->x[i], h = s->Lx / 2;
+        double vxp = s->vx[i] + vr, vxm = s->vx[i] - vr;
+        if (xi > h - 4 && xi < h && vxp > 0) {
+            double dtw = (h - xi - s->rad[i]) / (vxp);
+            if (dt > dtw) {
+                wall = 1;
+                dt = dtw;
+            }
+            xy = 0;
+        } else if (xi < h + 4 && xi > h && vxm < 0) {
+            double dtw = ((xi - s->rad[i]) - h) / (-vxm);
+            if (dt > dtw) {
+                wall = 1;
+                dt = dtw;
+            }
+            xy = 0;
+        }
+    }
     if (s->prm.wall_circle) {
         /* EDMD.c:3158-3172: the surface closes on the circle at |v| - vr */
         double y0 = s->y[i] - s->halfLx, x0 = s->x[i] - s->halfLx;
@@ -596,6 +618,25 @@ static void predict_collision(orc_ctx* s, int i, double* t_abs, int* partner) {
             xy = 0;
         } else if (X == s->ncx - 1 && s->vx[i] > 0) {
             double dtw = log_time(s, (s->Lx - s->x[i] - s->rad[i]) / s->vx[i]);
+            if (dt > dtw) {
+                wall = 1;
+                dt = dtw;
+            }
+            xy = 0;
+        }
+    }
+    if (s->prm.mid_wall && s->type[i] == 1) {
+        /* EDMD.c:2897-2915: from within 4 of x = Lx/2, moving towards it */
+        double xi = s->x[i], vx = s->vx[i], h = s->Lx / 2;
+        if (xi > h - 4 && xi < h && vx > 0) {
+            double dtw = log_time(s, (h - xi - s->rad[i]) / vx);
+            if (dt > dtw) {
+                wall = 1;
+                dt = dtw;
+            }
+            xy = 0;
+        } else if (xi < h + 4 && xi > h && vx < 0) {
+            double dtw = log_time(s, ((xi - s->rad[i]) - h) / (-vx));
             if (dt > dtw) {
                 wall = 1;
                 dt = dtw;
@@ -918,6 +959,17 @@ static void do_wall(orc_ctx* s, const ev_node* ev) {
     double rw = s->prm.res_wall, rt = s->prm.res_tan;
     if (xy == 0) {
         s->dp += fabs((rw + 1) * s->m[i] * s->vx[i]);
+        if (s->prm.mid_wall) {
+            /* EDMD.c:3393-3409: the side walls within 3 of the box's edges */
+            double dpi = fabs((rw + 1) * s->m[i] * s->vx[i]);
+            if (s->vx[i] < 0) {
+                if (s->x[i] - 3 < 0) s->dp_left += dpi;
+                else s->dp_mid += dpi;
+            } else {
+                if (s->x[i] + 3 > s->Lx) s->dp_right += dpi;
+                else s->dp_mid += dpi;
+            }
+        }
         s->vx[i] = -rw * s->vx[i];
         s->vy[i] = rt * s->vy[i] + 0.0;
     } else if (xy == 1) {
@@ -973,6 +1025,34 @@ static void do_thermo(orc_ctx* s) {
     row.pressure = (-1 / (dT)) * (s->vir_xx + s->vir_yy) / (2 * area) + s->E / area;
     s->vir_xx = s->vir_yy = s->vir_xy = s->vir_yx = 0;
     row.pwall = 0;
+    row.pmid = row.dp_lr = row.pack_left = row.pack_right = 0;
+    row.temp_left = row.temp_right = row.temp_solute = 0;
+    if (s->prm.mid_wall) {
+        int nl = 0, nr = 0, ns = 0;
+        double tsol = 0, tl = 0, tr = 0;
+        for (int i = 0; i < s->n; i++) {
+            if (s->type[i] == 0) {
+                nl += (s->x[i] < s->halfLx) ? 1 : 0;
+                nr += (s->x[i] > s->halfLx) ? 1 : 0;
+                double temp = 0.5 * s->m[i] * (s->vx[i] * s->vx[i] + s->vy[i] * s->vy[i]);
+                tl += (s->x[i] < s->halfLx) ? temp : 0;
+                tr += (s->x[i] > s->halfLx) ? temp : 0;
+            } else {
+                ns++;
+                tsol += 0.5 * s->m[i] * (s->vx[i] * s->vx[i] + s->vy[i] * s->vy[i]);
+            }
+        }
+        double ratio = M_PI / (s->halfLx * s->Ly);
+        double sr = s->prm.sizeratio;
+        row.pmid = s->dp_mid / (dT * s->Ly);
+        row.dp_lr = -(s->dp_right - s->dp_left) / (dT * s->Ly);
+        row.pack_left = ratio * nl * sr * sr;
+        row.pack_right = ratio * nr * sr * sr;
+        row.temp_left = tl / nl;
+        row.temp_right = tr / nr;
+        row.temp_solute = tsol / ns;
+        s->dp_mid = s->dp_right = s->dp_left = 0;
+    }
     if (s->prm.wall_x || s->prm.wall_y || s->prm.wall_circle) {
         double perimeter = 0;
         if (s->prm.wall_circle) perimeter = M_PI * s->Lx;
@@ -1118,6 +1198,9 @@ void orc_default_params(orc_params* p) {
     p->damping = 0;
     p->gamma = 0.02; /* gamm, EDMD.c:382 */
     p->wall_circle = 0;
+    p->mid_wall = 0;
+    p->type = NULL;
+    p->sizeratio = 0.4;
 }
 
 orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* y,
@@ -1167,9 +1250,10 @@ orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* 
     s->ev = calloc(2 * nd + 10, sizeof(ev_node));
     s->lists = malloc(((size_t)s->n_lists + 1) * sizeof(int));
     s->vr = calloc(nd, sizeof(double));
+    s->type = calloc(nd, sizeof(int));
     if (!s->x || !s->y || !s->vx || !s->vy || !s->tl || !s->rad || !s->m || !s->coll ||
         !s->cx || !s->cy || !s->crossX || !s->crossY || !s->head || !s->nxt || !s->prv ||
-        !s->ev || !s->lists || !s->vr) {
+        !s->ev || !s->lists || !s->vr || !s->type) {
         orc_destroy(s);
         return NULL;
     }
@@ -1181,7 +1265,9 @@ orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* 
         s->vy[i] = vy[i];
         s->rad[i] = rad[i];
         s->m[i] = m[i];
+        s->type[i] = s->prm.type ? s->prm.type[i] : 0;
     }
+    s->prm.type = NULL; /* copied */
     if (s->prm.vr > 0) {
         /* growth start (EDMD.c:1770-1773): radius 0, rate vr * target */
         s->growing = 1;
@@ -1206,7 +1292,7 @@ void orc_destroy(orc_ctx* s) {
     free(s->x); free(s->y); free(s->vx); free(s->vy); free(s->tl); free(s->rad);
     free(s->m); free(s->coll); free(s->cx); free(s->cy); free(s->crossX);
     free(s->crossY); free(s->head); free(s->nxt); free(s->prv); free(s->ev);
-    free(s->lists); free(s->rows); free(s->tape); free(s->vr);
+    free(s->lists); free(s->rows); free(s->tape); free(s->vr); free(s->type);
     free(s);
 }
 

@@ -87,6 +87,17 @@ typedef struct {
      * of the thermo row is pi halfLx^2 and the wall perimeter pi Lx.  Not
      * with the field (whose circle needs the quartic of quartic.c). */
     int wall_circle;
+    /* Mid wall (addMidWall, EDMD.c:269): a wall at x = Lx/2 for the disks
+     * of type 1 only (the solute; type 0, the solvent, passes), met from
+     * within 4 of it (EDMD.c:2897-2915, growth EDMD.c:3175-3195); a hit is
+     * an x-wall hit whose momentum also goes to dpLeft / dpMid / dpRight by
+     * the disk's position (EDMD.c:3393-3409); thermo rows add pmid,
+     * -(dpRight - dpLeft), the packing of the solvent on each side (with
+     * sizeratio) and the solvent's and solute's kinetic energies
+     * (EDMD.c:5453-5477).  type: one int per disk (NULL: all 0). */
+    int mid_wall;
+    const int32_t* type;
+    double sizeratio; /* EDMD.c:83, default 0.4 */
 } orc_params;
 
 typedef struct {
@@ -104,6 +115,9 @@ typedef struct {
     double pressure;
     double pxx, pyy, pxy, pyx;
     double pwall; /* wall pressure dp / (dt * perimeter), 0 without walls (EDMD.c:5428-5451) */
+    /* mid wall columns (EDMD.c:5453-5477), 0 without it: pmid, dp, packLeft,
+     * packRight, tempSolventLeft, tempSolventRight, tempSolute */
+    double pmid, dp_lr, pack_left, pack_right, temp_left, temp_right, temp_solute;
 } orc_thermo_row;
 
 typedef struct orc_ctx orc_ctx;

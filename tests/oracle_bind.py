@@ -24,13 +24,17 @@ class Params(C.Structure):
                 ("bucket_count", C.c_int), ("vr", C.c_double), ("einit", C.c_double),
                 ("wall_x", C.c_int), ("wall_y", C.c_int), ("res_wall", C.c_double),
                 ("res_tan", C.c_double), ("add_field", C.c_int), ("field", C.c_double),
-                ("damping", C.c_int), ("gamma", C.c_double), ("wall_circle", C.c_int)]
+                ("damping", C.c_int), ("gamma", C.c_double), ("wall_circle", C.c_int),
+                ("mid_wall", C.c_int), ("type", C.c_void_p), ("sizeratio", C.c_double)]
 
 
 class ThermoRow(C.Structure):
     _fields_ = [("t", C.c_double), ("ncol", C.c_uint64), ("e_per_n", C.c_double),
                 ("pressure", C.c_double), ("pxx", C.c_double), ("pyy", C.c_double),
-                ("pxy", C.c_double), ("pyx", C.c_double), ("pwall", C.c_double)]
+                ("pxy", C.c_double), ("pyx", C.c_double), ("pwall", C.c_double),
+                ("pmid", C.c_double), ("dp_lr", C.c_double), ("pack_left", C.c_double),
+                ("pack_right", C.c_double), ("temp_left", C.c_double),
+                ("temp_right", C.c_double), ("temp_solute", C.c_double)]
 
 
 TRACE_DT = np.dtype([("type", "<i4"), ("i", "<i4"), ("j", "<i4"),
@@ -187,6 +191,9 @@ class Oracle:
         self._keep = {k: np.ascontiguousarray(st[k], dtype=np.float64)
                       for k in ("x", "y", "vx", "vy", "rad", "m")}
         self.n = len(self._keep["x"])
+        if "type" in st:
+            self._type = np.ascontiguousarray(st["type"], dtype=np.int32)
+            params = dict(params, type=self._type.ctypes.data)
         self.params = default_params(**params)
         k = self._keep
         h = lib().orc_create(self.n, st["Lx"], st["Ly"], _dp(k["x"]), _dp(k["y"]),
@@ -273,7 +280,10 @@ class Oracle:
         arr = (ThermoRow * max(n, 1))()
         lib().orc_thermo_rows(self.h, arr, n)
         return [dict(t=r.t, ncol=r.ncol, e_per_n=r.e_per_n, pressure=r.pressure,
-                     pxx=r.pxx, pyy=r.pyy, pxy=r.pxy, pyx=r.pyx, pwall=r.pwall)
+                     pxx=r.pxx, pyy=r.pyy, pxy=r.pxy, pyx=r.pyx, pwall=r.pwall,
+                     pmid=r.pmid, dp_lr=r.dp_lr, pack_left=r.pack_left,
+                     pack_right=r.pack_right, temp_left=r.temp_left,
+                     temp_right=r.temp_right, temp_solute=r.temp_solute)
                 for r in arr[:n]]
 
     def collision_time(self, i, j):

@@ -365,3 +365,67 @@ def test_circular_wall_arguments_are_checked(gpu):
         gpu.System(st, wall_circle=1, add_field=1)
     with pytest.raises(gpu.EdmdError):
         gpu.System(st, wall_circle=2)
+
+
+# ---- the mid wall (addMidWall)
+
+def osmosis(n, L, seed, poly=False):
+    """Solvent (type 0) everywhere and solute (type 1, a third of the disks
+    left of x = L/2) in a closed box, non-overlapping at radius 0.5."""
+    rng = np.random.default_rng(seed)
+    xs, ys = [], []
+    while len(xs) < n:
+        x, y = rng.uniform(1.5, L - 1.5), rng.uniform(1.5, L - 1.5)
+        if abs(x - L / 2) > 1 and all((x - a) ** 2 + (y - b) ** 2 > 1.2 ** 2
+                                      for a, b in zip(xs, ys)):
+            xs.append(x)
+            ys.append(y)
+    xs, ys = np.array(xs), np.array(ys)
+    typ = ((xs < L / 2) & (np.arange(n) % 3 == 0)).astype(np.int32)
+    v = rng.standard_normal((n, 2))
+    return {"x": xs, "y": ys, "vx": v[:, 0].copy(), "vy": v[:, 1].copy(),
+            "rad": np.full(n, 0.5), "m": np.ones(n), "Lx": L, "Ly": L, "type": typ}
+
+
+MID_CASES = [
+    (300, 40.0, 91, 0.0, dict(wall_x=1, wall_y=1, res_tan=1.0)),
+    (260, 36.0, 92, 0.0, dict(wall_x=1, wall_y=1, res_wall=0.95, damping=1, gamma=0.01)),
+    (280, 40.0, 93, 0.1, dict(wall_x=1, wall_y=1, res_tan=1.0)),   # growth start
+]
+
+
+@pytest.mark.parametrize("n,L,seed,vr,kw", MID_CASES)
+def test_mid_wall_matches_oracle(gpu, oracle, n, L, seed, vr, kw):
+    """The solute stopped at x = Lx/2, the solvent passing, the hits' split
+    into dpLeft / dpMid / dpRight and the mid-wall thermo columns: traces,
+    states and every thermo column bit for bit against the oracle, from HBM
+    and from LDS, with growth through GROWSTOP."""
+    st = osmosis(n, L, seed)
+    kw = dict(kw, mid_wall=1, sizeratio=0.5, first_screen=1e9, dtime=1e9, first_thermo=1.0,
+              dtime_thermo=1.0)
+    nev = 20000
+    o = oracle.Oracle(st, vr=vr, **kw)
+    code, tr = o.run(nev)
+    assert code == 0
+    so, orows = o.state(), o.thermo_rows()
+    assert np.all(so["x"][st["type"] == 1] < L / 2)
+    assert len(orows) > 3 and sum(r["pmid"] > 0 for r in orows) >= 2
+    for mode in (gpu.RES_HBM, gpu.RES_LDS):
+        b = gpu.Batch([st], trace_capacity=nev, thermo_capacity=1024, vr=vr, **kw)
+        b.set_residency(mode)
+        b.run(max_events=nev)
+        # mirrored pair events compared unordered: slow damped disks file
+        # collisions beyond the reference's bucket ring, whose overflow list
+        # can pop the other orientation first (the calendar here does not
+        # reorder them); the states stay bitwise
+        g, c = F.canonical_trace(b.trace(0)), F.canonical_trace(tr)
+        for k in ("type", "i", "j", "valid"):
+            np.testing.assert_array_equal(g[k], c[k], err_msg=k)
+        np.testing.assert_array_equal(bits(g["t"]), bits(c["t"]))
+        assert_state_equal(b.state(0), so)
+        rows = b.thermo(0)
+        assert len(rows) == len(orows)
+        for r, q in zip(rows, orows):
+            for k in ("t", "e_per_n", "pressure", "pwall", "pmid", "dp_lr", "pack_left",
+                      "pack_right", "temp_left", "temp_right", "temp_solute"):
+                assert bits(np.float64(r[k])) == bits(np.float64(q[k])), k

@@ -437,3 +437,78 @@ def test_circular_wall_under_growth_and_growstop(oracle):
     E = 0.5 * np.sum(st["m"] * (s["vx"] ** 2 + s["vy"] ** 2))
     assert abs(Lz) < 1e-10 * np.sum(st["m"] * np.hypot(s["vx"], s["vy"]) * c)
     assert E / len(s["x"]) == pytest.approx(1.0, rel=1e-12)
+
+
+# ---- the mid wall (addMidWall, EDMD.c:2897-2915, 3175-3195, 3393-3409,
+# 5453-5477)
+
+def test_mid_wall_closed_form(oracle):
+    """x = Lx/2 = 10 stops the disks of type 1 met within 4 of it: from
+    x = 8 moving at +1 a type-1 disk (radius 0.5) hits at t = 1.5 and comes
+    back; from x = 12 moving at -1, also at 1.5; a type-0 disk on the same
+    path passes; growing type-1 disks close on it at v + vr."""
+    st = disks([(8.0, 5.0), (12.0, 15.0), (8.0, 10.0), (3.0, 18.0)],
+               [(1.0, 0.0), (-1.0, 0.0), (1.0, 0.0), (0.0, 0.01)])
+    st["type"] = np.array([1, 1, 0, 0])
+    o = oracle.Oracle(st, mid_wall=1, **QUIET)
+    for i in (0, 1):
+        t, p = o.predict_collision(i)
+        assert p == -5 and t == pytest.approx(1.5, rel=1e-15), i
+    t, p = o.predict_collision(2)
+    assert p >= 0 or t > 1e20
+    code, tr = o.run(12)
+    assert code == 0 and o.t > 3.0
+    walls = tr["i"][tr["type"] == WALL]
+    assert sorted(walls[:2]) == [0, 1] and 2 not in walls
+    oracle.lib().orc_sync_all(o.h, o.t)
+    s = o.state()
+    assert s["vx"][0] == -1.0 and s["vx"][1] == 1.0 and s["vx"][2] == 1.0
+    assert s["x"][2] == pytest.approx(8.0 + o.t, rel=1e-14)
+    # growth: surface speed v + vr (vr = 0.1 * 0.5)
+    o = oracle.Oracle(st, mid_wall=1, vr=0.1, **QUIET)
+    t, p = o.predict_collision(0)
+    assert p == -5 and t == pytest.approx(2.0 / 1.05, rel=1e-14)
+
+
+def test_mid_wall_thermo_columns(oracle):
+    """A channel with walls at x = 0, Lx and the mid wall: the solute
+    (type 1, big disks) starts left and stays there; every x-wall hit goes
+    to dp, and to dpLeft (within 3 of x = 0), dpMid or dpRight; each thermo
+    row carries pmid = dpMid / (dt Ly), -(dpRight - dpLeft) / (dt Ly), the
+    solvent's packing pi n sr^2 / (Lx/2 Ly) on each side and the mean
+    kinetic energies, taken from the stored disks at the row's time."""
+    rng = np.random.default_rng(12)
+    L, n = 40.0, 300
+    xs, ys = [], []
+    while len(xs) < n:
+        x, y = rng.uniform(1.5, L - 1.5), rng.uniform(1.5, L - 1.5)
+        if all((x - a) ** 2 + (y - b) ** 2 > 1.2 ** 2 for a, b in zip(xs, ys)) and abs(x - 20) > 1:
+            xs.append(x)
+            ys.append(y)
+    xs, ys = np.array(xs), np.array(ys)
+    typ = ((xs < 20) & (np.arange(n) % 3 == 0)).astype(np.int32)
+    v = rng.standard_normal((n, 2))
+    st = {"x": xs, "y": ys, "vx": v[:, 0].copy(), "vy": v[:, 1].copy(),
+          "rad": np.full(n, 0.5), "m": np.ones(n), "Lx": L, "Ly": L, "type": typ}
+    kw = dict(mid_wall=1, wall_x=1, wall_y=1, res_tan=1.0, sizeratio=0.5,
+              first_screen=1e9, dtime=1e9, first_thermo=2.0, dtime_thermo=2.0)
+    o = oracle.Oracle(st, **kw)
+    o.run_until(10.5)
+    rows = o.thermo_rows()
+    assert len(rows) == 5
+    assert all(r["pwall"] > 0 for r in rows) and sum(r["pmid"] > 0 for r in rows) >= 3
+    s = o.state()
+    assert np.all(s["x"][typ == 1] < 20)
+    o2 = oracle.Oracle(st, **kw)
+    o2.run_until(2.0)
+    s = o2.state()
+    r = o2.thermo_rows()[0]
+    solv = typ == 0
+    e = 0.5 * (s["vx"] ** 2 + s["vy"] ** 2)
+    nl, nr = np.sum(solv & (s["x"] < 20)), np.sum(solv & (s["x"] > 20))
+    ratio = np.pi / (20.0 * L)
+    assert r["pack_left"] == pytest.approx(ratio * nl * 0.25, rel=1e-14)
+    assert r["pack_right"] == pytest.approx(ratio * nr * 0.25, rel=1e-14)
+    assert r["temp_left"] == pytest.approx(np.sum(e[solv & (s["x"] < 20)]) / nl, rel=1e-12)
+    assert r["temp_right"] == pytest.approx(np.sum(e[solv & (s["x"] > 20)]) / nr, rel=1e-12)
+    assert r["temp_solute"] == pytest.approx(np.mean(e[typ == 1]), rel=1e-12)
