@@ -260,8 +260,6 @@ static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
     c.gamma = d->gamma;
     if (d->wall_circle != 0 && d->wall_circle != 1)
         return fail(EDMD_ERR_ARG, "system %d: wall_circle must be 0 or 1", idx);
-    if (d->wall_circle && d->add_field)
-        return fail(EDMD_ERR_ARG, "system %d: a circular wall goes without the field", idx);
     if (d->mid_wall != 0 && d->mid_wall != 1)
         return fail(EDMD_ERR_ARG, "system %d: mid_wall must be 0 or 1", idx);
     c.walls = (d->wall_x ? 1 : 0) | (d->wall_y ? 2 : 0) | (d->wall_circle ? 4 : 0) |

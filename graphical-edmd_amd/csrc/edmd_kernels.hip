@@ -296,6 +296,115 @@ __device__ __forceinline__ double edmd_atan2(double y, double x) {
     return __builtin_signbit(y) ? -a : a;
 }
 
+/* The quartic of the circular wall under the field (quartic.c:27-158):
+ * Ferrari's method through the cubic resolvent, step for step as there,
+ * with this library's own acos (atan2 of sqrt((1 - t)(1 + t)) and t), cos
+ * and cube root (exp(log(x) / 3), the pow with exponent 1./3).  The
+ * oracle runs the same steps. */
+__device__ __forceinline__ double edmd_cosine(double th) {
+    double sn, cs;
+    edmd_sincos(th, sn, cs);
+    return cs;
+}
+
+__device__ __forceinline__ unsigned edmd_solve_p3(double* x, double a, double b, double c) {
+    const double a2 = a * a;
+    double q = (a2 - 3 * b) / 9;
+    const double r = (a * (2 * a2 - 9 * b) + 27 * c) / 54;
+    const double r2 = r * r;
+    const double q3 = q * q * q;
+    if (r2 < q3) {
+        double t = r / sqrt(q3);
+        if (t < -1) t = -1;
+        if (t > 1) t = 1;
+        t = edmd_atan2(sqrt((1 - t) * (1 + t)), t);
+        a /= 3;
+        q = -2 * sqrt(q);
+        x[0] = q * edmd_cosine(t / 3) - a;
+        x[1] = q * edmd_cosine((t + (2.0 * 3.14159265358979323846)) / 3) - a;
+        x[2] = q * edmd_cosine((t - (2.0 * 3.14159265358979323846)) / 3) - a;
+        return 3;
+    }
+    const double m = fabs(r) + sqrt(r2 - q3);
+    double A = -(m == 0 ? 0.0 : edmd_exp(edmd_log(m) * (1. / 3)));
+    if (r < 0) A = -A;
+    const double B = (0 == A ? 0 : q / A);
+    a /= 3;
+    x[0] = (A + B) - a;
+    x[1] = -0.5 * (A + B) - a;
+    x[2] = 0.5 * sqrt(3.) * (A - B);
+    if (fabs(x[2]) < 1e-12) {
+        x[2] = x[1];
+        return 2;
+    }
+    return 1;
+}
+
+/* The smallest real root above 1e-9 of a t^4 + b t^3 + c t^2 + d t + e,
+ * 1e10 when there is none (smallestRoot, quartic.c:143-157). */
+__device__ __forceinline__ double edmd_smallest_root(double a, double b, double c, double d,
+                                                     double e) {
+    const double qa = b / a, qb = c / a, qc = d / a, qd = e / a;
+    const double a3 = -qb;
+    const double b3 = qa * qc - 4. * qd;
+    const double c3 = -qa * qa * qd - qc * qc + 4. * qb * qd;
+    double x3[3];
+    const unsigned zeroes = edmd_solve_p3(x3, a3, b3, c3);
+    double q1, q2, p1, p2, D, sqD, y = x3[0];
+    if (zeroes != 1) {
+        if (fabs(x3[1]) > fabs(y)) y = x3[1];
+        if (fabs(x3[2]) > fabs(y)) y = x3[2];
+    }
+    D = y * y - 4 * qd;
+    if (fabs(D) < 1e-12) {
+        q1 = q2 = y * 0.5;
+        D = qa * qa - 4 * (qb - y);
+        if (fabs(D) < 1e-12) {
+            p1 = p2 = qa * 0.5;
+        } else {
+            sqD = sqrt(D);
+            p1 = (qa + sqD) * 0.5;
+            p2 = (qa - sqD) * 0.5;
+        }
+    } else {
+        sqD = sqrt(D);
+        q1 = (y + sqD) * 0.5;
+        q2 = (y - sqD) * 0.5;
+        p1 = (qa * q1 - qc) / (q1 - q2);
+        p2 = (qc - qa * q2) / (q1 - q2);
+    }
+    double re[4], im[4];
+    D = p1 * p1 - 4 * q1;
+    if (D < 0.0) {
+        re[0] = -p1 * 0.5;
+        im[0] = sqrt(-D) * 0.5;
+        re[1] = re[0];
+        im[1] = -im[0];
+    } else {
+        sqD = sqrt(D);
+        re[0] = (-p1 + sqD) * 0.5;
+        re[1] = (-p1 - sqD) * 0.5;
+        im[0] = im[1] = 0.0;
+    }
+    D = p2 * p2 - 4 * q2;
+    if (D < 0.0) {
+        re[2] = -p2 * 0.5;
+        im[2] = sqrt(-D) * 0.5;
+        re[3] = re[2];
+        im[3] = -im[2];
+    } else {
+        sqD = sqrt(D);
+        re[2] = (-p2 + sqD) * 0.5;
+        re[3] = (-p2 - sqD) * 0.5;
+        im[2] = im[3] = 0.0;
+    }
+    double best = 10000000000;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        if (im[i] == 0 && re[i] > 0.000000001 && re[i] < best) best = re[i];
+    return best;
+}
+
 /* logTime, EDMD.c:5709-5721: under damping, the time at which the damped
  * motion has covered what the undamped one covers in t; never when it
  * cannot.  F = false compiles damping out. */
@@ -385,7 +494,21 @@ __device__ __forceinline__ bool wall_time(const EdmdConst& C, double x, double y
             xy = 0;
         }
     }
-    if (C.walls & 4) {
+    if ((C.walls & 4) && C.field_on) {
+        /* the circle under the field (EDMD.c:2918-2935): the parabolic
+         * flight meets it at the quartic's smallest positive root (y0 about
+         * halfLy here) */
+        const double y0 = y - C.halfLy, x0 = x - C.halfLx;
+        const double e = x0 * x0 + y0 * y0 - (C.halfLx - r) * (C.halfLx - r);
+        const double f = C.field;
+        const double a = f * f / 4;
+        const double b = f * vy;
+        const double c = vx * vx + vy * vy + f * y0;
+        const double d = 2 * vx * x0 + 2 * vy * y0;
+        dt = edmd_smallest_root(a, b, c, d, e);
+        wall = true;
+        xy = 2;
+    } else if (C.walls & 4) {
         /* the circle (EDMD.c:2938-2954, no field): the centre reaches the
          * circle of radius halfLx - r, for every disk; the time replaces
          * the other walls' */

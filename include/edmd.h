@@ -129,9 +129,11 @@ typedef struct edmd_system_desc {
      * of the C library's, the same steps on the CPU checker).  No minimum
      * image; thermo rows take the area pi (Lx/2)^2 and the wall perimeter
      * pi Lx; at GROWSTOP the angular momentum about the centre is removed
-     * instead of the momentum (EDMD.c:5723-5764).  The disks must start
-     * inside the circle.  Not with the field (the quartic of quartic.c) or
-     * taping. */
+     * instead of the momentum (EDMD.c:5723-5764).  Under the field the time
+     * is the smallest positive root of the quartic of the parabolic flight
+     * (EDMD.c:2918-2935, quartic.c's method with the library's own acos,
+     * cos and cube root).  The disks must start inside the circle.  Not
+     * with taping. */
     int wall_circle;
     /* Mid wall (addMidWall of the graphics build, EDMD.c:269): a wall at
      * x = Lx/2 for the disks of type 1 only (the solute; type 0, the

@@ -203,6 +203,116 @@ double orc_atan2(double y, double x) {
     return signbit(y) ? -a : a;
 }
 
+/* The quartic of the circular wall under the field (quartic.c:27-158, after
+ * github.com/sasamil/Quartic): Ferrari's method through the cubic
+ * resolvent, step for step as there, with this oracle's own acos (atan2 of
+ * sqrt((1 - t)(1 + t)) and t), cos and cube root (exp(log(x) / 3), the pow
+ * with exponent 1./3) in place of the C library's.  Same steps on the GPU. */
+#define ORC_QEPS 1e-12
+static double orc_acos(double t) { return orc_atan2(sqrt((1 - t) * (1 + t)), t); }
+static double orc_cosine(double th) {
+    double sn, cs;
+    orc_sincos(th, &sn, &cs);
+    return cs;
+}
+static double orc_cbrt_pos(double x) { return x == 0 ? 0.0 : orc_exp(orc_log(x) * (1. / 3)); }
+
+static unsigned orc_solve_p3(double* x, double a, double b, double c) {
+    double a2 = a * a;
+    double q = (a2 - 3 * b) / 9;
+    double r = (a * (2 * a2 - 9 * b) + 27 * c) / 54;
+    double r2 = r * r;
+    double q3 = q * q * q;
+    double A, B;
+    if (r2 < q3) {
+        double t = r / sqrt(q3);
+        if (t < -1) t = -1;
+        if (t > 1) t = 1;
+        t = orc_acos(t);
+        a /= 3;
+        q = -2 * sqrt(q);
+        x[0] = q * orc_cosine(t / 3) - a;
+        x[1] = q * orc_cosine((t + (2.0 * M_PI)) / 3) - a;
+        x[2] = q * orc_cosine((t - (2.0 * M_PI)) / 3) - a;
+        return 3;
+    }
+    A = -orc_cbrt_pos(fabs(r) + sqrt(r2 - q3));
+    if (r < 0) A = -A;
+    B = (0 == A ? 0 : q / A);
+    a /= 3;
+    x[0] = (A + B) - a;
+    x[1] = -0.5 * (A + B) - a;
+    x[2] = 0.5 * sqrt(3.) * (A - B);
+    if (fabs(x[2]) < ORC_QEPS) {
+        x[2] = x[1];
+        return 2;
+    }
+    return 1;
+}
+
+/* The smallest real root above 1e-9 of a t^4 + b t^3 + c t^2 + d t + e,
+ * 1e10 when there is none (smallestRoot, quartic.c:143-157). */
+double orc_smallest_root(double a, double b, double c, double d, double e) {
+    const double qa = b / a, qb = c / a, qc = d / a, qd = e / a;
+    double a3 = -qb;
+    double b3 = qa * qc - 4. * qd;
+    double c3 = -qa * qa * qd - qc * qc + 4. * qb * qd;
+    double x3[3];
+    unsigned zeroes = orc_solve_p3(x3, a3, b3, c3);
+    double q1, q2, p1, p2, D, sqD, y = x3[0];
+    if (zeroes != 1) {
+        if (fabs(x3[1]) > fabs(y)) y = x3[1];
+        if (fabs(x3[2]) > fabs(y)) y = x3[2];
+    }
+    D = y * y - 4 * qd;
+    if (fabs(D) < ORC_QEPS) {
+        q1 = q2 = y * 0.5;
+        D = qa * qa - 4 * (qb - y);
+        if (fabs(D) < ORC_QEPS) {
+            p1 = p2 = qa * 0.5;
+        } else {
+            sqD = sqrt(D);
+            p1 = (qa + sqD) * 0.5;
+            p2 = (qa - sqD) * 0.5;
+        }
+    } else {
+        sqD = sqrt(D);
+        q1 = (y + sqD) * 0.5;
+        q2 = (y - sqD) * 0.5;
+        p1 = (qa * q1 - qc) / (q1 - q2);
+        p2 = (qc - qa * q2) / (q1 - q2);
+    }
+    double re[4], im[4];
+    D = p1 * p1 - 4 * q1;
+    if (D < 0.0) {
+        re[0] = -p1 * 0.5;
+        im[0] = sqrt(-D) * 0.5;
+        re[1] = re[0];
+        im[1] = -im[0];
+    } else {
+        sqD = sqrt(D);
+        re[0] = (-p1 + sqD) * 0.5;
+        re[1] = (-p1 - sqD) * 0.5;
+        im[0] = im[1] = 0.0;
+    }
+    D = p2 * p2 - 4 * q2;
+    if (D < 0.0) {
+        re[2] = -p2 * 0.5;
+        im[2] = sqrt(-D) * 0.5;
+        re[3] = re[2];
+        im[3] = -im[2];
+    } else {
+        sqD = sqrt(D);
+        re[2] = (-p2 + sqD) * 0.5;
+        re[3] = (-p2 - sqD) * 0.5;
+        im[2] = im[3] = 0.0;
+    }
+    double best = 10000000000;
+    for (int i = 0; i < 4; i++)
+        if (im[i] == 0 && re[i] > 0.000000001 && re[i] < best) best = re[i];
+    return best;
+}
+
 /* logTime, EDMD.c:5709-5721: the time at which the damped motion has gone
  * as far as the undamped one in t_lin. */
 static double log_time(const orc_ctx* s, double t_lin) {
@@ -644,7 +754,20 @@ static void predict_collision(orc_ctx* s, int i, double* t_abs, int* partner) {
             xy = 0;
         }
     }
-    if (s->prm.wall_circle) {
+    if (s->prm.wall_circle && s->prm.add_field) {
+        /* EDMD.c:2918-2935: the parabolic flight meets the circle, the
+         * smallest positive root of the quartic (y0 about halfLy here) */
+        double y0 = s->y[i] - s->halfLy, x0 = s->x[i] - s->halfLx, rad = s->rad[i];
+        double e = x0 * x0 + y0 * y0 - (s->halfLx - rad) * (s->halfLx - rad);
+        double v0x = s->vx[i], v0y = s->vy[i], f = s->prm.field;
+        double a = f * f / 4;
+        double b = f * v0y;
+        double c = v0x * v0x + v0y * v0y + f * y0;
+        double d = 2 * v0x * x0 + 2 * v0y * y0;
+        dt = orc_smallest_root(a, b, c, d, e);
+        wall = 1;
+        xy = 2;
+    } else if (s->prm.wall_circle) {
         /* EDMD.c:2938-2954 (no field): the centre reaches the circle of
          * radius halfLx - rad; the time replaces any earlier wall's */
         double y0 = s->y[i] - s->halfLx, x0 = s->x[i] - s->halfLx, rad = s->rad[i];
@@ -1212,7 +1335,7 @@ orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* 
     s->n = n;
     if (prm) s->prm = *prm;
     else orc_default_params(&s->prm);
-    if ((s->prm.damping && !(s->prm.gamma > 0)) || (s->prm.wall_circle && s->prm.add_field)) {
+    if (s->prm.damping && !(s->prm.gamma > 0)) {
         free(s);
         return NULL;
     }

@@ -84,8 +84,9 @@ typedef struct {
      * reflects the velocity about the normal at atan2 of the disk's
      * position (EDMD.c:3441-3454, growth 3358-3376), with this oracle's
      * own atan2 / sin / cos; no minimum image (EDMD.c:5896-5913); the area
-     * of the thermo row is pi halfLx^2 and the wall perimeter pi Lx.  Not
-     * with the field (whose circle needs the quartic of quartic.c). */
+     * of the thermo row is pi halfLx^2 and the wall perimeter pi Lx.  Under
+     * the field the time is the quartic's smallest root (EDMD.c:2918-2935,
+     * quartic.c) with this oracle's own acos, cos and cube root. */
     int wall_circle;
     /* Mid wall (addMidWall, EDMD.c:269): a wall at x = Lx/2 for the disks
      * of type 1 only (the solute; type 0, the solvent, passes), met from
@@ -217,6 +218,9 @@ double orc_log(double x);
  * wall's normal. */
 void orc_sincos(double th, double* sn, double* cs);
 double orc_atan2(double y, double x);
+/* Smallest real root above 1e-9 of a t^4 + b t^3 + c t^2 + d t + e (1e10
+ * if none), the circular wall's time under the field. */
+double orc_smallest_root(double a, double b, double c, double d, double e);
 
 /* Sum of w_k = m_i m_j ff |a|^2 over the taped collisions, in tape order. */
 double orc_tape_wsum(const orc_ctx* s);

@@ -100,6 +100,8 @@ def lib():
         L.orc_sincos.argtypes = [C.c_double, C.c_void_p, C.c_void_p]
         L.orc_atan2.restype = C.c_double
         L.orc_atan2.argtypes = [C.c_double, C.c_double]
+        L.orc_smallest_root.restype = C.c_double
+        L.orc_smallest_root.argtypes = [C.c_double] * 5
         L.orc_tape_wsum.restype = C.c_double
         L.orc_tape_wsum.argtypes = [C.c_void_p]
         L.orc_replay.restype = C.c_double

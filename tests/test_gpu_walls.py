@@ -313,6 +313,8 @@ CIRCLE_CASES = [
     (450, 0.5, 82, 0.0, 0.0, dict(res_wall=0.9)),
     (400, 0.5, 83, 0.0, 0.0, dict(damping=1, gamma=0.01)),
     (450, 0.5, 84, 0.1, 0.1, dict()),                     # growth start through GROWSTOP
+    (400, 0.45, 85, 0.0, 0.0, dict(add_field=1, field=-0.05)),    # gravity: the quartic
+    (380, 0.45, 86, 0.0, 0.0, dict(add_field=1, field=-0.02, damping=1, gamma=0.01)),
 ]
 
 
@@ -362,9 +364,9 @@ def test_circular_wall_matches_oracle(gpu, oracle, n, phi, seed, poly, vr, kw):
 def test_circular_wall_arguments_are_checked(gpu):
     st = in_circle(F.lattice_state(200, 0.5, seed=5, lattice="fit"))
     with pytest.raises(gpu.EdmdError):
-        gpu.System(st, wall_circle=1, add_field=1)
-    with pytest.raises(gpu.EdmdError):
         gpu.System(st, wall_circle=2)
+    with pytest.raises(gpu.EdmdError):
+        gpu.System(st, mid_wall=3)
 
 
 # ---- the mid wall (addMidWall)

@@ -512,3 +512,39 @@ def test_mid_wall_thermo_columns(oracle):
     assert r["temp_left"] == pytest.approx(np.sum(e[solv & (s["x"] < 20)]) / nl, rel=1e-12)
     assert r["temp_right"] == pytest.approx(np.sum(e[solv & (s["x"] > 20)]) / nr, rel=1e-12)
     assert r["temp_solute"] == pytest.approx(np.mean(e[typ == 1]), rel=1e-12)
+
+
+def test_circular_wall_under_field_closed_form(oracle):
+    """Under the field the circle time is the smallest positive root of
+    the quartic |x0 + v t + f t^2/2 e_y|^2 = (R - r)^2 (EDMD.c:2918-2935,
+    quartic.c): a disk dropped from rest at the centre meets it at
+    sqrt(2 (R - r) / |f|); a thrown disk where bisection on the trajectory
+    finds the first contact.  The solver finds the smallest positive root
+    of quartics with known roots to 1e-8."""
+    L = oracle.lib()
+    rng = np.random.default_rng(3)
+    for _ in range(2000):
+        roots = rng.uniform(-5, 10, 4)
+        c = np.poly(roots) * rng.uniform(0.1, 2)
+        pos = roots[roots > 1e-9]
+        if len(pos):
+            assert L.orc_smallest_root(*map(float, c)) == pytest.approx(pos.min(), rel=1e-8)
+    f = -0.1
+    st = disks([(10.0, 10.0), (10.0, 4.0)], [(0.0, 0.0), (0.3, 0.0)])
+    o = oracle.Oracle(st, wall_circle=1, add_field=1, field=f, **QUIET)
+    t, p = o.predict_collision(0)
+    assert p == -7 and t == pytest.approx(np.sqrt(2 * 9.5 / 0.1), rel=1e-9)
+    x0, y0, vx, vy = 12.0, 11.0, 0.7, 0.9
+    st = disks([(x0, y0), (10.0, 4.0)], [(vx, vy), (0.3, 0.0)])
+    o = oracle.Oracle(st, wall_circle=1, add_field=1, field=f, **QUIET)
+    t, p = o.predict_collision(0)
+
+    def gap(s):
+        return np.hypot(x0 + vx * s - 10, y0 + vy * s + 0.5 * f * s * s - 10) - 9.5
+    ts = np.linspace(1e-6, 50, 200001)
+    k = np.nonzero(gap(ts) > 0)[0][0]
+    lo, hi = ts[k - 1], ts[k]
+    for _ in range(100):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if gap(mid) < 0 else (lo, mid)
+    assert p == -7 and t == pytest.approx(lo, rel=1e-9)
