@@ -1770,9 +1770,10 @@ extern "C" int edmd_batch_deriv_begin(edmd_batch* b, int64_t checkpoint_every, i
     if (b->any_grow)
         return fail(EDMD_ERR_ARG, "derivatives need systems whose growth has ended");
     for (int s = 0; s < b->nsys; s++)
-        if (b->hs[s].c.walls || b->hs[s].c.field_on || b->hs[s].c.damping)
+        if ((b->hs[s].c.walls & 4) || b->hs[s].c.field_on || b->hs[s].c.damping)
             return fail(EDMD_ERR_ARG,
-                        "system %d: derivatives need a system without walls, field or damping", s);
+                        "system %d: derivatives need a system without the circular wall, field "
+                        "or damping (flat walls are taped)", s);
     int rc = edmd_batch_deriv_end(b);
     if (rc) return rc;
     if ((double)b->nsys * (double)cap * sizeof(EdmdTapeRec) > 1.0e12)

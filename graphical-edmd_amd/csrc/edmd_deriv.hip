@@ -15,8 +15,14 @@
  *   dv_i += m_j u,  dv_j -= m_i u
  *   dr_i -= m_j ff dt a,  dr_j += m_i ff dt a   (velocity jump applied dt early)
  *   d(m_i m_j ff |a|^2) = m_i m_j |a|^2 dff     (|a| stays at contact)
- * Free flight moves the tangent as dr += (t' - t) dv.  k_adjoint applies the
- * transpose of the same steps in reverse order.
+ * Free flight moves the tangent as dr += (t' - t) dv.  A flat wall's hit
+ * (record j = -1 - xy, normal / tangential velocity vn, vt before it,
+ * resW, resT) maps the disk's tangent flown to the hit, with n / s its
+ * normal / tangential components, as
+ *   dt = -dr_n / vn, dr_n' = -resW dr_n, dr_s' = dr_s + (vt - resT vt) dt,
+ *   dv_n' = -resW dv_n, dv_s' = resT dv_s
+ * (the reflected disk taken back to the unperturbed hit time).  k_adjoint
+ * applies the transpose of the same steps in reverse order.
  *
  *   k_tangent      forward propagation of one tangent vector per system
  *   k_adjoint      reverse propagation of one cotangent vector per system
@@ -138,12 +144,34 @@ __global__ __launch_bounds__(64) void k_tangent(const EdmdSys* __restrict__ syst
             const TapeLane r = tape_bcast(mine, l);
             const int i = r.i, j = r.j;
             double* ti = V.v + 4 * i;
-            double* tj = V.v + 4 * j;
             double ti0 = ti[0], ti1 = ti[1], ti2 = ti[2], ti3 = ti[3];
-            double tj0 = tj[0], tj1 = tj[1], tj2 = tj[2], tj3 = tj[3];
             double h = r.t - V.tl[i];
             ti0 += h * ti2;
             ti1 += h * ti3;
+            if (j < 0) { /* uniform: the record is the wave's */
+                const double vn = r.ax, vt = r.ay, rw = r.cx, rt = r.cy;
+                if (j == -1) {
+                    const double dt = -ti0 / vn;
+                    ti0 = -rw * ti0;
+                    ti1 += (vt - rt * vt) * dt;
+                    ti2 = -rw * ti2;
+                    ti3 = rt * ti3;
+                } else {
+                    const double dt = -ti1 / vn;
+                    ti1 = -rw * ti1;
+                    ti0 += (vt - rt * vt) * dt;
+                    ti3 = -rw * ti3;
+                    ti2 = rt * ti2;
+                }
+                if (lane == 0) {
+                    ti[0] = ti0; ti[1] = ti1; ti[2] = ti2; ti[3] = ti3;
+                    V.tl[i] = r.t;
+                }
+                WAVE_SYNC();
+                continue;
+            }
+            double* tj = V.v + 4 * j;
+            double tj0 = tj[0], tj1 = tj[1], tj2 = tj[2], tj3 = tj[3];
             h = r.t - V.tl[j];
             tj0 += h * tj2;
             tj1 += h * tj3;
@@ -203,12 +231,32 @@ __global__ __launch_bounds__(64) void k_adjoint(const EdmdSys* __restrict__ syst
             const TapeLane r = tape_bcast(mine, l);
             const int i = r.i, j = r.j;
             double* li = V.v + 4 * i;
-            double* lj = V.v + 4 * j;
             double li0 = li[0], li1 = li[1], li2 = li[2], li3 = li[3];
-            double lj0 = lj[0], lj1 = lj[1], lj2 = lj[2], lj3 = lj[3];
             double h = V.tl[i] - r.t;
             li2 += h * li0;
             li3 += h * li1;
+            if (j < 0) {
+                const double vn = r.ax, vt = r.ay, rw = r.cx, rt = r.cy;
+                if (j == -1) {
+                    const double ldt = (vt - rt * vt) * li1;
+                    li0 = -rw * li0 - ldt / vn;
+                    li2 = -rw * li2;
+                    li3 = rt * li3;
+                } else {
+                    const double ldt = (vt - rt * vt) * li0;
+                    li1 = -rw * li1 - ldt / vn;
+                    li3 = -rw * li3;
+                    li2 = rt * li2;
+                }
+                if (lane == 0) {
+                    li[0] = li0; li[1] = li1; li[2] = li2; li[3] = li3;
+                    V.tl[i] = r.t;
+                }
+                WAVE_SYNC();
+                continue;
+            }
+            double* lj = V.v + 4 * j;
+            double lj0 = lj[0], lj1 = lj[1], lj2 = lj[2], lj3 = lj[3];
             h = V.tl[j] - r.t;
             lj2 += h * lj0;
             lj3 += h * lj1;

@@ -2745,12 +2745,12 @@ __device__ __forceinline__ void ev_single(const EdmdConst& C, const AR& A, int l
     /* The events that change one disk.  d = 1-4: a cell crossing in that
      * direction (doTheCrossing, EDMD.c:2501-2590; its freeFly is the
      * dispatch pointer, freeFlyGrow while growing).  d = 5 + xy: a wall hit
-     * (Normal dynamics only; doTheWallNormal, EDMD.c:3384-3430, 3454-3460,
-     * without mid, thermal or shear walls): counted as a collision, the
-     * momentum given to the wall accumulated for the thermo row, the normal
-     * velocity reversed times resW and the tangential one scaled by resT.
-     * Both then predict the crossing slot and the collision slot again, in
-     * that order. */
+     * (doTheWallNormal, EDMD.c:3384-3460, without thermal or shear walls;
+     * doTheWallGrow while growing): counted as a collision, the momentum
+     * given to the wall accumulated for the thermo row, the normal velocity
+     * reversed times resW and the tangential one scaled by resT, a flat
+     * wall's hit written to the tape when taping.  Both then predict the
+     * crossing slot and the collision slot again, in that order. */
     const bool wall = W && d >= 5;
     const EdmdPart pp = A.p[p];
     const double gvr = G ? uni(A.vr[p]) : 0.0;
@@ -2813,6 +2813,7 @@ __device__ __forceinline__ void ev_single(const EdmdConst& C, const AR& A, int l
          * times a negative velocity): the reference's crossing prediction
          * divides by a -0.0 velocity and flies the disk to t = -inf */
         double dpl = 0, dpm = 0, dpr = 0;
+        const double vx0 = vx, vy0 = vy; /* before the hit, for the tape */
         if (d == 5) {
             dp = fabs((C.res_wall + 1) * m * vx);
             if (C.walls & 8) {
@@ -2849,6 +2850,28 @@ __device__ __forceinline__ void ev_single(const EdmdConst& C, const AR& A, int l
                 A.st->dpl += dpl;
                 A.st->dpm += dpm;
                 A.st->dpr += dpr;
+            }
+            if (C.tape_cap > 0 && d != 7) {
+                /* a flat wall's record: j = -1 - xy, the normal and
+                 * tangential velocities before the hit, resW, resT and the
+                 * contact coordinate */
+                EdmdState* st = A.st;
+                const int64_t k = st->ntape;
+                if (k < C.tape_cap) {
+                    EdmdTapeRec rec;
+                    rec.t = t;
+                    rec.ax = d == 5 ? vx0 : vy0;
+                    rec.ay = d == 5 ? vy0 : vx0;
+                    rec.cx = C.res_wall;
+                    rec.cy = C.res_tan;
+                    rec.ff = d == 5 ? x : y;
+                    rec.i = p;
+                    rec.j = d == 5 ? -1 : -2;
+                    rec.pad[0] = 0;
+                    rec.pad[1] = 0;
+                    A.tape[k] = rec;
+                }
+                st->ntape = k + 1;
             }
             const auto q = A.p + p;
             q->x = x;

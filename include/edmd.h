@@ -89,7 +89,8 @@ typedef struct edmd_system_desc {
      * a growth start the wall slot closes the disk's surface at v -/+ vr on
      * the wall (EDMD.c:3118-3153) and a hit turns the normal velocity away
      * with 2 vr added, no momentum counted (doTheWallGrow, EDMD.c:
-     * 3333-3382); after GROWSTOP the walls above apply.  Not with taping. */
+     * 3333-3382); after GROWSTOP the walls above apply.  Taped hits enter
+     * the derivatives (edmd_tape_rec). */
     int wall_x, wall_y;
     double res_wall, res_tan;
     /* Uniform field along y (addField / field of the graphics build,
@@ -143,7 +144,8 @@ typedef struct edmd_system_desc {
      * position (EDMD.c:3393-3409); thermo rows carry the mid-wall columns
      * of EDMD.c:5453-5477.  type: one int per disk (NULL: all 0), copied;
      * sizeratio: the solvent's packing factor in those columns (EDMD.c:83,
-     * default 0.4). */
+     * default 0.4).  Taped hits enter the derivatives like the side
+     * walls'. */
     int mid_wall;
     const int32_t* type;
     double sizeratio;
@@ -451,12 +453,16 @@ int edmd_pcf_noalloc(int n, const double* x, const double* y, double Lx, double 
  *            W / (t_last - t_origin)),
  *   t_last = time of the last taped collision,
  *   and the state of every disk free-flown to the final time,
- * holding the collision sequence fixed (the map is smooth while it is).
- * Systems must not use the velocity-rescale thermostat.
+ * holding the event sequence fixed (the map is smooth while it is).
+ * Systems must not use the velocity-rescale thermostat, the circular wall,
+ * the field or damping; flat walls (wall_x, wall_y, mid_wall) are taped.
  *
  * One valid collision: time, contact vector a = r_j - r_i (minimum image),
  * relative velocity c = v_j - v_i before the collision, impulse factor ff
- * (v_i += ff m_j a, v_j -= ff m_i a). */
+ * (v_i += ff m_j a, v_j -= ff m_i a).  One flat wall's hit: j = -1 - xy
+ * (-1: a wall normal to x), ax / ay = the normal / tangential velocity
+ * before the hit, cx / cy = res_wall / res_tan, ff = the contact
+ * coordinate (x or y of the disk's centre at the hit); no virial term. */
 typedef struct edmd_tape_rec {
     double t, ax, ay, cx, cy, ff;
     int32_t i, j;
@@ -467,7 +473,8 @@ typedef struct edmd_tape_rec {
  * keeps a full checkpoint of every system every K events and the tape of
  * one K-event segment; the tangent and adjoint passes recompute each
  * segment's tape from its checkpoint.  K = 0: the whole tape is kept,
- * tape_capacity collisions per system (EDMD_ERR_TAPE_FULL beyond). */
+ * tape_capacity records (collisions and wall hits) per system
+ * (EDMD_ERR_TAPE_FULL beyond). */
 int edmd_batch_deriv_begin(edmd_batch* b, int64_t checkpoint_every, int64_t tape_capacity);
 /* edmd_batch_run with taping (and checkpoints); may be called repeatedly. */
 int edmd_batch_deriv_run(edmd_batch* b, int64_t max_events, uint64_t max_collisions, double tmax);

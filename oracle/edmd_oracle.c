@@ -1080,6 +1080,22 @@ static void do_wall(orc_ctx* s, const ev_node* ev) {
     s->ncol++;
     free_fly(s, i);
     double rw = s->prm.res_wall, rt = s->prm.res_tan;
+    if (s->tape && xy < 2) {
+        /* a wall record: j = -1 - xy, the normal and tangential velocities
+         * before the hit, resW and resT, and the contact coordinate */
+        if (s->ntape < s->captape) {
+            orc_tape_rec* r = &s->tape[s->ntape];
+            r->i = i;
+            r->j = -1 - xy;
+            r->t = s->t;
+            r->ax = xy == 0 ? s->vx[i] : s->vy[i];
+            r->ay = xy == 0 ? s->vy[i] : s->vx[i];
+            r->cx = rw;
+            r->cy = rt;
+            r->ff = xy == 0 ? s->x[i] : s->y[i];
+        }
+        s->ntape++;
+    }
     if (xy == 0) {
         s->dp += fabs((rw + 1) * s->m[i] * s->vx[i]);
         if (s->prm.mid_wall) {
@@ -1689,11 +1705,22 @@ double orc_tangent(int n, const double* m, const double* tl0, const orc_tape_rec
         const orc_tape_rec* r = &tape[k];
         const int i = r->i, j = r->j;
         double* ti = tan + 4 * i;
-        double* tj = tan + 4 * j;
         double h = r->t - tl[i];
         ti[0] += h * ti[2];
         ti[1] += h * ti[3];
         tl[i] = r->t;
+        if (j < 0) {
+            /* wall hit: normal axis a, tangential axis b */
+            const int a = -1 - j, b = 1 - a;
+            const double vn = r->ax, vt = r->ay, rw = r->cx, rt = r->cy;
+            const double dt = -ti[a] / vn;
+            ti[a] = -rw * ti[a];
+            ti[b] += (vt - rt * vt) * dt;
+            ti[2 + a] = -rw * ti[2 + a];
+            ti[2 + b] = rt * ti[2 + b];
+            continue;
+        }
+        double* tj = tan + 4 * j;
         h = r->t - tl[j];
         tj[0] += h * tj[2];
         tj[1] += h * tj[3];
@@ -1737,15 +1764,25 @@ void orc_adjoint(int n, const double* m, const double* tl0, const orc_tape_rec* 
     for (int i = 0; i < n; i++) tl[i] = t_end;
     if (lam_end) memcpy(grad, lam_end, (size_t)n * 4 * sizeof(double));
     else memset(grad, 0, (size_t)n * 4 * sizeof(double));
+    int pending = 1; /* lt goes to the last pair collision */
     for (int64_t k = ntape - 1; k >= 0; k--) {
         const orc_tape_rec* r = &tape[k];
         const int i = r->i, j = r->j;
         double* li = grad + 4 * i;
-        double* lj = grad + 4 * j;
         double h = tl[i] - r->t;
         li[2] += h * li[0];
         li[3] += h * li[1];
         tl[i] = r->t;
+        if (j < 0) {
+            const int a = -1 - j, b = 1 - a;
+            const double vn = r->ax, vt = r->ay, rw = r->cx, rt = r->cy;
+            const double ldt = (vt - rt * vt) * li[b];
+            li[a] = -rw * li[a] - ldt / vn;
+            li[2 + a] = -rw * li[2 + a];
+            li[2 + b] = rt * li[2 + b];
+            continue;
+        }
+        double* lj = grad + 4 * j;
         h = tl[j] - r->t;
         lj[2] += h * lj[0];
         lj[3] += h * lj[1];
@@ -1760,7 +1797,10 @@ void orc_adjoint(int n, const double* m, const double* tl0, const orc_tape_rec* 
         const double lrcx = ff * lux + kap * ldff * cx, lrcy = ff * luy + kap * ldff * cy;
         const double lqx = kap * ldff * ax, lqy = kap * ldff * ay;
         double ldt = ff * lsh + (cx * lrcx + cy * lrcy);
-        if (k == ntape - 1) ldt += lt;
+        if (pending) {
+            ldt += lt;
+            pending = 0;
+        }
         const double lpx = lrcx - (ldt / b) * ax, lpy = lrcy - (ldt / b) * ay;
         li[0] -= lpx;
         li[1] -= lpy;
@@ -1789,6 +1829,23 @@ double orc_replay(int n, double Lx, double Ly, const double* rad, const double* 
         const int i = r->i, j = r->j;
         const double tr = r->t;
         double xi = x[i] + (tr - tl[i]) * vx[i], yi = y[i] + (tr - tl[i]) * vy[i];
+        if (j < 0) {
+            /* wall hit: the disk flies on to the taped contact coordinate
+             * and reflects */
+            const int a = -1 - j;
+            const double sh = (r->ff - (a == 0 ? xi : yi)) / (a == 0 ? vx[i] : vy[i]);
+            x[i] = xi + sh * vx[i];
+            y[i] = yi + sh * vy[i];
+            tl[i] = tr + sh;
+            if (a == 0) {
+                vx[i] = -r->cx * vx[i];
+                vy[i] = r->cy * vy[i];
+            } else {
+                vy[i] = -r->cx * vy[i];
+                vx[i] = r->cy * vx[i];
+            }
+            continue;
+        }
         double xj = x[j] + (tr - tl[j]) * vx[j], yj = y[j] + (tr - tl[j]) * vy[j];
         double ax = xj - xi, ay = yj - yi;
         ax -= Lx * nearbyint(ax / Lx);
@@ -1831,6 +1888,21 @@ static ld replay_objective_ld(int n, ld Lx, ld Ly, const double* rad, const doub
         const int i = r->i, j = r->j;
         const ld tr = r->t;
         ld xi = x[i] + (tr - tl[i]) * vx[i], yi = y[i] + (tr - tl[i]) * vy[i];
+        if (j < 0) {
+            const int a = -1 - j;
+            const ld sh = ((ld)r->ff - (a == 0 ? xi : yi)) / (a == 0 ? vx[i] : vy[i]);
+            x[i] = xi + sh * vx[i];
+            y[i] = yi + sh * vy[i];
+            tl[i] = tr + sh;
+            if (a == 0) {
+                vx[i] = -(ld)r->cx * vx[i];
+                vy[i] = (ld)r->cy * vy[i];
+            } else {
+                vy[i] = -(ld)r->cx * vy[i];
+                vx[i] = (ld)r->cy * vx[i];
+            }
+            continue;
+        }
         ld xj = x[j] + (tr - tl[j]) * vx[j], yj = y[j] + (tr - tl[j]) * vy[j];
         ld ax = xj - xi, ay = yj - yi;
         ax -= Lx * nearbyintl(ax / Lx);

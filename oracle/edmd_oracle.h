@@ -188,7 +188,15 @@ void orc_sync_all(orc_ctx* s, double t);
  *   rc = p + c dt, dff = (ff/b)(rc.c + a.q), u = dff a + ff rc,
  *   dv_i += m_j u, dv_j -= m_i u, dr_i -= m_j ff dt a, dr_j += m_i ff dt a,
  *   dw = m_i m_j |a|^2 dff        (w = m_i m_j ff |a|^2, the virial term)
- * and free flight is dr += (t - t_last) dv. */
+ * and free flight is dr += (t - t_last) dv.  A hit of a flat wall (x or y
+ * walls, the mid wall) is a record with j = -1 - xy (xy = 0 for a wall
+ * normal to x), ax / ay = the normal / tangential velocity vn, vt before the
+ * hit, cx / cy = resW / resT and ff = the contact coordinate; with n / s
+ * the normal / tangential components of the tangent flown to the hit,
+ *   dt = -dr_n / vn, dr_n' = -resW dr_n, dr_s' = dr_s + (vt - resT vt) dt,
+ *   dv_n' = -resW dv_n, dv_s' = resT dv_s
+ * (the reflected disk evaluated at the unperturbed hit time); t_last is
+ * the time of the last pair collision. */
 typedef struct {
     int32_t i, j;
     double t, ax, ay, cx, cy, ff;

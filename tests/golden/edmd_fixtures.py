@@ -104,6 +104,20 @@ def lattice_state(n, phi, seed, rad=0.5, lattice="square", temperature=1.0):
     return st
 
 
+def walled_state(n, phi, seed, wall_x=True, wall_y=False, temperature=1.0):
+    """A fitted-lattice start pressed affinely into [r + 0.01, L - r - 0.01]
+    along each walled direction (the lattice's outer rows may sit closer
+    than a radius to the box edge; the squeeze leaves every pair apart)."""
+    st = lattice_state(n, phi, seed=seed, lattice="fit", temperature=temperature)
+    r, L = 0.5, st["Lx"]
+    lo, span = r + 0.01, L - 2 * (r + 0.01)
+    if wall_x:
+        st["x"] = lo + st["x"] / L * span
+    if wall_y:
+        st["y"] = lo + st["y"] / L * span
+    return st
+
+
 def growth_state(n, phi, seed, poly=0.0, rad=0.5, e_per=0.05):
     """Seeded growth start (the reference's default start, EDMD.c:1691-1810,
     with this build's generator): uniform random centres in a square box

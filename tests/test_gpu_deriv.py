@@ -3,7 +3,8 @@ recomputation) against the oracle, and the adjoint against finite
 differences through a torch.autograd.Function that exists only here.
 
 Tapes, tangents and gradients are compared bit for bit with the oracle's:
-the kernels perform the oracle's operations in the oracle's order.
+the kernels perform the oracle's operations in the oracle's order.  Flat
+walls (x, y and the mid wall) enter the tape as wall records.
 """
 import numpy as np
 import pytest
@@ -24,8 +25,8 @@ def states_mixed():
     return out
 
 
-def oracle_taped(oracle, st, ncoll):
-    o = oracle.Oracle(st)
+def oracle_taped(oracle, st, ncoll, **params):
+    o = oracle.Oracle(st, **params)
     s0 = o.state()
     o.tape_enable(ncoll + 16)
     o.run_collisions(ncoll)
@@ -35,9 +36,9 @@ def oracle_taped(oracle, st, ncoll):
 def canonical_tape(tape):
     """A mirrored pair event (i, j) / (j, i) ties exactly and either may be
     handled; the record of (j, i) is that of (i, j) with a and c negated.
-    Orders every record as i < j."""
+    Orders every pair record as i < j (wall records, j < 0, stay)."""
     t = tape.copy()
-    sw = t["i"] > t["j"]
+    sw = (t["i"] > t["j"]) & (t["j"] >= 0)
     t["i"][sw], t["j"][sw] = tape["j"][sw], tape["i"][sw]
     for f in ("ax", "ay", "cx", "cy"):
         t[f][sw] = -tape[f][sw]
@@ -219,3 +220,94 @@ def test_thermostatted_systems_are_rejected(gpu):
     with pytest.raises(gpu.EdmdError) as e:
         b.deriv_begin()
     assert e.value.code == 22
+
+
+# ---- flat walls on the tape
+
+def _osmosis(n, L, seed):
+    rng = np.random.default_rng(seed)
+    xs, ys = [], []
+    while len(xs) < n:
+        x, y = rng.uniform(1.5, L - 1.5), rng.uniform(1.5, L - 1.5)
+        if abs(x - L / 2) > 1 and all((x - a) ** 2 + (y - b) ** 2 > 1.2 ** 2
+                                      for a, b in zip(xs, ys)):
+            xs.append(x)
+            ys.append(y)
+    xs, ys = np.array(xs), np.array(ys)
+    v = rng.standard_normal((n, 2))
+    return {"x": xs, "y": ys, "vx": v[:, 0].copy(), "vy": v[:, 1].copy(),
+            "rad": np.full(n, 0.5), "m": np.ones(n), "Lx": L, "Ly": L,
+            "type": ((xs < L / 2) & (np.arange(n) % 2 == 0)).astype(np.int32)}
+
+
+WALL_CASES = [
+    ("channel", lambda: F.walled_state(400, 0.55, 21, wall_x=True), dict(wall_x=1), 4000),
+    ("box", lambda: F.walled_state(400, 0.55, 22, wall_x=True, wall_y=True),
+     dict(wall_x=1, wall_y=1, res_wall=0.9, res_tan=0.7), 4000),
+    ("mid", lambda: _osmosis(160, 30.0, 7), dict(wall_x=1, wall_y=1, mid_wall=1, res_tan=1.0),
+     3000),
+]
+
+
+@pytest.mark.parametrize("name,make,params,ncoll", WALL_CASES)
+def test_walled_tape_tangent_adjoint_match_oracle(gpu, oracle, name, make, params, ncoll):
+    """Wall hits taped on the GPU as the oracle tapes them; tangent and
+    gradient bit for bit, from the whole tape and with checkpoints every
+    97 events (each segment's tape recomputed)."""
+    st = make()
+    o, s0, tape = oracle_taped(oracle, st, ncoll, **params)
+    nwall = int(np.sum(tape["j"] < 0))
+    assert nwall >= 20, (name, nwall)
+    rng = np.random.default_rng(41)
+    tin = rng.standard_normal((len(st["x"]), 4))
+    lam = rng.standard_normal((len(st["x"]), 4))
+    tan = tin.copy()
+    rdw, rdtl = oracle.tangent(st["m"], s0["t"], tape, o.t, tan)
+    rg = oracle.adjoint(st["m"], s0["t"], tape, o.t, 0.8, -1.5, lam)
+    for every in (0, 97):
+        b = gpu.Batch([st], **params)
+        b.deriv_begin(checkpoint_every=every, tape_capacity=ncoll + 16)
+        b.deriv_run(max_collisions=ncoll)
+        w, tl, nrec = b.deriv_values(0)
+        assert bits(w) == bits(o.tape_wsum()), (name, every)
+        assert tl == tape["t"][tape["j"] >= 0][-1], (name, every)
+        if every == 0:
+            assert nrec == len(tape)
+            got, ref = canonical_tape(b.tape(0)), canonical_tape(tape)
+            for f in TAPE_FIELDS:
+                np.testing.assert_array_equal(got[f], ref[f], err_msg="%s %s" % (name, f))
+        tout, dw, dtl = b.tangent([tin])
+        np.testing.assert_array_equal(bits(tout[0]), bits(tan), err_msg="%s %d" % (name, every))
+        assert bits(dw[0]) == bits(rdw) and bits(dtl[0]) == bits(rdtl), (name, every)
+        g = b.adjoint(0.8, -1.5, [lam])
+        np.testing.assert_array_equal(bits(g[0]), bits(rg), err_msg="%s %d" % (name, every))
+
+
+def test_walled_gradient_matches_finite_differences(gpu, oracle):
+    """W / t_last over 600 events in a closed box with inelastic walls: the
+    GPU gradient against central differences of the oracle's replay."""
+    st = F.walled_state(400, 0.55, 24, wall_x=True, wall_y=True)
+    params = dict(wall_x=1, wall_y=1, res_wall=0.9, res_tan=0.7)
+    b = gpu.Batch([st], **params)
+    b.deriv_begin(checkpoint_every=256)
+    b.deriv_run(max_collisions=600)
+    W, T, _ = b.deriv_values(0)
+    grad = b.adjoint(lw=1 / T, lt=-W / T ** 2)[0]
+    o, s0, tape = oracle_taped(oracle, st, 600, **params)
+    assert np.sum(tape["j"] < 0) >= 8
+    s0.update(rad=st["rad"], m=st["m"], Lx=st["Lx"], Ly=st["Ly"])
+    rng = np.random.default_rng(6)
+    for _ in range(3):
+        e = rng.standard_normal((400, 4))
+        fd = oracle.replay_fd(s0, tape, o.t, e, 1e-7, 1 / T, -W / T ** 2)
+        ad = (grad * e).sum()
+        assert abs(fd - ad) <= 1e-6 * abs(ad)
+
+
+def test_unsupported_models_are_rejected_for_taping(gpu):
+    st = F.walled_state(400, 0.55, 25, wall_x=True)
+    for kw in (dict(add_field=1), dict(damping=1, gamma=0.01)):
+        b = gpu.Batch([st], wall_x=1, **kw)
+        with pytest.raises(gpu.EdmdError) as e:
+            b.deriv_begin()
+        assert e.value.code == 22

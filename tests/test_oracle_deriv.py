@@ -6,12 +6,15 @@ mathematics: closed forms for a head-on collision, central finite
 differences (step 1e-7) of the nonlinear fixed-sequence replay, the
 adjoint/tangent duality, and the invariants the collision rule keeps
 (momentum, energy, centre of mass) carried over to the linearised map.
+Flat walls (x, y and mid walls) enter the tape as wall records; a single
+bounce is checked against its closed form, walled systems by finite
+differences and duality.
 """
 import numpy as np
 import pytest
 
 import oracle_bind as ob
-from edmd_fixtures import lattice_state
+from edmd_fixtures import lattice_state, walled_state
 
 KEYS = ("x", "y", "vx", "vy")
 
@@ -199,3 +202,128 @@ def test_tangent_conserves_linearised_invariants(dense400):
     dE0 = (m * (s0["vx"] * e[:, 2] + s0["vy"] * e[:, 3])).sum()
     dE1 = (m * (fin["vx"] * tan[:, 2] + fin["vy"] * tan[:, 3])).sum()
     assert abs(dE1 - dE0) < 1e-9 * np.abs(m * fin["vx"] * tan[:, 2]).sum()
+
+
+# ---- flat walls on the tape
+
+def test_wall_bounce_closed_form():
+    """One disk reflected by the left x wall (resW 0.8, resT 0.5) and
+    flown to T = 4: x(T) = r - rw vx (T - th), y(T) = y0 + vy th + rt vy
+    (T - th) with th = (x0 - r) / (-vx); their partial derivatives."""
+    x0, y0, vx, vy, r, T, rw, rt = 3.0, 10.0, -1.0, 0.6, 0.5, 4.0, 0.8, 0.5
+    st = {"x": np.array([x0, 15.0]), "y": np.array([y0, 5.0]),
+          "vx": np.array([vx, 0.0]), "vy": np.array([vy, 0.0]),
+          "rad": np.array([r, r]), "m": np.array([1.0, 1.0]), "Lx": 20.0, "Ly": 20.0}
+    o, s0, tape = taped(st, 1, wall_x=1, res_wall=rw, res_tan=rt)
+    th = (x0 - r) / (-vx)
+    assert len(tape) == 1 and tape["j"][0] == -1 and tape["t"][0] == th
+    assert tape["ax"][0] == vx and tape["ay"][0] == vy and tape["ff"][0] == r
+    m, tl0 = st["m"], s0["t"]
+    want = {  # output component: gradient row of disk 0 (x0, y0, vx, vy)
+        0: [-rw, 0, -rw * T, 0],
+        1: [(1 - rt) * vy / (-vx), 1, (1 - rt) * vy * (x0 - r) / vx ** 2, th + rt * (T - th)],
+        2: [0, 0, -rw, 0],
+        3: [0, 0, 0, rt],
+    }
+    for c, row in want.items():
+        lam = np.zeros((2, 4))
+        lam[0, c] = 1.0
+        g = ob.adjoint(m, tl0, tape, T, 0.0, 0.0, lam)
+        np.testing.assert_allclose(g[0], row, rtol=1e-14, atol=1e-15, err_msg=str(c))
+        np.testing.assert_array_equal(g[1], 0.0)
+        tan = np.zeros((2, 4))
+        tan[0] = row  # the tangent along the gradient row gives its square norm
+        ob.tangent(m, tl0, tape, T, tan)
+        assert tan[0, c] == pytest.approx(np.dot(row, row), rel=1e-14)
+    # the wall is no collision: no virial, and t_last stays unset
+    assert o.tape_wsum() == 0.0
+    tan = np.ones((2, 4))
+    dW, dtl = ob.tangent(m, tl0, tape, T, tan)
+    assert dW == 0.0 and dtl == 0.0
+
+
+def _last_pair_time(tape):
+    return tape["t"][tape["j"] >= 0][-1]
+
+
+WALLED = [
+    ("channel", dict(wall_x=1), dict(wall_x=True)),
+    ("box", dict(wall_x=1, wall_y=1, res_wall=0.9, res_tan=0.7), dict(wall_x=True, wall_y=True)),
+    ("box_rt0", dict(wall_x=1, wall_y=1), dict(wall_x=True, wall_y=True)),
+]
+
+
+@pytest.mark.parametrize("name,params,walls", WALLED)
+def test_walled_adjoint_matches_finite_differences(name, params, walls):
+    """W / t_last plus a random functional of the final state over 600
+    events of N = 400 at phi = 0.55 between walls, by central differences
+    of the replay; the unperturbed replay reproduces the engine."""
+    st = walled_state(400, 0.55, 21, **walls)
+    o, s0, tape = taped(st, 600, **params)
+    assert np.sum(tape["j"] < 0) >= 8, name
+    te = o.t
+    S, tk, fin = ob.replay(s0, tape, te)
+    assert tk == pytest.approx(_last_pair_time(tape), abs=1e-12)
+    assert S == pytest.approx(o.tape_wsum(), rel=1e-12)
+    o.sync_all(te)
+    ref = o.state()
+    for k in ("x", "y", "vx", "vy"):
+        d = fin[k] - ref[k]
+        if k == "x" and not walls.get("wall_x") or k == "y" and not walls.get("wall_y"):
+            d -= st["Lx"] * np.round(d / st["Lx"])
+        assert np.abs(d).max() < 1e-10, (name, k)
+    rng = np.random.default_rng(3)
+    lam = rng.standard_normal((400, 4))
+    lw, lt = 1 / tk, -S / tk ** 2
+    g = ob.adjoint(st["m"], s0["t"], tape, te, lw, lt, lam)
+    for _ in range(2):
+        e = rng.standard_normal((400, 4))
+        fd = ob.replay_fd(s0, tape, te, e, 1e-7, lw, lt, lam)
+        ad = (g * e).sum()
+        assert abs(fd - ad) <= 1e-6 * abs(ad), name
+
+
+def test_mid_wall_adjoint_matches_finite_differences():
+    """The solute's hits of the mid wall are wall records at x = Lx/2 -/+ r."""
+    rng = np.random.default_rng(7)
+    L, n = 30.0, 160
+    xs, ys = [], []
+    while len(xs) < n:
+        x, y = rng.uniform(1.5, L - 1.5), rng.uniform(1.5, L - 1.5)
+        if abs(x - L / 2) > 1 and all((x - a) ** 2 + (y - b) ** 2 > 1.2 ** 2
+                                      for a, b in zip(xs, ys)):
+            xs.append(x)
+            ys.append(y)
+    xs, ys = np.array(xs), np.array(ys)
+    v = rng.standard_normal((n, 2))
+    st = {"x": xs, "y": ys, "vx": v[:, 0].copy(), "vy": v[:, 1].copy(),
+          "rad": np.full(n, 0.5), "m": np.ones(n), "Lx": L, "Ly": L,
+          "type": ((xs < L / 2) & (np.arange(n) % 2 == 0)).astype(np.int32)}
+    o, s0, tape = taped(st, 250, wall_x=1, wall_y=1, mid_wall=1, res_tan=1.0)
+    mid = (tape["j"] == -1) & (np.abs(tape["ff"] - L / 2) < 1)
+    assert mid.sum() >= 3
+    te = o.t
+    S, tk, _ = ob.replay(s0, tape, te)
+    lw, lt = 1 / tk, -S / tk ** 2
+    lam = rng.standard_normal((n, 4))
+    g = ob.adjoint(st["m"], s0["t"], tape, te, lw, lt, lam)
+    e = rng.standard_normal((n, 4))
+    fd = ob.replay_fd(s0, tape, te, e, 1e-7, lw, lt, lam)
+    assert abs(fd - (g * e).sum()) <= 1e-6 * abs((g * e).sum())
+
+
+def test_walled_duality_long_horizon():
+    """5000 events in a closed box with inelastic walls: <grad, e> equals
+    the tangent's functional to rounding."""
+    st = walled_state(400, 0.55, 23, wall_x=True, wall_y=True)
+    o, s0, tape = taped(st, 5000, wall_x=1, wall_y=1, res_wall=0.95, res_tan=0.8)
+    te = o.t
+    rng = np.random.default_rng(12)
+    lam = rng.standard_normal((400, 4))
+    g = ob.adjoint(st["m"], s0["t"], tape, te, 0.3, -2.0, lam)
+    e = rng.standard_normal((400, 4))
+    tan = e.copy()
+    dW, dtl = ob.tangent(st["m"], s0["t"], tape, te, tan)
+    lhs = (g * e).sum()
+    rhs = 0.3 * dW - 2.0 * dtl + (lam * tan).sum()
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
