@@ -2046,6 +2046,11 @@ struct QueueState {
      * event, written only here while the loop runs and copied to HBM when
      * it ends */
     EdmdNode top[64];
+    /* collision virial sums since the last thermo row (collTermX, collTermY,
+     * collTermXY, collTermYX), accumulated by lane 0 in the reference's
+     * order; here rather than in scalar registers, which the loop has too
+     * few of */
+    double vir[4];
 };
 
 /* Top level of the calendar between HBM and the LDS copy. */
@@ -2076,9 +2081,6 @@ struct LoopRegs {
     int64_t srank; /* rank of the next special event */
     int64_t swin;  /* bucket of the next special event (-1: overflow list) */
     int cur;       /* win modulo the bucket count */
-    /* collision virial sums since the last thermo row (collTermX, collTermY,
-     * collTermXY, collTermYX), accumulated here in the reference's order */
-    double vir[4];
     QueueState* q; /* LDS: candidate stage and the calendar's top level */
 };
 
@@ -2295,7 +2297,7 @@ __device__ __forceinline__ void ev_thermo(const EdmdConst& C, const AR& A, int l
     /* EDMD.c:5253-5256: the disc's area inside a circular wall */
     const double area = (C.walls & 4) ? 3.14159265358979323846 * C.halfLx * C.halfLx : C.Lx * C.Ly;
     const double dT = t - st->last_thermo;
-    const double v0 = R.vir[0], v1 = R.vir[1], v2 = R.vir[2], v3 = R.vir[3];
+    const double v0 = R.q->vir[0], v1 = R.q->vir[1], v2 = R.q->vir[2], v3 = R.q->vir[3];
     const int k = st->nthermo;
     if (lane == 0) {
         if (k < C.thermo_cap) {
@@ -2340,7 +2342,7 @@ __device__ __forceinline__ void ev_thermo(const EdmdConst& C, const AR& A, int l
         st->rank_thermo = rk;
         st->win_thermo = wk;
     }
-    R.vir[0] = R.vir[1] = R.vir[2] = R.vir[3] = 0;
+    if (lane == 0) R.q->vir[0] = R.q->vir[1] = R.q->vir[2] = R.q->vir[3] = 0;
     WAVE_SYNC();
 }
 
@@ -2480,11 +2482,12 @@ __device__ __forceinline__ void ev_collision(const EdmdConst& C, const AR& A, in
     vyj = uni(vyj - ff * mi * dy);
     xj = uni(xj);
     yj = uni(yj);
-    R.vir[0] += w * dx * dx;
-    R.vir[1] += w * dy * dy;
-    R.vir[2] += w * dx * dy;
-    R.vir[3] += w * dy * dx;
     if (lane == 0) {
+        double* vir = R.q->vir;
+        vir[0] += w * dx * dx;
+        vir[1] += w * dy * dy;
+        vir[2] += w * dx * dy;
+        vir[3] += w * dy * dx;
         EdmdState* st = A.st;
         auto q = A.p + i;
         q->x = xi; q->y = yi; q->vx = vxi; q->vy = vyi; q->tl = t; q->coll = colli + 1;
@@ -2985,7 +2988,8 @@ __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRu
     R.ctr = uni64(st->ctr);
     R.win = uni64(st->win);
     R.cur = uni((int)(R.win % C.nlists));
-    for (int k = 0; k < 4; ++k) R.vir[k] = uni(st->vir[k]);
+    if (lane == 0)
+        for (int k = 0; k < 4; ++k) R.q->vir[k] = st->vir[k];
     next_special(st, R);
     top_load(C, A, lane, R.q->top);
     uint64_t ncross = st->ncross;
@@ -3062,7 +3066,7 @@ __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRu
         st->paul = R.paul;
         st->ctr = R.ctr;
         st->win = R.win;
-        for (int k = 0; k < 4; ++k) st->vir[k] = R.vir[k];
+        for (int k = 0; k < 4; ++k) st->vir[k] = R.q->vir[k];
     }
     top_store(C, A, lane, R.q->top);
 }
