@@ -72,6 +72,9 @@ extern "C" void edmd_default_desc(edmd_system_desc* d) {
     d->mid_wall = 0;
     d->type = nullptr;
     d->sizeratio = 0.4; /* EDMD.c:83 */
+    d->thermal_wall = 0;
+    d->shear_wall = 0;
+    d->rng_seed = 5489; /* mersenne.c's seed when none is given */
 }
 
 /* ------------------------------------------------------------ handles */
@@ -111,6 +114,7 @@ struct edmd_batch {
     /* growth starts: per-particle growth rates; any_grow while a system grows */
     double* vr = nullptr;
     int32_t* ptype = nullptr; /* disk types, when a system has the mid wall */
+    uint32_t* mt = nullptr;   /* MT19937 streams, 625 words per system (thermal walls) */
     bool any_grow = false;
     /* results of the step-level calls (peek, pair time), allocated once */
     EdmdPeek* step_peek = nullptr;
@@ -265,6 +269,14 @@ static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
     c.walls = (d->wall_x ? 1 : 0) | (d->wall_y ? 2 : 0) | (d->wall_circle ? 4 : 0) |
               (d->mid_wall ? 8 : 0);
     c.sizeratio = d->sizeratio;
+    if ((d->thermal_wall != 0 && d->thermal_wall != 1) || (d->shear_wall != 0 && d->shear_wall != 1))
+        return fail(EDMD_ERR_ARG, "system %d: thermal_wall and shear_wall must be 0 or 1", idx);
+    if (d->thermal_wall && !c.walls)
+        return fail(EDMD_ERR_ARG, "system %d: thermal walls need walls", idx);
+    if (d->shear_wall && !d->wall_y)
+        return fail(EDMD_ERR_ARG, "system %d: shear walls need the walls normal to y", idx);
+    c.thermal_wall = d->thermal_wall;
+    c.shear_wall = d->shear_wall;
     c.res_wall = d->res_wall;
     c.res_tan = d->res_tan;
     c.mihx = (d->wall_x || d->wall_circle) ? __builtin_huge_val() : c.halfLx;
@@ -352,6 +364,15 @@ static int upload_system(edmd_batch* b, int s, const edmd_system_desc* d) {
     HIPCHK(hipMemcpy(b->ccnt + h.cell0, cnt.data(), ncell * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->cslot + h.slot0, slot.data(), ncell * c.cap * sizeof(int32_t),
                      hipMemcpyHostToDevice));
+    if (b->mt) {
+        /* init_genrand(rng_seed) of mersenne.c; the index 624 makes the
+         * first draw twist */
+        uint32_t w[625];
+        w[0] = d->rng_seed;
+        for (int k = 1; k < 624; k++) w[k] = 1812433253u * (w[k - 1] ^ (w[k - 1] >> 30)) + (uint32_t)k;
+        w[624] = 624;
+        HIPCHK(hipMemcpy(b->mt + (size_t)s * 625, w, sizeof w, hipMemcpyHostToDevice));
+    }
     if (b->ptype) {
         std::vector<int32_t> ty(n, 0);
         if (d->type)
@@ -492,6 +513,9 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
     bool any_mid = false;
     for (int s = 0; s < nsys; s++) any_mid |= descs[s].mid_wall != 0;
     if (!rc && any_mid) rc = dalloc(b, &b->ptype, np);
+    bool any_thermal = false;
+    for (int s = 0; s < nsys; s++) any_thermal |= descs[s].thermal_wall != 0;
+    if (!rc && any_thermal) rc = dalloc(b, &b->mt, (size_t)nsys * 625);
     if (rc) { batch_free(b); return rc; }
     if (hipStreamCreate(&b->stream) != hipSuccess) {
         batch_free(b);
@@ -518,6 +542,7 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
         A.thermo = b->thermo + h.thermo0;
         A.vr = b->vr ? b->vr + p0 : nullptr;
         A.ptype = b->ptype ? b->ptype + p0 : nullptr;
+        A.mt = b->mt ? b->mt + (size_t)s * 625 : nullptr;
         A.erank = b->erank + 2 * p0;
         A.lrank = b->lrank + p0;
         rc = upload_system(b, s, descs + s);
@@ -1700,7 +1725,8 @@ static std::vector<PoolSpan> state_pools(edmd_batch* b) {
             {b->cslot, b->nslot * sizeof(int32_t)},
             {b->st, (size_t)b->nsys * sizeof(EdmdState)},
             {b->erank, 2 * b->np * sizeof(int64_t)},
-            {b->lrank, b->np * sizeof(int64_t)}};
+            {b->lrank, b->np * sizeof(int64_t)},
+            {b->mt, b->mt ? (size_t)b->nsys * 625 * sizeof(uint32_t) : 0}};
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1770,10 +1796,11 @@ extern "C" int edmd_batch_deriv_begin(edmd_batch* b, int64_t checkpoint_every, i
     if (b->any_grow)
         return fail(EDMD_ERR_ARG, "derivatives need systems whose growth has ended");
     for (int s = 0; s < b->nsys; s++)
-        if ((b->hs[s].c.walls & 4) || b->hs[s].c.field_on || b->hs[s].c.damping)
+        if ((b->hs[s].c.walls & 4) || b->hs[s].c.field_on || b->hs[s].c.damping ||
+            b->hs[s].c.thermal_wall || b->hs[s].c.shear_wall)
             return fail(EDMD_ERR_ARG,
-                        "system %d: derivatives need a system without the circular wall, field "
-                        "or damping (flat walls are taped)", s);
+                        "system %d: derivatives need a system without the circular wall, field, "
+                        "damping, thermal or shear walls (flat walls are taped)", s);
     int rc = edmd_batch_deriv_end(b);
     if (rc) return rc;
     if ((double)b->nsys * (double)cap * sizeof(EdmdTapeRec) > 1.0e12)

@@ -405,6 +405,67 @@ __device__ __forceinline__ double edmd_smallest_root(double a, double b, double 
     return best;
 }
 
+/* MT19937 (Matsumoto and Nishimura, ACM TOMACS 8(1), 1998), the generator
+ * behind the reference's drand (mersenne.c genrand_int32; drand(a, b) =
+ * a + (b - a) k / 2^32, EDMD.c:5699-5701): one stream per system, its 624
+ * state words and the index of the next one (word 624) in HBM.  Word k of
+ * the twist is mt[(k + 397) % 624] ^ (y >> 1) ^ (y odd ? 0x9908b0df : 0)
+ * with y the upper bit of mt[k] and the lower 31 of mt[(k + 1) % 624]; in
+ * the sequential order of the algorithm the words [0, 227) read only old
+ * words, [227, 454) the new [0, 227), [454, 623) the new [227, 396) and
+ * word 623 the new words 0 and 396, so each range runs wave-parallel once
+ * the one before is stored. */
+__device__ __forceinline__ void mt_twist_range(uint32_t EDMD_GLOBAL* mt, int lane, int a, int b) {
+    uint32_t nw[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int k = a + lane + 64 * r;
+        nw[r] = 0;
+        if (k < b) {
+            const uint32_t y = (mt[k] & 0x80000000u) | (mt[k + 1 == 624 ? 0 : k + 1] & 0x7fffffffu);
+            nw[r] = mt[k + 397 < 624 ? k + 397 : k - 227] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int k = a + lane + 64 * r;
+        if (k < b) mt[k] = nw[r];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+}
+
+/* The system's next 32-bit output (uniform). */
+template <class AR>
+__device__ __forceinline__ uint32_t mt_next(const AR& A, int lane) {
+    uint32_t EDMD_GLOBAL* mt = A.mt;
+    int idx = uni((int)mt[624]);
+    if (idx >= 624) {
+        mt_twist_range(mt, lane, 0, 227);
+        mt_twist_range(mt, lane, 227, 454);
+        mt_twist_range(mt, lane, 454, 623);
+        mt_twist_range(mt, lane, 623, 624);
+        idx = 0;
+    }
+    uint32_t y = (uint32_t)uni((int)mt[idx]);
+    y ^= (y >> 11);
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= (y >> 18);
+    if (lane == 0) mt[624] = (uint32_t)(idx + 1);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    return y;
+}
+
+/* drand(0, hi) of the reference (EDMD.c:5699-5701) on the system's stream. */
+template <class AR>
+__device__ __forceinline__ double mt_drand0(const AR& A, int lane, double hi) {
+    const double g = mt_next(A, lane) * (1.0 / 4294967296.0);
+    return (g * (hi - 0.0)) + 0.0;
+}
+
+/* sign of EDMD.c:5703-5705: -1, 0 or 1. */
+__device__ __forceinline__ double ref_sign(double x) { return (x < 0) ? -1.0 : (double)(x > 0); }
+
 /* logTime, EDMD.c:5709-5721: under damping, the time at which the damped
  * motion has covered what the undamped one covers in t; never when it
  * cannot.  F = false compiles damping out. */
@@ -2829,20 +2890,38 @@ __device__ __forceinline__ void ev_single(const EdmdConst& C, const AR& A, int l
                     else dpm = dp;
                 }
             }
-            vx = -C.res_wall * vx;
-            vy = C.res_tan * vy + 0.0;
+            if (C.thermal_wall) {
+                /* thermoWall (EDMD.c:3411-3413): the normal speed drawn */
+                vx = -ref_sign(vx) * mt_drand0(A, lane, C.temperature / m);
+            } else {
+                vx = -C.res_wall * vx;
+                vy = C.res_tan * vy + 0.0;
+            }
         } else if (d == 6) {
             dp = fabs((C.res_wall + 1) * m * vy);
-            vy = -C.res_wall * vy;
-            vx = C.res_tan * vx + 0.0;
+            if (C.thermal_wall) {
+                vy = -ref_sign(vy) * mt_drand0(A, lane, C.temperature / m);
+            } else {
+                vy = -C.res_wall * vy;
+                vx = C.res_tan * vx + 0.0;
+            }
+            /* addShearWall (EDMD.c:3429-3439, without the delta model's
+             * kick): the top wall drives +0.1 along x, the bottom -0.1 */
+            if (C.shear_wall) vx = (y + 2 * r > C.Ly) ? 0.1 : -0.1;
         } else {
             /* the circle (EDMD.c:3441-3454): the normal velocity reversed
-             * times resW, |m mix| to the wall */
+             * times resW (thermal: both components drawn), |m mix| to the
+             * wall */
             double nx, ny;
             edmd_sincos(edmd_atan2(y - C.halfLx, x - C.halfLx), ny, nx);
             const double mix = -(1 + C.res_wall) * (vx * nx + vy * ny);
-            vx += mix * nx;
-            vy += mix * ny;
+            if (C.thermal_wall) {
+                vx = -nx * mt_drand0(A, lane, C.temperature);
+                vy = -ny * mt_drand0(A, lane, C.temperature);
+            } else {
+                vx += mix * nx;
+                vy += mix * ny;
+            }
             dp = fabs(m * mix);
         }
         vx = uni(vx);
@@ -3130,6 +3209,7 @@ struct EdmdArraysL {
     double EDMD_GLOBAL* otl;
     EdmdDeriv EDMD_GLOBAL* dr;
     int32_t EDMD_GLOBAL* ptype;
+    uint32_t EDMD_GLOBAL* mt;
 };
 
 /* Copies one array between HBM and LDS with the whole wavefront, in 16-,
@@ -3217,6 +3297,7 @@ __global__ __launch_bounds__(64) void k_run_lds(const EdmdSys* __restrict__ syst
     A.otl = H.otl;
     A.dr = H.dr;
     A.ptype = H.ptype;
+    A.mt = H.mt;
     WAVE_SYNC();
     lds_system<true, G>(C, H, A, lane);
     WAVE_SYNC();

@@ -89,6 +89,8 @@ typedef struct EdmdConst {
     double field;     /* its value (< 0) */
     double gamma;     /* damping rate (gamm) */
     int32_t mono;     /* every disk has the same radius (bitwise) */
+    int32_t thermal_wall; /* wall hits draw the new normal speed (thermoWall) */
+    int32_t shear_wall;   /* y-wall hits set vx to +-0.1 (addShearWall) */
     int32_t pad_m;
 } EdmdConst;
 
@@ -259,6 +261,7 @@ typedef struct EdmdArrays {
     double EDMD_GLOBAL* otl;         /* particle local times when taping began */
     EdmdDeriv EDMD_GLOBAL* dr;
     int32_t EDMD_GLOBAL* ptype;      /* disk types (mid wall) */
+    uint32_t EDMD_GLOBAL* mt;        /* MT19937 state (624 words) and index (thermal walls) */
 } EdmdArrays;
 
 typedef struct EdmdSys {

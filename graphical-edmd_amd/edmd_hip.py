@@ -46,7 +46,8 @@ class SystemDesc(C.Structure):
                 ("wall_x", C.c_int), ("wall_y", C.c_int), ("res_wall", C.c_double),
                 ("res_tan", C.c_double), ("add_field", C.c_int), ("field", C.c_double),
                 ("damping", C.c_int), ("gamma", C.c_double), ("wall_circle", C.c_int),
-                ("mid_wall", C.c_int), ("type", C.c_void_p), ("sizeratio", C.c_double)]
+                ("mid_wall", C.c_int), ("type", C.c_void_p), ("sizeratio", C.c_double),
+                ("thermal_wall", C.c_int), ("shear_wall", C.c_int), ("rng_seed", C.c_uint32)]
 
 
 class Observables(C.Structure):

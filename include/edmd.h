@@ -68,7 +68,8 @@ typedef struct edmd_system_desc {
     double dtime_thermo;    /* thermo period, EDMD.c:247 (-o) */
     int noise;              /* 0 none, 2 velocity-rescale thermostat, EDMD.c:4880-4915 */
     double dtnoise;         /* thermostat period, EDMD.c:386 */
-    double temperature;     /* thermostat target E/N, EDMD.c:379 */
+    double temperature;     /* T: thermostat target E/N and thermal walls' scale, EDMD.c:383
+                               (the reference's default is 0.05, this one's 1) */
     int cell_capacity;      /* max disks per cell, 0 = derived from the radii */
     int64_t trace_capacity; /* device event-trace records kept, 0 = none */
     int thermo_capacity;    /* thermo rows kept, 0 = 4096 */
@@ -149,6 +150,21 @@ typedef struct edmd_system_desc {
     int mid_wall;
     const int32_t* type;
     double sizeratio;
+    /* Thermal walls (thermoWall of the graphics build, EDMD.c:280): a hit
+     * of a flat wall sets the normal velocity to -sign(v_n) drand(0, T/m)
+     * and keeps the tangential one (EDMD.c:3411-3426); a hit of the
+     * circular wall sets v = -n drand(0, T), one draw per component
+     * (EDMD.c:3446-3452); T = temperature.  drand(a, b) = a + (b - a)
+     * k / 2^32 for the next 32-bit output k of the system's own MT19937
+     * stream (mersenne.c), seeded by init_genrand(rng_seed).  Needs walls;
+     * not with taping. */
+    int thermal_wall;
+    /* Shear walls (addShearWall, EDMD.c:283): after a hit of a wall normal
+     * to y the disk's x velocity is +0.1 at the top (y + 2 rad > Ly) and
+     * -0.1 at the bottom (EDMD.c:3429-3435; the delta model's kick of
+     * EDMD.c:3436-3438 is not built).  Needs wall_y; not with taping. */
+    int shear_wall;
+    uint32_t rng_seed; /* default 5489, mersenne.c's seed when none is given */
 } edmd_system_desc;
 
 /* A collision slot holding a wall event reports partner -5 - xy

@@ -63,7 +63,52 @@ struct orc_ctx {
     double dp; /* momentum given to the walls since the last thermo row */
     int* type; /* disk types (the mid wall stops type 1) */
     double dp_left, dp_mid, dp_right; /* dpLeft, dpMid, dpRight (EDMD.c:146-148) */
+    uint32_t mt[624]; /* MT19937 stream of the thermal walls */
+    int mti;
 };
+
+/* ------------------------------------------------------------ MT19937 */
+
+/* init_genrand and genrand_int32 of mersenne.c (Matsumoto and Nishimura,
+ * ACM TOMACS 8(1), 1998, with the 2002 initialisation), restated in order:
+ * the state from a 32-bit seed, the twist of all 624 words when they are
+ * used up, the tempering of the word drawn. */
+static void mt_seed(uint32_t* mt, int* mti, uint32_t seed) {
+    mt[0] = seed;
+    for (int k = 1; k < 624; k++) mt[k] = 1812433253u * (mt[k - 1] ^ (mt[k - 1] >> 30)) + (uint32_t)k;
+    *mti = 624;
+}
+
+static uint32_t mt_next(uint32_t* mt, int* mti) {
+    if (*mti >= 624) {
+        for (int k = 0; k < 624; k++) {
+            uint32_t y = (mt[k] & 0x80000000u) | (mt[(k + 1) % 624] & 0x7fffffffu);
+            mt[k] = mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+        *mti = 0;
+    }
+    uint32_t y = mt[(*mti)++];
+    y ^= (y >> 11);
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= (y >> 18);
+    return y;
+}
+
+void orc_mt_draws(uint32_t seed, int n, uint32_t* out) {
+    uint32_t mt[624];
+    int mti;
+    mt_seed(mt, &mti, seed);
+    for (int k = 0; k < n; k++) out[k] = mt_next(mt, &mti);
+}
+
+/* drand(0, hi) (EDMD.c:5699-5701) and sign (EDMD.c:5703-5705). */
+static double drand0(orc_ctx* s, double hi) {
+    double g = mt_next(s->mt, &s->mti) * (1.0 / 4294967296.0);
+    return (g * (hi - 0.0)) + 0.0;
+}
+
+static double ref_sign(double x) { return (x < 0) ? -1.0 : (double)(x > 0); }
 
 /* ------------------------------------------------------------ kinematics */
 
@@ -1109,20 +1154,34 @@ static void do_wall(orc_ctx* s, const ev_node* ev) {
                 else s->dp_mid += dpi;
             }
         }
-        s->vx[i] = -rw * s->vx[i];
-        s->vy[i] = rt * s->vy[i] + 0.0;
+        if (s->prm.thermal_wall) {
+            s->vx[i] = -ref_sign(s->vx[i]) * drand0(s, s->prm.temperature / s->m[i]);
+        } else {
+            s->vx[i] = -rw * s->vx[i];
+            s->vy[i] = rt * s->vy[i] + 0.0;
+        }
     } else if (xy == 1) {
         s->dp += fabs((rw + 1) * s->m[i] * s->vy[i]);
-        s->vy[i] = -rw * s->vy[i];
-        s->vx[i] = rt * s->vx[i] + 0.0;
+        if (s->prm.thermal_wall) {
+            s->vy[i] = -ref_sign(s->vy[i]) * drand0(s, s->prm.temperature / s->m[i]);
+        } else {
+            s->vy[i] = -rw * s->vy[i];
+            s->vx[i] = rt * s->vx[i] + 0.0;
+        }
+        if (s->prm.shear_wall) s->vx[i] = (s->y[i] + 2 * s->rad[i] > s->Ly) ? 0.1 : -0.1;
     } else {
         /* the circle (EDMD.c:3441-3454): the normal velocity reversed
          * times resW, the momentum |m mix| to the wall */
         double nx, ny;
         orc_sincos(orc_atan2(s->y[i] - s->halfLx, s->x[i] - s->halfLx), &ny, &nx);
         double mix = -(1 + rw) * (s->vx[i] * nx + s->vy[i] * ny);
-        s->vx[i] += mix * nx;
-        s->vy[i] += mix * ny;
+        if (s->prm.thermal_wall) {
+            s->vx[i] = -nx * drand0(s, s->prm.temperature);
+            s->vy[i] = -ny * drand0(s, s->prm.temperature);
+        } else {
+            s->vx[i] += mix * nx;
+            s->vy[i] += mix * ny;
+        }
         s->dp += fabs(s->m[i] * mix);
     }
     s->coll[i]++;
@@ -1340,6 +1399,9 @@ void orc_default_params(orc_params* p) {
     p->mid_wall = 0;
     p->type = NULL;
     p->sizeratio = 0.4;
+    p->thermal_wall = 0;
+    p->shear_wall = 0;
+    p->rng_seed = 5489;
 }
 
 orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* y,
@@ -1407,6 +1469,7 @@ orc_ctx* orc_create(int n, double Lx, double Ly, const double* x, const double* 
         s->type[i] = s->prm.type ? s->prm.type[i] : 0;
     }
     s->prm.type = NULL; /* copied */
+    mt_seed(s->mt, &s->mti, s->prm.rng_seed);
     if (s->prm.vr > 0) {
         /* growth start (EDMD.c:1770-1773): radius 0, rate vr * target */
         s->growing = 1;

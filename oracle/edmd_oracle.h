@@ -99,6 +99,15 @@ typedef struct {
     int mid_wall;
     const int32_t* type;
     double sizeratio; /* EDMD.c:83, default 0.4 */
+    /* Thermal walls (thermoWall, EDMD.c:3411-3452): a flat wall's hit sets
+     * the normal velocity to -sign(v_n) drand(0, T/m), the circle's sets
+     * v = -n drand(0, T) (one draw per component); drand(a, b) = a + (b - a)
+     * k / 2^32 for the next output k of an MT19937 stream seeded by
+     * init_genrand(rng_seed).  Shear walls (addShearWall, EDMD.c:3429-3435):
+     * a y-wall hit then sets vx to +0.1 (y + 2 rad > Ly) or -0.1. */
+    int thermal_wall;
+    int shear_wall;
+    uint32_t rng_seed; /* default 5489 */
 } orc_params;
 
 typedef struct {
@@ -229,6 +238,10 @@ double orc_atan2(double y, double x);
 /* Smallest real root above 1e-9 of a t^4 + b t^3 + c t^2 + d t + e (1e10
  * if none), the circular wall's time under the field. */
 double orc_smallest_root(double a, double b, double c, double d, double e);
+
+/* The first n outputs of MT19937 seeded by init_genrand(seed) (the
+ * generator of the thermal walls). */
+void orc_mt_draws(uint32_t seed, int n, uint32_t* out);
 
 /* Sum of w_k = m_i m_j ff |a|^2 over the taped collisions, in tape order. */
 double orc_tape_wsum(const orc_ctx* s);

@@ -25,7 +25,8 @@ class Params(C.Structure):
                 ("wall_x", C.c_int), ("wall_y", C.c_int), ("res_wall", C.c_double),
                 ("res_tan", C.c_double), ("add_field", C.c_int), ("field", C.c_double),
                 ("damping", C.c_int), ("gamma", C.c_double), ("wall_circle", C.c_int),
-                ("mid_wall", C.c_int), ("type", C.c_void_p), ("sizeratio", C.c_double)]
+                ("mid_wall", C.c_int), ("type", C.c_void_p), ("sizeratio", C.c_double),
+                ("thermal_wall", C.c_int), ("shear_wall", C.c_int), ("rng_seed", C.c_uint32)]
 
 
 class ThermoRow(C.Structure):
@@ -102,6 +103,8 @@ def lib():
         L.orc_atan2.argtypes = [C.c_double, C.c_double]
         L.orc_smallest_root.restype = C.c_double
         L.orc_smallest_root.argtypes = [C.c_double] * 5
+        L.orc_mt_draws.restype = None
+        L.orc_mt_draws.argtypes = [C.c_uint32, C.c_int, C.c_void_p]
         L.orc_tape_wsum.restype = C.c_double
         L.orc_tape_wsum.argtypes = [C.c_void_p]
         L.orc_replay.restype = C.c_double
@@ -113,6 +116,13 @@ def lib():
 
 TAPE_DT = np.dtype([("i", "<i4"), ("j", "<i4"), ("t", "<f8"), ("ax", "<f8"),
                     ("ay", "<f8"), ("cx", "<f8"), ("cy", "<f8"), ("ff", "<f8")])
+
+
+def mt_draws(seed, n):
+    """The first n outputs of the oracle's MT19937 seeded by init_genrand."""
+    out = np.zeros(n, dtype=np.uint32)
+    lib().orc_mt_draws(int(seed), int(n), out.ctypes.data)
+    return out
 
 
 def tangent(m, tl0, tape, t_end, tan):

@@ -144,7 +144,12 @@ def test_wall_arguments_are_checked(gpu):
     st = F.lattice_state(100, 0.4, seed=5, lattice="fit")
     with pytest.raises(gpu.EdmdError):
         gpu.System(st, wall_y=2)
+    # flat walls are taped (tests/test_gpu_deriv.py), the circle is not
     b = gpu.Batch([st], wall_y=1)
+    b.deriv_begin(checkpoint_every=256)
+    b.deriv_end()
+    c = in_circle(st)
+    b = gpu.Batch([c], wall_circle=1)
     with pytest.raises(gpu.EdmdError):
         b.deriv_begin(checkpoint_every=256)
 
@@ -431,3 +436,82 @@ def test_mid_wall_matches_oracle(gpu, oracle, n, L, seed, vr, kw):
             for k in ("t", "e_per_n", "pressure", "pwall", "pmid", "dp_lr", "pack_left",
                       "pack_right", "temp_left", "temp_right", "temp_solute"):
                 assert bits(np.float64(r[k])) == bits(np.float64(q[k])), k
+
+
+# ---- thermal and shear walls (thermoWall, addShearWall)
+
+def _thermal_state(kind, seed):
+    if kind == "box":
+        return walled(400, 0.45, seed, dict(wall_x=1, wall_y=1))
+    if kind == "channel":
+        return walled(400, 0.45, seed, dict(wall_y=1))
+    if kind == "circle":
+        return in_circle(F.lattice_state(450, 0.5, seed, lattice="fit"))
+    if kind == "mid":
+        return osmosis(280, 40.0, seed)
+    if kind == "grow":
+        return F.growth_state(350, 0.55, seed, poly=0.0)
+    raise ValueError(kind)
+
+
+THERMAL_CASES = [
+    ("box", 101, 0.0, dict(wall_x=1, wall_y=1, temperature=0.6, rng_seed=7)),
+    ("channel", 102, 0.0, dict(wall_y=1, shear_wall=1, thermal_wall=0, res_tan=1.0)),
+    ("channel", 103, 0.0, dict(wall_y=1, shear_wall=1, temperature=1.5, rng_seed=8)),
+    ("circle", 104, 0.0, dict(wall_circle=1, temperature=0.9, rng_seed=9)),
+    ("mid", 105, 0.0, dict(wall_x=1, wall_y=1, mid_wall=1, sizeratio=0.5, temperature=1.0,
+                            rng_seed=10)),
+    ("grow", 106, 0.1, dict(wall_x=1, wall_y=1, temperature=0.8, rng_seed=11)),
+]
+
+
+@pytest.mark.parametrize("kind,seed,vr,kw", THERMAL_CASES)
+def test_thermal_and_shear_walls_match_oracle(gpu, oracle, kind, seed, vr, kw):
+    """Wall hits that draw the new normal speed from the system's own
+    MT19937 stream (twisted wave-parallel on the GPU, sequentially in the
+    checker), the circle's two draws, shear walls driving +-0.1 along x,
+    the mid wall's solute and a growth start whose walls turn thermal at
+    GROWSTOP: traces, states and thermo rows bit for bit against the
+    oracle, from HBM and LDS; then the same after the autotuner's trials,
+    which must restore the streams as well."""
+    kw = dict(kw, first_screen=1e9, dtime=1e9, first_thermo=1.0, dtime_thermo=1.0)
+    kw.setdefault("thermal_wall", 1)
+    st = _thermal_state(kind, seed)
+    nev = 40000 if vr > 0 else 20000
+    o = oracle.Oracle(st, vr=vr, **kw)
+    code, tr = o.run(nev)
+    assert code == 0
+    if vr > 0:
+        assert np.sum(tr["type"] == F.GROWSTOP) == 1
+    assert np.sum(tr["type"] == WALL) > (60 if kw["thermal_wall"] else 20)
+    so, orows = o.state(), o.thermo_rows()
+    for mode in (gpu.RES_HBM, gpu.RES_LDS, "tuned"):
+        b = gpu.Batch([st], trace_capacity=nev, thermo_capacity=256, vr=vr, **kw)
+        if mode == "tuned":
+            b.autotune(sweep_reps=1, trial_events=3000)
+        else:
+            b.set_residency(mode)
+        b.run(max_events=nev)
+        g, c = F.canonical_trace(b.trace(0)), F.canonical_trace(tr)
+        for k in ("type", "i", "j", "valid"):
+            np.testing.assert_array_equal(g[k], c[k], err_msg="%s %s" % (mode, k))
+        np.testing.assert_array_equal(bits(g["t"]), bits(c["t"]), err_msg=str(mode))
+        assert_state_equal(b.state(0), so)
+        rows = b.thermo(0)
+        assert len(rows) == len(orows)
+        for r, q in zip(rows, orows):
+            for k in ("t", "e_per_n", "pressure", "pwall"):
+                assert bits(np.float64(r[k])) == bits(np.float64(q[k])), (mode, k)
+
+
+def test_thermal_and_shear_arguments_are_checked(gpu):
+    st = walled(200, 0.4, 5, dict(wall_y=1))
+    with pytest.raises(gpu.EdmdError):
+        gpu.System(st, thermal_wall=1)             # no walls
+    with pytest.raises(gpu.EdmdError):
+        gpu.System(st, wall_x=1, shear_wall=1)     # shear walls need wall_y
+    with pytest.raises(gpu.EdmdError):
+        gpu.System(st, wall_y=1, thermal_wall=2)
+    b = gpu.Batch([st], wall_y=1, thermal_wall=1)
+    with pytest.raises(gpu.EdmdError):
+        b.deriv_begin()
