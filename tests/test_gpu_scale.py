@@ -10,6 +10,11 @@
   (growth traces canonically, states and radii bitwise) over their first
   events, then all 64 through GROWSTOP and two more rescales, with the
   thermostat's E/N, the momentum stopGrow removed and every calendar.
+* 512 replicas of N = 400 at phi = 0.60, 10^4 collisions each, taped with a
+  checkpoint every 256 events: three replicas' virial sums, tangents and
+  gradients against the oracle bit for bit, and for every replica the
+  adjoint/tangent duality <grad, e> = lw dW + lt dt_last and a positive
+  finite-time growth rate.
 """
 import numpy as np
 import pytest
@@ -108,3 +113,39 @@ def test_dense_polydisperse_growth_thermostat_64x65536(gpu, oracle):
         np.testing.assert_allclose(b.radii(k), states[k]["rad"], rtol=1e-12, err_msg="%d" % k)
         bad = check_calendar(b.calendar(k), b.events(k))
         assert bad == [], (k, bad[:3])
+
+
+def test_adjoint_batch_512x400(gpu, oracle):
+    nrep, n, ncoll, every, seed = 512, 400, 10000, 256, 1000
+    states = [gpu.init_lattice(n, 0.60, rad=0.5, lattice=2, seed=seed + k) for k in range(nrep)]
+    rng = np.random.default_rng(seed)
+    tin = [rng.standard_normal((n, 4)) for _ in range(nrep)]
+    b = gpu.Batch(states)
+    b.deriv_begin(checkpoint_every=every)
+    b.deriv_run(max_collisions=ncoll)
+    vals = [b.deriv_values(k) for k in range(nrep)]
+    W = np.array([v[0] for v in vals])
+    T = np.array([v[1] for v in vals])
+    lw, lt = 1.0 / T, -W / T ** 2
+    grad = b.adjoint(lw=lw, lt=lt)
+    tout, dw, dtl = b.tangent(tin)
+    for k in (0, 255, 511):
+        o = oracle.Oracle(states[k])
+        s0 = o.state()
+        o.tape_enable(ncoll + 16)
+        o.run_collisions(ncoll)
+        tape = o.tape()
+        assert bits(W[k]) == bits(o.tape_wsum()) and T[k] == tape["t"][-1], k
+        tan = tin[k].copy()
+        rdw, rdtl = oracle.tangent(states[k]["m"], s0["t"], tape, o.t, tan)
+        np.testing.assert_array_equal(bits(tout[k]), bits(tan), err_msg="tangent %d" % k)
+        assert bits(dw[k]) == bits(rdw) and bits(dtl[k]) == bits(rdtl), k
+        g = oracle.adjoint(states[k]["m"], s0["t"], tape, o.t, lw[k], lt[k])
+        np.testing.assert_array_equal(bits(grad[k]), bits(g), err_msg="adjoint %d" % k)
+    for k in range(nrep):
+        assert np.all(np.isfinite(grad[k])) and np.all(np.isfinite(tout[k])), k
+        lhs = float((grad[k] * tin[k]).sum())
+        rhs = lw[k] * dw[k] + lt[k] * dtl[k]
+        assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), abs(lw[k] * dw[k])), k
+        growth = np.log(np.linalg.norm(tout[k]) / np.linalg.norm(tin[k])) / T[k]
+        assert 1.0 < growth < 20.0, (k, growth)
