@@ -403,6 +403,19 @@ def main():
     plain = (o1["ncol"] - o0["ncol"]) / (t1 - t0)
     plain_ev = (o1["nevents"] - o0["nevents"]) / (t1 - t0)
     log("plain: %d collisions in %.3f s = %.4e coll/s" % (o1["ncol"] - o0["ncol"], t1 - t0, plain))
+    # the sweep's conservative root screen on the state the loop left:
+    # timed against the unscreened sweep, then counted (roots skipped, and
+    # misses: skipped roots the minimum would have taken)
+    screen = {"ms_unscreened": big.time_sweep(args.sweep_reps)}
+    big.set_screen(1)
+    screen["ms_screened"] = big.time_sweep(args.sweep_reps)
+    big.set_screen(2)
+    big.sweep_predict()
+    marks, solved, misses = big.screen_stats()
+    screen.update(marked=marks, solved=solved, misses=misses,
+                  solved_fraction=solved / marks if marks else None)
+    log("sweep screen: %.4f ms against %.4f ms unscreened, %d of %d roots solved, %d misses"
+        % (screen["ms_screened"], screen["ms_unscreened"], solved, marks, misses))
     big.close()
 
     adj = None
@@ -445,7 +458,7 @@ def main():
         "plain_collisions_per_s": plain,
         "plain_events_per_s": plain_ev,
         "batched_over_plain": value / plain,
-        "sweep": {"n": args.plain_n, "ms": sweep_ms, "GB_per_s": sweep_gbs,
+        "sweep": {"n": args.plain_n, "ms": sweep_ms, "GB_per_s": sweep_gbs, "screen": screen,
                   "tile": tuning["plain"]["sweep_tile"]},
         "tuning": tuning,
         "roofline": {"kernel": "k_sweep_tiled", "bound": "hbm", "achieved": sweep_gbs,

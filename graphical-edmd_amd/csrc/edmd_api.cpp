@@ -745,6 +745,43 @@ extern "C" int edmd_batch_get_sweep_shape(edmd_batch* b, int* shape, int* tile_x
     return EDMD_OK;
 }
 
+static int upload_table(edmd_batch* b);
+
+/* The sweep's conservative root screen for every system of the batch; the
+ * counters start again from zero. */
+extern "C" int edmd_batch_set_screen(edmd_batch* b, int mode) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    if (mode < 0 || mode > 2) return fail(EDMD_ERR_ARG, "screen mode %d not in {0, 1, 2}", mode);
+    std::vector<EdmdState> st(b->nsys);
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(st.data(), b->st, b->nsys * sizeof(EdmdState), hipMemcpyDeviceToHost));
+    for (int s = 0; s < b->nsys; s++) {
+        st[s].scr_marks = st[s].scr_solved = st[s].scr_miss = 0;
+        b->hs[s].c.screen = mode;
+        b->sys[s].c.screen = mode;
+    }
+    HIPCHK(hipMemcpy(b->st, st.data(), b->nsys * sizeof(EdmdState), hipMemcpyHostToDevice));
+    return upload_table(b);
+}
+
+extern "C" int edmd_batch_screen_stats(edmd_batch* b, uint64_t* marks, uint64_t* solved,
+                                       uint64_t* misses) {
+    if (!b) return fail(EDMD_ERR_ARG, "null batch");
+    std::vector<EdmdState> st(b->nsys);
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(st.data(), b->st, b->nsys * sizeof(EdmdState), hipMemcpyDeviceToHost));
+    uint64_t m = 0, so = 0, mi = 0;
+    for (int s = 0; s < b->nsys; s++) {
+        m += st[s].scr_marks;
+        so += st[s].scr_solved;
+        mi += st[s].scr_miss;
+    }
+    if (marks) *marks = m;
+    if (solved) *solved = so;
+    if (misses) *misses = mi;
+    return EDMD_OK;
+}
+
 static int check_status(edmd_batch* b);
 static size_t ckpt_bytes(edmd_batch* b);
 static int ckpt_copy(edmd_batch* b, void* mem, bool save);

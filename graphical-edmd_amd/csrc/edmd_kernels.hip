@@ -1870,6 +1870,17 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
                 }
             }
         }
+        /* the screen (uniform; not under damping, whose logTime could
+         * merge distinct times): with q = -b > 0, the computed root
+         * (q - fl(sqrt(det)))/v2 of a pair with d > 0 is at least
+         * d/(2q) - 2^-25 q/v2 (its exact value lies in [d/(2q), d/q] and
+         * the cancellation in det costs at most sqrt(3 2^-53) q/v2), so a
+         * mark with d v2 - 2^-19 q^2 > 2 q v2 dt (1 + 2^-40) has a root
+         * above the lane's best so far dt and is skipped: the strict-less
+         * minimum never takes it (a skip needs d > 0, so d <= 0 pairs are
+         * always solved) */
+        const bool scr = C.screen != 0 && !(X && C.damping);
+        uint32_t nmark = 0, nsolve = 0, nmiss = 0;
         while (m0 | m1 | m2 | m3) {
             int k;
             if (m0) { k = __builtin_ctzll(m0); m0 &= m0 - 1; }
@@ -1881,10 +1892,36 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
             pair_terms<WRAP>(C, ox, oy, ovx, ovy, orad, L.pos[v], L.vel[v],
                              MONO ? orad : L.rad[v], b, v2, d, det);
             if (!WRAP) ov = ov || d < -0.01; /* EDMD.c:2708-2717 */
+            if (scr) {
+                ++nmark;
+                const double q = -b;
+                const bool skip = dt > 0 && dt < EDMD_NEVER &&
+                                  d * v2 - 0x1p-19 * (q * q) > (2 * q * v2 * dt) * (1 + 0x1p-40);
+                if (skip) {
+                    if (C.screen == 2 && !(log_time<X>(C, (-b - sqrt(det)) / v2) > dt)) ++nmiss;
+                    continue;
+                }
+                ++nsolve;
+            }
             const double tt = log_time<X>(C, (-b - sqrt(det)) / v2);
             if (dt > tt) {
                 dt = tt;
                 best = v;
+            }
+        }
+        if (scr && C.screen == 2) { /* counted in the check mode only: the
+                                       atomics contend on one address */
+            uint32_t s0 = act ? nmark : 0, s1 = act ? nsolve : 0, s2 = act ? nmiss : 0;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                s0 += __shfl_xor(s0, off, 64);
+                s1 += __shfl_xor(s1, off, 64);
+                s2 += __shfl_xor(s2, off, 64);
+            }
+            if (lane == 0) {
+                atomicAdd((unsigned long long*)&A.st->scr_marks, (unsigned long long)s0);
+                atomicAdd((unsigned long long*)&A.st->scr_solved, (unsigned long long)s1);
+                if (s2) atomicAdd((unsigned long long*)&A.st->scr_miss, (unsigned long long)s2);
             }
         }
         if (!act) continue;

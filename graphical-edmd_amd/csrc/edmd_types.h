@@ -91,7 +91,7 @@ typedef struct EdmdConst {
     int32_t mono;     /* every disk has the same radius (bitwise) */
     int32_t thermal_wall; /* wall hits draw the new normal speed (thermoWall) */
     int32_t shear_wall;   /* y-wall hits set vx to +-0.1 (addShearWall) */
-    int32_t pad_m;
+    int32_t screen;       /* sweep root screen: 0 off, 1 on, 2 on and misses counted */
 } EdmdConst;
 
 /* Order among events of equal time.  The reference pops equal times in the
@@ -133,6 +133,9 @@ typedef struct EdmdState {
     uint64_t coll_limit;        /* per-system bound on ncol (epochs) */
     double dpw;                 /* momentum given to the walls since the last thermo row */
     double dpl, dpm, dpr;       /* dpLeft, dpMid, dpRight of the mid wall (EDMD.c:146-148) */
+    /* the sweep's root screen: marked candidates, roots solved, and (check
+     * mode) screened-out roots that the minimum would have taken */
+    uint64_t scr_marks, scr_solved, scr_miss;
 } EdmdState;
 
 /* LDS-resident runs: byte offsets of a system's arrays in its workgroup's

@@ -137,6 +137,8 @@ def lib():
             "edmd_batch_get_residency": (i32, [vp, vp, vp, vp]),
             "edmd_batch_set_sweep_shape": (i32, [vp, i32]),
             "edmd_batch_get_sweep_shape": (i32, [vp, vp, vp, vp, vp]),
+            "edmd_batch_set_screen": (i32, [vp, i32]),
+            "edmd_batch_screen_stats": (i32, [vp, vp, vp, vp]),
             "edmd_batch_autotune": (i32, [vp, i32, i64, vp]),
             "edmd_batch_set_loop_waves": (i32, [vp, i32]),
             "edmd_batch_get_loop_waves": (i32, [vp, vp]),
@@ -521,6 +523,17 @@ class Batch:
         v = [C.c_int() for _ in range(4)]
         check(lib().edmd_batch_get_sweep_shape(self.h, *[C.byref(x) for x in v]))
         return dict(zip(("shape", "tile_x", "tile_y", "threads"), [x.value for x in v]))
+
+    def set_screen(self, mode):
+        """The sweep's conservative root screen: 0 off, 1 on, 2 on and
+        misses counted (edmd_batch_set_screen); clears the counters."""
+        check(lib().edmd_batch_set_screen(self.h, int(mode)))
+
+    def screen_stats(self):
+        """(marked candidates, roots solved, misses) since set_screen."""
+        m, so, mi = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(lib().edmd_batch_screen_stats(self.h, C.byref(m), C.byref(so), C.byref(mi)))
+        return m.value, so.value, mi.value
 
     def autotune(self, sweep_reps=10, trial_events=2000):
         """Times every sweep shape and every event-loop configuration (arrays

@@ -358,6 +358,19 @@ int edmd_batch_get_residency(edmd_batch* b, int* in_lds, int64_t* lds_bytes,
 #define EDMD_TUNE_SWEEP_SHAPES 8
 int edmd_batch_set_sweep_shape(edmd_batch* b, int shape);
 int edmd_batch_get_sweep_shape(edmd_batch* b, int* shape, int* tile_x, int* tile_y, int* threads);
+/* The full sweep's conservative root screen (no counterpart in the
+ * reference; the FP32-or-cheaper far-pair screen of the N = 2^20
+ * configuration): mode 1 skips the square root and division of a marked
+ * candidate whose root provably exceeds the best found so far for its
+ * disk (with q = -b: d v2 - 2^-19 q^2 > 2 q v2 dt (1 + 2^-40), a bound that
+ * covers the rounding of the reference's formula), so the predictions are
+ * unchanged; mode 2 also counts, per system, the marked candidates seen,
+ * the roots solved and the skipped roots the minimum would have taken
+ * (misses, 0 by construction; the skipped roots are solved for that), which
+ * edmd_batch_screen_stats sums over the batch; 0 (default) off.  Not under
+ * damping.  Setting a mode clears the counters. */
+int edmd_batch_set_screen(edmd_batch* b, int mode);
+int edmd_batch_screen_stats(edmd_batch* b, uint64_t* marks, uint64_t* solved, uint64_t* misses);
 
 /* Systems per workgroup of the event loop with arrays in HBM, no
  * counterpart in the reference: 1 (default), 2 or 4, one wavefront per
