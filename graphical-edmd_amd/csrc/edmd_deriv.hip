@@ -21,8 +21,11 @@
  * normal / tangential components, as
  *   dt = -dr_n / vn, dr_n' = -resW dr_n, dr_s' = dr_s + (vt - resT vt) dt,
  *   dv_n' = -resW dv_n, dv_s' = resT dv_s
- * (the reflected disk taken back to the unperturbed hit time).  k_adjoint
- * applies the transpose of the same steps in reverse order.
+ * (the reflected disk taken back to the unperturbed hit time); under the
+ * field g along y the velocity along y also gains the field's pull over dt,
+ * (resT - 1) g dt for a wall normal to x and -(1 + resW) g dt for one
+ * normal to y.  k_adjoint applies the transpose of the same steps in
+ * reverse order.
  *
  *   k_tangent      forward propagation of one tangent vector per system
  *   k_adjoint      reverse propagation of one cotangent vector per system
@@ -100,15 +103,16 @@ __device__ __forceinline__ void view_close(const EdmdConst& C, const EdmdArrays&
 }
 
 struct TapeLane {
-    double t, ax, ay, cx, cy, ff;
+    double t, ax, ay, cx, cy, ff, g;
     int i, j;
 };
 
 __device__ __forceinline__ TapeLane tape_load(const EdmdTapeRec* tape, int64_t k, bool ok) {
-    TapeLane r{0, 0, 0, 0, 0, 0, 0, 0};
+    TapeLane r{0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (ok) {
         const EdmdTapeRec q = tape[k];
         r.t = q.t; r.ax = q.ax; r.ay = q.ay; r.cx = q.cx; r.cy = q.cy; r.ff = q.ff;
+        r.g = q.g;
         r.i = q.i; r.j = q.j;
     }
     return r;
@@ -122,6 +126,7 @@ __device__ __forceinline__ TapeLane tape_bcast(const TapeLane& r, int l) {
     o.cx = lane_bcast(r.cx, l);
     o.cy = lane_bcast(r.cy, l);
     o.ff = lane_bcast(r.ff, l);
+    o.g = lane_bcast(r.g, l);
     o.i = lane_bcast(r.i, l);
     o.j = lane_bcast(r.j, l);
     return o;
@@ -156,12 +161,14 @@ __global__ __launch_bounds__(64) void k_tangent(const EdmdSys* __restrict__ syst
                     ti1 += (vt - rt * vt) * dt;
                     ti2 = -rw * ti2;
                     ti3 = rt * ti3;
+                    if (r.g != 0) ti3 += (rt - 1) * r.g * dt;
                 } else {
                     const double dt = -ti1 / vn;
                     ti1 = -rw * ti1;
                     ti0 += (vt - rt * vt) * dt;
                     ti3 = -rw * ti3;
                     ti2 = rt * ti2;
+                    if (r.g != 0) ti3 -= (1 + rw) * r.g * dt;
                 }
                 if (lane == 0) {
                     ti[0] = ti0; ti[1] = ti1; ti[2] = ti2; ti[3] = ti3;
@@ -238,12 +245,14 @@ __global__ __launch_bounds__(64) void k_adjoint(const EdmdSys* __restrict__ syst
             if (j < 0) {
                 const double vn = r.ax, vt = r.ay, rw = r.cx, rt = r.cy;
                 if (j == -1) {
-                    const double ldt = (vt - rt * vt) * li1;
+                    double ldt = (vt - rt * vt) * li1;
+                    if (r.g != 0) ldt += (rt - 1) * r.g * li3;
                     li0 = -rw * li0 - ldt / vn;
                     li2 = -rw * li2;
                     li3 = rt * li3;
                 } else {
-                    const double ldt = (vt - rt * vt) * li0;
+                    double ldt = (vt - rt * vt) * li0;
+                    if (r.g != 0) ldt -= (1 + rw) * r.g * li3;
                     li1 = -rw * li1 - ldt / vn;
                     li3 = -rw * li3;
                     li2 = rt * li2;

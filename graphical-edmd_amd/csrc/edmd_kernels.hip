@@ -2596,7 +2596,7 @@ __device__ __forceinline__ void ev_collision(const EdmdConst& C, const AR& A, in
             if (k < C.tape_cap) {
                 EdmdTapeRec rec;
                 rec.t = t; rec.ax = dx; rec.ay = dy; rec.cx = dvx; rec.cy = dvy; rec.ff = ff;
-                rec.i = i; rec.j = j; rec.pad[0] = 0; rec.pad[1] = 0;
+                rec.i = i; rec.j = j; rec.g = 0;
                 A.tape[k] = rec;
             }
             st->ntape = k + 1;
@@ -2986,8 +2986,7 @@ __device__ __forceinline__ void ev_single(const EdmdConst& C, const AR& A, int l
                     rec.ff = d == 5 ? x : y;
                     rec.i = p;
                     rec.j = d == 5 ? -1 : -2;
-                    rec.pad[0] = 0;
-                    rec.pad[1] = 0;
+                    rec.g = C.field_on ? C.field : 0.0;
                     A.tape[k] = rec;
                 }
                 st->ntape = k + 1;

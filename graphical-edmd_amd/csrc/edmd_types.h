@@ -188,7 +188,7 @@ typedef struct EdmdEpochRow {
 typedef struct __attribute__((aligned(64))) EdmdTapeRec {
     double t, ax, ay, cx, cy, ff;
     int32_t i, j;
-    int32_t pad[2];
+    double g; /* wall records: the field along y (0 without it); pairs: 0 */
 } EdmdTapeRec;
 
 /* Per-system accumulators of the tangent / adjoint passes. */

@@ -80,7 +80,7 @@ class TuneReport(C.Structure):
 
 TAPE_DT = np.dtype([("t", "<f8"), ("ax", "<f8"), ("ay", "<f8"), ("cx", "<f8"),
                     ("cy", "<f8"), ("ff", "<f8"), ("i", "<i4"), ("j", "<i4"),
-                    ("pad", "<i4", (2,))])
+                    ("g", "<f8")])
 
 _lib = None
 

@@ -100,8 +100,8 @@ typedef struct edmd_system_desc {
      * taken (EDMD.c:2664-2666), crossings and y-wall hits solving the
      * quadratic (EDMD.c:2428-2442, 2846-2860).  field < 0 (default -0.02).
      * During a growth start the Grow dispatch ignores it (straight flights
-     * and crossings, EDMD.c:2343-2400, 4992-5007) until GROWSTOP.  Not with
-     * taping. */
+     * and crossings, EDMD.c:2343-2400, 4992-5007) until GROWSTOP.  Taped
+     * runs differentiate through it (edmd_tape_rec's g). */
     int add_field;
     double field;
     /* Damping (damping / gamm of the graphics build, EDMD.c:294, 382):
@@ -484,7 +484,8 @@ int edmd_pcf_noalloc(int n, const double* x, const double* y, double Lx, double 
  *   and the state of every disk free-flown to the final time,
  * holding the event sequence fixed (the map is smooth while it is).
  * Systems must not use the velocity-rescale thermostat, the circular wall,
- * the field or damping; flat walls (wall_x, wall_y, mid_wall) are taped.
+ * damping, thermal or shear walls; flat walls (wall_x, wall_y, mid_wall)
+ * and the field are taped.
  *
  * One valid collision: time, contact vector a = r_j - r_i (minimum image),
  * relative velocity c = v_j - v_i before the collision, impulse factor ff
@@ -495,7 +496,7 @@ int edmd_pcf_noalloc(int n, const double* x, const double* y, double Lx, double 
 typedef struct edmd_tape_rec {
     double t, ax, ay, cx, cy, ff;
     int32_t i, j;
-    int32_t pad[2];
+    double g; /* wall records: the field along y at the hit (0 without it) */
 } edmd_tape_rec;
 
 /* Starts taping at the current state.  checkpoint_every = K > 0: the run

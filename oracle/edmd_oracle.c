@@ -1028,6 +1028,7 @@ static void do_collision(orc_ctx* s, const ev_node* ev) {
         r->cx = dvx;
         r->cy = dvy;
         r->ff = ff;
+        r->g = 0;
     }
     if (s->tape) {
         s->ntape++;
@@ -1138,6 +1139,7 @@ static void do_wall(orc_ctx* s, const ev_node* ev) {
             r->cx = rw;
             r->cy = rt;
             r->ff = xy == 0 ? s->x[i] : s->y[i];
+            r->g = s->prm.add_field ? s->prm.field : 0.0;
         }
         s->ntape++;
     }
@@ -1781,6 +1783,10 @@ double orc_tangent(int n, const double* m, const double* tl0, const orc_tape_rec
             ti[b] += (vt - rt * vt) * dt;
             ti[2 + a] = -rw * ti[2 + a];
             ti[2 + b] = rt * ti[2 + b];
+            if (r->g != 0) {
+                if (a == 0) ti[3] += (rt - 1) * r->g * dt;
+                else ti[3] -= (1 + rw) * r->g * dt;
+            }
             continue;
         }
         double* tj = tan + 4 * j;
@@ -1839,7 +1845,11 @@ void orc_adjoint(int n, const double* m, const double* tl0, const orc_tape_rec* 
         if (j < 0) {
             const int a = -1 - j, b = 1 - a;
             const double vn = r->ax, vt = r->ay, rw = r->cx, rt = r->cy;
-            const double ldt = (vt - rt * vt) * li[b];
+            double ldt = (vt - rt * vt) * li[b];
+            if (r->g != 0) {
+                if (a == 0) ldt += (rt - 1) * r->g * li[3];
+                else ldt -= (1 + rw) * r->g * li[3];
+            }
             li[a] = -rw * li[a] - ldt / vn;
             li[2 + a] = -rw * li[2 + a];
             li[2 + b] = rt * li[2 + b];
@@ -1882,24 +1892,43 @@ void orc_adjoint(int n, const double* m, const double* tl0, const orc_tape_rec* 
     free(tl);
 }
 
+/* Free flight of disk i to time T under the field f along y. */
+static void replay_fly(int i, double T, double f, double* x, double* y, double* vx, double* vy,
+                       double* tl) {
+    const double h = T - tl[i];
+    x[i] += h * vx[i];
+    y[i] += h * vy[i] + 0.5 * f * h * h;
+    vy[i] += f * h;
+    tl[i] = T;
+}
+
+/* The time s from now at which a disk moving along the wall normal at v
+ * (with acceleration acc) covers gap: the root near 0 of acc/2 s^2 + v s =
+ * gap, in the form without cancellation. */
+static double replay_wall_shift(double gap, double v, double acc) {
+    if (acc == 0) return gap / v;
+    const double disc = v * v + 2 * acc * gap;
+    return 2 * gap / (v + (v < 0 ? -sqrt(disc) : sqrt(disc)));
+}
+
 double orc_replay(int n, double Lx, double Ly, const double* rad, const double* m,
                   double res, const orc_tape_rec* tape, int64_t ntape, double t_end,
                   double* x, double* y, double* vx, double* vy, double* tl,
-                  double* t_last) {
+                  double* t_last, double field) {
     double S = 0, tk = 0;
+    const double f = field;
     for (int64_t k = 0; k < ntape; k++) {
         const orc_tape_rec* r = &tape[k];
         const int i = r->i, j = r->j;
         const double tr = r->t;
-        double xi = x[i] + (tr - tl[i]) * vx[i], yi = y[i] + (tr - tl[i]) * vy[i];
+        replay_fly(i, tr, f, x, y, vx, vy, tl);
         if (j < 0) {
             /* wall hit: the disk flies on to the taped contact coordinate
              * and reflects */
             const int a = -1 - j;
-            const double sh = (r->ff - (a == 0 ? xi : yi)) / (a == 0 ? vx[i] : vy[i]);
-            x[i] = xi + sh * vx[i];
-            y[i] = yi + sh * vy[i];
-            tl[i] = tr + sh;
+            const double sh = a == 0 ? replay_wall_shift(r->ff - x[i], vx[i], 0.0)
+                                     : replay_wall_shift(r->ff - y[i], vy[i], f);
+            replay_fly(i, tr + sh, f, x, y, vx, vy, tl);
             if (a == 0) {
                 vx[i] = -r->cx * vx[i];
                 vy[i] = r->cy * vy[i];
@@ -1909,8 +1938,8 @@ double orc_replay(int n, double Lx, double Ly, const double* rad, const double* 
             }
             continue;
         }
-        double xj = x[j] + (tr - tl[j]) * vx[j], yj = y[j] + (tr - tl[j]) * vy[j];
-        double ax = xj - xi, ay = yj - yi;
+        replay_fly(j, tr, f, x, y, vx, vy, tl);
+        double ax = x[j] - x[i], ay = y[j] - y[i];
         ax -= Lx * nearbyint(ax / Lx);
         ay -= Ly * nearbyint(ay / Ly);
         const double cx = vx[j] - vx[i], cy = vy[j] - vy[i];
@@ -1921,42 +1950,51 @@ double orc_replay(int n, double Lx, double Ly, const double* rad, const double* 
         if (!(det >= 0) || !(b < 0)) return NAN;
         const double sh = d / (-b + sqrt(det)); /* smaller root, cancellation free */
         tk = tr + sh;
-        x[i] = xi + sh * vx[i]; y[i] = yi + sh * vy[i];
-        x[j] = xj + sh * vx[j]; y[j] = yj + sh * vy[j];
+        replay_fly(i, tk, f, x, y, vx, vy, tl);
+        replay_fly(j, tk, f, x, y, vx, vy, tl);
+        /* the relative motion is free: the contact vector moves with c */
         ax += sh * cx;
         ay += sh * cy;
-        tl[i] = tl[j] = tk;
         const double ff = (1 + res) / (m[i] + m[j]) * (ax * cx + ay * cy) / sig2;
         S += m[i] * m[j] * ff * (ax * ax + ay * ay);
         vx[i] += ff * m[j] * ax; vy[i] += ff * m[j] * ay;
         vx[j] -= ff * m[i] * ax; vy[j] -= ff * m[i] * ay;
     }
-    for (int i = 0; i < n; i++) {
-        x[i] += (t_end - tl[i]) * vx[i];
-        y[i] += (t_end - tl[i]) * vy[i];
-        tl[i] = t_end;
-    }
+    for (int i = 0; i < n; i++) replay_fly(i, t_end, f, x, y, vx, vy, tl);
     if (t_last) *t_last = tk;
     return S;
 }
 
 typedef long double ld;
 
+static void replay_fly_ld(int i, ld T, ld f, ld* x, ld* y, ld* vx, ld* vy, ld* tl) {
+    const ld h = T - tl[i];
+    x[i] += h * vx[i];
+    y[i] += h * vy[i] + 0.5L * f * h * h;
+    vy[i] += f * h;
+    tl[i] = T;
+}
+
+static ld replay_wall_shift_ld(ld gap, ld v, ld acc) {
+    if (acc == 0) return gap / v;
+    const ld disc = v * v + 2 * acc * gap;
+    return 2 * gap / (v + (v < 0 ? -sqrtl(disc) : sqrtl(disc)));
+}
+
 static ld replay_objective_ld(int n, ld Lx, ld Ly, const double* rad, const double* m, ld res,
                               const orc_tape_rec* tape, int64_t ntape, ld t_end, ld* x, ld* y,
-                              ld* vx, ld* vy, ld* tl, ld lw, ld lt, const double* lam) {
+                              ld* vx, ld* vy, ld* tl, ld lw, ld lt, const double* lam, ld f) {
     ld S = 0, tk = 0;
     for (int64_t k = 0; k < ntape; k++) {
         const orc_tape_rec* r = &tape[k];
         const int i = r->i, j = r->j;
         const ld tr = r->t;
-        ld xi = x[i] + (tr - tl[i]) * vx[i], yi = y[i] + (tr - tl[i]) * vy[i];
+        replay_fly_ld(i, tr, f, x, y, vx, vy, tl);
         if (j < 0) {
             const int a = -1 - j;
-            const ld sh = ((ld)r->ff - (a == 0 ? xi : yi)) / (a == 0 ? vx[i] : vy[i]);
-            x[i] = xi + sh * vx[i];
-            y[i] = yi + sh * vy[i];
-            tl[i] = tr + sh;
+            const ld sh = a == 0 ? replay_wall_shift_ld((ld)r->ff - x[i], vx[i], 0)
+                                 : replay_wall_shift_ld((ld)r->ff - y[i], vy[i], f);
+            replay_fly_ld(i, tr + sh, f, x, y, vx, vy, tl);
             if (a == 0) {
                 vx[i] = -(ld)r->cx * vx[i];
                 vy[i] = (ld)r->cy * vy[i];
@@ -1966,8 +2004,8 @@ static ld replay_objective_ld(int n, ld Lx, ld Ly, const double* rad, const doub
             }
             continue;
         }
-        ld xj = x[j] + (tr - tl[j]) * vx[j], yj = y[j] + (tr - tl[j]) * vy[j];
-        ld ax = xj - xi, ay = yj - yi;
+        replay_fly_ld(j, tr, f, x, y, vx, vy, tl);
+        ld ax = x[j] - x[i], ay = y[j] - y[i];
         ax -= Lx * nearbyintl(ax / Lx);
         ay -= Ly * nearbyintl(ay / Ly);
         const ld cx = vx[j] - vx[i], cy = vy[j] - vy[i];
@@ -1978,11 +2016,10 @@ static ld replay_objective_ld(int n, ld Lx, ld Ly, const double* rad, const doub
         if (!(det >= 0) || !(b < 0)) return NAN;
         const ld sh = d / (-b + sqrtl(det));
         tk = tr + sh;
-        x[i] = xi + sh * vx[i]; y[i] = yi + sh * vy[i];
-        x[j] = xj + sh * vx[j]; y[j] = yj + sh * vy[j];
+        replay_fly_ld(i, tk, f, x, y, vx, vy, tl);
+        replay_fly_ld(j, tk, f, x, y, vx, vy, tl);
         ax += sh * cx;
         ay += sh * cy;
-        tl[i] = tl[j] = tk;
         const ld mi = m[i], mj = m[j];
         const ld ff = (1 + res) / (mi + mj) * (ax * cx + ay * cy) / sig2;
         S += mi * mj * ff * (ax * ax + ay * ay);
@@ -1991,8 +2028,7 @@ static ld replay_objective_ld(int n, ld Lx, ld Ly, const double* rad, const doub
     }
     ld J = lw * S + lt * tk;
     for (int i = 0; i < n; i++) {
-        x[i] += (t_end - tl[i]) * vx[i];
-        y[i] += (t_end - tl[i]) * vy[i];
+        replay_fly_ld(i, t_end, f, x, y, vx, vy, tl);
         if (lam)
             J += lam[4 * i] * x[i] + lam[4 * i + 1] * y[i] + lam[4 * i + 2] * vx[i] +
                  lam[4 * i + 3] * vy[i];
@@ -2004,7 +2040,7 @@ double orc_replay_fd(int n, double Lx, double Ly, const double* rad, const doubl
                      double res, const orc_tape_rec* tape, int64_t ntape, double t_end,
                      const double* x, const double* y, const double* vx, const double* vy,
                      const double* tl, const double* e, double h, double lw, double lt,
-                     const double* lam) {
+                     const double* lam, double field) {
     ld* buf = malloc((size_t)n * 5 * sizeof(ld));
     if (!buf) return NAN;
     ld J[2];
@@ -2019,7 +2055,7 @@ double orc_replay_fd(int n, double Lx, double Ly, const double* rad, const doubl
             TL[i] = tl[i];
         }
         J[side] = replay_objective_ld(n, Lx, Ly, rad, m, res, tape, ntape, t_end, X, Y, VX, VY,
-                                      TL, lw, lt, lam);
+                                      TL, lw, lt, lam, (ld)field);
     }
     free(buf);
     return (double)((J[0] - J[1]) / (2 * (ld)h));

@@ -204,11 +204,16 @@ void orc_sync_all(orc_ctx* s, double t);
  * the normal / tangential components of the tangent flown to the hit,
  *   dt = -dr_n / vn, dr_n' = -resW dr_n, dr_s' = dr_s + (vt - resT vt) dt,
  *   dv_n' = -resW dv_n, dv_s' = resT dv_s
- * (the reflected disk evaluated at the unperturbed hit time); t_last is
- * the time of the last pair collision. */
+ * (the reflected disk evaluated at the unperturbed hit time); under the
+ * field g along y (record field g) the velocity along y gains the field's
+ * pull over dt as well: dv_y' += (resT - 1) g dt for a wall normal to x,
+ * dv_y' -= (1 + resW) g dt for one normal to y (pair collisions and the
+ * tangent's free flight are unchanged by a uniform field).  t_last is the
+ * time of the last pair collision. */
 typedef struct {
     int32_t i, j;
     double t, ax, ay, cx, cy, ff;
+    double g; /* wall records: the field along y at the hit (0 without it) */
 } orc_tape_rec;
 
 int orc_tape_enable(orc_ctx* s, int64_t capacity);
@@ -251,13 +256,15 @@ double orc_tape_wsum(const orc_ctx* s);
  * place, 4 arrays + tl), every taped collision (i, j) is redone in tape
  * order: both disks fly to the taped time, the contact time is re-solved
  * from the perturbed state (stable root of |a + c s|^2 = 4 r_i r_j), both
- * fly to it and the collision rule is applied; finally every disk flies to
- * t_end.  Returns sum of w_k; *t_last = time of the last replayed collision.
+ * fly to it and the collision rule is applied; a wall record's disk flies
+ * on to the taped contact coordinate and reflects; finally every disk flies
+ * to t_end.  Flights follow the field along y (field; 0 without it).
+ * Returns sum of w_k; *t_last = time of the last replayed collision.
  * Returns NaN when a pair no longer meets. */
 double orc_replay(int n, double Lx, double Ly, const double* rad, const double* m,
                   double res, const orc_tape_rec* tape, int64_t ntape, double t_end,
                   double* x, double* y, double* vx, double* vy, double* tl,
-                  double* t_last);
+                  double* t_last, double field);
 
 /* Central difference (J(s0 + h e) - J(s0 - h e)) / 2h of
  *   J = lw * sum_k w_k + lt * t_last + <lam, state at t_end>   (lam may be NULL)
@@ -268,7 +275,7 @@ double orc_replay_fd(int n, double Lx, double Ly, const double* rad, const doubl
                      double res, const orc_tape_rec* tape, int64_t ntape, double t_end,
                      const double* x, const double* y, const double* vx, const double* vy,
                      const double* tl, const double* e, double h, double lw, double lt,
-                     const double* lam);
+                     const double* lam, double field);
 
 /* g(r) histogram of calculate_pcf (pcf.c:16-75).  Returns the bin count;
  * r and g must hold at least (int)(max_r/dr) values. */

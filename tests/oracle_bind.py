@@ -93,7 +93,7 @@ def lib():
         L.orc_replay_fd.restype = C.c_double
         L.orc_replay_fd.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp, C.c_double,
                                     C.c_void_p, C.c_int64, C.c_double] + [dp] * 6 + [
-                                    C.c_double, C.c_double, C.c_double, C.c_void_p]
+                                    C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_double]
         L.orc_exp.restype = C.c_double
         L.orc_exp.argtypes = [C.c_double]
         L.orc_log.restype = C.c_double
@@ -109,13 +109,13 @@ def lib():
         L.orc_tape_wsum.argtypes = [C.c_void_p]
         L.orc_replay.restype = C.c_double
         L.orc_replay.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp, C.c_double,
-                                 C.c_void_p, C.c_int64, C.c_double] + [dp] * 5 + [dp]
+                                 C.c_void_p, C.c_int64, C.c_double] + [dp] * 5 + [dp, C.c_double]
         _lib = L
     return _lib
 
 
 TAPE_DT = np.dtype([("i", "<i4"), ("j", "<i4"), ("t", "<f8"), ("ax", "<f8"),
-                    ("ay", "<f8"), ("cx", "<f8"), ("cy", "<f8"), ("ff", "<f8")])
+                    ("ay", "<f8"), ("cx", "<f8"), ("cy", "<f8"), ("ff", "<f8"), ("g", "<f8")])
 
 
 def mt_draws(seed, n):
@@ -154,9 +154,10 @@ def adjoint(m, tl0, tape, t_end, lw, lt=0.0, lam_end=None):
     return g
 
 
-def replay_fd(st, tape, t_end, e, h=1e-7, lw=1.0, lt=0.0, lam=None, res=1.0):
+def replay_fd(st, tape, t_end, e, h=1e-7, lw=1.0, lt=0.0, lam=None, res=1.0, field=0.0):
     """Extended-precision central difference of lw * sum w + lt * t_last +
-    <lam, final state> along e (n x 4) over the fixed-sequence replay."""
+    <lam, final state> along e (n x 4) over the fixed-sequence replay
+    (flights under the field along y when field != 0)."""
     k = {key: np.ascontiguousarray(st[key], dtype=np.float64)
          for key in ("x", "y", "vx", "vy", "t", "rad", "m")}
     e = np.ascontiguousarray(e, dtype=np.float64).reshape(-1, 4)
@@ -165,10 +166,11 @@ def replay_fd(st, tape, t_end, e, h=1e-7, lw=1.0, lt=0.0, lam=None, res=1.0):
     return lib().orc_replay_fd(len(k["m"]), st["Lx"], st["Ly"], _dp(k["rad"]), _dp(k["m"]),
                                res, tape.ctypes.data, len(tape), t_end, _dp(k["x"]),
                                _dp(k["y"]), _dp(k["vx"]), _dp(k["vy"]), _dp(k["t"]), _dp(e),
-                               h, lw, lt, la.ctypes.data if la is not None else None)
+                               h, lw, lt, la.ctypes.data if la is not None else None,
+                               float(field))
 
 
-def replay(st, tape, t_end, res=1.0):
+def replay(st, tape, t_end, res=1.0, field=0.0):
     """Fixed-sequence replay of tape from st (x y vx vy and t per disk).
 
     Returns (sum w, time of the last collision, final state dict)."""
@@ -180,7 +182,7 @@ def replay(st, tape, t_end, res=1.0):
     tk = C.c_double()
     S = lib().orc_replay(len(m), st["Lx"], st["Ly"], _dp(rad), _dp(m), res,
                          tape.ctypes.data, len(tape), t_end, _dp(out["x"]), _dp(out["y"]),
-                         _dp(out["vx"]), _dp(out["vy"]), _dp(tl), C.byref(tk))
+                         _dp(out["vx"]), _dp(out["vy"]), _dp(tl), C.byref(tk), float(field))
     return S, tk.value, out
 
 

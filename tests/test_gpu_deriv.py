@@ -14,7 +14,7 @@ from conftest import bits, golden_state, load_golden
 
 pytestmark = pytest.mark.gpu
 
-TAPE_FIELDS = ("t", "ax", "ay", "cx", "cy", "ff", "i", "j")
+TAPE_FIELDS = ("t", "ax", "ay", "cx", "cy", "ff", "i", "j", "g")
 NCOLL = 10000
 
 
@@ -246,6 +246,10 @@ WALL_CASES = [
      dict(wall_x=1, wall_y=1, res_wall=0.9, res_tan=0.7), 4000),
     ("mid", lambda: _osmosis(160, 30.0, 7), dict(wall_x=1, wall_y=1, mid_wall=1, res_tan=1.0),
      3000),
+    ("gravity box", lambda: F.walled_state(400, 0.55, 26, wall_x=True, wall_y=True),
+     dict(wall_x=1, wall_y=1, add_field=1, field=-0.1, res_wall=0.9, res_tan=0.5), 4000),
+    ("gravity channel", lambda: F.walled_state(400, 0.55, 27, wall_y=True),
+     dict(wall_y=1, add_field=1, field=-0.2, res_tan=1.0), 4000),
 ]
 
 
@@ -283,11 +287,15 @@ def test_walled_tape_tangent_adjoint_match_oracle(gpu, oracle, name, make, param
         np.testing.assert_array_equal(bits(g[0]), bits(rg), err_msg="%s %d" % (name, every))
 
 
-def test_walled_gradient_matches_finite_differences(gpu, oracle):
-    """W / t_last over 600 events in a closed box with inelastic walls: the
-    GPU gradient against central differences of the oracle's replay."""
+@pytest.mark.parametrize("field", [0.0, -0.1])
+def test_walled_gradient_matches_finite_differences(gpu, oracle, field):
+    """W / t_last over 600 events in a closed box with inelastic walls,
+    with and without gravity: the GPU gradient against central differences
+    of the oracle's replay."""
     st = F.walled_state(400, 0.55, 24, wall_x=True, wall_y=True)
     params = dict(wall_x=1, wall_y=1, res_wall=0.9, res_tan=0.7)
+    if field:
+        params.update(add_field=1, field=field)
     b = gpu.Batch([st], **params)
     b.deriv_begin(checkpoint_every=256)
     b.deriv_run(max_collisions=600)
@@ -299,14 +307,14 @@ def test_walled_gradient_matches_finite_differences(gpu, oracle):
     rng = np.random.default_rng(6)
     for _ in range(3):
         e = rng.standard_normal((400, 4))
-        fd = oracle.replay_fd(s0, tape, o.t, e, 1e-7, 1 / T, -W / T ** 2)
+        fd = oracle.replay_fd(s0, tape, o.t, e, 1e-7, 1 / T, -W / T ** 2, field=field)
         ad = (grad * e).sum()
         assert abs(fd - ad) <= 1e-6 * abs(ad)
 
 
 def test_unsupported_models_are_rejected_for_taping(gpu):
     st = F.walled_state(400, 0.55, 25, wall_x=True)
-    for kw in (dict(add_field=1), dict(damping=1, gamma=0.01)):
+    for kw in (dict(damping=1, gamma=0.01), dict(add_field=1, damping=1, gamma=0.01)):
         b = gpu.Batch([st], wall_x=1, **kw)
         with pytest.raises(gpu.EdmdError) as e:
             b.deriv_begin()

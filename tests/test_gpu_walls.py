@@ -207,7 +207,11 @@ def test_field_arguments_are_checked(gpu):
     st = F.lattice_state(100, 0.4, seed=5, lattice="fit")
     with pytest.raises(gpu.EdmdError):
         gpu.System(st, add_field=1, field=0.1)
+    # the field is taped (tests/test_gpu_deriv.py), damping with it is not
     b = gpu.Batch([st], add_field=1)
+    b.deriv_begin(checkpoint_every=256)
+    b.deriv_end()
+    b = gpu.Batch([st], add_field=1, damping=1, gamma=0.01)
     with pytest.raises(gpu.EdmdError):
         b.deriv_begin(checkpoint_every=256)
 

@@ -327,3 +327,73 @@ def test_walled_duality_long_horizon():
     lhs = (g * e).sum()
     rhs = 0.3 * dW - 2.0 * dtl + (lam * tan).sum()
     assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
+
+
+# ---- the field on the tape
+
+def test_bounce_under_the_field_closed_form():
+    """A disk dropped from rest (vy = 0, vx = 0.3) onto the bottom wall
+    under the field f < 0 (resW 0.9, resT 0.5), flown to T after the hit:
+    th = sqrt(2 (y0 - r)/|f|), the bounce speed resW |f| th, and the
+    partial derivatives of x(T), y(T), vx(T), vy(T) that follow; the
+    field's pull over the shifted hit time enters the velocity's."""
+    x0, y0, vx, r, f, rw, rt, T = 5.0, 6.0, 0.3, 0.5, -0.5, 0.9, 0.5, 6.0
+    af = -f
+    st = {"x": np.array([x0, 15.0]), "y": np.array([y0, 15.0]),
+          "vx": np.array([vx, 0.0]), "vy": np.array([0.0, 0.0]),
+          "rad": np.array([r, r]), "m": np.array([1.0, 1.0]), "Lx": 20.0, "Ly": 20.0}
+    o, s0, tape = taped(st, 1, wall_y=1, add_field=1, field=f, res_wall=rw, res_tan=rt)
+    th = np.sqrt(2 * (y0 - r) / af)
+    assert len(tape) == 1 and tape["j"][0] == -2 and tape["g"][0] == f
+    assert tape["t"][0] == pytest.approx(th, rel=1e-13)
+    want = {
+        0: [1, (1 - rt) * vx / (af * th), th + rt * (T - th), (1 - rt) * vx / af],
+        1: [0, ((1 + rw) * (T - th) - rw * th) / th, 0, (T - th) - rw * th],
+        2: [0, 0, rt, 0],
+        3: [0, (1 + rw) / th, 0, 1],
+    }
+    m, tl0 = st["m"], s0["t"]
+    for c, row in want.items():
+        lam = np.zeros((2, 4))
+        lam[0, c] = 1.0
+        g = ob.adjoint(m, tl0, tape, T, 0.0, 0.0, lam)
+        np.testing.assert_allclose(g[0], row, rtol=1e-12, atol=1e-13, err_msg=str(c))
+        tan = np.zeros((2, 4))
+        tan[0] = row
+        ob.tangent(m, tl0, tape, T, tan)
+        assert tan[0, c] == pytest.approx(np.dot(row, row), rel=1e-12)
+
+
+FIELD_CASES = [
+    ("gravity box", dict(wall_x=1, wall_y=1, add_field=1, field=-0.1, res_wall=0.9, res_tan=0.5),
+     dict(wall_x=True, wall_y=True)),
+    ("gravity channel", dict(wall_y=1, add_field=1, field=-0.2, res_tan=1.0), dict(wall_y=True)),
+    ("periodic fall", dict(add_field=1, field=-0.05), dict(wall_x=False)),
+]
+
+
+@pytest.mark.parametrize("name,params,walls", FIELD_CASES)
+def test_field_adjoint_matches_finite_differences(name, params, walls):
+    st = walled_state(400, 0.55, 31, **walls)
+    o, s0, tape = taped(st, 600, **params)
+    f = params["field"]
+    te = o.t
+    S, tk, fin = ob.replay(s0, tape, te, field=f)
+    assert tk == pytest.approx(_last_pair_time(tape), abs=1e-12)
+    assert S == pytest.approx(o.tape_wsum(), rel=1e-11)
+    o.sync_all(te)
+    ref = o.state()
+    for k in ("x", "y", "vx", "vy"):
+        d = fin[k] - ref[k]
+        if k == "x" and not walls.get("wall_x") or k == "y" and not walls.get("wall_y"):
+            d -= st["Lx"] * np.round(d / st["Lx"])
+        assert np.abs(d).max() < 1e-9, (name, k)
+    rng = np.random.default_rng(5)
+    lam = rng.standard_normal((400, 4))
+    lw, lt = 1 / tk, -S / tk ** 2
+    g = ob.adjoint(st["m"], s0["t"], tape, te, lw, lt, lam)
+    for _ in range(2):
+        e = rng.standard_normal((400, 4))
+        fd = ob.replay_fd(s0, tape, te, e, 1e-7, lw, lt, lam, field=f)
+        ad = (g * e).sum()
+        assert abs(fd - ad) <= 1e-6 * abs(ad), name
