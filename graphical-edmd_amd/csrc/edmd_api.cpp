@@ -1833,11 +1833,12 @@ extern "C" int edmd_batch_deriv_begin(edmd_batch* b, int64_t checkpoint_every, i
     if (b->any_grow)
         return fail(EDMD_ERR_ARG, "derivatives need systems whose growth has ended");
     for (int s = 0; s < b->nsys; s++)
-        if ((b->hs[s].c.walls & 4) || b->hs[s].c.damping || b->hs[s].c.thermal_wall ||
-            b->hs[s].c.shear_wall)
+        if ((b->hs[s].c.walls & 4) || (b->hs[s].c.damping && b->hs[s].c.field_on) ||
+            b->hs[s].c.thermal_wall || b->hs[s].c.shear_wall)
             return fail(EDMD_ERR_ARG,
                         "system %d: derivatives need a system without the circular wall, "
-                        "damping, thermal or shear walls (flat walls and the field are taped)", s);
+                        "thermal or shear walls, or the field together with damping (flat "
+                        "walls, the field and damping are taped)", s);
     int rc = edmd_batch_deriv_end(b);
     if (rc) return rc;
     if ((double)b->nsys * (double)cap * sizeof(EdmdTapeRec) > 1.0e12)

@@ -24,7 +24,11 @@
  * (the reflected disk taken back to the unperturbed hit time); under the
  * field g along y the velocity along y also gains the field's pull over dt,
  * (resT - 1) g dt for a wall normal to x and -(1 + resW) g dt for one
- * normal to y.  k_adjoint applies the transpose of the same steps in
+ * normal to y.  Under damping at rate gamma the tangent's flight over h is
+ * dr += dv (1 - e)/gamma, dv *= e with e = exp(-gamma h) (edmd_exp), a
+ * pair's dff loses gamma ff dt and its velocity jump, applied dt early,
+ * decays less (dv_i += gamma m_j ff dt a, dv_j -= gamma m_i ff dt a); wall
+ * maps are unchanged.  k_adjoint applies the transpose of the same steps in
  * reverse order.
  *
  *   k_tangent      forward propagation of one tangent vector per system
@@ -47,6 +51,7 @@
 #include <stdint.h>
 
 #include "edmd_internal.h"
+#include "edmd_math.h"
 #include "edmd_types.h"
 
 #define WAVE_SYNC() __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront")
@@ -102,6 +107,36 @@ __device__ __forceinline__ void view_close(const EdmdConst& C, const EdmdArrays&
     for (int q = lane; q < n; q += 64) A.dtl[q] = V.tl[q];
 }
 
+/* The tangent's free flight over h (damped at rate g when g != 0). */
+__device__ __forceinline__ void tan_fly(double& v0, double& v1, double& v2, double& v3, double h,
+                                        double g) {
+    if (g == 0) {
+        v0 += h * v2;
+        v1 += h * v3;
+        return;
+    }
+    const double e = edmd_exp(-g * h);
+    const double ph = (1 - e) / g;
+    v0 += ph * v2;
+    v1 += ph * v3;
+    v2 *= e;
+    v3 *= e;
+}
+
+/* Its transpose, the adjoint's flight back over h. */
+__device__ __forceinline__ void adj_fly(double& l0, double& l1, double& l2, double& l3, double h,
+                                        double g) {
+    if (g == 0) {
+        l2 += h * l0;
+        l3 += h * l1;
+        return;
+    }
+    const double e = edmd_exp(-g * h);
+    const double ph = (1 - e) / g;
+    l2 = e * l2 + ph * l0;
+    l3 = e * l3 + ph * l1;
+}
+
 struct TapeLane {
     double t, ax, ay, cx, cy, ff, g;
     int i, j;
@@ -142,6 +177,7 @@ __global__ __launch_bounds__(64) void k_tangent(const EdmdSys* __restrict__ syst
     if (nt > C.tape_cap) nt = C.tape_cap;
     const DerivView V = view_open(C, A, lane, LDS ? lds : nullptr);
     double dsum = A.dr->dsum, dtlast = A.dr->dtlast;
+    const double g = C.damping ? C.gamma : 0.0;
     for (int64_t base = 0; base < nt; base += 64) {
         const int cnt = nt - base < 64 ? (int)(nt - base) : 64;
         const TapeLane mine = tape_load(A.tape, base + lane, lane < cnt);
@@ -150,9 +186,7 @@ __global__ __launch_bounds__(64) void k_tangent(const EdmdSys* __restrict__ syst
             const int i = r.i, j = r.j;
             double* ti = V.v + 4 * i;
             double ti0 = ti[0], ti1 = ti[1], ti2 = ti[2], ti3 = ti[3];
-            double h = r.t - V.tl[i];
-            ti0 += h * ti2;
-            ti1 += h * ti3;
+            tan_fly(ti0, ti1, ti2, ti3, r.t - V.tl[i], g);
             if (j < 0) { /* uniform: the record is the wave's */
                 const double vn = r.ax, vt = r.ay, rw = r.cx, rt = r.cy;
                 if (j == -1) {
@@ -179,9 +213,7 @@ __global__ __launch_bounds__(64) void k_tangent(const EdmdSys* __restrict__ syst
             }
             double* tj = V.v + 4 * j;
             double tj0 = tj[0], tj1 = tj[1], tj2 = tj[2], tj3 = tj[3];
-            h = r.t - V.tl[j];
-            tj0 += h * tj2;
-            tj1 += h * tj3;
+            tan_fly(tj0, tj1, tj2, tj3, r.t - V.tl[j], g);
             const double mi = V.mass(i), mj = V.mass(j);
             const double ax = r.ax, ay = r.ay, cx = r.cx, cy = r.cy, ff = r.ff;
             const double b = ax * cx + ay * cy;
@@ -190,7 +222,8 @@ __global__ __launch_bounds__(64) void k_tangent(const EdmdSys* __restrict__ syst
             const double dt = -(ax * px + ay * py) / b;
             const double rcx = px + cx * dt, rcy = py + cy * dt;
             const double kap = ff / b;
-            const double dff = kap * ((rcx * cx + rcy * cy) + (ax * qx + ay * qy));
+            double dff = kap * ((rcx * cx + rcy * cy) + (ax * qx + ay * qy));
+            if (g != 0) dff -= g * ff * dt;
             const double ux = dff * ax + ff * rcx, uy = dff * ay + ff * rcy;
             ti2 += mj * ux;
             ti3 += mj * uy;
@@ -201,6 +234,12 @@ __global__ __launch_bounds__(64) void k_tangent(const EdmdSys* __restrict__ syst
             ti1 -= mj * sh * ay;
             tj0 += mi * sh * ax;
             tj1 += mi * sh * ay;
+            if (g != 0) {
+                ti2 += g * mj * sh * ax;
+                ti3 += g * mj * sh * ay;
+                tj2 -= g * mi * sh * ax;
+                tj3 -= g * mi * sh * ay;
+            }
             dsum += mi * mj * (ax * ax + ay * ay) * dff;
             dtlast = dt;
             if (lane == 0) {
@@ -229,6 +268,7 @@ __global__ __launch_bounds__(64) void k_adjoint(const EdmdSys* __restrict__ syst
     if (nt > C.tape_cap) nt = C.tape_cap;
     const DerivView V = view_open(C, A, lane, LDS ? lds : nullptr);
     const double lw = A.dr->lw, lt = A.dr->lt;
+    const double g = C.damping ? C.gamma : 0.0;
     int pending = A.dr->lt_pending;
     for (int64_t top = nt; top > 0; top -= 64) {
         const int cnt = top < 64 ? (int)top : 64;
@@ -239,9 +279,7 @@ __global__ __launch_bounds__(64) void k_adjoint(const EdmdSys* __restrict__ syst
             const int i = r.i, j = r.j;
             double* li = V.v + 4 * i;
             double li0 = li[0], li1 = li[1], li2 = li[2], li3 = li[3];
-            double h = V.tl[i] - r.t;
-            li2 += h * li0;
-            li3 += h * li1;
+            adj_fly(li0, li1, li2, li3, V.tl[i] - r.t, g);
             if (j < 0) {
                 const double vn = r.ax, vt = r.ay, rw = r.cx, rt = r.cy;
                 if (j == -1) {
@@ -266,19 +304,19 @@ __global__ __launch_bounds__(64) void k_adjoint(const EdmdSys* __restrict__ syst
             }
             double* lj = V.v + 4 * j;
             double lj0 = lj[0], lj1 = lj[1], lj2 = lj[2], lj3 = lj[3];
-            h = V.tl[j] - r.t;
-            lj2 += h * lj0;
-            lj3 += h * lj1;
+            adj_fly(lj0, lj1, lj2, lj3, V.tl[j] - r.t, g);
             const double mi = V.mass(i), mj = V.mass(j);
             const double ax = r.ax, ay = r.ay, cx = r.cx, cy = r.cy, ff = r.ff;
             const double b = ax * cx + ay * cy;
             const double kap = ff / b;
             const double lux = mj * li2 - mi * lj2, luy = mj * li3 - mi * lj3;
-            const double lsh = -mj * (ax * li0 + ay * li1) + mi * (ax * lj0 + ay * lj1);
+            double lsh = -mj * (ax * li0 + ay * li1) + mi * (ax * lj0 + ay * lj1);
+            if (g != 0) lsh += g * (ax * lux + ay * luy);
             const double ldff = (ax * lux + ay * luy) + lw * mi * mj * (ax * ax + ay * ay);
             const double lrcx = ff * lux + kap * ldff * cx, lrcy = ff * luy + kap * ldff * cy;
             const double lqx = kap * ldff * ax, lqy = kap * ldff * ay;
             double ldt = ff * lsh + (cx * lrcx + cy * lrcy);
+            if (g != 0) ldt -= g * ff * ldff;
             if (pending) {
                 ldt += lt;
                 pending = 0;
@@ -322,17 +360,19 @@ __global__ __launch_bounds__(256) void k_deriv_flush(const EdmdSys* __restrict__
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= C.n) return;
     double* v = A.dv + 4 * q;
+    const double g = C.damping ? C.gamma : 0.0;
+    double v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3];
     if (mode == 0) {
-        const double h = A.st->t - A.dtl[q];
-        v[0] += h * v[2];
-        v[1] += h * v[3];
+        tan_fly(v0, v1, v2, v3, A.st->t - A.dtl[q], g);
         A.dtl[q] = A.st->t;
     } else {
-        const double h = A.dtl[q] - A.otl[q];
-        v[2] += h * v[0];
-        v[3] += h * v[1];
+        adj_fly(v0, v1, v2, v3, A.dtl[q] - A.otl[q], g);
         A.dtl[q] = A.otl[q];
     }
+    v[0] = v0;
+    v[1] = v1;
+    v[2] = v2;
+    v[3] = v3;
 }
 
 /* Records the taping origin: local time of every particle, cleared tape and

@@ -116,8 +116,8 @@ typedef struct edmd_system_desc {
      * converts the straight-line times of x crossings, x walls and
      * collisions; that pairing of damped times with undamped flights can
      * end in overlaps and NaN states, as in the reference.  During a growth
-     * start damping waits for GROWSTOP, as the field does.  Not with
-     * taping. */
+     * start damping waits for GROWSTOP, as the field does.  Taped runs
+     * differentiate through it, not together with the field. */
     int damping;
     double gamma;
     /* Circular wall (addCircularWall of the graphics build, EDMD.c:270): a
@@ -128,7 +128,7 @@ typedef struct edmd_system_desc {
      * other walls' times; a hit reflects the velocity about the normal at
      * the angle atan2 of the disk's position (EDMD.c:3441-3454, growth
      * EDMD.c:3358-3376), this library's own atan2 / sin / cos (within 2 ulp
-     * of the C library's, the same steps on the CPU checker).  No minimum
+     * of the C library's, the same steps in the oracle).  No minimum
      * image; thermo rows take the area pi (Lx/2)^2 and the wall perimeter
      * pi Lx; at GROWSTOP the angular momentum about the centre is removed
      * instead of the momentum (EDMD.c:5723-5764).  Under the field the time
@@ -484,8 +484,8 @@ int edmd_pcf_noalloc(int n, const double* x, const double* y, double Lx, double 
  *   and the state of every disk free-flown to the final time,
  * holding the event sequence fixed (the map is smooth while it is).
  * Systems must not use the velocity-rescale thermostat, the circular wall,
- * damping, thermal or shear walls; flat walls (wall_x, wall_y, mid_wall)
- * and the field are taped.
+ * thermal or shear walls, or the field together with damping; flat walls
+ * (wall_x, wall_y, mid_wall), the field and damping are taped.
  *
  * One valid collision: time, contact vector a = r_j - r_i (minimum image),
  * relative velocity c = v_j - v_i before the collision, impulse factor ff

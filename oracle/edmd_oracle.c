@@ -1761,8 +1761,38 @@ void orc_tape_clear(orc_ctx* s) {
 
 double orc_tape_wsum(const orc_ctx* s) { return s->wsum; }
 
+/* The tangent's free flight over h: dr += h dv, or under damping (rate g)
+ * dr += dv (1 - e)/g, dv *= e with e = exp(-g h) (the own exp). */
+static void tan_fly(double* v, double h, double g) {
+    if (g == 0) {
+        v[0] += h * v[2];
+        v[1] += h * v[3];
+        return;
+    }
+    const double e = orc_exp(-g * h);
+    const double ph = (1 - e) / g;
+    v[0] += ph * v[2];
+    v[1] += ph * v[3];
+    v[2] *= e;
+    v[3] *= e;
+}
+
+/* Its transpose, the adjoint's flight back over h. */
+static void adj_fly(double* l, double h, double g) {
+    if (g == 0) {
+        l[2] += h * l[0];
+        l[3] += h * l[1];
+        return;
+    }
+    const double e = orc_exp(-g * h);
+    const double ph = (1 - e) / g;
+    l[2] = e * l[2] + ph * l[0];
+    l[3] = e * l[3] + ph * l[1];
+}
+
 double orc_tangent(int n, const double* m, const double* tl0, const orc_tape_rec* tape,
-                   int64_t ntape, double t_end, double* tan, double* dt_last) {
+                   int64_t ntape, double t_end, double* tan, double* dt_last, double gamma) {
+    const double g = gamma;
     double* tl = malloc((size_t)n * sizeof(double));
     memcpy(tl, tl0, (size_t)n * sizeof(double));
     double dsum = 0, dtl = 0;
@@ -1770,9 +1800,7 @@ double orc_tangent(int n, const double* m, const double* tl0, const orc_tape_rec
         const orc_tape_rec* r = &tape[k];
         const int i = r->i, j = r->j;
         double* ti = tan + 4 * i;
-        double h = r->t - tl[i];
-        ti[0] += h * ti[2];
-        ti[1] += h * ti[3];
+        tan_fly(ti, r->t - tl[i], g);
         tl[i] = r->t;
         if (j < 0) {
             /* wall hit: normal axis a, tangential axis b */
@@ -1790,9 +1818,7 @@ double orc_tangent(int n, const double* m, const double* tl0, const orc_tape_rec
             continue;
         }
         double* tj = tan + 4 * j;
-        h = r->t - tl[j];
-        tj[0] += h * tj[2];
-        tj[1] += h * tj[3];
+        tan_fly(tj, r->t - tl[j], g);
         tl[j] = r->t;
         const double ax = r->ax, ay = r->ay, cx = r->cx, cy = r->cy, ff = r->ff;
         const double mi = m[i], mj = m[j];
@@ -1802,7 +1828,9 @@ double orc_tangent(int n, const double* m, const double* tl0, const orc_tape_rec
         const double dt = -(ax * px + ay * py) / b;
         const double rcx = px + cx * dt, rcy = py + cy * dt;
         const double kap = ff / b;
-        const double dff = kap * ((rcx * cx + rcy * cy) + (ax * qx + ay * qy));
+        double dff = kap * ((rcx * cx + rcy * cy) + (ax * qx + ay * qy));
+        /* damping: the relative velocity decays over the shift dt */
+        if (g != 0) dff -= g * ff * dt;
         const double ux = dff * ax + ff * rcx, uy = dff * ay + ff * rcy;
         ti[2] += mj * ux;
         ti[3] += mj * uy;
@@ -1813,14 +1841,17 @@ double orc_tangent(int n, const double* m, const double* tl0, const orc_tape_rec
         ti[1] -= mj * sh * ay;
         tj[0] += mi * sh * ax;
         tj[1] += mi * sh * ay;
+        if (g != 0) {
+            /* damping: the jump, applied dt early, decays less */
+            ti[2] += g * mj * sh * ax;
+            ti[3] += g * mj * sh * ay;
+            tj[2] -= g * mi * sh * ax;
+            tj[3] -= g * mi * sh * ay;
+        }
         dsum += mi * mj * (ax * ax + ay * ay) * dff;
         dtl = dt;
     }
-    for (int i = 0; i < n; i++) {
-        double h = t_end - tl[i];
-        tan[4 * i] += h * tan[4 * i + 2];
-        tan[4 * i + 1] += h * tan[4 * i + 3];
-    }
+    for (int i = 0; i < n; i++) tan_fly(tan + 4 * i, t_end - tl[i], g);
     free(tl);
     if (dt_last) *dt_last = dtl;
     return dsum;
@@ -1828,7 +1859,8 @@ double orc_tangent(int n, const double* m, const double* tl0, const orc_tape_rec
 
 void orc_adjoint(int n, const double* m, const double* tl0, const orc_tape_rec* tape,
                  int64_t ntape, double t_end, double lw, double lt, const double* lam_end,
-                 double* grad) {
+                 double* grad, double gamma) {
+    const double g = gamma;
     double* tl = malloc((size_t)n * sizeof(double));
     for (int i = 0; i < n; i++) tl[i] = t_end;
     if (lam_end) memcpy(grad, lam_end, (size_t)n * 4 * sizeof(double));
@@ -1838,9 +1870,7 @@ void orc_adjoint(int n, const double* m, const double* tl0, const orc_tape_rec* 
         const orc_tape_rec* r = &tape[k];
         const int i = r->i, j = r->j;
         double* li = grad + 4 * i;
-        double h = tl[i] - r->t;
-        li[2] += h * li[0];
-        li[3] += h * li[1];
+        adj_fly(li, tl[i] - r->t, g);
         tl[i] = r->t;
         if (j < 0) {
             const int a = -1 - j, b = 1 - a;
@@ -1856,20 +1886,20 @@ void orc_adjoint(int n, const double* m, const double* tl0, const orc_tape_rec* 
             continue;
         }
         double* lj = grad + 4 * j;
-        h = tl[j] - r->t;
-        lj[2] += h * lj[0];
-        lj[3] += h * lj[1];
+        adj_fly(lj, tl[j] - r->t, g);
         tl[j] = r->t;
         const double ax = r->ax, ay = r->ay, cx = r->cx, cy = r->cy, ff = r->ff;
         const double mi = m[i], mj = m[j];
         const double b = ax * cx + ay * cy;
         const double kap = ff / b;
         const double lux = mj * li[2] - mi * lj[2], luy = mj * li[3] - mi * lj[3];
-        const double lsh = -mj * (ax * li[0] + ay * li[1]) + mi * (ax * lj[0] + ay * lj[1]);
+        double lsh = -mj * (ax * li[0] + ay * li[1]) + mi * (ax * lj[0] + ay * lj[1]);
+        if (g != 0) lsh += g * (ax * lux + ay * luy);
         const double ldff = (ax * lux + ay * luy) + lw * mi * mj * (ax * ax + ay * ay);
         const double lrcx = ff * lux + kap * ldff * cx, lrcy = ff * luy + kap * ldff * cy;
         const double lqx = kap * ldff * ax, lqy = kap * ldff * ay;
         double ldt = ff * lsh + (cx * lrcx + cy * lrcy);
+        if (g != 0) ldt -= g * ff * ldff;
         if (pending) {
             ldt += lt;
             pending = 0;
@@ -1884,22 +1914,34 @@ void orc_adjoint(int n, const double* m, const double* tl0, const orc_tape_rec* 
         lj[2] += lqx;
         lj[3] += lqy;
     }
-    for (int i = 0; i < n; i++) {
-        double h = tl[i] - tl0[i];
-        grad[4 * i + 2] += h * grad[4 * i];
-        grad[4 * i + 3] += h * grad[4 * i + 1];
-    }
+    for (int i = 0; i < n; i++) adj_fly(grad + 4 * i, tl[i] - tl0[i], g);
     free(tl);
 }
 
-/* Free flight of disk i to time T under the field f along y. */
-static void replay_fly(int i, double T, double f, double* x, double* y, double* vx, double* vy,
-                       double* tl) {
+/* Free flight of disk i to time T under the field f along y, or damped at
+ * rate g (velocities times exp(-g h), displacements v (1 - exp(-g h))/g). */
+static void replay_fly(int i, double T, double f, double g, double* x, double* y, double* vx,
+                       double* vy, double* tl) {
     const double h = T - tl[i];
-    x[i] += h * vx[i];
-    y[i] += h * vy[i] + 0.5 * f * h * h;
-    vy[i] += f * h;
+    if (g != 0) {
+        const double e = exp(-g * h);
+        x[i] += vx[i] * (1 - e) / g;
+        y[i] += vy[i] * (1 - e) / g;
+        vx[i] *= e;
+        vy[i] *= e;
+    } else {
+        x[i] += h * vx[i];
+        y[i] += h * vy[i] + 0.5 * f * h * h;
+        vy[i] += f * h;
+    }
     tl[i] = T;
+}
+
+/* Under damping a displacement that straight flight covers in tau takes
+ * -log(1 - g tau)/g (NaN when it is never covered). */
+static double replay_damped_time(double tau, double g) {
+    if (g == 0) return tau;
+    return -log(1 - g * tau) / g;
 }
 
 /* The time s from now at which a disk moving along the wall normal at v
@@ -1914,21 +1956,22 @@ static double replay_wall_shift(double gap, double v, double acc) {
 double orc_replay(int n, double Lx, double Ly, const double* rad, const double* m,
                   double res, const orc_tape_rec* tape, int64_t ntape, double t_end,
                   double* x, double* y, double* vx, double* vy, double* tl,
-                  double* t_last, double field) {
+                  double* t_last, double field, double gamma) {
     double S = 0, tk = 0;
-    const double f = field;
+    const double f = field, g = gamma;
     for (int64_t k = 0; k < ntape; k++) {
         const orc_tape_rec* r = &tape[k];
         const int i = r->i, j = r->j;
         const double tr = r->t;
-        replay_fly(i, tr, f, x, y, vx, vy, tl);
+        replay_fly(i, tr, f, g, x, y, vx, vy, tl);
         if (j < 0) {
             /* wall hit: the disk flies on to the taped contact coordinate
              * and reflects */
             const int a = -1 - j;
-            const double sh = a == 0 ? replay_wall_shift(r->ff - x[i], vx[i], 0.0)
-                                     : replay_wall_shift(r->ff - y[i], vy[i], f);
-            replay_fly(i, tr + sh, f, x, y, vx, vy, tl);
+            const double sh = replay_damped_time(
+                a == 0 ? replay_wall_shift(r->ff - x[i], vx[i], 0.0)
+                       : replay_wall_shift(r->ff - y[i], vy[i], f), g);
+            replay_fly(i, tr + sh, f, g, x, y, vx, vy, tl);
             if (a == 0) {
                 vx[i] = -r->cx * vx[i];
                 vy[i] = r->cy * vy[i];
@@ -1938,7 +1981,7 @@ double orc_replay(int n, double Lx, double Ly, const double* rad, const double* 
             }
             continue;
         }
-        replay_fly(j, tr, f, x, y, vx, vy, tl);
+        replay_fly(j, tr, f, g, x, y, vx, vy, tl);
         double ax = x[j] - x[i], ay = y[j] - y[i];
         ax -= Lx * nearbyint(ax / Lx);
         ay -= Ly * nearbyint(ay / Ly);
@@ -1948,31 +1991,47 @@ double orc_replay(int n, double Lx, double Ly, const double* rad, const double* 
         const double d = ax * ax + ay * ay - sig2;
         const double det = b * b - c2 * d;
         if (!(det >= 0) || !(b < 0)) return NAN;
-        const double sh = d / (-b + sqrt(det)); /* smaller root, cancellation free */
-        tk = tr + sh;
-        replay_fly(i, tk, f, x, y, vx, vy, tl);
-        replay_fly(j, tk, f, x, y, vx, vy, tl);
-        /* the relative motion is free: the contact vector moves with c */
-        ax += sh * cx;
-        ay += sh * cy;
-        const double ff = (1 + res) / (m[i] + m[j]) * (ax * cx + ay * cy) / sig2;
+        const double tau = d / (-b + sqrt(det)); /* smaller root, cancellation free */
+        tk = tr + replay_damped_time(tau, g);
+        replay_fly(i, tk, f, g, x, y, vx, vy, tl);
+        replay_fly(j, tk, f, g, x, y, vx, vy, tl);
+        /* the relative motion is straight in the displacement: the contact
+         * vector moves with c over tau; the relative velocity is the flown
+         * disks' */
+        ax += tau * cx;
+        ay += tau * cy;
+        const double ux = vx[j] - vx[i], uy = vy[j] - vy[i];
+        const double ff = (1 + res) / (m[i] + m[j]) * (ax * ux + ay * uy) / sig2;
         S += m[i] * m[j] * ff * (ax * ax + ay * ay);
         vx[i] += ff * m[j] * ax; vy[i] += ff * m[j] * ay;
         vx[j] -= ff * m[i] * ax; vy[j] -= ff * m[i] * ay;
     }
-    for (int i = 0; i < n; i++) replay_fly(i, t_end, f, x, y, vx, vy, tl);
+    for (int i = 0; i < n; i++) replay_fly(i, t_end, f, g, x, y, vx, vy, tl);
     if (t_last) *t_last = tk;
     return S;
 }
 
 typedef long double ld;
 
-static void replay_fly_ld(int i, ld T, ld f, ld* x, ld* y, ld* vx, ld* vy, ld* tl) {
+static void replay_fly_ld(int i, ld T, ld f, ld g, ld* x, ld* y, ld* vx, ld* vy, ld* tl) {
     const ld h = T - tl[i];
-    x[i] += h * vx[i];
-    y[i] += h * vy[i] + 0.5L * f * h * h;
-    vy[i] += f * h;
+    if (g != 0) {
+        const ld e = expl(-g * h);
+        x[i] += vx[i] * (1 - e) / g;
+        y[i] += vy[i] * (1 - e) / g;
+        vx[i] *= e;
+        vy[i] *= e;
+    } else {
+        x[i] += h * vx[i];
+        y[i] += h * vy[i] + 0.5L * f * h * h;
+        vy[i] += f * h;
+    }
     tl[i] = T;
+}
+
+static ld replay_damped_time_ld(ld tau, ld g) {
+    if (g == 0) return tau;
+    return -logl(1 - g * tau) / g;
 }
 
 static ld replay_wall_shift_ld(ld gap, ld v, ld acc) {
@@ -1983,18 +2042,20 @@ static ld replay_wall_shift_ld(ld gap, ld v, ld acc) {
 
 static ld replay_objective_ld(int n, ld Lx, ld Ly, const double* rad, const double* m, ld res,
                               const orc_tape_rec* tape, int64_t ntape, ld t_end, ld* x, ld* y,
-                              ld* vx, ld* vy, ld* tl, ld lw, ld lt, const double* lam, ld f) {
+                              ld* vx, ld* vy, ld* tl, ld lw, ld lt, const double* lam, ld f,
+                              ld g) {
     ld S = 0, tk = 0;
     for (int64_t k = 0; k < ntape; k++) {
         const orc_tape_rec* r = &tape[k];
         const int i = r->i, j = r->j;
         const ld tr = r->t;
-        replay_fly_ld(i, tr, f, x, y, vx, vy, tl);
+        replay_fly_ld(i, tr, f, g, x, y, vx, vy, tl);
         if (j < 0) {
             const int a = -1 - j;
-            const ld sh = a == 0 ? replay_wall_shift_ld((ld)r->ff - x[i], vx[i], 0)
-                                 : replay_wall_shift_ld((ld)r->ff - y[i], vy[i], f);
-            replay_fly_ld(i, tr + sh, f, x, y, vx, vy, tl);
+            const ld sh = replay_damped_time_ld(
+                a == 0 ? replay_wall_shift_ld((ld)r->ff - x[i], vx[i], 0)
+                       : replay_wall_shift_ld((ld)r->ff - y[i], vy[i], f), g);
+            replay_fly_ld(i, tr + sh, f, g, x, y, vx, vy, tl);
             if (a == 0) {
                 vx[i] = -(ld)r->cx * vx[i];
                 vy[i] = (ld)r->cy * vy[i];
@@ -2004,7 +2065,7 @@ static ld replay_objective_ld(int n, ld Lx, ld Ly, const double* rad, const doub
             }
             continue;
         }
-        replay_fly_ld(j, tr, f, x, y, vx, vy, tl);
+        replay_fly_ld(j, tr, f, g, x, y, vx, vy, tl);
         ld ax = x[j] - x[i], ay = y[j] - y[i];
         ax -= Lx * nearbyintl(ax / Lx);
         ay -= Ly * nearbyintl(ay / Ly);
@@ -2014,21 +2075,22 @@ static ld replay_objective_ld(int n, ld Lx, ld Ly, const double* rad, const doub
         const ld d = ax * ax + ay * ay - sig2;
         const ld det = b * b - c2 * d;
         if (!(det >= 0) || !(b < 0)) return NAN;
-        const ld sh = d / (-b + sqrtl(det));
-        tk = tr + sh;
-        replay_fly_ld(i, tk, f, x, y, vx, vy, tl);
-        replay_fly_ld(j, tk, f, x, y, vx, vy, tl);
-        ax += sh * cx;
-        ay += sh * cy;
+        const ld tau = d / (-b + sqrtl(det));
+        tk = tr + replay_damped_time_ld(tau, g);
+        replay_fly_ld(i, tk, f, g, x, y, vx, vy, tl);
+        replay_fly_ld(j, tk, f, g, x, y, vx, vy, tl);
+        ax += tau * cx;
+        ay += tau * cy;
+        const ld ux = vx[j] - vx[i], uy = vy[j] - vy[i];
         const ld mi = m[i], mj = m[j];
-        const ld ff = (1 + res) / (mi + mj) * (ax * cx + ay * cy) / sig2;
+        const ld ff = (1 + res) / (mi + mj) * (ax * ux + ay * uy) / sig2;
         S += mi * mj * ff * (ax * ax + ay * ay);
         vx[i] += ff * mj * ax; vy[i] += ff * mj * ay;
         vx[j] -= ff * mi * ax; vy[j] -= ff * mi * ay;
     }
     ld J = lw * S + lt * tk;
     for (int i = 0; i < n; i++) {
-        replay_fly_ld(i, t_end, f, x, y, vx, vy, tl);
+        replay_fly_ld(i, t_end, f, g, x, y, vx, vy, tl);
         if (lam)
             J += lam[4 * i] * x[i] + lam[4 * i + 1] * y[i] + lam[4 * i + 2] * vx[i] +
                  lam[4 * i + 3] * vy[i];
@@ -2040,7 +2102,7 @@ double orc_replay_fd(int n, double Lx, double Ly, const double* rad, const doubl
                      double res, const orc_tape_rec* tape, int64_t ntape, double t_end,
                      const double* x, const double* y, const double* vx, const double* vy,
                      const double* tl, const double* e, double h, double lw, double lt,
-                     const double* lam, double field) {
+                     const double* lam, double field, double gamma) {
     ld* buf = malloc((size_t)n * 5 * sizeof(ld));
     if (!buf) return NAN;
     ld J[2];
@@ -2055,7 +2117,7 @@ double orc_replay_fd(int n, double Lx, double Ly, const double* rad, const doubl
             TL[i] = tl[i];
         }
         J[side] = replay_objective_ld(n, Lx, Ly, rad, m, res, tape, ntape, t_end, X, Y, VX, VY,
-                                      TL, lw, lt, lam, (ld)field);
+                                      TL, lw, lt, lam, (ld)field, (ld)gamma);
     }
     free(buf);
     return (double)((J[0] - J[1]) / (2 * (ld)h));

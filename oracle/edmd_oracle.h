@@ -1,7 +1,7 @@
 /*
  * CPU restatement of the reference's event-driven hard-disk engine.
  *
- * TEST INFRASTRUCTURE ONLY.  This is the checker that the GPU library is
+ * TEST INFRASTRUCTURE ONLY.  This is what the GPU library is
  * compared against; it is never linked into, called by, or shipped with the
  * product.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
  * leg may load it.
@@ -208,8 +208,13 @@ void orc_sync_all(orc_ctx* s, double t);
  * field g along y (record field g) the velocity along y gains the field's
  * pull over dt as well: dv_y' += (resT - 1) g dt for a wall normal to x,
  * dv_y' -= (1 + resW) g dt for one normal to y (pair collisions and the
- * tangent's free flight are unchanged by a uniform field).  t_last is the
- * time of the last pair collision. */
+ * tangent's free flight are unchanged by a uniform field).  Under damping
+ * at rate gamma (the gamma argument; 0 without) the tangent's flight over h
+ * is dr += dv (1 - e)/gamma, dv *= e with e = exp(-gamma h) (orc_exp), a
+ * pair collision's dff loses gamma ff dt and its velocity jump, applied dt
+ * early, decays less: dv_i += gamma m_j ff dt a, dv_j -= gamma m_i ff dt a;
+ * wall maps are unchanged.  t_last is the time of the last pair
+ * collision. */
 typedef struct {
     int32_t i, j;
     double t, ax, ay, cx, cy, ff;
@@ -225,14 +230,14 @@ void orc_tape_clear(orc_ctx* s);
  * Returns the derivative of sum_k w_k; *dt_last = derivative of the time of
  * the last taped collision. */
 double orc_tangent(int n, const double* m, const double* tl0, const orc_tape_rec* tape,
-                   int64_t ntape, double t_end, double* tan, double* dt_last);
+                   int64_t ntape, double t_end, double* tan, double* dt_last, double gamma);
 /* Adjoint of the same map: the objective is lw * sum_k w_k + lt * t_last
  * + <lam_end, state at t_end> (lam_end may be NULL).  Writes its gradient
  * with respect to (x, y, vx, vy) stored at times tl0 into grad (4 per
  * particle). */
 void orc_adjoint(int n, const double* m, const double* tl0, const orc_tape_rec* tape,
                  int64_t ntape, double t_end, double lw, double lt, const double* lam_end,
-                 double* grad);
+                 double* grad, double gamma);
 /* The exponential and logarithm used under damping (see orc_params). */
 double orc_exp(double x);
 double orc_log(double x);
@@ -258,13 +263,15 @@ double orc_tape_wsum(const orc_ctx* s);
  * from the perturbed state (stable root of |a + c s|^2 = 4 r_i r_j), both
  * fly to it and the collision rule is applied; a wall record's disk flies
  * on to the taped contact coordinate and reflects; finally every disk flies
- * to t_end.  Flights follow the field along y (field; 0 without it).
+ * to t_end.  Flights follow the field along y (field; 0 without it) or
+ * are damped at rate gamma (0 without), a displacement tau of the straight
+ * motion then taking -log(1 - gamma tau)/gamma.
  * Returns sum of w_k; *t_last = time of the last replayed collision.
  * Returns NaN when a pair no longer meets. */
 double orc_replay(int n, double Lx, double Ly, const double* rad, const double* m,
                   double res, const orc_tape_rec* tape, int64_t ntape, double t_end,
                   double* x, double* y, double* vx, double* vy, double* tl,
-                  double* t_last, double field);
+                  double* t_last, double field, double gamma);
 
 /* Central difference (J(s0 + h e) - J(s0 - h e)) / 2h of
  *   J = lw * sum_k w_k + lt * t_last + <lam, state at t_end>   (lam may be NULL)
@@ -275,7 +282,7 @@ double orc_replay_fd(int n, double Lx, double Ly, const double* rad, const doubl
                      double res, const orc_tape_rec* tape, int64_t ntape, double t_end,
                      const double* x, const double* y, const double* vx, const double* vy,
                      const double* tl, const double* e, double h, double lw, double lt,
-                     const double* lam, double field);
+                     const double* lam, double field, double gamma);
 
 /* g(r) histogram of calculate_pcf (pcf.c:16-75).  Returns the bin count;
  * r and g must hold at least (int)(max_r/dr) values. */

@@ -1,6 +1,6 @@
 """ctypes binding of the oracle (oracle/build/liboracle.so).
 
-Test infrastructure: the oracle is the checker the GPU library is compared
+Test infrastructure: the oracle is what the GPU library is compared
 against, never a code path of the library itself.
 """
 from __future__ import annotations
@@ -87,13 +87,15 @@ def lib():
         L.orc_get_tape.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_tape_clear.argtypes = [C.c_void_p]
         L.orc_tangent.restype = C.c_double
-        L.orc_tangent.argtypes = [C.c_int, dp, dp, C.c_void_p, C.c_int64, C.c_double, dp, dp]
+        L.orc_tangent.argtypes = [C.c_int, dp, dp, C.c_void_p, C.c_int64, C.c_double, dp, dp,
+                                  C.c_double]
         L.orc_adjoint.argtypes = [C.c_int, dp, dp, C.c_void_p, C.c_int64, C.c_double,
-                                  C.c_double, C.c_double, C.c_void_p, dp]
+                                  C.c_double, C.c_double, C.c_void_p, dp, C.c_double]
         L.orc_replay_fd.restype = C.c_double
         L.orc_replay_fd.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp, C.c_double,
                                     C.c_void_p, C.c_int64, C.c_double] + [dp] * 6 + [
-                                    C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_double]
+                                    C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_double,
+                                    C.c_double]
         L.orc_exp.restype = C.c_double
         L.orc_exp.argtypes = [C.c_double]
         L.orc_log.restype = C.c_double
@@ -109,7 +111,8 @@ def lib():
         L.orc_tape_wsum.argtypes = [C.c_void_p]
         L.orc_replay.restype = C.c_double
         L.orc_replay.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp, C.c_double,
-                                 C.c_void_p, C.c_int64, C.c_double] + [dp] * 5 + [dp, C.c_double]
+                                 C.c_void_p, C.c_int64, C.c_double] + [dp] * 5 + [dp, C.c_double,
+                                                                                 C.c_double]
         _lib = L
     return _lib
 
@@ -125,8 +128,8 @@ def mt_draws(seed, n):
     return out
 
 
-def tangent(m, tl0, tape, t_end, tan):
-    """Propagates tan (n x 4: dx dy dvx dvy) in place.
+def tangent(m, tl0, tape, t_end, tan, gamma=0.0):
+    """Propagates tan (n x 4: dx dy dvx dvy) in place (gamma: damping rate).
 
     Returns (d sum w, d t_last)."""
     m = np.ascontiguousarray(m, dtype=np.float64)
@@ -135,11 +138,11 @@ def tangent(m, tl0, tape, t_end, tan):
     tape = np.ascontiguousarray(tape, dtype=TAPE_DT)
     dtl = C.c_double()
     dS = lib().orc_tangent(len(m), _dp(m), _dp(tl0), tape.ctypes.data, len(tape), t_end,
-                           _dp(tan), C.byref(dtl))
+                           _dp(tan), C.byref(dtl), float(gamma))
     return dS, dtl.value
 
 
-def adjoint(m, tl0, tape, t_end, lw, lt=0.0, lam_end=None):
+def adjoint(m, tl0, tape, t_end, lw, lt=0.0, lam_end=None, gamma=0.0):
     """Gradient (n x 4) of lw * sum w + lt * t_last + <lam_end, final state>
     with respect to (x, y, vx, vy) at tl0."""
     m = np.ascontiguousarray(m, dtype=np.float64)
@@ -150,11 +153,12 @@ def adjoint(m, tl0, tape, t_end, lw, lt=0.0, lam_end=None):
     if lam_end is not None:
         le = np.ascontiguousarray(lam_end, dtype=np.float64).reshape(len(m), 4)
     lib().orc_adjoint(len(m), _dp(m), _dp(tl0), tape.ctypes.data, len(tape), t_end, lw, lt,
-                      le.ctypes.data if le is not None else None, _dp(g))
+                      le.ctypes.data if le is not None else None, _dp(g), float(gamma))
     return g
 
 
-def replay_fd(st, tape, t_end, e, h=1e-7, lw=1.0, lt=0.0, lam=None, res=1.0, field=0.0):
+def replay_fd(st, tape, t_end, e, h=1e-7, lw=1.0, lt=0.0, lam=None, res=1.0, field=0.0,
+              gamma=0.0):
     """Extended-precision central difference of lw * sum w + lt * t_last +
     <lam, final state> along e (n x 4) over the fixed-sequence replay
     (flights under the field along y when field != 0)."""
@@ -167,10 +171,10 @@ def replay_fd(st, tape, t_end, e, h=1e-7, lw=1.0, lt=0.0, lam=None, res=1.0, fie
                                res, tape.ctypes.data, len(tape), t_end, _dp(k["x"]),
                                _dp(k["y"]), _dp(k["vx"]), _dp(k["vy"]), _dp(k["t"]), _dp(e),
                                h, lw, lt, la.ctypes.data if la is not None else None,
-                               float(field))
+                               float(field), float(gamma))
 
 
-def replay(st, tape, t_end, res=1.0, field=0.0):
+def replay(st, tape, t_end, res=1.0, field=0.0, gamma=0.0):
     """Fixed-sequence replay of tape from st (x y vx vy and t per disk).
 
     Returns (sum w, time of the last collision, final state dict)."""
@@ -182,7 +186,8 @@ def replay(st, tape, t_end, res=1.0, field=0.0):
     tk = C.c_double()
     S = lib().orc_replay(len(m), st["Lx"], st["Ly"], _dp(rad), _dp(m), res,
                          tape.ctypes.data, len(tape), t_end, _dp(out["x"]), _dp(out["y"]),
-                         _dp(out["vx"]), _dp(out["vy"]), _dp(tl), C.byref(tk), float(field))
+                         _dp(out["vx"]), _dp(out["vy"]), _dp(tl), C.byref(tk), float(field),
+                         float(gamma))
     return S, tk.value, out
 
 

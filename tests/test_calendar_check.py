@@ -1,4 +1,4 @@
-"""The host-side calendar checker used by the GPU tests: it accepts a
+"""The host-side calendar consistency test used by the GPU tests: it accepts a
 calendar built by the rule it states and flags planted mistakes."""
 import numpy as np
 

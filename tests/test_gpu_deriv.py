@@ -250,6 +250,10 @@ WALL_CASES = [
      dict(wall_x=1, wall_y=1, add_field=1, field=-0.1, res_wall=0.9, res_tan=0.5), 4000),
     ("gravity channel", lambda: F.walled_state(400, 0.55, 27, wall_y=True),
      dict(wall_y=1, add_field=1, field=-0.2, res_tan=1.0), 4000),
+    ("damped box", lambda: F.walled_state(400, 0.55, 28, wall_x=True, wall_y=True),
+     dict(wall_x=1, wall_y=1, damping=1, gamma=0.01, res_wall=0.9, res_tan=0.5), 4000),
+    ("damped periodic", lambda: F.lattice_state(400, 0.60, seed=29, lattice="fit"),
+     dict(damping=1, gamma=0.02), 4000),
 ]
 
 
@@ -261,13 +265,14 @@ def test_walled_tape_tangent_adjoint_match_oracle(gpu, oracle, name, make, param
     st = make()
     o, s0, tape = oracle_taped(oracle, st, ncoll, **params)
     nwall = int(np.sum(tape["j"] < 0))
-    assert nwall >= 20, (name, nwall)
+    assert nwall >= (20 if "wall_x" in params or "wall_y" in params else 0), (name, nwall)
+    gam = params.get("gamma", 0.0) if params.get("damping") else 0.0
     rng = np.random.default_rng(41)
     tin = rng.standard_normal((len(st["x"]), 4))
     lam = rng.standard_normal((len(st["x"]), 4))
     tan = tin.copy()
-    rdw, rdtl = oracle.tangent(st["m"], s0["t"], tape, o.t, tan)
-    rg = oracle.adjoint(st["m"], s0["t"], tape, o.t, 0.8, -1.5, lam)
+    rdw, rdtl = oracle.tangent(st["m"], s0["t"], tape, o.t, tan, gamma=gam)
+    rg = oracle.adjoint(st["m"], s0["t"], tape, o.t, 0.8, -1.5, lam, gamma=gam)
     for every in (0, 97):
         b = gpu.Batch([st], **params)
         b.deriv_begin(checkpoint_every=every, tape_capacity=ncoll + 16)
@@ -287,15 +292,17 @@ def test_walled_tape_tangent_adjoint_match_oracle(gpu, oracle, name, make, param
         np.testing.assert_array_equal(bits(g[0]), bits(rg), err_msg="%s %d" % (name, every))
 
 
-@pytest.mark.parametrize("field", [0.0, -0.1])
-def test_walled_gradient_matches_finite_differences(gpu, oracle, field):
+@pytest.mark.parametrize("field,gamma", [(0.0, 0.0), (-0.1, 0.0), (0.0, 0.01)])
+def test_walled_gradient_matches_finite_differences(gpu, oracle, field, gamma):
     """W / t_last over 600 events in a closed box with inelastic walls,
-    with and without gravity: the GPU gradient against central differences
-    of the oracle's replay."""
+    plain, with gravity and damped: the GPU gradient against central
+    differences of the oracle's replay."""
     st = F.walled_state(400, 0.55, 24, wall_x=True, wall_y=True)
     params = dict(wall_x=1, wall_y=1, res_wall=0.9, res_tan=0.7)
     if field:
         params.update(add_field=1, field=field)
+    if gamma:
+        params.update(damping=1, gamma=gamma)
     b = gpu.Batch([st], **params)
     b.deriv_begin(checkpoint_every=256)
     b.deriv_run(max_collisions=600)
@@ -307,14 +314,16 @@ def test_walled_gradient_matches_finite_differences(gpu, oracle, field):
     rng = np.random.default_rng(6)
     for _ in range(3):
         e = rng.standard_normal((400, 4))
-        fd = oracle.replay_fd(s0, tape, o.t, e, 1e-7, 1 / T, -W / T ** 2, field=field)
+        fd = oracle.replay_fd(s0, tape, o.t, e, 1e-7, 1 / T, -W / T ** 2, field=field,
+                              gamma=gamma)
         ad = (grad * e).sum()
         assert abs(fd - ad) <= 1e-6 * abs(ad)
 
 
 def test_unsupported_models_are_rejected_for_taping(gpu):
-    st = F.walled_state(400, 0.55, 25, wall_x=True)
-    for kw in (dict(damping=1, gamma=0.01), dict(add_field=1, damping=1, gamma=0.01)):
+    st = F.walled_state(400, 0.55, 25, wall_x=True, wall_y=True)
+    for kw in (dict(add_field=1, damping=1, gamma=0.01), dict(thermal_wall=1),
+               dict(wall_y=1, shear_wall=1)):
         b = gpu.Batch([st], wall_x=1, **kw)
         with pytest.raises(gpu.EdmdError) as e:
             b.deriv_begin()

@@ -259,7 +259,11 @@ def test_damping_arguments_are_checked(gpu):
     st = F.lattice_state(100, 0.4, seed=5, lattice="fit")
     with pytest.raises(gpu.EdmdError):
         gpu.System(st, damping=1, gamma=0.0)
+    # damping is taped (tests/test_gpu_deriv.py), not together with the field
     b = gpu.Batch([st], damping=1)
+    b.deriv_begin(checkpoint_every=256)
+    b.deriv_end()
+    b = gpu.Batch([st], damping=1, add_field=1)
     with pytest.raises(gpu.EdmdError):
         b.deriv_begin(checkpoint_every=256)
 
@@ -331,7 +335,7 @@ CIRCLE_CASES = [
 def test_circular_wall_matches_oracle(gpu, oracle, n, phi, seed, poly, vr, kw):
     """Disks inside a circular wall: every disk's circle time, hits that
     reflect about the normal at the library's own atan2 / sin / cos (the
-    checker runs the same steps), no minimum image, thermo rows over the
+    oracle runs the same steps), no minimum image, thermo rows over the
     disc's area with the wall pressure over pi Lx, growth starts through
     GROWSTOP's angular-momentum removal: traces, states and thermo rows bit
     for bit against the oracle, from HBM and from LDS."""
@@ -473,7 +477,7 @@ THERMAL_CASES = [
 def test_thermal_and_shear_walls_match_oracle(gpu, oracle, kind, seed, vr, kw):
     """Wall hits that draw the new normal speed from the system's own
     MT19937 stream (twisted wave-parallel on the GPU, sequentially in the
-    checker), the circle's two draws, shear walls driving +-0.1 along x,
+    oracle), the circle's two draws, shear walls driving +-0.1 along x,
     the mid wall's solute and a growth start whose walls turn thermal at
     GROWSTOP: traces, states and thermo rows bit for bit against the
     oracle, from HBM and LDS; then the same after the autotuner's trials,

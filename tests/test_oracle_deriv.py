@@ -397,3 +397,88 @@ def test_field_adjoint_matches_finite_differences(name, params, walls):
         fd = ob.replay_fd(s0, tape, te, e, 1e-7, lw, lt, lam, field=f)
         ad = (g * e).sum()
         assert abs(fd - ad) <= 1e-6 * abs(ad), name
+
+
+# ---- damping on the tape
+
+def test_damped_head_on_closed_form():
+    """Two unit disks head on under damping at rate g: the relative speed
+    decays with the common factor E(t) = exp(-g t), contact comes when the
+    displacement (v0 - v1)(1 - E)/g closes the gap x1 - x0 - 1, so
+    E = 1 - g gap/(v0 - v1) and W = (a.c) = (v1 - v0) + g gap, and
+    t_c = -log(E)/g.  Undamped, W would not depend on the positions."""
+    g = 0.05
+    st = two_disks(dx=2.0, v=1.0)
+    o, s0, tape = taped(st, 1, damping=1, gamma=g)
+    assert len(tape) == 1
+    gap, dv = 1.0, 2.0
+    E = 1 - g * gap / dv
+    assert o.tape_wsum() == pytest.approx(-dv + g * gap, rel=1e-13)
+    assert tape["t"][0] == pytest.approx(-np.log(E) / g, rel=1e-13)
+    m, tl0 = st["m"], s0["t"]
+    G = ob.adjoint(m, tl0, tape, 1.0, 1.0, gamma=g)
+    np.testing.assert_allclose(G, [[-g, 0, -1, 0], [g, 0, 1, 0]], atol=1e-14)
+    G = ob.adjoint(m, tl0, tape, 1.0, 0.0, 1.0, gamma=g)
+    dtc = 1 / (E * dv)
+    np.testing.assert_allclose(G[:, 0], [-dtc, dtc], rtol=1e-13)
+
+
+def test_damped_wall_bounce_closed_form():
+    """A disk damped at rate g meets the left x wall (resW 0.8, resT 0.5)
+    and is flown to T: its velocity at the hit is vx + g (x0 - r), so
+    vx(T) = -resW vx e^{-gT} and x(T) = r - resW (vh - vx e^{-gT})/g,
+    vy(T) = resT vy e^{-gT}: the undamped partials with T replaced by
+    (1 - e^{-gT})/g where the flight enters."""
+    g, x0, y0, vx, vy, r, T, rw, rt = 0.1, 3.0, 10.0, -1.0, 0.6, 0.5, 4.0, 0.8, 0.5
+    st = {"x": np.array([x0, 15.0]), "y": np.array([y0, 5.0]),
+          "vx": np.array([vx, 0.0]), "vy": np.array([vy, 0.0]),
+          "rad": np.array([r, r]), "m": np.array([1.0, 1.0]), "Lx": 20.0, "Ly": 20.0}
+    o, s0, tape = taped(st, 1, wall_x=1, res_wall=rw, res_tan=rt, damping=1, gamma=g)
+    assert len(tape) == 1 and tape["j"][0] == -1
+    eT = np.exp(-g * T)
+    E = 1 - g * (r - x0) / vx  # e^{-g th}
+    phT = (1 - eT) / g
+    want = {
+        0: [-rw, 0, -rw * phT, 0],
+        1: [(1 - rt) * vy / (-vx), 1, (1 - rt) * vy * (x0 - r) / vx ** 2,
+            (1 - E) / g + rt * (E - eT) / g],
+        2: [0, 0, -rw * eT, 0],
+        3: [0, 0, 0, rt * eT],
+    }
+    m, tl0 = st["m"], s0["t"]
+    for c, row in want.items():
+        lam = np.zeros((2, 4))
+        lam[0, c] = 1.0
+        G = ob.adjoint(m, tl0, tape, T, 0.0, 0.0, lam, gamma=g)
+        np.testing.assert_allclose(G[0], row, rtol=1e-12, atol=1e-13, err_msg=str(c))
+
+
+DAMPED_CASES = [
+    ("damped periodic", dict(damping=1, gamma=0.02), dict(wall_x=False)),
+    ("damped box", dict(damping=1, gamma=0.01, wall_x=1, wall_y=1, res_wall=0.9, res_tan=0.5),
+     dict(wall_x=True, wall_y=True)),
+]
+
+
+@pytest.mark.parametrize("name,params,walls", DAMPED_CASES)
+def test_damped_adjoint_matches_finite_differences(name, params, walls):
+    st = walled_state(400, 0.55, 41, **walls)
+    o, s0, tape = taped(st, 600, **params)
+    g = params["gamma"]
+    te = o.t
+    S, tk, fin = ob.replay(s0, tape, te, gamma=g)
+    assert tk == pytest.approx(_last_pair_time(tape), abs=1e-11)
+    assert S == pytest.approx(o.tape_wsum(), rel=1e-10)
+    rng = np.random.default_rng(8)
+    lam = rng.standard_normal((400, 4))
+    lw, lt = 1 / tk, -S / tk ** 2
+    G = ob.adjoint(st["m"], s0["t"], tape, te, lw, lt, lam, gamma=g)
+    for _ in range(2):
+        e = rng.standard_normal((400, 4))
+        fd = ob.replay_fd(s0, tape, te, e, 1e-7, lw, lt, lam, gamma=g)
+        ad = (G * e).sum()
+        assert abs(fd - ad) <= 1e-6 * abs(ad), name
+    e = rng.standard_normal((400, 4))
+    tan = e.copy()
+    dW, dtl = ob.tangent(st["m"], s0["t"], tape, te, tan, gamma=g)
+    assert abs((G * e).sum() - (lw * dW + lt * dtl + (lam * tan).sum())) <= 1e-11 * abs((G * e).sum())
