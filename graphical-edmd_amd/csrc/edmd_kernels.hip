@@ -3655,12 +3655,13 @@ hipError_t edmd_launch_sync_all(const EdmdSys* d_sys, int nsys, int max_n, hipSt
     return hipGetLastError();
 }
 
-/* W: some system of the launch has walls (never together with growth). */
+/* W: some system of the launch has walls, the field or damping. */
 template <bool G, bool W>
 static void launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, EdmdRunArgs ra, hipStream_t s) {
     ra.nsys = nsys;
     if (max_nlev <= 1) hipLaunchKernelGGL((k_run_small<1, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
     else if (max_nlev == 2) hipLaunchKernelGGL((k_run_small<2, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    else if (max_nlev == 3) hipLaunchKernelGGL((k_run<3, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
     else if (max_nlev <= 4) hipLaunchKernelGGL((k_run<4, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
     else hipLaunchKernelGGL((k_run<EDMD_MAX_LEVELS, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
 }
