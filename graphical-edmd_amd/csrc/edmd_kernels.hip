@@ -50,13 +50,19 @@
  * memory operations across the point. */
 #define WAVE_SYNC() __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront")
 
-/* Values that are equal on every lane, pinned to scalar registers. */
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ double uni(double v) {
-    long long b = __double_as_longlong(v);
-    int lo = __builtin_amdgcn_readfirstlane((int)b);
-    int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+/* Marks a value that is equal on every lane.  It stays in vector
+ * registers: the event loop has few scalar registers to spare, and pinning
+ * these values to them (readfirstlane) made the compiler spill about 130
+ * scalar registers into vector-register lanes, whose reloads cost more
+ * vector instructions than they saved (8 % of the batched loop's rate). */
+__device__ __forceinline__ int uni(int v) { return v; }
+__device__ __forceinline__ double uni(double v) { return v; }
+
+/* The wavefront's index within its workgroup, in a scalar register: the
+ * system a wavefront runs is addressed through it, so its record is read
+ * with scalar loads. */
+__device__ __forceinline__ int wave_in_group() {
+    return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 }
 
 __device__ __forceinline__ int64_t uni64(int64_t v) {
@@ -3078,7 +3084,7 @@ __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRu
     EdmdState* st = A.st;
     __shared__ QueueState Qs[WPB];
     LoopRegs R;
-    R.q = &Qs[WPB > 1 ? uni((int)(threadIdx.x >> 6)) : 0];
+    R.q = &Qs[WPB > 1 ? wave_in_group() : 0];
     R.t = uni(st->t);
     R.ncol = st->ncol;
     R.nevents = st->nevents;
@@ -3180,7 +3186,7 @@ __global__ __launch_bounds__(64 * WPB, 4) void k_run_small(const EdmdSys* __rest
                                                            EdmdRunArgs ra) {
     /* WPB systems per workgroup, one per wavefront (the launch's systems
      * per workgroup are tuned; the last workgroup may hold fewer) */
-    const int sys = WPB > 1 ? (int)blockIdx.x * WPB + uni((int)(threadIdx.x >> 6)) : (int)blockIdx.x;
+    const int sys = WPB > 1 ? (int)blockIdx.x * WPB + wave_in_group() : (int)blockIdx.x;
     if (WPB > 1 && sys >= ra.nsys) return;
     const EdmdSys& S = systems[sys];
     /* G launches run the systems that are growing, the others the rest */
