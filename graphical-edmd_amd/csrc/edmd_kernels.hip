@@ -3667,9 +3667,11 @@ static void launch_run(const EdmdSys* d_sys, int nsys, int max_nlev, EdmdRunArgs
     ra.nsys = nsys;
     /* the _small loop is bounded to 128 vector registers so that four
      * wavefronts share each SIMD; a batch of at most one system per SIMD
-     * gains nothing from that, and the depth-2 loop spills under the bound */
+     * gains nothing from that and runs the unbounded loop (the depth-2
+     * loop spills to scratch under the bound) */
     const bool few = nsys <= 1024;
-    if (max_nlev <= 1) hipLaunchKernelGGL((k_run_small<1, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    if (max_nlev <= 1 && few) hipLaunchKernelGGL((k_run<1, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
+    else if (max_nlev <= 1) hipLaunchKernelGGL((k_run_small<1, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
     else if (max_nlev == 2 && few) hipLaunchKernelGGL((k_run<2, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
     else if (max_nlev == 2) hipLaunchKernelGGL((k_run_small<2, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
     else if (max_nlev == 3) hipLaunchKernelGGL((k_run<3, G, W>), dim3(nsys), dim3(64), 0, s, d_sys, ra);
