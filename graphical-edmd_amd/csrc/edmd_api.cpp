@@ -146,6 +146,8 @@ struct edmd_batch {
     bool use_lds = false;
     /* some system has walls, the field or damping: the kernels' variants for them */
     bool any_walls = false;
+    /* some system's radii differ (or grow): the sweep stages each disk's radius */
+    bool any_poly = false;
     /* systems per workgroup of the HBM-resident loop (1, 2 or 4; more than
      * one only for calendars of depth <= 2 without walls, field or damping,
      * and not for the growing systems' launch) */
@@ -432,9 +434,14 @@ static int upload_system(edmd_batch* b, int s, const edmd_system_desc* d) {
     return EDMD_OK;
 }
 
+/* The sweep kernel's variant for the batch's systems. */
+static int sweep_ext(const edmd_batch* b) {
+    return (b->any_walls ? EDMD_SWEEP_EXT : 0) | (b->any_poly ? EDMD_SWEEP_POLY : 0);
+}
+
 static int batch_predict_all(edmd_batch* b) {
     HIPCHK(edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->sweep_variant,
-                                   b->any_walls ? 1 : 0, b->stream));
+                                   sweep_ext(b), b->stream));
     if (b->any_grow) HIPCHK(edmd_launch_sweep_grow(b->d_sys, b->nsys, b->max_n, b->stream));
     HIPCHK(edmd_launch_sweep_count(b->d_sys, b->nsys, b->stream));
     HIPCHK(edmd_launch_tree(b->d_sys, b->nsys, b->max_lsize, b->max_nlev, b->stream));
@@ -509,6 +516,7 @@ extern "C" int edmd_batch_create(const edmd_system_desc* descs, int nsys, edmd_b
     for (int s = 0; s < nsys; s++) b->any_grow |= descs[s].vr > 0;
     for (int s = 0; s < nsys; s++)
         b->any_walls |= b->hs[s].c.walls != 0 || b->hs[s].c.field_on || b->hs[s].c.damping;
+    for (int s = 0; s < nsys; s++) b->any_poly |= !b->hs[s].c.mono;
     if (!rc && b->any_grow) rc = dalloc(b, &b->vr, np);
     bool any_mid = false;
     for (int s = 0; s < nsys; s++) any_mid |= descs[s].mid_wall != 0;
@@ -815,12 +823,12 @@ extern "C" int edmd_batch_autotune(edmd_batch* b, int sweep_reps, int64_t trial_
         double bt = 1e300;
         for (int v = 0; v < EDMD_SWEEP_VARIANTS && e == hipSuccess; v++) {
             set_sweep_variant(b, v);
-            e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, v, b->any_walls ? 1 : 0,
+            e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, v, sweep_ext(b),
                                         b->stream);
             if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
             for (int k = 0; k < sweep_reps && e == hipSuccess; k++)
-                e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, v,
-                                            b->any_walls ? 1 : 0, b->stream);
+                e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, v, sweep_ext(b),
+                                            b->stream);
             if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
@@ -1015,7 +1023,7 @@ extern "C" int edmd_batch_time_sweep(edmd_batch* b, int reps, double* avg_ms) {
         e = hipEventRecord(e0, b->stream);
         for (int r = 0; r < reps && e == hipSuccess; r++)
             e = edmd_launch_sweep_tiled(b->d_sys, b->nsys, b->max_tiles, b->sweep_variant,
-                                        b->any_walls ? 1 : 0, b->stream);
+                                        sweep_ext(b), b->stream);
         if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);

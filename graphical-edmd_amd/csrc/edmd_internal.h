@@ -13,8 +13,10 @@ typedef struct EdmdSweepShape {
 } EdmdSweepShape;
 #define EDMD_SWEEP_VARIANTS 8
 EdmdSweepShape edmd_sweep_shape(int variant);
-/* ext: some system has walls, the field or damping (the variant of the
- * kernel that handles them) */
+/* ext: EDMD_SWEEP_EXT when some system has walls, the field or damping
+ * (the variant of the kernel that handles them), EDMD_SWEEP_POLY when some
+ * system's radii differ (the layout that stages each disk's radius) */
+enum { EDMD_SWEEP_EXT = 1, EDMD_SWEEP_POLY = 2 };
 hipError_t edmd_launch_sweep_tiled(const EdmdSys* d_sys, int nsys, int max_tiles, int variant,
                                    int ext, hipStream_t s);
 int edmd_sweep_tiles(int ncx, int ncy, int variant);

@@ -1653,21 +1653,28 @@ using SweepV5 = SweepShape<8, 8, 256>;
 using SweepV6 = SweepShape<8, 8, 64>;
 using SweepV7 = SweepShape<16, 8, 128>;
 
-template <class S>
+/* The workgroup's LDS.  POLY = false is the layout of launches whose
+ * systems all have equal radii: no radius per staged disk (4 KB less, one
+ * more workgroup per CU).  The cell counts and the own-cell counts of the
+ * two scans live in the velocity stage, which is written only after them. */
+template <class S, bool POLY_>
 struct SweepLds {
+    static constexpr bool POLY = POLY_;
     /* staged disks: position flown to the sweep time, velocity, radius,
      * index, staged cell */
     double2 pos[S::STAGE], vel[S::STAGE];
-    double rad[S::STAGE];
+    double rad[POLY ? S::STAGE : 1];
     int32_t id[S::STAGE];
     uint32_t coll[S::STAGE]; /* collision counter (the partner counter a prediction records) */
     uint8_t hcell[S::STAGE];
     int16_t own[S::STAGE];     /* staged index of each of the tile's own disks */
-    int32_t cnt[S::NH];
     int32_t start[S::NH + 1];  /* first staged index of each cell; [S::NH] = total */
     int32_t ostart[S::NH];     /* first own-list index of each tile cell */
     int32_t gcell[S::NH];
     int32_t ninner, total;
+    static_assert(2 * S::NH * sizeof(int32_t) <= sizeof(vel), "scan counts fit the velocity stage");
+    __device__ int32_t* cnt() { return reinterpret_cast<int32_t*>(&vel[0]); }
+    __device__ int32_t* owncnt() { return reinterpret_cast<int32_t*>(&vel[0]) + S::NH; }
 };
 
 __device__ __forceinline__ int mod_cell(int a, int nc) {
@@ -1743,11 +1750,12 @@ __device__ __forceinline__ uint64_t keep_bits(int tot, int kself, int base) {
  * takes), flagging overlaps as collisionTime does; pass 2 solves the marked
  * roots in increasing k.  A disk with more than 256 candidates solves its
  * roots in pass 1 instead. */
-template <class S, bool WRAP, bool X, bool MONO, class AR>
-__device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR& A,
-                                                   const SweepLds<S>& L, int tid, int ninner, double t,
-                                                   int tx0, int ty0, double paul, int64_t ctr0,
+template <class S, bool WRAP, bool X, bool MONO, class AR, class LD>
+__device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR& A, const LD& L,
+                                                   int tid, int ninner, double t, int tx0, int ty0,
+                                                   double paul, int64_t ctr0, double rad0,
                                                    bool& ov) {
+    static_assert(MONO || LD::POLY, "unequal radii need the staged radii");
     const int lane = tid & 63;
     for (int u0 = tid - lane; u0 < ninner; u0 += S::THREADS) { /* uniform over the wave */
         const int u = u0 + lane;
@@ -1757,7 +1765,8 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
         const int lx = h % S::HW - 1, ly = h / S::HW - 1;
         /* the owner is synchronised at t (its staged position is its own) */
         const double2 opos = L.pos[me], ovel = L.vel[me];
-        const double ox = opos.x, oy = opos.y, ovx = ovel.x, ovy = ovel.y, orad = L.rad[me];
+        const double ox = opos.x, oy = opos.y, ovx = ovel.x, ovy = ovel.y;
+        const double orad = LD::POLY ? L.rad[me] : rad0;
         const int p = L.id[me];
         /* row runs: cells (lx - 1 .. lx + 1, ly + j) are staged contiguously */
         const int h0 = ly * S::HW + lx;
@@ -1946,10 +1955,11 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
     }
 }
 
-template <class S, bool X>
+template <class S, bool X, bool POLY>
 __global__ __launch_bounds__(S::THREADS) void k_sweep_tiled(const EdmdSys* __restrict__ systems) {
-    __shared__ SweepLds<S> L;
-    __shared__ int vtmp[S::NH];
+    __shared__ SweepLds<S, POLY> L;
+    int32_t* const cnt = L.cnt();
+    int32_t* const vtmp = L.owncnt();
     const EdmdSys* Sy = systems + blockIdx.y;
     const EdmdConst& C = Sy->c;
     const EdmdArrays& A = Sy->a;
@@ -1972,7 +1982,7 @@ __global__ __launch_bounds__(S::THREADS) void k_sweep_tiled(const EdmdSys* __res
         const int lx = h % S::HW - 1, ly = h / S::HW - 1;
         const int g = mod_cell(ty0 + ly, ncy) * ncx + mod_cell(tx0 + lx, ncx);
         L.gcell[h] = g;
-        L.cnt[h] = A.ccnt[g];
+        cnt[h] = A.ccnt[g];
     }
     __syncthreads();
     /* staged order: cells row-major (h), each cell newest entry first;
@@ -1982,10 +1992,10 @@ __global__ __launch_bounds__(S::THREADS) void k_sweep_tiled(const EdmdSys* __res
         for (int h = PER * tid; h < PER * tid + PER && h < S::NH; ++h) {
             const int lx = h % S::HW - 1, ly = h / S::HW - 1;
             const bool own = lx >= 0 && lx < vw && ly >= 0 && ly < vh;
-            vtmp[h] = own ? L.cnt[h] : 0;
+            vtmp[h] = own ? cnt[h] : 0;
         }
         const int nin = sweep_scan<S::NH>(tid, vtmp, L.ostart, 0);
-        const int tot = sweep_scan<S::NH>(tid, L.cnt, L.start, 0);
+        const int tot = sweep_scan<S::NH>(tid, cnt, L.start, 0);
         if (tid == 0) {
             L.start[S::NH] = tot;
             L.ninner = nin;
@@ -1998,7 +2008,7 @@ __global__ __launch_bounds__(S::THREADS) void k_sweep_tiled(const EdmdSys* __res
     if (total <= S::STAGE) {
         const int cap = C.cap;
         for (int h = tid; h < S::NH; h += S::THREADS) {
-            const int st0 = L.start[h], c = L.cnt[h];
+            const int st0 = L.start[h], c = cnt[h];
             const int lx = h % S::HW - 1, ly = h / S::HW - 1;
             const bool own = lx >= 0 && lx < vw && ly >= 0 && ly < vh;
             for (int sl = 0; sl < c; ++sl) L.hcell[st0 + sl] = (uint8_t)h;
@@ -2024,14 +2034,14 @@ __global__ __launch_bounds__(S::THREADS) void k_sweep_tiled(const EdmdSys* __res
             const double lat0 = t - r0.tl;
             L.pos[u0] = make_double2(r0.x + lat0 * r0.vx, r0.y + lat0 * r0.vy);
             L.vel[u0] = make_double2(r0.vx, r0.vy);
-            L.rad[u0] = r0.rad;
+            if (POLY) L.rad[u0] = r0.rad;
             L.id[u0] = q0;
             L.coll[u0] = r0.coll;
             if (q1 >= 0) {
                 const double lat1 = t - r1.tl;
                 L.pos[u1] = make_double2(r1.x + lat1 * r1.vx, r1.y + lat1 * r1.vy);
                 L.vel[u1] = make_double2(r1.vx, r1.vy);
-                L.rad[u1] = r1.rad;
+                if (POLY) L.rad[u1] = r1.rad;
                 L.id[u1] = q1;
                 L.coll[u1] = r1.coll;
             }
@@ -2049,13 +2059,28 @@ __global__ __launch_bounds__(S::THREADS) void k_sweep_tiled(const EdmdSys* __res
         const bool nowrap = tx0 >= 1 && ty0 >= 1 && tx0 + vw <= ncx - 2 && ty0 + vh <= ncy - 2 &&
                             ncx >= 5 && ncy >= 5;
         /* equal radii: the contact term 4 r r of every pair is the owner's
-         * own (bitwise the same product), no radius read per candidate */
-        if (nowrap && C.mono)
-            sweep_tile_predict<S, false, X, true>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
-        else if (nowrap)
-            sweep_tile_predict<S, false, X, false>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
-        else
-            sweep_tile_predict<S, true, X, false>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0, ov);
+         * own (bitwise the same product), no radius read per candidate;
+         * without staged radii it is disk 0's (every radius is bitwise the
+         * same) */
+        const double rad0 = POLY ? 0.0 : A.p[0].rad;
+        if constexpr (!POLY) {
+            if (nowrap)
+                sweep_tile_predict<S, false, X, true>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0,
+                                                      rad0, ov);
+            else
+                sweep_tile_predict<S, true, X, true>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0,
+                                                     rad0, ov);
+        } else {
+            if (nowrap && C.mono)
+                sweep_tile_predict<S, false, X, true>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0,
+                                                      rad0, ov);
+            else if (nowrap)
+                sweep_tile_predict<S, false, X, false>(C, A, L, tid, ninner, t, tx0, ty0, paul,
+                                                       ctr0, rad0, ov);
+            else
+                sweep_tile_predict<S, true, X, false>(C, A, L, tid, ninner, t, tx0, ty0, paul, ctr0,
+                                                      rad0, ov);
+        }
     } else {
         const int cap = C.cap;
         for (int w = tid; w < S::TX * S::TY * cap; w += S::THREADS) {
@@ -3598,8 +3623,12 @@ __global__ void k_fp64_check(int n, const double* __restrict__ a, const double* 
 template <class S>
 static void launch_sweep(const EdmdSys* d_sys, int nsys, int max_tiles, int ext, hipStream_t s) {
     dim3 grid((max_tiles + 7) / 8 * 8, nsys);
-    if (ext) hipLaunchKernelGGL((k_sweep_tiled<S, true>), grid, dim3(S::THREADS), 0, s, d_sys);
-    else hipLaunchKernelGGL((k_sweep_tiled<S, false>), grid, dim3(S::THREADS), 0, s, d_sys);
+    if (ext & EDMD_SWEEP_EXT)
+        hipLaunchKernelGGL((k_sweep_tiled<S, true, true>), grid, dim3(S::THREADS), 0, s, d_sys);
+    else if (ext & EDMD_SWEEP_POLY)
+        hipLaunchKernelGGL((k_sweep_tiled<S, false, true>), grid, dim3(S::THREADS), 0, s, d_sys);
+    else
+        hipLaunchKernelGGL((k_sweep_tiled<S, false, false>), grid, dim3(S::THREADS), 0, s, d_sys);
 }
 
 hipError_t edmd_launch_sweep_tiled(const EdmdSys* d_sys, int nsys, int max_tiles, int variant,
