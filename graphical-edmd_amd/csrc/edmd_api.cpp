@@ -285,6 +285,7 @@ static int setup_system(const edmd_system_desc* d, HostSys* h, int idx) {
     c.mihy = (d->wall_y || d->wall_circle) ? __builtin_huge_val() : c.halfLy;
     c.dtp = 5.0 / (double)d->n; /* dtPaul, EDMD.c:712-714 */
     c.nlists = d->n;
+    c.horizon = c.dtp * c.nlists;
     c.res = d->res;
     c.dtime = d->dtime;
     c.dtime_thermo = d->dtime_thermo;

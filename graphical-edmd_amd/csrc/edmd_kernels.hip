@@ -863,9 +863,14 @@ __device__ __forceinline__ void cross_time(const EdmdConst& C, double t, double 
 
 /* Rank of an event scheduled at t_ev with the queue at paulTime = paul after
  * ctr earlier insertions (addEventToQueue, EDMD.c:2144-2170): straight into
- * the tree when t_ev - paulTime < dtPaul, otherwise into a LIFO bucket. */
-__device__ __forceinline__ int64_t rank_of(double t_ev, double paul, double dtp, int64_t ctr) {
-    return (t_ev - paul < dtp) ? EDMD_RANK_DIRECT + ctr : EDMD_RANK_DIRECT - 1 - ctr;
+ * the tree when t_ev - paulTime < dtPaul, into the overflow list when it is
+ * dtPaul * nlists or more ahead, otherwise into a LIFO bucket. */
+__device__ __forceinline__ int64_t rank_of(double t_ev, double paul, double dtp, double horizon,
+                                           int64_t ctr) {
+    const double dt = t_ev - paul;
+    if (dt < dtp) return EDMD_RANK_DIRECT + ctr;
+    if (dt >= horizon) return EDMD_RANK_OVF + ctr;
+    return EDMD_RANK_DIRECT - 1 - ctr;
 }
 
 /* Calendar leaf of a particle: its earlier slot by (time, rank) (a NaN key
@@ -1597,8 +1602,8 @@ __device__ __forceinline__ bool sweep_one(const EdmdConst& C, const AR& A, doubl
     e.dir = dir;
     e.pad = 0;
     A.e[p] = e;
-    const int64_t rk = rank_of(tk, paul, C.dtp, ctr_coll);
-    const int64_t rx = rank_of(tc, paul, C.dtp, ctr_cross);
+    const int64_t rk = rank_of(tk, paul, C.dtp, C.horizon, ctr_coll);
+    const int64_t rx = rank_of(tc, paul, C.dtp, C.horizon, ctr_cross);
     A.erank[2 * p] = rk;
     A.erank[2 * p + 1] = rx;
     const Leaf L = make_leaf(tk, rk, tc, rx, partner, dir);
@@ -1941,8 +1946,8 @@ __device__ __forceinline__ void sweep_tile_predict(const EdmdConst& C, const AR&
         e.dir = dir;
         e.pad = 0;
         A.e[p] = e;
-        const int64_t rk = rank_of(tk, paul, C.dtp, ctr0 + 2 * (int64_t)p + 1);
-        const int64_t rx = rank_of(tc, paul, C.dtp, ctr0 + 2 * (int64_t)p);
+        const int64_t rk = rank_of(tk, paul, C.dtp, C.horizon, ctr0 + 2 * (int64_t)p + 1);
+        const int64_t rx = rank_of(tc, paul, C.dtp, C.horizon, ctr0 + 2 * (int64_t)p);
         A.erank[2 * p] = rk;
         A.erank[2 * p + 1] = rx;
         const Leaf Lf = make_leaf(tk, rk, tc, rx, partner, dir);
@@ -2166,6 +2171,10 @@ struct QueueState {
      * order; here rather than in scalar registers, which the loop has too
      * few of */
     double vir[4];
+    /* the calendar's minimum after a wrap moved particle events of the
+     * overflow list (moved != 0) */
+    double mkey;
+    int32_t mleaf, maux, moved;
 };
 
 /* Top level of the calendar between HBM and the LDS copy. */
@@ -2229,10 +2238,9 @@ __device__ __host__ __forceinline__ int64_t window_of(double t_ev, double paul, 
     return win + (int64_t)(int)(dt / dtp);
 }
 
-/* The overflow list is filed again when the ring of buckets wraps
- * (EDMD.c:2292-2302): the special events waiting there get their bucket
- * and rank anew.  Rare; kept out of line. */
-__device__ __noinline__ void refile_specials(const EdmdConst& C, EdmdState* st, double paul,
+/* The overflow list's special events filed again when the ring of buckets
+ * wraps (EDMD.c:2292-2302): they get their bucket and rank anew. */
+__device__ __forceinline__ void refile_specials(const EdmdConst& C, EdmdState* st, double paul,
                                              int64_t win, int64_t& ctr) {
     double* ts[4] = {&st->next_screen, &st->next_thermo, &st->next_noise, &st->next_grow};
     int64_t* ws[4] = {&st->win_screen, &st->win_thermo, &st->win_noise, &st->win_grow};
@@ -2243,7 +2251,7 @@ __device__ __noinline__ void refile_specials(const EdmdConst& C, EdmdState* st, 
         const int64_t w = window_of(tk, paul, C.dtp, C.nlists, win);
         /* filed straight into the tree during the wrap, before the new
          * bucket is drained: ahead of both drained and later direct events */
-        const int64_t r = (w == win) ? ctr : rank_of(tk, paul, C.dtp, ctr);
+        const int64_t r = (w == win) ? ctr : rank_of(tk, paul, C.dtp, C.horizon, ctr);
         ctr++;
         WAVE_SYNC();
         if ((threadIdx.x & 63) == 0) {
@@ -2254,15 +2262,97 @@ __device__ __noinline__ void refile_specials(const EdmdConst& C, EdmdState* st, 
     }
 }
 
+/* Rank of a particle event of the overflow list filed again at a wrap with
+ * counter c (addEventToQueue from addNextPaulEvent, EDMD.c:2144-2170,
+ * 2292-2302): straight into the tree, which is empty during a wrap, so
+ * ahead of the drained bucket and of every later insertion (rank c, as a
+ * special filed there); the overflow list again; or a bucket. */
+__device__ __forceinline__ int64_t refiled_rank(const EdmdConst& C, double t_ev, double paul,
+                                                int64_t c) {
+    const double dt = t_ev - paul;
+    if (dt < C.dtp) return c;
+    if (dt >= C.horizon) return EDMD_RANK_OVF + c;
+    return EDMD_RANK_DIRECT - 1 - c;
+}
+
+/* The particle events of the overflow list filed again at a wrap
+ * (EDMD.c:2292-2302), after its special events, in the order of the list's
+ * walk from its head (the newest first): a slot ranked EDMD_RANK_OVF + c
+ * gets the counter 2 now - 1 - c and the rank of where it goes now; the
+ * counter moves past them all (the list holds counters from the previous
+ * wrap's on).  The leaves of the disks concerned are made again and the
+ * calendar is rebuilt.  Once per wrap of the bucket ring (5 time units);
+ * run from the loop's body, like the thermostat, not from the bucket walk,
+ * which stays small. */
+template <class AR>
+__device__ void rebuild_tree(const EdmdConst& C, const AR& A, int lane);
+
+/* The overflow list filed again at a wrap (EDMD.c:2292-2302): its special
+ * events, then its particle events in the order of the list's walk from
+ * its head (the newest first): a slot ranked EDMD_RANK_OVF + c gets the
+ * counter 2 now - 1 - c and the rank of where it goes now, and the counter
+ * moves past them all (the list holds counters from the previous wrap's
+ * on).  When any slot moved, the leaves of the disks concerned are made
+ * again, the calendar is rebuilt, its top level reloaded and its minimum
+ * left in q (q->moved = 1).  Once per wrap of the bucket ring (5 time
+ * units); kept out of line, with the loop's registers out of its way. */
+template <class AR>
+__device__ __noinline__ void refile_wrap(const EdmdConst& C, const AR& A, QueueState* q,
+                                         double paul, int64_t win, double t, int64_t& ctr) {
+    const int lane = threadIdx.x & 63;
+    refile_specials(C, A.st, paul, win, ctr);
+    const int64_t now = ctr;
+    bool any = false;
+    for (int k = lane; k < C.n; k += 64) {
+        int64_t rk = A.erank[2 * k], rx = A.erank[2 * k + 1];
+        const bool fk = EDMD_RANK_IS_OVF(rk), fx = EDMD_RANK_IS_OVF(rx);
+        if (!fk && !fx) continue;
+        const EdmdEv e = A.e[k];
+        if (fk) rk = refiled_rank(C, e.tcoll, paul, 2 * now - 1 - (rk - EDMD_RANK_OVF));
+        if (fx) rx = refiled_rank(C, e.tcross, paul, 2 * now - 1 - (rx - EDMD_RANK_OVF));
+        A.erank[2 * k] = rk;
+        A.erank[2 * k + 1] = rx;
+        const Leaf Lf = make_leaf(e.tcoll, rk, e.tcross, rx, e.partner, e.dir);
+        EdmdNode nd;
+        nd.key = Lf.key;
+        nd.idx = k;
+        nd.aux = Lf.aux;
+        A.leaf[k] = nd;
+        A.lrank[k] = Lf.rank;
+        any = true;
+    }
+    const bool moved = __ballot(any) != 0ull;
+    const int64_t lo = A.st->ovf_lo;
+    WAVE_SYNC();
+    if (lane == 0) A.st->ovf_lo = now;
+    ctr = 2 * now - lo;
+    if (moved) {
+        WAVE_SYNC();
+        rebuild_tree(C, A, lane);
+        top_load(C, A, lane, q->top);
+        double key;
+        int leaf, aux;
+        top_min(C, A, lane, q->top, t, key, leaf, aux);
+        if (lane == 0) {
+            q->mkey = key;
+            q->mleaf = leaf;
+            q->maux = aux;
+        }
+    }
+    if (lane == 0) q->moved = moved ? 1 : 0;
+    WAVE_SYNC();
+}
+
 /* One bucket further (addNextPaulEvent, EDMD.c:2287-2307); true when the
- * ring wrapped. */
-__device__ __forceinline__ bool step_window(const EdmdConst& C, EdmdState* st, LoopRegs& R) {
+ * ring wrapped (the overflow list filed again, refile_wrap). */
+template <class AR>
+__device__ __forceinline__ bool step_window(const EdmdConst& C, const AR& A, LoopRegs& R) {
     R.paul = uni(R.paul + C.dtp);
     R.win++;
     if (++R.cur < C.nlists) return false;
     R.cur = 0;
     int64_t ctr = R.ctr;
-    refile_specials(C, st, R.paul, R.win, ctr);
+    refile_wrap(C, A, R.q, R.paul, R.win, R.t, ctr);
     R.ctr = uni64(ctr);
     return true;
 }
@@ -2271,12 +2361,14 @@ __device__ __forceinline__ bool step_window(const EdmdConst& C, EdmdState* st, L
  * reference's tree -- a particle event within dtPaul of paulTime, or a
  * special event whose bucket has come up -- move on one bucket (refiling
  * the overflow list when the ring wraps).  Then the earlier in the tree by
- * (time, rank) is next.  Returns true for the special event.  A queue whose
- * next event lies more than 2^24 buckets ahead is not walked bucket by
- * bucket (the reference would); paulTime then jumps to that event. */
+ * (time, rank) is next.  Returns true for the special event, false for
+ * the particle event (key, leaf, aux: the calendar's minimum, taken anew
+ * when a wrap moved particle events of the overflow list).  A queue whose next event lies
+ * more than 2^24 buckets ahead is not walked bucket by bucket (the
+ * reference would); paulTime then jumps to that event. */
 template <class AR>
 __device__ __forceinline__ bool pick_next(const EdmdConst& C, const AR& A, EdmdState* st,
-                                          LoopRegs& R, double key, int leaf) {
+                                          LoopRegs& R, double& key, int& leaf, int& aux) {
     /* the particle event's rank matters only against a special at its time */
     auto krank = [&]() { return key == R.next_special ? uni64(A.lrank[leaf]) : (int64_t)0; };
     for (int guard = 0;; ++guard) {
@@ -2294,13 +2386,20 @@ __device__ __forceinline__ bool pick_next(const EdmdConst& C, const AR& A, EdmdS
             }
             return before(R.next_special, R.srank, key, krank());
         }
-        if (step_window(C, st, R)) next_special(st, R);
+        if (step_window(C, A, R)) {
+            next_special(st, R);
+            if (R.q->moved) {
+                key = R.q->mkey;
+                leaf = R.q->mleaf;
+                aux = R.q->maux;
+            }
+        }
     }
 }
 
 /* Rank of the next insertion; counts it. */
 __device__ __forceinline__ int64_t take_rank(const EdmdConst& C, LoopRegs& R, double t_ev) {
-    const int64_t r = rank_of(t_ev, R.paul, C.dtp, R.ctr);
+    const int64_t r = rank_of(t_ev, R.paul, C.dtp, C.horizon, R.ctr);
     R.ctr++;
     return r;
 }
@@ -3140,7 +3239,7 @@ __device__ __forceinline__ void run_loop(const EdmdConst& C, const AR& A, EdmdRu
         int p, aux;
         top_min(C, A, lane, R.q->top, R.t, key, p, aux);
         if (!(key < INF_D) && !(R.next_special < INF_D)) { reason = EDMD_STOP_EMPTY; break; }
-        if (pick_next(C, A, st, R, key, p)) {
+        if (pick_next(C, A, st, R, key, p, aux)) {
             const int sp = R.special;
             R.t = R.next_special;
             trace_push(C, A, lane, R.t, sp, 0, 0, 1);
@@ -3425,7 +3524,7 @@ __global__ __launch_bounds__(64) void k_repredict(const EdmdSys* __restrict__ sy
     int64_t rcoll = A.erank[2 * i], rcross = A.erank[2 * i + 1];
     if (which & 1) {
         cross_time(C, t, pi.x, pi.y, pi.vx, pi.vy, pi.cx, pi.cy, e.tcross, e.dir);
-        rcross = rank_of(e.tcross, paul, C.dtp, ctr++);
+        rcross = rank_of(e.tcross, paul, C.dtp, C.horizon, ctr++);
     }
     bool ov = false;
     if (which & 2) {
@@ -3439,7 +3538,7 @@ __global__ __launch_bounds__(64) void k_repredict(const EdmdSys* __restrict__ sy
         else
             predict_owners<false>(C, A, lane, t, 1, o, o, dt, e.partner, e.cact, dtb, pb, cpb, ov, stage);
         e.tcoll = t + dt;
-        rcoll = rank_of(e.tcoll, paul, C.dtp, ctr++);
+        rcoll = rank_of(e.tcoll, paul, C.dtp, C.horizon, ctr++);
     }
     if (lane == 0) {
         A.e[i] = e;
@@ -3479,7 +3578,7 @@ __global__ __launch_bounds__(64) void k_queue_edit(const EdmdSys* __restrict__ s
     const bool remove = mode == 2;
     if (kind == EDMD_EV_SCREENSHOT || kind == EDMD_EV_THERMO || kind == EDMD_EV_ADDINGNOISE) {
         const double tn = remove ? INF_D : t;
-        const int64_t rk = remove ? INT64_MAX : rank_of(t, paul, C.dtp, ctr);
+        const int64_t rk = remove ? INT64_MAX : rank_of(t, paul, C.dtp, C.horizon, ctr);
         const int64_t wk = remove ? INT64_MAX : window_of(t, paul, C.dtp, C.nlists, st->win);
         if (!remove) ctr++;
         if (lane == 0) {
@@ -3504,7 +3603,7 @@ __global__ __launch_bounds__(64) void k_queue_edit(const EdmdSys* __restrict__ s
         } else {
             e.tcross = t;
             e.dir = j;
-            rcross = rank_of(t, paul, C.dtp, ctr++);
+            rcross = rank_of(t, paul, C.dtp, C.horizon, ctr++);
         }
     } else {
         if (remove) {
@@ -3513,7 +3612,7 @@ __global__ __launch_bounds__(64) void k_queue_edit(const EdmdSys* __restrict__ s
             e.tcoll = t;
             e.partner = kind == EDMD_EV_WALL ? EDMD_WALL_CODE(j) : j;
             e.cact = kind == EDMD_EV_WALL ? 0u : (mode == 1 ? A.p[j].coll : cact);
-            rcoll = rank_of(t, paul, C.dtp, ctr++);
+            rcoll = rank_of(t, paul, C.dtp, C.horizon, ctr++);
         }
     }
     if (lane == 0) {

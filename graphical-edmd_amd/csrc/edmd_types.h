@@ -92,6 +92,8 @@ typedef struct EdmdConst {
     int32_t thermal_wall; /* wall hits draw the new normal speed (thermoWall) */
     int32_t shear_wall;   /* y-wall hits set vx to +-0.1 (addShearWall) */
     int32_t screen;       /* sweep root screen: 0 off, 1 on, 2 on and misses counted */
+    double horizon;       /* dtPaul * paulListN: events this far ahead of paulTime or
+                             more go to the overflow list (EDMD.c:2154) */
 } EdmdConst;
 
 /* Order among events of equal time.  The reference pops equal times in the
@@ -107,8 +109,21 @@ typedef struct EdmdConst {
  * and the calendar orders by (time, rank).  Events of the overflow list
  * (scheduled 5 or more time units ahead) are filed again when the bucket
  * ring wraps, before the new bucket is drained; a special event filed
- * straight into the tree then gets rank = counter, ahead of both classes. */
+ * straight into the tree then gets rank = counter, ahead of both classes.
+ * A particle event filed into the overflow list gets
+ *   overflow: EDMD_RANK_OVF + counter
+ * and keeps it until the next wrap.  The list is LIFO and is walked from
+ * its head at the wrap, so its events are filed again newest first: the
+ * wrap gives each the counter 2 now - 1 - c (now = the counter at the wrap,
+ * c = its own), which orders them as the walk does without a sort, and
+ * moves the counter past them (the reference refiles them one by one);
+ * each is then ranked as the class it is filed into (straight into the
+ * tree: rank = that counter, like a special; a bucket; the overflow list
+ * again). */
 #define EDMD_RANK_DIRECT ((int64_t)1 << 62)
+#define EDMD_RANK_OVF ((int64_t)3 << 61)
+/* rank r belongs to a particle event waiting in the overflow list */
+#define EDMD_RANK_IS_OVF(r) ((r) >= EDMD_RANK_OVF && (r) < EDMD_RANK_OVF + ((int64_t)1 << 60))
 
 typedef struct EdmdState {
     double t;
@@ -136,6 +151,9 @@ typedef struct EdmdState {
     /* the sweep's root screen: marked candidates, roots solved, and (check
      * mode) screened-out roots that the minimum would have taken */
     uint64_t scr_marks, scr_solved, scr_miss;
+    /* counter at the last wrap of the bucket ring: the overflow list's
+     * particle events hold counters from here on */
+    int64_t ovf_lo;
 } EdmdState;
 
 /* LDS-resident runs: byte offsets of a system's arrays in its workgroup's

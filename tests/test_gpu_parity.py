@@ -20,17 +20,10 @@ pytestmark = pytest.mark.gpu
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def assert_trace_equal(got, ref, exact=True):
+def assert_trace_equal(got, ref):
     """Event traces equal in order, kinds, disks (including the orientation
-    of mirrored pair events), validity and time bits.  exact=False compares
-    mirrored pairs unordered and equal-time runs as sets (the reference
-    reorders events scheduled more than 5 time units ahead when its bucket
-    ring wraps, EDMD.c:2287-2307; the calendar here does not)."""
-    if exact:
-        a, b = ref, got
-    else:
-        a = F.canonical_trace(ref)
-        b = F.canonical_trace(got)
+    of mirrored pair events), validity and time bits."""
+    a, b = ref, got
     assert len(a) == len(b)
     for k in ("type", "i", "j", "valid"):
         bad = np.nonzero(a[k] != b[k])[0]
