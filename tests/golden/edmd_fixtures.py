@@ -307,39 +307,5 @@ def parse_pcf(path):
     return r, g
 
 
-def canonical_pairs(tr):
-    """Collision pairs as unordered (min, max): the mirrored events (i, j) and
-    (j, i) of a pair predicted from a common synchronised state carry equal
-    times and either may pop first (EDMD.c:2179, 2164-2168)."""
-    out = tr.copy()
-    coll = tr["type"] == COLLISION
-    lo = np.minimum(tr["i"], tr["j"])
-    hi = np.maximum(tr["i"], tr["j"])
-    out["i"] = np.where(coll, lo, tr["i"])
-    out["j"] = np.where(coll, hi, tr["j"])
-    return out
-
-
-def canonical_trace(tr):
-    """canonical_pairs, and events of exactly equal time as a set: the
-    reference orders simultaneous events by the insertion history of its
-    queue (BST ties go right, bucket lists are LIFO, EDMD.c:2144-2203), not
-    by anything physical.  Runs of equal time are sorted by (kind, i, j)."""
-    out = canonical_pairs(tr)
-    t = out["t"]
-    k = 0
-    n = len(out)
-    while k < n:
-        e = k + 1
-        while e < n and t[e] == t[k]:
-            e += 1
-        if e - k > 1:
-            seg = out[k:e]
-            order = np.lexsort((seg["j"], seg["i"], seg["type"]))
-            out[k:e] = seg[order]
-        k = e
-    return out
-
-
 def bits_equal(a, b):
     return np.float64(a).view(np.uint64) == np.float64(b).view(np.uint64)
