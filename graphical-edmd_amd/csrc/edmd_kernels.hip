@@ -888,7 +888,8 @@ struct Leaf {
 
 __device__ __forceinline__ Leaf make_leaf(double tcoll, int64_t rcoll, double tcross,
                                           int64_t rcross, int partner, int dir) {
-    const bool coll = tcoll < tcross || (tcoll == tcross && rcoll < rcross);
+    /* non-short-circuit: the compiler keeps a select, not a branch */
+    const bool coll = (tcoll < tcross) | ((tcoll == tcross) & (rcoll < rcross));
     Leaf L;
     const double k = coll ? tcoll : tcross;
     L.key = (k == k) ? k : INF_D;
@@ -898,7 +899,7 @@ __device__ __forceinline__ Leaf make_leaf(double tcoll, int64_t rcoll, double tc
 }
 
 __device__ __forceinline__ bool before(double ka, int64_t ra, double kb, int64_t rb) {
-    return ka < kb || (ka == kb && ra < rb);
+    return (ka < kb) | ((ka == kb) & (ra < rb));
 }
 
 /* ------------------------------------------------------------ predictions */
