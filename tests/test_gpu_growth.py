@@ -99,20 +99,22 @@ def test_growth_blocks_derivatives(gpu):
 
 
 def test_cli_default_start_is_the_growth_start(gpu, tmp_path):
-    """`edmd -N 300 -p 0.6 -t 11.5 -D 1 --full-precision` without -l or
+    """`edmd -N 300 -p 0.6 -v 7 -t 11.5 -D 1 --full-precision` without -l or
     --lattice starts as the reference does by default: a binary mixture
     (30% of radius 0.4, the rest radius 1) at random centres, growing at the
     rate optimizeGrowConstant picks (0.1 at phi = 0.6, so GROWSTOP at t = 10);
     screenshot times below 1/vr move by 1/vr, so the one frame is at t = 11.
     It equals the library's run from the disks particlesInit makes for the
-    same command line (the reference's MT19937 start)."""
+    same command line (the reference's MT19937 start).  Both are seeded by
+    -v: without it the reference seeds from the clock's seconds (EDMD.c:500),
+    and the two processes could read different seconds."""
     import ctypes as C
     from test_start_reference import start
     import os
     import subprocess
     repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     cli = os.path.join(repo, "graphical-edmd_amd", "bin", "edmd")
-    r = subprocess.run([cli, "-N", "300", "-p", "0.6", "-t", "11.5", "-D", "1",
+    r = subprocess.run([cli, "-N", "300", "-p", "0.6", "-v", "7", "-t", "11.5", "-D", "1",
                         "--full-precision"], cwd=tmp_path, capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stderr
@@ -121,7 +123,7 @@ def test_cli_default_start_is_the_growth_start(gpu, tmp_path):
     frame = F.read_dump(str(tmp_path / "dump" / dumps[0]))
     assert gpu.growth_rate(0.6) == 0.1
     L = gpu.lib()
-    assert start(L, ["-N", "300", "-p", "0.6", "--no-output"]) == 0
+    assert start(L, ["-N", "300", "-p", "0.6", "-v", "7", "--no-output"]) == 0
     st = {k: np.zeros(300) for k in ("x", "y", "vx", "vy", "rad", "m", "vr")}
     L.edmd_compat_particles.argtypes = [C.c_void_p] * 7
     assert L.edmd_compat_particles(*[st[k].ctypes.data for k in
